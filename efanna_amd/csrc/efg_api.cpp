@@ -1,0 +1,715 @@
+// C ABI of libefg.so: index lifetime, parameters, search entry points and the
+// reference's file formats.  Kernels live in k_*.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "efg.h"
+#include "efg_internal.hpp"
+
+namespace efg {
+
+thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return EFG_OK;
+  return fail(EFG_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+// Host copy of the trees: flat node arrays.
+struct HostTrees {
+  int T = 0;
+  std::vector<int32_t> dim, left, right, roots;
+  std::vector<float> val;
+  std::vector<uint32_t> start, end, leaf;
+  std::vector<int8_t> leaf_dim;
+};
+
+int upload_trees(efg_index* ix, const HostTrees& h) {
+  const size_t M = h.dim.size();
+  std::vector<int4> n4(M);
+  std::vector<int32_t> par(M, -1);
+  std::vector<uint2> rg(M);
+  for (size_t i = 0; i < M; ++i) {
+    int32_t vb;
+    std::memcpy(&vb, &h.val[i], 4);
+    n4[i] = make_int4(h.dim[i], vb, h.left[i], h.right[i]);
+    rg[i] = make_uint2(h.start[i], h.end[i]);
+  }
+  for (size_t i = 0; i < M; ++i) {
+    if (h.left[i] >= 0) par[h.left[i]] = int32_t(i);
+    if (h.right[i] >= 0) par[h.right[i]] = int32_t(i);
+  }
+  if (!ix->node4.alloc(M) || !ix->parent.alloc(M) || !ix->range.alloc(M) || !ix->roots.alloc(h.T) ||
+      !ix->leaf.alloc(size_t(h.T) * ix->n))
+    return fail(EFG_ERR_MEMORY, "out of device memory (trees)");
+  hipStream_t st = ix->stream;
+  int rc = 0;
+  rc |= check_hip(hipMemcpyAsync(ix->node4.p, n4.data(), M * sizeof(int4), hipMemcpyHostToDevice, st), "copy");
+  rc |= check_hip(hipMemcpyAsync(ix->parent.p, par.data(), M * 4, hipMemcpyHostToDevice, st), "copy");
+  rc |= check_hip(hipMemcpyAsync(ix->range.p, rg.data(), M * sizeof(uint2), hipMemcpyHostToDevice, st), "copy");
+  rc |= check_hip(hipMemcpyAsync(ix->roots.p, h.roots.data(), h.T * 4, hipMemcpyHostToDevice, st), "copy");
+  rc |= check_hip(hipMemcpyAsync(ix->leaf.p, h.leaf.data(), h.leaf.size() * 4, hipMemcpyHostToDevice, st), "copy");
+  rc |= check_hip(hipStreamSynchronize(st), "sync");
+  if (rc) return EFG_ERR_DEVICE;
+  ix->T = h.T;
+  ix->M = M;
+  ix->leaf_dim = h.leaf_dim;
+  if (ix->leaf_dim.size() != M) ix->leaf_dim.assign(M, -1);
+  return EFG_OK;
+}
+
+int download_trees(const efg_index* ix, HostTrees& h) {
+  const size_t M = ix->M;
+  std::vector<int4> n4(M);
+  std::vector<uint2> rg(M);
+  h.T = ix->T;
+  h.roots.resize(ix->T);
+  h.leaf.resize(size_t(ix->T) * ix->n);
+  int rc = 0;
+  rc |= check_hip(hipMemcpy(n4.data(), ix->node4.p, M * sizeof(int4), hipMemcpyDeviceToHost), "copy");
+  rc |= check_hip(hipMemcpy(rg.data(), ix->range.p, M * sizeof(uint2), hipMemcpyDeviceToHost), "copy");
+  rc |= check_hip(hipMemcpy(h.roots.data(), ix->roots.p, ix->T * 4, hipMemcpyDeviceToHost), "copy");
+  rc |= check_hip(hipMemcpy(h.leaf.data(), ix->leaf.p, h.leaf.size() * 4, hipMemcpyDeviceToHost), "copy");
+  if (rc) return EFG_ERR_DEVICE;
+  h.dim.resize(M);
+  h.val.resize(M);
+  h.left.resize(M);
+  h.right.resize(M);
+  h.start.resize(M);
+  h.end.resize(M);
+  for (size_t i = 0; i < M; ++i) {
+    h.dim[i] = n4[i].x;
+    std::memcpy(&h.val[i], &n4[i].y, 4);
+    h.left[i] = n4[i].z;
+    h.right[i] = n4[i].w;
+    h.start[i] = rg[i].x;
+    h.end[i] = rg[i].y;
+  }
+  h.leaf_dim = ix->leaf_dim;
+  return EFG_OK;
+}
+
+int upload_graph(efg_index* ix, const std::vector<uint64_t>& off, const std::vector<uint32_t>& ids) {
+  if (off.size() != ix->n + 1) return fail(EFG_ERR_FORMAT, "graph row count differs from the data set");
+  for (uint32_t x : ids)
+    if (x >= ix->n) return fail(EFG_ERR_FORMAT, "graph holds an id outside the data set");
+  int maxd = 0;
+  for (size_t i = 0; i < ix->n; ++i) maxd = std::max<int>(maxd, int(off[i + 1] - off[i]));
+  if (!ix->goff.alloc(off.size()) || !ix->gids.alloc(ids.size()))
+    return fail(EFG_ERR_MEMORY, "out of device memory (graph)");
+  int rc = 0;
+  rc |= check_hip(hipMemcpyAsync(ix->goff.p, off.data(), off.size() * 8, hipMemcpyHostToDevice, ix->stream), "copy");
+  if (!ids.empty())
+    rc |= check_hip(hipMemcpyAsync(ix->gids.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, ix->stream), "copy");
+  rc |= check_hip(hipStreamSynchronize(ix->stream), "sync");
+  if (rc) return EFG_ERR_DEVICE;
+  ix->nnz = ids.size();
+  ix->max_deg = maxd;
+  return EFG_OK;
+}
+
+int download_graph(const efg_index* ix, std::vector<uint64_t>& off, std::vector<uint32_t>& ids) {
+  off.assign(ix->n + 1, 0);
+  ids.assign(ix->nnz, 0);
+  if (ix->nnz == 0) return EFG_OK;
+  int rc = 0;
+  rc |= check_hip(hipMemcpy(off.data(), ix->goff.p, off.size() * 8, hipMemcpyDeviceToHost), "copy");
+  rc |= check_hip(hipMemcpy(ids.data(), ix->gids.p, ids.size() * 4, hipMemcpyDeviceToHost), "copy");
+  return rc ? EFG_ERR_DEVICE : EFG_OK;
+}
+
+// ---- trees file (kdtreeub_index.hpp:197-242 / 139-196) -----------------------
+// Per tree: int node_num, int node_size (sizeof(Node) = 48 in the reference),
+// node_num post-order records {int DivDim, float DivVal, size_t Start,
+// size_t End, Node* L, Node* R}; the pointer fields carry no meaning for the
+// loader (it rebuilds links from Start/End adjacency, 672-700) and are written
+// as child index + 1 (0 for leaves).
+void write_tree_nodes(std::ofstream& out, const HostTrees& h, int32_t root, int& count) {
+  // iterative post-order
+  std::vector<std::pair<int32_t, int>> st{{root, 0}};
+  while (!st.empty()) {
+    auto& top = st.back();
+    int32_t x = top.first;
+    if (h.left[x] >= 0 && top.second == 0) {
+      top.second = 1;
+      st.push_back({h.left[x], 0});
+      continue;
+    }
+    if (h.right[x] >= 0 && top.second == 1) {
+      top.second = 2;
+      st.push_back({h.right[x], 0});
+      continue;
+    }
+    st.pop_back();
+    bool leaf = h.left[x] < 0;
+    int32_t dd = leaf ? int32_t(h.leaf_dim.empty() ? -1 : h.leaf_dim[x]) : h.dim[x];
+    float dv = leaf ? 0.f : h.val[x];
+    uint64_t s = h.start[x], e = h.end[x];
+    uint64_t lp = leaf ? 0 : uint64_t(h.left[x]) + 1, rp = leaf ? 0 : uint64_t(h.right[x]) + 1;
+    out.write(reinterpret_cast<const char*>(&dd), 4);
+    out.write(reinterpret_cast<const char*>(&dv), 4);
+    out.write(reinterpret_cast<const char*>(&s), 8);
+    out.write(reinterpret_cast<const char*>(&e), 8);
+    out.write(reinterpret_cast<const char*>(&lp), 8);
+    out.write(reinterpret_cast<const char*>(&rp), 8);
+    ++count;
+  }
+}
+
+int write_trees(const efg_index* ix, std::ofstream& out, uint32_t K) {
+  HostTrees h;
+  if (int rc = download_trees(ix, h)) return rc;
+  uint32_t tn = uint32_t(h.T);
+  uint64_t num = ix->n, dim = ix->d;
+  out.write(reinterpret_cast<const char*>(&K), 4);
+  out.write(reinterpret_cast<const char*>(&tn), 4);
+  out.write(reinterpret_cast<const char*>(&num), 8);
+  out.write(reinterpret_cast<const char*>(&dim), 8);
+  for (int t = 0; t < h.T; ++t) {
+    std::streampos at = out.tellp();
+    int32_t zero = 0, node_size = 48;
+    out.write(reinterpret_cast<const char*>(&zero), 4);
+    out.write(reinterpret_cast<const char*>(&node_size), 4);
+    int cnt = 0;
+    write_tree_nodes(out, h, h.roots[t], cnt);
+    std::streampos endp = out.tellp();
+    out.seekp(at);
+    out.write(reinterpret_cast<const char*>(&cnt), 4);
+    out.seekp(endp);
+  }
+  for (int t = 0; t < h.T; ++t) out.write(reinterpret_cast<const char*>(&h.leaf[size_t(t) * ix->n]), ix->n * 4);
+  return out ? EFG_OK : fail(EFG_ERR_IO, "write failed");
+}
+
+int read_trees(efg_index* ix, std::ifstream& in, uint32_t* K_out) {
+  uint32_t K = 0, tn = 0;
+  uint64_t num = 0, dim = 0;
+  in.read(reinterpret_cast<char*>(&K), 4);
+  in.read(reinterpret_cast<char*>(&tn), 4);
+  in.read(reinterpret_cast<char*>(&num), 8);
+  in.read(reinterpret_cast<char*>(&dim), 8);
+  if (!in) return fail(EFG_ERR_FORMAT, "truncated trees header");
+  if (num != ix->n || dim != ix->d) return fail(EFG_ERR_FORMAT, "trees file was built for another data set");
+  if (tn == 0 || tn > 4096) return fail(EFG_ERR_FORMAT, "bad tree count in trees file");
+  HostTrees h;
+  h.T = int(tn);
+  for (uint32_t t = 0; t < tn; ++t) {
+    int32_t node_num = 0, node_size = 0;
+    in.read(reinterpret_cast<char*>(&node_num), 4);
+    in.read(reinterpret_cast<char*>(&node_size), 4);
+    if (!in || node_num <= 0 || uint64_t(node_num) > 4 * num + 8) return fail(EFG_ERR_FORMAT, "bad node count in trees file");
+    const int32_t base = int32_t(h.dim.size());
+    std::vector<int32_t> st;  // DepthFirstBuildTree (672-700)
+    for (int32_t j = 0; j < node_num; ++j) {
+      int32_t dd;
+      float dv;
+      uint64_t s, e, lp, rp;
+      in.read(reinterpret_cast<char*>(&dd), 4);
+      in.read(reinterpret_cast<char*>(&dv), 4);
+      in.read(reinterpret_cast<char*>(&s), 8);
+      in.read(reinterpret_cast<char*>(&e), 8);
+      in.read(reinterpret_cast<char*>(&lp), 8);
+      in.read(reinterpret_cast<char*>(&rp), 8);
+      if (!in) return fail(EFG_ERR_FORMAT, "truncated trees file");
+      if (s > e || e > num) return fail(EFG_ERR_FORMAT, "node range outside the data set");
+      int32_t id = base + j;
+      h.dim.push_back(dd);
+      h.val.push_back(dv);
+      h.start.push_back(uint32_t(s));
+      h.end.push_back(uint32_t(e));
+      h.left.push_back(-1);
+      h.right.push_back(-1);
+      h.leaf_dim.push_back(int8_t(dd < 0 ? -1 : 0));
+      size_t rs = st.size();
+      if (rs >= 2) {
+        int32_t l1 = st[rs - 1], l2 = st[rs - 2];
+        if (h.end[l1] == e && h.start[l2] == s) {
+          h.right[id] = l1;
+          h.left[id] = l2;
+          st.pop_back();
+          st.pop_back();
+        }
+      }
+      st.push_back(id);
+    }
+    if (st.size() != 1) return fail(EFG_ERR_FORMAT, "Error constructing trees");
+    h.roots.push_back(st[0]);
+  }
+  for (size_t i = 0; i < h.dim.size(); ++i) {
+    if (h.left[i] >= 0) {
+      if (h.dim[i] < 0 || uint64_t(h.dim[i]) >= dim) return fail(EFG_ERR_FORMAT, "split dimension out of range");
+      h.leaf_dim[i] = 0;
+    }
+  }
+  h.leaf.resize(size_t(tn) * num);
+  in.read(reinterpret_cast<char*>(h.leaf.data()), h.leaf.size() * 4);
+  if (!in) return fail(EFG_ERR_FORMAT, "truncated leaf lists");
+  for (uint32_t x : h.leaf)
+    if (x >= num) return fail(EFG_ERR_FORMAT, "leaf list holds an id outside the data set");
+  if (K_out) *K_out = K;
+  int rc = upload_trees(ix, h);
+  if (rc) return rc;
+  ix->s_trees = int(tn);  // SP.tree_num = tree_num (kdtreeub_index.hpp:151)
+  return EFG_OK;
+}
+
+int sync_results_to_host(efg_index* ix, size_t nq, int k) {
+  ix->res_ids.resize(nq * size_t(k));
+  ix->res_d.resize(nq * size_t(k));
+  int rc = check_hip(hipMemcpyAsync(ix->res_ids.data(), ix->ws.ids.p, nq * k * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+  rc |= check_hip(hipMemcpyAsync(ix->res_d.data(), ix->ws.dists.p, nq * k * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+  rc |= check_hip(hipStreamSynchronize(ix->stream), "sync");
+  ix->res_nq = nq;
+  ix->res_k = k;
+  return rc ? EFG_ERR_DEVICE : EFG_OK;
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+}  // namespace efg
+
+using namespace efg;
+
+extern "C" {
+
+const char* efg_last_error(void) { return g_err.c_str(); }
+
+void efg_default_build_params(efg_build_params* p, int trees) {
+  // KDTreeUbIndexParams(rnn_used, tree_num_total, ml=4, epoches=4, check=25,
+  // L=30, K=10, tree_num_build=0, S=10) + IndexParams::TNS = 10
+  p->trees = trees;
+  p->trees_build = 0;
+  p->merge_level = 4;
+  p->epochs = 4;
+  p->check = 25;
+  p->L = 30;
+  p->K = 10;
+  p->S = 10;
+  p->tns = 10;
+  p->seed = 1998;
+}
+
+int efg_create(const float* data, size_t n, size_t d, size_t row_stride, const efg_build_params* params,
+               efg_index** out) {
+  if (!out) return fail(EFG_ERR_ARG, "null output");
+  *out = nullptr;
+  if (!data || n == 0 || d == 0 || row_stride < d) return fail(EFG_ERR_ARG, "invalid data matrix");
+  if (n >= (size_t(1) << 30)) return fail(EFG_ERR_ARG, "at most 2^30 - 1 rows");
+  efg_index* ix = new efg_index();
+  ix->n = n;
+  ix->d = d;
+  ix->D = (d + 7) & ~size_t(7);
+  if (params) ix->bp = *params;
+  else efg_default_build_params(&ix->bp, 4);
+  if (ix->bp.trees_build <= 0) ix->bp.trees_build = ix->bp.trees;
+  if (ix->bp.tns <= 0) ix->bp.tns = 10;
+  ix->s_trees = ix->bp.trees;  // SP.tree_num = TreeNum (kdtreeub_index.hpp:59)
+  if (int rc = check_hip(hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking), "stream")) {
+    delete ix;
+    return rc;
+  }
+  if (!ix->data.alloc(n * ix->D)) {
+    efg_destroy(ix);
+    return fail(EFG_ERR_MEMORY, "out of device memory (data)");
+  }
+  std::vector<float> pad;
+  const size_t rows_per = std::max<size_t>(1, (size_t(64) << 20) / (ix->D * 4));
+  for (size_t r0 = 0; r0 < n; r0 += rows_per) {
+    size_t r1 = std::min(n, r0 + rows_per);
+    pad.assign((r1 - r0) * ix->D, 0.f);
+    for (size_t i = r0; i < r1; ++i) std::memcpy(&pad[(i - r0) * ix->D], data + i * row_stride, d * 4);
+    int rc = check_hip(hipMemcpyAsync(ix->data.p + r0 * ix->D, pad.data(), pad.size() * 4, hipMemcpyHostToDevice, ix->stream), "copy");
+    rc |= check_hip(hipStreamSynchronize(ix->stream), "sync");
+    if (rc) {
+      efg_destroy(ix);
+      return EFG_ERR_DEVICE;
+    }
+  }
+  *out = ix;
+  return EFG_OK;
+}
+
+int efg_destroy(efg_index* ix) {
+  if (!ix) return EFG_OK;
+  if (ix->stream) hipStreamSynchronize(ix->stream);
+  hipStream_t st = ix->stream;
+  delete ix;
+  if (st) hipStreamDestroy(st);
+  return EFG_OK;
+}
+
+int efg_build_trees(efg_index* ix) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  double t0 = now_ms();
+  int rc = gpu_build_trees(ix);
+  ix->times[0] = now_ms() - t0;
+  return rc;
+}
+
+int efg_build_stage(efg_index* ix, int stage) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  switch (stage) {
+    case 0: return efg_build_trees(ix);
+    case 1: return gpu_merge_init(ix);
+    case 2: return gpu_refine_iter(ix);
+    case 3: return gpu_finalize_graph(ix);
+    default: return fail(EFG_ERR_ARG, "no such build stage");
+  }
+}
+
+int efg_build_index(efg_index* ix) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  const efg_build_params& p = ix->bp;
+  if (p.K <= 0 || p.S <= 0 || p.L <= 0 || p.check <= 0 || p.trees <= 0 || p.epochs < 0 || p.merge_level < 0)
+    return fail(EFG_ERR_ARG, "invalid build parameters");
+  if (p.K > p.L) return fail(EFG_ERR_ARG, "K must not exceed L");
+  ix->join_pairs.clear();
+  double t0 = now_ms();
+  if (int rc = gpu_build_trees(ix)) return rc;
+  double t1 = now_ms();
+  if (int rc = gpu_merge_init(ix)) return rc;
+  double t2 = now_ms();
+  ix->times[0] = t1 - t0;
+  ix->times[1] = t2 - t1;
+  ix->times[2] = ix->times[3] = 0;
+  for (int e = 0; e < p.epochs; ++e)
+    if (int rc = gpu_refine_iter(ix)) return rc;
+  return gpu_finalize_graph(ix);
+}
+
+int efg_set_search_params(efg_index* ix, int epochs, int init_num, int extend_to, int search_trees, int search_lv,
+                          int search_method) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  ix->s_epochs = epochs;
+  ix->s_init = init_num;
+  ix->s_extend = extend_to > init_num ? init_num : extend_to;
+  ix->s_lv = search_lv;
+  ix->s_trees = search_trees;
+  ix->s_method = search_method;
+  ix->s_set = true;
+  return EFG_OK;
+}
+
+int efg_get_search_params(const efg_index* ix, int* epochs, int* init_num, int* extend_to, int* search_trees,
+                          int* search_method) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (epochs) *epochs = ix->s_epochs;
+  if (init_num) *init_num = ix->s_init;
+  if (extend_to) *extend_to = ix->s_extend;
+  if (search_trees) *search_trees = ix->s_trees;
+  if (search_method) *search_method = ix->s_method;
+  return EFG_OK;
+}
+
+int efg_knn_search_device(efg_index* ix, int k, const float* d_queries, size_t nq, size_t q_stride, int32_t* d_ids,
+                          float* d_dists, uint64_t* d_ndist, void* stream) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ix->stream;
+  return search_device(ix, k, d_queries, nq, q_stride, d_ids, d_dists, d_ndist, st);
+}
+
+int efg_knn_search(efg_index* ix, int k, const float* queries, size_t nq, size_t q_stride) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (nq && (!queries || q_stride < ix->d)) return fail(EFG_ERR_ARG, "invalid query matrix");
+  if (k <= 0) return fail(EFG_ERR_ARG, "k must be positive");
+  efg::Workspace& w = ix->ws;
+  const size_t D = ix->D;
+  if (!w.qbuf.ensure(std::max<size_t>(1, nq * D)) || !w.ids.ensure(std::max<size_t>(1, nq * k)) ||
+      !w.dists.ensure(std::max<size_t>(1, nq * k)))
+    return fail(EFG_ERR_MEMORY, "out of device memory (queries)");
+  std::vector<float> pad(nq * D, 0.f);
+  for (size_t i = 0; i < nq; ++i) std::memcpy(&pad[i * D], queries + i * q_stride, ix->d * 4);
+  if (nq) {
+    if (int rc = check_hip(hipMemcpyAsync(w.qbuf.p, pad.data(), pad.size() * 4, hipMemcpyHostToDevice, ix->stream), "copy"))
+      return rc;
+  }
+  int rc = search_device(ix, k, w.qbuf.p, nq, D, w.ids.p, w.dists.p, nullptr, ix->stream);
+  if (rc) return rc;
+  return sync_results_to_host(ix, nq, k);
+}
+
+int efg_get_results(const efg_index* ix, size_t* nq, int* k, int32_t* ids, float* dists) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (nq) *nq = ix->res_nq;
+  if (k) *k = ix->res_k;
+  if (ids) std::memcpy(ids, ix->res_ids.data(), ix->res_ids.size() * 4);
+  if (dists) std::memcpy(dists, ix->res_d.data(), ix->res_d.size() * 4);
+  return EFG_OK;
+}
+
+int efg_graph_size(const efg_index* ix, size_t* n) {
+  if (!ix || !n) return fail(EFG_ERR_ARG, "null argument");
+  *n = ix->nnz ? ix->n : 0;
+  return EFG_OK;
+}
+
+int efg_graph_row(const efg_index* ix, size_t row, uint32_t* out, size_t cap, size_t* len) {
+  if (!ix || !len) return fail(EFG_ERR_ARG, "null argument");
+  *len = 0;
+  if (ix->nnz == 0 || row >= ix->n) return EFG_OK;  // empty row (base_index.hpp:495-501)
+  uint64_t o[2];
+  if (int rc = check_hip(hipMemcpy(o, ix->goff.p + row, 16, hipMemcpyDeviceToHost), "copy")) return rc;
+  size_t m = size_t(o[1] - o[0]);
+  *len = m;
+  if (out && cap) {
+    size_t c = std::min(cap, m);
+    if (c) {
+      if (int rc = check_hip(hipMemcpy(out, ix->gids.p + o[0], c * 4, hipMemcpyDeviceToHost), "copy")) return rc;
+    }
+  }
+  return EFG_OK;
+}
+
+int efg_save_graph(const efg_index* ix, const char* path) {
+  if (!ix || !path) return fail(EFG_ERR_ARG, "null argument");
+  std::vector<uint64_t> off;
+  std::vector<uint32_t> ids;
+  if (int rc = download_graph(ix, off, ids)) return rc;
+  std::ofstream out(path, std::ios::binary);
+  if (!out.is_open()) return fail(EFG_ERR_IO, "open file error");
+  const size_t rows = ix->nnz ? ix->n : 0;
+  for (size_t i = 0; i < rows; ++i) {
+    uint32_t k = uint32_t(off[i + 1] - off[i]);
+    out.write(reinterpret_cast<const char*>(&k), 4);
+    out.write(reinterpret_cast<const char*>(&ids[off[i]]), size_t(k) * 4);
+  }
+  return out ? EFG_OK : fail(EFG_ERR_IO, "write failed");
+}
+
+int efg_load_graph(efg_index* ix, const char* path) {
+  if (!ix || !path) return fail(EFG_ERR_ARG, "null argument");
+  std::ifstream in(path, std::ios::binary);
+  if (!in.is_open()) return fail(EFG_ERR_IO, "open file error");
+  in.seekg(0, std::ios::end);
+  size_t fsize = size_t(in.tellg());
+  in.seekg(0, std::ios::beg);
+  int32_t k0 = 0;
+  in.read(reinterpret_cast<char*>(&k0), 4);
+  if (!in || k0 < 0) return fail(EFG_ERR_FORMAT, "bad graph header");
+  size_t N = fsize / (size_t(k0) + 1) / 4;  // kdtreeub_index.hpp:253
+  if (N != ix->n) return fail(EFG_ERR_FORMAT, "graph row count differs from the data set");
+  in.seekg(0, std::ios::beg);
+  std::vector<uint64_t> off(N + 1, 0);
+  std::vector<uint32_t> ids;
+  ids.reserve(N * size_t(k0));
+  for (size_t i = 0; i < N; ++i) {
+    uint32_t k = 0;
+    in.read(reinterpret_cast<char*>(&k), 4);
+    if (!in || k > (1u << 20)) return fail(EFG_ERR_FORMAT, "truncated graph file");
+    size_t at = ids.size();
+    ids.resize(at + k);
+    in.read(reinterpret_cast<char*>(&ids[at]), size_t(k) * 4);
+    if (!in) return fail(EFG_ERR_FORMAT, "truncated graph file");
+    off[i + 1] = ids.size();
+  }
+  return upload_graph(ix, off, ids);
+}
+
+int efg_save_trees(const efg_index* ix, const char* path) {
+  if (!ix || !path) return fail(EFG_ERR_ARG, "null argument");
+  if (ix->T == 0) return fail(EFG_ERR_STATE, "no trees to save");
+  std::ofstream out(path, std::ios::binary);
+  if (!out.is_open()) return fail(EFG_ERR_IO, "open file error");
+  return write_trees(ix, out, uint32_t(ix->bp.K));
+}
+
+int efg_load_trees(efg_index* ix, const char* path) {
+  if (!ix || !path) return fail(EFG_ERR_ARG, "null argument");
+  std::ifstream in(path, std::ios::binary);
+  if (!in.is_open()) return fail(EFG_ERR_IO, "open file error");
+  return read_trees(ix, in, nullptr);
+}
+
+// saveIndex (kdtreeub_index.hpp:771-825): trees file followed by n rows of K
+// int ids.  The reference writes its merge heaps here, which are cleared after
+// a build (1424) so it exits; the graph (first K pool ids) is written instead.
+int efg_save_index(const efg_index* ix, const char* path) {
+  if (!ix || !path) return fail(EFG_ERR_ARG, "null argument");
+  if (ix->T == 0 || ix->nnz == 0) return fail(EFG_ERR_STATE, "index needs trees and a graph");
+  std::vector<uint64_t> off;
+  std::vector<uint32_t> ids;
+  if (int rc = download_graph(ix, off, ids)) return rc;
+  uint32_t K = uint32_t(off[1] - off[0]);
+  for (size_t i = 0; i < ix->n; ++i)
+    if (off[i + 1] - off[i] != K) return fail(EFG_ERR_STATE, "index file needs a graph of uniform degree");
+  std::ofstream out(path, std::ios::binary);
+  if (!out.is_open()) return fail(EFG_ERR_IO, "open file error");
+  if (int rc = write_trees(ix, out, K)) return rc;
+  out.write(reinterpret_cast<const char*>(ids.data()), ids.size() * 4);
+  return out ? EFG_OK : fail(EFG_ERR_IO, "write failed");
+}
+
+// loadIndex (kdtreeub_index.hpp:701-770): the K ids per row become the search
+// graph (the reference fills heaps the search never reads).
+int efg_load_index(efg_index* ix, const char* path) {
+  if (!ix || !path) return fail(EFG_ERR_ARG, "null argument");
+  std::ifstream in(path, std::ios::binary);
+  if (!in.is_open()) return fail(EFG_ERR_IO, "open file error");
+  uint32_t K = 0;
+  if (int rc = read_trees(ix, in, &K)) return rc;
+  std::vector<uint64_t> off(ix->n + 1);
+  std::vector<uint32_t> ids(ix->n * size_t(K));
+  in.read(reinterpret_cast<char*>(ids.data()), ids.size() * 4);
+  if (!in) return fail(EFG_ERR_FORMAT, "truncated graph section of the index file");
+  for (size_t i = 0; i <= ix->n; ++i) off[i] = i * K;
+  return upload_graph(ix, off, ids);
+}
+
+// saveResults (base_index.hpp:136-151): per query int k, then k int ids.
+// Rows report their valid entries only (method 1 can find fewer than k).
+int efg_save_results(const efg_index* ix, const char* path) {
+  if (!ix || !path) return fail(EFG_ERR_ARG, "null argument");
+  std::ofstream out(path, std::ios::binary);
+  if (!out.is_open()) return fail(EFG_ERR_IO, "open file error");
+  for (size_t q = 0; q < ix->res_nq; ++q) {
+    const int32_t* r = &ix->res_ids[q * ix->res_k];
+    int32_t m = 0;
+    while (m < ix->res_k && r[m] >= 0) ++m;
+    out.write(reinterpret_cast<const char*>(&m), 4);
+    out.write(reinterpret_cast<const char*>(r), size_t(m) * 4);
+  }
+  return out ? EFG_OK : fail(EFG_ERR_IO, "write failed");
+}
+
+int efg_exact_knn_device(const float* d_base, size_t n, size_t d, size_t base_stride, const float* d_queries,
+                         size_t nq, size_t q_stride, int k, int exclude_self, int32_t* d_ids, float* d_dists,
+                         void* stream) {
+  if (nq == 0) return EFG_OK;
+  if (!d_base || !d_queries || !d_ids || q_stride < d) return fail(EFG_ERR_ARG, "invalid arguments");
+  if (nq >= (size_t(1) << 31) || n >= (size_t(1) << 31)) return fail(EFG_ERR_ARG, "too large");
+  std::string err;
+  int rc = exact_knn(d_base, n, int(d), base_stride, d_queries, int(nq), q_stride, k, exclude_self, d_ids, d_dists,
+                     static_cast<hipStream_t>(stream), &err);
+  if (rc) return fail(rc, err);
+  return EFG_OK;
+}
+
+int efg_tree_shape(const efg_index* ix, int* trees, size_t* nodes) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (trees) *trees = ix->T;
+  if (nodes) *nodes = ix->M;
+  return EFG_OK;
+}
+
+int efg_get_trees(const efg_index* ix, int32_t* dim, float* val, int32_t* left, int32_t* right, uint32_t* start,
+                  uint32_t* end, int32_t* roots, uint32_t* leaf_lists) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  HostTrees h;
+  if (int rc = download_trees(ix, h)) return rc;
+  const size_t M = ix->M;
+  if (dim) std::memcpy(dim, h.dim.data(), M * 4);
+  if (val) std::memcpy(val, h.val.data(), M * 4);
+  if (left) std::memcpy(left, h.left.data(), M * 4);
+  if (right) std::memcpy(right, h.right.data(), M * 4);
+  if (start) std::memcpy(start, h.start.data(), M * 4);
+  if (end) std::memcpy(end, h.end.data(), M * 4);
+  if (roots) std::memcpy(roots, h.roots.data(), size_t(h.T) * 4);
+  if (leaf_lists) std::memcpy(leaf_lists, h.leaf.data(), h.leaf.size() * 4);
+  return EFG_OK;
+}
+
+int efg_set_trees(efg_index* ix, int trees, size_t nodes, const int32_t* dim, const float* val, const int32_t* left,
+                  const int32_t* right, const uint32_t* start, const uint32_t* end, const int32_t* roots,
+                  const uint32_t* leaf_lists) {
+  if (!ix || trees <= 0 || nodes == 0) return fail(EFG_ERR_ARG, "invalid trees");
+  HostTrees h;
+  h.T = trees;
+  h.dim.assign(dim, dim + nodes);
+  h.val.assign(val, val + nodes);
+  h.left.assign(left, left + nodes);
+  h.right.assign(right, right + nodes);
+  h.start.assign(start, start + nodes);
+  h.end.assign(end, end + nodes);
+  h.roots.assign(roots, roots + trees);
+  h.leaf.assign(leaf_lists, leaf_lists + size_t(trees) * ix->n);
+  for (size_t i = 0; i < nodes; ++i) {
+    bool inner = h.left[i] >= 0;
+    if (inner != (h.right[i] >= 0)) return fail(EFG_ERR_ARG, "node with one child");
+    if (inner && (h.left[i] >= int64_t(nodes) || h.right[i] >= int64_t(nodes) || h.dim[i] < 0 ||
+                  size_t(h.dim[i]) >= ix->d))
+      return fail(EFG_ERR_ARG, "bad node link or split dimension");
+    if (h.end[i] > ix->n || h.start[i] > h.end[i]) return fail(EFG_ERR_ARG, "bad node range");
+  }
+  for (uint32_t x : h.leaf)
+    if (x >= ix->n) return fail(EFG_ERR_ARG, "leaf list id out of range");
+  h.leaf_dim.assign(nodes, -1);
+  int rc = upload_trees(ix, h);
+  if (!rc) ix->s_trees = trees;
+  return rc;
+}
+
+int efg_get_graph(const efg_index* ix, uint64_t* off, uint32_t* ids, size_t cap) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  std::vector<uint64_t> o;
+  std::vector<uint32_t> g;
+  if (int rc = download_graph(ix, o, g)) return rc;
+  if (off) std::memcpy(off, o.data(), o.size() * 8);
+  if (ids) std::memcpy(ids, g.data(), std::min(cap, g.size()) * 4);
+  return EFG_OK;
+}
+
+int efg_set_graph(efg_index* ix, const uint64_t* off, const uint32_t* ids) {
+  if (!ix || !off) return fail(EFG_ERR_ARG, "null argument");
+  std::vector<uint64_t> o(off, off + ix->n + 1);
+  for (size_t i = 0; i < ix->n; ++i)
+    if (o[i + 1] < o[i]) return fail(EFG_ERR_ARG, "offsets must be non-decreasing");
+  std::vector<uint32_t> g(ids, ids + o[ix->n]);
+  return upload_graph(ix, o, g);
+}
+
+int efg_device_data(const efg_index* ix, const float** d_data, size_t* stride) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (d_data) *d_data = ix->data.p;
+  if (stride) *stride = ix->D;
+  return EFG_OK;
+}
+
+int efg_get_counters(const efg_index* ix, uint64_t* out, int n) {
+  if (!ix || !out) return fail(EFG_ERR_ARG, "null argument");
+  uint64_t c[8];
+  std::memcpy(c, ix->counters, sizeof c);
+  if (ix->ws.ndist.p) {
+    hipStreamSynchronize(ix->stream);
+    unsigned long long nd = 0;
+    int nv[2] = {0, 0};
+    hipMemcpy(&nd, ix->ws.ndist.p, 8, hipMemcpyDeviceToHost);
+    if (ix->ws.nover.p) hipMemcpy(nv, ix->ws.nover.p, 8, hipMemcpyDeviceToHost);
+    c[0] = nd;
+    c[4] = uint64_t(nv[0]);
+    c[5] = uint64_t(nv[1]);
+  }
+  for (int i = 0; i < n && i < 8; ++i) out[i] = c[i];
+  return EFG_OK;
+}
+
+int efg_get_join_pairs(const efg_index* ix, uint64_t* out, int cap, int* n) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (n) *n = int(ix->join_pairs.size());
+  for (int i = 0; i < cap && i < int(ix->join_pairs.size()); ++i) out[i] = ix->join_pairs[i];
+  return EFG_OK;
+}
+
+int efg_get_build_times(const efg_index* ix, double* out, int n) {
+  if (!ix || !out) return fail(EFG_ERR_ARG, "null argument");
+  for (int i = 0; i < n && i < 4; ++i) out[i] = ix->times[i];
+  return EFG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,146 @@
+// Host-side internals of libefg.so: the index object and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "efg.h"
+
+namespace efg {
+
+// Owning device buffer.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // grow-only (contents not preserved)
+  bool ensure(size_t count) {
+    if (count <= n && p) return true;
+    reset();
+    if (count == 0) count = 1;
+    if (hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      return false;
+    }
+    n = count;
+    return true;
+  }
+  bool alloc(size_t count) {
+    reset();
+    return ensure(count);
+  }
+};
+
+// Search launch configuration (all sizes in elements).
+struct SearchCfg {
+  int n;        // base rows
+  int d, D;     // dimension, padded row stride (multiple of 8)
+  int T;        // trees searched
+  int lsize;    // leaves per tree (kdtreeub_index.hpp:392)
+  int nleaf;    // T * lsize
+  int init_num; // seed pool
+  int E;        // extend_to
+  int epochs;
+  int K;        // results per query
+  int R;        // pool kept: max(extend_to, K)
+  int kb;       // key buffer (power of two)
+  int logcap;   // visited-set capacity (log2)
+  uint64_t seed;
+};
+
+// Device state of the NN-descent build (see k_graph.hip).
+struct BuildState {
+  int L = 0;                    // pool capacity
+  DevBuf<uint64_t> pool;        // n x L keys
+  DevBuf<uint8_t> flag;         // n x L
+  DevBuf<uint32_t> fill, range; // n
+  DevBuf<float> radiusM;        // n
+  DevBuf<uint8_t> found;        // n
+  DevBuf<uint64_t> new_off, old_off;  // CSR join lists
+  DevBuf<uint32_t> new_ids, old_ids;
+  size_t new_n = 0, old_n = 0;
+  int iter = 0;
+  bool ready = false;
+};
+
+struct Workspace {
+  DevBuf<int32_t> leaves, overflow;
+  DevBuf<int> nover;
+  DevBuf<unsigned long long> ndist;
+  DevBuf<uint32_t> gtab;
+  DevBuf<float> qbuf;
+  DevBuf<int32_t> ids;
+  DevBuf<float> dists;
+};
+
+}  // namespace efg
+
+struct efg_index {
+  size_t n = 0, d = 0, D = 0;
+  efg_build_params bp{};
+  // search parameters (base_index.hpp:153-161)
+  int s_epochs = 0, s_init = 0, s_extend = 0, s_trees = 0, s_lv = -1, s_method = 0;
+  bool s_set = false;
+  hipStream_t stream = nullptr;
+  efg::DevBuf<float> data;  // n x D, zero padded
+  // trees
+  int T = 0;
+  size_t M = 0;
+  efg::DevBuf<int4> node4;       // (dim, val bits, left, right)
+  efg::DevBuf<int32_t> parent;
+  efg::DevBuf<uint2> range;      // [start, end)
+  efg::DevBuf<int32_t> roots;
+  efg::DevBuf<uint32_t> leaf;    // T x n
+  std::vector<int8_t> leaf_dim;  // value written as DivDim of a leaf (0 or -1)
+  // graph (CSR)
+  size_t nnz = 0;
+  int max_deg = 0;
+  efg::DevBuf<uint64_t> goff;
+  efg::DevBuf<uint32_t> gids;
+  // build
+  efg::BuildState bs;
+  std::vector<uint64_t> join_pairs;
+  double times[4] = {0, 0, 0, 0};
+  uint64_t counters[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // results of the last host search
+  std::vector<int32_t> res_ids;
+  std::vector<float> res_d;
+  size_t res_nq = 0;
+  int res_k = 0;
+  efg::Workspace ws;
+};
+
+namespace efg {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int check_hip(hipError_t e, const char* what);
+
+// k_search.hip
+int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int32_t* d_ids, float* d_dists,
+                  uint64_t* d_ndist, hipStream_t st);
+// k_exact.hip
+int exact_knn(const float* X, size_t n, int d, size_t x_stride, const float* Q, int nq, size_t q_stride, int k,
+              int exclude_self, int32_t* ids, float* dists, hipStream_t st, std::string* err);
+// k_tree.hip
+int gpu_build_trees(efg_index* ix);
+// Exclusive scan of m ints into out[0..m]; out[m] and *total_dev get the total.
+int device_exclusive_scan(const int32_t* in, int m, int32_t* out, int32_t* total_dev, DevBuf<int32_t>& sums,
+                          hipStream_t st);
+// k_graph.hip
+int gpu_merge_init(efg_index* ix);
+int gpu_refine_iter(efg_index* ix);
+int gpu_finalize_graph(efg_index* ix);
+
+}  // namespace efg

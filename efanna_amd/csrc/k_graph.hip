@@ -1,0 +1,994 @@
+// kNN-graph construction after the trees: conquer-to-depth merge, random
+// init, NN-descent refine, graph output.
+//
+// Merge (kdtreeub_index.hpp:1000-1081, run as one thread runs it, 1367-1374):
+//   every point sees its candidates in the reference's single-thread order —
+//   trees in order; inside a tree, internal nodes X at depth >= ml deepest
+//   first (post-order along the point's root path); at X the point is either a
+//   prober of the smaller child (it meets the members of the leaf it reaches
+//   in the larger child, in range order) or a member of a leaf of the larger
+//   child (it meets, in range order, the probers that reach its leaf).
+//   Probe records (leaf reached, depth of X, prober position) are bucketed by
+//   leaf and sorted; one thread per point then replays its arrivals into an
+//   S-bounded (distance, id) set, appending rejected ids to nn_new while it
+//   holds fewer than 2S (1051-1077).  The result equals the sequential merge.
+// Init (1377-1423): nn_new below 2S is replaced by GenRandom's draw (counter-
+//   based generator); the set is topped up to S with generator ids; pool =
+//   the set in (distance, id) order, all flagged new; nn_new[0..S) = pool ids.
+// Join (base_index.hpp:328-371), batched: every pair of every node's new x
+//   new (i < j) and new x old lists is measured once; a candidate for a
+//   target whose pool is full is kept only when it beats the pool's last
+//   entry (as it stood when the iteration began).  Candidates are bucketed by
+//   target; each target's pool becomes the best L of (pool + candidates) by
+//   (distance, id), duplicates removed, existing entries keeping their flag.
+//   `found` of a joining node: one of its first-endpoint candidates sits at
+//   rank < Check_K in the merged pool (the lowest joining node gets it).
+// Update (base_index.hpp:373-437): Range / radiusM, forward new/old lists,
+//   reverse lists for edges longer than the other end's radiusM, reverse lists
+//   above 100 keep the 100 ids with the smallest generator value.
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "efg_device.hpp"
+#include "efg_internal.hpp"
+
+namespace efg {
+
+namespace {
+
+constexpr uint32_t kFirst = 0x80000000u;  // staging src flag: first endpoint of the pair
+
+__device__ inline float l2_thread(const float* __restrict__ a, const float* __restrict__ b, int D) {
+  // single-thread form of the reference's AVX order
+  const int DR = D & 31, DD = D - DR;
+  float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int c = DD; c < D; c += 8) {
+    const float4 x0 = *reinterpret_cast<const float4*>(a + c), x1 = *reinterpret_cast<const float4*>(a + c + 4);
+    const float4 y0 = *reinterpret_cast<const float4*>(b + c), y1 = *reinterpret_cast<const float4*>(b + c + 4);
+    float t;
+    t = x0.x - y0.x; acc[0] = __builtin_fmaf(t, t, acc[0]);
+    t = x0.y - y0.y; acc[1] = __builtin_fmaf(t, t, acc[1]);
+    t = x0.z - y0.z; acc[2] = __builtin_fmaf(t, t, acc[2]);
+    t = x0.w - y0.w; acc[3] = __builtin_fmaf(t, t, acc[3]);
+    t = x1.x - y1.x; acc[4] = __builtin_fmaf(t, t, acc[4]);
+    t = x1.y - y1.y; acc[5] = __builtin_fmaf(t, t, acc[5]);
+    t = x1.z - y1.z; acc[6] = __builtin_fmaf(t, t, acc[6]);
+    t = x1.w - y1.w; acc[7] = __builtin_fmaf(t, t, acc[7]);
+  }
+  for (int c = 0; c < DD; c += 8) {
+    const float4 x0 = *reinterpret_cast<const float4*>(a + c), x1 = *reinterpret_cast<const float4*>(a + c + 4);
+    const float4 y0 = *reinterpret_cast<const float4*>(b + c), y1 = *reinterpret_cast<const float4*>(b + c + 4);
+    float t;
+    t = x0.x - y0.x; acc[0] = __builtin_fmaf(t, t, acc[0]);
+    t = x0.y - y0.y; acc[1] = __builtin_fmaf(t, t, acc[1]);
+    t = x0.z - y0.z; acc[2] = __builtin_fmaf(t, t, acc[2]);
+    t = x0.w - y0.w; acc[3] = __builtin_fmaf(t, t, acc[3]);
+    t = x1.x - y1.x; acc[4] = __builtin_fmaf(t, t, acc[4]);
+    t = x1.y - y1.y; acc[5] = __builtin_fmaf(t, t, acc[5]);
+    t = x1.z - y1.z; acc[6] = __builtin_fmaf(t, t, acc[6]);
+    t = x1.w - y1.w; acc[7] = __builtin_fmaf(t, t, acc[7]);
+  }
+  float s = acc[0];
+  for (int j = 1; j < 8; ++j) s = s + acc[j];
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// merge
+// ---------------------------------------------------------------------------
+__global__ void k_node_depth(const int32_t* __restrict__ parent, int M, int8_t* __restrict__ depth) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  int dd = 0, x = i;
+  while (parent[x] >= 0) {
+    x = parent[x];
+    ++dd;
+  }
+  depth[i] = int8_t(dd > 127 ? 127 : dd);
+}
+
+// leaf_at[t][pos] = leaf holding position pos; error flag for leaves shallower
+// than ml in the first TB trees (getMergeLevelNodeList, 1000-1011).
+__global__ void k_leaf_at(const int4* __restrict__ node4, const uint2* __restrict__ range,
+                          const int8_t* __restrict__ depth, const int32_t* __restrict__ tree_of, int M, uint32_t n,
+                          int TB, int ml, int32_t* __restrict__ leaf_at, int* __restrict__ err_depth) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  if (node4[i].z >= 0) return;
+  int t = tree_of[i];
+  uint2 r = range[i];
+  for (uint32_t p = r.x; p < r.y; ++p) leaf_at[size_t(t) * n + p] = i;
+  if (t < TB && depth[i] < ml) atomicMin(err_depth, int(depth[i]));
+}
+
+__device__ inline int32_t descend(const int4* __restrict__ node4, int32_t x, const float* __restrict__ q) {
+  for (;;) {
+    int4 nd = node4[x];
+    if (nd.z < 0) return x;
+    x = (q[nd.x] < __int_as_float(nd.y)) ? nd.z : nd.w;
+  }
+}
+
+// For each position: walk the ancestors at depth >= ml; where the position is
+// in the smaller child, probe the larger one.  Pass 0 counts records per leaf
+// reached, pass 1 writes (63 - depth(X)) << 32 | pos into the leaf's bucket.
+__global__ void k_probe(const float* __restrict__ X, int D, const int4* __restrict__ node4,
+                        const int32_t* __restrict__ parent, const uint2* __restrict__ range,
+                        const int8_t* __restrict__ depth, const uint32_t* __restrict__ leaf_t,
+                        const int32_t* __restrict__ leaf_at_t, uint32_t n, int ml, int pass,
+                        uint32_t* __restrict__ cnt, const int32_t* __restrict__ boff,
+                        uint32_t* __restrict__ cursor, uint64_t* __restrict__ rec) {
+  uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= n) return;
+  const uint32_t p = leaf_t[pos];
+  const float* xp = X + size_t(p) * D;
+  int32_t child = leaf_at_t[pos];
+  int32_t x = parent[child];
+  while (x >= 0 && depth[x] >= ml) {
+    const int4 nd = node4[x];
+    const uint2 rl = range[nd.z], rr = range[nd.w];
+    const uint32_t numL = rl.y - rl.x, numR = rr.y - rr.x;
+    const int32_t small = (numL < numR) ? nd.z : nd.w;
+    if (child == small) {
+      const int32_t big = (small == nd.z) ? nd.w : nd.z;
+      const int32_t lam = descend(node4, big, xp);
+      if (pass == 0) {
+        atomicAdd(&cnt[lam], 1u);
+      } else {
+        uint32_t at = atomicAdd(&cursor[lam], 1u);
+        rec[size_t(boff[lam]) + at] = (uint64_t(63 - depth[x]) << 32) | pos;
+      }
+    }
+    child = x;
+    x = parent[x];
+  }
+}
+
+__global__ void k_sort_buckets(const int32_t* __restrict__ boff, int M, uint64_t* __restrict__ rec) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  int b0 = boff[i], b1 = boff[i + 1];
+  for (int a = b0 + 1; a < b1; ++a) {  // insertion sort (buckets are small)
+    uint64_t v = rec[a];
+    int j = a - 1;
+    while (j >= b0 && rec[j] > v) {
+      rec[j + 1] = rec[j];
+      --j;
+    }
+    rec[j + 1] = v;
+  }
+}
+
+// bounded (distance, id) set of capacity S kept ascending in h[0..sz)
+__device__ inline void heap_offer(uint64_t* h, int& sz, int S, uint32_t* extra, uint32_t& ecnt, uint32_t id,
+                                  float dist) {
+  if (sz < S || dist < key_dist(h[sz - 1])) {
+    uint64_t k = make_key(dist, id);
+    int i = sz;
+    while (i > 0 && h[i - 1] > k) --i;
+    if (i > 0 && h[i - 1] == k) return;  // already present
+    int last = sz < S ? sz : S - 1;
+    for (int j = last; j > i; --j) h[j] = h[j - 1];
+    h[i] = k;
+    if (sz < S) ++sz;
+  } else if (ecnt < uint32_t(2 * S)) {
+    extra[ecnt++] = id;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_replay(const float* __restrict__ X, int D, const int4* __restrict__ node4,
+                                                   const int32_t* __restrict__ parent, const uint2* __restrict__ range,
+                                                   const int8_t* __restrict__ depth,
+                                                   const uint32_t* __restrict__ leaf_t,
+                                                   const int32_t* __restrict__ leaf_at_t, uint32_t n, int ml, int S,
+                                                   const int32_t* __restrict__ boff, const uint64_t* __restrict__ rec,
+                                                   uint64_t* __restrict__ hkey, uint32_t* __restrict__ hsize,
+                                                   uint32_t* __restrict__ extra, uint32_t* __restrict__ ecount,
+                                                   unsigned long long* __restrict__ ndist) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  uint64_t* h = reinterpret_cast<uint64_t*>(smem_raw) + size_t(threadIdx.x) * S;
+  uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long nd_local = 0;
+  if (pos < n) {
+    const uint32_t p = leaf_t[pos];
+    const float* xp = X + size_t(p) * D;
+    int sz = int(hsize[p]);
+    for (int j = 0; j < sz; ++j) h[j] = hkey[size_t(p) * S + j];
+    uint32_t ecnt = ecount[p];
+    uint32_t* ex = extra + size_t(p) * 2 * S;
+    const int32_t lf = leaf_at_t[pos];
+    int rb = boff[lf];
+    const int re = boff[lf + 1];
+    int32_t child = lf;
+    int32_t x = parent[child];
+    while (x >= 0 && depth[x] >= ml) {
+      const int4 nd = node4[x];
+      const uint2 rl = range[nd.z], rr = range[nd.w];
+      const uint32_t numL = rl.y - rl.x, numR = rr.y - rr.x;
+      const int32_t small = (numL < numR) ? nd.z : nd.w;
+      if (child == small) {
+        const int32_t big = (small == nd.z) ? nd.w : nd.z;
+        const int32_t lam = descend(node4, big, xp);
+        const uint2 r = range[lam];
+        for (uint32_t i = r.x; i < r.y; ++i) {
+          uint32_t q = leaf_t[i];
+          float dist = l2_thread(X + size_t(q) * D, xp, D);
+          ++nd_local;
+          heap_offer(h, sz, S, ex, ecnt, q, dist);
+        }
+      } else {
+        const uint64_t want = uint64_t(63 - depth[x]);
+        while (rb < re && (rec[rb] >> 32) < want) ++rb;
+        while (rb < re && (rec[rb] >> 32) == want) {
+          uint32_t q = leaf_t[uint32_t(rec[rb] & 0xffffffffu)];
+          float dist = l2_thread(xp, X + size_t(q) * D, D);
+          heap_offer(h, sz, S, ex, ecnt, q, dist);
+          ++rb;
+        }
+      }
+      child = x;
+      x = parent[x];
+    }
+    for (int j = 0; j < sz; ++j) hkey[size_t(p) * S + j] = h[j];
+    hsize[p] = uint32_t(sz);
+    ecount[p] = ecnt;
+  }
+  // per-block distance count
+  __shared__ unsigned long long red[kBlock / 64];
+  unsigned long long v = nd_local;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long s = 0;
+    for (int w = 0; w < kBlock / 64; ++w) s += red[w];
+    atomicAdd(ndist, s);
+  }
+}
+
+// Random init (kdtreeub_index.hpp:1377-1423).
+__global__ void k_init(const float* __restrict__ X, int D, uint32_t n, int S, int L, uint64_t seed,
+                       uint64_t* __restrict__ hkey, uint32_t* __restrict__ hsize, const uint32_t* __restrict__ extra,
+                       const uint32_t* __restrict__ ecount, uint64_t* __restrict__ pool, uint8_t* __restrict__ flag,
+                       uint32_t* __restrict__ fill, uint32_t* __restrict__ rng, uint32_t* __restrict__ new_ids,
+                       unsigned long long* __restrict__ ndist) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint32_t* nw = new_ids + size_t(p) * 2 * S;
+  const uint32_t S2 = 2u * S;
+  if (ecount[p] < S2) {  // GenRandom(rng, nn_new, 2S, N) (1121-1135)
+    for (uint32_t i = 0; i < S2; ++i) nw[i] = uint32_t(hash4(seed, ST_INIT_NEW, p, i) % uint64_t(n - S2));
+    for (uint32_t i = 1; i < S2; ++i) {  // sort
+      uint32_t v = nw[i];
+      int j = int(i) - 1;
+      while (j >= 0 && nw[j] > v) {
+        nw[j + 1] = nw[j];
+        --j;
+      }
+      nw[j + 1] = v;
+    }
+    for (uint32_t i = 1; i < S2; ++i)
+      if (nw[i] <= nw[i - 1]) nw[i] = nw[i - 1] + 1;
+    uint32_t off = uint32_t(hash4(seed, ST_INIT_NEW, p, S2) % uint64_t(n));
+    for (uint32_t i = 0; i < S2; ++i) nw[i] = (nw[i] + off) % n;
+  } else {
+    for (uint32_t i = 0; i < S2; ++i) nw[i] = extra[size_t(p) * S2 + i];
+  }
+  uint64_t* h = hkey + size_t(p) * S;
+  int sz = int(hsize[p]);
+  uint64_t r = 0;
+  unsigned long long cnt = 0;
+  while (sz < S) {  // top up to S (1399-1405)
+    uint32_t rid = uint32_t(hash4(seed, ST_INIT_TOP, p, r++) % uint64_t(n));
+    float dist = l2_thread(X + size_t(p) * D, X + size_t(rid) * D, D);
+    ++cnt;
+    uint64_t k = make_key(dist, rid);
+    int i = sz;
+    while (i > 0 && h[i - 1] > k) --i;
+    if (i > 0 && h[i - 1] == k) continue;
+    for (int j = sz; j > i; --j) h[j] = h[j - 1];
+    h[i] = k;
+    ++sz;
+  }
+  for (int l = 0; l < L; ++l) {
+    pool[size_t(p) * L + l] = l < S ? h[l] : kKeyMax;
+    flag[size_t(p) * L + l] = l < S ? 1 : 0;
+  }
+  for (int l = 0; l < S; ++l) nw[l] = key_id(h[l]);
+  fill[p] = uint32_t(S);
+  rng[p] = uint32_t(S);
+  if (cnt) atomicAdd(ndist, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// join
+// ---------------------------------------------------------------------------
+__global__ void k_worst(const uint64_t* __restrict__ pool, const uint32_t* __restrict__ fill, uint32_t n, int L,
+                        uint64_t* __restrict__ worst) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  worst[p] = fill[p] < uint32_t(L) ? kKeyMax : pool[size_t(p) * L + L - 1];
+}
+
+struct Stage {
+  uint32_t target;
+  uint32_t src;  // joining node | kFirst
+  uint64_t key;
+};
+
+// One workgroup per joining node; two lanes per pair.
+__global__ void __launch_bounds__(kBlock) k_join(const float* __restrict__ X, int D, uint32_t n,
+                                                 const uint64_t* __restrict__ new_off,
+                                                 const uint32_t* __restrict__ new_ids,
+                                                 const uint64_t* __restrict__ old_off,
+                                                 const uint32_t* __restrict__ old_ids,
+                                                 const uint64_t* __restrict__ worst, int stage_rows,
+                                                 Stage* __restrict__ stage, unsigned long long cap,
+                                                 unsigned long long* __restrict__ nstage,
+                                                 uint32_t* __restrict__ tcount,
+                                                 unsigned long long* __restrict__ npairs) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float* rows = reinterpret_cast<float*>(smem_raw);
+  __shared__ uint32_t ids_s[512];
+  const int tid = threadIdx.x, pr = tid >> 1, h = tid & 1;
+  unsigned long long pairs_local = 0;
+  for (uint32_t nn = blockIdx.x; nn < n; nn += gridDim.x) {
+    const uint64_t a0 = new_off[nn], a1 = new_off[nn + 1], b0 = old_off[nn], b1 = old_off[nn + 1];
+    const int na = int(a1 - a0), nb = int(b1 - b0);
+    if (na == 0) continue;
+    const int m = na + nb;
+    const bool staged = m <= stage_rows && m <= 512;
+    if (m <= 512) {
+      for (int i = tid; i < m; i += kBlock) ids_s[i] = i < na ? new_ids[a0 + i] : old_ids[b0 + i - na];
+    }
+    __syncthreads();
+    if (staged) {
+      const int D4 = D / 4;
+      for (int i = tid; i < m * D4; i += kBlock) {
+        int r = i / D4, c = i % D4;
+        reinterpret_cast<float4*>(rows)[size_t(r) * D4 + c] =
+            reinterpret_cast<const float4*>(X + size_t(ids_s[r]) * D)[c];
+      }
+    }
+    __syncthreads();
+    const long long total = (long long)na * na + (long long)na * nb;
+    for (long long k0 = 0; k0 < total; k0 += kBlock / 2) {
+      long long k = k0 + pr;
+      bool valid = k < total;
+      int ia = 0, ib = 0;
+      uint32_t a = 0, b = 0;
+      if (valid) {
+        if (k < (long long)na * na) {
+          ia = int(k / na);
+          ib = int(k % na);
+        } else {
+          long long kk = k - (long long)na * na;
+          ia = int(kk / nb);
+          ib = na + int(kk % nb);
+        }
+        a = m <= 512 ? ids_s[ia] : (ia < na ? new_ids[a0 + ia] : old_ids[b0 + ia - na]);
+        b = m <= 512 ? ids_s[ib] : (ib < na ? new_ids[a0 + ib] : old_ids[b0 + ib - na]);
+        if (ib < na && !(a < b)) valid = false;
+      }
+      // both lanes of a pair agree on `valid`
+      if (valid) {
+        const float* ra = staged ? rows + size_t(ia) * D : X + size_t(a) * D;
+        const float* rb = staged ? rows + size_t(ib) * D : X + size_t(b) * D;
+        float dist = l2_pair_lanes(ra, rb, D, h);
+        if (h) {
+          ++pairs_local;
+          uint64_t ka = make_key(dist, b), kb = make_key(dist, a);
+          if (ka < worst[a]) {
+            unsigned long long at = atomicAdd(nstage, 1ull);
+            if (at < cap) {
+              stage[at] = Stage{a, nn | kFirst, ka};
+              atomicAdd(&tcount[a], 1u);
+            }
+          }
+          if (kb < worst[b]) {
+            unsigned long long at = atomicAdd(nstage, 1ull);
+            if (at < cap) {
+              stage[at] = Stage{b, nn, kb};
+              atomicAdd(&tcount[b], 1u);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  atomicAdd(npairs, pairs_local);
+}
+
+__global__ void k_bucket(const Stage* __restrict__ stage, unsigned long long m, const int32_t* __restrict__ toff,
+                         uint32_t* __restrict__ cursor, uint64_t* __restrict__ bkey, uint32_t* __restrict__ bsrc) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  Stage s = stage[i];
+  uint32_t at = atomicAdd(&cursor[s.target], 1u);
+  bkey[size_t(toff[s.target]) + at] = s.key;
+  bsrc[size_t(toff[s.target]) + at] = s.src;
+}
+
+// Per target: pool' = best L of (pool + candidates).  Entries are sorted by
+// (key, tag): existing entries carry tag = flag (0/1); candidates carry
+// 1<<31 | (not first endpoint) << 30 | joining node, so duplicates resolve to
+// the existing entry, else to the lowest first-endpoint joining node.
+__global__ void __launch_bounds__(kBlock) k_merge_pool(uint32_t n, int L, int CK, const int32_t* __restrict__ toff,
+                                                       const uint64_t* __restrict__ bkey,
+                                                       const uint32_t* __restrict__ bsrc,
+                                                       uint64_t* __restrict__ pool, uint8_t* __restrict__ flag,
+                                                       uint32_t* __restrict__ fill, uint8_t* __restrict__ credit,
+                                                       int chunk) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  uint64_t* k = reinterpret_cast<uint64_t*>(smem_raw);
+  uint32_t* tg = reinterpret_cast<uint32_t*>(k + chunk);
+  __shared__ uint64_t ck[kBlock];
+  __shared__ uint32_t ctg[kBlock];
+  __shared__ int keep_s;
+  const int tid = threadIdx.x;
+  for (uint32_t x = blockIdx.x; x < n; x += gridDim.x) {
+    const int c0 = toff[x], c1 = toff[x + 1];
+    if (c0 == c1) continue;
+    const int Lx = int(fill[x]);
+    // current list: existing pool
+    int cur = Lx;
+    for (int i = tid; i < Lx; i += kBlock) {
+      k[i] = pool[size_t(x) * L + i];
+      tg[i] = flag[size_t(x) * L + i];
+    }
+    for (int base = c0; base < c1;) {
+      int take = c1 - base;
+      if (take > chunk - cur) take = chunk - cur;
+      for (int i = tid; i < take; i += kBlock) {
+        uint32_t s = bsrc[base + i];
+        k[cur + i] = bkey[base + i];
+        tg[cur + i] = 0x80000000u | ((s & kFirst) ? 0u : 0x40000000u) | (s & 0x3fffffffu);
+      }
+      base += take;
+      int m = cur + take;
+      int p2 = next_pow2(m < 2 ? 2 : m);
+      for (int i = m + tid; i < p2; i += kBlock) {
+        k[i] = kKeyMax;
+        tg[i] = 0xffffffffu;
+      }
+      __syncthreads();
+      // bitonic sort on (key, tag)
+      for (int kk = 2; kk <= p2; kk <<= 1) {
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+          for (int i = tid; i < p2; i += kBlock) {
+            int ixj = i ^ j;
+            if (ixj > i) {
+              uint64_t ka = k[i], kb = k[ixj];
+              uint32_t ta = tg[i], tb = tg[ixj];
+              bool gt = (ka > kb) || (ka == kb && ta > tb);
+              bool up = (i & kk) == 0;
+              if (gt == up) {
+                k[i] = kb;
+                k[ixj] = ka;
+                tg[i] = tb;
+                tg[ixj] = ta;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      // dedup keeping the first of equal keys, keep the first L
+      int kept = 0;
+      for (int b0 = 0; b0 < m; b0 += kBlock) {
+        int i = b0 + tid;
+        int first = 0;
+        uint64_t kv = 0;
+        uint32_t tv = 0;
+        if (i < m) {
+          kv = k[i];
+          tv = tg[i];
+          first = (i == 0 || k[i - 1] != kv) ? 1 : 0;
+        }
+        __shared__ int scan_tmp[kBlock / 64];
+        int tot;
+        int rk = block_exclusive_scan(first, scan_tmp, &tot);
+        if (first && kept + rk < L) {
+          ck[tid] = kv;  // staged per chunk of threads
+          ctg[tid] = tv;
+        }
+        __syncthreads();
+        // write back compacted entries to the front (positions kept + rk < b0 + tid)
+        if (first && kept + rk < L) {
+          k[kept + rk] = ck[tid];
+          tg[kept + rk] = ctg[tid];
+        }
+        __syncthreads();
+        kept += tot;
+        if (kept >= L) {
+          kept = L;
+          break;
+        }
+      }
+      cur = kept;
+      if (tid == 0) keep_s = kept;
+      __syncthreads();
+    }
+    // write pool and credits
+    const int out = cur;
+    for (int i = tid; i < L; i += kBlock) {
+      if (i < out) {
+        uint32_t t = tg[i];
+        pool[size_t(x) * L + i] = k[i];
+        flag[size_t(x) * L + i] = (t & 0x80000000u) ? 1 : uint8_t(t & 1u);
+        if ((t & 0xc0000000u) == 0x80000000u && i < CK) credit[t & 0x3fffffffu] = 1;
+      } else {
+        pool[size_t(x) * L + i] = kKeyMax;
+        flag[size_t(x) * L + i] = 0;
+      }
+    }
+    if (tid == 0) fill[x] = uint32_t(out);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// update
+// ---------------------------------------------------------------------------
+__global__ void k_range(const uint64_t* __restrict__ pool, const uint8_t* __restrict__ flag,
+                        const uint32_t* __restrict__ fill, const uint8_t* __restrict__ found, uint32_t n, int L, int S,
+                        uint32_t* __restrict__ rng, float* __restrict__ radiusM) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint32_t R = rng[p];
+  if (found[p]) {
+    uint32_t maxl = R + uint32_t(S) < fill[p] ? R + uint32_t(S) : fill[p];
+    uint32_t c = 0, l = 0;
+    while (l < maxl && c < uint32_t(S)) {
+      if (flag[size_t(p) * L + l]) ++c;
+      ++l;
+    }
+    R = l;
+    rng[p] = R;
+  }
+  radiusM[p] = key_dist(pool[size_t(p) * L + R - 1]);
+}
+
+// pass 0: forward counts and reverse counts; pass 1: reverse fill
+__global__ void k_lists(const uint64_t* __restrict__ pool, const uint8_t* __restrict__ flag,
+                        const uint32_t* __restrict__ rng, const float* __restrict__ radiusM, uint32_t n, int L, int pass,
+                        uint32_t* __restrict__ fcnt_new, uint32_t* __restrict__ fcnt_old,
+                        uint32_t* __restrict__ rcnt_new, uint32_t* __restrict__ rcnt_old,
+                        const int32_t* __restrict__ roff_new, const int32_t* __restrict__ roff_old,
+                        uint32_t* __restrict__ rcur_new, uint32_t* __restrict__ rcur_old,
+                        uint32_t* __restrict__ rids_new, uint32_t* __restrict__ rids_old) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t R = rng[p];
+  uint32_t fn = 0, fo = 0;
+  for (uint32_t l = 0; l < R; ++l) {
+    uint64_t e = pool[size_t(p) * L + l];
+    uint32_t o = key_id(e);
+    float dist = key_dist(e);
+    bool isnew = flag[size_t(p) * L + l] != 0;
+    bool rev = dist > radiusM[o];
+    if (isnew) ++fn; else ++fo;
+    if (!rev) continue;
+    if (pass == 0) {
+      atomicAdd(isnew ? &rcnt_new[o] : &rcnt_old[o], 1u);
+    } else if (isnew) {
+      rids_new[size_t(roff_new[o]) + atomicAdd(&rcur_new[o], 1u)] = p;
+    } else {
+      rids_old[size_t(roff_old[o]) + atomicAdd(&rcur_old[o], 1u)] = p;
+    }
+  }
+  if (pass == 0) {
+    fcnt_new[p] = fn;
+    fcnt_old[p] = fo;
+  }
+}
+
+// Reverse lists above `cap` keep the `cap` members with the smallest
+// generator value (hash of iteration, owner, member); one workgroup per list.
+__global__ void __launch_bounds__(kBlock) k_sample(uint32_t n, const int32_t* __restrict__ roff,
+                                                   uint32_t* __restrict__ rids, uint32_t* __restrict__ rsize, int cap,
+                                                   uint64_t seed, uint64_t stream, uint64_t iter, int chunk) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  uint64_t* k = reinterpret_cast<uint64_t*>(smem_raw);
+  uint32_t* v = reinterpret_cast<uint32_t*>(k + chunk);
+  const int tid = threadIdx.x;
+  for (uint32_t o = blockIdx.x; o < n; o += gridDim.x) {
+    const int c0 = roff[o], c1 = roff[o + 1];
+    const int c = c1 - c0;
+    if (c <= cap) {
+      if (tid == 0) rsize[o] = uint32_t(c);
+      continue;
+    }
+    int cur = 0;
+    for (int base = c0; base < c1;) {
+      int take = c1 - base;
+      if (take > chunk - cur) take = chunk - cur;
+      for (int i = tid; i < take; i += kBlock) {
+        uint32_t x = rids[base + i];
+        k[cur + i] = hash4(seed, stream, (iter << 32) | o, x);
+        v[cur + i] = x;
+      }
+      base += take;
+      int m = cur + take;
+      int p2 = next_pow2(m);
+      for (int i = m + tid; i < p2; i += kBlock) {
+        k[i] = kKeyMax;
+        v[i] = 0xffffffffu;
+      }
+      __syncthreads();
+      for (int kk = 2; kk <= p2; kk <<= 1) {
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+          for (int i = tid; i < p2; i += kBlock) {
+            int ixj = i ^ j;
+            if (ixj > i) {
+              uint64_t ka = k[i], kb = k[ixj];
+              uint32_t va = v[i], vb = v[ixj];
+              bool gt = (ka > kb) || (ka == kb && va > vb);
+              bool up = (i & kk) == 0;
+              if (gt == up) {
+                k[i] = kb;
+                k[ixj] = ka;
+                v[i] = vb;
+                v[ixj] = va;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      cur = cap;
+    }
+    for (int i = tid; i < cap; i += kBlock) rids[c0 + i] = v[i];
+    if (tid == 0) rsize[o] = uint32_t(cap);
+    __syncthreads();
+  }
+}
+
+__global__ void k_list_sizes(uint32_t n, const uint32_t* __restrict__ fnew, const uint32_t* __restrict__ rnew,
+                             const uint32_t* __restrict__ fold, const uint32_t* __restrict__ rold,
+                             int32_t* __restrict__ snew, int32_t* __restrict__ sold) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  snew[p] = int32_t(fnew[p] + rnew[p]);
+  sold[p] = int32_t(fold[p] + rold[p]);
+}
+
+// Final join lists: forward part from the pool (new flags cleared), then the
+// sampled reverse part.
+__global__ void k_list_fill(uint32_t n, int L, const uint64_t* __restrict__ pool, uint8_t* __restrict__ flag,
+                            const uint32_t* __restrict__ rng, const int32_t* __restrict__ onew,
+                            const int32_t* __restrict__ oold, const int32_t* __restrict__ roff_new,
+                            const uint32_t* __restrict__ rnew_ids, const uint32_t* __restrict__ rnew_sz,
+                            const int32_t* __restrict__ roff_old, const uint32_t* __restrict__ rold_ids,
+                            const uint32_t* __restrict__ rold_sz, uint64_t* __restrict__ new_off,
+                            uint32_t* __restrict__ new_ids, uint64_t* __restrict__ old_off,
+                            uint32_t* __restrict__ old_ids) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > n) return;
+  new_off[p] = uint64_t(onew[p]);
+  old_off[p] = uint64_t(oold[p]);
+  if (p == n) return;
+  uint32_t wn = uint32_t(onew[p]), wo = uint32_t(oold[p]);
+  const uint32_t R = rng[p];
+  for (uint32_t l = 0; l < R; ++l) {
+    uint32_t o = key_id(pool[size_t(p) * L + l]);
+    if (flag[size_t(p) * L + l]) {
+      new_ids[wn++] = o;
+      flag[size_t(p) * L + l] = 0;
+    } else {
+      old_ids[wo++] = o;
+    }
+  }
+  for (uint32_t i = 0; i < rnew_sz[p]; ++i) new_ids[wn++] = rnew_ids[size_t(roff_new[p]) + i];
+  for (uint32_t i = 0; i < rold_sz[p]; ++i) old_ids[wo++] = rold_ids[size_t(roff_old[p]) + i];
+}
+
+__global__ void k_graph_out(uint32_t n, int L, int K, const uint64_t* __restrict__ pool,
+                            const uint32_t* __restrict__ fill, uint64_t* __restrict__ goff,
+                            uint32_t* __restrict__ gids) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > n) return;
+  goff[p] = uint64_t(p) * K;
+  if (p == n) return;
+  uint32_t f = fill[p];
+  for (int j = 0; j < K; ++j) {
+    uint32_t jj = uint32_t(j) < f ? uint32_t(j) : f - 1;
+    gids[size_t(p) * K + j] = key_id(pool[size_t(p) * L + jj]);
+  }
+}
+
+__global__ void k_u64_from_i32(const int32_t* __restrict__ in, uint64_t* __restrict__ out, int m) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= m) out[i] = uint64_t(uint32_t(in[i]));
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int grid_for(size_t n) { return int((n + 255) / 256); }
+
+}  // namespace
+
+int gpu_merge_init(efg_index* ix) {
+  const efg_build_params& p = ix->bp;
+  if (ix->T == 0) return fail(EFG_ERR_STATE, "no trees: build trees first");
+  const int TB = p.trees_build > 0 ? p.trees_build : p.trees;
+  if (TB > ix->T) return fail(EFG_ERR_ARG, "invalid number of trees for the graph");
+  const uint32_t n = uint32_t(ix->n);
+  const int S = p.S, L = p.L, D = int(ix->D);
+  if (n <= uint32_t(2 * S + 1)) return fail(EFG_ERR_ARG, "too few points for the requested S");
+  if (S > L) return fail(EFG_ERR_ARG, "S must not exceed L");
+  if (S > 64) return fail(EFG_ERR_ARG, "S above 64 is not supported");
+  if (L > 1024) return fail(EFG_ERR_ARG, "L above 1024 is not supported");
+  hipStream_t st = ix->stream;
+  const int M = int(ix->M);
+  BuildState& bs = ix->bs;
+  bs.ready = false;
+  DevBuf<int8_t> depth;
+  DevBuf<int32_t> tree_of, leaf_at, boff, sums, scal;
+  DevBuf<uint32_t> cnt, cursor, hsize, ecount, extra;
+  DevBuf<uint64_t> hkey, rec;
+  DevBuf<unsigned long long> nd;
+  if (!depth.alloc(M) || !tree_of.alloc(M) || !leaf_at.alloc(size_t(ix->T) * n) || !boff.alloc(M + 1) ||
+      !scal.alloc(2) || !cnt.alloc(M) || !cursor.alloc(M) || !hsize.alloc(n) || !ecount.alloc(n) ||
+      !extra.alloc(size_t(n) * 2 * S) || !hkey.alloc(size_t(n) * S) || !nd.alloc(2))
+    return fail(EFG_ERR_MEMORY, "out of device memory (merge)");
+  // tree of each node (from the roots' ranges: walk parents)
+  {
+    std::vector<int32_t> h_roots(ix->T);
+    hipMemcpy(h_roots.data(), ix->roots.p, ix->T * 4, hipMemcpyDeviceToHost);
+    std::vector<int32_t> par(M);
+    hipMemcpy(par.data(), ix->parent.p, size_t(M) * 4, hipMemcpyDeviceToHost);
+    std::vector<int32_t> tof(M, -1);
+    for (int t = 0; t < ix->T; ++t) tof[h_roots[t]] = t;
+    // nodes are created after their parents in every tree this library builds
+    // or loads from the reference format (post-order files are re-linked, so
+    // resolve by walking up)
+    for (int i = 0; i < M; ++i) {
+      if (tof[i] >= 0) continue;
+      int x = i;
+      std::vector<int> path;
+      while (tof[x] < 0 && par[x] >= 0) {
+        path.push_back(x);
+        x = par[x];
+      }
+      for (int y : path) tof[y] = tof[x];
+    }
+    hipMemcpyAsync(tree_of.p, tof.data(), size_t(M) * 4, hipMemcpyHostToDevice, st);
+    hipStreamSynchronize(st);
+  }
+  k_node_depth<<<grid_for(M), 256, 0, st>>>(ix->parent.p, M, depth.p);
+  int big = 127;
+  hipMemcpyAsync(scal.p, &big, 4, hipMemcpyHostToDevice, st);
+  k_leaf_at<<<grid_for(M), 256, 0, st>>>(ix->node4.p, ix->range.p, depth.p, tree_of.p, M, n, TB, p.merge_level,
+                                         leaf_at.p, scal.p);
+  hipMemsetAsync(hsize.p, 0, size_t(n) * 4, st);
+  hipMemsetAsync(ecount.p, 0, size_t(n) * 4, st);
+  hipMemsetAsync(nd.p, 0, 16, st);
+  int err_depth = 127;
+  hipMemcpyAsync(&err_depth, scal.p, 4, hipMemcpyDeviceToHost, st);
+  if (int rc = check_hip(hipStreamSynchronize(st), "merge")) return rc;
+  const size_t lds = size_t(kBlock) * S * 8;
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replay), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  for (int t = 0; t < TB; ++t) {
+    const uint32_t* lt = ix->leaf.p + size_t(t) * n;
+    const int32_t* la = leaf_at.p + size_t(t) * n;
+    hipMemsetAsync(cnt.p, 0, size_t(M) * 4, st);
+    k_probe<<<grid_for(n), 256, 0, st>>>(ix->data.p, D, ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la, n,
+                                         p.merge_level, 0, cnt.p, nullptr, nullptr, nullptr);
+    if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(cnt.p), M, boff.p, scal.p + 1, sums, st)) return rc;
+    int total = 0;
+    hipMemcpyAsync(&total, scal.p + 1, 4, hipMemcpyDeviceToHost, st);
+    if (int rc = check_hip(hipStreamSynchronize(st), "merge")) return rc;
+    if (!rec.ensure(size_t(total) + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (merge records)");
+    hipMemsetAsync(cursor.p, 0, size_t(M) * 4, st);
+    k_probe<<<grid_for(n), 256, 0, st>>>(ix->data.p, D, ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la, n,
+                                         p.merge_level, 1, cnt.p, boff.p, cursor.p, rec.p);
+    k_sort_buckets<<<grid_for(M), 256, 0, st>>>(boff.p, M, rec.p);
+    k_replay<<<grid_for(n), kBlock, lds, st>>>(ix->data.p, D, ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la,
+                                               n, p.merge_level, S, boff.p, rec.p, hkey.p, hsize.p, extra.p,
+                                               ecount.p, nd.p);
+  }
+  // init
+  bs.L = L;
+  if (!bs.pool.alloc(size_t(n) * L) || !bs.flag.alloc(size_t(n) * L) || !bs.fill.alloc(n) || !bs.range.alloc(n) ||
+      !bs.radiusM.alloc(n) || !bs.found.alloc(n) || !bs.new_off.alloc(n + 1) || !bs.new_ids.alloc(size_t(n) * 2 * S) ||
+      !bs.old_off.alloc(n + 1) || !bs.old_ids.alloc(1))
+    return fail(EFG_ERR_MEMORY, "out of device memory (pools)");
+  k_init<<<grid_for(n), 256, 0, st>>>(ix->data.p, D, n, S, L, p.seed, hkey.p, hsize.p, extra.p, ecount.p, bs.pool.p,
+                                      bs.flag.p, bs.fill.p, bs.range.p, bs.new_ids.p, nd.p + 1);
+  {
+    std::vector<uint64_t> off(n + 1);
+    for (uint32_t i = 0; i <= n; ++i) off[i] = uint64_t(i) * 2 * S;
+    hipMemcpyAsync(bs.new_off.p, off.data(), off.size() * 8, hipMemcpyHostToDevice, st);
+    hipMemsetAsync(bs.old_off.p, 0, size_t(n + 1) * 8, st);
+    unsigned long long h_nd[2] = {0, 0};
+    hipMemcpyAsync(h_nd, nd.p, 16, hipMemcpyDeviceToHost, st);
+    if (int rc = check_hip(hipStreamSynchronize(st), "init")) return rc;
+    ix->counters[1] = h_nd[0];
+    ix->counters[2] = h_nd[1];
+  }
+  bs.new_n = size_t(n) * 2 * S;
+  bs.old_n = 0;
+  bs.iter = 0;
+  bs.ready = true;
+  ix->counters[3] = 0;
+  ix->join_pairs.clear();
+  if (err_depth < p.merge_level) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "merge level deeper than tree, max merge deepth is %d", err_depth - 1);
+    bs.ready = false;
+    return fail(EFG_ERR_STATE, buf);
+  }
+  return EFG_OK;
+}
+
+int gpu_refine_iter(efg_index* ix) {
+  BuildState& bs = ix->bs;
+  if (!bs.ready) return fail(EFG_ERR_STATE, "no build state: run the merge first");
+  const efg_build_params& p = ix->bp;
+  const uint32_t n = uint32_t(ix->n);
+  const int L = bs.L, S = p.S, D = int(ix->D);
+  hipStream_t st = ix->stream;
+  double t0 = now_ms();
+  DevBuf<uint64_t> worst;
+  DevBuf<uint32_t> tcount, cursor;
+  DevBuf<int32_t> toff, sums, scal;
+  DevBuf<unsigned long long> ctr;
+  if (!worst.alloc(n) || !tcount.alloc(n) || !cursor.alloc(n) || !toff.alloc(n + 1) || !scal.alloc(4) ||
+      !ctr.alloc(2))
+    return fail(EFG_ERR_MEMORY, "out of device memory (join)");
+  k_worst<<<grid_for(n), 256, 0, st>>>(bs.pool.p, bs.fill.p, n, L, worst.p);
+  // staging capacity: the previous size or an estimate, grown on overflow
+  static thread_local size_t stage_cap_hint = 0;
+  size_t cap = std::max<size_t>(stage_cap_hint, size_t(n) * 64);
+  DevBuf<Stage> stage;
+  const int stage_rows = int((48 * 1024) / (size_t(D) * 4));
+  const size_t lds_join = size_t(stage_rows > 0 ? stage_rows : 0) * D * 4;
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_join), hipFuncAttributeMaxDynamicSharedMemorySize,
+                      int(lds_join));
+  int dev = 0, ncu = 256;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  unsigned long long h_ctr[2] = {0, 0};
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    if (!stage.alloc(cap)) return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
+    hipMemsetAsync(ctr.p, 0, 16, st);
+    hipMemsetAsync(tcount.p, 0, size_t(n) * 4, st);
+    unsigned grid = unsigned(std::min<size_t>(n, size_t(ncu) * 16));
+    k_join<<<grid, kBlock, lds_join, st>>>(ix->data.p, D, n, bs.new_off.p, bs.new_ids.p, bs.old_off.p, bs.old_ids.p,
+                                           worst.p, stage_rows, stage.p, (unsigned long long)cap, ctr.p, tcount.p,
+                                           ctr.p + 1);
+    hipMemcpyAsync(h_ctr, ctr.p, 16, hipMemcpyDeviceToHost, st);
+    if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
+    if (h_ctr[0] <= cap) break;
+    cap = size_t(h_ctr[0]) + size_t(h_ctr[0]) / 8 + 1024;
+    if (attempt == 2) return fail(EFG_ERR_STATE, "join staging did not converge");
+  }
+  stage_cap_hint = cap;
+  const unsigned long long nst = h_ctr[0];
+  ix->join_pairs.push_back(h_ctr[1]);
+  ix->counters[3] += h_ctr[1];
+  if (nst >= (1ull << 31)) return fail(EFG_ERR_STATE, "too many join candidates for one iteration");
+  if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(tcount.p), int(n), toff.p, scal.p, sums, st)) return rc;
+  DevBuf<uint64_t> bkey;
+  DevBuf<uint32_t> bsrc;
+  if (!bkey.alloc(nst + 1) || !bsrc.alloc(nst + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (buckets)");
+  hipMemsetAsync(cursor.p, 0, size_t(n) * 4, st);
+  if (nst) k_bucket<<<unsigned((nst + 255) / 256), 256, 0, st>>>(stage.p, nst, toff.p, cursor.p, bkey.p, bsrc.p);
+  hipMemsetAsync(bs.found.p, 0, n, st);
+  const int chunk = 4096;
+  const size_t lds_m = size_t(chunk) * 12;
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_merge_pool), hipFuncAttributeMaxDynamicSharedMemorySize,
+                      int(lds_m));
+  {
+    unsigned grid = unsigned(std::min<size_t>(n, size_t(ncu) * 32));
+    k_merge_pool<<<grid, kBlock, lds_m, st>>>(n, L, p.check, toff.p, bkey.p, bsrc.p, bs.pool.p, bs.flag.p, bs.fill.p,
+                                              bs.found.p, chunk);
+  }
+  if (int rc = check_hip(hipStreamSynchronize(st), "join merge")) return rc;
+  double t1 = now_ms();
+  // ---- update(100)
+  bs.iter += 1;
+  DevBuf<uint32_t> fnew, fold, rcn, rco, rcurn, rcuro, rszn, rszo, ridn, rido;
+  DevBuf<int32_t> roffn, roffo, snew, sold, onew, oold;
+  if (!fnew.alloc(n) || !fold.alloc(n) || !rcn.alloc(n) || !rco.alloc(n) || !rcurn.alloc(n) || !rcuro.alloc(n) ||
+      !rszn.alloc(n) || !rszo.alloc(n) || !roffn.alloc(n + 1) || !roffo.alloc(n + 1) || !snew.alloc(n + 1) ||
+      !sold.alloc(n + 1) || !onew.alloc(n + 1) || !oold.alloc(n + 1))
+    return fail(EFG_ERR_MEMORY, "out of device memory (update)");
+  k_range<<<grid_for(n), 256, 0, st>>>(bs.pool.p, bs.flag.p, bs.fill.p, bs.found.p, n, L, S, bs.range.p, bs.radiusM.p);
+  hipMemsetAsync(rcn.p, 0, size_t(n) * 4, st);
+  hipMemsetAsync(rco.p, 0, size_t(n) * 4, st);
+  k_lists<<<grid_for(n), 256, 0, st>>>(bs.pool.p, bs.flag.p, bs.range.p, bs.radiusM.p, n, L, 0, fnew.p, fold.p, rcn.p,
+                                       rco.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(rcn.p), int(n), roffn.p, scal.p, sums, st)) return rc;
+  if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(rco.p), int(n), roffo.p, scal.p + 1, sums, st)) return rc;
+  int rtot[2] = {0, 0};
+  hipMemcpyAsync(rtot, scal.p, 8, hipMemcpyDeviceToHost, st);
+  if (int rc = check_hip(hipStreamSynchronize(st), "update")) return rc;
+  if (!ridn.alloc(size_t(rtot[0]) + 1) || !rido.alloc(size_t(rtot[1]) + 1))
+    return fail(EFG_ERR_MEMORY, "out of device memory (reverse lists)");
+  hipMemsetAsync(rcurn.p, 0, size_t(n) * 4, st);
+  hipMemsetAsync(rcuro.p, 0, size_t(n) * 4, st);
+  k_lists<<<grid_for(n), 256, 0, st>>>(bs.pool.p, bs.flag.p, bs.range.p, bs.radiusM.p, n, L, 1, nullptr, nullptr,
+                                       nullptr, nullptr, roffn.p, roffo.p, rcurn.p, rcuro.p, ridn.p, rido.p);
+  const int rcap = 100;
+  const int schunk = 4096;
+  const size_t lds_s = size_t(schunk) * 12;
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sample), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_s));
+  {
+    unsigned grid = unsigned(std::min<size_t>(n, size_t(ncu) * 32));
+    k_sample<<<grid, kBlock, lds_s, st>>>(n, roffn.p, ridn.p, rszn.p, rcap, p.seed, ST_RNN_NEW, uint64_t(bs.iter),
+                                          schunk);
+    k_sample<<<grid, kBlock, lds_s, st>>>(n, roffo.p, rido.p, rszo.p, rcap, p.seed, ST_RNN_OLD, uint64_t(bs.iter),
+                                          schunk);
+  }
+  k_list_sizes<<<grid_for(n), 256, 0, st>>>(n, fnew.p, rszn.p, fold.p, rszo.p, snew.p, sold.p);
+  if (int rc = device_exclusive_scan(snew.p, int(n), onew.p, scal.p, sums, st)) return rc;
+  if (int rc = device_exclusive_scan(sold.p, int(n), oold.p, scal.p + 1, sums, st)) return rc;
+  int ltot[2] = {0, 0};
+  hipMemcpyAsync(ltot, scal.p, 8, hipMemcpyDeviceToHost, st);
+  if (int rc = check_hip(hipStreamSynchronize(st), "update")) return rc;
+  if (!bs.new_ids.ensure(size_t(ltot[0]) + 1) || !bs.old_ids.ensure(size_t(ltot[1]) + 1))
+    return fail(EFG_ERR_MEMORY, "out of device memory (join lists)");
+  k_list_fill<<<grid_for(n + 1), 256, 0, st>>>(n, L, bs.pool.p, bs.flag.p, bs.range.p, onew.p, oold.p, roffn.p,
+                                               ridn.p, rszn.p, roffo.p, rido.p, rszo.p, bs.new_off.p, bs.new_ids.p,
+                                               bs.old_off.p, bs.old_ids.p);
+  bs.new_n = size_t(ltot[0]);
+  bs.old_n = size_t(ltot[1]);
+  if (int rc = check_hip(hipStreamSynchronize(st), "update")) return rc;
+  double t2 = now_ms();
+  ix->times[2] += t1 - t0;
+  ix->times[3] += t2 - t1;
+  return EFG_OK;
+}
+
+int gpu_finalize_graph(efg_index* ix) {
+  BuildState& bs = ix->bs;
+  if (!bs.ready) return fail(EFG_ERR_STATE, "no build state: run the merge first");
+  const uint32_t n = uint32_t(ix->n);
+  const int K = ix->bp.K;
+  if (!ix->goff.alloc(n + 1) || !ix->gids.alloc(size_t(n) * K)) return fail(EFG_ERR_MEMORY, "out of device memory (graph)");
+  k_graph_out<<<grid_for(n + 1), 256, 0, ix->stream>>>(n, bs.L, K, bs.pool.p, bs.fill.p, ix->goff.p, ix->gids.p);
+  if (int rc = check_hip(hipStreamSynchronize(ix->stream), "graph")) return rc;
+  ix->nnz = size_t(n) * K;
+  ix->max_deg = K;
+  return EFG_OK;
+}
+
+}  // namespace efg
+
+extern "C" {
+
+int efg_get_pools(const efg_index* ix, uint64_t* keys, uint8_t* flags, uint32_t* fill, uint32_t* range) {
+  if (!ix) return efg::fail(EFG_ERR_ARG, "null index");
+  const efg::BuildState& bs = ix->bs;
+  if (!bs.pool.p) return efg::fail(EFG_ERR_STATE, "no build state");
+  const size_t n = ix->n, L = size_t(bs.L);
+  hipStreamSynchronize(ix->stream);
+  int rc = 0;
+  if (keys) rc |= efg::check_hip(hipMemcpy(keys, bs.pool.p, n * L * 8, hipMemcpyDeviceToHost), "copy");
+  if (flags) rc |= efg::check_hip(hipMemcpy(flags, bs.flag.p, n * L, hipMemcpyDeviceToHost), "copy");
+  if (fill) rc |= efg::check_hip(hipMemcpy(fill, bs.fill.p, n * 4, hipMemcpyDeviceToHost), "copy");
+  if (range) rc |= efg::check_hip(hipMemcpy(range, bs.range.p, n * 4, hipMemcpyDeviceToHost), "copy");
+  return rc ? EFG_ERR_DEVICE : EFG_OK;
+}
+
+int efg_get_lists(const efg_index* ix, int which, uint64_t* off, uint32_t* ids, size_t cap) {
+  if (!ix) return efg::fail(EFG_ERR_ARG, "null index");
+  const efg::BuildState& bs = ix->bs;
+  if (!bs.new_off.p) return efg::fail(EFG_ERR_STATE, "no build state");
+  hipStreamSynchronize(ix->stream);
+  const uint64_t* o = which ? bs.old_off.p : bs.new_off.p;
+  const uint32_t* g = which ? bs.old_ids.p : bs.new_ids.p;
+  size_t tot = which ? bs.old_n : bs.new_n;
+  int rc = 0;
+  if (off) rc |= efg::check_hip(hipMemcpy(off, o, (ix->n + 1) * 8, hipMemcpyDeviceToHost), "copy");
+  if (ids && cap && tot) rc |= efg::check_hip(hipMemcpy(ids, g, std::min(cap, tot) * 4, hipMemcpyDeviceToHost), "copy");
+  return rc ? EFG_ERR_DEVICE : EFG_OK;
+}
+
+}  // extern "C"

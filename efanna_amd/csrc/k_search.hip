@@ -1,0 +1,487 @@
+// Graph ANN search, method 0 (getNeighbors_nnexp, kdtreeub_index.hpp:363-494).
+//
+// Stage 1, k_descend: one thread per (query, tree) walks its tree depth first,
+// query side first, and records the first `lsize` leaves (getSearchNodeList,
+// kdtreeub_index.hpp:332-346).  The walk is stackless (parent links) so every
+// thread carries only its current node.
+//
+// Stage 2, k_search_nnexp: one workgroup of 256 threads per query.
+//   * seeds: the leaves are visited round robin (leaf rank outer, tree inner,
+//     396-410); the first init_num distinct ids are kept.  Candidates are taken
+//     256 at a time; a candidate is kept when it is not yet visited and is the
+//     first occurrence of its id inside the chunk (a 512-slot LDS table keyed by
+//     id holds the minimum sequence position).  Missing seeds come from the
+//     counter-based generator (stands in for rand(), 411-417).
+//   * the visited set is an exact open-addressing hash set (LDS; a per-query
+//     region of global memory when the LDS table would pass 3/4 load — such
+//     queries are redone in a second launch).  Bit 31 of a slot marks
+//     "expanded" (newflag cleared, 450).
+//   * distances: two lanes per row, float4 loads, the reference's AVX
+//     accumulation order (bit-identical FP32).
+//   * pool = the best max(extend_to, K) by (distance, id) (partial_sort on
+//     pair<float, size_t>, 438-442), kept sorted in LDS; each epoch expands the
+//     unexpanded ids among the first extend_to, collects unvisited neighbours,
+//     and re-selects (446-484).  An epoch that expands nothing ends the loop
+//     (later epochs could not change the pool).
+#include "efg_device.hpp"
+#include "efg_internal.hpp"
+
+namespace efg {
+
+__global__ void k_descend(const float* __restrict__ Q, size_t q_stride, int nq, int T, int lsize,
+                          const int32_t* __restrict__ roots, const int4* __restrict__ node4,
+                          const int32_t* __restrict__ parent, int32_t* __restrict__ leaves) {
+  const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= size_t(nq) * T) return;
+  const int qi = int(gid / T), t = int(gid % T);
+  const float* q = Q + size_t(qi) * q_stride;
+  int32_t* out = leaves + size_t(qi) * size_t(lsize) * T;  // [ni][t]
+  int node = roots[t];
+  int cnt = 0;
+  // descend to the first leaf, query side first
+  for (;;) {
+    int4 nd = node4[node];
+    if (nd.z < 0) break;
+    node = (q[nd.x] < __int_as_float(nd.y)) ? nd.z : nd.w;
+  }
+  for (;;) {
+    out[size_t(cnt) * T + t] = node;
+    if (++cnt == lsize) break;
+    bool more = false;
+    for (;;) {
+      int p = parent[node];
+      if (p < 0) break;
+      int4 pd = node4[p];
+      int first = (q[pd.x] < __int_as_float(pd.y)) ? pd.z : pd.w;
+      if (node == first) {
+        node = (first == pd.z) ? pd.w : pd.z;
+        more = true;
+        break;
+      }
+      node = p;
+    }
+    if (!more) break;
+    for (;;) {
+      int4 nd = node4[node];
+      if (nd.z < 0) break;
+      node = (q[nd.x] < __int_as_float(nd.y)) ? nd.z : nd.w;
+    }
+  }
+  for (; cnt < lsize; ++cnt) out[size_t(cnt) * T + t] = -1;
+}
+
+namespace {
+
+constexpr uint32_t kEmpty = 0xffffffffu;
+constexpr uint32_t kExp = 0x80000000u;
+constexpr int kCtab = 512;
+
+// visited-set probe; returns slot index and whether the id is present
+__device__ inline int tab_find(const uint32_t* tab, int logcap, uint32_t id, bool* present) {
+  const uint32_t mask = (1u << logcap) - 1;
+  uint32_t s = hslot(id, logcap);
+  for (;;) {
+    uint32_t v = tab[s];
+    if (v == kEmpty) { *present = false; return int(s); }
+    if ((v & ~kExp) == id) { *present = true; return int(s); }
+    s = (s + 1) & mask;
+  }
+}
+// insert; returns true when this call inserted the id
+__device__ inline bool tab_insert(uint32_t* tab, int logcap, uint32_t id, int* slot) {
+  const uint32_t mask = (1u << logcap) - 1;
+  uint32_t s = hslot(id, logcap);
+  for (;;) {
+    uint32_t v = tab[s];
+    if (v == kEmpty) {
+      uint32_t old = atomicCAS(&tab[s], kEmpty, id);
+      if (old == kEmpty) { *slot = int(s); return true; }
+      v = old;
+    }
+    if ((v & ~kExp) == id) { *slot = int(s); return false; }
+    s = (s + 1) & mask;
+  }
+}
+
+struct Smem {
+  float* q;
+  uint64_t* buf;
+  uint32_t* tab;     // LDS table (or global when kGlobal)
+  uint32_t* ctab_id;
+  uint32_t* ctab_pos;
+  int32_t* lnode;    // leaf node per sequence slot
+  int32_t* lpos;     // exclusive prefix of leaf sizes (nleaf + 1)
+  int32_t* lstart;   // leaf start in the tree's leaf list
+  uint32_t* explist; // expanded ids of the epoch
+  int32_t* expoff;   // prefix of their degrees (E + 1)
+  int* scan;         // 4 ints
+  int* misc;         // 8 ints
+};
+
+__device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+}  // namespace
+
+// Dynamic LDS layout for the search kernel; also used by the host to size it.
+__host__ __device__ inline size_t search_lds_bytes(const SearchCfg& c, bool global_table) {
+  size_t b = 0;
+  auto a16 = [](size_t x) { return (x + 15) & ~size_t(15); };
+  b += a16(sizeof(float) * c.D);
+  b += a16(sizeof(uint64_t) * c.kb);
+  if (!global_table) b += a16(sizeof(uint32_t) << c.logcap);
+  b += a16(sizeof(uint32_t) * kCtab) * 2;
+  b += a16(sizeof(int32_t) * c.nleaf) * 2 + a16(sizeof(int32_t) * (c.nleaf + 1));
+  b += a16(sizeof(uint32_t) * c.E) + a16(sizeof(int32_t) * (c.E + 1));
+  b += a16(sizeof(int) * 4) + a16(sizeof(int) * 8);
+  return b;
+}
+
+template <bool kGlobal>
+__global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const float* __restrict__ Q, size_t q_stride,
+                                                         const float* __restrict__ X,
+                                                         const int32_t* __restrict__ leaves,
+                                                         const uint2* __restrict__ range,
+                                                         const uint32_t* __restrict__ leaf_lists,
+                                                         const uint64_t* __restrict__ goff,
+                                                         const uint32_t* __restrict__ gids, int32_t* __restrict__ out_ids,
+                                                         float* __restrict__ out_d, uint64_t* __restrict__ ndist_q,
+                                                         unsigned long long* __restrict__ ndist_total,
+                                                         const int32_t* __restrict__ qlist, int nlist_host,
+                                                         const int* __restrict__ nlist_dev,
+                                                         uint32_t* __restrict__ gtab, int32_t* __restrict__ overflow,
+                                                         int* __restrict__ noverflow) {
+  const int nlist = nlist_dev ? *nlist_dev : nlist_host;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Smem s;
+  {
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+      unsigned char* p = smem_raw + o;
+      o += align16(bytes);
+      return p;
+    };
+    s.q = reinterpret_cast<float*>(take(sizeof(float) * c.D));
+    s.buf = reinterpret_cast<uint64_t*>(take(sizeof(uint64_t) * c.kb));
+    if (!kGlobal) s.tab = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) << c.logcap));
+    else s.tab = gtab + (size_t(blockIdx.x) << c.logcap);
+    s.ctab_id = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * kCtab));
+    s.ctab_pos = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * kCtab));
+    s.lnode = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * c.nleaf));
+    s.lstart = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * c.nleaf));
+    s.lpos = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * (c.nleaf + 1)));
+    s.explist = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * c.E));
+    s.expoff = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * (c.E + 1)));
+    s.scan = reinterpret_cast<int*>(take(sizeof(int) * 4));
+    s.misc = reinterpret_cast<int*>(take(sizeof(int) * 8));
+  }
+  const int tid = threadIdx.x;
+  const int cap = 1 << c.logcap;
+  const int limit = cap - cap / 4;  // keep the table under 3/4 load
+  const int pr = tid >> 1, h = tid & 1;  // lane pair for distances
+
+  // distances for buf[from, to) whose low 32 bits hold ids
+  auto distances = [&](int from, int to) {
+    for (int i = from + pr; i < to; i += kBlock / 2) {
+      uint32_t id = key_id(s.buf[i]);
+      float dd = l2_pair_lanes(s.q, X + size_t(id) * c.D, c.D, h);
+      if (h) s.buf[i] = make_key(dd, id);
+    }
+  };
+  // sort buf[0, m) ascending (pads to a power of two)
+  auto sort_buf = [&](int m) {
+    int p2 = next_pow2(m < 2 ? 2 : m);
+    for (int i = m + tid; i < p2; i += kBlock) s.buf[i] = kKeyMax;
+    __syncthreads();
+    block_bitonic_sort(s.buf, p2);
+  };
+
+  for (int w = blockIdx.x; w < nlist; w += gridDim.x) {
+    const int qi = qlist ? qlist[w] : w;
+    const float* qg = Q + size_t(qi) * q_stride;
+    for (int i = tid; i < c.D; i += kBlock) s.q[i] = (i < c.d) ? qg[i] : 0.f;
+    for (int i = tid; i < cap; i += kBlock) s.tab[i] = kEmpty;
+    // leaf sequence: sizes and prefix in (rank, tree) order
+    int run = 0;
+    for (int b0 = 0; b0 < c.nleaf; b0 += kBlock) {
+      int li = b0 + tid;
+      int sz = 0;
+      if (li < c.nleaf) {
+        int nd = leaves[size_t(qi) * c.nleaf + li];
+        s.lnode[li] = nd;
+        if (nd >= 0) {
+          uint2 r = range[nd];
+          s.lstart[li] = int(r.x);
+          sz = int(r.y - r.x);
+        } else {
+          s.lstart[li] = 0;
+        }
+      }
+      int tot;
+      int ex = block_exclusive_scan(sz, s.scan, &tot);
+      if (li < c.nleaf) s.lpos[li] = run + ex;
+      run += tot;
+    }
+    if (tid == 0) s.lpos[c.nleaf] = run;
+    if (tid == 0) { s.misc[0] = 0; s.misc[1] = 0; }
+    __syncthreads();
+    const int total = run;
+
+    // ---- seed pool -------------------------------------------------------
+    int p = 0;  // kept so far (uniform)
+    bool ovf = false;
+    // generic chunk step: candidate id for sequence slot `sq` (valid flag)
+    auto seed_chunk = [&](bool valid, uint32_t id, uint32_t sq) {
+      for (int i = tid; i < kCtab; i += kBlock) { s.ctab_id[i] = kEmpty; s.ctab_pos[i] = kEmpty; }
+      __syncthreads();
+      bool present = false;
+      int cs = -1;
+      if (valid) {
+        tab_find(s.tab, c.logcap, id, &present);
+        if (!present) {
+          uint32_t sl = hslot(id, 9);
+          for (;;) {
+            uint32_t v = s.ctab_id[sl];
+            if (v == kEmpty) {
+              uint32_t old = atomicCAS(&s.ctab_id[sl], kEmpty, id);
+              if (old == kEmpty || old == id) break;
+              v = old;
+            }
+            if (v == id) break;
+            sl = (sl + 1) & (kCtab - 1);
+          }
+          cs = int(sl);
+          atomicMin(&s.ctab_pos[sl], sq);
+        }
+      }
+      __syncthreads();
+      int first = (valid && !present && s.ctab_pos[cs] == sq) ? 1 : 0;
+      int cnt;
+      int rank = block_exclusive_scan(first, s.scan, &cnt);
+      if (first && p + rank < c.init_num) {
+        int sl;
+        tab_insert(s.tab, c.logcap, id, &sl);
+        s.buf[p + rank] = uint64_t(id);
+      }
+      __syncthreads();
+      p = min(c.init_num, p + cnt);
+    };
+    for (int b0 = 0; b0 < total && p < c.init_num; b0 += kBlock) {
+      int sq = b0 + tid;
+      bool valid = sq < total;
+      uint32_t id = 0;
+      if (valid) {
+        int lo = 0, hi = c.nleaf;  // largest li with lpos[li] <= sq
+        while (hi - lo > 1) {
+          int mid = (lo + hi) >> 1;
+          if (s.lpos[mid] <= sq) lo = mid; else hi = mid;
+        }
+        int t = lo % c.T;
+        id = leaf_lists[size_t(t) * c.n + s.lstart[lo] + (sq - s.lpos[lo])];
+      }
+      seed_chunk(valid, id, uint32_t(sq));
+    }
+    for (uint32_t r0 = 0; p < c.init_num; r0 += kBlock) {
+      uint32_t id = uint32_t(hash4(c.seed, ST_SEARCH, uint64_t(qi), r0 + tid) % uint64_t(c.n));
+      seed_chunk(true, id, r0 + tid);
+    }
+    int inserted = c.init_num;
+    distances(0, c.init_num);
+    __syncthreads();
+    sort_buf(c.init_num);
+    uint64_t ndq = uint64_t(c.init_num);
+
+    // ---- epochs ------------------------------------------------------------
+    for (int ep = 0; ep < c.epochs && !ovf; ++ep) {
+      // expansion list: unexpanded ids among the first extend_to (448-451)
+      int nexp = 0;
+      for (int b0 = 0; b0 < c.E; b0 += kBlock) {
+        int j = b0 + tid;
+        int take = 0;
+        uint32_t id = 0;
+        int sl = -1;
+        if (j < c.E) {
+          id = key_id(s.buf[j]);
+          bool pres;
+          sl = tab_find(s.tab, c.logcap, id, &pres);
+          take = (s.tab[sl] & kExp) ? 0 : 1;
+        }
+        int tot;
+        int rk = block_exclusive_scan(take, s.scan, &tot);
+        if (take) {
+          s.tab[sl] |= kExp;
+          s.explist[nexp + rk] = id;
+        }
+        nexp += tot;
+      }
+      __syncthreads();
+      if (nexp == 0) break;
+      // neighbour offsets of the expanded ids
+      int nnb = 0;
+      for (int b0 = 0; b0 < nexp; b0 += kBlock) {
+        int j = b0 + tid;
+        int deg = 0;
+        if (j < nexp) {
+          uint32_t e = s.explist[j];
+          deg = int(goff[e + 1] - goff[e]);
+        }
+        int tot;
+        int ex = block_exclusive_scan(deg, s.scan, &tot);
+        if (j < nexp) s.expoff[j] = nnb + ex;
+        nnb += tot;
+      }
+      if (tid == 0) { s.expoff[nexp] = nnb; s.misc[0] = c.R; }
+      __syncthreads();
+      int dd = c.R;     // buf[0, dd) carry distances
+      int mprev = c.R;  // fill after the previous chunk
+      for (int b0 = 0; b0 < nnb; b0 += kBlock) {
+        int sq = b0 + tid;
+        if (sq < nnb) {
+          int lo = 0, hi = nexp;
+          while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (s.expoff[mid] <= sq) lo = mid; else hi = mid;
+          }
+          uint32_t e = s.explist[lo];
+          uint32_t id = gids[goff[e] + uint64_t(sq - s.expoff[lo])];
+          int sl;
+          if (tab_insert(s.tab, c.logcap, id, &sl)) s.buf[atomicAdd(&s.misc[0], 1)] = uint64_t(id);
+        }
+        __syncthreads();
+        int m = s.misc[0];
+        inserted += m - mprev;
+        mprev = m;
+        if (inserted + kBlock > limit) { ovf = true; break; }
+        if (m + kBlock > c.kb) {  // buffer full: keep the best R so far
+          distances(dd, m);
+          ndq += uint64_t(m - dd);
+          __syncthreads();
+          sort_buf(m);
+          dd = c.R;
+          mprev = c.R;
+          if (tid == 0) s.misc[0] = c.R;
+          __syncthreads();
+        }
+      }
+      if (ovf) break;
+      int m = s.misc[0];
+      if (m > dd) {
+        distances(dd, m);
+        ndq += uint64_t(m - dd);
+        __syncthreads();
+        sort_buf(m);
+      }
+      __syncthreads();
+    }
+    if (ovf) {
+      if (tid == 0) overflow[atomicAdd(noverflow, 1)] = qi;
+    } else {
+      for (int j = tid; j < c.K; j += kBlock) {
+        uint64_t k = s.buf[j];
+        out_ids[size_t(qi) * c.K + j] = int32_t(key_id(k));
+        if (out_d) out_d[size_t(qi) * c.K + j] = key_dist(k);
+      }
+      if (tid == 0) {
+        if (ndist_q) ndist_q[qi] = ndq;
+        atomicAdd(ndist_total, (unsigned long long)ndq);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+namespace {
+int ceil_log2(size_t x) {
+  int l = 0;
+  while ((size_t(1) << l) < x) ++l;
+  return l;
+}
+}  // namespace
+
+// knnSearch, method 0, on device buffers (kdtreeub_index.hpp:363-494).
+int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int32_t* d_ids, float* d_dists,
+                  uint64_t* d_ndist, hipStream_t st) {
+  if (!ix->s_set) return fail(EFG_ERR_STATE, "search parameters not set (setSearchParams)");
+  if (ix->s_method != 0) return fail(EFG_ERR_METHOD, "no such searching method");
+  if (ix->T == 0) return fail(EFG_ERR_STATE, "no trees: build or load trees first");
+  if (ix->nnz == 0) return fail(EFG_ERR_STATE, "no graph: build or load a graph first");
+  if (ix->s_trees <= 0) return fail(EFG_ERR_ARG, "search tree number must be positive");
+  if (ix->s_trees > ix->T) return fail(EFG_ERR_TREES, "wrong tree number");
+  if (nq == 0) return EFG_OK;
+  if (nq > (size_t(1) << 31)) return fail(EFG_ERR_ARG, "too many queries");
+  if (q_stride < ix->d) return fail(EFG_ERR_ARG, "query stride below the dimension");
+  const int init = ix->s_init, E = ix->s_extend, epochs = ix->s_epochs;
+  if (init <= 0 || E < 0 || epochs < 0) return fail(EFG_ERR_ARG, "invalid search parameters");
+  if (size_t(init) > ix->n) return fail(EFG_ERR_ARG, "pool size larger than the data set");
+  const int R = E > k ? E : k;
+  if (k <= 0 || R > init) return fail(EFG_ERR_ARG, "k must be positive and at most the pool size");
+  SearchCfg c{};
+  c.n = int(ix->n);
+  c.d = int(ix->d);
+  c.D = int(ix->D);
+  c.T = ix->s_trees;
+  c.lsize = init * 2 / (5 * c.T) + 1;
+  c.nleaf = c.T * c.lsize;
+  c.init_num = init;
+  c.E = E;
+  c.epochs = epochs;
+  c.K = k;
+  c.R = R;
+  c.kb = next_pow2(init > R + 512 ? init : R + 512);
+  c.logcap = ceil_log2(size_t(init + 512) * 4 / 3 + 1);
+  if (c.logcap < 10) c.logcap = 10;
+  c.seed = ix->bp.seed;
+  size_t lds = search_lds_bytes(c, false);
+  const size_t lds_cap = 150 * 1024;
+  bool table_in_lds = lds <= lds_cap;
+  if (!table_in_lds && search_lds_bytes(c, true) > lds_cap) return fail(EFG_ERR_ARG, "pool too large for one workgroup");
+  efg::Workspace& w = ix->ws;
+  if (!w.leaves.ensure(nq * size_t(c.nleaf)) || !w.overflow.ensure(2 * nq) || !w.nover.ensure(2) || !w.ndist.ensure(1))
+    return fail(EFG_ERR_MEMORY, "out of device memory (search workspace)");
+  if (int rc = check_hip(hipMemsetAsync(w.nover.p, 0, 2 * sizeof(int), st), "memset")) return rc;
+  if (int rc = check_hip(hipMemsetAsync(w.ndist.p, 0, sizeof(unsigned long long), st), "memset")) return rc;
+  {
+    size_t threads = nq * size_t(c.T);
+    k_descend<<<unsigned((threads + 255) / 256), 256, 0, st>>>(dq, q_stride, int(nq), c.T, c.lsize, ix->roots.p,
+                                                               ix->node4.p, ix->parent.p, w.leaves.p);
+  }
+  int dev = 0, ncu = 256;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (table_in_lds) {
+    int per_cu = int(lds_cap / lds);
+    if (per_cu < 1) per_cu = 1;
+    size_t grid = size_t(ncu) * per_cu * 2;
+    if (grid > nq) grid = nq;
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_search_nnexp<false>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    k_search_nnexp<false><<<unsigned(grid), kBlock, lds, st>>>(
+        c, dq, q_stride, ix->data.p, w.leaves.p, ix->range.p, ix->leaf.p, ix->goff.p, ix->gids.p, d_ids, d_dists,
+        d_ndist, w.ndist.p, nullptr, int(nq), nullptr, nullptr, w.overflow.p, w.nover.p);
+  }
+  // queries whose visited set outgrew the LDS table (or all of them when the
+  // table never fits) run again with a per-workgroup table in global memory
+  SearchCfg c2 = c;
+  size_t worst = size_t(init) + size_t(epochs) * size_t(E) * size_t(ix->max_deg > 0 ? ix->max_deg : 1) + 512;
+  c2.logcap = ceil_log2(worst * 4 / 3 + 1);
+  if (c2.logcap < c.logcap) c2.logcap = c.logcap;
+  if (c2.logcap > 26) c2.logcap = 26;
+  size_t lds2 = search_lds_bytes(c2, true);
+  unsigned grid2 = 128;
+  if (!w.gtab.ensure(size_t(grid2) << c2.logcap)) return fail(EFG_ERR_MEMORY, "out of device memory (search table)");
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_search_nnexp<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                      int(lds2));
+  if (table_in_lds) {
+    k_search_nnexp<true><<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->range.p, ix->leaf.p,
+                                                      ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
+                                                      w.overflow.p, 0, w.nover.p, w.gtab.p, w.overflow.p + nq,
+                                                      w.nover.p + 1);
+  } else {
+    k_search_nnexp<true><<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->range.p, ix->leaf.p,
+                                                      ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
+                                                      nullptr, int(nq), nullptr, w.gtab.p, w.overflow.p + nq,
+                                                      w.nover.p + 1);
+  }
+  return check_hip(hipGetLastError(), "search launch");
+}
+
+}  // namespace efg

@@ -1,0 +1,480 @@
+// Truncated randomized KD-trees: kdtreeub_index.hpp:1192-1276 (buildTrees,
+// identical to the tree part of initGraph 1295-1363), meanSplit /
+// planeSplit / selectDivision (902-999), DFSbuild (1138-1167).
+//
+// Level-synchronous over all trees at once.  Level l holds the active nodes
+// (to be split); children of active node i get indices base+2i, base+2i+1.
+// The reference's phase rule is kept: while 0 < |active| < 1100 (BFS) every
+// active node is split and a child stays active iff its size > S; the first
+// level where |active| >= 1100 keeps only nodes larger than TNS and from then
+// on (DFS) a node is split iff its size > TNS.
+//
+// Per level:
+//   k_split_stats  one workgroup per active node: mean and unnormalised
+//                  variance over the first min(101, count) ids of the node's
+//                  range (sequential sums per dimension), top-5 variance
+//                  dimensions, the choice among them from the counter-based
+//                  generator, cut value = mean.
+//   k_part_count / k_part_node / k_part_scatter / k_part_copy
+//                  stable three-way partition (<, ==, > cut) of every active
+//                  range in blocks of 1024 ids: per-block class counts, a
+//                  per-node scan that also applies the balancing rule and
+//                  creates the children, then the scatter and copy back.
+//   k_next_flags + scan + k_next_compact
+//                  children that stay active, in child order.
+#include <vector>
+
+#include "efg_device.hpp"
+#include "efg_internal.hpp"
+
+namespace efg {
+
+namespace {
+
+constexpr int kPartB = 1024;  // ids per partition block (4 per thread)
+
+__device__ __host__ inline uint64_t feistel_perm(uint64_t seed, uint64_t tree, uint64_t n, uint64_t i) {
+  unsigned bits = 2;
+  while ((1ULL << bits) < n) ++bits;
+  unsigned h = (bits + 1) / 2;
+  uint64_t mask = (1ULL << h) - 1;
+  uint64_t x = i;
+  do {
+    uint64_t L = x >> h, R = x & mask;
+    for (uint64_t r = 0; r < 4; ++r) {
+      uint64_t F = hash4(seed, ST_PERM, (tree << 8) | r, R) & mask;
+      uint64_t nL = R, nR = L ^ F;
+      L = nL;
+      R = nR;
+    }
+    x = (L << h) | R;
+  } while (x >= n);
+  return x;
+}
+
+__global__ void k_init_perm(uint32_t* __restrict__ leaf, int T, uint32_t n, uint64_t seed) {
+  size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= size_t(T) * n) return;
+  uint32_t t = uint32_t(gid / n), i = uint32_t(gid % n);
+  leaf[gid] = uint32_t(feistel_perm(seed, t, n, i));
+}
+
+__global__ void k_init_roots(int4* node4, int32_t* parent, uint2* range, int32_t* roots, int T, uint32_t n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  node4[t] = make_int4(-1, 0, -1, -1);
+  parent[t] = -1;
+  range[t] = make_uint2(0, n);
+  roots[t] = t;
+}
+
+// One workgroup per active node.
+__global__ void __launch_bounds__(kBlock) k_split_stats(const float* __restrict__ X, int d, int D,
+                                                        const int32_t* __restrict__ active, int na,
+                                                        const uint2* __restrict__ range,
+                                                        const int32_t* __restrict__ tree_of,
+                                                        const uint32_t* __restrict__ leaf, uint32_t n, uint64_t seed,
+                                                        int4* __restrict__ node4) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float* mean = reinterpret_cast<float*>(smem_raw);
+  float* var = mean + d;
+  __shared__ int cut_s;
+  for (int a = blockIdx.x; a < na; a += gridDim.x) {
+    const int ni = active[a];
+    const uint2 r = range[ni];
+    const int t = tree_of[ni];
+    const uint32_t* ids = leaf + size_t(t) * n + r.x;
+    const unsigned count = r.y - r.x;
+    const unsigned cnt = count < 101u ? count : 101u;
+    const float div = 1.0f / float(cnt);
+    for (int k = threadIdx.x; k < d; k += kBlock) {
+      float m = 0.f;
+      for (unsigned j = 0; j < cnt; ++j) m = m + X[size_t(ids[j]) * D + k];
+      m = m * div;
+      float v = 0.f;
+      for (unsigned j = 0; j < cnt; ++j) {
+        float tt = X[size_t(ids[j]) * D + k] - m;
+        v = __builtin_fmaf(tt, tt, v);
+      }
+      mean[k] = m;
+      var[k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int num = 0;
+      int top[5];
+      for (int i = 0; i < d; ++i) {
+        if (num < 5 || var[i] > var[top[num - 1]]) {
+          if (num < 5) top[num++] = i;
+          else top[num - 1] = i;
+          int j = num - 1;
+          while (j > 0 && var[top[j]] > var[top[j - 1]]) {
+            int tmp = top[j];
+            top[j] = top[j - 1];
+            top[j - 1] = tmp;
+            --j;
+          }
+        }
+      }
+      uint64_t rnd = hash4(seed, ST_SPLIT, uint64_t(t), (uint64_t(r.x) << 32) | r.y) % uint64_t(num);
+      cut_s = top[rnd];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int cd = cut_s;
+      int4 nd = node4[ni];
+      nd.x = cd;
+      nd.y = __float_as_int(mean[cd]);
+      node4[ni] = nd;
+    }
+    __syncthreads();
+  }
+}
+
+// block -> (active slot, block index within the node) via the block prefix
+__device__ inline int find_node(const int32_t* __restrict__ bstart, int na, int b) {
+  int lo = 0, hi = na;  // largest i with bstart[i] <= b
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (bstart[mid] <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline int classify(float v, float cut) { return v < cut ? 0 : (v <= cut ? 1 : 2); }
+
+__global__ void __launch_bounds__(kBlock) k_part_count(const float* __restrict__ X, int D,
+                                                       const int32_t* __restrict__ active, int na,
+                                                       const int32_t* __restrict__ bstart, const int* __restrict__ nblk,
+                                                       const uint2* __restrict__ range,
+                                                       const int32_t* __restrict__ tree_of,
+                                                       const int4* __restrict__ node4,
+                                                       const uint32_t* __restrict__ leaf, uint32_t n,
+                                                       uint8_t* __restrict__ cls, uint32_t* __restrict__ bcnt) {
+  __shared__ int tmp[kBlock / 64];
+  const int total = *nblk;
+  for (int b = blockIdx.x; b < total; b += gridDim.x) {
+    const int a = find_node(bstart, na, b);
+    const int ni = active[a];
+    const uint2 r = range[ni];
+    const int t = tree_of[ni];
+    const int4 nd = node4[ni];
+    const float cut = __int_as_float(nd.y);
+    const uint32_t s0 = r.x + uint32_t(b - bstart[a]) * kPartB;
+    // counts of '<' and '==' packed in 16-bit fields; '>' follows from the
+    // number of valid ids (a prefix of the block)
+    const uint32_t nvalid = (r.y - s0) < uint32_t(kPartB) ? (r.y - s0) : uint32_t(kPartB);
+    uint32_t packed = 0;
+    for (int e = 0; e < 4; ++e) {
+      uint32_t pos = s0 + threadIdx.x * 4 + e;
+      if (pos < r.y) {
+        uint32_t id = leaf[size_t(t) * n + pos];
+        int c = classify(X[size_t(id) * D + nd.x], cut);
+        cls[size_t(t) * n + pos] = uint8_t(c);
+        if (c < 2) packed += 1u << (16 * c);
+      }
+    }
+    int tot;
+    block_exclusive_scan(int(packed), tmp, &tot);
+    if (threadIdx.x == 0) {
+      uint32_t u = uint32_t(tot);
+      uint32_t c0 = u & 0xffffu, c1 = u >> 16;
+      bcnt[size_t(b) * 3 + 0] = c0;
+      bcnt[size_t(b) * 3 + 1] = c1;
+      bcnt[size_t(b) * 3 + 2] = nvalid - c0 - c1;
+    }
+  }
+}
+
+// One thread per active node: per-block exclusive offsets, lim1/lim2, the
+// balancing rule (kdtreeub_index.hpp:943-950), children.
+__global__ void k_part_node(const int32_t* __restrict__ active, int na, const int32_t* __restrict__ bstart,
+                            uint32_t* __restrict__ bcnt, int4* __restrict__ node4, int32_t* __restrict__ parent,
+                            uint2* __restrict__ range, int32_t* __restrict__ tree_of, int32_t base) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= na) return;
+  const int ni = active[a];
+  const int b0 = bstart[a], b1 = bstart[a + 1];
+  uint32_t c0 = 0, c1 = 0, c2 = 0;
+  for (int b = b0; b < b1; ++b) {
+    uint32_t x0 = bcnt[size_t(b) * 3], x1 = bcnt[size_t(b) * 3 + 1], x2 = bcnt[size_t(b) * 3 + 2];
+    bcnt[size_t(b) * 3] = c0;
+    bcnt[size_t(b) * 3 + 1] = c1;
+    bcnt[size_t(b) * 3 + 2] = c2;
+    c0 += x0;
+    c1 += x1;
+    c2 += x2;
+  }
+  for (int b = b0; b < b1; ++b) {  // class bases: [<][==][>]
+    bcnt[size_t(b) * 3 + 1] += c0;
+    bcnt[size_t(b) * 3 + 2] += c0 + c1;
+  }
+  const uint2 r = range[ni];
+  const unsigned count = r.y - r.x;
+  const unsigned lim1 = c0, lim2 = c0 + c1;
+  unsigned index;
+  if (lim1 > count / 2) index = lim1;
+  else if (lim2 < count / 2) index = lim2;
+  else index = count / 2;
+  if (lim1 == count || lim2 == 0) index = count / 2;
+  const int32_t L = base + 2 * a, R = base + 2 * a + 1;
+  int4 nd = node4[ni];
+  nd.z = L;
+  nd.w = R;
+  node4[ni] = nd;
+  node4[L] = make_int4(-1, 0, -1, -1);
+  node4[R] = make_int4(-1, 0, -1, -1);
+  parent[L] = ni;
+  parent[R] = ni;
+  range[L] = make_uint2(r.x, r.x + index);
+  range[R] = make_uint2(r.x + index, r.y);
+  tree_of[L] = tree_of[ni];
+  tree_of[R] = tree_of[ni];
+}
+
+__global__ void __launch_bounds__(kBlock) k_part_scatter(const int32_t* __restrict__ active, int na,
+                                                         const int32_t* __restrict__ bstart,
+                                                         const int* __restrict__ nblk, const uint2* __restrict__ range,
+                                                         const int32_t* __restrict__ tree_of,
+                                                         const uint32_t* __restrict__ bcnt,
+                                                         const uint8_t* __restrict__ cls,
+                                                         const uint32_t* __restrict__ leaf, uint32_t n,
+                                                         uint32_t* __restrict__ tmpbuf) {
+  __shared__ int tmp[kBlock / 64];
+  const int total = *nblk;
+  for (int b = blockIdx.x; b < total; b += gridDim.x) {
+    const int a = find_node(bstart, na, b);
+    const int ni = active[a];
+    const uint2 r = range[ni];
+    const int t = tree_of[ni];
+    const uint32_t s0 = r.x + uint32_t(b - bstart[a]) * kPartB;
+    const uint32_t nvalid = (r.y - s0) < uint32_t(kPartB) ? (r.y - s0) : uint32_t(kPartB);
+    int c[4];
+    uint32_t packed = 0;
+    for (int e = 0; e < 4; ++e) {
+      uint32_t pos = s0 + threadIdx.x * 4 + e;
+      c[e] = pos < r.y ? int(cls[size_t(t) * n + pos]) : 3;
+      if (c[e] < 2) packed += 1u << (16 * c[e]);
+    }
+    int tot;
+    uint32_t ex = uint32_t(block_exclusive_scan(int(packed), tmp, &tot));
+    uint32_t before = threadIdx.x * 4u < nvalid ? threadIdx.x * 4u : nvalid;
+    uint32_t r0 = ex & 0xffffu, r1 = ex >> 16, r2 = before - r0 - r1;
+    for (int e = 0; e < 4; ++e) {
+      if (c[e] == 3) continue;
+      uint32_t pos = s0 + threadIdx.x * 4 + e;
+      uint32_t rk = c[e] == 0 ? r0++ : (c[e] == 1 ? r1++ : r2++);
+      uint32_t dst = r.x + bcnt[size_t(b) * 3 + c[e]] + rk;
+      tmpbuf[size_t(t) * n + dst] = leaf[size_t(t) * n + pos];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_part_copy(const int32_t* __restrict__ active, int na,
+                                                      const int32_t* __restrict__ bstart, const int* __restrict__ nblk,
+                                                      const uint2* __restrict__ range,
+                                                      const int32_t* __restrict__ tree_of,
+                                                      const uint32_t* __restrict__ tmpbuf, uint32_t n,
+                                                      uint32_t* __restrict__ leaf) {
+  const int total = *nblk;
+  for (int b = blockIdx.x; b < total; b += gridDim.x) {
+    const int a = find_node(bstart, na, b);
+    const int ni = active[a];
+    const uint2 r = range[ni];
+    const int t = tree_of[ni];
+    const uint32_t s0 = r.x + uint32_t(b - bstart[a]) * kPartB;
+    for (int e = threadIdx.x; e < kPartB; e += kBlock) {
+      uint32_t pos = s0 + e;
+      if (pos < r.y) leaf[size_t(t) * n + pos] = tmpbuf[size_t(t) * n + pos];
+    }
+  }
+}
+
+// blocks per active node
+__global__ void k_blocks_per_node(const int32_t* __restrict__ active, int na, const uint2* __restrict__ range,
+                                  int32_t* __restrict__ nb) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= na) return;
+  uint2 r = range[active[a]];
+  nb[a] = int((r.y - r.x + kPartB - 1) / kPartB);
+}
+
+// flags[i] = node (cands[i]) stays active: size > thr
+__global__ void k_flags(const int32_t* __restrict__ cands, int m, const uint2* __restrict__ range, uint32_t thr,
+                        int base_is_children, int32_t base, int32_t* __restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int32_t ni = base_is_children ? base + i : cands[i];
+  uint2 r = range[ni];
+  flags[i] = (r.y - r.x > thr) ? 1 : 0;
+}
+
+__global__ void k_compact(const int32_t* __restrict__ cands, int m, int base_is_children, int32_t base,
+                          const int32_t* __restrict__ flags, const int32_t* __restrict__ pos,
+                          int32_t* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  if (flags[i]) out[pos[i]] = base_is_children ? base + i : cands[i];
+}
+
+// ---- device exclusive scan (int32) -------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_scan_blocks(const int32_t* __restrict__ in, int m,
+                                                        int32_t* __restrict__ out, int32_t* __restrict__ sums) {
+  __shared__ int tmp[kBlock / 64];
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  int v = i < m ? in[i] : 0;
+  int tot;
+  int ex = block_exclusive_scan(v, tmp, &tot);
+  if (i < m) out[i] = ex;
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+__global__ void __launch_bounds__(kBlock) k_scan_sums(int32_t* __restrict__ sums, int nb, int32_t* __restrict__ total) {
+  __shared__ int tmp[kBlock / 64];
+  int run = 0;
+  for (int b0 = 0; b0 < nb; b0 += kBlock) {
+    int i = b0 + threadIdx.x;
+    int v = i < nb ? sums[i] : 0;
+    int tot;
+    int ex = block_exclusive_scan(v, tmp, &tot);
+    if (i < nb) sums[i] = run + ex;
+    run += tot;
+  }
+  if (threadIdx.x == 0) *total = run;
+}
+__global__ void k_scan_add(int32_t* __restrict__ out, int m, const int32_t* __restrict__ sums,
+                           const int32_t* __restrict__ total) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < m) out[i] += sums[blockIdx.x];
+  if (i == m) out[m] = *total;
+}
+
+}  // namespace
+
+// Exclusive scan of m ints; out[m] = total; *total_dev = total.
+int device_exclusive_scan(const int32_t* in, int m, int32_t* out, int32_t* total_dev, DevBuf<int32_t>& sums,
+                          hipStream_t st) {
+  int nb = (m + kBlock - 1) / kBlock;
+  if (nb < 1) nb = 1;
+  if (!sums.ensure(nb)) return fail(EFG_ERR_MEMORY, "out of device memory (scan)");
+  k_scan_blocks<<<nb, kBlock, 0, st>>>(in, m, out, sums.p);
+  k_scan_sums<<<1, kBlock, 0, st>>>(sums.p, nb, total_dev);
+  k_scan_add<<<nb + ((m % kBlock) == 0 ? 1 : 0), kBlock, 0, st>>>(out, m, sums.p, total_dev);
+  return EFG_OK;
+}
+
+int gpu_build_trees(efg_index* ix) {
+  const efg_build_params& p = ix->bp;
+  if (p.trees <= 0) return fail(EFG_ERR_ARG, "tree number must be positive");
+  if (p.S <= 0 || p.tns <= 0) return fail(EFG_ERR_ARG, "invalid S or TNS");
+  const int T = p.trees;
+  const uint32_t n = uint32_t(ix->n);
+  hipStream_t st = ix->stream;
+  const size_t cap = size_t(T) * (2 * size_t(n) + 1);
+  if (!ix->node4.alloc(cap) || !ix->parent.alloc(cap) || !ix->range.alloc(cap) || !ix->roots.alloc(T) ||
+      !ix->leaf.alloc(size_t(T) * n))
+    return fail(EFG_ERR_MEMORY, "out of device memory (trees)");
+  DevBuf<int32_t> tree_of, act, nxt, flags, pos, nb, bstart, sums, scal;
+  DevBuf<uint32_t> tmpbuf, bcnt;
+  DevBuf<uint8_t> cls;
+  const size_t act_cap = cap;
+  if (!tree_of.alloc(cap) || !act.alloc(act_cap) || !nxt.alloc(act_cap) || !flags.alloc(act_cap + 1) ||
+      !pos.alloc(act_cap + 1) || !nb.alloc(act_cap + 1) || !bstart.alloc(act_cap + 1) || !scal.alloc(4) ||
+      !tmpbuf.alloc(size_t(T) * n) || !cls.alloc(size_t(T) * n))
+    return fail(EFG_ERR_MEMORY, "out of device memory (tree build)");
+  const size_t blk_cap = size_t(T) * n / kPartB + act_cap + 1;
+  if (!bcnt.alloc(blk_cap * 3)) return fail(EFG_ERR_MEMORY, "out of device memory (tree build)");
+  {
+    size_t tot = size_t(T) * n;
+    k_init_perm<<<unsigned((tot + 255) / 256), 256, 0, st>>>(ix->leaf.p, T, n, p.seed);
+    k_init_roots<<<(T + 255) / 256, 256, 0, st>>>(ix->node4.p, ix->parent.p, ix->range.p, ix->roots.p, T, n);
+    std::vector<int32_t> h_act(T);
+    for (int t = 0; t < T; ++t) h_act[t] = t;
+    hipMemcpyAsync(act.p, h_act.data(), T * 4, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(tree_of.p, h_act.data(), T * 4, hipMemcpyHostToDevice, st);
+    if (int rc = check_hip(hipStreamSynchronize(st), "tree build")) return rc;
+  }
+  int dev = 0, ncu = 256;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  std::vector<int8_t> level_of;  // per node level
+  level_of.assign(T, 0);
+  int na = T;
+  int32_t M = T;
+  bool bfs = true;
+  int Lb = -1, level = 0;
+  const size_t lds_stats = size_t(2) * ix->d * sizeof(float);
+  if (lds_stats > 150 * 1024) return fail(EFG_ERR_ARG, "dimension too large for the split statistics");
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_split_stats), hipFuncAttributeMaxDynamicSharedMemorySize,
+                      int(lds_stats));
+  int32_t h_total = 0;
+  while (na > 0) {
+    if (bfs && !(na < 1100)) {  // switch to DFS: keep nodes larger than TNS
+      bfs = false;
+      Lb = level;
+      k_flags<<<(na + 255) / 256, 256, 0, st>>>(act.p, na, ix->range.p, uint32_t(p.tns), 0, 0, flags.p);
+      if (int rc = device_exclusive_scan(flags.p, na, pos.p, scal.p, sums, st)) return rc;
+      k_compact<<<(na + 255) / 256, 256, 0, st>>>(act.p, na, 0, 0, flags.p, pos.p, nxt.p);
+      hipMemcpyAsync(&h_total, scal.p, 4, hipMemcpyDeviceToHost, st);
+      if (int rc = check_hip(hipStreamSynchronize(st), "tree build")) return rc;
+      std::swap(act.p, nxt.p);
+      std::swap(act.n, nxt.n);
+      na = h_total;
+      if (na == 0) break;
+    }
+    const int32_t base = M;
+    if (size_t(base) + 2 * size_t(na) > cap) return fail(EFG_ERR_STATE, "tree node capacity exceeded");
+    // split statistics
+    {
+      unsigned g = unsigned(na < 65535 * 4 ? na : 65535 * 4);
+      k_split_stats<<<g, kBlock, lds_stats, st>>>(ix->data.p, int(ix->d), int(ix->D), act.p, na, ix->range.p,
+                                                   tree_of.p, ix->leaf.p, n, p.seed, ix->node4.p);
+    }
+    // blocks
+    k_blocks_per_node<<<(na + 255) / 256, 256, 0, st>>>(act.p, na, ix->range.p, nb.p);
+    if (int rc = device_exclusive_scan(nb.p, na, bstart.p, scal.p + 1, sums, st)) return rc;
+    {
+      unsigned g = unsigned(size_t(ncu) * 16);
+      k_part_count<<<g, kBlock, 0, st>>>(ix->data.p, int(ix->D), act.p, na, bstart.p, scal.p + 1, ix->range.p,
+                                         tree_of.p, ix->node4.p, ix->leaf.p, n, cls.p, bcnt.p);
+      k_part_node<<<(na + 255) / 256, 256, 0, st>>>(act.p, na, bstart.p, bcnt.p, ix->node4.p, ix->parent.p,
+                                                    ix->range.p, tree_of.p, base);
+      k_part_scatter<<<g, kBlock, 0, st>>>(act.p, na, bstart.p, scal.p + 1, ix->range.p, tree_of.p, bcnt.p, cls.p,
+                                           ix->leaf.p, n, tmpbuf.p);
+      k_part_copy<<<g, kBlock, 0, st>>>(act.p, na, bstart.p, scal.p + 1, ix->range.p, tree_of.p, tmpbuf.p, n,
+                                        ix->leaf.p);
+    }
+    M = base + 2 * na;
+    level_of.resize(M, int8_t(level + 1));
+    // next active list: children larger than S (BFS) or TNS (DFS)
+    const int m = 2 * na;
+    k_flags<<<(m + 255) / 256, 256, 0, st>>>(nullptr, m, ix->range.p, uint32_t(bfs ? p.S : p.tns), 1, base, flags.p);
+    if (int rc = device_exclusive_scan(flags.p, m, pos.p, scal.p, sums, st)) return rc;
+    k_compact<<<(m + 255) / 256, 256, 0, st>>>(nullptr, m, 1, base, flags.p, pos.p, nxt.p);
+    hipMemcpyAsync(&h_total, scal.p, 4, hipMemcpyDeviceToHost, st);
+    if (int rc = check_hip(hipStreamSynchronize(st), "tree build")) return rc;
+    std::swap(act.p, nxt.p);
+    std::swap(act.n, nxt.n);
+    na = h_total;
+    ++level;
+  }
+  if (int rc = check_hip(hipStreamSynchronize(st), "tree build")) return rc;
+  ix->T = T;
+  ix->M = size_t(M);
+  // DivDim written for leaves in the trees file: leaves the DFS phase made
+  // (and roots) carry -1, leaves left behind by the BFS phase carry 0
+  // (zero-initialised nodes, kdtreeub_index.hpp:1249, 1142-1143).
+  ix->leaf_dim.assign(size_t(M), -1);
+  if (Lb != 0) {
+    std::vector<uint2> rg(M);
+    hipMemcpy(rg.data(), ix->range.p, size_t(M) * sizeof(uint2), hipMemcpyDeviceToHost);
+    for (int32_t i = T; i < M; ++i) {
+      int lv = level_of[i];
+      bool dfs_made = (Lb >= 0) && (lv > Lb || (lv == Lb && rg[i].y - rg[i].x > uint32_t(p.S)));
+      ix->leaf_dim[i] = dfs_made ? -1 : 0;
+    }
+  }
+  ix->s_trees = T;
+  return EFG_OK;
+}
+
+}  // namespace efg

@@ -1,0 +1,158 @@
+/* efg.h — C ABI of the MI355X (gfx950) EFANNA library, libefg.so.
+ *
+ * Drop-in boundary for efanna::FIndex<float> with the KDTREE_UB index
+ * (efanna.hpp:9-75, algorithm/kdtreeub_index.hpp).  Every entry point below
+ * names the reference call it replaces.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Every call returns an int status: EFG_OK (0) or a negative EFG_ERR_*;
+ *    efg_last_error() returns the message of the calling thread's last failure.
+ *    The library never calls exit() (the reference exits on I/O errors).
+ *  - Calls are synchronous: they return when the device work is complete,
+ *    except the *_device entry points, which are stream-ordered on `stream`
+ *    (a hipStream_t; NULL = the index's own stream).
+ *  - An index owns one HIP stream and device copies of the data, trees and
+ *    graph; distinct indexes may be used from distinct host threads.
+ *  - Ids are 32-bit; n must be below 2^30.
+ */
+#ifndef EFG_H_
+#define EFG_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EFG_OK 0
+#define EFG_ERR_ARG -2     /* invalid argument or parameter combination */
+#define EFG_ERR_STATE -3   /* e.g. merge level deeper than a tree (kdtreeub_index.hpp:108-111) */
+#define EFG_ERR_TREES -4   /* "wrong tree number" (kdtreeub_index.hpp:367-369) */
+#define EFG_ERR_METHOD -5  /* "no such searching method" (kdtreeub_index.hpp:357-358) */
+#define EFG_ERR_IO -6      /* file cannot be opened/read/written (kdtreeub_index.hpp:141,202) */
+#define EFG_ERR_FORMAT -7  /* malformed file (kdtreeub_index.hpp:179,234) */
+#define EFG_ERR_DEVICE -8  /* HIP runtime failure */
+#define EFG_ERR_MEMORY -9  /* device allocation failed */
+
+typedef struct efg_index efg_index;
+
+/* KDTreeUbIndexParams (algorithm/kdtreeub_index.hpp:14-35) + IndexParams
+ * (general/params.hpp:19-33).  `seed` keys the counter-based generator that
+ * replaces rand()/mt19937 (1998 is the reference's seed). */
+typedef struct efg_build_params {
+  int trees;        /* tree_num_total  -> TreeNum (kdtreeub_index.hpp:46-58) */
+  int trees_build;  /* tree_num_build  -> TreeNumBuild; 0 means `trees` */
+  int merge_level;  /* ml (conquer-to-depth level) */
+  int epochs;       /* build_epoches (NN-descent iterations) */
+  int check;        /* Check_K */
+  int L;            /* pool size */
+  int K;            /* graph degree */
+  int S;            /* initial neighbour-set size */
+  int tns;          /* TNS, DFS leaf size (params.hpp:25, default 10) */
+  uint64_t seed;
+} efg_build_params;
+
+/* Default parameters of KDTreeUbIndexParams(true, trees). */
+void efg_default_build_params(efg_build_params* p, int trees);
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* efg_last_error(void);
+
+/* FIndex ctor (efanna.hpp:14-20) + KDTreeUbIndex ctor (kdtreeub_index.hpp:42-90).
+ * `data` is n rows of d floats, row i at data + i*row_stride (host memory).
+ * The library copies it to the device (the reference keeps a non-owning
+ * view, general/matrix.hpp:13-38); the caller's buffer may be freed after. */
+int efg_create(const float* data, size_t n, size_t d, size_t row_stride, const efg_build_params* params,
+               efg_index** out);
+/* ~FIndex (the reference never frees its index, efanna.hpp:22-23). */
+int efg_destroy(efg_index* ix);
+
+/* FIndex::buildIndex -> KDTreeUbIndex::buildIndexImpl (kdtreeub_index.hpp:92-113):
+ * trees, conquer-to-depth merge, random init, NN-descent refine; the graph
+ * keeps the first K pool entries (base_index.hpp:469-479). */
+int efg_build_index(efg_index* ix);
+/* FIndex::buildTrees -> KDTreeUbIndex::buildTrees (kdtreeub_index.hpp:1192-1276). */
+int efg_build_trees(efg_index* ix);
+
+/* FIndex::setSearchParams (efanna.hpp:56-58 -> base_index.hpp:153-161):
+ * extend_to is clamped to init_num; search_lv is stored and unused as in the
+ * reference; search_trees == 0 is rejected (the reference divides by zero). */
+int efg_set_search_params(efg_index* ix, int epochs, int init_num, int extend_to, int search_trees, int search_lv,
+                          int search_method);
+/* Current search parameters (method 1 raises init_num, kdtreeub_index.hpp:510-514). */
+int efg_get_search_params(const efg_index* ix, int* epochs, int* init_num, int* extend_to, int* search_trees,
+                          int* search_method);
+
+/* FIndex::knnSearch (efanna.hpp:32-34 -> kdtreeub_index.hpp:349-361): host
+ * queries (nq rows of the index's d floats, stride q_stride); results are
+ * kept in the index (read them with efg_get_results / efg_save_results). */
+int efg_knn_search(efg_index* ix, int k, const float* queries, size_t nq, size_t q_stride);
+/* Results of the last efg_knn_search: nq*k ids (-1 where fewer were found)
+ * and, if dists != NULL, squared distances.  Either output may be NULL to
+ * query the shape only. */
+int efg_get_results(const efg_index* ix, size_t* nq, int* k, int32_t* ids, float* dists);
+/* Device-resident search: queries, ids and dists are device pointers; the
+ * call is ordered on `stream` and does not synchronise.  ndist (device,
+ * nullable) receives the distance evaluations per query. */
+int efg_knn_search_device(efg_index* ix, int k, const float* d_queries, size_t nq, size_t q_stride, int32_t* d_ids,
+                          float* d_dists, uint64_t* d_ndist, void* stream);
+
+/* getGraphSize / getGraphRow (base_index.hpp:494-501). */
+int efg_graph_size(const efg_index* ix, size_t* n);
+int efg_graph_row(const efg_index* ix, size_t row, uint32_t* out, size_t cap, size_t* len);
+
+/* File formats of the reference (byte compatible):
+ *  graph   kdtreeub_index.hpp:295-312 (save) / 243-277 (load)
+ *  trees   kdtreeub_index.hpp:197-242, 658-671 (save) / 139-196, 672-700 (load)
+ *  index   kdtreeub_index.hpp:771-825 (save) / 701-770 (load): trees + graph
+ *  results base_index.hpp:136-151 */
+int efg_save_graph(const efg_index* ix, const char* path);
+int efg_load_graph(efg_index* ix, const char* path);
+int efg_save_trees(const efg_index* ix, const char* path);
+int efg_load_trees(efg_index* ix, const char* path);
+int efg_save_index(const efg_index* ix, const char* path);
+int efg_load_index(efg_index* ix, const char* path);
+int efg_save_results(const efg_index* ix, const char* path);
+
+/* Exact k nearest neighbours by brute force over device rows (the oracle
+ * the reference only has as Matrix::brute_force_search, general/matrix.hpp:57-68):
+ * k smallest (distance, id), ties to the lower id; exclude_self skips row q for
+ * query q.  Base rows must be padded with zeros to a multiple of 8 floats. */
+int efg_exact_knn_device(const float* d_base, size_t n, size_t d, size_t base_stride, const float* d_queries,
+                         size_t nq, size_t q_stride, int k, int exclude_self, int32_t* d_ids, float* d_dists,
+                         void* stream);
+
+/* ---- inspection (tests, tools) ------------------------------------------ */
+/* Trees as flat arrays over all nodes (leaves: dim = left = right = -1). */
+int efg_tree_shape(const efg_index* ix, int* trees, size_t* nodes);
+int efg_get_trees(const efg_index* ix, int32_t* dim, float* val, int32_t* left, int32_t* right, uint32_t* start,
+                  uint32_t* end, int32_t* roots, uint32_t* leaf_lists);
+int efg_set_trees(efg_index* ix, int trees, size_t nodes, const int32_t* dim, const float* val, const int32_t* left,
+                  const int32_t* right, const uint32_t* start, const uint32_t* end, const int32_t* roots,
+                  const uint32_t* leaf_lists);
+/* Graph as CSR: off[n+1], ids[off[n]]. */
+int efg_get_graph(const efg_index* ix, uint64_t* off, uint32_t* ids, size_t cap);
+int efg_set_graph(efg_index* ix, const uint64_t* off, const uint32_t* ids);
+/* Device pointer of the padded base rows and their stride (floats). */
+int efg_device_data(const efg_index* ix, const float** d_data, size_t* stride);
+/* Counters: [0] distances in the last search, [1] merge distances, [2] init
+ * distances, [3] join distances (pairs), [4] last search overflow queries. */
+int efg_get_counters(const efg_index* ix, uint64_t* out, int n);
+/* Per-iteration join pair counts of the last build. */
+int efg_get_join_pairs(const efg_index* ix, uint64_t* out, int cap, int* n);
+/* Build stages for stepwise checking: 0 trees, 1 merge+init, 2 one refine
+ * iteration, 3 write the graph. */
+int efg_build_stage(efg_index* ix, int stage);
+/* Build state after a stage: pools as n x L keys (float distance bits << 32 |
+ * id, all-ones when empty), flags, fill counts, Range; join lists as CSR
+ * (which = 0 new, 1 old). */
+int efg_get_pools(const efg_index* ix, uint64_t* keys, uint8_t* flags, uint32_t* fill, uint32_t* range);
+int efg_get_lists(const efg_index* ix, int which, uint64_t* off, uint32_t* ids, size_t cap);
+/* Phase timings of the last build in milliseconds: [trees, merge+init, join, update]. */
+int efg_get_build_times(const efg_index* ix, double* out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EFG_H_ */

@@ -1,0 +1,104 @@
+"""GPU parity of the graph build against the oracle, stage by stage.
+
+With the counter-based generator both sides make the same random choices, so
+trees, merge/init state, every NN-descent iteration and the final graph must
+be identical.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from efanna_amd.index import FIndex, KDTreeUbIndexParams, L2DistanceAVX
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(base, trees=8, ml=8, epochs=8, check=25, L=30, K=10, S=10, tns=10):
+    params = KDTreeUbIndexParams(True, trees, ml, epochs, check, L, K, trees, S)
+    params.TNS = tns
+    g = FIndex(base, L2DistanceAVX(), params)
+    o = oracle.Index(base, trees=trees, ml=ml, epochs=epochs, check=check, L=L, K=K, S=S, tns=tns)
+    return g, o
+
+
+def _cmp_trees(g, o):
+    tg, to = g.trees(), o.trees()
+    for key in ("dim", "left", "right", "start", "end", "roots", "leaf"):
+        np.testing.assert_array_equal(tg[key], to[key], err_msg=key)
+    np.testing.assert_array_equal(tg["val"].view(np.uint32), to["val"].view(np.uint32))
+
+
+def _cmp_state(g, o, lists=True):
+    kg, fg, ng, rg = g.pools()
+    ko, fo, no, ro = o.pools()
+    np.testing.assert_array_equal(ng, no, err_msg="fill")
+    np.testing.assert_array_equal(rg, ro, err_msg="range")
+    np.testing.assert_array_equal(kg, ko, err_msg="pool keys")
+    np.testing.assert_array_equal(fg, fo, err_msg="flags")
+    if lists:
+        for which in (0, 1):
+            og, ig = g.lists(which)
+            oo, io = o.lists(which)
+            np.testing.assert_array_equal(og, oo, err_msg=f"list offsets {which}")
+            for i in range(len(og) - 1):
+                a = np.sort(ig[og[i]:og[i + 1]])
+                b = np.sort(io[oo[i]:oo[i + 1]])
+                if not np.array_equal(a, b):
+                    raise AssertionError(f"list {which} of node {i} differs: {a} vs {b}")
+
+
+@pytest.mark.parametrize("n,d,trees", [(10_000, 128, 8), (3000, 20, 3)])
+def test_trees_match_oracle(cuda, n, d, trees):
+    base, _ = helpers.uniform(n=n, d=d, nq=1)
+    g, o = _pair(base, trees=trees)
+    g.buildTrees()
+    o.build_trees()
+    _cmp_trees(g, o)
+
+
+def test_trees_match_oracle_mixture_dfs_phase(cuda):
+    # enough nodes per level to leave the BFS phase (|active| >= 1100)
+    base, _ = helpers.mixture(n=40_000, nq=1)
+    g, o = _pair(base, trees=8, S=6, tns=12)
+    g.buildTrees()
+    o.build_trees()
+    _cmp_trees(g, o)
+
+
+def test_build_stages_match_oracle(cuda):
+    base, _ = helpers.uniform()
+    g, o = _pair(base)
+    g.build_stage(0)
+    o.build_trees()
+    _cmp_trees(g, o)
+    g.build_stage(1)
+    o.merge_init()
+    assert g.counters()["merge"] == o.counters()["merge"]
+    assert g.counters()["init"] == o.counters()["init"]
+    _cmp_state(g, o)
+    for it in range(8):
+        g.build_stage(2)
+        o.refine_iter()
+        _cmp_state(g, o)
+    assert g.join_pairs() == o.join_pairs()
+    g.build_stage(3)
+    o.finalize()
+    np.testing.assert_array_equal(g.graph_matrix(), o.graph_matrix())
+
+
+def test_build_index_mixture_matches_oracle(cuda):
+    base, _ = helpers.mixture(n=30_000, nq=1)
+    g, o = _pair(base, trees=8, ml=8, epochs=6)
+    g.buildIndex()
+    o.build_index()
+    np.testing.assert_array_equal(g.graph_matrix(), o.graph_matrix())
+
+
+def test_merge_level_error(cuda):
+    base, _ = helpers.uniform(n=300, d=8, nq=1)
+    g, _o = _pair(base, trees=2, ml=12, epochs=1)
+    from efanna_amd._lib import EfgError
+    with pytest.raises(EfgError) as e:
+        g.buildIndex()
+    assert e.value.code == -3 and "merge level deeper than tree" in e.value.msg

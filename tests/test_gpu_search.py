@@ -1,0 +1,148 @@
+"""GPU parity: graph search (method 0) and exact kNN against the CPU oracle.
+
+Search is deterministic given (graph, trees): the HIP kernels must return the
+same ids, the same FP32 distances (same accumulation order) and the same
+number of distance evaluations per query as the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from efanna_amd.index import FIndex, KDTreeUbIndexParams, L2DistanceAVX, Matrix
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_index(base, trees, graph):
+    ix = FIndex(Matrix(base.shape[0], base.shape[1], base), L2DistanceAVX(), KDTreeUbIndexParams(True, 8, 8, 8))
+    ix.set_trees(trees)
+    ix.set_graph(*graph)
+    return ix
+
+
+def _search_both(oix, gix, q, k, epochs, pool, extend, trees):
+    gix.setSearchParams(epochs, pool, extend, trees, -1, 0)
+    gix.knnSearch(k, q)
+    gids, gd = gix.results()
+    oids, od, ond = oix.search(q, k, epochs, pool, extend, trees, 0)
+    return gids, gd, oids, od, ond
+
+
+@pytest.mark.parametrize("epochs,pool,extend,trees,k", [
+    (4, 120, 20, 4, 10),    # the 10K x 128 CPU configuration
+    (0, 120, 20, 4, 10),    # seeds only
+    (4, 120, 10, 8, 20),    # k > extend_to
+    (6, 300, 300, 8, 10),   # extend_to == init_num
+    (2, 60, 200, 2, 5),     # extend_to clamped to init_num
+])
+def test_search_matches_oracle_uniform(cuda, epochs, pool, extend, trees, k):
+    base, q, oix = helpers.oracle_index("uniform")
+    gix = _gpu_index(base, oix.trees(), oix.graph())
+    gids, gd, oids, od, ond = _search_both(oix, gix, q, k, epochs, pool, extend, trees)
+    np.testing.assert_array_equal(gids, oids)
+    np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32))
+    assert gix.counters()["search_dist"] == int(ond.sum())
+
+
+def test_search_matches_oracle_float_mixture(cuda):
+    # non-integer data: bit-identical distances need the same FMA order
+    base, q, oix = helpers.oracle_index("mixture", n=20_000, nq=300, centres=40, sigma=0.05, hi=1.0,
+                                        rounded=False, d=100, ml=4)
+    gix = _gpu_index(base, oix.trees(), oix.graph())
+    gids, gd, oids, od, ond = _search_both(oix, gix, q, 10, 4, 400, 60, 8)
+    np.testing.assert_array_equal(gids, oids)
+    np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32))
+    assert gix.counters()["search_dist"] == int(ond.sum())
+
+
+def test_search_large_pool_mixture(cuda):
+    base, q, oix = helpers.oracle_index("mixture", n=40_000, nq=200, centres=40, trees=16, ml=8)
+    gix = _gpu_index(base, oix.trees(), oix.graph())
+    gids, gd, oids, od, ond = _search_both(oix, gix, q, 10, 4, 1200, 200, 16)
+    np.testing.assert_array_equal(gids, oids)
+    np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32))
+    assert gix.counters()["search_dist"] == int(ond.sum())
+
+
+def test_search_visited_table_fallback(cuda):
+    # dense random graph: expansions overflow the LDS visited table and the
+    # queries are redone with the global-memory table
+    base, q = helpers.uniform(n=6000, d=32, nq=64)
+    oix = oracle.Index(base, trees=4, ml=4, epochs=1)
+    oix.build_trees()
+    rng = np.random.default_rng(1)
+    deg = 64
+    ids = rng.integers(0, base.shape[0], size=base.shape[0] * deg).astype(np.uint32)
+    off = np.arange(base.shape[0] + 1, dtype=np.uint64) * deg
+    oix.set_graph(off, ids)
+    gix = _gpu_index(base, oix.trees(), (off, ids))
+    gids, gd, oids, od, ond = _search_both(oix, gix, q, 10, 4, 100, 100, 4)
+    assert gix.counters()["overflow"] > 0
+    np.testing.assert_array_equal(gids, oids)
+    np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32))
+
+
+def test_search_random_topup(cuda):
+    # pool larger than what the leaves hold: the generator tops the pool up
+    base, q = helpers.uniform(n=700, d=16, nq=40)
+    oix = oracle.Index(base, trees=2, ml=2, epochs=2, S=2, K=2, L=6, check=4, tns=2)
+    oix.build_index()
+    gix = _gpu_index(base, oix.trees(), oix.graph())
+    gids, gd, oids, od, ond = _search_both(oix, gix, q, 5, 3, 300, 20, 2)
+    np.testing.assert_array_equal(gids, oids)
+    np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32))
+
+
+def test_search_errors(cuda):
+    base, q, oix = helpers.oracle_index("uniform")
+    gix = _gpu_index(base, oix.trees(), oix.graph())
+    from efanna_amd._lib import EfgError
+    gix.setSearchParams(4, 120, 20, 9, -1, 0)
+    with pytest.raises(EfgError) as e:
+        gix.knnSearch(10, q)
+    assert e.value.code == -4 and "wrong tree number" in e.value.msg
+    gix.setSearchParams(4, 120, 20, 0, -1, 0)
+    with pytest.raises(EfgError):
+        gix.knnSearch(10, q)
+    gix.setSearchParams(4, 120, 20, 4, -1, 7)
+    with pytest.raises(EfgError) as e:
+        gix.knnSearch(10, q)
+    assert e.value.code == -5
+
+
+@pytest.mark.parametrize("k,exclude", [(10, False), (100, True), (1, False)])
+def test_exact_knn_matches_oracle(cuda, k, exclude):
+    import torch
+    base, q = helpers.mixture(n=20_000, nq=200)
+    if exclude:
+        q = base[:150].copy()
+    oids, od = oracle.exact_knn(base, q, k, exclude_self=exclude)
+    gix = FIndex(base, L2DistanceAVX(), KDTreeUbIndexParams(True, 4))
+    dptr, stride = gix.device_data()
+    tq = torch.from_numpy(q).to(cuda)
+    ids = torch.empty((q.shape[0], k), dtype=torch.int32, device=cuda)
+    dd = torch.empty((q.shape[0], k), dtype=torch.float32, device=cuda)
+    from efanna_amd.index import exact_knn_device
+    exact_knn_device(dptr, base.shape[0], base.shape[1], stride, tq.data_ptr(), q.shape[0], q.shape[1], k, exclude,
+                     ids.data_ptr(), dd.data_ptr())
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(ids.cpu().numpy(), oids)
+    np.testing.assert_array_equal(dd.cpu().numpy().view(np.uint32), od.view(np.uint32))
+
+
+def test_exact_knn_float_data_matches_oracle(cuda):
+    import torch
+    base, q = helpers.mixture(n=9_000, nq=64, d=96, centres=20, sigma=0.05, hi=1.0, rounded=False)
+    oids, od = oracle.exact_knn(base, q, 16)
+    gix = FIndex(base, L2DistanceAVX(), KDTreeUbIndexParams(True, 4))
+    dptr, stride = gix.device_data()
+    tq = torch.from_numpy(q).to(cuda)
+    ids = torch.empty((q.shape[0], 16), dtype=torch.int32, device=cuda)
+    dd = torch.empty((q.shape[0], 16), dtype=torch.float32, device=cuda)
+    from efanna_amd.index import exact_knn_device
+    exact_knn_device(dptr, base.shape[0], base.shape[1], stride, tq.data_ptr(), q.shape[0], q.shape[1], 16, False,
+                     ids.data_ptr(), dd.data_ptr())
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(ids.cpu().numpy(), oids)
+    np.testing.assert_array_equal(dd.cpu().numpy().view(np.uint32), od.view(np.uint32))
