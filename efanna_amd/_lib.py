@@ -62,6 +62,9 @@ _SIGS = {
     "efg_get_pools": (c_int, [c_vp, c_u64p, c_u8p, c_u32p, c_u32p]),
     "efg_get_lists": (c_int, [c_vp, c_int, c_u64p, c_u32p, c_sz]),
     "efg_get_build_times": (c_int, [c_vp, ctypes.POINTER(ctypes.c_double), c_int]),
+    "efg_set_profiling": (c_int, [c_vp, c_int]),
+    "efg_get_kernel_times": (c_int, [c_vp, ctypes.POINTER(ctypes.c_double), c_int, c_intp]),
+    "efg_get_search_work": (c_int, [c_vp, c_u64p, c_int]),
 }
 
 _lib = None

@@ -349,6 +349,7 @@ int efg_create(const float* data, size_t n, size_t d, size_t row_stride, const e
 int efg_destroy(efg_index* ix) {
   if (!ix) return EFG_OK;
   if (ix->stream) hipStreamSynchronize(ix->stream);
+  for (hipEvent_t e : ix->prof_events) hipEventDestroy(e);
   hipStream_t st = ix->stream;
   delete ix;
   if (st) hipStreamDestroy(st);
@@ -703,6 +704,45 @@ int efg_get_join_pairs(const efg_index* ix, uint64_t* out, int cap, int* n) {
   if (!ix) return fail(EFG_ERR_ARG, "null index");
   if (n) *n = int(ix->join_pairs.size());
   for (int i = 0; i < cap && i < int(ix->join_pairs.size()); ++i) out[i] = ix->join_pairs[i];
+  return EFG_OK;
+}
+
+int efg_set_profiling(efg_index* ix, int on) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  ix->profiling = on != 0;
+  return EFG_OK;
+}
+
+int efg_get_kernel_times(efg_index* ix, double* ms, int n, int* searches) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  double acc[3] = {0, 0, 0};
+  const size_t m = ix->prof_events.size() / 4;
+  for (size_t i = 0; i < m; ++i) {
+    hipEvent_t* e = &ix->prof_events[4 * i];
+    hipEventSynchronize(e[3]);
+    for (int j = 0; j < 3; ++j) {
+      float t = 0.f;
+      hipEventElapsedTime(&t, e[j], e[j + 1]);
+      acc[j] += t;
+    }
+    for (int j = 0; j < 4; ++j) hipEventDestroy(e[j]);
+  }
+  ix->prof_events.clear();
+  for (int j = 0; j < n && j < 3; ++j) ms[j] = acc[j];
+  if (searches) *searches = int(m);
+  return EFG_OK;
+}
+
+int efg_get_search_work(efg_index* ix, uint64_t* out, int n) {
+  if (!ix || !out) return fail(EFG_ERR_ARG, "null argument");
+  unsigned long long w[3] = {0, 0, 0};
+  if (ix->work.p) {
+    hipStreamSynchronize(ix->stream);
+    if (int rc = check_hip(hipDeviceSynchronize(), "sync")) return rc;
+    if (int rc = check_hip(hipMemcpy(w, ix->work.p, sizeof w, hipMemcpyDeviceToHost), "copy")) return rc;
+    if (int rc = check_hip(hipMemset(ix->work.p, 0, sizeof w), "memset")) return rc;
+  }
+  for (int i = 0; i < n && i < 3; ++i) out[i] = w[i];
   return EFG_OK;
 }
 

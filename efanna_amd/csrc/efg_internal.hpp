@@ -119,6 +119,10 @@ struct efg_index {
   size_t res_nq = 0;
   int res_k = 0;
   efg::Workspace ws;
+  // search profiling: events per recorded search (4 each)
+  bool profiling = false;
+  std::vector<hipEvent_t> prof_events;
+  efg::DevBuf<unsigned long long> work;  // [distances, seed ids read, expanded]
 };
 
 namespace efg {

@@ -149,7 +149,8 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
                                                          const int32_t* __restrict__ qlist, int nlist_host,
                                                          const int* __restrict__ nlist_dev,
                                                          uint32_t* __restrict__ gtab, int32_t* __restrict__ overflow,
-                                                         int* __restrict__ noverflow) {
+                                                         int* __restrict__ noverflow,
+                                                         unsigned long long* __restrict__ work) {
   const int nlist = nlist_dev ? *nlist_dev : nlist_host;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Smem s;
@@ -265,7 +266,9 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       __syncthreads();
       p = min(c.init_num, p + cnt);
     };
+    int seq_read = 0;
     for (int b0 = 0; b0 < total && p < c.init_num; b0 += kBlock) {
+      seq_read = min(total, b0 + kBlock);
       int sq = b0 + tid;
       bool valid = sq < total;
       uint32_t id = 0;
@@ -285,6 +288,8 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       seed_chunk(true, id, r0 + tid);
     }
     int inserted = c.init_num;
+    uint64_t nexp_total = 0;
+    const uint64_t seeds_read = uint64_t(seq_read);
     distances(0, c.init_num);
     __syncthreads();
     sort_buf(c.init_num);
@@ -315,6 +320,7 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       }
       __syncthreads();
       if (nexp == 0) break;
+      nexp_total += uint64_t(nexp);
       // neighbour offsets of the expanded ids
       int nnb = 0;
       for (int b0 = 0; b0 < nexp; b0 += kBlock) {
@@ -383,6 +389,13 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       if (tid == 0) {
         if (ndist_q) ndist_q[qi] = ndq;
         atomicAdd(ndist_total, (unsigned long long)ndq);
+        atomicAdd(ndist_total + 1, (unsigned long long)seeds_read);
+        atomicAdd(ndist_total + 2, (unsigned long long)nexp_total);
+        if (work) {
+          atomicAdd(work, (unsigned long long)ndq);
+          atomicAdd(work + 1, (unsigned long long)seeds_read);
+          atomicAdd(work + 2, (unsigned long long)nexp_total);
+        }
       }
     }
     __syncthreads();
@@ -435,15 +448,26 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
   bool table_in_lds = lds <= lds_cap;
   if (!table_in_lds && search_lds_bytes(c, true) > lds_cap) return fail(EFG_ERR_ARG, "pool too large for one workgroup");
   efg::Workspace& w = ix->ws;
-  if (!w.leaves.ensure(nq * size_t(c.nleaf)) || !w.overflow.ensure(2 * nq) || !w.nover.ensure(2) || !w.ndist.ensure(1))
+  if (!w.leaves.ensure(nq * size_t(c.nleaf)) || !w.overflow.ensure(2 * nq) || !w.nover.ensure(2) || !w.ndist.ensure(3))
     return fail(EFG_ERR_MEMORY, "out of device memory (search workspace)");
   if (int rc = check_hip(hipMemsetAsync(w.nover.p, 0, 2 * sizeof(int), st), "memset")) return rc;
-  if (int rc = check_hip(hipMemsetAsync(w.ndist.p, 0, sizeof(unsigned long long), st), "memset")) return rc;
+  if (int rc = check_hip(hipMemsetAsync(w.ndist.p, 0, 3 * sizeof(unsigned long long), st), "memset")) return rc;
+  if (!ix->work.p) {
+    if (!ix->work.ensure(3)) return fail(EFG_ERR_MEMORY, "out of device memory (counters)");
+    if (int rc = check_hip(hipMemsetAsync(ix->work.p, 0, 3 * sizeof(unsigned long long), st), "memset")) return rc;
+  }
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  const bool prof = ix->profiling && ix->prof_events.size() < 4 * 4096;
+  if (prof) {
+    for (auto& e : ev) hipEventCreate(&e);
+    hipEventRecord(ev[0], st);
+  }
   {
     size_t threads = nq * size_t(c.T);
     k_descend<<<unsigned((threads + 255) / 256), 256, 0, st>>>(dq, q_stride, int(nq), c.T, c.lsize, ix->roots.p,
                                                                ix->node4.p, ix->parent.p, w.leaves.p);
   }
+  if (prof) hipEventRecord(ev[1], st);
   int dev = 0, ncu = 256;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -456,8 +480,9 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     k_search_nnexp<false><<<unsigned(grid), kBlock, lds, st>>>(
         c, dq, q_stride, ix->data.p, w.leaves.p, ix->range.p, ix->leaf.p, ix->goff.p, ix->gids.p, d_ids, d_dists,
-        d_ndist, w.ndist.p, nullptr, int(nq), nullptr, nullptr, w.overflow.p, w.nover.p);
+        d_ndist, w.ndist.p, nullptr, int(nq), nullptr, nullptr, w.overflow.p, w.nover.p, ix->work.p);
   }
+  if (prof) hipEventRecord(ev[2], st);
   // queries whose visited set outgrew the LDS table (or all of them when the
   // table never fits) run again with a per-workgroup table in global memory
   SearchCfg c2 = c;
@@ -474,12 +499,16 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
     k_search_nnexp<true><<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->range.p, ix->leaf.p,
                                                       ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
                                                       w.overflow.p, 0, w.nover.p, w.gtab.p, w.overflow.p + nq,
-                                                      w.nover.p + 1);
+                                                      w.nover.p + 1, ix->work.p);
   } else {
     k_search_nnexp<true><<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->range.p, ix->leaf.p,
                                                       ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
                                                       nullptr, int(nq), nullptr, w.gtab.p, w.overflow.p + nq,
-                                                      w.nover.p + 1);
+                                                      w.nover.p + 1, ix->work.p);
+  }
+  if (prof) {
+    hipEventRecord(ev[3], st);
+    for (auto& e : ev) ix->prof_events.push_back(e);
   }
   return check_hip(hipGetLastError(), "search launch");
 }

@@ -260,6 +260,22 @@ class FIndex:
         check(lib().efg_get_join_pairs(self.h, _p(out, _lib.c_u64p), 64, ctypes.byref(m)))
         return out[:m.value].tolist()
 
+    def set_profiling(self, on: bool):
+        check(lib().efg_set_profiling(self.h, int(on)))
+
+    def kernel_times(self):
+        """Summed ms [tree descent, search kernel, overflow re-run] and the search count."""
+        out = (ctypes.c_double * 3)()
+        m = ctypes.c_int()
+        check(lib().efg_get_kernel_times(self.h, out, 3, ctypes.byref(m)))
+        return list(out), m.value
+
+    def search_work(self):
+        """[distances, seed leaf ids read, nodes expanded] since the last call."""
+        c = np.zeros(3, np.uint64)
+        check(lib().efg_get_search_work(self.h, _p(c, _lib.c_u64p), 3))
+        return [int(x) for x in c]
+
     def build_times(self):
         out = (ctypes.c_double * 4)()
         check(lib().efg_get_build_times(self.h, out, 4))

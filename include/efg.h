@@ -151,6 +151,15 @@ int efg_get_pools(const efg_index* ix, uint64_t* keys, uint8_t* flags, uint32_t*
 int efg_get_lists(const efg_index* ix, int which, uint64_t* off, uint32_t* ids, size_t cap);
 /* Phase timings of the last build in milliseconds: [trees, merge+init, join, update]. */
 int efg_get_build_times(const efg_index* ix, double* out, int n);
+/* Kernel timing of searches (HIP events on the search stream).  While on,
+ * every search records its stages; efg_get_kernel_times returns the summed
+ * milliseconds [tree descent, search kernel, overflow re-run], the number of
+ * searches recorded, and clears the record. */
+int efg_set_profiling(efg_index* ix, int on);
+int efg_get_kernel_times(efg_index* ix, double* ms, int n, int* searches);
+/* Algorithmic work of the searches since the last call: [distances, leaf ids
+ * read for seeds, nodes expanded]; clears the counters. */
+int efg_get_search_work(efg_index* ix, uint64_t* out, int n);
 
 #ifdef __cplusplus
 }
