@@ -1,0 +1,529 @@
+"""Pins the CPU oracle (oracle/efg_oracle.cpp) without the reference binary.
+
+The reference needs boost, which this image lacks, so it cannot be compiled
+here; each oracle function is pinned by published test vectors, closed
+forms, brute force, hand-derived cases, an independent pure-Python
+restatement on small inputs, invariants, or the reference's own measured
+statistics recorded for the 10K x 128 configuration.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import helpers
+
+G = 0x9E3779B97F4A7C15
+M64 = (1 << 64) - 1
+
+
+# --------------------------------------------------------------------------
+# generator
+# --------------------------------------------------------------------------
+def test_splitmix64_published_vectors():
+    def seq(seed, k):
+        out, s = [], seed
+        for _ in range(k):
+            out.append(oracle.mix64(s))
+            s = (s + G) & M64
+        return out
+    assert seq(0, 3) == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    assert seq(1234567, 5) == [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                               4593380528125082431, 16408922859458223821]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 17, 64, 100, 1000, 4097])
+def test_perm_is_bijection(n):
+    for tree in (0, 3):
+        p = [oracle.perm(1998, tree, n, i) for i in range(n)]
+        assert sorted(p) == list(range(n))
+
+
+def test_perm_is_not_identity_and_varies_by_tree():
+    n = 1000
+    a = [oracle.perm(1998, 0, n, i) for i in range(n)]
+    b = [oracle.perm(1998, 1, n, i) for i in range(n)]
+    assert a != list(range(n)) and a != b
+    # roughly uniform: fixed points ~ Poisson(1)
+    assert sum(1 for i, x in enumerate(a) if i == x) < 10
+
+
+# --------------------------------------------------------------------------
+# squared L2 (general/distance.hpp:299-323)
+# --------------------------------------------------------------------------
+def _fma32(a, b, c):
+    return np.float32(np.float64(a) * np.float64(b) + np.float64(c))
+
+
+def _l2_avx_py(a, b):
+    """Independent restatement: 8 lane chains (tail chunks first), FMA, left-to-right sum."""
+    d = len(a)
+    D = (d + 7) // 8 * 8
+    aa = np.zeros(D, np.float32)
+    bb = np.zeros(D, np.float32)
+    aa[:d] = a
+    bb[:d] = b
+    DR = D % 32
+    DD = D - DR
+    acc = [np.float32(0)] * 8
+    order = list(range(DD, D, 8)) + list(range(0, DD, 8))
+    for c in order:
+        for j in range(8):
+            t = np.float32(aa[c + j] - bb[c + j])
+            acc[j] = _fma32(t, t, acc[j])
+    s = acc[0]
+    for j in range(1, 8):
+        s = np.float32(s + acc[j])
+    return s
+
+
+@pytest.mark.parametrize("d", [1, 7, 8, 16, 24, 32, 48, 64, 100, 128, 129, 960])
+def test_l2_integer_closed_form(d):
+    rng = np.random.default_rng(d)
+    a = rng.integers(0, 256, d).astype(np.float32)
+    b = rng.integers(0, 256, d).astype(np.float32)
+    exact = float(np.sum((a.astype(np.int64) - b.astype(np.int64)) ** 2))
+    # also the dimensions where the reference's AVX kernel drops elements
+    assert oracle.l2(a, b) == exact
+
+
+def test_l2_float_order_and_tolerance():
+    rng = np.random.default_rng(7)
+    for d in (64, 100, 128, 960):
+        for _ in range(20):
+            a = rng.random(d).astype(np.float32)
+            b = rng.random(d).astype(np.float32)
+            got = np.float32(oracle.l2(a, b))
+            assert got == _l2_avx_py(a, b)
+            exact = float(np.sum((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+            assert abs(float(got) - exact) <= 1e-5 * exact
+
+
+def test_l2_order_is_observable():
+    # a case where a plain sequential sum differs from the lane order: the
+    # order test above has teeth
+    rng = np.random.default_rng(11)
+    seen = False
+    for _ in range(200):
+        a = (rng.random(128) * 1e3).astype(np.float32)
+        b = np.zeros(128, np.float32)
+        s = np.float32(0)
+        for x in a:
+            s = _fma32(x, x, s)
+        if s != np.float32(oracle.l2(a, b)):
+            seen = True
+            break
+    assert seen
+
+
+def test_l2_symmetric():
+    rng = np.random.default_rng(3)
+    a = rng.random(200).astype(np.float32)
+    b = rng.random(200).astype(np.float32)
+    assert oracle.l2(a, b) == oracle.l2(b, a)
+
+
+# --------------------------------------------------------------------------
+# InsertIntoKnn (base_index.hpp:37-62)
+# --------------------------------------------------------------------------
+def test_insert_into_knn_hand_cases():
+    ids = [1, 2, 3, 4, 99]
+    ds = [1.0, 2.0, 2.0, 5.0, 0.0]
+    # goes after the equal distances
+    r, i2, d2 = oracle.insert_into_knn(ids, ds, 4, 7, 2.0)
+    assert r == 3 and list(i2[:5]) == [1, 2, 3, 7, 4]
+    # duplicate among equal distances -> K + 1, unchanged
+    r, i2, d2 = oracle.insert_into_knn(ids, ds, 4, 2, 2.0)
+    assert r == 5 and list(i2[:5]) == ids
+    # same id with another distance is not detected
+    r, i2, d2 = oracle.insert_into_knn(ids, ds, 4, 2, 3.0)
+    assert r == 3 and list(i2[:5]) == [1, 2, 3, 2, 4]
+    # worst -> written to the spare slot K
+    r, i2, d2 = oracle.insert_into_knn(ids, ds, 4, 8, 9.0)
+    assert r == 4 and list(i2[:5]) == [1, 2, 3, 4, 8]
+    # new best
+    r, i2, d2 = oracle.insert_into_knn(ids, ds, 4, 9, 0.5)
+    assert r == 0 and list(i2[:5]) == [9, 1, 2, 3, 4]
+    # empty list
+    r, i2, d2 = oracle.insert_into_knn([0, 0], [0.0, 0.0], 0, 5, 1.0)
+    assert r == 0 and i2[0] == 5
+
+
+# --------------------------------------------------------------------------
+# exact kNN (matrix.hpp:57-68)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("exclude", [False, True])
+def test_exact_knn_brute_force(exclude):
+    base, q = helpers.uniform(n=3000, d=32, nq=50)
+    rng = np.random.default_rng(0)
+    base = rng.integers(0, 8, size=base.shape).astype(np.float32)  # many ties
+    qq = base[:40] if exclude else rng.integers(0, 8, size=(40, 32)).astype(np.float32)
+    ids, ds = oracle.exact_knn(base, qq, 15, exclude_self=exclude)
+    d64 = ((qq[:, None, :].astype(np.int64) - base[None].astype(np.int64)) ** 2).sum(-1)
+    for i in range(qq.shape[0]):
+        dd = d64[i].astype(np.float64)
+        if exclude:
+            dd[i] = np.inf
+        order = np.lexsort((np.arange(len(dd)), dd))[:15]
+        assert list(ids[i]) == list(order)
+        np.testing.assert_array_equal(ds[i], dd[order].astype(np.float32))
+
+
+# --------------------------------------------------------------------------
+# trees (kdtreeub_index.hpp:902-999, 1192-1276)
+# --------------------------------------------------------------------------
+def _py_trees(data, T, S, TNS, seed):
+    """Pure-Python restatement of the level-synchronous build."""
+    n, d = data.shape
+    leaf = np.array([[oracle.perm(seed, t, n, i) for i in range(n)] for t in range(T)], dtype=np.uint32)
+    nodes = [dict(tree=t, start=0, end=n, dim=-1, val=0.0, left=-1, right=-1) for t in range(T)]
+    active = list(range(T))
+    bfs = True
+    while active:
+        if bfs and not (len(active) < 1100):
+            bfs = False
+            active = [a for a in active if nodes[a]["end"] - nodes[a]["start"] > TNS]
+            if not active:
+                break
+        base = len(nodes)
+        for i, a in enumerate(active):
+            nodes[a]["left"], nodes[a]["right"] = base + 2 * i, base + 2 * i + 1
+        nodes.extend(dict() for _ in range(2 * len(active)))
+        for a in active:
+            nd = nodes[a]
+            t, s, e = nd["tree"], nd["start"], nd["end"]
+            ids = [int(x) for x in leaf[t, s:e]]
+            count = e - s
+            cnt = min(101, count)
+            mean = [np.float32(0)] * d
+            for j in range(cnt):
+                for k in range(d):
+                    mean[k] = np.float32(mean[k] + data[ids[j], k])
+            div = np.float32(np.float32(1) / np.float32(cnt))
+            mean = [np.float32(m * div) for m in mean]
+            var = [np.float32(0)] * d
+            for j in range(cnt):
+                for k in range(d):
+                    tt = np.float32(data[ids[j], k] - mean[k])
+                    var[k] = _fma32(tt, tt, var[k])
+            top = []
+            for i in range(d):  # top-5 with the reference's bubble insertion
+                if len(top) < 5 or var[i] > var[top[-1]]:
+                    if len(top) < 5:
+                        top.append(i)
+                    else:
+                        top[-1] = i
+                    j = len(top) - 1
+                    while j > 0 and var[top[j]] > var[top[j - 1]]:
+                        top[j], top[j - 1] = top[j - 1], top[j]
+                        j -= 1
+            rnd = oracle.hash4(seed, 2, t, (s << 32) | e) % len(top)
+            cd = top[rnd]
+            cv = mean[cd]
+            lt = [x for x in ids if data[x, cd] < cv]
+            eq = [x for x in ids if data[x, cd] == cv]
+            gt = [x for x in ids if data[x, cd] > cv]
+            leaf[t, s:e] = lt + eq + gt
+            lim1, lim2 = len(lt), len(lt) + len(eq)
+            if lim1 > count // 2:
+                index = lim1
+            elif lim2 < count // 2:
+                index = lim2
+            else:
+                index = count // 2
+            if lim1 == count or lim2 == 0:
+                index = count // 2
+            nd["dim"], nd["val"] = cd, float(cv)
+            nodes[nd["left"]] = dict(tree=t, start=s, end=s + index, dim=-1, val=0.0, left=-1, right=-1)
+            nodes[nd["right"]] = dict(tree=t, start=s + index, end=e, dim=-1, val=0.0, left=-1, right=-1)
+        thr = S if bfs else TNS
+        nxt = []
+        for i in range(len(active)):
+            for c in (base + 2 * i, base + 2 * i + 1):
+                if nodes[c]["end"] - nodes[c]["start"] > thr:
+                    nxt.append(c)
+        active = nxt
+    return nodes, leaf
+
+
+@pytest.mark.parametrize("n,d,T,S,TNS", [(300, 6, 3, 3, 5), (2400, 3, 8, 1, 3)])
+def test_trees_match_python_restatement(n, d, T, S, TNS):
+    rng = np.random.default_rng(n)
+    data = rng.integers(0, 50, size=(n, d)).astype(np.float32)
+    data[:, 1] = (data[:, 1] / 7).astype(np.float32)  # repeated values -> equal-to-cut cases
+    o = oracle.Index(data, trees=T, S=S, tns=TNS)
+    o.build_trees()
+    t = o.trees()
+    nodes, leaf = _py_trees(data, T, S, TNS, 1998)
+    assert len(nodes) == len(t["dim"])
+    np.testing.assert_array_equal(t["leaf"], leaf)
+    for i, nd in enumerate(nodes):
+        assert (t["start"][i], t["end"][i], t["left"][i], t["right"][i], t["dim"][i]) == \
+               (nd["start"], nd["end"], nd["left"], nd["right"], nd["dim"]), i
+        assert np.float32(t["val"][i]) == np.float32(nd["val"])
+
+
+def test_tree_invariants_and_phase_rule():
+    base, _ = helpers.mixture(n=40_000, nq=1)
+    S, TNS = 6, 12
+    o = oracle.Index(base, trees=8, S=S, tns=TNS)
+    o.build_trees()
+    t = o.trees()
+    n = base.shape[0]
+    for tr in range(8):
+        assert np.array_equal(np.sort(t["leaf"][tr]), np.arange(n))
+    left, right, start, end, dim, val = (t[k] for k in ("left", "right", "start", "end", "dim", "val"))
+    depth = np.zeros(len(left), int)
+    covered = np.zeros((8, n), int)
+    tree_of = np.zeros(len(left), int)
+    tree_of[t["roots"]] = np.arange(8)
+    for i in range(len(left)):  # parents precede children
+        if left[i] >= 0:
+            for c in (left[i], right[i]):
+                depth[c] = depth[i] + 1
+                tree_of[c] = tree_of[i]
+            assert start[left[i]] == start[i] and end[right[i]] == end[i] and end[left[i]] == start[right[i]]
+            ids_l = t["leaf"][tree_of[i], start[left[i]]:end[left[i]]]
+            ids_r = t["leaf"][tree_of[i], start[right[i]]:end[right[i]]]
+            vl, vr = base[ids_l, dim[i]], base[ids_r, dim[i]]
+            if not (vl.size and (vl == vl[0]).all() and (vr == vl[0]).all()):
+                assert vl.max() <= val[i] <= vr.min()
+        else:
+            covered[tree_of[i], start[i]:end[i]] += 1
+    assert (covered == 1).all()
+    # level of the BFS->DFS switch: first level with >= 1100 nodes to split
+    sizes = end - start
+    per_level = {}
+    for i in range(len(left)):
+        if left[i] >= 0:
+            per_level.setdefault(depth[i], 0)
+            per_level[depth[i]] += 1
+    leaves = left < 0
+    lb = 0
+    while lb in per_level and per_level[lb] < 1100:
+        lb += 1
+    assert lb < max(per_level)  # the switch happens in this case
+    assert (sizes[leaves & (depth < lb)] <= S).all()
+    assert (sizes[leaves & (depth > lb)] <= TNS).all()
+    assert (sizes[~leaves & (depth > lb)] > TNS).all()
+
+
+# --------------------------------------------------------------------------
+# merge + init (kdtreeub_index.hpp:1000-1081, 1377-1423)
+# --------------------------------------------------------------------------
+def _py_merge(data, t, TB, ml, S):
+    n = data.shape[0]
+    left, right, start, end, dim, val = (t[k] for k in ("left", "right", "start", "end", "dim", "val"))
+    heaps = [[] for _ in range(n)]
+    extra = [[] for _ in range(n)]
+
+    def offer(x, y, dd):
+        h = heaps[x]
+        if len(h) < S or dd < h[-1][0]:
+            c = (dd, y)
+            if c in h:
+                return
+            h.append(c)
+            h.sort()
+            if len(h) > S:
+                h.pop()
+        elif len(extra[x]) < 2 * S:
+            extra[x].append(y)
+
+    def to_leaf(node, p):
+        while left[node] >= 0:
+            node = left[node] if data[p, dim[node]] < val[node] else right[node]
+        return node
+
+    def merge(tr, node):
+        if left[node] < 0:
+            return
+        merge(tr, left[node])
+        merge(tr, right[node])
+        L_, R_ = left[node], right[node]
+        if end[L_] - start[L_] < end[R_] - start[R_]:
+            root, s, e = R_, start[L_], end[L_]
+        else:
+            root, s, e = L_, start[R_], end[R_]
+        for pos in range(s, e):
+            p = int(t["leaf"][tr, pos])
+            lf = to_leaf(root, p)
+            for i in range(start[lf], end[lf]):
+                q = int(t["leaf"][tr, i])
+                dd = np.float32(oracle.l2(data[q], data[p]))
+                offer(q, p, dd)
+                offer(p, q, dd)
+
+    def ml_nodes(node, depth, out):
+        if left[node] >= 0 and depth < ml:
+            ml_nodes(left[node], depth + 1, out)
+            ml_nodes(right[node], depth + 1, out)
+        elif depth == ml:
+            out.append(node)
+
+    for tr in range(TB):
+        lst = []
+        ml_nodes(t["roots"][tr], 0, lst)
+        for x in lst:
+            merge(tr, x)
+    return heaps, extra
+
+
+def test_merge_matches_python_restatement():
+    base, _ = helpers.uniform(n=1500, d=16, nq=1)
+    S = 5
+    o = oracle.Index(base, trees=3, ml=3, epochs=0, S=S, K=S, L=12, check=10)
+    o.build_trees()
+    o.merge_init()
+    t = o.trees()
+    heaps, extra = _py_merge(base, t, 3, 3, S)
+    keys, flags, fill, _ = o.pools()
+    off, ids = o.lists(0)
+    checked = 0
+    for p in range(base.shape[0]):
+        if len(heaps[p]) == S:  # no random top-up
+            got = [(np.float32(np.uint32(k >> 32).view(np.float32)), int(k & 0xFFFFFFFF)) for k in keys[p, :S]]
+            assert got == heaps[p], p
+            assert (flags[p, :S] == 1).all() and fill[p] == S
+            nw = ids[off[p]:off[p + 1]]
+            assert list(nw[:S]) == [h[1] for h in heaps[p]]
+            if len(extra[p]) == 2 * S:
+                assert list(nw[S:]) == extra[p][S:2 * S]
+            checked += 1
+    assert checked > base.shape[0] * 0.9
+
+
+def test_merge_distance_count_matches_reference_10k():
+    # the reference's merge at the 10K x 128 configuration does 6.46e5 distances
+    base, _ = helpers.uniform()
+    o = oracle.Index(base, trees=8, ml=8, epochs=0)
+    o.build_trees()
+    o.merge_init()
+    assert abs(o.counters()["merge"] - 6.46e5) / 6.46e5 < 0.02
+
+
+# --------------------------------------------------------------------------
+# NN-descent (base_index.hpp:328-481)
+# --------------------------------------------------------------------------
+def test_nndescent_invariants():
+    base, _ = helpers.uniform(n=4000, d=32, nq=1)
+    o = oracle.Index(base, trees=4, ml=4, epochs=3, L=20, check=15, K=8, S=6)
+    o.build_trees()
+    o.merge_init()
+    for _ in range(3):
+        o.refine_iter()
+        keys, flags, fill, rng = o.pools()
+        for p in range(0, base.shape[0], 37):
+            f = fill[p]
+            k = keys[p, :f]
+            assert (np.diff(k.astype(np.uint64).view(np.int64)) > 0).all()  # strictly sorted, unique
+            ids_ = (k & np.uint64(0xFFFFFFFF)).astype(np.int64)
+            assert len(set(ids_.tolist())) == f
+            d = (k >> np.uint64(32)).astype(np.uint32).view(np.float32)
+            for j in range(0, f, 5):
+                assert d[j] == np.float32(oracle.l2(base[p], base[ids_[j]]))
+            assert 1 <= rng[p] <= f
+
+
+def test_graph_recall_matches_reference_10k():
+    """Reference (survey, 1 and 8 threads): graph recall@10 0.894-0.896 at the
+    10K x 128 configuration.  The sequential form reproduces the reference's
+    order of inserts; the batched form (the GPU's) must stay within 0.005."""
+    base, _ = helpers.uniform()
+    gt, _ = oracle.exact_knn(base, base, 10, exclude_self=True)
+    rec = {}
+    for mode in (0, 1):
+        o = oracle.Index(base, trees=8, ml=8, epochs=8, check=25, L=30, K=10, S=10, join_mode=mode)
+        o.build_index()
+        g = o.graph_matrix()
+        rec[mode] = helpers.recall(g, gt, 10)
+    assert 0.889 <= rec[1] <= 0.901, rec
+    assert abs(rec[0] - 0.895) <= 0.0075, rec
+    assert abs(rec[0] - rec[1]) <= 0.005, rec
+
+
+# --------------------------------------------------------------------------
+# search (kdtreeub_index.hpp:332-346, 363-494, 496-654)
+# --------------------------------------------------------------------------
+def _toy():
+    x = np.array([[0], [1], [10], [11], [20], [21]], np.float32)
+    o = oracle.Index(x, trees=1)
+    # root: dim 0 at 5 -> leaf {0,1} | node dim 0 at 15 -> leaf {2,3} | leaf {4,5}
+    t = dict(dim=np.array([0, -1, 0, -1, -1], np.int32), val=np.array([5, 0, 15, 0, 0], np.float32),
+             left=np.array([1, -1, 3, -1, -1], np.int32), right=np.array([2, -1, 4, -1, -1], np.int32),
+             start=np.array([0, 0, 2, 2, 4], np.uint32), end=np.array([6, 2, 6, 4, 6], np.uint32),
+             roots=np.array([0], np.int32), leaf=np.arange(6, dtype=np.uint32)[None])
+    o.set_trees(t)
+    g = [[1, 2], [0, 2], [3, 1], [2, 4], [5, 3], [4, 3]]
+    off = np.arange(7, dtype=np.uint64) * 2
+    o.set_graph(off, np.array(g, np.uint32).ravel())
+    return o
+
+
+def test_search_nnexp_hand_derived():
+    o = _toy()
+    q = np.array([[19]], np.float32)
+    ids, ds, nd = o.search(q, 2, 1, 2, 2, 1, 0)
+    assert ids.tolist() == [[4, 5]] and ds.tolist() == [[1, 4]] and nd.tolist() == [3]
+    ids, ds, nd = o.search(q, 3, 1, 3, 2, 1, 0)
+    assert ids.tolist() == [[4, 5, 3]] and ds.tolist() == [[1, 4, 64]] and nd.tolist() == [4]
+
+
+def test_search_kgraph_hand_derived():
+    o = _toy()
+    q = np.array([[19]], np.float32)
+    ids, ds, nd = o.search(q, 2, 1, 3, 2, 1, 1)
+    assert ids.tolist() == [[4, 5]] and ds.tolist() == [[1, 4]] and nd.tolist() == [6]
+
+
+def test_search_ties_lower_id_first():
+    x = np.array([[0], [2], [2], [4]], np.float32)
+    o = oracle.Index(x, trees=1)
+    t = dict(dim=np.array([-1], np.int32), val=np.array([0], np.float32), left=np.array([-1], np.int32),
+             right=np.array([-1], np.int32), start=np.array([0], np.uint32), end=np.array([4], np.uint32),
+             roots=np.array([0], np.int32), leaf=np.array([[3, 2, 1, 0]], np.uint32))
+    o.set_trees(t)
+    o.set_graph(np.arange(5, dtype=np.uint64), np.array([1, 2, 3, 0], np.uint32))
+    ids, ds, _ = o.search(np.array([[3]], np.float32), 3, 0, 4, 4, 1, 0)
+    assert ids.tolist() == [[1, 2, 3]]  # distance 1 for ids 1, 2 and 3: lower id first
+
+
+def test_search_full_pool_is_exact_knn():
+    base, q = helpers.uniform(n=800, d=24, nq=20)
+    o = oracle.Index(base, trees=2, ml=2, epochs=2, S=4, K=4, L=10, check=8)
+    o.build_index()
+    ids, ds, nd = o.search(q, 10, 0, 800, 20, 2, 0)
+    gt, gd = oracle.exact_knn(base, q, 10)
+    np.testing.assert_array_equal(ids, gt)
+    np.testing.assert_array_equal(ds, gd)
+    assert (nd == 800).all()
+
+
+def test_search_recall_matches_reference_10k():
+    """Reference (survey): search recall@10 method 0 0.342-0.382, method 1
+    0.586-0.620 at 4 trees, 4 epochs, pool 120, extend 20; about 499 and 1230
+    distances per query."""
+    base, q, o = helpers.oracle_index("uniform")
+    gt, _ = oracle.exact_knn(base, q, 10)
+    ids0, _, nd0 = o.search(q, 10, 4, 120, 20, 4, 0)
+    ids1, _, nd1 = o.search(q, 10, 4, 120, 20, 4, 1)
+    r0, r1 = helpers.recall(ids0, gt, 10), helpers.recall(ids1, gt, 10)
+    assert 0.33 <= r0 <= 0.40, r0
+    assert 0.57 <= r1 <= 0.64, r1
+    assert abs(nd0.mean() - 499) / 499 < 0.05
+    assert abs(nd1.mean() - 1230) / 1230 < 0.05
+
+
+def test_search_errors():
+    base, q, o = helpers.oracle_index("uniform")
+    with pytest.raises(oracle.OracleError):
+        o.search(q, 10, 4, 120, 20, 9, 0)  # wrong tree number
+    with pytest.raises(oracle.OracleError):
+        o.search(q, 10, 4, 120, 20, 0, 0)  # zero trees
+    with pytest.raises(oracle.OracleError):
+        o.search(q, 10, 4, 120, 20, 4, 2)  # no such method
+
+
+def test_kgraph_raises_pool_for_good():
+    base, q, o = helpers.oracle_index("uniform")
+    o.search(q[:2], 10, 4, 60, 20, 4, 1)
+    assert o.init_num_state == 80  # epochs * extend_to (kdtreeub_index.hpp:510-514)
