@@ -54,7 +54,8 @@ struct SearchCfg {
   int epochs;
   int K;        // results per query
   int R;        // pool kept: max(extend_to, K)
-  int kb;       // key buffer (power of two)
+  int Rp;       // R rounded up to a power of two
+  int kb;       // key buffer (seeds, then pool / new / merge areas)
   int logcap;   // visited-set capacity (log2)
   uint64_t seed;
 };
@@ -75,7 +76,8 @@ struct BuildState {
 };
 
 struct Workspace {
-  DevBuf<int32_t> leaves, overflow;
+  DevBuf<int32_t> leaves, overflow, qorder, qkey, qoff, scan_sums;
+  DevBuf<uint32_t> qhist;
   DevBuf<int> nover;
   DevBuf<unsigned long long> ndist;
   DevBuf<uint32_t> gtab;

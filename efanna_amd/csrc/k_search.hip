@@ -2,27 +2,36 @@
 //
 // Stage 1, k_descend: one thread per (query, tree) walks its tree depth first,
 // query side first, and records the first `lsize` leaves (getSearchNodeList,
-// kdtreeub_index.hpp:332-346).  The walk is stackless (parent links) so every
-// thread carries only its current node.
+// kdtreeub_index.hpp:332-346).  The walk is stackless (parent links).
 //
-// Stage 2, k_search_nnexp: one workgroup of 256 threads per query.
-//   * seeds: the leaves are visited round robin (leaf rank outer, tree inner,
-//     396-410); the first init_num distinct ids are kept.  Candidates are taken
-//     256 at a time; a candidate is kept when it is not yet visited and is the
-//     first occurrence of its id inside the chunk (a 512-slot LDS table keyed by
-//     id holds the minimum sequence position).  Missing seeds come from the
-//     counter-based generator (stands in for rand(), 411-417).
-//   * the visited set is an exact open-addressing hash set (LDS; a per-query
-//     region of global memory when the LDS table would pass 3/4 load — such
-//     queries are redone in a second launch).  Bit 31 of a slot marks
-//     "expanded" (newflag cleared, 450).
+// Stage 2, query order: queries are ordered by the position of their first
+// leaf in tree 0 (a counting sort), so queries that read the same base rows
+// run close together; the search kernel hands each XCD (workgroup id mod 8)
+// one contiguous slice of that order, keeping the shared rows in one L2.
+// Results do not depend on the order.
+//
+// Stage 3, k_search_nnexp: one workgroup of 256 threads per query.
+//   * seeds: leaves visited round robin (leaf rank outer, tree inner,
+//     396-410); the first init_num distinct ids are kept.  Candidates are
+//     taken 256 at a time; a candidate is kept when it is not yet visited and
+//     is the first occurrence of its id in the chunk (a 512-slot LDS table
+//     keyed by id holds the minimum sequence position).  Missing seeds come
+//     from the counter-based generator (stands in for rand(), 411-417).
+//   * visited set: exact open-addressing hash set in LDS (bit 31 of a slot =
+//     expanded, i.e. newflag cleared, 450); a query whose set would pass 3/4
+//     load is redone by a second launch with a per-workgroup table in global
+//     memory.
 //   * distances: two lanes per row, float4 loads, the reference's AVX
 //     accumulation order (bit-identical FP32).
-//   * pool = the best max(extend_to, K) by (distance, id) (partial_sort on
-//     pair<float, size_t>, 438-442), kept sorted in LDS; each epoch expands the
-//     unexpanded ids among the first extend_to, collects unvisited neighbours,
-//     and re-selects (446-484).  An epoch that expands nothing ends the loop
-//     (later epochs could not change the pool).
+//   * pool = the best R = max(extend_to, K) by (distance, id) (partial_sort
+//     on pair<float, size_t>, 438-442): a block radix select over the seed
+//     keys finds the R-th key, the R keys are sorted.  Each epoch expands the
+//     unexpanded ids among the first extend_to, collects unvisited
+//     neighbours, and merges them into the sorted pool by rank (446-484).  An
+//     epoch that expands nothing ends the loop (later ones could not change
+//     the pool).
+#include <algorithm>
+
 #include "efg_device.hpp"
 #include "efg_internal.hpp"
 
@@ -38,7 +47,6 @@ __global__ void k_descend(const float* __restrict__ Q, size_t q_stride, int nq, 
   int32_t* out = leaves + size_t(qi) * size_t(lsize) * T;  // [ni][t]
   int node = roots[t];
   int cnt = 0;
-  // descend to the first leaf, query side first
   for (;;) {
     int4 nd = node4[node];
     if (nd.z < 0) break;
@@ -70,13 +78,32 @@ __global__ void k_descend(const float* __restrict__ Q, size_t q_stride, int nq, 
   for (; cnt < lsize; ++cnt) out[size_t(cnt) * T + t] = -1;
 }
 
+// query order: bucket = start of the first leaf in tree 0 >> shift
+__global__ void k_qkey(const int32_t* __restrict__ leaves, int nleaf, int nq, const uint2* __restrict__ range,
+                       int shift, int32_t* __restrict__ key, uint32_t* __restrict__ hist) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  int nd = leaves[size_t(q) * nleaf];
+  int k = nd >= 0 ? int(range[nd].x >> shift) : 0;
+  key[q] = k;
+  atomicAdd(&hist[k], 1u);
+}
+__global__ void k_qscatter(const int32_t* __restrict__ key, int nq, const int32_t* __restrict__ off,
+                           uint32_t* __restrict__ cursor, int32_t* __restrict__ order) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  int k = key[q];
+  order[off[k] + int(atomicAdd(&cursor[k], 1u))] = q;
+}
+
 namespace {
 
 constexpr uint32_t kEmpty = 0xffffffffu;
 constexpr uint32_t kExp = 0x80000000u;
 constexpr int kCtab = 512;
+constexpr int kNB = 512;   // new-candidate buffer between merges
+constexpr int kKPT = 16;   // seed keys per thread held in registers (init_num <= 4096)
 
-// visited-set probe; returns slot index and whether the id is present
 __device__ inline int tab_find(const uint32_t* tab, int logcap, uint32_t id, bool* present) {
   const uint32_t mask = (1u << logcap) - 1;
   uint32_t s = hslot(id, logcap);
@@ -87,38 +114,85 @@ __device__ inline int tab_find(const uint32_t* tab, int logcap, uint32_t id, boo
     s = (s + 1) & mask;
   }
 }
-// insert; returns true when this call inserted the id
-__device__ inline bool tab_insert(uint32_t* tab, int logcap, uint32_t id, int* slot) {
+__device__ inline bool tab_insert(uint32_t* tab, int logcap, uint32_t id) {
   const uint32_t mask = (1u << logcap) - 1;
   uint32_t s = hslot(id, logcap);
   for (;;) {
     uint32_t v = tab[s];
     if (v == kEmpty) {
       uint32_t old = atomicCAS(&tab[s], kEmpty, id);
-      if (old == kEmpty) { *slot = int(s); return true; }
+      if (old == kEmpty) return true;
       v = old;
     }
-    if ((v & ~kExp) == id) { *slot = int(s); return false; }
+    if ((v & ~kExp) == id) return false;
     s = (s + 1) & mask;
   }
 }
 
 struct Smem {
   float* q;
-  uint64_t* buf;
-  uint32_t* tab;     // LDS table (or global when kGlobal)
+  uint64_t* buf;  // seeds; then pool P[Rp] | new N[kNB] | merge output O[Rp]
+  uint32_t* tab;
   uint32_t* ctab_id;
   uint32_t* ctab_pos;
-  int32_t* lnode;    // leaf node per sequence slot
-  int32_t* lpos;     // exclusive prefix of leaf sizes (nleaf + 1)
-  int32_t* lstart;   // leaf start in the tree's leaf list
-  uint32_t* explist; // expanded ids of the epoch
-  int32_t* expoff;   // prefix of their degrees (E + 1)
-  int* scan;         // 4 ints
-  int* misc;         // 8 ints
+  int32_t* lpos;
+  int32_t* lstart;
+  uint32_t* explist;
+  int32_t* expoff;
+  uint32_t* hist;
+  int* scan;
+  int* misc;
 };
 
 __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+// Block radix select over unique keys buf[0, m): returns T with exactly R
+// keys <= T.
+__device__ uint64_t block_select(const uint64_t* buf, int m, int R, uint32_t* hist, int* misc) {
+  const int tid = threadIdx.x;
+  uint64_t prefix = 0, mask = 0;
+  int need = R;
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    hist[tid] = 0;
+    __syncthreads();
+    for (int i = tid; i < m; i += kBlock) {
+      uint64_t k = buf[i];
+      if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (tid < 64) {
+      uint32_t h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2], h3 = hist[4 * tid + 3];
+      int loc = int(h0 + h1 + h2 + h3);
+      int x = loc;
+      for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o);
+        if (tid >= o) x += y;
+      }
+      int before = x - loc;  // count in bins < 4*tid
+      if (before < need && need <= x) {
+        int c = before;
+        int b = 4 * tid;
+        uint32_t hh[4] = {h0, h1, h2, h3};
+        int j = 0;
+        while (c + int(hh[j]) < need) {
+          c += int(hh[j]);
+          ++j;
+        }
+        misc[4] = b + j;
+        misc[5] = c;
+        misc[6] = int(hh[j]);
+      }
+    }
+    __syncthreads();
+    const int b = misc[4], c = misc[5], cnt = misc[6];
+    need -= c;
+    prefix |= uint64_t(b) << shift;
+    mask |= uint64_t(0xff) << shift;
+    __syncthreads();
+    if (cnt == need) return prefix | ~mask;
+  }
+  return prefix;
+}
 
 }  // namespace
 
@@ -130,9 +204,9 @@ __host__ __device__ inline size_t search_lds_bytes(const SearchCfg& c, bool glob
   b += a16(sizeof(uint64_t) * c.kb);
   if (!global_table) b += a16(sizeof(uint32_t) << c.logcap);
   b += a16(sizeof(uint32_t) * kCtab) * 2;
-  b += a16(sizeof(int32_t) * c.nleaf) * 2 + a16(sizeof(int32_t) * (c.nleaf + 1));
+  b += a16(sizeof(int32_t) * (c.nleaf + 1)) + a16(sizeof(int32_t) * c.nleaf);
   b += a16(sizeof(uint32_t) * c.E) + a16(sizeof(int32_t) * (c.E + 1));
-  b += a16(sizeof(int) * 4) + a16(sizeof(int) * 8);
+  b += a16(sizeof(uint32_t) * 256) + a16(sizeof(int) * 4) + a16(sizeof(int) * 8);
   return b;
 }
 
@@ -147,7 +221,7 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
                                                          float* __restrict__ out_d, uint64_t* __restrict__ ndist_q,
                                                          unsigned long long* __restrict__ ndist_total,
                                                          const int32_t* __restrict__ qlist, int nlist_host,
-                                                         const int* __restrict__ nlist_dev,
+                                                         const int* __restrict__ nlist_dev, int xcd_slices,
                                                          uint32_t* __restrict__ gtab, int32_t* __restrict__ overflow,
                                                          int* __restrict__ noverflow,
                                                          unsigned long long* __restrict__ work) {
@@ -167,11 +241,11 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
     else s.tab = gtab + (size_t(blockIdx.x) << c.logcap);
     s.ctab_id = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * kCtab));
     s.ctab_pos = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * kCtab));
-    s.lnode = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * c.nleaf));
-    s.lstart = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * c.nleaf));
     s.lpos = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * (c.nleaf + 1)));
+    s.lstart = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * c.nleaf));
     s.explist = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * c.E));
     s.expoff = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * (c.E + 1)));
+    s.hist = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * 256));
     s.scan = reinterpret_cast<int*>(take(sizeof(int) * 4));
     s.misc = reinterpret_cast<int*>(take(sizeof(int) * 8));
   }
@@ -179,24 +253,52 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
   const int cap = 1 << c.logcap;
   const int limit = cap - cap / 4;  // keep the table under 3/4 load
   const int pr = tid >> 1, h = tid & 1;  // lane pair for distances
+  const int R = c.R, Rp = c.Rp;
+  uint64_t* P = s.buf;
+  uint64_t* N = s.buf + Rp;
+  uint64_t* O = s.buf + Rp + kNB;
 
-  // distances for buf[from, to) whose low 32 bits hold ids
-  auto distances = [&](int from, int to) {
+  // distances for keys a[from, to) whose low 32 bits hold ids
+  auto distances = [&](uint64_t* a, int from, int to) {
     for (int i = from + pr; i < to; i += kBlock / 2) {
-      uint32_t id = key_id(s.buf[i]);
+      uint32_t id = key_id(a[i]);
       float dd = l2_pair_lanes(s.q, X + size_t(id) * c.D, c.D, h);
-      if (h) s.buf[i] = make_key(dd, id);
+      if (h) a[i] = make_key(dd, id);
     }
   };
-  // sort buf[0, m) ascending (pads to a power of two)
-  auto sort_buf = [&](int m) {
-    int p2 = next_pow2(m < 2 ? 2 : m);
-    for (int i = m + tid; i < p2; i += kBlock) s.buf[i] = kKeyMax;
+  // merge N[0, m) (unique, unsorted, disjoint from P) into the sorted pool P[0, R)
+  auto merge_new = [&](int m) {
+    for (int i = tid; i < m; i += kBlock) {
+      uint64_t key = N[i];
+      int lo = 0, hi = R;  // number of pool keys below key
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (P[mid] < key) lo = mid + 1; else hi = mid;
+      }
+      int rn = 0;
+      for (int j = 0; j < m; ++j) rn += (N[j] < key) ? 1 : 0;
+      if (lo + rn < R) O[lo + rn] = key;
+    }
+    for (int i = tid; i < R; i += kBlock) {
+      uint64_t key = P[i];
+      int rn = 0;
+      for (int j = 0; j < m; ++j) rn += (N[j] < key) ? 1 : 0;
+      if (i + rn < R) O[i + rn] = key;
+    }
     __syncthreads();
-    block_bitonic_sort(s.buf, p2);
+    for (int i = tid; i < R; i += kBlock) P[i] = O[i];
+    __syncthreads();
   };
 
-  for (int w = blockIdx.x; w < nlist; w += gridDim.x) {
+  // work distribution: slices of the query order per XCD
+  int w0 = blockIdx.x, wstep = gridDim.x, wend = nlist;
+  if (xcd_slices && (gridDim.x & 7) == 0) {
+    const int x = blockIdx.x & 7, j = blockIdx.x >> 3;
+    w0 = int((long long)nlist * x / 8) + j;
+    wend = int((long long)nlist * (x + 1) / 8);
+    wstep = gridDim.x >> 3;
+  }
+  for (int w = w0; w < wend; w += wstep) {
     const int qi = qlist ? qlist[w] : w;
     const float* qg = Q + size_t(qi) * q_stride;
     for (int i = tid; i < c.D; i += kBlock) s.q[i] = (i < c.d) ? qg[i] : 0.f;
@@ -208,14 +310,13 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       int sz = 0;
       if (li < c.nleaf) {
         int nd = leaves[size_t(qi) * c.nleaf + li];
-        s.lnode[li] = nd;
+        int st = 0;
         if (nd >= 0) {
           uint2 r = range[nd];
-          s.lstart[li] = int(r.x);
+          st = int(r.x);
           sz = int(r.y - r.x);
-        } else {
-          s.lstart[li] = 0;
         }
+        s.lstart[li] = st;
       }
       int tot;
       int ex = block_exclusive_scan(sz, s.scan, &tot);
@@ -223,14 +324,11 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       run += tot;
     }
     if (tid == 0) s.lpos[c.nleaf] = run;
-    if (tid == 0) { s.misc[0] = 0; s.misc[1] = 0; }
     __syncthreads();
     const int total = run;
 
     // ---- seed pool -------------------------------------------------------
     int p = 0;  // kept so far (uniform)
-    bool ovf = false;
-    // generic chunk step: candidate id for sequence slot `sq` (valid flag)
     auto seed_chunk = [&](bool valid, uint32_t id, uint32_t sq) {
       for (int i = tid; i < kCtab; i += kBlock) { s.ctab_id[i] = kEmpty; s.ctab_pos[i] = kEmpty; }
       __syncthreads();
@@ -259,8 +357,7 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       int cnt;
       int rank = block_exclusive_scan(first, s.scan, &cnt);
       if (first && p + rank < c.init_num) {
-        int sl;
-        tab_insert(s.tab, c.logcap, id, &sl);
+        tab_insert(s.tab, c.logcap, id);
         s.buf[p + rank] = uint64_t(id);
       }
       __syncthreads();
@@ -290,13 +387,35 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
     int inserted = c.init_num;
     uint64_t nexp_total = 0;
     const uint64_t seeds_read = uint64_t(seq_read);
-    distances(0, c.init_num);
+    distances(s.buf, 0, c.init_num);
     __syncthreads();
-    sort_buf(c.init_num);
+    // best R seeds: select the R-th key, gather the R keys, sort them
+    {
+      const uint64_t thr = (R < c.init_num) ? block_select(s.buf, c.init_num, R, s.hist, s.misc) : kKeyMax;
+      uint64_t mine[kKPT];
+      int nm = 0;
+#pragma unroll
+      for (int j = 0; j < kKPT; ++j) {
+        const int i = tid + j * kBlock;
+        mine[j] = (i < c.init_num) ? s.buf[i] : kKeyMax;
+        nm += (i < c.init_num && mine[j] <= thr) ? 1 : 0;
+      }
+      int tot;
+      int o = block_exclusive_scan(nm, s.scan, &tot);  // barriers inside: all reads precede the writes
+#pragma unroll
+      for (int j = 0; j < kKPT; ++j) {
+        const int i = tid + j * kBlock;
+        if (i < c.init_num && mine[j] <= thr) P[o++] = mine[j];
+      }
+      for (int i = R + tid; i < Rp; i += kBlock) P[i] = kKeyMax;
+      __syncthreads();
+      block_bitonic_sort(P, Rp);
+    }
     uint64_t ndq = uint64_t(c.init_num);
+    bool ovf = false;
 
     // ---- epochs ------------------------------------------------------------
-    for (int ep = 0; ep < c.epochs && !ovf; ++ep) {
+    for (int ep = 0; ep < c.epochs; ++ep) {
       // expansion list: unexpanded ids among the first extend_to (448-451)
       int nexp = 0;
       for (int b0 = 0; b0 < c.E; b0 += kBlock) {
@@ -305,7 +424,7 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
         uint32_t id = 0;
         int sl = -1;
         if (j < c.E) {
-          id = key_id(s.buf[j]);
+          id = key_id(P[j]);
           bool pres;
           sl = tab_find(s.tab, c.logcap, id, &pres);
           take = (s.tab[sl] & kExp) ? 0 : 1;
@@ -321,7 +440,6 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       __syncthreads();
       if (nexp == 0) break;
       nexp_total += uint64_t(nexp);
-      // neighbour offsets of the expanded ids
       int nnb = 0;
       for (int b0 = 0; b0 < nexp; b0 += kBlock) {
         int j = b0 + tid;
@@ -335,10 +453,12 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
         if (j < nexp) s.expoff[j] = nnb + ex;
         nnb += tot;
       }
-      if (tid == 0) { s.expoff[nexp] = nnb; s.misc[0] = c.R; }
+      if (tid == 0) {
+        s.expoff[nexp] = nnb;
+        s.misc[0] = 0;
+      }
       __syncthreads();
-      int dd = c.R;     // buf[0, dd) carry distances
-      int mprev = c.R;  // fill after the previous chunk
+      int m = 0;
       for (int b0 = 0; b0 < nnb; b0 += kBlock) {
         int sq = b0 + tid;
         if (sq < nnb) {
@@ -349,40 +469,34 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
           }
           uint32_t e = s.explist[lo];
           uint32_t id = gids[goff[e] + uint64_t(sq - s.expoff[lo])];
-          int sl;
-          if (tab_insert(s.tab, c.logcap, id, &sl)) s.buf[atomicAdd(&s.misc[0], 1)] = uint64_t(id);
+          if (tab_insert(s.tab, c.logcap, id)) N[atomicAdd(&s.misc[0], 1)] = uint64_t(id);
         }
         __syncthreads();
-        int m = s.misc[0];
-        inserted += m - mprev;
-        mprev = m;
-        if (inserted + kBlock > limit) { ovf = true; break; }
-        if (m + kBlock > c.kb) {  // buffer full: keep the best R so far
-          distances(dd, m);
-          ndq += uint64_t(m - dd);
+        const int m2 = s.misc[0];
+        inserted += m2 - m;
+        m = m2;
+        if (inserted + kBlock > limit) {
+          ovf = true;
+          break;
+        }
+        const bool last = b0 + kBlock >= nnb;
+        if (m > 0 && (last || m + kBlock > kNB)) {
+          distances(N, 0, m);
+          ndq += uint64_t(m);
           __syncthreads();
-          sort_buf(m);
-          dd = c.R;
-          mprev = c.R;
-          if (tid == 0) s.misc[0] = c.R;
+          merge_new(m);
+          m = 0;
+          if (tid == 0) s.misc[0] = 0;
           __syncthreads();
         }
       }
       if (ovf) break;
-      int m = s.misc[0];
-      if (m > dd) {
-        distances(dd, m);
-        ndq += uint64_t(m - dd);
-        __syncthreads();
-        sort_buf(m);
-      }
-      __syncthreads();
     }
     if (ovf) {
       if (tid == 0) overflow[atomicAdd(noverflow, 1)] = qi;
     } else {
       for (int j = tid; j < c.K; j += kBlock) {
-        uint64_t k = s.buf[j];
+        uint64_t k = P[j];
         out_ids[size_t(qi) * c.K + j] = int32_t(key_id(k));
         if (out_d) out_d[size_t(qi) * c.K + j] = key_dist(k);
       }
@@ -425,6 +539,7 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
   const int init = ix->s_init, E = ix->s_extend, epochs = ix->s_epochs;
   if (init <= 0 || E < 0 || epochs < 0) return fail(EFG_ERR_ARG, "invalid search parameters");
   if (size_t(init) > ix->n) return fail(EFG_ERR_ARG, "pool size larger than the data set");
+  if (init > kKPT * kBlock) return fail(EFG_ERR_ARG, "pool size above 4096 is not supported");
   const int R = E > k ? E : k;
   if (k <= 0 || R > init) return fail(EFG_ERR_ARG, "k must be positive and at most the pool size");
   SearchCfg c{};
@@ -439,7 +554,8 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
   c.epochs = epochs;
   c.K = k;
   c.R = R;
-  c.kb = next_pow2(init > R + 512 ? init : R + 512);
+  c.Rp = next_pow2(R < 2 ? 2 : R);
+  c.kb = std::max(init, 2 * c.Rp + kNB);
   c.logcap = ceil_log2(size_t(init + 512) * 4 / 3 + 1);
   if (c.logcap < 10) c.logcap = 10;
   c.seed = ix->bp.seed;
@@ -448,7 +564,10 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
   bool table_in_lds = lds <= lds_cap;
   if (!table_in_lds && search_lds_bytes(c, true) > lds_cap) return fail(EFG_ERR_ARG, "pool too large for one workgroup");
   efg::Workspace& w = ix->ws;
-  if (!w.leaves.ensure(nq * size_t(c.nleaf)) || !w.overflow.ensure(2 * nq) || !w.nover.ensure(2) || !w.ndist.ensure(3))
+  const int nb_q = 4096;
+  if (!w.leaves.ensure(nq * size_t(c.nleaf)) || !w.overflow.ensure(2 * nq) || !w.nover.ensure(2) ||
+      !w.ndist.ensure(3) || !w.qorder.ensure(nq) || !w.qkey.ensure(nq) || !w.qhist.ensure(nb_q + 1) ||
+      !w.qoff.ensure(nb_q + 1))
     return fail(EFG_ERR_MEMORY, "out of device memory (search workspace)");
   if (int rc = check_hip(hipMemsetAsync(w.nover.p, 0, 2 * sizeof(int), st), "memset")) return rc;
   if (int rc = check_hip(hipMemsetAsync(w.ndist.p, 0, 3 * sizeof(unsigned long long), st), "memset")) return rc;
@@ -466,6 +585,16 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
     size_t threads = nq * size_t(c.T);
     k_descend<<<unsigned((threads + 255) / 256), 256, 0, st>>>(dq, q_stride, int(nq), c.T, c.lsize, ix->roots.p,
                                                                ix->node4.p, ix->parent.p, w.leaves.p);
+    // query order by first leaf
+    int shift = std::max(0, ceil_log2(ix->n + 1) - 12);
+    hipMemsetAsync(w.qhist.p, 0, size_t(nb_q + 1) * 4, st);
+    k_qkey<<<unsigned((nq + 255) / 256), 256, 0, st>>>(w.leaves.p, c.nleaf, int(nq), ix->range.p, shift, w.qkey.p,
+                                                       w.qhist.p);
+    if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(w.qhist.p), nb_q, w.qoff.p, w.qoff.p + nb_q,
+                                       w.scan_sums, st))
+      return rc;
+    hipMemsetAsync(w.qhist.p, 0, size_t(nb_q + 1) * 4, st);
+    k_qscatter<<<unsigned((nq + 255) / 256), 256, 0, st>>>(w.qkey.p, int(nq), w.qoff.p, w.qhist.p, w.qorder.p);
   }
   if (prof) hipEventRecord(ev[1], st);
   int dev = 0, ncu = 256;
@@ -474,13 +603,13 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
   if (table_in_lds) {
     int per_cu = int(lds_cap / lds);
     if (per_cu < 1) per_cu = 1;
-    size_t grid = size_t(ncu) * per_cu * 2;
-    if (grid > nq) grid = nq;
+    size_t grid = size_t(ncu) * per_cu;
+    if (grid > nq) grid = (nq + 7) & ~size_t(7);
     hipFuncSetAttribute(reinterpret_cast<const void*>(&k_search_nnexp<false>),
                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     k_search_nnexp<false><<<unsigned(grid), kBlock, lds, st>>>(
         c, dq, q_stride, ix->data.p, w.leaves.p, ix->range.p, ix->leaf.p, ix->goff.p, ix->gids.p, d_ids, d_dists,
-        d_ndist, w.ndist.p, nullptr, int(nq), nullptr, nullptr, w.overflow.p, w.nover.p, ix->work.p);
+        d_ndist, w.ndist.p, w.qorder.p, int(nq), nullptr, 1, nullptr, w.overflow.p, w.nover.p, ix->work.p);
   }
   if (prof) hipEventRecord(ev[2], st);
   // queries whose visited set outgrew the LDS table (or all of them when the
@@ -498,12 +627,12 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
   if (table_in_lds) {
     k_search_nnexp<true><<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->range.p, ix->leaf.p,
                                                       ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
-                                                      w.overflow.p, 0, w.nover.p, w.gtab.p, w.overflow.p + nq,
+                                                      w.overflow.p, 0, w.nover.p, 0, w.gtab.p, w.overflow.p + nq,
                                                       w.nover.p + 1, ix->work.p);
   } else {
     k_search_nnexp<true><<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->range.p, ix->leaf.p,
                                                       ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
-                                                      nullptr, int(nq), nullptr, w.gtab.p, w.overflow.p + nq,
+                                                      w.qorder.p, int(nq), nullptr, 0, w.gtab.p, w.overflow.p + nq,
                                                       w.nover.p + 1, ix->work.p);
   }
   if (prof) {
