@@ -140,8 +140,15 @@ def main() -> int:
     alg_bytes = 4 * d * per_launch[0] + 4 * per_launch[1] + 4 * K * per_launch[2]
     kern_ms = kms[1] / max(1, nsearch)
     achieved = alg_bytes / (kern_ms / 1e3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "search_mixture1m_traffic.json")
+    if os.path.exists(tpath) and n == 1_000_000 and nq == 10_000:
+        with open(tpath) as fh:
+            traffic = json.load(fh)["hbm_bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": "profiles/search_mixture1m_traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE)"
+                if traffic else None,
                 "kernel": "k_search_nnexp", "kernel_ms": round(kern_ms, 4),
                 "descend_ms": round(kms[0] / max(1, nsearch), 4),
                 "bytes_per_launch": int(alg_bytes)}

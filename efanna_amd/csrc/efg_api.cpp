@@ -67,6 +67,7 @@ int upload_trees(efg_index* ix, const HostTrees& h) {
   if (rc) return EFG_ERR_DEVICE;
   ix->T = h.T;
   ix->M = M;
+  ix->snode_valid = false;
   ix->leaf_dim = h.leaf_dim;
   if (ix->leaf_dim.size() != M) ix->leaf_dim.assign(M, -1);
   return EFG_OK;

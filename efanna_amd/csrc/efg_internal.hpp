@@ -42,6 +42,16 @@ struct DevBuf {
   }
 };
 
+// Tree node in the search layout: nodes of one tree in depth-first pre-order
+// (left child first), so a subtree occupies a contiguous block.
+struct alignas(32) SNode {
+  int32_t dim;
+  float val;
+  int32_t left, right, parent;
+  uint32_t start, end;
+  int32_t pad;
+};
+
 // Search launch configuration (all sizes in elements).
 struct SearchCfg {
   int n;        // base rows
@@ -105,6 +115,9 @@ struct efg_index {
   efg::DevBuf<int32_t> roots;
   efg::DevBuf<uint32_t> leaf;    // T x n
   std::vector<int8_t> leaf_dim;  // value written as DivDim of a leaf (0 or -1)
+  efg::DevBuf<efg::SNode> snode;  // search layout of the trees
+  efg::DevBuf<int32_t> sroots;
+  bool snode_valid = false;
   // graph (CSR)
   size_t nnz = 0;
   int max_deg = 0;

@@ -145,18 +145,92 @@ __global__ void k_probe(const float* __restrict__ X, int D, const int4* __restri
   }
 }
 
-__global__ void k_sort_buckets(const int32_t* __restrict__ boff, int M, uint64_t* __restrict__ rec) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M) return;
-  int b0 = boff[i], b1 = boff[i + 1];
-  for (int a = b0 + 1; a < b1; ++a) {  // insertion sort (buckets are small)
-    uint64_t v = rec[a];
-    int j = a - 1;
-    while (j >= b0 && rec[j] > v) {
-      rec[j + 1] = rec[j];
-      --j;
+__device__ inline uint64_t shfl_xor_u64(uint64_t v, int m) {
+  uint32_t lo = uint32_t(v), hi = uint32_t(v >> 32);
+  lo = uint32_t(__shfl_xor(int(lo), m));
+  hi = uint32_t(__shfl_xor(int(hi), m));
+  return (uint64_t(hi) << 32) | lo;
+}
+
+// Sort every bucket ascending: one wavefront per bucket; up to 64 records in
+// registers (bitonic network across lanes), up to 1024 in LDS, longer ones
+// by one lane.
+__global__ void __launch_bounds__(kBlock) k_sort_buckets(const int32_t* __restrict__ boff, int M,
+                                                         uint64_t* __restrict__ rec, int32_t* __restrict__ big,
+                                                         int* __restrict__ nbig) {
+  __shared__ uint64_t sh[kBlock / 64][1024];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int i = blockIdx.x * (kBlock / 64) + wv; i < M; i += gridDim.x * (kBlock / 64)) {
+    const int b0 = boff[i], m = boff[i + 1] - b0;
+    if (m <= 1) continue;
+    if (m <= 64) {
+      uint64_t v = lane < m ? rec[b0 + lane] : kKeyMax;
+      for (int k = 2; k <= 64; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          uint64_t o = shfl_xor_u64(v, j);
+          bool up = (lane & k) == 0, lower = (lane & j) == 0;
+          uint64_t mn = v < o ? v : o, mx = v < o ? o : v;
+          v = (lower == up) ? mn : mx;
+        }
+      }
+      if (lane < m) rec[b0 + lane] = v;
+    } else if (m <= 1024) {
+      uint64_t* b = sh[wv];
+      int p2 = next_pow2(m);
+      for (int t = lane; t < p2; t += 64) b[t] = t < m ? rec[b0 + t] : kKeyMax;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      for (int k = 2; k <= p2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          for (int t = lane; t < p2; t += 64) {
+            int u = t ^ j;
+            if (u > t) {
+              uint64_t x = b[t], y = b[u];
+              bool up = (t & k) == 0;
+              if ((x > y) == up) {
+                b[t] = y;
+                b[u] = x;
+              }
+            }
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+      }
+      for (int t = lane; t < m; t += 64) rec[b0 + t] = b[t];
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    } else if (lane == 0) {
+      big[atomicAdd(nbig, 1)] = i;  // sorted by k_sort_big
     }
-    rec[j + 1] = v;
+  }
+}
+
+// Buckets above 1024 records: one workgroup each, bitonic sort in LDS (up to
+// 8192 records); longer buckets are sorted by one thread.
+__global__ void __launch_bounds__(kBlock) k_sort_big(const int32_t* __restrict__ boff, const int32_t* __restrict__ big,
+                                                     const int* __restrict__ nbig, uint64_t* __restrict__ rec) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  uint64_t* b = reinterpret_cast<uint64_t*>(smem_raw);
+  const int nb = *nbig;
+  for (int w = blockIdx.x; w < nb; w += gridDim.x) {
+    const int i = big[w];
+    const int b0 = boff[i], m = boff[i + 1] - b0;
+    if (m <= 8192) {
+      const int p2 = next_pow2(m);
+      for (int t = threadIdx.x; t < p2; t += kBlock) b[t] = t < m ? rec[b0 + t] : kKeyMax;
+      __syncthreads();
+      block_bitonic_sort(b, p2);
+      for (int t = threadIdx.x; t < m; t += kBlock) rec[b0 + t] = b[t];
+    } else if (threadIdx.x == 0) {
+      for (int a = b0 + 1; a < b0 + m; ++a) {
+        uint64_t v = rec[a];
+        int j = a - 1;
+        while (j >= b0 && rec[j] > v) {
+          rec[j + 1] = rec[j];
+          --j;
+        }
+        rec[j + 1] = v;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -311,6 +385,19 @@ __global__ void k_worst(const uint64_t* __restrict__ pool, const uint32_t* __res
   worst[p] = fill[p] < uint32_t(L) ? kKeyMax : pool[size_t(p) * L + L - 1];
 }
 
+// upper bound of the candidates of one join: two per pair
+__global__ void k_pair_bound(uint32_t n, const uint64_t* __restrict__ new_off, const uint64_t* __restrict__ old_off,
+                             unsigned long long* __restrict__ out) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = 0;
+  if (p < n) {
+    unsigned long long na = new_off[p + 1] - new_off[p], nb = old_off[p + 1] - old_off[p];
+    v = na * (na > 0 ? na - 1 : 0) / 2 + na * nb;
+  }
+  for (int o = 32; o > 0; o >>= 1) v += (unsigned long long)__shfl_down((long long)v, o);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+}
+
 struct Stage {
   uint32_t target;
   uint32_t src;  // joining node | kFirst
@@ -372,26 +459,43 @@ __global__ void __launch_bounds__(kBlock) k_join(const float* __restrict__ X, in
         if (ib < na && !(a < b)) valid = false;
       }
       // both lanes of a pair agree on `valid`
+      bool ca = false, cb = false;
+      uint64_t ka = 0, kb = 0;
       if (valid) {
         const float* ra = staged ? rows + size_t(ia) * D : X + size_t(a) * D;
         const float* rb = staged ? rows + size_t(ib) * D : X + size_t(b) * D;
         float dist = l2_pair_lanes(ra, rb, D, h);
         if (h) {
           ++pairs_local;
-          uint64_t ka = make_key(dist, b), kb = make_key(dist, a);
-          if (ka < worst[a]) {
-            unsigned long long at = atomicAdd(nstage, 1ull);
-            if (at < cap) {
-              stage[at] = Stage{a, nn | kFirst, ka};
-              atomicAdd(&tcount[a], 1u);
-            }
+          ka = make_key(dist, b);
+          kb = make_key(dist, a);
+          ca = ka < worst[a];
+          cb = kb < worst[b];
+        }
+      }
+      // one reservation per wavefront
+      const uint64_t ma = __ballot(ca), mb = __ballot(cb);
+      const int lane = threadIdx.x & 63;
+      const int na_ = __popcll(ma), tot = na_ + __popcll(mb);
+      if (tot) {
+        const uint64_t any = ma | mb;
+        const int leader = __ffsll((unsigned long long)any) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(nstage, (unsigned long long)tot);
+        base = (unsigned long long)__shfl((long long)base, leader);
+        const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        if (ca) {
+          unsigned long long at = base + __popcll(ma & lt);
+          if (at < cap) {
+            stage[at] = Stage{a, nn | kFirst, ka};
+            atomicAdd(&tcount[a], 1u);
           }
-          if (kb < worst[b]) {
-            unsigned long long at = atomicAdd(nstage, 1ull);
-            if (at < cap) {
-              stage[at] = Stage{b, nn, kb};
-              atomicAdd(&tcount[b], 1u);
-            }
+        }
+        if (cb) {
+          unsigned long long at = base + na_ + __popcll(mb & lt);
+          if (at < cap) {
+            stage[at] = Stage{b, nn, kb};
+            atomicAdd(&tcount[b], 1u);
           }
         }
       }
@@ -773,6 +877,16 @@ int gpu_merge_init(efg_index* ix) {
   if (int rc = check_hip(hipStreamSynchronize(st), "merge")) return rc;
   const size_t lds = size_t(kBlock) * S * 8;
   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replay), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sort_big), hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8);
+  DevBuf<int32_t> bigl;
+  DevBuf<int> nbig;
+  if (!bigl.alloc(M) || !nbig.alloc(1)) return fail(EFG_ERR_MEMORY, "out of device memory (merge)");
+  int ncu_m = 256;
+  {
+    int dv = 0;
+    hipGetDevice(&dv);
+    hipDeviceGetAttribute(&ncu_m, hipDeviceAttributeMultiprocessorCount, dv);
+  }
   for (int t = 0; t < TB; ++t) {
     const uint32_t* lt = ix->leaf.p + size_t(t) * n;
     const int32_t* la = leaf_at.p + size_t(t) * n;
@@ -787,7 +901,10 @@ int gpu_merge_init(efg_index* ix) {
     hipMemsetAsync(cursor.p, 0, size_t(M) * 4, st);
     k_probe<<<grid_for(n), 256, 0, st>>>(ix->data.p, D, ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la, n,
                                          p.merge_level, 1, cnt.p, boff.p, cursor.p, rec.p);
-    k_sort_buckets<<<grid_for(M), 256, 0, st>>>(boff.p, M, rec.p);
+    hipMemsetAsync(nbig.p, 0, sizeof(int), st);
+    k_sort_buckets<<<unsigned(std::min<size_t>((M + 3) / 4, 65536)), kBlock, 0, st>>>(boff.p, M, rec.p, bigl.p,
+                                                                                       nbig.p);
+    k_sort_big<<<unsigned(ncu_m * 2), kBlock, size_t(8192) * 8, st>>>(boff.p, bigl.p, nbig.p, rec.p);
     k_replay<<<grid_for(n), kBlock, lds, st>>>(ix->data.p, D, ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la,
                                                n, p.merge_level, S, boff.p, rec.p, hkey.p, hsize.p, extra.p,
                                                ecount.p, nd.p);
@@ -842,9 +959,13 @@ int gpu_refine_iter(efg_index* ix) {
       !ctr.alloc(2))
     return fail(EFG_ERR_MEMORY, "out of device memory (join)");
   k_worst<<<grid_for(n), 256, 0, st>>>(bs.pool.p, bs.fill.p, n, L, worst.p);
-  // staging capacity: the previous size or an estimate, grown on overflow
-  static thread_local size_t stage_cap_hint = 0;
-  size_t cap = std::max<size_t>(stage_cap_hint, size_t(n) * 64);
+  // staging capacity: two candidates per pair at most
+  hipMemsetAsync(ctr.p, 0, 16, st);
+  k_pair_bound<<<grid_for(n), 256, 0, st>>>(n, bs.new_off.p, bs.old_off.p, ctr.p);
+  unsigned long long bound = 0;
+  hipMemcpyAsync(&bound, ctr.p, 8, hipMemcpyDeviceToHost, st);
+  if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
+  size_t cap = size_t(2 * bound + 1024);
   DevBuf<Stage> stage;
   const int stage_rows = int((48 * 1024) / (size_t(D) * 4));
   const size_t lds_join = size_t(stage_rows > 0 ? stage_rows : 0) * D * 4;
@@ -868,7 +989,6 @@ int gpu_refine_iter(efg_index* ix) {
     cap = size_t(h_ctr[0]) + size_t(h_ctr[0]) / 8 + 1024;
     if (attempt == 2) return fail(EFG_ERR_STATE, "join staging did not converge");
   }
-  stage_cap_hint = cap;
   const unsigned long long nst = h_ctr[0];
   ix->join_pairs.push_back(h_ctr[1]);
   ix->counters[3] += h_ctr[1];

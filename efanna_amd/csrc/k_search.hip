@@ -31,6 +31,8 @@
 //     epoch that expands nothing ends the loop (later ones could not change
 //     the pool).
 #include <algorithm>
+#include <cstring>
+#include <vector>
 
 #include "efg_device.hpp"
 #include "efg_internal.hpp"
@@ -38,53 +40,53 @@
 namespace efg {
 
 __global__ void k_descend(const float* __restrict__ Q, size_t q_stride, int nq, int T, int lsize,
-                          const int32_t* __restrict__ roots, const int4* __restrict__ node4,
-                          const int32_t* __restrict__ parent, int32_t* __restrict__ leaves) {
+                          const int32_t* __restrict__ roots, const SNode* __restrict__ sn,
+                          int32_t* __restrict__ leaves) {
   const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= size_t(nq) * T) return;
   const int qi = int(gid / T), t = int(gid % T);
   const float* q = Q + size_t(qi) * q_stride;
   int32_t* out = leaves + size_t(qi) * size_t(lsize) * T;  // [ni][t]
   int node = roots[t];
+  SNode cur = sn[node];
   int cnt = 0;
-  for (;;) {
-    int4 nd = node4[node];
-    if (nd.z < 0) break;
-    node = (q[nd.x] < __int_as_float(nd.y)) ? nd.z : nd.w;
+  while (cur.left >= 0) {  // first leaf, query side first
+    node = (q[cur.dim] < cur.val) ? cur.left : cur.right;
+    cur = sn[node];
   }
   for (;;) {
     out[size_t(cnt) * T + t] = node;
     if (++cnt == lsize) break;
     bool more = false;
-    for (;;) {
-      int p = parent[node];
-      if (p < 0) break;
-      int4 pd = node4[p];
-      int first = (q[pd.x] < __int_as_float(pd.y)) ? pd.z : pd.w;
+    while (cur.parent >= 0) {  // up to the first ancestor whose second child is unvisited
+      const int p = cur.parent;
+      const SNode pd = sn[p];
+      const int first = (q[pd.dim] < pd.val) ? pd.left : pd.right;
       if (node == first) {
-        node = (first == pd.z) ? pd.w : pd.z;
+        node = (first == pd.left) ? pd.right : pd.left;
+        cur = sn[node];
         more = true;
         break;
       }
       node = p;
+      cur = pd;
     }
     if (!more) break;
-    for (;;) {
-      int4 nd = node4[node];
-      if (nd.z < 0) break;
-      node = (q[nd.x] < __int_as_float(nd.y)) ? nd.z : nd.w;
+    while (cur.left >= 0) {
+      node = (q[cur.dim] < cur.val) ? cur.left : cur.right;
+      cur = sn[node];
     }
   }
   for (; cnt < lsize; ++cnt) out[size_t(cnt) * T + t] = -1;
 }
 
 // query order: bucket = start of the first leaf in tree 0 >> shift
-__global__ void k_qkey(const int32_t* __restrict__ leaves, int nleaf, int nq, const uint2* __restrict__ range,
+__global__ void k_qkey(const int32_t* __restrict__ leaves, int nleaf, int nq, const SNode* __restrict__ sn,
                        int shift, int32_t* __restrict__ key, uint32_t* __restrict__ hist) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
   int nd = leaves[size_t(q) * nleaf];
-  int k = nd >= 0 ? int(range[nd].x >> shift) : 0;
+  int k = nd >= 0 ? int(sn[nd].start >> shift) : 0;
   key[q] = k;
   atomicAdd(&hist[k], 1u);
 }
@@ -214,7 +216,7 @@ template <bool kGlobal>
 __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const float* __restrict__ Q, size_t q_stride,
                                                          const float* __restrict__ X,
                                                          const int32_t* __restrict__ leaves,
-                                                         const uint2* __restrict__ range,
+                                                         const SNode* __restrict__ sn,
                                                          const uint32_t* __restrict__ leaf_lists,
                                                          const uint64_t* __restrict__ goff,
                                                          const uint32_t* __restrict__ gids, int32_t* __restrict__ out_ids,
@@ -312,9 +314,8 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
         int nd = leaves[size_t(qi) * c.nleaf + li];
         int st = 0;
         if (nd >= 0) {
-          uint2 r = range[nd];
-          st = int(r.x);
-          sz = int(r.y - r.x);
+          st = int(sn[nd].start);
+          sz = int(sn[nd].end - sn[nd].start);
         }
         s.lstart[li] = st;
       }
@@ -524,6 +525,62 @@ int ceil_log2(size_t x) {
 }
 }  // namespace
 
+// Depth-first pre-order copy of the trees for the search descent.
+int build_search_layout(efg_index* ix) {
+  const size_t M = ix->M;
+  std::vector<int4> n4(M);
+  std::vector<uint2> rg(M);
+  std::vector<int32_t> roots(ix->T);
+  int rc = 0;
+  rc |= check_hip(hipMemcpy(n4.data(), ix->node4.p, M * sizeof(int4), hipMemcpyDeviceToHost), "copy");
+  rc |= check_hip(hipMemcpy(rg.data(), ix->range.p, M * sizeof(uint2), hipMemcpyDeviceToHost), "copy");
+  rc |= check_hip(hipMemcpy(roots.data(), ix->roots.p, ix->T * 4, hipMemcpyDeviceToHost), "copy");
+  if (rc) return EFG_ERR_DEVICE;
+  std::vector<int32_t> nid(M, -1), order;
+  order.reserve(M);
+  std::vector<int32_t> st;
+  for (int t = 0; t < ix->T; ++t) {
+    st.push_back(roots[t]);
+    while (!st.empty()) {
+      int32_t x = st.back();
+      st.pop_back();
+      nid[x] = int32_t(order.size());
+      order.push_back(x);
+      if (n4[x].z >= 0) {
+        st.push_back(n4[x].w);
+        st.push_back(n4[x].z);  // left first
+      }
+    }
+  }
+  std::vector<SNode> out(order.size());
+  for (size_t i = 0; i < order.size(); ++i) {
+    const int32_t x = order[i];
+    SNode& o = out[i];
+    o.dim = n4[x].x;
+    std::memcpy(&o.val, &n4[x].y, 4);
+    o.left = n4[x].z >= 0 ? nid[n4[x].z] : -1;
+    o.right = n4[x].w >= 0 ? nid[n4[x].w] : -1;
+    o.parent = -1;
+    o.start = rg[x].x;
+    o.end = rg[x].y;
+    o.pad = 0;
+  }
+  for (size_t i = 0; i < out.size(); ++i) {
+    if (out[i].left >= 0) {
+      out[out[i].left].parent = int32_t(i);
+      out[out[i].right].parent = int32_t(i);
+    }
+  }
+  std::vector<int32_t> sroots(ix->T);
+  for (int t = 0; t < ix->T; ++t) sroots[t] = nid[roots[t]];
+  if (!ix->snode.alloc(out.size()) || !ix->sroots.alloc(ix->T)) return fail(EFG_ERR_MEMORY, "out of device memory (search trees)");
+  rc |= check_hip(hipMemcpy(ix->snode.p, out.data(), out.size() * sizeof(SNode), hipMemcpyHostToDevice), "copy");
+  rc |= check_hip(hipMemcpy(ix->sroots.p, sroots.data(), ix->T * 4, hipMemcpyHostToDevice), "copy");
+  if (rc) return EFG_ERR_DEVICE;
+  ix->snode_valid = true;
+  return EFG_OK;
+}
+
 // knnSearch, method 0, on device buffers (kdtreeub_index.hpp:363-494).
 int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int32_t* d_ids, float* d_dists,
                   uint64_t* d_ndist, hipStream_t st) {
@@ -563,6 +620,10 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
   const size_t lds_cap = 150 * 1024;
   bool table_in_lds = lds <= lds_cap;
   if (!table_in_lds && search_lds_bytes(c, true) > lds_cap) return fail(EFG_ERR_ARG, "pool too large for one workgroup");
+  if (!ix->snode_valid) {
+    if (int rc = check_hip(hipStreamSynchronize(st), "sync")) return rc;
+    if (int rc = build_search_layout(ix)) return rc;
+  }
   efg::Workspace& w = ix->ws;
   const int nb_q = 4096;
   if (!w.leaves.ensure(nq * size_t(c.nleaf)) || !w.overflow.ensure(2 * nq) || !w.nover.ensure(2) ||
@@ -583,12 +644,12 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
   }
   {
     size_t threads = nq * size_t(c.T);
-    k_descend<<<unsigned((threads + 255) / 256), 256, 0, st>>>(dq, q_stride, int(nq), c.T, c.lsize, ix->roots.p,
-                                                               ix->node4.p, ix->parent.p, w.leaves.p);
+    k_descend<<<unsigned((threads + 255) / 256), 256, 0, st>>>(dq, q_stride, int(nq), c.T, c.lsize, ix->sroots.p,
+                                                               ix->snode.p, w.leaves.p);
     // query order by first leaf
     int shift = std::max(0, ceil_log2(ix->n + 1) - 12);
     hipMemsetAsync(w.qhist.p, 0, size_t(nb_q + 1) * 4, st);
-    k_qkey<<<unsigned((nq + 255) / 256), 256, 0, st>>>(w.leaves.p, c.nleaf, int(nq), ix->range.p, shift, w.qkey.p,
+    k_qkey<<<unsigned((nq + 255) / 256), 256, 0, st>>>(w.leaves.p, c.nleaf, int(nq), ix->snode.p, shift, w.qkey.p,
                                                        w.qhist.p);
     if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(w.qhist.p), nb_q, w.qoff.p, w.qoff.p + nb_q,
                                        w.scan_sums, st))
@@ -608,7 +669,7 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
     hipFuncSetAttribute(reinterpret_cast<const void*>(&k_search_nnexp<false>),
                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     k_search_nnexp<false><<<unsigned(grid), kBlock, lds, st>>>(
-        c, dq, q_stride, ix->data.p, w.leaves.p, ix->range.p, ix->leaf.p, ix->goff.p, ix->gids.p, d_ids, d_dists,
+        c, dq, q_stride, ix->data.p, w.leaves.p, ix->snode.p, ix->leaf.p, ix->goff.p, ix->gids.p, d_ids, d_dists,
         d_ndist, w.ndist.p, w.qorder.p, int(nq), nullptr, 1, nullptr, w.overflow.p, w.nover.p, ix->work.p);
   }
   if (prof) hipEventRecord(ev[2], st);
@@ -625,12 +686,12 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_search_nnexp<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
                       int(lds2));
   if (table_in_lds) {
-    k_search_nnexp<true><<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->range.p, ix->leaf.p,
+    k_search_nnexp<true><<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->snode.p, ix->leaf.p,
                                                       ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
                                                       w.overflow.p, 0, w.nover.p, 0, w.gtab.p, w.overflow.p + nq,
                                                       w.nover.p + 1, ix->work.p);
   } else {
-    k_search_nnexp<true><<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->range.p, ix->leaf.p,
+    k_search_nnexp<true><<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->snode.p, ix->leaf.p,
                                                       ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
                                                       w.qorder.p, int(nq), nullptr, 0, w.gtab.p, w.overflow.p + nq,
                                                       w.nover.p + 1, ix->work.p);

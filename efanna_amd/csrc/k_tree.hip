@@ -460,6 +460,7 @@ int gpu_build_trees(efg_index* ix) {
   if (int rc = check_hip(hipStreamSynchronize(st), "tree build")) return rc;
   ix->T = T;
   ix->M = size_t(M);
+  ix->snode_valid = false;
   // DivDim written for leaves in the trees file: leaves the DFS phase made
   // (and roots) carry -1, leaves left behind by the BFS phase carry 0
   // (zero-initialised nodes, kdtreeub_index.hpp:1249, 1142-1143).
