@@ -68,6 +68,8 @@ struct SearchCfg {
   int kb;       // key buffer (seeds, then pool / new / merge areas)
   int logcap;   // visited-set capacity (log2)
   uint64_t seed;
+  int method;   // 0 nnexp, 1 kgraph
+  int sorted;   // method 1: seeds reduced to the best epochs * extend_to
 };
 
 // Device state of the NN-descent build (see k_graph.hip).

@@ -196,6 +196,77 @@ __device__ uint64_t block_select(const uint64_t* buf, int m, int R, uint32_t* hi
   return prefix;
 }
 
+// Wavefront-wide InsertIntoKnn (base_index.hpp:37-62) on LDS arrays sorted by
+// distance: insert after every entry with distance <= nd, duplicate check only
+// among equal distances (returns K + 1), entries [i, K) shift up one slot
+// (the slot K is written).  Executed by one full wavefront.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+__device__ int wave_insert(float* dist, uint32_t* ids, uint8_t* fl, int K, float nd, uint32_t nid) {
+  const int lane = threadIdx.x & 63;
+  int cnt = 0;
+  bool dup = false;
+  for (int b = 0; b < K; b += 64) {
+    const int j = b + lane;
+    const bool in = j < K;
+    const float dj = in ? dist[j] : 0.f;
+    cnt += __popcll(__ballot(in && dj <= nd));
+    dup = dup || (in && dj == nd && ids[j] == nid);
+  }
+  if (__ballot(dup)) return K + 1;
+  const int i = cnt;
+  wave_sync();
+  for (int b = ((K - 1) >> 6) << 6; b >= 0 && K > 0; b -= 64) {
+    const int j = b + lane;
+    const bool act = j >= i && j < K;
+    float dv = 0.f;
+    uint32_t iv = 0;
+    uint8_t fv = 0;
+    if (act) {
+      dv = dist[j];
+      iv = ids[j];
+      fv = fl[j];
+    }
+    wave_sync();
+    if (act) {
+      dist[j + 1] = dv;
+      ids[j + 1] = iv;
+      fl[j + 1] = fv;
+    }
+    wave_sync();
+  }
+  if (lane == 0) {
+    dist[i] = nd;
+    ids[i] = nid;
+    fl[i] = 1;
+  }
+  wave_sync();
+  return i;
+}
+
+// wavefront bitonic sort of n (power of two) keys in LDS
+__device__ void wave_bitonic_sort(uint64_t* b, int n) {
+  const int lane = threadIdx.x & 63;
+  for (int k = 2; k <= n; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = lane; t < n; t += 64) {
+        int u = t ^ j;
+        if (u > t) {
+          uint64_t x = b[t], y = b[u];
+          bool up = (t & k) == 0;
+          if ((x > y) == up) {
+            b[t] = y;
+            b[u] = x;
+          }
+        }
+      }
+      wave_sync();
+    }
+  }
+}
+
 }  // namespace
 
 // Dynamic LDS layout for the search kernel; also used by the host to size it.
@@ -209,10 +280,17 @@ __host__ __device__ inline size_t search_lds_bytes(const SearchCfg& c, bool glob
   b += a16(sizeof(int32_t) * (c.nleaf + 1)) + a16(sizeof(int32_t) * c.nleaf);
   b += a16(sizeof(uint32_t) * c.E) + a16(sizeof(int32_t) * (c.E + 1));
   b += a16(sizeof(uint32_t) * 256) + a16(sizeof(int) * 4) + a16(sizeof(int) * 8);
+  if (c.method == 1) {
+    const int kc = c.K + c.E + 1;
+    b += a16(4 * size_t(kc)) * 2 + a16(size_t(kc));            // knn: dist, id, flag
+    b += a16(4 * size_t(c.K + 1)) * 2 + a16(size_t(c.K + 1));  // results
+    b += a16(8 * size_t(next_pow2(c.E < 2 ? 2 : c.E)));        // epoch seed order
+    b += a16(8 * 64);                                           // new neighbours
+  }
   return b;
 }
 
-template <bool kGlobal>
+template <bool kGlobal, int kMethod>
 __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const float* __restrict__ Q, size_t q_stride,
                                                          const float* __restrict__ X,
                                                          const int32_t* __restrict__ leaves,
@@ -250,6 +328,34 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
     s.hist = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * 256));
     s.scan = reinterpret_cast<int*>(take(sizeof(int) * 4));
     s.misc = reinterpret_cast<int*>(take(sizeof(int) * 8));
+  }
+  // method 1 (kgraph) working arrays
+  float* kd = nullptr;
+  uint32_t* ki = nullptr;
+  uint8_t* kf = nullptr;
+  float* rd = nullptr;
+  uint32_t* ri = nullptr;
+  uint8_t* rf = nullptr;
+  uint64_t* eord = nullptr;
+  uint64_t* nwl = nullptr;
+  if (kMethod == 1) {
+    SearchCfg c0 = c;
+    c0.method = 0;
+    size_t o = search_lds_bytes(c0, kGlobal);
+    auto take = [&](size_t bytes) {
+      unsigned char* p = smem_raw + o;
+      o += align16(bytes);
+      return p;
+    };
+    const int kc = c.K + c.E + 1;
+    kd = reinterpret_cast<float*>(take(4 * size_t(kc)));
+    ki = reinterpret_cast<uint32_t*>(take(4 * size_t(kc)));
+    kf = reinterpret_cast<uint8_t*>(take(size_t(kc)));
+    rd = reinterpret_cast<float*>(take(4 * size_t(c.K + 1)));
+    ri = reinterpret_cast<uint32_t*>(take(4 * size_t(c.K + 1)));
+    rf = reinterpret_cast<uint8_t*>(take(size_t(c.K + 1)));
+    eord = reinterpret_cast<uint64_t*>(take(8 * size_t(next_pow2(c.E < 2 ? 2 : c.E))));
+    nwl = reinterpret_cast<uint64_t*>(take(8 * 64));
   }
   const int tid = threadIdx.x;
   const int cap = 1 << c.logcap;
@@ -391,7 +497,8 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
     distances(s.buf, 0, c.init_num);
     __syncthreads();
     // best R seeds: select the R-th key, gather the R keys, sort them
-    {
+    // (method 1 without sorting keeps the seed order, kdtreeub_index.hpp:567-570)
+    if (kMethod == 0 || c.sorted) {
       const uint64_t thr = (R < c.init_num) ? block_select(s.buf, c.init_num, R, s.hist, s.misc) : kKeyMax;
       uint64_t mine[kKPT];
       int nm = 0;
@@ -415,8 +522,129 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
     uint64_t ndq = uint64_t(c.init_num);
     bool ovf = false;
 
-    // ---- epochs ------------------------------------------------------------
-    for (int ep = 0; ep < c.epochs; ++ep) {
+    int kg_valid = 0;  // method 1: valid result entries
+    if (kMethod == 1) {
+      // ---- method 1: KGraph-style expansion (kdtreeub_index.hpp:572-648) -----
+      for (int i = tid; i < cap; i += kBlock) s.tab[i] = kEmpty;  // flags.reset() (572)
+      __syncthreads();
+      const uint64_t* pool = (c.sorted) ? P : s.buf;
+      if (tid < 64) {
+        const int lane = tid;
+        const int kcap = c.K + c.E + 1;
+        int rsize = 0;  // results.size() (spare included)
+        int ins = 0;
+        for (int it = 0; it < c.epochs && !ovf; ++it) {
+          // this epoch's seeds not yet flagged, ordered by (distance, pool rank)
+          int L = 0;
+          for (int b = 0; b < c.E; b += 64) {
+            const int j = b + lane;
+            bool take = false;
+            uint64_t e = 0;
+            if (j < c.E) {
+              e = pool[size_t(it) * c.E + j];
+              take = tab_insert(s.tab, c.logcap, key_id(e));
+            }
+            const uint64_t mk = __ballot(take);
+            if (take) {
+              const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+              eord[L + __popcll(mk & lt)] = (e & 0xffffffff00000000ull) | uint32_t(it * c.E + j);
+            }
+            L += __popcll(mk);
+            ins += __popcll(mk);
+          }
+          const int p2 = next_pow2(L < 2 ? 2 : L);
+          for (int t = L + lane; t < p2; t += 64) eord[t] = kKeyMax;
+          wave_sync();
+          wave_bitonic_sort(eord, p2);
+          for (int t = lane; t < L; t += 64) {
+            const uint64_t e = eord[t];
+            kd[t] = key_dist(e);
+            ki[t] = key_id(pool[uint32_t(e)]);
+            kf[t] = 1;
+          }
+          wave_sync();
+          int k = 0;
+          while (k < L) {
+            int nk = L;
+            if (kf[k]) {
+              wave_sync();
+              if (lane == 0) kf[k] = 0;
+              const uint32_t nd = ki[k];
+              const uint64_t g0 = goff[nd], deg = goff[nd + 1] - g0;
+              const int maxM = int(deg < uint64_t(c.E) ? deg : uint64_t(c.E));
+              for (int b = 0; b < maxM; b += 64) {
+                if (ins + 64 > limit) {
+                  ovf = true;
+                  break;
+                }
+                const int m = b + lane;
+                uint32_t id = m < maxM ? gids[g0 + m] : 0xffffffffu;
+                // first occurrence in this slice and not flagged
+                bool dupl = false;
+                for (int o2 = 0; o2 < 64 && b + o2 < maxM; ++o2) {
+                  const uint32_t other = uint32_t(__shfl(int(id), o2));
+                  dupl = dupl || (o2 < lane && other == id);
+                }
+                bool isnew = false;
+                if (m < maxM && !dupl) isnew = tab_insert(s.tab, c.logcap, id);
+                const uint64_t mk = __ballot(isnew);
+                const int nn = __popcll(mk);
+                ins += nn;
+                if (isnew) {
+                  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+                  nwl[__popcll(mk & lt)] = uint64_t(id);
+                }
+                wave_sync();
+                // distances, two lanes per row
+                for (int r = (lane >> 1); r < nn; r += 32) {
+                  const uint32_t rid = uint32_t(nwl[r]);
+                  const float dd = l2_pair_lanes(s.q, X + size_t(rid) * c.D, c.D, lane & 1);
+                  if (lane & 1) nwl[r] = make_key(dd, rid);
+                }
+                wave_sync();
+                ndq += uint64_t(nn);
+                for (int r = 0; r < nn; ++r) {  // inserts in neighbour order
+                  const uint64_t e = nwl[r];
+                  const int rr = wave_insert(kd, ki, kf, L, key_dist(e), key_id(e));
+                  if (L + 1 < kcap) ++L;
+                  if (rr < nk) nk = rr;
+                }
+              }
+              if (ovf) break;
+            }
+            if (nk <= k) k = nk;
+            else ++k;
+          }
+          if (ovf) break;
+          if (L > c.K) L = c.K;
+          if (rsize == 0) {  // results = first L of knn (+ spare)
+            for (int t = lane; t < L; t += 64) {
+              rd[t] = kd[t];
+              ri[t] = ki[t];
+              rf[t] = kf[t];
+            }
+            rsize = L + 1;
+            wave_sync();
+          } else {
+            for (int l = 0; l < L; ++l) {
+              const int rr = wave_insert(rd, ri, rf, rsize - 1, kd[l], ki[l]);
+              if (rr < rsize && rsize < c.K + 1) ++rsize;
+            }
+          }
+        }
+        if (lane == 0) {
+          s.misc[1] = ovf ? 1 : 0;
+          s.misc[2] = rsize > 0 ? rsize - 1 : 0;
+          s.misc[3] = int(ndq - uint64_t(c.init_num));
+        }
+      }
+      __syncthreads();
+      ovf = s.misc[1] != 0;
+      kg_valid = s.misc[2];
+      ndq = uint64_t(c.init_num) + uint64_t(s.misc[3]);
+    }
+    // ---- epochs (method 0) ---------------------------------------------------
+    for (int ep = 0; kMethod == 0 && ep < c.epochs; ++ep) {
       // expansion list: unexpanded ids among the first extend_to (448-451)
       int nexp = 0;
       for (int b0 = 0; b0 < c.E; b0 += kBlock) {
@@ -497,9 +725,15 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       if (tid == 0) overflow[atomicAdd(noverflow, 1)] = qi;
     } else {
       for (int j = tid; j < c.K; j += kBlock) {
-        uint64_t k = P[j];
-        out_ids[size_t(qi) * c.K + j] = int32_t(key_id(k));
-        if (out_d) out_d[size_t(qi) * c.K + j] = key_dist(k);
+        if (kMethod == 0) {
+          uint64_t k = P[j];
+          out_ids[size_t(qi) * c.K + j] = int32_t(key_id(k));
+          if (out_d) out_d[size_t(qi) * c.K + j] = key_dist(k);
+        } else {
+          const bool ok = j < kg_valid;
+          out_ids[size_t(qi) * c.K + j] = ok ? int32_t(ri[j]) : -1;
+          if (out_d) out_d[size_t(qi) * c.K + j] = ok ? rd[j] : 3.402823466e38f;
+        }
       }
       if (tid == 0) {
         if (ndist_q) ndist_q[qi] = ndq;
@@ -581,11 +815,13 @@ int build_search_layout(efg_index* ix) {
   return EFG_OK;
 }
 
-// knnSearch, method 0, on device buffers (kdtreeub_index.hpp:363-494).
+// knnSearch on device buffers: method 0 (getNeighbors_nnexp,
+// kdtreeub_index.hpp:363-494) or method 1 (getNeighbors_kgraph, 496-654).
 int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int32_t* d_ids, float* d_dists,
                   uint64_t* d_ndist, hipStream_t st) {
   if (!ix->s_set) return fail(EFG_ERR_STATE, "search parameters not set (setSearchParams)");
-  if (ix->s_method != 0) return fail(EFG_ERR_METHOD, "no such searching method");
+  const int method = ix->s_method;
+  if (method != 0 && method != 1) return fail(EFG_ERR_METHOD, "no such searching method");
   if (ix->T == 0) return fail(EFG_ERR_STATE, "no trees: build or load trees first");
   if (ix->nnz == 0) return fail(EFG_ERR_STATE, "no graph: build or load a graph first");
   if (ix->s_trees <= 0) return fail(EFG_ERR_ARG, "search tree number must be positive");
@@ -593,18 +829,32 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
   if (nq == 0) return EFG_OK;
   if (nq > (size_t(1) << 31)) return fail(EFG_ERR_ARG, "too many queries");
   if (q_stride < ix->d) return fail(EFG_ERR_ARG, "query stride below the dimension");
-  const int init = ix->s_init, E = ix->s_extend, epochs = ix->s_epochs;
+  int init = ix->s_init;
+  const int E = ix->s_extend, epochs = ix->s_epochs;
   if (init <= 0 || E < 0 || epochs < 0) return fail(EFG_ERR_ARG, "invalid search parameters");
+  const int lsize = init * 2 / (5 * ix->s_trees) + 1;  // before method 1 raises init_num (507)
+  int sorted = 1, R = E > k ? E : k;
+  if (method == 1) {
+    if (E <= 0) return fail(EFG_ERR_ARG, "method 1 needs extend_to > 0");
+    const int pool_size = epochs * E;  // kdtreeub_index.hpp:510-514: init_num raised for good
+    if (pool_size >= init) {
+      init = pool_size;
+      ix->s_init = pool_size;
+      sorted = 0;
+    }
+    R = pool_size;
+    if (k <= 0 || k > 4096) return fail(EFG_ERR_ARG, "invalid k");
+  } else if (k <= 0 || R > init) {
+    return fail(EFG_ERR_ARG, "k must be positive and at most the pool size");
+  }
   if (size_t(init) > ix->n) return fail(EFG_ERR_ARG, "pool size larger than the data set");
   if (init > kKPT * kBlock) return fail(EFG_ERR_ARG, "pool size above 4096 is not supported");
-  const int R = E > k ? E : k;
-  if (k <= 0 || R > init) return fail(EFG_ERR_ARG, "k must be positive and at most the pool size");
   SearchCfg c{};
   c.n = int(ix->n);
   c.d = int(ix->d);
   c.D = int(ix->D);
   c.T = ix->s_trees;
-  c.lsize = init * 2 / (5 * c.T) + 1;
+  c.lsize = lsize;
   c.nleaf = c.T * c.lsize;
   c.init_num = init;
   c.E = E;
@@ -612,6 +862,8 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
   c.K = k;
   c.R = R;
   c.Rp = next_pow2(R < 2 ? 2 : R);
+  c.method = method;
+  c.sorted = sorted;
   c.kb = std::max(init, 2 * c.Rp + kNB);
   c.logcap = ceil_log2(size_t(init + 512) * 4 / 3 + 1);
   if (c.logcap < 10) c.logcap = 10;
@@ -666,32 +918,33 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
     if (per_cu < 1) per_cu = 1;
     size_t grid = size_t(ncu) * per_cu;
     if (grid > nq) grid = (nq + 7) & ~size_t(7);
-    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_search_nnexp<false>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-    k_search_nnexp<false><<<unsigned(grid), kBlock, lds, st>>>(
-        c, dq, q_stride, ix->data.p, w.leaves.p, ix->snode.p, ix->leaf.p, ix->goff.p, ix->gids.p, d_ids, d_dists,
-        d_ndist, w.ndist.p, w.qorder.p, int(nq), nullptr, 1, nullptr, w.overflow.p, w.nover.p, ix->work.p);
+    auto kern = method == 0 ? &k_search_nnexp<false, 0> : &k_search_nnexp<false, 1>;
+    hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    kern<<<unsigned(grid), kBlock, lds, st>>>(c, dq, q_stride, ix->data.p, w.leaves.p, ix->snode.p, ix->leaf.p,
+                                              ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p, w.qorder.p,
+                                              int(nq), nullptr, 1, nullptr, w.overflow.p, w.nover.p, ix->work.p);
   }
   if (prof) hipEventRecord(ev[2], st);
   // queries whose visited set outgrew the LDS table (or all of them when the
   // table never fits) run again with a per-workgroup table in global memory
   SearchCfg c2 = c;
-  size_t worst = size_t(init) + size_t(epochs) * size_t(E) * size_t(ix->max_deg > 0 ? ix->max_deg : 1) + 512;
+  size_t worst = size_t(init) + size_t(epochs) * size_t(E + k + 1) * size_t(ix->max_deg > 0 ? ix->max_deg : 1) + 512;
+  if (worst > ix->n + 512) worst = ix->n + 512;
   c2.logcap = ceil_log2(worst * 4 / 3 + 1);
   if (c2.logcap < c.logcap) c2.logcap = c.logcap;
   if (c2.logcap > 26) c2.logcap = 26;
   size_t lds2 = search_lds_bytes(c2, true);
   unsigned grid2 = 128;
   if (!w.gtab.ensure(size_t(grid2) << c2.logcap)) return fail(EFG_ERR_MEMORY, "out of device memory (search table)");
-  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_search_nnexp<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                      int(lds2));
+  auto kern2 = method == 0 ? &k_search_nnexp<true, 0> : &k_search_nnexp<true, 1>;
+  hipFuncSetAttribute(reinterpret_cast<const void*>(kern2), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2));
   if (table_in_lds) {
-    k_search_nnexp<true><<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->snode.p, ix->leaf.p,
+    kern2<<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->snode.p, ix->leaf.p,
                                                       ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
                                                       w.overflow.p, 0, w.nover.p, 0, w.gtab.p, w.overflow.p + nq,
                                                       w.nover.p + 1, ix->work.p);
   } else {
-    k_search_nnexp<true><<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->snode.p, ix->leaf.p,
+    kern2<<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->snode.p, ix->leaf.p,
                                                       ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
                                                       w.qorder.p, int(nq), nullptr, 0, w.gtab.p, w.overflow.p + nq,
                                                       w.nover.p + 1, ix->work.p);

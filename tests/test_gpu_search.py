@@ -146,3 +146,34 @@ def test_exact_knn_float_data_matches_oracle(cuda):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(ids.cpu().numpy(), oids)
     np.testing.assert_array_equal(dd.cpu().numpy().view(np.uint32), od.view(np.uint32))
+
+
+@pytest.mark.parametrize("epochs,pool,extend,trees,k", [
+    (4, 120, 20, 4, 10),    # the 10K x 128 CPU configuration, method 1
+    (3, 60, 30, 4, 10),     # epochs * extend >= pool: unsorted seeds, pool raised to 90
+    (2, 200, 10, 8, 20),    # k > extend_to
+    (5, 300, 40, 8, 5),
+])
+def test_search_kgraph_matches_oracle(cuda, epochs, pool, extend, trees, k):
+    base, q, oix = helpers.oracle_index("uniform")
+    gix = _gpu_index(base, oix.trees(), oix.graph())
+    gix.setSearchParams(epochs, pool, extend, trees, -1, 1)
+    gix.knnSearch(k, q)
+    gids, gd = gix.results()
+    oids, od, ond = oix.search(q, k, epochs, pool, extend, trees, 1)
+    np.testing.assert_array_equal(gids, oids)
+    np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32))
+    assert gix.counters()["search_dist"] == int(ond.sum())
+    assert gix.getSearchParams()["init_num"] == oix.init_num_state
+
+
+def test_search_kgraph_float_mixture(cuda):
+    base, q, oix = helpers.oracle_index("mixture", n=20_000, nq=300, centres=40, sigma=0.05, hi=1.0,
+                                        rounded=False, d=100, ml=4)
+    gix = _gpu_index(base, oix.trees(), oix.graph())
+    gix.setSearchParams(4, 400, 60, 8, -1, 1)
+    gix.knnSearch(10, q)
+    gids, gd = gix.results()
+    oids, od, _ = oix.search(q, 10, 4, 400, 60, 8, 1)
+    np.testing.assert_array_equal(gids, oids)
+    np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32))
