@@ -123,16 +123,53 @@ def main() -> int:
     ms_per_step = wall * 1e3 / args.steps
     qps = nq / (ms_per_step / 1e3)
 
+    res_plain = ids.cpu().numpy()
+
+    # the precision form: BF16 rows, FP32 re-rank of the best 2k
+    gix.setSearchPrecision(1)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    bf_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    res_bf = ids.cpu().numpy()
+    gix.setSearchPrecision(0)
+
+    # method 1 (kgraph) at the same parameters, for reference
+    gix.setSearchParams(s_epochs, s_pool, s_extend, s_trees, -1, 1)
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m1_steps = max(1, args.steps // 4)
+    for _ in range(m1_steps):
+        gix.setSearchParams(s_epochs, s_pool, s_extend, s_trees, -1, 1)
+        step()
+    torch.cuda.synchronize()
+    m1_ms = (time.perf_counter() - t0) * 1e3 / m1_steps
+    res_m1 = ids.cpu().numpy()
+    gix.setSearchParams(s_epochs, s_pool, s_extend, s_trees, -1, 0)
+
     # recall@10 against exact kNN on the GPU (ties to the lower id)
     dptr, stride = gix.device_data()
     gt = torch.empty((nq, k), dtype=torch.int32, device=dev)
     exact_knn_device(dptr, n, d, stride, qd.data_ptr(), nq, d, k, False, gt.data_ptr(), None,
                      stream.cuda_stream)
     torch.cuda.synchronize()
-    res = ids.cpu().numpy()
+    res = res_plain
     gtn = gt.cpu().numpy()
-    hits = sum(len(set(a.tolist()) & set(b.tolist())) for a, b in zip(res, gtn))
-    recall = hits / (nq * k)
+
+    def rec(r):
+        return sum(len(set(a.tolist()) & set(b.tolist())) for a, b in zip(r, gtn)) / (nq * k)
+
+    recall = rec(res)
+    forms = {"bf16_rows": {"ms_per_step": round(bf_ms, 4), "qps": round(nq / (bf_ms / 1e3), 1),
+                           "speedup_vs_plain": round(ms_per_step / bf_ms, 3), "recall_at_10": round(rec(res_bf), 5),
+                           "identical_to_plain": bool(np.array_equal(res_bf, res))},
+             "method1": {"ms_per_step": round(m1_ms, 4), "qps": round(nq / (m1_ms / 1e3), 1),
+                         "recall_at_10": round(rec(res_m1), 5)}}
 
     # roofline of the search kernel: algorithmic bytes per launch (per query:
     # 4d per distance, 4 per seed id read, 4*K per expanded node)
@@ -188,6 +225,7 @@ def main() -> int:
                    "search": {"trees": s_trees, "epochs": s_epochs, "pool": s_pool, "extend": s_extend},
                    "form": "plain fp32"},
         "recall_at_10": round(recall, 5),
+        "forms": forms,
         "build_s": round(build_s, 3),
         "build_phase_ms": {"trees": round(phase_ms[0], 1), "merge_init": round(phase_ms[1], 1),
                            "join": round(phase_ms[2], 1), "update": round(phase_ms[3], 1)},

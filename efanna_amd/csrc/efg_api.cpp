@@ -409,6 +409,13 @@ int efg_set_search_params(efg_index* ix, int epochs, int init_num, int extend_to
   return EFG_OK;
 }
 
+int efg_set_search_precision(efg_index* ix, int precision) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (precision != 0 && precision != 1) return fail(EFG_ERR_ARG, "precision must be 0 (FP32) or 1 (BF16)");
+  ix->s_precision = precision;
+  return EFG_OK;
+}
+
 int efg_get_search_params(const efg_index* ix, int* epochs, int* init_num, int* extend_to, int* search_trees,
                           int* search_method) {
   if (!ix) return fail(EFG_ERR_ARG, "null index");

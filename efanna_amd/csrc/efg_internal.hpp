@@ -108,6 +108,8 @@ struct efg_index {
   bool s_set = false;
   hipStream_t stream = nullptr;
   efg::DevBuf<float> data;  // n x D, zero padded
+  efg::DevBuf<uint16_t> data_bf16;  // BF16 copy, lane-split rows (precision form)
+  int s_precision = 0;              // 0 FP32 rows, 1 BF16 rows + FP32 re-rank
   // trees
   int T = 0;
   size_t M = 0;

@@ -267,7 +267,48 @@ __device__ void wave_bitonic_sort(uint64_t* b, int n) {
   }
 }
 
+// Squared L2 of an FP32 query (LDS) against a BF16 row stored lane-split:
+// for lane h of a pair the row holds, chunk by chunk, the four elements
+// 8m + 4h .. 8m + 4h + 3 (so each lane reads its part contiguously).  Same
+// accumulator chains and order as l2_pair_lanes, on the widened values.
+__device__ inline float l2_pair_lanes_bf16(const float* __restrict__ q, const uint16_t* __restrict__ row, int D,
+                                           int h) {
+  const int DR = D & 31, DD = D - DR;
+  const uint16_t* part = row + size_t(h) * (D / 2);  // this lane's D/2 values
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto step = [&](int m) {  // chunk m: elements 8m + 4h + 0..3
+    const uint2 raw = *reinterpret_cast<const uint2*>(part + 4 * m);
+    const float4 x = *reinterpret_cast<const float4*>(q + 8 * m + 4 * h);
+    const float y0 = __uint_as_float(raw.x << 16), y1 = __uint_as_float(raw.x & 0xffff0000u);
+    const float y2 = __uint_as_float(raw.y << 16), y3 = __uint_as_float(raw.y & 0xffff0000u);
+    float t0 = x.x - y0, t1 = x.y - y1, t2 = x.z - y2, t3 = x.w - y3;
+    acc.x = __builtin_fmaf(t0, t0, acc.x);
+    acc.y = __builtin_fmaf(t1, t1, acc.y);
+    acc.z = __builtin_fmaf(t2, t2, acc.z);
+    acc.w = __builtin_fmaf(t3, t3, acc.w);
+  };
+  for (int m = DD / 8; m < D / 8; ++m) step(m);
+#pragma unroll 8
+  for (int m = 0; m < DD / 8; ++m) step(m);
+  float s0 = ((acc.x + acc.y) + acc.z) + acc.w;
+  float p = __shfl_xor(s0, 1);
+  return (((p + acc.x) + acc.y) + acc.z) + acc.w;
+}
+
 }  // namespace
+
+// BF16 copy of the base rows in the lane-split layout of l2_pair_lanes_bf16
+// (round to nearest even).
+__global__ void k_pack_bf16(const float* __restrict__ X, int D, size_t n, uint16_t* __restrict__ out) {
+  const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= n * size_t(D)) return;
+  const size_t r = gid / D;
+  const int e = int(gid % D);
+  const int m = e / 8, w = e % 8, h = w / 4, i = w % 4;
+  const uint32_t u = __float_as_uint(X[gid]);
+  const uint32_t rounded = u + 0x7fffu + ((u >> 16) & 1u);
+  out[r * D + size_t(h) * (D / 2) + 4 * m + i] = uint16_t(rounded >> 16);
+}
 
 // Dynamic LDS layout for the search kernel; also used by the host to size it.
 __host__ __device__ inline size_t search_lds_bytes(const SearchCfg& c, bool global_table) {
@@ -290,9 +331,9 @@ __host__ __device__ inline size_t search_lds_bytes(const SearchCfg& c, bool glob
   return b;
 }
 
-template <bool kGlobal, int kMethod>
+template <bool kGlobal, int kMethod, bool kBf16>
 __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const float* __restrict__ Q, size_t q_stride,
-                                                         const float* __restrict__ X,
+                                                         const float* __restrict__ X, const uint16_t* __restrict__ Xb,
                                                          const int32_t* __restrict__ leaves,
                                                          const SNode* __restrict__ sn,
                                                          const uint32_t* __restrict__ leaf_lists,
@@ -370,7 +411,8 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
   auto distances = [&](uint64_t* a, int from, int to) {
     for (int i = from + pr; i < to; i += kBlock / 2) {
       uint32_t id = key_id(a[i]);
-      float dd = l2_pair_lanes(s.q, X + size_t(id) * c.D, c.D, h);
+      float dd = kBf16 ? l2_pair_lanes_bf16(s.q, Xb + size_t(id) * c.D, c.D, h)
+                       : l2_pair_lanes(s.q, X + size_t(id) * c.D, c.D, h);
       if (h) a[i] = make_key(dd, id);
     }
   };
@@ -721,6 +763,22 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       }
       if (ovf) break;
     }
+    if (kBf16 && kMethod == 0 && !ovf) {
+      // re-rank the best min(2K, R) by direct FP32 difference
+      const int rr = (2 * c.K < c.R) ? 2 * c.K : c.R;
+      const int p2 = next_pow2(rr < 2 ? 2 : rr);
+      for (int i = pr; i < rr; i += kBlock / 2) {
+        const uint32_t id = key_id(P[i]);
+        const float dd = l2_pair_lanes(s.q, X + size_t(id) * c.D, c.D, h);
+        if (h) O[i] = make_key(dd, id);
+      }
+      for (int i = rr + tid; i < p2; i += kBlock) O[i] = kKeyMax;
+      __syncthreads();
+      block_bitonic_sort(O, p2);
+      for (int i = tid; i < rr; i += kBlock) P[i] = O[i];
+      __syncthreads();
+      ndq += uint64_t(rr);
+    }
     if (ovf) {
       if (tid == 0) overflow[atomicAdd(noverflow, 1)] = qi;
     } else {
@@ -876,6 +934,12 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
     if (int rc = check_hip(hipStreamSynchronize(st), "sync")) return rc;
     if (int rc = build_search_layout(ix)) return rc;
   }
+  const bool bf16 = method == 0 && ix->s_precision == 1;
+  if (bf16 && !ix->data_bf16.p) {
+    if (!ix->data_bf16.alloc(ix->n * ix->D)) return fail(EFG_ERR_MEMORY, "out of device memory (bf16 rows)");
+    const size_t tot = ix->n * ix->D;
+    k_pack_bf16<<<unsigned((tot + 255) / 256), 256, 0, st>>>(ix->data.p, int(ix->D), ix->n, ix->data_bf16.p);
+  }
   efg::Workspace& w = ix->ws;
   const int nb_q = 4096;
   if (!w.leaves.ensure(nq * size_t(c.nleaf)) || !w.overflow.ensure(2 * nq) || !w.nover.ensure(2) ||
@@ -918,9 +982,10 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
     if (per_cu < 1) per_cu = 1;
     size_t grid = size_t(ncu) * per_cu;
     if (grid > nq) grid = (nq + 7) & ~size_t(7);
-    auto kern = method == 0 ? &k_search_nnexp<false, 0> : &k_search_nnexp<false, 1>;
+    auto kern = method == 1 ? &k_search_nnexp<false, 1, false>
+                            : (bf16 ? &k_search_nnexp<false, 0, true> : &k_search_nnexp<false, 0, false>);
     hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-    kern<<<unsigned(grid), kBlock, lds, st>>>(c, dq, q_stride, ix->data.p, w.leaves.p, ix->snode.p, ix->leaf.p,
+    kern<<<unsigned(grid), kBlock, lds, st>>>(c, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, ix->snode.p, ix->leaf.p,
                                               ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p, w.qorder.p,
                                               int(nq), nullptr, 1, nullptr, w.overflow.p, w.nover.p, ix->work.p);
   }
@@ -936,15 +1001,16 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
   size_t lds2 = search_lds_bytes(c2, true);
   unsigned grid2 = 128;
   if (!w.gtab.ensure(size_t(grid2) << c2.logcap)) return fail(EFG_ERR_MEMORY, "out of device memory (search table)");
-  auto kern2 = method == 0 ? &k_search_nnexp<true, 0> : &k_search_nnexp<true, 1>;
+  auto kern2 = method == 1 ? &k_search_nnexp<true, 1, false>
+                           : (bf16 ? &k_search_nnexp<true, 0, true> : &k_search_nnexp<true, 0, false>);
   hipFuncSetAttribute(reinterpret_cast<const void*>(kern2), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2));
   if (table_in_lds) {
-    kern2<<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->snode.p, ix->leaf.p,
+    kern2<<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, ix->snode.p, ix->leaf.p,
                                                       ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
                                                       w.overflow.p, 0, w.nover.p, 0, w.gtab.p, w.overflow.p + nq,
                                                       w.nover.p + 1, ix->work.p);
   } else {
-    kern2<<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, w.leaves.p, ix->snode.p, ix->leaf.p,
+    kern2<<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, ix->snode.p, ix->leaf.p,
                                                       ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
                                                       w.qorder.p, int(nq), nullptr, 0, w.gtab.p, w.overflow.p + nq,
                                                       w.nover.p + 1, ix->work.p);

@@ -147,6 +147,10 @@ class FIndex:
         check(lib().efg_set_search_params(self.h, epochs, init_num, extend_to, search_trees, search_lv,
                                           search_method))
 
+    def setSearchPrecision(self, precision: int):
+        """0 = FP32 rows; 1 = BF16 rows with FP32 re-rank of the best 2k."""
+        check(lib().efg_set_search_precision(self.h, int(precision)))
+
     def getSearchParams(self):
         v = [ctypes.c_int() for _ in range(5)]
         check(lib().efg_get_search_params(self.h, *[ctypes.byref(x) for x in v]))

@@ -98,6 +98,14 @@ int efg_get_results(const efg_index* ix, size_t* nq, int* k, int32_t* ids, float
 int efg_knn_search_device(efg_index* ix, int k, const float* d_queries, size_t nq, size_t q_stride, int32_t* d_ids,
                           float* d_dists, uint64_t* d_ndist, void* stream);
 
+/* Search precision (no counterpart in the reference; a form beside the plain
+ * FP32 search): 0 = FP32 rows (default, bit-exact with the reference's
+ * arithmetic); 1 = BF16 copy of the rows for the pool and expansion
+ * distances, the best min(2k, pool) re-ranked by direct FP32 difference.  On
+ * integer data up to 256 in magnitude BF16 is exact and both give the same
+ * results.  Applies to method 0. */
+int efg_set_search_precision(efg_index* ix, int precision);
+
 /* getGraphSize / getGraphRow (base_index.hpp:494-501). */
 int efg_graph_size(const efg_index* ix, size_t* n);
 int efg_graph_row(const efg_index* ix, size_t row, uint32_t* out, size_t cap, size_t* len);

@@ -177,3 +177,35 @@ def test_search_kgraph_float_mixture(cuda):
     oids, od, _ = oix.search(q, 10, 4, 400, 60, 8, 1)
     np.testing.assert_array_equal(gids, oids)
     np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32))
+
+
+def test_search_bf16_rows_identical_on_integer_data(cuda):
+    # BF16 holds 0..255 exactly: the precision form must return the plain results
+    base, q, oix = helpers.oracle_index("mixture", n=40_000, nq=200, centres=40, trees=16, ml=8)
+    gix = _gpu_index(base, oix.trees(), oix.graph())
+    gix.setSearchParams(4, 1200, 200, 16, -1, 0)
+    gix.knnSearch(10, q)
+    plain_ids, plain_d = gix.results()
+    gix.setSearchPrecision(1)
+    gix.knnSearch(10, q)
+    ids, d = gix.results()
+    np.testing.assert_array_equal(ids, plain_ids)
+    np.testing.assert_array_equal(d.view(np.uint32), plain_d.view(np.uint32))
+
+
+def test_search_bf16_rows_float_data(cuda):
+    base, q, oix = helpers.oracle_index("mixture", n=20_000, nq=300, centres=40, sigma=0.05, hi=1.0,
+                                        rounded=False, d=100, ml=4)
+    gix = _gpu_index(base, oix.trees(), oix.graph())
+    gix.setSearchParams(4, 400, 60, 8, -1, 0)
+    gix.setSearchPrecision(1)
+    gix.knnSearch(10, q)
+    ids, d = gix.results()
+    # returned distances are the exact FP32 distances of the returned ids, ascending
+    for i in range(0, q.shape[0], 7):
+        for j in range(10):
+            assert d[i, j] == np.float32(oracle.l2(q[i], base[ids[i, j]]))
+        assert (np.diff(d[i]) >= 0).all()
+    gt, _ = oracle.exact_knn(base, q, 10)
+    oids, _, _ = oix.search(q, 10, 4, 400, 60, 8, 0)
+    assert helpers.recall(ids, gt, 10) >= helpers.recall(oids, gt, 10) - 0.01
