@@ -38,6 +38,7 @@ namespace efg {
 namespace {
 
 constexpr uint32_t kFirst = 0x80000000u;  // staging src flag: first endpoint of the pair
+constexpr int kJoinMaxBR = 32;            // rows per join block at most
 
 __device__ inline float l2_thread(const float* __restrict__ a, const float* __restrict__ b, int D) {
   // single-thread form of the reference's AVX order
@@ -74,10 +75,18 @@ __device__ inline float l2_thread(const float* __restrict__ a, const float* __re
   return s;
 }
 
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // ---------------------------------------------------------------------------
 // merge
 // ---------------------------------------------------------------------------
-__global__ void k_node_depth(const int32_t* __restrict__ parent, int M, int8_t* __restrict__ depth) {
+// Depth and tree of every node (walk to the root); the largest depth goes to
+// *max_depth.
+__global__ void k_node_depth(const int32_t* __restrict__ parent, int M, const int32_t* __restrict__ roots, int T,
+                             int8_t* __restrict__ depth, int32_t* __restrict__ tree_of, int* __restrict__ max_depth) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
   int dd = 0, x = i;
@@ -86,13 +95,18 @@ __global__ void k_node_depth(const int32_t* __restrict__ parent, int M, int8_t* 
     ++dd;
   }
   depth[i] = int8_t(dd > 127 ? 127 : dd);
+  int t = 0;
+  while (t < T - 1 && roots[t] != x) ++t;
+  tree_of[i] = t;
+  atomicMax(max_depth, dd);
 }
 
 // leaf_at[t][pos] = leaf holding position pos; error flag for leaves shallower
 // than ml in the first TB trees (getMergeLevelNodeList, 1000-1011).
 __global__ void k_leaf_at(const int4* __restrict__ node4, const uint2* __restrict__ range,
                           const int8_t* __restrict__ depth, const int32_t* __restrict__ tree_of, int M, uint32_t n,
-                          int TB, int ml, int32_t* __restrict__ leaf_at, int* __restrict__ err_depth) {
+                          int TB, int ml, int32_t* __restrict__ leaf_at, int* __restrict__ err_depth,
+                          int* __restrict__ max_leaf) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
   if (node4[i].z >= 0) return;
@@ -100,6 +114,7 @@ __global__ void k_leaf_at(const int4* __restrict__ node4, const uint2* __restric
   uint2 r = range[i];
   for (uint32_t p = r.x; p < r.y; ++p) leaf_at[size_t(t) * n + p] = i;
   if (t < TB && depth[i] < ml) atomicMin(err_depth, int(depth[i]));
+  if (t < TB) atomicMax(max_leaf, int(r.y - r.x));
 }
 
 __device__ inline int32_t descend(const int4* __restrict__ node4, int32_t x, const float* __restrict__ q) {
@@ -321,6 +336,124 @@ __global__ void __launch_bounds__(kBlock) k_replay(const float* __restrict__ X, 
   }
 }
 
+// Merge distances as leaf tiles: one wavefront per leaf bucket stages the
+// leaf's member rows and, PB at a time, its probers' rows in LDS (stride
+// D + 4) and writes dist[record * LM + member] for every record of the
+// bucket (sorted order), and pslot[pos * nlev + depth(X) - ml] = (record,
+// leaf) for the prober side of the replay.  Squared L2 is symmetric bit for
+// bit, so one value serves both endpoints.
+__global__ void k_leaf_dist(const float* __restrict__ X, int D, const uint2* __restrict__ range,
+                            const uint32_t* __restrict__ leaf_t, const int32_t* __restrict__ boff, int M,
+                            const uint64_t* __restrict__ rec, int LM, int PB, int ml, int nlev,
+                            float* __restrict__ dist, uint2* __restrict__ pslot) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ uint32_t pids[4][8];  // prober ids of the batch, per wavefront (PB <= 8, at most 4 wavefronts)
+  const int RS = D + 4, D4 = D >> 2;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  float* rows_m = reinterpret_cast<float*>(smem_raw) + size_t(w) * (LM + PB) * RS;
+  float* rows_p = rows_m + size_t(LM) * RS;
+  for (int l = blockIdx.x * nw + w; l < M; l += gridDim.x * nw) {
+    const int b0 = boff[l], R = boff[l + 1] - b0;
+    if (R == 0) continue;
+    const uint2 r = range[l];
+    const int m = int(r.y - r.x);
+    for (int i = lane; i < m * D4; i += 64) {
+      const int rr = i / D4, c = i - rr * D4;
+      *reinterpret_cast<float4*>(rows_m + size_t(rr) * RS + 4 * c) =
+          reinterpret_cast<const float4*>(X + size_t(leaf_t[r.x + rr]) * D)[c];
+    }
+    for (int rb = 0; rb < R; rb += PB) {
+      const int nb = min(PB, R - rb);
+      if (lane < nb) {
+        const uint64_t v = rec[b0 + rb + lane];
+        const uint32_t pos = uint32_t(v & 0xffffffffu);
+        const int dep = 63 - int(v >> 32);
+        pids[w][lane] = leaf_t[pos];
+        pslot[size_t(pos) * nlev + (dep - ml)] = make_uint2(uint32_t(b0 + rb + lane), uint32_t(l));
+      }
+      wave_sync();
+      for (int i = lane; i < nb * D4; i += 64) {
+        const int rr = i / D4, c = i - rr * D4;
+        *reinterpret_cast<float4*>(rows_p + size_t(rr) * RS + 4 * c) =
+            reinterpret_cast<const float4*>(X + size_t(pids[w][rr]) * D)[c];
+      }
+      wave_sync();
+      for (int t = lane; t < nb * m; t += 64) {
+        const int b = t / m, j = t - b * m;
+        dist[size_t(b0 + rb + b) * LM + j] = l2_thread(rows_m + size_t(j) * RS, rows_p + size_t(b) * RS, D);
+      }
+      wave_sync();
+    }
+  }
+}
+
+// The replay of k_replay over the distances of k_leaf_dist.
+__global__ void __launch_bounds__(kBlock) k_replay_dist(const int4* __restrict__ node4, const int32_t* __restrict__ parent,
+                                                        const uint2* __restrict__ range, const int8_t* __restrict__ depth,
+                                                        const uint32_t* __restrict__ leaf_t,
+                                                        const int32_t* __restrict__ leaf_at_t, uint32_t n, int ml, int S,
+                                                        const int32_t* __restrict__ boff,
+                                                        const uint64_t* __restrict__ rec, const float* __restrict__ dist,
+                                                        const uint2* __restrict__ pslot, int LM, int nlev,
+                                                        uint64_t* __restrict__ hkey, uint32_t* __restrict__ hsize,
+                                                        uint32_t* __restrict__ extra, uint32_t* __restrict__ ecount,
+                                                        unsigned long long* __restrict__ ndist) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  uint64_t* h = reinterpret_cast<uint64_t*>(smem_raw) + size_t(threadIdx.x) * S;
+  uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long nd_local = 0;
+  if (pos < n) {
+    const uint32_t p = leaf_t[pos];
+    int sz = int(hsize[p]);
+    for (int j = 0; j < sz; ++j) h[j] = hkey[size_t(p) * S + j];
+    uint32_t ecnt = ecount[p];
+    uint32_t* ex = extra + size_t(p) * 2 * S;
+    const int32_t lf = leaf_at_t[pos];
+    const int j0 = int(pos - range[lf].x);
+    int rb = boff[lf];
+    const int re = boff[lf + 1];
+    int32_t child = lf;
+    int32_t x = parent[child];
+    while (x >= 0 && depth[x] >= ml) {
+      const int4 nd = node4[x];
+      const uint2 rl = range[nd.z], rr = range[nd.w];
+      const uint32_t numL = rl.y - rl.x, numR = rr.y - rr.x;
+      const int32_t small = (numL < numR) ? nd.z : nd.w;
+      if (child == small) {
+        const uint2 e = pslot[size_t(pos) * nlev + (depth[x] - ml)];
+        const uint2 r = range[e.y];
+        const float* dr = dist + size_t(e.x) * LM;
+        for (uint32_t i = r.x; i < r.y; ++i) {
+          ++nd_local;
+          heap_offer(h, sz, S, ex, ecnt, leaf_t[i], dr[i - r.x]);
+        }
+      } else {
+        const uint64_t want = uint64_t(63 - depth[x]);
+        while (rb < re && (rec[rb] >> 32) < want) ++rb;
+        while (rb < re && (rec[rb] >> 32) == want) {
+          heap_offer(h, sz, S, ex, ecnt, leaf_t[uint32_t(rec[rb] & 0xffffffffu)], dist[size_t(rb) * LM + j0]);
+          ++rb;
+        }
+      }
+      child = x;
+      x = parent[x];
+    }
+    for (int j = 0; j < sz; ++j) hkey[size_t(p) * S + j] = h[j];
+    hsize[p] = uint32_t(sz);
+    ecount[p] = ecnt;
+  }
+  __shared__ unsigned long long red[kBlock / 64];
+  unsigned long long v = nd_local;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long s2 = 0;
+    for (int w = 0; w < kBlock / 64; ++w) s2 += red[w];
+    atomicAdd(ndist, s2);
+  }
+}
+
 // Random init (kdtreeub_index.hpp:1377-1423).
 __global__ void k_init(const float* __restrict__ X, int D, uint32_t n, int S, int L, uint64_t seed,
                        uint64_t* __restrict__ hkey, uint32_t* __restrict__ hsize, const uint32_t* __restrict__ extra,
@@ -404,105 +537,175 @@ struct Stage {
   uint64_t key;
 };
 
-// One workgroup per joining node; two lanes per pair.
-__global__ void __launch_bounds__(kBlock) k_join(const float* __restrict__ X, int D, uint32_t n,
-                                                 const uint64_t* __restrict__ new_off,
-                                                 const uint32_t* __restrict__ new_ids,
-                                                 const uint64_t* __restrict__ old_off,
-                                                 const uint32_t* __restrict__ old_ids,
-                                                 const uint64_t* __restrict__ worst, int stage_rows,
-                                                 Stage* __restrict__ stage, unsigned long long cap,
-                                                 unsigned long long* __restrict__ nstage,
-                                                 uint32_t* __restrict__ tcount,
-                                                 unsigned long long* __restrict__ npairs) {
+// Join tasks.  The rows of a joining node are its new list followed by its
+// old list, cut into blocks of BR rows.  Every pair the reference measures is
+// a pair of positions p < q with p new (new x new pairs once per unordered
+// position pair, new x old pairs always have p < q), so task (node, I, J),
+// I <= J, I a block holding new rows, covers all pairs with p in block I and
+// q in block J.
+__device__ inline uint32_t join_task_count(uint32_t na, uint32_t nb, int BR) {
+  if (na == 0) return 0;
+  const uint32_t nI = (na + BR - 1) / BR, nJ = (na + nb + BR - 1) / BR;
+  return nI * nJ - nI * (nI - 1) / 2;
+}
+
+__global__ void k_join_count(uint32_t n, const uint64_t* __restrict__ new_off, const uint64_t* __restrict__ old_off,
+                             int BR, int32_t* __restrict__ cnt) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  cnt[p] = int32_t(join_task_count(uint32_t(new_off[p + 1] - new_off[p]), uint32_t(old_off[p + 1] - old_off[p]), BR));
+}
+
+__global__ void k_join_tasks(uint32_t n, const uint64_t* __restrict__ new_off, const uint64_t* __restrict__ old_off,
+                             int BR, const int32_t* __restrict__ toff, uint64_t* __restrict__ tasks) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t na = uint32_t(new_off[p + 1] - new_off[p]), nb = uint32_t(old_off[p + 1] - old_off[p]);
+  if (na == 0) return;
+  const uint32_t nI = (na + BR - 1) / BR, nJ = (na + nb + BR - 1) / BR;
+  uint64_t* out = tasks + toff[p];
+  for (uint32_t I = 0; I < nI; ++I)
+    for (uint32_t J = I; J < nJ; ++J) *out++ = (uint64_t(p) << 32) | (uint64_t(I) << 16) | J;
+}
+
+// Staging slots are reserved per wavefront in runs of at least kStageRun, so
+// the shared counter sees one atomic per run.  The slots a wavefront leaves
+// unused at its end are marked with target kNoTarget and skipped by the
+// bucketing.
+constexpr unsigned long long kStageRun = 512;
+constexpr uint32_t kNoTarget = 0xffffffffu;
+__device__ inline void stage_put(Stage* stage, unsigned long long cap, unsigned long long base,
+                                 unsigned long long nb, unsigned long long rem, int c, Stage v,
+                                 uint32_t* tcount) {
+  const unsigned long long at = (unsigned long long)c < rem ? base + c : nb + c;
+  if (at < cap) {
+    stage[at] = v;
+    atomicAdd(&tcount[v.target], 1u);
+  }
+}
+__device__ inline void stage_abandon(Stage* stage, unsigned long long cap, unsigned long long from,
+                                     unsigned long long to, int lane) {
+  for (unsigned long long i = from + lane; i < to && i < cap; i += 64) stage[i].target = kNoTarget;
+}
+
+// Squared L2 of two LDS rows, one thread, in the reference's AVX order.
+__device__ inline float l2_lds(const float* a, const float* b, int D) { return l2_thread(a, b, D); }
+
+// One workgroup per task: both blocks' rows in LDS (row stride D + 4 floats,
+// so 16 consecutive rows start on distinct bank groups), one thread per pair,
+// the targets' worst keys in LDS.  A candidate is kept when it beats its
+// target's worst key; kept candidates are staged {target, joining node |
+// kFirst for the first endpoint, key} with one reservation per wavefront.
+__global__ void __launch_bounds__(kBlock) k_join_tiles(const float* __restrict__ X, int D, int BR, uint32_t ntask,
+                                                       const uint64_t* __restrict__ tasks,
+                                                       const uint64_t* __restrict__ new_off,
+                                                       const uint32_t* __restrict__ new_ids,
+                                                       const uint64_t* __restrict__ old_off,
+                                                       const uint32_t* __restrict__ old_ids,
+                                                       const uint64_t* __restrict__ worst,
+                                                       Stage* __restrict__ stage, unsigned long long cap,
+                                                       unsigned long long* __restrict__ nstage,
+                                                       uint32_t* __restrict__ tcount,
+                                                       unsigned long long* __restrict__ npairs) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  float* rows = reinterpret_cast<float*>(smem_raw);
-  __shared__ uint32_t ids_s[512];
-  const int tid = threadIdx.x, pr = tid >> 1, h = tid & 1;
+  __shared__ uint32_t ids_s[2][kJoinMaxBR];
+  __shared__ uint64_t w_s[2][kJoinMaxBR];
+  const int RS = D + 4;
+  float* rows0 = reinterpret_cast<float*>(smem_raw);
+  float* rows1 = rows0 + size_t(BR) * RS;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   unsigned long long pairs_local = 0;
-  for (uint32_t nn = blockIdx.x; nn < n; nn += gridDim.x) {
-    const uint64_t a0 = new_off[nn], a1 = new_off[nn + 1], b0 = old_off[nn], b1 = old_off[nn + 1];
-    const int na = int(a1 - a0), nb = int(b1 - b0);
-    if (na == 0) continue;
-    const int m = na + nb;
-    const bool staged = m <= stage_rows && m <= 512;
-    if (m <= 512) {
-      for (int i = tid; i < m; i += kBlock) ids_s[i] = i < na ? new_ids[a0 + i] : old_ids[b0 + i - na];
+  unsigned long long wbase = 0, wend = 0;  // this wavefront's reserved staging run
+  for (uint32_t t = blockIdx.x; t < ntask; t += gridDim.x) {
+    const uint64_t task = tasks[t];
+    const uint32_t nn = uint32_t(task >> 32);
+    const int I = int((task >> 16) & 0xffff), J = int(task & 0xffff);
+    const uint64_t a0 = new_off[nn], b0 = old_off[nn];
+    const int na = int(new_off[nn + 1] - a0), m = na + int(old_off[nn + 1] - b0);
+    const bool same = I == J;
+    const int sizeA = min(BR, m - I * BR), newA = min(BR, na - I * BR), sizeB = min(BR, m - J * BR);
+    if (tid < sizeA + (same ? 0 : sizeB)) {
+      const int side = tid < sizeA ? 0 : 1, r = side ? tid - sizeA : tid;
+      const int pos = (side ? J : I) * BR + r;
+      const uint32_t id = pos < na ? new_ids[a0 + pos] : old_ids[b0 + pos - na];
+      ids_s[side][r] = id;
+      w_s[side][r] = worst[id];
     }
     __syncthreads();
-    if (staged) {
-      const int D4 = D / 4;
-      for (int i = tid; i < m * D4; i += kBlock) {
-        int r = i / D4, c = i % D4;
-        reinterpret_cast<float4*>(rows)[size_t(r) * D4 + c] =
-            reinterpret_cast<const float4*>(X + size_t(ids_s[r]) * D)[c];
-      }
+    const int D4 = D >> 2, nrows = sizeA + (same ? 0 : sizeB);
+    for (int i = tid; i < nrows * D4; i += kBlock) {
+      const int r = i / D4, c = i - r * D4;
+      const int side = r < sizeA ? 0 : 1, rr = side ? r - sizeA : r;
+      float* dst = (side ? rows1 : rows0) + size_t(rr) * RS + 4 * c;
+      *reinterpret_cast<float4*>(dst) = reinterpret_cast<const float4*>(X + size_t(ids_s[side][rr]) * D)[c];
     }
     __syncthreads();
-    const long long total = (long long)na * na + (long long)na * nb;
-    for (long long k0 = 0; k0 < total; k0 += kBlock / 2) {
-      long long k = k0 + pr;
-      bool valid = k < total;
-      int ia = 0, ib = 0;
+    const float* rowsB = same ? rows0 : rows1;
+    const int sB = same ? 0 : 1;
+    const int cells = newA * sizeB;
+    for (int k0 = 0; k0 < cells; k0 += kBlock) {
+      const int k = k0 + tid;
+      bool valid = k < cells;
+      int p = 0, q = 0;
       uint32_t a = 0, b = 0;
+      uint64_t wa = 0, wb = 0;
       if (valid) {
-        if (k < (long long)na * na) {
-          ia = int(k / na);
-          ib = int(k % na);
-        } else {
-          long long kk = k - (long long)na * na;
-          ia = int(kk / nb);
-          ib = na + int(kk % nb);
+        p = k / sizeB;
+        q = k - p * sizeB;
+        if (same && q <= p) valid = false;
+        a = ids_s[0][p];
+        b = ids_s[sB][q];
+        wa = w_s[0][p];
+        wb = w_s[sB][q];
+        if (J * BR + q < na) {  // new x new: once, first endpoint = lower id
+          if (a == b) valid = false;
+          if (b < a) {
+            uint32_t t2 = a; a = b; b = t2;
+            uint64_t w2 = wa; wa = wb; wb = w2;
+          }
         }
-        a = m <= 512 ? ids_s[ia] : (ia < na ? new_ids[a0 + ia] : old_ids[b0 + ia - na]);
-        b = m <= 512 ? ids_s[ib] : (ib < na ? new_ids[a0 + ib] : old_ids[b0 + ib - na]);
-        if (ib < na && !(a < b)) valid = false;
       }
-      // both lanes of a pair agree on `valid`
       bool ca = false, cb = false;
       uint64_t ka = 0, kb = 0;
       if (valid) {
-        const float* ra = staged ? rows + size_t(ia) * D : X + size_t(a) * D;
-        const float* rb = staged ? rows + size_t(ib) * D : X + size_t(b) * D;
-        float dist = l2_pair_lanes(ra, rb, D, h);
-        if (h) {
-          ++pairs_local;
-          ka = make_key(dist, b);
-          kb = make_key(dist, a);
-          ca = ka < worst[a];
-          cb = kb < worst[b];
-        }
+        const float dist = l2_lds(rows0 + size_t(p) * RS, rowsB + size_t(q) * RS, D);
+        ++pairs_local;
+        ka = make_key(dist, b);
+        kb = make_key(dist, a);
+        ca = ka < wa;
+        cb = kb < wb;
       }
-      // one reservation per wavefront
       const uint64_t ma = __ballot(ca), mb = __ballot(cb);
-      const int lane = threadIdx.x & 63;
       const int na_ = __popcll(ma), tot = na_ + __popcll(mb);
       if (tot) {
-        const uint64_t any = ma | mb;
-        const int leader = __ffsll((unsigned long long)any) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(nstage, (unsigned long long)tot);
-        base = (unsigned long long)__shfl((long long)base, leader);
-        const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-        if (ca) {
-          unsigned long long at = base + __popcll(ma & lt);
-          if (at < cap) {
-            stage[at] = Stage{a, nn | kFirst, ka};
-            atomicAdd(&tcount[a], 1u);
-          }
-        }
-        if (cb) {
-          unsigned long long at = base + na_ + __popcll(mb & lt);
-          if (at < cap) {
-            stage[at] = Stage{b, nn, kb};
-            atomicAdd(&tcount[b], 1u);
-          }
+        // candidate c of this step goes to slot c of the wavefront's run, and
+        // past its end into a freshly reserved run
+        const unsigned long long rem = wend - wbase;
+        unsigned long long nb = 0;
+        if ((unsigned long long)tot > rem) {
+          const unsigned long long want = (unsigned long long)tot - rem > kStageRun ? tot - rem : kStageRun;
+          if (lane == 0) nb = atomicAdd(nstage, want);
+          nb = (unsigned long long)__shfl((long long)nb, 0);
+          const unsigned long long base = wbase;
+          wbase = nb + (tot - rem);
+          wend = nb + want;
+          nb -= rem;  // slot of candidate c >= rem is nb + c
+          if (ca) stage_put(stage, cap, base, nb, rem, __popcll(ma & lt), Stage{a, nn | kFirst, ka}, tcount);
+          if (cb) stage_put(stage, cap, base, nb, rem, na_ + __popcll(mb & lt), Stage{b, nn, kb}, tcount);
+        } else {
+          const unsigned long long base = wbase;
+          wbase += tot;
+          if (ca) stage_put(stage, cap, base, 0, rem, __popcll(ma & lt), Stage{a, nn | kFirst, ka}, tcount);
+          if (cb) stage_put(stage, cap, base, 0, rem, na_ + __popcll(mb & lt), Stage{b, nn, kb}, tcount);
         }
       }
     }
     __syncthreads();
   }
-  atomicAdd(npairs, pairs_local);
+  stage_abandon(stage, cap, wbase, wend, lane);
+  for (int o = 32; o > 0; o >>= 1) pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
+  if (lane == 0 && pairs_local) atomicAdd(npairs, pairs_local);
 }
 
 __global__ void k_bucket(const Stage* __restrict__ stage, unsigned long long m, const int32_t* __restrict__ toff,
@@ -510,126 +713,112 @@ __global__ void k_bucket(const Stage* __restrict__ stage, unsigned long long m, 
   unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   Stage s = stage[i];
+  if (s.target == kNoTarget) return;
   uint32_t at = atomicAdd(&cursor[s.target], 1u);
   bkey[size_t(toff[s.target]) + at] = s.key;
   bsrc[size_t(toff[s.target]) + at] = s.src;
 }
 
-// Per target: pool' = best L of (pool + candidates).  Entries are sorted by
-// (key, tag): existing entries carry tag = flag (0/1); candidates carry
+// Candidate tags: existing entries carry their flag (0/1); candidates carry
 // 1<<31 | (not first endpoint) << 30 | joining node, so duplicates resolve to
 // the existing entry, else to the lowest first-endpoint joining node.
-__global__ void __launch_bounds__(kBlock) k_merge_pool(uint32_t n, int L, int CK, const int32_t* __restrict__ toff,
+__device__ inline uint32_t cand_tag(uint32_t s) {
+  return 0x80000000u | ((s & kFirst) ? 0u : 0x40000000u) | (s & 0x3fffffffu);
+}
+
+// Per target: pool' = best L of (pool + candidates) by (key, tag), one
+// wavefront per target, in rounds: the running list (at most L entries,
+// sorted, unique) plus the next
+// kCap - L candidates are sorted by (key, tag) in the wavefront's LDS window,
+// duplicates dropped, and the first L kept.  An entry beyond the L-th unique
+// key of a subset is beyond it in the union, so rounds give the one-shot
+// result.
+template <int kCap>
+__global__ void __launch_bounds__(kBlock) k_merge_wave(uint32_t n, int L, int CK, const int32_t* __restrict__ toff,
                                                        const uint64_t* __restrict__ bkey,
                                                        const uint32_t* __restrict__ bsrc,
                                                        uint64_t* __restrict__ pool, uint8_t* __restrict__ flag,
-                                                       uint32_t* __restrict__ fill, uint8_t* __restrict__ credit,
-                                                       int chunk) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  uint64_t* k = reinterpret_cast<uint64_t*>(smem_raw);
-  uint32_t* tg = reinterpret_cast<uint32_t*>(k + chunk);
-  __shared__ uint64_t ck[kBlock];
-  __shared__ uint32_t ctg[kBlock];
-  __shared__ int keep_s;
-  const int tid = threadIdx.x;
-  for (uint32_t x = blockIdx.x; x < n; x += gridDim.x) {
+                                                       uint32_t* __restrict__ fill, uint8_t* __restrict__ credit) {
+  constexpr int kW = kCap >= 1024 ? 2 : 4;  // wavefronts per workgroup
+  __shared__ uint64_t ks[kW][kCap];
+  __shared__ uint32_t ts[kW][kCap];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t* k = ks[w];
+  uint32_t* tg = ts[w];
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (uint32_t x = blockIdx.x * kW + w; x < n; x += gridDim.x * kW) {
     const int c0 = toff[x], c1 = toff[x + 1];
     if (c0 == c1) continue;
-    const int Lx = int(fill[x]);
-    // current list: existing pool
-    int cur = Lx;
-    for (int i = tid; i < Lx; i += kBlock) {
+    int cur = int(fill[x]);
+    for (int i = lane; i < cur; i += 64) {
       k[i] = pool[size_t(x) * L + i];
       tg[i] = flag[size_t(x) * L + i];
     }
     for (int base = c0; base < c1;) {
-      int take = c1 - base;
-      if (take > chunk - cur) take = chunk - cur;
-      for (int i = tid; i < take; i += kBlock) {
-        uint32_t s = bsrc[base + i];
-        k[cur + i] = bkey[base + i];
-        tg[cur + i] = 0x80000000u | ((s & kFirst) ? 0u : 0x40000000u) | (s & 0x3fffffffu);
+      const int take = min(c1 - base, kCap - cur), m = cur + take, p2 = next_pow2(m < 2 ? 2 : m);
+      for (int i = cur + lane; i < p2; i += 64) {
+        const bool in = i < m;
+        k[i] = in ? bkey[base + i - cur] : kKeyMax;
+        tg[i] = in ? cand_tag(bsrc[base + i - cur]) : 0xffffffffu;
       }
       base += take;
-      int m = cur + take;
-      int p2 = next_pow2(m < 2 ? 2 : m);
-      for (int i = m + tid; i < p2; i += kBlock) {
-        k[i] = kKeyMax;
-        tg[i] = 0xffffffffu;
-      }
-      __syncthreads();
-      // bitonic sort on (key, tag)
+      wave_sync();
       for (int kk = 2; kk <= p2; kk <<= 1) {
         for (int j = kk >> 1; j > 0; j >>= 1) {
-          for (int i = tid; i < p2; i += kBlock) {
-            int ixj = i ^ j;
-            if (ixj > i) {
-              uint64_t ka = k[i], kb = k[ixj];
-              uint32_t ta = tg[i], tb = tg[ixj];
-              bool gt = (ka > kb) || (ka == kb && ta > tb);
-              bool up = (i & kk) == 0;
-              if (gt == up) {
-                k[i] = kb;
-                k[ixj] = ka;
-                tg[i] = tb;
-                tg[ixj] = ta;
+          for (int t = lane; t < p2; t += 64) {
+            const int u = t ^ j;
+            if (u > t) {
+              const uint64_t ka = k[t], kb = k[u];
+              const uint32_t ta = tg[t], tb = tg[u];
+              const bool gt = (ka > kb) || (ka == kb && ta > tb);
+              if (gt == ((t & kk) == 0)) {
+                k[t] = kb;
+                k[u] = ka;
+                tg[t] = tb;
+                tg[u] = ta;
               }
             }
           }
-          __syncthreads();
+          wave_sync();
         }
       }
-      // dedup keeping the first of equal keys, keep the first L
+      // drop duplicates (the first of equal keys stays), compact to the front
       int kept = 0;
-      for (int b0 = 0; b0 < m; b0 += kBlock) {
-        int i = b0 + tid;
-        int first = 0;
+      for (int b0 = 0; b0 < m && kept < L; b0 += 64) {
+        const int i = b0 + lane;
+        bool first = false;
         uint64_t kv = 0;
         uint32_t tv = 0;
         if (i < m) {
           kv = k[i];
           tv = tg[i];
-          first = (i == 0 || k[i - 1] != kv) ? 1 : 0;
+          first = i == 0 || k[i - 1] != kv;
         }
-        __shared__ int scan_tmp[kBlock / 64];
-        int tot;
-        int rk = block_exclusive_scan(first, scan_tmp, &tot);
-        if (first && kept + rk < L) {
-          ck[tid] = kv;  // staged per chunk of threads
-          ctg[tid] = tv;
+        const uint64_t fm = __ballot(first);
+        const int at = kept + __popcll(fm & lt);
+        wave_sync();
+        if (first && at < L) {  // at <= i: only this lane's own slot or earlier ones are written
+          k[at] = kv;
+          tg[at] = tv;
         }
-        __syncthreads();
-        // write back compacted entries to the front (positions kept + rk < b0 + tid)
-        if (first && kept + rk < L) {
-          k[kept + rk] = ck[tid];
-          tg[kept + rk] = ctg[tid];
-        }
-        __syncthreads();
-        kept += tot;
-        if (kept >= L) {
-          kept = L;
-          break;
-        }
+        wave_sync();
+        kept += __popcll(fm);
       }
-      cur = kept;
-      if (tid == 0) keep_s = kept;
-      __syncthreads();
+      cur = kept < L ? kept : L;
     }
-    // write pool and credits
-    const int out = cur;
-    for (int i = tid; i < L; i += kBlock) {
-      if (i < out) {
-        uint32_t t = tg[i];
+    for (int i = lane; i < L; i += 64) {
+      if (i < cur) {
+        const uint32_t tv = tg[i];
         pool[size_t(x) * L + i] = k[i];
-        flag[size_t(x) * L + i] = (t & 0x80000000u) ? 1 : uint8_t(t & 1u);
-        if ((t & 0xc0000000u) == 0x80000000u && i < CK) credit[t & 0x3fffffffu] = 1;
+        flag[size_t(x) * L + i] = (tv & 0x80000000u) ? 1 : uint8_t(tv & 1u);
+        if ((tv & 0xc0000000u) == 0x80000000u && i < CK) credit[tv & 0x3fffffffu] = 1;
       } else {
         pool[size_t(x) * L + i] = kKeyMax;
         flag[size_t(x) * L + i] = 0;
       }
     }
-    if (tid == 0) fill[x] = uint32_t(out);
-    __syncthreads();
+    if (lane == 0) fill[x] = uint32_t(cur);
+    wave_sync();
   }
 }
 
@@ -837,44 +1026,36 @@ int gpu_merge_init(efg_index* ix) {
   DevBuf<uint64_t> hkey, rec;
   DevBuf<unsigned long long> nd;
   if (!depth.alloc(M) || !tree_of.alloc(M) || !leaf_at.alloc(size_t(ix->T) * n) || !boff.alloc(M + 1) ||
-      !scal.alloc(2) || !cnt.alloc(M) || !cursor.alloc(M) || !hsize.alloc(n) || !ecount.alloc(n) ||
+      !scal.alloc(4) || !cnt.alloc(M) || !cursor.alloc(M) || !hsize.alloc(n) || !ecount.alloc(n) ||
       !extra.alloc(size_t(n) * 2 * S) || !hkey.alloc(size_t(n) * S) || !nd.alloc(2))
     return fail(EFG_ERR_MEMORY, "out of device memory (merge)");
-  // tree of each node (from the roots' ranges: walk parents)
-  {
-    std::vector<int32_t> h_roots(ix->T);
-    hipMemcpy(h_roots.data(), ix->roots.p, ix->T * 4, hipMemcpyDeviceToHost);
-    std::vector<int32_t> par(M);
-    hipMemcpy(par.data(), ix->parent.p, size_t(M) * 4, hipMemcpyDeviceToHost);
-    std::vector<int32_t> tof(M, -1);
-    for (int t = 0; t < ix->T; ++t) tof[h_roots[t]] = t;
-    // nodes are created after their parents in every tree this library builds
-    // or loads from the reference format (post-order files are re-linked, so
-    // resolve by walking up)
-    for (int i = 0; i < M; ++i) {
-      if (tof[i] >= 0) continue;
-      int x = i;
-      std::vector<int> path;
-      while (tof[x] < 0 && par[x] >= 0) {
-        path.push_back(x);
-        x = par[x];
-      }
-      for (int y : path) tof[y] = tof[x];
-    }
-    hipMemcpyAsync(tree_of.p, tof.data(), size_t(M) * 4, hipMemcpyHostToDevice, st);
-    hipStreamSynchronize(st);
-  }
-  k_node_depth<<<grid_for(M), 256, 0, st>>>(ix->parent.p, M, depth.p);
-  int big = 127;
-  hipMemcpyAsync(scal.p, &big, 4, hipMemcpyHostToDevice, st);
+  int init4[4] = {127, 0, 0, 0};
+  hipMemcpyAsync(scal.p, init4, 16, hipMemcpyHostToDevice, st);
+  k_node_depth<<<grid_for(M), 256, 0, st>>>(ix->parent.p, M, ix->roots.p, ix->T, depth.p, tree_of.p, scal.p + 2);
   k_leaf_at<<<grid_for(M), 256, 0, st>>>(ix->node4.p, ix->range.p, depth.p, tree_of.p, M, n, TB, p.merge_level,
-                                         leaf_at.p, scal.p);
+                                         leaf_at.p, scal.p, scal.p + 3);
   hipMemsetAsync(hsize.p, 0, size_t(n) * 4, st);
   hipMemsetAsync(ecount.p, 0, size_t(n) * 4, st);
   hipMemsetAsync(nd.p, 0, 16, st);
-  int err_depth = 127;
-  hipMemcpyAsync(&err_depth, scal.p, 4, hipMemcpyDeviceToHost, st);
+  int h4[4] = {127, 0, 0, 0};
+  hipMemcpyAsync(h4, scal.p, 16, hipMemcpyDeviceToHost, st);
   if (int rc = check_hip(hipStreamSynchronize(st), "merge")) return rc;
+  const int err_depth = h4[0], max_depth = h4[2], LM = h4[3];
+  // distance tiles: LDS per wavefront (LM + PB) rows of D + 4 floats
+  const int PB = std::max(1, std::min(8, 64 / std::max(1, LM)));
+  const size_t wave_lds = size_t(LM + PB) * (D + 4) * 4;
+  const int tile_waves = int(std::min<size_t>(4, (64 * 1024) / std::max<size_t>(wave_lds, 1)));
+  const bool tiles = LM > 0 && tile_waves >= 1;
+  const int nlev = std::max(1, max_depth - p.merge_level + 1);
+  DevBuf<float> mdist;
+  DevBuf<uint2> pslot;
+  if (tiles) {
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf_dist), hipFuncAttributeMaxDynamicSharedMemorySize,
+                        int(wave_lds * tile_waves));
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replay_dist), hipFuncAttributeMaxDynamicSharedMemorySize,
+                        int(size_t(kBlock) * S * 8));
+    if (!pslot.alloc(size_t(n) * nlev)) return fail(EFG_ERR_MEMORY, "out of device memory (merge)");
+  }
   const size_t lds = size_t(kBlock) * S * 8;
   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replay), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sort_big), hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8);
@@ -905,9 +1086,18 @@ int gpu_merge_init(efg_index* ix) {
     k_sort_buckets<<<unsigned(std::min<size_t>((M + 3) / 4, 65536)), kBlock, 0, st>>>(boff.p, M, rec.p, bigl.p,
                                                                                        nbig.p);
     k_sort_big<<<unsigned(ncu_m * 2), kBlock, size_t(8192) * 8, st>>>(boff.p, bigl.p, nbig.p, rec.p);
-    k_replay<<<grid_for(n), kBlock, lds, st>>>(ix->data.p, D, ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la,
-                                               n, p.merge_level, S, boff.p, rec.p, hkey.p, hsize.p, extra.p,
-                                               ecount.p, nd.p);
+    if (tiles) {
+      if (!mdist.ensure(size_t(total) * LM + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (merge distances)");
+      k_leaf_dist<<<unsigned(ncu_m * 8), 64 * tile_waves, wave_lds * tile_waves, st>>>(
+          ix->data.p, D, ix->range.p, lt, boff.p, M, rec.p, LM, PB, p.merge_level, nlev, mdist.p, pslot.p);
+      k_replay_dist<<<grid_for(n), kBlock, lds, st>>>(ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la, n,
+                                                      p.merge_level, S, boff.p, rec.p, mdist.p, pslot.p, LM, nlev,
+                                                      hkey.p, hsize.p, extra.p, ecount.p, nd.p);
+    } else {
+      k_replay<<<grid_for(n), kBlock, lds, st>>>(ix->data.p, D, ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt,
+                                                 la, n, p.merge_level, S, boff.p, rec.p, hkey.p, hsize.p, extra.p,
+                                                 ecount.p, nd.p);
+    }
   }
   // init
   bs.L = L;
@@ -965,24 +1155,40 @@ int gpu_refine_iter(efg_index* ix) {
   unsigned long long bound = 0;
   hipMemcpyAsync(&bound, ctr.p, 8, hipMemcpyDeviceToHost, st);
   if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
-  size_t cap = size_t(2 * bound + 1024);
   DevBuf<Stage> stage;
-  const int stage_rows = int((48 * 1024) / (size_t(D) * 4));
-  const size_t lds_join = size_t(stage_rows > 0 ? stage_rows : 0) * D * 4;
-  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_join), hipFuncAttributeMaxDynamicSharedMemorySize,
+  // join tasks over blocks of BR rows; two blocks of padded rows in LDS
+  int BR = kJoinMaxBR;
+  while (BR > 4 && size_t(2) * BR * (D + 4) * 4 > 64 * 1024) BR >>= 1;
+  const size_t lds_join = size_t(2) * BR * (D + 4) * 4;
+  DevBuf<int32_t> tcnt, tskoff;
+  if (!tcnt.alloc(n) || !tskoff.alloc(n + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (join tasks)");
+  k_join_count<<<grid_for(n), 256, 0, st>>>(n, bs.new_off.p, bs.old_off.p, BR, tcnt.p);
+  if (int rc = device_exclusive_scan(tcnt.p, int(n), tskoff.p, scal.p + 2, sums, st)) return rc;
+  int ntask = 0;
+  hipMemcpyAsync(&ntask, scal.p + 2, 4, hipMemcpyDeviceToHost, st);
+  if (int rc = check_hip(hipStreamSynchronize(st), "join tasks")) return rc;
+  DevBuf<uint64_t> tasks;
+  if (!tasks.alloc(size_t(ntask) + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (join tasks)");
+  k_join_tasks<<<grid_for(n), 256, 0, st>>>(n, bs.new_off.p, bs.old_off.p, BR, tskoff.p, tasks.p);
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_join_tiles), hipFuncAttributeMaxDynamicSharedMemorySize,
                       int(lds_join));
   int dev = 0, ncu = 256;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const int per_cu = std::max(1, std::min(8, int((160 * 1024) / (lds_join + 1024))));
+  const size_t join_grid = std::min<size_t>(size_t(ntask), size_t(ncu) * per_cu);
+  // two candidates per pair at most, plus one partly used run per wavefront
+  size_t cap = size_t(2 * bound) + join_grid * (kBlock / 64) * kStageRun + 1024;
   unsigned long long h_ctr[2] = {0, 0};
   for (int attempt = 0; attempt < 3; ++attempt) {
     if (!stage.alloc(cap)) return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
     hipMemsetAsync(ctr.p, 0, 16, st);
     hipMemsetAsync(tcount.p, 0, size_t(n) * 4, st);
-    unsigned grid = unsigned(std::min<size_t>(n, size_t(ncu) * 16));
-    k_join<<<grid, kBlock, lds_join, st>>>(ix->data.p, D, n, bs.new_off.p, bs.new_ids.p, bs.old_off.p, bs.old_ids.p,
-                                           worst.p, stage_rows, stage.p, (unsigned long long)cap, ctr.p, tcount.p,
-                                           ctr.p + 1);
+    if (ntask) {
+      k_join_tiles<<<unsigned(join_grid), kBlock, lds_join, st>>>(ix->data.p, D, BR, uint32_t(ntask), tasks.p, bs.new_off.p,
+                                                   bs.new_ids.p, bs.old_off.p, bs.old_ids.p, worst.p, stage.p,
+                                                   (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
+    }
     hipMemcpyAsync(h_ctr, ctr.p, 16, hipMemcpyDeviceToHost, st);
     if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
     if (h_ctr[0] <= cap) break;
@@ -1000,14 +1206,22 @@ int gpu_refine_iter(efg_index* ix) {
   hipMemsetAsync(cursor.p, 0, size_t(n) * 4, st);
   if (nst) k_bucket<<<unsigned((nst + 255) / 256), 256, 0, st>>>(stage.p, nst, toff.p, cursor.p, bkey.p, bsrc.p);
   hipMemsetAsync(bs.found.p, 0, n, st);
-  const int chunk = 4096;
-  const size_t lds_m = size_t(chunk) * 12;
-  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_merge_pool), hipFuncAttributeMaxDynamicSharedMemorySize,
-                      int(lds_m));
   {
-    unsigned grid = unsigned(std::min<size_t>(n, size_t(ncu) * 32));
-    k_merge_pool<<<grid, kBlock, lds_m, st>>>(n, L, p.check, toff.p, bkey.p, bsrc.p, bs.pool.p, bs.flag.p, bs.fill.p,
-                                              bs.found.p, chunk);
+    // one wavefront per target; the LDS window holds at least 2L entries
+    const unsigned grid = unsigned(std::min<size_t>((n + 3) / 4, size_t(ncu) * 8));
+    if (2 * L <= 256)
+      k_merge_wave<256><<<grid, kBlock, 0, st>>>(n, L, p.check, toff.p, bkey.p, bsrc.p, bs.pool.p, bs.flag.p,
+                                                 bs.fill.p, bs.found.p);
+    else if (2 * L <= 512)
+      k_merge_wave<512><<<grid, kBlock, 0, st>>>(n, L, p.check, toff.p, bkey.p, bsrc.p, bs.pool.p, bs.flag.p,
+                                                 bs.fill.p, bs.found.p);
+    else if (2 * L <= 1024)
+      k_merge_wave<1024><<<grid * 2, 128, 0, st>>>(n, L, p.check, toff.p, bkey.p, bsrc.p, bs.pool.p, bs.flag.p,
+                                                   bs.fill.p, bs.found.p);
+    else
+      k_merge_wave<2048><<<grid * 2, 128, 0, st>>>(n, L, p.check, toff.p, bkey.p, bsrc.p, bs.pool.p, bs.flag.p,
+                                                   bs.fill.p, bs.found.p);
+    if (int rc = check_hip(hipStreamSynchronize(st), "join merge")) return rc;
   }
   if (int rc = check_hip(hipStreamSynchronize(st), "join merge")) return rc;
   double t1 = now_ms();

@@ -22,6 +22,7 @@
 //                  creates the children, then the scatter and copy back.
 //   k_next_flags + scan + k_next_compact
 //                  children that stay active, in child order.
+#include <algorithm>
 #include <vector>
 
 #include "efg_device.hpp"
@@ -68,64 +69,114 @@ __global__ void k_init_roots(int4* node4, int32_t* parent, uint2* range, int32_t
   roots[t] = t;
 }
 
-// One workgroup per active node.
+// Split statistics: a group of `tpn` threads (a multiple of 64) per active
+// node, `kBlock / tpn` nodes per workgroup.  A thread owns dimensions k,
+// k + tpn, ...; the up to 101 sampled values of its dimension are loaded
+// kChunk at a time into registers (the row index is clamped, values past the
+// sample are never used) and summed in row order as the reference does; the
+// variance pass loads them again.  The first
+// wavefront of the group picks the top-5 variance dimensions by five arg-max
+// rounds over (variance descending, dimension ascending), which is the order
+// the reference's insertion keeps.
+constexpr int kSample = 101;
+constexpr int kChunk = 32;
 __global__ void __launch_bounds__(kBlock) k_split_stats(const float* __restrict__ X, int d, int D,
                                                         const int32_t* __restrict__ active, int na,
                                                         const uint2* __restrict__ range,
                                                         const int32_t* __restrict__ tree_of,
                                                         const uint32_t* __restrict__ leaf, uint32_t n, uint64_t seed,
-                                                        int4* __restrict__ node4) {
+                                                        int4* __restrict__ node4, int tpn) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  float* mean = reinterpret_cast<float*>(smem_raw);
+  const int G = kBlock / tpn, g = threadIdx.x / tpn, lt = threadIdx.x - g * tpn;
+  float* mean = reinterpret_cast<float*>(smem_raw) + size_t(g) * 2 * d;
   float* var = mean + d;
-  __shared__ int cut_s;
-  for (int a = blockIdx.x; a < na; a += gridDim.x) {
-    const int ni = active[a];
-    const uint2 r = range[ni];
-    const int t = tree_of[ni];
-    const uint32_t* ids = leaf + size_t(t) * n + r.x;
+  __shared__ uint32_t ids_s[kBlock / 64][kSample];
+  for (int a0 = blockIdx.x * G; a0 < na; a0 += gridDim.x * G) {
+    const int a = a0 + g;
+    const bool valid = a < na;
+    int ni = 0, t = 0;
+    uint2 r = make_uint2(0, 1);
+    if (valid) {
+      ni = active[a];
+      r = range[ni];
+      t = tree_of[ni];
+    }
     const unsigned count = r.y - r.x;
-    const unsigned cnt = count < 101u ? count : 101u;
-    const float div = 1.0f / float(cnt);
-    for (int k = threadIdx.x; k < d; k += kBlock) {
-      float m = 0.f;
-      for (unsigned j = 0; j < cnt; ++j) m = m + X[size_t(ids[j]) * D + k];
-      m = m * div;
-      float v = 0.f;
-      for (unsigned j = 0; j < cnt; ++j) {
-        float tt = X[size_t(ids[j]) * D + k] - m;
-        v = __builtin_fmaf(tt, tt, v);
+    const int cnt = int(count < unsigned(kSample) ? count : unsigned(kSample));
+    if (valid)
+      for (int j = lt; j < cnt; j += tpn) ids_s[g][j] = leaf[size_t(t) * n + r.x + j];
+    __syncthreads();
+    if (valid) {
+      const float div = 1.0f / float(cnt);
+      for (int k = lt; k < d; k += tpn) {
+        float m = 0.f;
+        for (int j0 = 0; j0 < cnt; j0 += kChunk) {
+          float xv[kChunk];
+#pragma unroll
+          for (int j = 0; j < kChunk; ++j) {
+            const int jj = j0 + j < cnt ? j0 + j : cnt - 1;
+            xv[j] = X[size_t(ids_s[g][jj]) * D + k];
+          }
+#pragma unroll
+          for (int j = 0; j < kChunk; ++j)
+            if (j0 + j < cnt) m = m + xv[j];
+        }
+        m = m * div;
+        float v = 0.f;
+        for (int j0 = 0; j0 < cnt; j0 += kChunk) {
+          float xv[kChunk];
+#pragma unroll
+          for (int j = 0; j < kChunk; ++j) {
+            const int jj = j0 + j < cnt ? j0 + j : cnt - 1;
+            xv[j] = X[size_t(ids_s[g][jj]) * D + k];
+          }
+#pragma unroll
+          for (int j = 0; j < kChunk; ++j)
+            if (j0 + j < cnt) {
+              const float tt = xv[j] - m;
+              v = __builtin_fmaf(tt, tt, v);
+            }
+        }
+        mean[k] = m;
+        var[k] = v;
       }
-      mean[k] = m;
-      var[k] = v;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      int num = 0;
-      int top[5];
-      for (int i = 0; i < d; ++i) {
-        if (num < 5 || var[i] > var[top[num - 1]]) {
-          if (num < 5) top[num++] = i;
-          else top[num - 1] = i;
-          int j = num - 1;
-          while (j > 0 && var[top[j]] > var[top[j - 1]]) {
-            int tmp = top[j];
-            top[j] = top[j - 1];
-            top[j - 1] = tmp;
-            --j;
+    if (valid && lt < 64) {
+      const int num = d < 5 ? d : 5;
+      int top[5] = {0, 0, 0, 0, 0};
+      for (int q = 0; q < num; ++q) {
+        float bv = -1.f;
+        int bi = 0x7fffffff;
+        for (int k = lt; k < d; k += 64) {
+          bool taken = false;
+          for (int z = 0; z < q; ++z) taken = taken || top[z] == k;
+          const float vk = var[k];
+          if (!taken && (vk > bv || (vk == bv && k < bi))) {
+            bv = vk;
+            bi = k;
           }
         }
+        for (int o = 32; o > 0; o >>= 1) {
+          const float ov = __shfl_xor(bv, o);
+          const int oi = __shfl_xor(bi, o);
+          if (ov > bv || (ov == bv && oi < bi)) {
+            bv = ov;
+            bi = oi;
+          }
+        }
+        top[q] = bi;
       }
-      uint64_t rnd = hash4(seed, ST_SPLIT, uint64_t(t), (uint64_t(r.x) << 32) | r.y) % uint64_t(num);
-      cut_s = top[rnd];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int cd = cut_s;
-      int4 nd = node4[ni];
-      nd.x = cd;
-      nd.y = __float_as_int(mean[cd]);
-      node4[ni] = nd;
+      if (lt == 0) {
+        const uint64_t rnd = hash4(seed, ST_SPLIT, uint64_t(t), (uint64_t(r.x) << 32) | r.y) % uint64_t(num);
+        int cd = top[0];
+        for (int z = 1; z < 5; ++z)
+          if (uint64_t(z) == rnd) cd = top[z];
+        int4 nd = node4[ni];
+        nd.x = cd;
+        nd.y = __float_as_int(mean[cd]);
+        node4[ni] = nd;
+      }
     }
     __syncthreads();
   }
@@ -402,7 +453,9 @@ int gpu_build_trees(efg_index* ix) {
   int32_t M = T;
   bool bfs = true;
   int Lb = -1, level = 0;
-  const size_t lds_stats = size_t(2) * ix->d * sizeof(float);
+  // threads per node: the dimensions rounded up to wavefronts, at most a workgroup
+  const int tpn = int(std::min<size_t>(kBlock, (ix->d + 63) / 64 * 64));
+  const size_t lds_stats = size_t(kBlock / tpn) * 2 * ix->d * sizeof(float);
   if (lds_stats > 150 * 1024) return fail(EFG_ERR_ARG, "dimension too large for the split statistics");
   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_split_stats), hipFuncAttributeMaxDynamicSharedMemorySize,
                       int(lds_stats));
@@ -425,9 +478,10 @@ int gpu_build_trees(efg_index* ix) {
     if (size_t(base) + 2 * size_t(na) > cap) return fail(EFG_ERR_STATE, "tree node capacity exceeded");
     // split statistics
     {
-      unsigned g = unsigned(na < 65535 * 4 ? na : 65535 * 4);
+      const int per = kBlock / tpn;
+      unsigned g = unsigned(std::min<int>((na + per - 1) / per, 65535 * 4));
       k_split_stats<<<g, kBlock, lds_stats, st>>>(ix->data.p, int(ix->d), int(ix->D), act.p, na, ix->range.p,
-                                                   tree_of.p, ix->leaf.p, n, p.seed, ix->node4.p);
+                                                   tree_of.p, ix->leaf.p, n, p.seed, ix->node4.p, tpn);
     }
     // blocks
     k_blocks_per_node<<<(na + 255) / 256, 256, 0, st>>>(act.p, na, ix->range.p, nb.p);
