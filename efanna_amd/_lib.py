@@ -66,6 +66,19 @@ _SIGS = {
     "efg_set_profiling": (c_int, [c_vp, c_int]),
     "efg_get_kernel_times": (c_int, [c_vp, ctypes.POINTER(ctypes.c_double), c_int, c_intp]),
     "efg_get_search_work": (c_int, [c_vp, c_u64p, c_int]),
+    "efg_set_search_threads": (c_int, [c_vp, c_int]),
+    "efg_autotune_search": (c_int, [c_vp, c_int, c_f32p, c_sz, c_sz, c_intp]),
+    "efg_mlp_param_count": (c_int, []),
+    "efg_mlp_init": (c_int, [c_f32p, ctypes.c_uint64]),
+    "efg_mlp_logits": (c_int, [c_f32p, c_f32p, c_f32p, c_sz, c_f32p]),
+    "efg_mlp_gradient": (c_int, [c_f32p, c_f32p, c_f32p, c_u8p, c_sz, c_f32p, c_f32p]),
+    "efg_mlp_train": (c_int, [c_f32p, c_u8p, c_sz, c_f32p, c_f32p, c_int, c_int, ctypes.c_float, ctypes.c_uint64,
+                              c_f32p]),
+    "efg_set_probe_params": (c_int, [c_vp, c_int, c_int, c_int]),
+    "efg_search_traces": (c_int, [c_vp, c_int, c_f32p, c_sz, c_sz, c_i32p, c_f32p, c_u8p]),
+    "efg_set_stopping_network": (c_int, [c_vp, c_f32p, c_f32p, ctypes.c_float]),
+    "efg_set_search_mode": (c_int, [c_vp, c_int]),
+    "efg_get_learned_stats": (c_int, [c_vp, c_u64p, c_u64p, c_u64p]),
 }
 
 _lib = None

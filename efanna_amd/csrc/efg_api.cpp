@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -12,6 +13,7 @@
 
 #include "efg.h"
 #include "efg_internal.hpp"
+#include "efg_mlp.hpp"
 
 namespace efg {
 
@@ -413,6 +415,152 @@ int efg_set_search_precision(efg_index* ix, int precision) {
   if (!ix) return fail(EFG_ERR_ARG, "null index");
   if (precision != 0 && precision != 1) return fail(EFG_ERR_ARG, "precision must be 0 (FP32) or 1 (BF16)");
   ix->s_precision = precision;
+  return EFG_OK;
+}
+
+int efg_set_search_threads(efg_index* ix, int threads) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (threads != 0 && threads != 64 && threads != 128 && threads != 256)
+    return fail(EFG_ERR_ARG, "threads per query must be 0, 64, 128 or 256");
+  ix->s_threads = threads;
+  return EFG_OK;
+}
+
+int efg_autotune_search(efg_index* ix, int k, const float* queries, size_t nq, size_t q_stride, int* threads_out) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (!queries || nq == 0) return fail(EFG_ERR_ARG, "no queries");
+  if (q_stride < ix->d) return fail(EFG_ERR_ARG, "query stride below the dimension");
+  efg::DevBuf<float> dq;
+  const size_t qs = ix->D;
+  if (!dq.alloc(nq * qs)) return fail(EFG_ERR_MEMORY, "out of device memory");
+  hipStream_t st = ix->stream;
+  hipMemsetAsync(dq.p, 0, nq * qs * 4, st);
+  if (int rc = efg::check_hip(hipMemcpy2DAsync(dq.p, qs * 4, queries, q_stride * 4, ix->d * 4, nq,
+                                               hipMemcpyHostToDevice, st),
+                              "copy"))
+    return rc;
+  int chosen[2] = {0, 0};
+  if (int rc = efg::search_autotune(ix, k, dq.p, nq, qs, chosen, st)) return rc;
+  if (threads_out) {
+    threads_out[0] = chosen[0];
+    threads_out[1] = chosen[1];
+  }
+  return efg::check_hip(hipStreamSynchronize(st), "autotune");
+}
+
+int efg_mlp_param_count(void) { return efg::kMlpParams; }
+
+int efg_mlp_init(float* params, uint64_t seed) {
+  if (!params) return fail(EFG_ERR_ARG, "null parameters");
+  efg::mlp_init(params, seed);
+  return EFG_OK;
+}
+
+int efg_mlp_logits(const float* params, const float* norm, const float* feat, size_t n, float* z) {
+  if (!params || !norm || (n && (!feat || !z))) return fail(EFG_ERR_ARG, "null buffer");
+  return efg::mlp_logits(params, norm, feat, n, z, nullptr);
+}
+
+int efg_mlp_gradient(const float* params, const float* norm, const float* feat, const uint8_t* label, size_t n,
+                     float* grad, float* loss) {
+  if (!params || !norm || !feat || !label || !grad) return fail(EFG_ERR_ARG, "null buffer");
+  return efg::mlp_gradient(params, norm, feat, label, n, grad, loss, nullptr);
+}
+
+int efg_mlp_train(const float* feat, const uint8_t* label, size_t n, const float* norm, float* params, int epochs,
+                  int batch, float lr, uint64_t seed, float* loss_hist) {
+  if (!feat || !label || !norm || !params) return fail(EFG_ERR_ARG, "null buffer");
+  return efg::mlp_train(feat, label, n, norm, params, epochs, batch, lr, seed, loss_hist, nullptr);
+}
+
+int efg_set_probe_params(efg_index* ix, int pool, int extend_to, int epochs) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (pool < 0 || extend_to < 0 || epochs < 0) return fail(EFG_ERR_ARG, "invalid probe parameters");
+  ix->lp_pool = pool;
+  ix->lp_extend = extend_to;
+  ix->lp_epochs = epochs;
+  return EFG_OK;
+}
+
+int efg_search_traces(efg_index* ix, int k, const float* queries, size_t nq, size_t q_stride,
+                      const int32_t* plain_ids, float* feat, uint8_t* label) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (nq && (!queries || !plain_ids || !feat || !label)) return fail(EFG_ERR_ARG, "null buffer");
+  if (k <= 0) return fail(EFG_ERR_ARG, "k must be positive");
+  const int ep = ix->lp_epochs > 0 ? ix->lp_epochs : ix->s_epochs;
+  const size_t C = size_t(ep) + 1;
+  efg::DevBuf<float> dq, df;
+  efg::DevBuf<int32_t> dp;
+  efg::DevBuf<uint8_t> dl;
+  const size_t qs = ix->d;
+  if (!dq.alloc(nq * qs + 1) || !dp.alloc(nq * size_t(k) + 1) || !df.alloc(nq * C * efg::kFeat + 1) ||
+      !dl.alloc(nq * C + 1))
+    return fail(EFG_ERR_MEMORY, "out of device memory (traces)");
+  hipStream_t st = ix->stream;
+  if (nq) {
+    if (int rc = efg::check_hip(hipMemcpy2DAsync(dq.p, qs * 4, queries, q_stride * 4, ix->d * 4, nq,
+                                                 hipMemcpyHostToDevice, st),
+                                "copy"))
+      return rc;
+    hipMemcpyAsync(dp.p, plain_ids, nq * size_t(k) * 4, hipMemcpyHostToDevice, st);
+  }
+  if (int rc = efg::search_traces(ix, k, dq.p, nq, qs, dp.p, df.p, dl.p, st)) return rc;
+  if (nq) {
+    hipMemcpyAsync(feat, df.p, nq * C * efg::kFeat * 4, hipMemcpyDeviceToHost, st);
+    hipMemcpyAsync(label, dl.p, nq * C, hipMemcpyDeviceToHost, st);
+  }
+  return efg::check_hip(hipStreamSynchronize(st), "traces");
+}
+
+int efg_set_stopping_network(efg_index* ix, const float* params, const float* norm, float threshold) {
+  if (!ix || !params || !norm) return fail(EFG_ERR_ARG, "null argument");
+  if (!(threshold > 0.f && threshold < 1.f)) return fail(EFG_ERR_ARG, "threshold must lie in (0, 1)");
+  using namespace efg;
+  // packed columns (efg_mlp.hpp), then W3 | b3 as floats, all BF16-rounded
+  std::vector<uint32_t> pack(kPackLen + kHid + 1);
+  auto bf = [&](int i) { return uint32_t(float_to_bf16(params[i])); };
+  for (int j = 0; j < kHid; ++j) {
+    for (int i = 0; i < kFeat / 2; ++i)
+      pack[i * kHid + j] = bf(kOffW1 + (2 * i) * kHid + j) | (bf(kOffW1 + (2 * i + 1) * kHid + j) << 16);
+    for (int i = 0; i < kHid / 2; ++i)
+      pack[(kFeat / 2 + i) * kHid + j] = bf(kOffW2 + (2 * i) * kHid + j) | (bf(kOffW2 + (2 * i + 1) * kHid + j) << 16);
+    pack[(kPackWords - 1) * kHid + j] = bf(kOffB1 + j) | (bf(kOffB2 + j) << 16);
+  }
+  for (int i = 0; i <= kHid; ++i) {
+    const float v = bf16_to_float(float_to_bf16(params[kOffW3 + i]));
+    std::memcpy(&pack[kPackLen + i], &v, 4);
+  }
+  if (!ix->net_pack.ensure(pack.size()) || !ix->net_norm.ensure(2 * kFeat))
+    return fail(EFG_ERR_MEMORY, "out of device memory");
+  hipStream_t st = ix->stream;
+  hipMemcpyAsync(ix->net_pack.p, pack.data(), pack.size() * 4, hipMemcpyHostToDevice, st);
+  hipMemcpyAsync(ix->net_norm.p, norm, 2 * kFeat * 4, hipMemcpyHostToDevice, st);
+  if (int rc = check_hip(hipStreamSynchronize(st), "copy")) return rc;
+  // probability >= threshold  <=>  logit >= log(threshold / (1 - threshold))
+  ix->net_zthr = float(std::log(double(threshold) / (1.0 - double(threshold))));
+  ix->net_set = true;
+  return EFG_OK;
+}
+
+int efg_set_search_mode(efg_index* ix, int mode) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (mode != 0 && mode != 1) return fail(EFG_ERR_ARG, "search mode must be 0 (plain) or 1 (learned stopping)");
+  if (mode == 1 && !ix->net_set) return fail(EFG_ERR_STATE, "no stopping network set");
+  ix->s_mode = mode;
+  return EFG_OK;
+}
+
+int efg_get_learned_stats(efg_index* ix, uint64_t* last_fallbacks, uint64_t* total_fallbacks,
+                          uint64_t* total_queries) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  int h[3] = {0, 0, 0};
+  if (ix->fb_count.p) {
+    hipMemcpyAsync(h, ix->fb_count.p, sizeof(h), hipMemcpyDeviceToHost, ix->stream);
+    if (int rc = efg::check_hip(hipStreamSynchronize(ix->stream), "copy")) return rc;
+  }
+  if (last_fallbacks) *last_fallbacks = uint64_t(h[0]);
+  if (total_fallbacks) *total_fallbacks = uint64_t(h[1]);
+  if (total_queries) *total_queries = uint64_t(h[2]);
   return EFG_OK;
 }
 

@@ -19,6 +19,8 @@ enum : uint64_t {
   ST_RNN_NEW = 5,   // reverse-list sampling (base_index.hpp:426-429)
   ST_RNN_OLD = 6,   // reverse-list sampling (base_index.hpp:431-434)
   ST_SEARCH = 7,    // search seed top-up (kdtreeub_index.hpp:411-417)
+  ST_MLP_INIT = 8,  // learned stopping network: initial weights
+  ST_MLP_PERM = 9,  // learned stopping network: example order per training epoch
 };
 
 __host__ __device__ inline uint64_t mix64(uint64_t x) {
@@ -29,6 +31,28 @@ __host__ __device__ inline uint64_t mix64(uint64_t x) {
 }
 __host__ __device__ inline uint64_t hash4(uint64_t seed, uint64_t stream, uint64_t a, uint64_t b) {
   return mix64(mix64(mix64(seed ^ (stream * 0xD1B54A32D192ED03ULL)) ^ a) ^ b);
+}
+
+// Bijection of [0, n) keyed by (seed, stream, key): 4-round Feistel network on
+// the next power-of-two domain with cycle walking.
+__host__ __device__ inline uint64_t feistel_perm(uint64_t seed, uint64_t stream, uint64_t key, uint64_t n,
+                                                 uint64_t i) {
+  unsigned bits = 2;
+  while ((1ULL << bits) < n) ++bits;
+  const unsigned h = (bits + 1) / 2;
+  const uint64_t mask = (1ULL << h) - 1;
+  uint64_t x = i;
+  do {
+    uint64_t L = x >> h, R = x & mask;
+    for (uint64_t r = 0; r < 4; ++r) {
+      const uint64_t F = hash4(seed, stream, (key << 8) | r, R) & mask;
+      const uint64_t nL = R, nR = L ^ F;
+      L = nL;
+      R = nR;
+    }
+    x = (L << h) | R;
+  } while (x >= n);
+  return x;
 }
 
 // (distance, id) packed so that unsigned order = (distance, id) order for

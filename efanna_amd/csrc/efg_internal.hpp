@@ -4,6 +4,8 @@
 #include <stdint.h>
 
 #include <string>
+#include <array>
+#include <map>
 #include <vector>
 
 #include "efg.h"
@@ -66,6 +68,8 @@ struct SearchCfg {
   int R;        // pool kept: max(extend_to, K)
   int Rp;       // R rounded up to a power of two
   int kb;       // key buffer (seeds, then pool / new / merge areas)
+  int stop;     // 0 plain; 1 probe search stopped by the network; 2 probe search writing traces
+  int threads;  // threads per query (one workgroup): 64, 128 or 256
   int logcap;   // visited-set capacity (log2)
   uint64_t seed;
   int method;   // 0 nnexp, 1 kgraph
@@ -142,6 +146,18 @@ struct efg_index {
   bool profiling = false;
   std::vector<hipEvent_t> prof_events;
   efg::DevBuf<unsigned long long> work;  // [distances, seed ids read, expanded]
+  // learned stopping: probe search parameters, network, mode
+  int lp_pool = 0, lp_extend = 0, lp_epochs = 0;  // 0 = derived from the search parameters
+  int s_mode = 0;                                // 0 plain, 1 probe search + network + fallback
+  bool net_set = false;
+  float net_zthr = 0.f;                          // stop when the logit reaches this value
+  efg::DevBuf<uint32_t> net_pack;                // BF16 parameters packed for the search kernel
+  efg::DevBuf<float> net_norm;                   // feature mean | 1 / deviation
+  efg::DevBuf<int32_t> fb_list;                  // queries handed to the full search
+  efg::DevBuf<int> fb_count;                     // [fallbacks of the last search, total fallbacks, total learned queries]
+  // threads per query: forced (64/128/256), else tuned per configuration, else a rule of thumb
+  int s_threads = 0;
+  std::map<std::array<int, 6>, int> tuned;  // (init_num, extend_to, epochs, k, trees, stop) -> threads
 };
 
 namespace efg {
@@ -156,6 +172,21 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
 // k_exact.hip
 int exact_knn(const float* X, size_t n, int d, size_t x_stride, const float* Q, int nq, size_t q_stride, int k,
               int exclude_self, int32_t* ids, float* dists, hipStream_t st, std::string* err);
+// Times the search with 64, 128 and 256 threads per query on the host queries
+// (a sample of them) for the current parameters and mode; keeps the fastest.
+int search_autotune(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int* chosen,
+                    hipStream_t st);
+// Probe search traces: per query and checkpoint, features and whether the
+// top k then equals plain_ids (device buffers).
+int search_traces(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, const int32_t* d_plain,
+                  float* d_feat, uint8_t* d_label, hipStream_t st);
+// k_mlp.hip
+int mlp_logits(const float* params, const float* norm, const float* feat, size_t n, float* z, hipStream_t st);
+int mlp_gradient(const float* params, const float* norm, const float* feat, const uint8_t* label, size_t n,
+                 float* grad, float* loss, hipStream_t st);
+int mlp_train(const float* feat, const uint8_t* label, size_t n, const float* norm, float* params, int epochs,
+              int batch, float lr, uint64_t seed, float* loss_hist, hipStream_t st);
+void mlp_init(float* params, uint64_t seed);
 // k_tree.hip
 int gpu_build_trees(efg_index* ix);
 // Exclusive scan of m ints into out[0..m]; out[m] and *total_dev get the total.

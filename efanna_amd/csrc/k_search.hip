@@ -36,15 +36,18 @@
 
 #include "efg_device.hpp"
 #include "efg_internal.hpp"
+#include "efg_mlp.hpp"
 
 namespace efg {
 
 __global__ void k_descend(const float* __restrict__ Q, size_t q_stride, int nq, int T, int lsize,
                           const int32_t* __restrict__ roots, const SNode* __restrict__ sn,
-                          int32_t* __restrict__ leaves) {
+                          int32_t* __restrict__ leaves, const int32_t* __restrict__ qlist,
+                          const int* __restrict__ nlist) {
   const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= size_t(nq) * T) return;
-  const int qi = int(gid / T), t = int(gid % T);
+  const int nw = qlist ? *nlist : nq;  // queries of the list only, when given
+  if (gid >= size_t(nw) * T) return;
+  const int qi = qlist ? qlist[gid / T] : int(gid / T), t = int(gid % T);
   const float* q = Q + size_t(qi) * q_stride;
   int32_t* out = leaves + size_t(qi) * size_t(lsize) * T;  // [ni][t]
   int node = roots[t];
@@ -102,8 +105,10 @@ namespace {
 
 constexpr uint32_t kEmpty = 0xffffffffu;
 constexpr uint32_t kExp = 0x80000000u;
-constexpr int kCtab = 512;
-constexpr int kNB = 512;   // new-candidate buffer between merges
+// Per query, with T threads: the seed chunk is T positions, deduplicated in a
+// 2T-slot hash table; new candidates gather in a 2T-key buffer between merges.
+__host__ __device__ constexpr int ctab_slots(int threads) { return 2 * threads; }
+__host__ __device__ constexpr int new_buf(int threads) { return 2 * threads; }
 constexpr int kKPT = 16;   // seed keys per thread held in registers (init_num <= 4096)
 
 __device__ inline int tab_find(const uint32_t* tab, int logcap, uint32_t id, bool* present) {
@@ -133,7 +138,7 @@ __device__ inline bool tab_insert(uint32_t* tab, int logcap, uint32_t id) {
 
 struct Smem {
   float* q;
-  uint64_t* buf;  // seeds; then pool P[Rp] | new N[kNB] | merge output O[Rp]
+  uint64_t* buf;  // seeds; then pool P[Rp] | new N[new_buf] | merge output O[Rp]
   uint32_t* tab;
   uint32_t* ctab_id;
   uint32_t* ctab_pos;
@@ -149,15 +154,15 @@ struct Smem {
 __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
 // Block radix select over unique keys buf[0, m): returns T with exactly R
-// keys <= T.
+// keys <= T.  Any block of at least 64 threads.
 __device__ uint64_t block_select(const uint64_t* buf, int m, int R, uint32_t* hist, int* misc) {
   const int tid = threadIdx.x;
   uint64_t prefix = 0, mask = 0;
   int need = R;
   for (int shift = 56; shift >= 0; shift -= 8) {
-    hist[tid] = 0;
+    for (int i = tid; i < 256; i += blockDim.x) hist[i] = 0;
     __syncthreads();
-    for (int i = tid; i < m; i += kBlock) {
+    for (int i = tid; i < m; i += blockDim.x) {
       uint64_t k = buf[i];
       if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
     }
@@ -317,10 +322,17 @@ __host__ __device__ inline size_t search_lds_bytes(const SearchCfg& c, bool glob
   b += a16(sizeof(float) * c.D);
   b += a16(sizeof(uint64_t) * c.kb);
   if (!global_table) b += a16(sizeof(uint32_t) << c.logcap);
-  b += a16(sizeof(uint32_t) * kCtab) * 2;
+  b += a16(sizeof(uint32_t) * ctab_slots(c.threads)) * 2;
   b += a16(sizeof(int32_t) * (c.nleaf + 1)) + a16(sizeof(int32_t) * c.nleaf);
   b += a16(sizeof(uint32_t) * c.E) + a16(sizeof(int32_t) * (c.E + 1));
   b += a16(sizeof(uint32_t) * 256) + a16(sizeof(int) * 4) + a16(sizeof(int) * 8);
+  if (c.stop) {
+    b += a16(sizeof(int32_t) * c.K);            // top k ids of the last checkpoint
+    b += a16(sizeof(float) * (kFeat + 2 * kHid + 8 + kHid + 1 + 2 * kFeat));  // features, hidden units,
+                                                                             // history, W3 | b3, norm
+    if (c.stop == 1) b += a16(sizeof(uint32_t) * (kHid / 2) * kHid);            // W2, packed BF16 pairs
+
+  }
   if (c.method == 1) {
     const int kc = c.K + c.E + 1;
     b += a16(4 * size_t(kc)) * 2 + a16(size_t(kc));            // knn: dist, id, flag
@@ -331,8 +343,21 @@ __host__ __device__ inline size_t search_lds_bytes(const SearchCfg& c, bool glob
   return b;
 }
 
-template <bool kGlobal, int kMethod, bool kBf16>
-__global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const float* __restrict__ Q, size_t q_stride,
+// Probe search state for the learned stopping rule (c.stop != 0).
+struct LearnArgs {
+  const uint32_t* wpack;     // network: packed BF16 columns, then W3 | b3 as floats
+  const float* norm;         // feature mean | 1 / deviation
+  float zthr;                // stop when the logit reaches zthr
+  int32_t* fb_list;          // stop 1: queries left to the full search
+  int* fb_count;             // stop 1: [count, running total of fallbacks, running total of queries]
+  float* tr_feat;            // stop 2: [query][checkpoint][kFeat]
+  uint8_t* tr_label;         // stop 2: [query][checkpoint], top k ids found in plain_ids (254 at most), 255 = not reached
+  const int32_t* plain_ids;  // stop 2: [query][K]
+};
+
+template <bool kGlobal, int kMethod, bool kBf16, int kStop, int kThreads>
+__global__ void __launch_bounds__(kThreads) k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
+                                                         size_t q_stride,
                                                          const float* __restrict__ X, const uint16_t* __restrict__ Xb,
                                                          const int32_t* __restrict__ leaves,
                                                          const SNode* __restrict__ sn,
@@ -347,6 +372,9 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
                                                          int* __restrict__ noverflow,
                                                          unsigned long long* __restrict__ work) {
   const int nlist = nlist_dev ? *nlist_dev : nlist_host;
+  constexpr int kCtab = ctab_slots(kThreads), kNB = new_buf(kThreads);
+  constexpr int kCtabLog = kThreads == 64 ? 7 : (kThreads == 128 ? 8 : 9);
+  static_assert((1 << kCtabLog) == kCtab, "chunk table size");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Smem s;
   {
@@ -369,6 +397,33 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
     s.hist = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * 256));
     s.scan = reinterpret_cast<int*>(take(sizeof(int) * 4));
     s.misc = reinterpret_cast<int*>(take(sizeof(int) * 8));
+  }
+  // learned stopping working arrays (after the common layout)
+  int32_t* prevk = nullptr;
+  float* ckf = nullptr;  // features [kFeat] | h1 [kHid] | h2 [kHid] | history [8] | W3 | b3 | norm
+  uint32_t* w2s = nullptr;  // stop 1: W2 rows (2i, 2i+1) of hidden unit j at [i * kHid + j]
+  if (kStop) {
+    SearchCfg c0 = c;
+    c0.stop = 0;
+    c0.method = 0;
+    size_t o = search_lds_bytes(c0, kGlobal);
+    auto take = [&](size_t bytes) {
+      unsigned char* p = smem_raw + o;
+      o += align16(bytes);
+      return p;
+    };
+    prevk = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * c.K));
+    ckf = reinterpret_cast<float*>(take(sizeof(float) * (kFeat + 2 * kHid + 8 + kHid + 1 + 2 * kFeat)));
+    if (kStop == 1) w2s = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * (kHid / 2) * kHid));
+  }
+  float* w3s = ckf ? ckf + kFeat + 2 * kHid + 8 : nullptr;  // W3 | b3
+  float* nrm = w3s ? w3s + kHid + 1 : nullptr;              // feature mean | 1 / deviation
+  if (kStop == 1) {
+    const float* w3 = reinterpret_cast<const float*>(la.wpack + kPackLen);
+    for (int i = threadIdx.x; i <= kHid; i += kThreads) w3s[i] = w3[i];
+    for (int i = threadIdx.x; i < 2 * kFeat; i += kThreads) nrm[i] = la.norm[i];
+    for (int i = threadIdx.x; i < (kHid / 2) * kHid; i += kThreads) w2s[i] = la.wpack[(kFeat / 2) * kHid + i];
+    __syncthreads();
   }
   // method 1 (kgraph) working arrays
   float* kd = nullptr;
@@ -409,7 +464,8 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
 
   // distances for keys a[from, to) whose low 32 bits hold ids
   auto distances = [&](uint64_t* a, int from, int to) {
-    for (int i = from + pr; i < to; i += kBlock / 2) {
+#pragma unroll 2
+    for (int i = from + pr; i < to; i += kThreads / 2) {
       uint32_t id = key_id(a[i]);
       float dd = kBf16 ? l2_pair_lanes_bf16(s.q, Xb + size_t(id) * c.D, c.D, h)
                        : l2_pair_lanes(s.q, X + size_t(id) * c.D, c.D, h);
@@ -418,7 +474,7 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
   };
   // merge N[0, m) (unique, unsorted, disjoint from P) into the sorted pool P[0, R)
   auto merge_new = [&](int m) {
-    for (int i = tid; i < m; i += kBlock) {
+    for (int i = tid; i < m; i += kThreads) {
       uint64_t key = N[i];
       int lo = 0, hi = R;  // number of pool keys below key
       while (lo < hi) {
@@ -429,15 +485,147 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       for (int j = 0; j < m; ++j) rn += (N[j] < key) ? 1 : 0;
       if (lo + rn < R) O[lo + rn] = key;
     }
-    for (int i = tid; i < R; i += kBlock) {
+    for (int i = tid; i < R; i += kThreads) {
       uint64_t key = P[i];
       int rn = 0;
       for (int j = 0; j < m; ++j) rn += (N[j] < key) ? 1 : 0;
       if (i + rn < R) O[i + rn] = key;
     }
     __syncthreads();
-    for (int i = tid; i < R; i += kBlock) P[i] = O[i];
+    for (int i = tid; i < R; i += kThreads) P[i] = O[i];
     __syncthreads();
+  };
+
+  // Probe-search checkpoint (all threads): features of the sorted pool P;
+  // stop 2 writes them with the label, stop 1 evaluates the network and
+  // returns true when its logit reaches la.zthr.  e: checkpoint index,
+  // nexp_last: ids expanded in the last epoch, visited: ids measured so far.
+  auto checkpoint = [&](int qi, int e, int nexp_last, int visited) -> bool {
+    // stop 1: thread j < 64 loads its packed W1 column and biases now (W2 is
+    // in LDS); the loads complete while the pool statistics are gathered
+    constexpr int kW1Words = kFeat / 2;
+    uint32_t wreg[kW1Words + 1];
+    if (kStop == 1 && tid < kHid) {
+#pragma unroll
+      for (int i = 0; i < kW1Words; ++i) wreg[i] = la.wpack[i * kHid + tid];
+      wreg[kW1Words] = la.wpack[(kPackWords - 1) * kHid + tid];
+    }
+    const int ER = c.E < R ? c.E : R;
+    int fr = 0;  // unexpanded ids among the first extend_to
+    for (int b0 = 0; b0 < ER; b0 += kThreads) {
+      const int j = b0 + tid;
+      int v = 0;
+      if (j < ER) {
+        bool pres;
+        const int sl = tab_find(s.tab, c.logcap, key_id(P[j]), &pres);
+        v = (s.tab[sl] & kExp) ? 0 : 1;
+      }
+      int tot;
+      block_exclusive_scan(v, s.scan, &tot);
+      fr += tot;
+    }
+    int ch = c.K;  // top k ids that were not there at the last checkpoint
+    if (e > 0) {
+      ch = 0;
+      for (int b0 = 0; b0 < c.K; b0 += kThreads) {
+        const int j = b0 + tid;
+        int v = 0;
+        if (j < c.K) {
+          const int32_t id = int32_t(key_id(P[j]));
+          v = 1;
+          for (int t2 = 0; t2 < c.K; ++t2)
+            if (prevk[t2] == id) {
+              v = 0;
+              break;
+            }
+        }
+        int tot;
+        block_exclusive_scan(v, s.scan, &tot);
+        ch += tot;
+      }
+    }
+    int match = 0;  // stop 2: top k ids that are in the plain result's top k
+    if (kStop == 2) {
+      const int32_t* pl = la.plain_ids + size_t(qi) * c.K;
+      for (int b0 = 0; b0 < c.K; b0 += kThreads) {
+        const int j = b0 + tid;
+        int v = 0;
+        if (j < c.K) {
+          const int32_t id = int32_t(key_id(P[j]));
+          for (int t2 = 0; t2 < c.K; ++t2)
+            if (pl[t2] == id) {
+              v = 1;
+              break;
+            }
+        }
+        int tot;
+        block_exclusive_scan(v, s.scan, &tot);
+        match += tot;
+      }
+    }
+    for (int j = tid; j < c.K; j += kThreads) prevk[j] = int32_t(key_id(P[j]));
+    float* hist = ckf + kFeat + 2 * kHid;  // k-th [0..2] and best [3..5] distance, latest first
+    if (tid == 0) {
+      CkptState st;
+      st.e = e;
+      st.d1 = key_dist(P[0]);
+      st.dk = key_dist(P[c.K - 1]);
+      st.dE = key_dist(P[(ER > 0 ? ER : 1) - 1]);
+      st.dR = key_dist(P[R - 1]);
+      for (int j = 0; j < 3; ++j) {
+        st.dk_hist[j] = hist[j];
+        st.d1_hist[j] = hist[3 + j];
+      }
+      st.visited = float(visited);
+      st.expanded = float(nexp_last);
+      st.changed = float(ch);
+      st.frontier = float(fr);
+      ckpt_features(st, ckf);
+      hist[2] = hist[1];
+      hist[1] = hist[0];
+      hist[0] = st.dk;
+      hist[5] = hist[4];
+      hist[4] = hist[3];
+      hist[3] = st.d1;
+    }
+    __syncthreads();
+    if (kStop == 2) {
+      const size_t slot = size_t(qi) * (c.epochs + 1) + e;
+      if (tid < kFeat) la.tr_feat[slot * kFeat + tid] = ckf[tid];
+      if (tid == 0) la.tr_label[slot] = uint8_t(match < 254 ? match : 254);
+      return false;
+    }
+    float* h1 = ckf + kFeat;
+    float* h2 = h1 + kHid;
+    auto lo = [](uint32_t v) { return __uint_as_float(v << 16); };
+    auto hi = [](uint32_t v) { return __uint_as_float(v & 0xffff0000u); };
+    if (tid < kHid) {
+      float a = lo(wreg[kW1Words]);
+#pragma unroll
+      for (int i = 0; i < kFeat; ++i)
+        a = __builtin_fmaf((i & 1) ? hi(wreg[i / 2]) : lo(wreg[i / 2]), mlp_input(ckf, nrm, i), a);
+      h1[tid] = a > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    if (tid < kHid) {
+      float a = hi(wreg[kW1Words]);
+      for (int i = 0; i < kHid; i += 2) {
+        const uint32_t v = w2s[(i / 2) * kHid + tid];
+        a = __builtin_fmaf(lo(v), h1[i], a);
+        a = __builtin_fmaf(hi(v), h1[i + 1], a);
+      }
+      h2[tid] = a > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      float z = w3s[kHid];
+      for (int j = 0; j < kHid; ++j) z = __builtin_fmaf(w3s[j], h2[j], z);
+      s.misc[7] = z >= la.zthr ? 1 : 0;
+    }
+    __syncthreads();
+    const bool stop = s.misc[7] != 0;
+    __syncthreads();
+    return stop;
   };
 
   // work distribution: slices of the query order per XCD
@@ -448,14 +636,16 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
     wend = int((long long)nlist * (x + 1) / 8);
     wstep = gridDim.x >> 3;
   }
+  unsigned long long acc_nd = 0, acc_seed = 0, acc_exp = 0;  // thread 0: this workgroup's counters
+  int acc_q = 0, acc_fb = 0;
   for (int w = w0; w < wend; w += wstep) {
     const int qi = qlist ? qlist[w] : w;
     const float* qg = Q + size_t(qi) * q_stride;
-    for (int i = tid; i < c.D; i += kBlock) s.q[i] = (i < c.d) ? qg[i] : 0.f;
-    for (int i = tid; i < cap; i += kBlock) s.tab[i] = kEmpty;
+    for (int i = tid; i < c.D; i += kThreads) s.q[i] = (i < c.d) ? qg[i] : 0.f;
+    for (int i = tid; i < cap; i += kThreads) s.tab[i] = kEmpty;
     // leaf sequence: sizes and prefix in (rank, tree) order
     int run = 0;
-    for (int b0 = 0; b0 < c.nleaf; b0 += kBlock) {
+    for (int b0 = 0; b0 < c.nleaf; b0 += kThreads) {
       int li = b0 + tid;
       int sz = 0;
       if (li < c.nleaf) {
@@ -479,14 +669,14 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
     // ---- seed pool -------------------------------------------------------
     int p = 0;  // kept so far (uniform)
     auto seed_chunk = [&](bool valid, uint32_t id, uint32_t sq) {
-      for (int i = tid; i < kCtab; i += kBlock) { s.ctab_id[i] = kEmpty; s.ctab_pos[i] = kEmpty; }
+      for (int i = tid; i < kCtab; i += kThreads) { s.ctab_id[i] = kEmpty; s.ctab_pos[i] = kEmpty; }
       __syncthreads();
       bool present = false;
       int cs = -1;
       if (valid) {
         tab_find(s.tab, c.logcap, id, &present);
         if (!present) {
-          uint32_t sl = hslot(id, 9);
+          uint32_t sl = hslot(id, kCtabLog);
           for (;;) {
             uint32_t v = s.ctab_id[sl];
             if (v == kEmpty) {
@@ -513,23 +703,26 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       p = min(c.init_num, p + cnt);
     };
     int seq_read = 0;
-    for (int b0 = 0; b0 < total && p < c.init_num; b0 += kBlock) {
-      seq_read = min(total, b0 + kBlock);
-      int sq = b0 + tid;
-      bool valid = sq < total;
-      uint32_t id = 0;
-      if (valid) {
-        int lo = 0, hi = c.nleaf;  // largest li with lpos[li] <= sq
-        while (hi - lo > 1) {
-          int mid = (lo + hi) >> 1;
-          if (s.lpos[mid] <= sq) lo = mid; else hi = mid;
-        }
-        int t = lo % c.T;
-        id = leaf_lists[size_t(t) * c.n + s.lstart[lo] + (sq - s.lpos[lo])];
+    auto seed_id = [&](int sq) -> uint32_t {  // id at position sq of the leaf sequence
+      int lo = 0, hi = c.nleaf;  // largest li with lpos[li] <= sq
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (s.lpos[mid] <= sq) lo = mid; else hi = mid;
       }
+      const int t = lo % c.T;
+      return leaf_lists[size_t(t) * c.n + s.lstart[lo] + (sq - s.lpos[lo])];
+    };
+    // the next chunk's ids are loaded before this chunk is processed
+    uint32_t id_next = tid < total ? seed_id(tid) : 0u;
+    for (int b0 = 0; b0 < total && p < c.init_num; b0 += kThreads) {
+      seq_read = min(total, b0 + kThreads);
+      const int sq = b0 + tid;
+      const bool valid = sq < total;
+      const uint32_t id = id_next;
+      if (sq + kThreads < total) id_next = seed_id(sq + kThreads);
       seed_chunk(valid, id, uint32_t(sq));
     }
-    for (uint32_t r0 = 0; p < c.init_num; r0 += kBlock) {
+    for (uint32_t r0 = 0; p < c.init_num; r0 += kThreads) {
       uint32_t id = uint32_t(hash4(c.seed, ST_SEARCH, uint64_t(qi), r0 + tid) % uint64_t(c.n));
       seed_chunk(true, id, r0 + tid);
     }
@@ -546,7 +739,7 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       int nm = 0;
 #pragma unroll
       for (int j = 0; j < kKPT; ++j) {
-        const int i = tid + j * kBlock;
+        const int i = tid + j * kThreads;
         mine[j] = (i < c.init_num) ? s.buf[i] : kKeyMax;
         nm += (i < c.init_num && mine[j] <= thr) ? 1 : 0;
       }
@@ -554,20 +747,22 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       int o = block_exclusive_scan(nm, s.scan, &tot);  // barriers inside: all reads precede the writes
 #pragma unroll
       for (int j = 0; j < kKPT; ++j) {
-        const int i = tid + j * kBlock;
+        const int i = tid + j * kThreads;
         if (i < c.init_num && mine[j] <= thr) P[o++] = mine[j];
       }
-      for (int i = R + tid; i < Rp; i += kBlock) P[i] = kKeyMax;
+      for (int i = R + tid; i < Rp; i += kThreads) P[i] = kKeyMax;
       __syncthreads();
       block_bitonic_sort(P, Rp);
     }
     uint64_t ndq = uint64_t(c.init_num);
     bool ovf = false;
+    bool stopped = false;
+    if (kStop) stopped = checkpoint(qi, 0, 0, inserted);
 
     int kg_valid = 0;  // method 1: valid result entries
     if (kMethod == 1) {
       // ---- method 1: KGraph-style expansion (kdtreeub_index.hpp:572-648) -----
-      for (int i = tid; i < cap; i += kBlock) s.tab[i] = kEmpty;  // flags.reset() (572)
+      for (int i = tid; i < cap; i += kThreads) s.tab[i] = kEmpty;  // flags.reset() (572)
       __syncthreads();
       const uint64_t* pool = (c.sorted) ? P : s.buf;
       if (tid < 64) {
@@ -686,10 +881,10 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       ndq = uint64_t(c.init_num) + uint64_t(s.misc[3]);
     }
     // ---- epochs (method 0) ---------------------------------------------------
-    for (int ep = 0; kMethod == 0 && ep < c.epochs; ++ep) {
+    for (int ep = 0; kMethod == 0 && ep < c.epochs && !stopped; ++ep) {
       // expansion list: unexpanded ids among the first extend_to (448-451)
       int nexp = 0;
-      for (int b0 = 0; b0 < c.E; b0 += kBlock) {
+      for (int b0 = 0; b0 < c.E; b0 += kThreads) {
         int j = b0 + tid;
         int take = 0;
         uint32_t id = 0;
@@ -712,7 +907,7 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       if (nexp == 0) break;
       nexp_total += uint64_t(nexp);
       int nnb = 0;
-      for (int b0 = 0; b0 < nexp; b0 += kBlock) {
+      for (int b0 = 0; b0 < nexp; b0 += kThreads) {
         int j = b0 + tid;
         int deg = 0;
         if (j < nexp) {
@@ -730,7 +925,7 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       }
       __syncthreads();
       int m = 0;
-      for (int b0 = 0; b0 < nnb; b0 += kBlock) {
+      for (int b0 = 0; b0 < nnb; b0 += kThreads) {
         int sq = b0 + tid;
         if (sq < nnb) {
           int lo = 0, hi = nexp;
@@ -746,12 +941,12 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
         const int m2 = s.misc[0];
         inserted += m2 - m;
         m = m2;
-        if (inserted + kBlock > limit) {
+        if (inserted + kThreads > limit) {
           ovf = true;
           break;
         }
-        const bool last = b0 + kBlock >= nnb;
-        if (m > 0 && (last || m + kBlock > kNB)) {
+        const bool last = b0 + kThreads >= nnb;
+        if (m > 0 && (last || m + kThreads > kNB)) {
           distances(N, 0, m);
           ndq += uint64_t(m);
           __syncthreads();
@@ -762,27 +957,40 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
         }
       }
       if (ovf) break;
+      if (kStop) stopped = checkpoint(qi, ep + 1, nexp, inserted);
     }
     if (kBf16 && kMethod == 0 && !ovf) {
       // re-rank the best min(2K, R) by direct FP32 difference
       const int rr = (2 * c.K < c.R) ? 2 * c.K : c.R;
       const int p2 = next_pow2(rr < 2 ? 2 : rr);
-      for (int i = pr; i < rr; i += kBlock / 2) {
+      for (int i = pr; i < rr; i += kThreads / 2) {
         const uint32_t id = key_id(P[i]);
         const float dd = l2_pair_lanes(s.q, X + size_t(id) * c.D, c.D, h);
         if (h) O[i] = make_key(dd, id);
       }
-      for (int i = rr + tid; i < p2; i += kBlock) O[i] = kKeyMax;
+      for (int i = rr + tid; i < p2; i += kThreads) O[i] = kKeyMax;
       __syncthreads();
       block_bitonic_sort(O, p2);
-      for (int i = tid; i < rr; i += kBlock) P[i] = O[i];
+      for (int i = tid; i < rr; i += kThreads) P[i] = O[i];
       __syncthreads();
       ndq += uint64_t(rr);
     }
-    if (ovf) {
-      if (tid == 0) overflow[atomicAdd(noverflow, 1)] = qi;
+    // stop 1: queries the network did not stop (or whose visited set
+    // overflowed) go to the full search, which writes their results
+    const bool to_full = kStop == 1 && (ovf || !stopped);
+    if (kStop == 1 && tid == 0) {
+      ++acc_q;
+      if (to_full) {
+        la.fb_list[atomicAdd(la.fb_count, 1)] = qi;
+        ++acc_fb;
+      }
+    }
+    if (to_full) {
+      acc_nd += ndq;
+    } else if (ovf) {
+      if (kStop == 0 && tid == 0) overflow[atomicAdd(noverflow, 1)] = qi;
     } else {
-      for (int j = tid; j < c.K; j += kBlock) {
+      for (int j = tid; j < c.K; j += kThreads) {
         if (kMethod == 0) {
           uint64_t k = P[j];
           out_ids[size_t(qi) * c.K + j] = int32_t(key_id(k));
@@ -795,17 +1003,30 @@ __global__ void __launch_bounds__(kBlock) k_search_nnexp(SearchCfg c, const floa
       }
       if (tid == 0) {
         if (ndist_q) ndist_q[qi] = ndq;
-        atomicAdd(ndist_total, (unsigned long long)ndq);
-        atomicAdd(ndist_total + 1, (unsigned long long)seeds_read);
-        atomicAdd(ndist_total + 2, (unsigned long long)nexp_total);
-        if (work) {
-          atomicAdd(work, (unsigned long long)ndq);
-          atomicAdd(work + 1, (unsigned long long)seeds_read);
-          atomicAdd(work + 2, (unsigned long long)nexp_total);
-        }
+        acc_nd += ndq;
+        acc_seed += seeds_read;
+        acc_exp += nexp_total;
       }
     }
     __syncthreads();
+  }
+  // counters: one atomic per workgroup (same-address atomics per query would
+  // serialise the whole grid)
+  if (tid == 0) {
+    if (acc_nd | acc_seed | acc_exp) {
+      atomicAdd(ndist_total, acc_nd);
+      atomicAdd(ndist_total + 1, acc_seed);
+      atomicAdd(ndist_total + 2, acc_exp);
+      if (work) {
+        atomicAdd(work, acc_nd);
+        atomicAdd(work + 1, acc_seed);
+        atomicAdd(work + 2, acc_exp);
+      }
+    }
+    if (kStop == 1 && acc_q) {
+      atomicAdd(la.fb_count + 1, acc_fb);
+      atomicAdd(la.fb_count + 2, acc_q);
+    }
   }
 }
 
@@ -875,29 +1096,27 @@ int build_search_layout(efg_index* ix) {
 
 // knnSearch on device buffers: method 0 (getNeighbors_nnexp,
 // kdtreeub_index.hpp:363-494) or method 1 (getNeighbors_kgraph, 496-654).
-int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int32_t* d_ids, float* d_dists,
-                  uint64_t* d_ndist, hipStream_t st) {
-  if (!ix->s_set) return fail(EFG_ERR_STATE, "search parameters not set (setSearchParams)");
-  const int method = ix->s_method;
-  if (method != 0 && method != 1) return fail(EFG_ERR_METHOD, "no such searching method");
-  if (ix->T == 0) return fail(EFG_ERR_STATE, "no trees: build or load trees first");
-  if (ix->nnz == 0) return fail(EFG_ERR_STATE, "no graph: build or load a graph first");
-  if (ix->s_trees <= 0) return fail(EFG_ERR_ARG, "search tree number must be positive");
-  if (ix->s_trees > ix->T) return fail(EFG_ERR_TREES, "wrong tree number");
-  if (nq == 0) return EFG_OK;
-  if (nq > (size_t(1) << 31)) return fail(EFG_ERR_ARG, "too many queries");
-  if (q_stride < ix->d) return fail(EFG_ERR_ARG, "query stride below the dimension");
-  int init = ix->s_init;
-  const int E = ix->s_extend, epochs = ix->s_epochs;
+namespace {
+
+// Threads per query and the key buffer that depends on it (seeds, or pool +
+// new-candidate buffer + merge output).
+void set_threads(SearchCfg& c, int t) {
+  c.threads = t;
+  c.kb = std::max(c.init_num, 2 * c.Rp + new_buf(t));
+}
+
+// Search configuration from (k, init_num, extend_to, epochs, trees, method)
+// with the reference's checks (kdtreeub_index.hpp:363-392, 496-514).
+int make_cfg(efg_index* ix, int k, int init, int E, int epochs, int trees, int method, int stop, SearchCfg* out,
+             int* init_eff) {
   if (init <= 0 || E < 0 || epochs < 0) return fail(EFG_ERR_ARG, "invalid search parameters");
-  const int lsize = init * 2 / (5 * ix->s_trees) + 1;  // before method 1 raises init_num (507)
+  const int lsize = init * 2 / (5 * trees) + 1;  // before method 1 raises init_num (507)
   int sorted = 1, R = E > k ? E : k;
   if (method == 1) {
     if (E <= 0) return fail(EFG_ERR_ARG, "method 1 needs extend_to > 0");
     const int pool_size = epochs * E;  // kdtreeub_index.hpp:510-514: init_num raised for good
     if (pool_size >= init) {
       init = pool_size;
-      ix->s_init = pool_size;
       sorted = 0;
     }
     R = pool_size;
@@ -906,12 +1125,12 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
     return fail(EFG_ERR_ARG, "k must be positive and at most the pool size");
   }
   if (size_t(init) > ix->n) return fail(EFG_ERR_ARG, "pool size larger than the data set");
-  if (init > kKPT * kBlock) return fail(EFG_ERR_ARG, "pool size above 4096 is not supported");
+  if (init > kKPT * 256) return fail(EFG_ERR_ARG, "pool size above 4096 is not supported");
   SearchCfg c{};
   c.n = int(ix->n);
   c.d = int(ix->d);
   c.D = int(ix->D);
-  c.T = ix->s_trees;
+  c.T = trees;
   c.lsize = lsize;
   c.nleaf = c.T * c.lsize;
   c.init_num = init;
@@ -922,104 +1141,322 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
   c.Rp = next_pow2(R < 2 ? 2 : R);
   c.method = method;
   c.sorted = sorted;
-  c.kb = std::max(init, 2 * c.Rp + kNB);
+  set_threads(c, 256);
   c.logcap = ceil_log2(size_t(init + 512) * 4 / 3 + 1);
   if (c.logcap < 10) c.logcap = 10;
   c.seed = ix->bp.seed;
-  size_t lds = search_lds_bytes(c, false);
-  const size_t lds_cap = 150 * 1024;
-  bool table_in_lds = lds <= lds_cap;
-  if (!table_in_lds && search_lds_bytes(c, true) > lds_cap) return fail(EFG_ERR_ARG, "pool too large for one workgroup");
-  if (!ix->snode_valid) {
-    if (int rc = check_hip(hipStreamSynchronize(st), "sync")) return rc;
-    if (int rc = build_search_layout(ix)) return rc;
-  }
-  const bool bf16 = method == 0 && ix->s_precision == 1;
-  if (bf16 && !ix->data_bf16.p) {
-    if (!ix->data_bf16.alloc(ix->n * ix->D)) return fail(EFG_ERR_MEMORY, "out of device memory (bf16 rows)");
-    const size_t tot = ix->n * ix->D;
-    k_pack_bf16<<<unsigned((tot + 255) / 256), 256, 0, st>>>(ix->data.p, int(ix->D), ix->n, ix->data_bf16.p);
-  }
+  c.stop = stop;
+  *out = c;
+  if (init_eff) *init_eff = init;
+  return EFG_OK;
+}
+
+using SearchKernel = void (*)(SearchCfg, LearnArgs, const float*, size_t, const float*, const uint16_t*,
+                              const int32_t*, const SNode*, const uint32_t*, const uint64_t*, const uint32_t*,
+                              int32_t*, float*, uint64_t*, unsigned long long*, const int32_t*, int, const int*, int,
+                              uint32_t*, int32_t*, int*, unsigned long long*);
+
+// One search pass over the queries of qlist (all nq, in leaf order, when
+// qlist is null; *nlist_dev of them otherwise): descent, then the search
+// kernel with the visited table in LDS, then the queries whose table
+// overflowed again with a table in global memory.
+int run_pass(efg_index* ix, const SearchCfg& c, const LearnArgs& la, bool bf16, const float* dq, size_t nq,
+             size_t q_stride, int32_t* d_ids, float* d_dists, uint64_t* d_ndist, const int32_t* qlist,
+             const int* nlist_dev, hipEvent_t* ev, hipStream_t st, bool count_work = true) {
+  unsigned long long* work = count_work ? ix->work.p : nullptr;
   efg::Workspace& w = ix->ws;
   const int nb_q = 4096;
   if (!w.leaves.ensure(nq * size_t(c.nleaf)) || !w.overflow.ensure(2 * nq) || !w.nover.ensure(2) ||
       !w.ndist.ensure(3) || !w.qorder.ensure(nq) || !w.qkey.ensure(nq) || !w.qhist.ensure(nb_q + 1) ||
       !w.qoff.ensure(nb_q + 1))
     return fail(EFG_ERR_MEMORY, "out of device memory (search workspace)");
+  size_t lds = search_lds_bytes(c, false);
+  const size_t lds_cap = 150 * 1024;
+  const bool table_in_lds = lds <= lds_cap;
+  if (!table_in_lds && search_lds_bytes(c, true) > lds_cap) return fail(EFG_ERR_ARG, "pool too large for one workgroup");
   if (int rc = check_hip(hipMemsetAsync(w.nover.p, 0, 2 * sizeof(int), st), "memset")) return rc;
+  if (ev) hipEventRecord(ev[0], st);
+  {
+    size_t threads = nq * size_t(c.T);
+    k_descend<<<unsigned((threads + 255) / 256), 256, 0, st>>>(dq, q_stride, int(nq), c.T, c.lsize, ix->sroots.p,
+                                                               ix->snode.p, w.leaves.p, qlist, nlist_dev);
+    if (!qlist) {  // query order by first leaf
+      int shift = std::max(0, ceil_log2(ix->n + 1) - 12);
+      hipMemsetAsync(w.qhist.p, 0, size_t(nb_q + 1) * 4, st);
+      k_qkey<<<unsigned((nq + 255) / 256), 256, 0, st>>>(w.leaves.p, c.nleaf, int(nq), ix->snode.p, shift, w.qkey.p,
+                                                         w.qhist.p);
+      if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(w.qhist.p), nb_q, w.qoff.p, w.qoff.p + nb_q,
+                                         w.scan_sums, st))
+        return rc;
+      hipMemsetAsync(w.qhist.p, 0, size_t(nb_q + 1) * 4, st);
+      k_qscatter<<<unsigned((nq + 255) / 256), 256, 0, st>>>(w.qkey.p, int(nq), w.qoff.p, w.qhist.p, w.qorder.p);
+    }
+  }
+  if (ev) hipEventRecord(ev[1], st);
+  int dev = 0, ncu = 256;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const int32_t* order = qlist ? qlist : w.qorder.p;
+  const int tq = c.threads;
+  auto pick = [&](bool global) -> SearchKernel {
+#define EFG_K(G, M, B, S, T) &k_search_nnexp<G, M, B, S, T>
+#define EFG_KT(G, M, B, S) (tq == 64 ? EFG_K(G, M, B, S, 64) : tq == 128 ? EFG_K(G, M, B, S, 128) : EFG_K(G, M, B, S, 256))
+    if (c.method == 1) return global ? EFG_K(true, 1, false, 0, 256) : EFG_K(false, 1, false, 0, 256);
+    if (c.stop == 1) return global ? nullptr : EFG_KT(false, 0, false, 1);
+    if (c.stop == 2) return global ? nullptr : EFG_KT(false, 0, false, 2);
+    if (bf16) return global ? EFG_K(true, 0, true, 0, 256) : EFG_K(false, 0, true, 0, 256);
+    return global ? EFG_K(true, 0, false, 0, 256) : EFG_KT(false, 0, false, 0);
+#undef EFG_KT
+#undef EFG_K
+  };
+  if (table_in_lds) {
+    // workgroups per CU: LDS bound, at most 32 wavefronts
+    int per_cu = int(lds_cap / lds);
+    per_cu = std::max(1, std::min(per_cu, 32 / (tq / 64)));
+    size_t grid = size_t(ncu) * per_cu;
+    if (grid > nq) grid = (nq + 7) & ~size_t(7);
+    auto kern = pick(false);
+    hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    kern<<<unsigned(grid), unsigned(tq), lds, st>>>(c, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p,
+                                                    ix->snode.p, ix->leaf.p, ix->goff.p, ix->gids.p, d_ids, d_dists,
+                                                    d_ndist, w.ndist.p, order, int(nq), nlist_dev, qlist ? 0 : 1,
+                                                    nullptr, w.overflow.p, w.nover.p, work);
+  } else if (c.stop) {
+    return fail(EFG_ERR_ARG, "probe pool too large for one workgroup");
+  }
+  if (ev) hipEventRecord(ev[2], st);
+  if (c.stop == 0) {
+    // queries whose visited set outgrew the LDS table (or all of them when
+    // the table never fits) run again with a per-workgroup table in global
+    // memory
+    SearchCfg c2 = c;
+    set_threads(c2, 256);
+    size_t worst = size_t(c.init_num) + size_t(c.epochs) * size_t(c.E + c.K + 1) *
+                                            size_t(ix->max_deg > 0 ? ix->max_deg : 1) + 512;
+    if (worst > ix->n + 512) worst = ix->n + 512;
+    c2.logcap = ceil_log2(worst * 4 / 3 + 1);
+    if (c2.logcap < c.logcap) c2.logcap = c.logcap;
+    if (c2.logcap > 26) c2.logcap = 26;
+    size_t lds2 = search_lds_bytes(c2, true);
+    unsigned grid2 = 128;
+    if (!w.gtab.ensure(size_t(grid2) << c2.logcap)) return fail(EFG_ERR_MEMORY, "out of device memory (search table)");
+    auto kern2 = pick(true);
+    hipFuncSetAttribute(reinterpret_cast<const void*>(kern2), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2));
+    if (table_in_lds) {
+      kern2<<<grid2, kBlock, lds2, st>>>(c2, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, ix->snode.p,
+                                         ix->leaf.p, ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
+                                         w.overflow.p, 0, w.nover.p, 0, w.gtab.p, w.overflow.p + nq, w.nover.p + 1,
+                                         work);
+    } else {
+      kern2<<<grid2, kBlock, lds2, st>>>(c2, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, ix->snode.p,
+                                         ix->leaf.p, ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
+                                         order, int(nq), nlist_dev, 0, w.gtab.p, w.overflow.p + nq, w.nover.p + 1,
+                                         work);
+    }
+  }
+  if (ev) hipEventRecord(ev[3], st);
+  return check_hip(hipGetLastError(), "search launch");
+}
+
+// Probe search parameters: set explicitly, else pool = max(k, init_num / 6),
+// extend_to = max(1, extend_to / 4), the same epochs.
+void probe_params(const efg_index* ix, int k, int* pool, int* ext, int* ep) {
+  *pool = ix->lp_pool > 0 ? ix->lp_pool : std::max(k, ix->s_init / 6);
+  *ext = ix->lp_extend > 0 ? ix->lp_extend : std::max(1, ix->s_extend / 4);
+  if (*ext > *pool) *ext = *pool;
+  *ep = ix->lp_epochs > 0 ? ix->lp_epochs : ix->s_epochs;
+}
+
+// Threads per query for a configuration: forced, tuned, or by pool size
+// (a wavefront for pools up to 256, two up to 1024, else four); the seed
+// keys held in registers need init_num <= 16 * threads.
+int choose_threads(const efg_index* ix, const SearchCfg& c) {
+  if (c.method == 1) return 256;
+  int t = 0;
+  if (ix->s_threads) {
+    t = ix->s_threads;
+  } else {
+    auto it = ix->tuned.find({c.init_num, c.E, c.epochs, c.K, c.T, c.stop});
+    if (it != ix->tuned.end()) t = it->second;
+    else t = c.init_num <= 256 ? 64 : (c.init_num <= 1024 ? 128 : 256);
+  }
+  while (t < 256 && c.init_num > kKPT * t) t *= 2;
+  return t;
+}
+
+int search_prepare(efg_index* ix, size_t nq, size_t q_stride, hipStream_t st) {
+  if (!ix->s_set) return fail(EFG_ERR_STATE, "search parameters not set (setSearchParams)");
+  if (ix->s_method != 0 && ix->s_method != 1) return fail(EFG_ERR_METHOD, "no such searching method");
+  if (ix->T == 0) return fail(EFG_ERR_STATE, "no trees: build or load trees first");
+  if (ix->nnz == 0) return fail(EFG_ERR_STATE, "no graph: build or load a graph first");
+  if (ix->s_trees <= 0) return fail(EFG_ERR_ARG, "search tree number must be positive");
+  if (ix->s_trees > ix->T) return fail(EFG_ERR_TREES, "wrong tree number");
+  if (nq > (size_t(1) << 31)) return fail(EFG_ERR_ARG, "too many queries");
+  if (q_stride < ix->d) return fail(EFG_ERR_ARG, "query stride below the dimension");
+  if (!ix->snode_valid) {
+    if (int rc = check_hip(hipStreamSynchronize(st), "sync")) return rc;
+    if (int rc = build_search_layout(ix)) return rc;
+  }
+  efg::Workspace& w = ix->ws;
+  if (!w.ndist.ensure(3)) return fail(EFG_ERR_MEMORY, "out of device memory (search workspace)");
   if (int rc = check_hip(hipMemsetAsync(w.ndist.p, 0, 3 * sizeof(unsigned long long), st), "memset")) return rc;
   if (!ix->work.p) {
     if (!ix->work.ensure(3)) return fail(EFG_ERR_MEMORY, "out of device memory (counters)");
     if (int rc = check_hip(hipMemsetAsync(ix->work.p, 0, 3 * sizeof(unsigned long long), st), "memset")) return rc;
   }
+  return EFG_OK;
+}
+
+}  // namespace
+
+int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int32_t* d_ids, float* d_dists,
+                  uint64_t* d_ndist, hipStream_t st) {
+  if (int rc = search_prepare(ix, nq, q_stride, st)) return rc;
+  if (nq == 0) return EFG_OK;
+  const int method = ix->s_method;
+  SearchCfg c;
+  int init_eff = 0;
+  if (int rc = make_cfg(ix, k, ix->s_init, ix->s_extend, ix->s_epochs, ix->s_trees, method, 0, &c, &init_eff))
+    return rc;
+  if (method == 1) ix->s_init = init_eff;
+  const bool bf16 = method == 0 && ix->s_precision == 1;
+  set_threads(c, bf16 ? 256 : choose_threads(ix, c));
+  if (bf16 && !ix->data_bf16.p) {
+    if (!ix->data_bf16.alloc(ix->n * ix->D)) return fail(EFG_ERR_MEMORY, "out of device memory (bf16 rows)");
+    const size_t tot = ix->n * ix->D;
+    k_pack_bf16<<<unsigned((tot + 255) / 256), 256, 0, st>>>(ix->data.p, int(ix->D), ix->n, ix->data_bf16.p);
+  }
+  const bool learned = method == 0 && ix->s_mode == 1;
+  if (learned && !ix->net_set) return fail(EFG_ERR_STATE, "no stopping network set");
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   const bool prof = ix->profiling && ix->prof_events.size() < 4 * 4096;
-  if (prof) {
+  if (prof)
     for (auto& e : ev) hipEventCreate(&e);
-    hipEventRecord(ev[0], st);
-  }
-  {
-    size_t threads = nq * size_t(c.T);
-    k_descend<<<unsigned((threads + 255) / 256), 256, 0, st>>>(dq, q_stride, int(nq), c.T, c.lsize, ix->sroots.p,
-                                                               ix->snode.p, w.leaves.p);
-    // query order by first leaf
-    int shift = std::max(0, ceil_log2(ix->n + 1) - 12);
-    hipMemsetAsync(w.qhist.p, 0, size_t(nb_q + 1) * 4, st);
-    k_qkey<<<unsigned((nq + 255) / 256), 256, 0, st>>>(w.leaves.p, c.nleaf, int(nq), ix->snode.p, shift, w.qkey.p,
-                                                       w.qhist.p);
-    if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(w.qhist.p), nb_q, w.qoff.p, w.qoff.p + nb_q,
-                                       w.scan_sums, st))
+  LearnArgs la{};
+  if (!learned) {
+    if (int rc = run_pass(ix, c, la, bf16, dq, nq, q_stride, d_ids, d_dists, d_ndist, nullptr, nullptr,
+                          prof ? ev : nullptr, st))
       return rc;
-    hipMemsetAsync(w.qhist.p, 0, size_t(nb_q + 1) * 4, st);
-    k_qscatter<<<unsigned((nq + 255) / 256), 256, 0, st>>>(w.qkey.p, int(nq), w.qoff.p, w.qhist.p, w.qorder.p);
-  }
-  if (prof) hipEventRecord(ev[1], st);
-  int dev = 0, ncu = 256;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  if (table_in_lds) {
-    int per_cu = int(lds_cap / lds);
-    if (per_cu < 1) per_cu = 1;
-    size_t grid = size_t(ncu) * per_cu;
-    if (grid > nq) grid = (nq + 7) & ~size_t(7);
-    auto kern = method == 1 ? &k_search_nnexp<false, 1, false>
-                            : (bf16 ? &k_search_nnexp<false, 0, true> : &k_search_nnexp<false, 0, false>);
-    hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-    kern<<<unsigned(grid), kBlock, lds, st>>>(c, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, ix->snode.p, ix->leaf.p,
-                                              ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p, w.qorder.p,
-                                              int(nq), nullptr, 1, nullptr, w.overflow.p, w.nover.p, ix->work.p);
-  }
-  if (prof) hipEventRecord(ev[2], st);
-  // queries whose visited set outgrew the LDS table (or all of them when the
-  // table never fits) run again with a per-workgroup table in global memory
-  SearchCfg c2 = c;
-  size_t worst = size_t(init) + size_t(epochs) * size_t(E + k + 1) * size_t(ix->max_deg > 0 ? ix->max_deg : 1) + 512;
-  if (worst > ix->n + 512) worst = ix->n + 512;
-  c2.logcap = ceil_log2(worst * 4 / 3 + 1);
-  if (c2.logcap < c.logcap) c2.logcap = c.logcap;
-  if (c2.logcap > 26) c2.logcap = 26;
-  size_t lds2 = search_lds_bytes(c2, true);
-  unsigned grid2 = 128;
-  if (!w.gtab.ensure(size_t(grid2) << c2.logcap)) return fail(EFG_ERR_MEMORY, "out of device memory (search table)");
-  auto kern2 = method == 1 ? &k_search_nnexp<true, 1, false>
-                           : (bf16 ? &k_search_nnexp<true, 0, true> : &k_search_nnexp<true, 0, false>);
-  hipFuncSetAttribute(reinterpret_cast<const void*>(kern2), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2));
-  if (table_in_lds) {
-    kern2<<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, ix->snode.p, ix->leaf.p,
-                                                      ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
-                                                      w.overflow.p, 0, w.nover.p, 0, w.gtab.p, w.overflow.p + nq,
-                                                      w.nover.p + 1, ix->work.p);
   } else {
-    kern2<<<grid2, kBlock, lds2, st>>>(c2, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, ix->snode.p, ix->leaf.p,
-                                                      ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
-                                                      w.qorder.p, int(nq), nullptr, 0, w.gtab.p, w.overflow.p + nq,
-                                                      w.nover.p + 1, ix->work.p);
+    // probe search stopped by the network; the rest runs the full search
+    int pp, pe, pep;
+    probe_params(ix, k, &pp, &pe, &pep);
+    SearchCfg cp;
+    if (int rc = make_cfg(ix, k, pp, pe, pep, ix->s_trees, 0, 1, &cp, nullptr)) return rc;
+    set_threads(cp, choose_threads(ix, cp));
+    if (!ix->fb_count.p) {
+      if (!ix->fb_count.ensure(3)) return fail(EFG_ERR_MEMORY, "out of device memory");
+      hipMemsetAsync(ix->fb_count.p, 0, 3 * sizeof(int), st);
+    }
+    if (!ix->fb_list.ensure(nq)) return fail(EFG_ERR_MEMORY, "out of device memory");
+    hipMemsetAsync(ix->fb_count.p, 0, sizeof(int), st);
+    la.wpack = ix->net_pack.p;
+    la.norm = ix->net_norm.p;
+    la.zthr = ix->net_zthr;
+    la.fb_list = ix->fb_list.p;
+    la.fb_count = ix->fb_count.p;
+    if (int rc = run_pass(ix, cp, la, false, dq, nq, q_stride, d_ids, d_dists, d_ndist, nullptr, nullptr, nullptr, st))
+      return rc;
+    if (int rc = run_pass(ix, c, LearnArgs{}, false, dq, nq, q_stride, d_ids, d_dists, d_ndist, ix->fb_list.p,
+                          ix->fb_count.p, prof ? ev : nullptr, st))
+      return rc;
   }
-  if (prof) {
-    hipEventRecord(ev[3], st);
+  if (prof)
     for (auto& e : ev) ix->prof_events.push_back(e);
+  return EFG_OK;
+}
+
+int search_autotune(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int* chosen,
+                    hipStream_t st) {
+  if (int rc = search_prepare(ix, nq, q_stride, st)) return rc;
+  if (nq == 0) return fail(EFG_ERR_ARG, "no queries to tune on");
+  if (ix->s_method != 0) {
+    if (chosen) chosen[0] = chosen[1] = 256;
+    return EFG_OK;
   }
-  return check_hip(hipGetLastError(), "search launch");
+  const bool learned = ix->s_mode == 1;
+  if (learned && !ix->net_set) return fail(EFG_ERR_STATE, "no stopping network set");
+  // the configurations this mode runs: the full search, and the probe when learned
+  SearchCfg cfg[2];
+  int ncfg = 0;
+  if (int rc = make_cfg(ix, k, ix->s_init, ix->s_extend, ix->s_epochs, ix->s_trees, 0, 0, &cfg[ncfg++], nullptr))
+    return rc;
+  if (learned) {
+    int pp, pe, pep;
+    probe_params(ix, k, &pp, &pe, &pep);
+    if (int rc = make_cfg(ix, k, pp, pe, pep, ix->s_trees, 0, 1, &cfg[ncfg++], nullptr)) return rc;
+    if (!ix->fb_list.ensure(nq)) return fail(EFG_ERR_MEMORY, "out of device memory");
+  }
+  DevBuf<int32_t> ids;
+  DevBuf<float> dd;
+  DevBuf<int> tcount;  // fallback counters of the tuning runs (zeroed before each)
+  if (!ids.alloc(nq * size_t(k)) || !dd.alloc(nq * size_t(k)) || !tcount.alloc(3))
+    return fail(EFG_ERR_MEMORY, "out of device memory");
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  int rc = EFG_OK;
+  for (int ci = 0; ci < ncfg && rc == EFG_OK; ++ci) {
+    SearchCfg c = cfg[ci];
+    LearnArgs la{};
+    if (c.stop == 1) {
+      la.wpack = ix->net_pack.p;
+      la.norm = ix->net_norm.p;
+      la.zthr = ix->net_zthr;
+      la.fb_list = ix->fb_list.p;
+      la.fb_count = tcount.p;
+    }
+    float best = 0.f;
+    int best_t = 0;
+    for (int t : {64, 128, 256}) {
+      if (c.init_num > kKPT * t) continue;
+      set_threads(c, t);
+      float ms_sum = 0.f;
+      for (int rep = 0; rep < 3 && rc == EFG_OK; ++rep) {
+        hipMemsetAsync(tcount.p, 0, 3 * sizeof(int), st);
+        hipEventRecord(e0, st);
+        rc = run_pass(ix, c, la, false, dq, nq, q_stride, ids.p, dd.p, nullptr, nullptr, nullptr, nullptr, st, false);
+        hipEventRecord(e1, st);
+        if (rc == EFG_OK) rc = check_hip(hipEventSynchronize(e1), "autotune");
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0) ms_sum += ms;  // the first run warms up
+      }
+      if (rc == EFG_OK && (best_t == 0 || ms_sum < best)) {
+        best = ms_sum;
+        best_t = t;
+      }
+    }
+    if (rc == EFG_OK) {
+      ix->tuned[{c.init_num, c.E, c.epochs, c.K, c.T, c.stop}] = best_t;
+      if (chosen) chosen[ci] = best_t;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (ncfg == 1 && chosen) chosen[1] = 0;
+  return rc;
+}
+
+int search_traces(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, const int32_t* d_plain,
+                  float* d_feat, uint8_t* d_label, hipStream_t st) {
+  if (int rc = search_prepare(ix, nq, q_stride, st)) return rc;
+  if (nq == 0) return EFG_OK;
+  int pp, pe, pep;
+  probe_params(ix, k, &pp, &pe, &pep);
+  SearchCfg cp;
+  if (int rc = make_cfg(ix, k, pp, pe, pep, ix->s_trees, 0, 2, &cp, nullptr)) return rc;
+  set_threads(cp, choose_threads(ix, cp));
+  DevBuf<int32_t> ids;
+  if (!ids.alloc(nq * size_t(k))) return fail(EFG_ERR_MEMORY, "out of device memory");
+  hipMemsetAsync(d_label, 0xff, nq * size_t(pep + 1), st);
+  LearnArgs la{};
+  la.tr_feat = d_feat;
+  la.tr_label = d_label;
+  la.plain_ids = d_plain;
+  if (int rc = run_pass(ix, cp, la, false, dq, nq, q_stride, ids.p, nullptr, nullptr, nullptr, nullptr, nullptr, st))
+    return rc;
+  return check_hip(hipStreamSynchronize(st), "traces");
 }
 
 }  // namespace efg

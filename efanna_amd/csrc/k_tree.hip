@@ -34,30 +34,11 @@ namespace {
 
 constexpr int kPartB = 1024;  // ids per partition block (4 per thread)
 
-__device__ __host__ inline uint64_t feistel_perm(uint64_t seed, uint64_t tree, uint64_t n, uint64_t i) {
-  unsigned bits = 2;
-  while ((1ULL << bits) < n) ++bits;
-  unsigned h = (bits + 1) / 2;
-  uint64_t mask = (1ULL << h) - 1;
-  uint64_t x = i;
-  do {
-    uint64_t L = x >> h, R = x & mask;
-    for (uint64_t r = 0; r < 4; ++r) {
-      uint64_t F = hash4(seed, ST_PERM, (tree << 8) | r, R) & mask;
-      uint64_t nL = R, nR = L ^ F;
-      L = nL;
-      R = nR;
-    }
-    x = (L << h) | R;
-  } while (x >= n);
-  return x;
-}
-
 __global__ void k_init_perm(uint32_t* __restrict__ leaf, int T, uint32_t n, uint64_t seed) {
   size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= size_t(T) * n) return;
   uint32_t t = uint32_t(gid / n), i = uint32_t(gid % n);
-  leaf[gid] = uint32_t(feistel_perm(seed, t, n, i));
+  leaf[gid] = uint32_t(feistel_perm(seed, ST_PERM, t, n, i));
 }
 
 __global__ void k_init_roots(int4* node4, int32_t* parent, uint2* range, int32_t* roots, int T, uint32_t n) {

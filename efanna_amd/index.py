@@ -105,6 +105,7 @@ class FIndex:
             raise EfgError(-2, "only the KDTREE_UB index is supported")
         m = features if isinstance(features, Matrix) else Matrix(*np.asarray(features).shape, features)
         self.n, self.d = m.rows, m.cols
+        self._probe_epochs = None
         bp = BuildParams()
         lib().efg_default_build_params(ctypes.byref(bp), 4)
         ep = params.extra_params
@@ -150,6 +151,54 @@ class FIndex:
     def setSearchPrecision(self, precision: int):
         """0 = FP32 rows; 1 = BF16 rows with FP32 re-rank of the best 2k."""
         check(lib().efg_set_search_precision(self.h, int(precision)))
+
+    # --- launch tuning (no counterpart in the reference)
+    def setSearchThreads(self, threads: int):
+        """Threads per query: 64, 128, 256, or 0 for tuned / rule of thumb."""
+        check(lib().efg_set_search_threads(self.h, int(threads)))
+
+    def autotune(self, k, query):
+        """Times 64/128/256 threads per query on these queries; returns (full, probe) choices."""
+        q = query.data if isinstance(query, Matrix) else np.ascontiguousarray(query, np.float32)
+        out = (ctypes.c_int * 2)()
+        check(lib().efg_autotune_search(self.h, int(k), _p(q, _lib.c_f32p), q.shape[0], q.shape[1], out))
+        return out[0], out[1]
+
+    # --- learned stopping (no counterpart in the reference)
+    def setProbeParams(self, pool=0, extend_to=0, epochs=0):
+        """Probe search of the learned stopping rule (0 = derived from the search parameters)."""
+        check(lib().efg_set_probe_params(self.h, int(pool), int(extend_to), int(epochs)))
+        self._probe_epochs = int(epochs) if epochs > 0 else None
+
+    def searchTraces(self, k, query, plain_ids):
+        """Probe-search traces: features [nq, C, 16] and labels [nq, C] (255 = not reached)."""
+        q = query.data if isinstance(query, Matrix) else np.ascontiguousarray(query, np.float32)
+        plain = np.ascontiguousarray(plain_ids, np.int32)
+        nq = q.shape[0]
+        assert plain.shape == (nq, k)
+        ep = self.getSearchParams()["epochs"] if self._probe_epochs is None else self._probe_epochs
+        C = ep + 1
+        feat = np.empty((nq, C, 16), np.float32)
+        label = np.empty((nq, C), np.uint8)
+        check(lib().efg_search_traces(self.h, int(k), _p(q, _lib.c_f32p), nq, q.shape[1], _p(plain, _lib.c_i32p),
+                                      _p(feat, _lib.c_f32p), _p(label, _lib.c_u8p)))
+        return feat, label
+
+    def setStoppingNetwork(self, params, norm, threshold=0.9):
+        params = np.ascontiguousarray(params, np.float32)
+        norm = np.ascontiguousarray(norm, np.float32)
+        assert params.shape == (lib().efg_mlp_param_count(),) and norm.shape == (32,)
+        check(lib().efg_set_stopping_network(self.h, _p(params, _lib.c_f32p), _p(norm, _lib.c_f32p),
+                                             float(threshold)))
+
+    def setSearchMode(self, mode: int):
+        """0 = plain search; 1 = probe search stopped by the network, full search as fallback."""
+        check(lib().efg_set_search_mode(self.h, int(mode)))
+
+    def learned_stats(self):
+        v = [ctypes.c_uint64() for _ in range(3)]
+        check(lib().efg_get_learned_stats(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(["last_fallbacks", "total_fallbacks", "total_queries"], [x.value for x in v]))
 
     def getSearchParams(self):
         v = [ctypes.c_int() for _ in range(5)]

@@ -106,6 +106,56 @@ int efg_knn_search_device(efg_index* ix, int k, const float* d_queries, size_t n
  * results.  Applies to method 0. */
 int efg_set_search_precision(efg_index* ix, int precision);
 
+/* Learned stopping (no counterpart in the reference; a form beside the plain
+ * search).  A small network (16 pool-state features -> 64 -> 64 -> 1 logit,
+ * ReLU, 5,313 parameters laid out W1[16][64] | b1[64] | W2[64][64] | b2[64] |
+ * W3[64] | b3, evaluated on BF16 parameters with FP32 accumulation) stands in
+ * for the fixed epoch/pool stopping rule of method 0.  A probe search with a
+ * small pool evaluates it after the seeds and after every epoch; a query
+ * stops when the network's probability reaches the threshold, otherwise it
+ * runs the full plain search (setSearchParams) from scratch. */
+int efg_mlp_param_count(void);
+/* Initial parameters from the generator (uniform Glorot weights, zero biases). */
+int efg_mlp_init(float* params, uint64_t seed);
+/* Logits of n feature rows (host buffers); norm = feature mean[16] | 1 / deviation[16]. */
+int efg_mlp_logits(const float* params, const float* norm, const float* feat, size_t n, float* z);
+/* Gradient of the mean binary cross-entropy over n labelled rows (host buffers). */
+int efg_mlp_gradient(const float* params, const float* norm, const float* feat, const uint8_t* label, size_t n,
+                     float* grad, float* loss);
+/* Adam training on the GPU: `epochs` passes in generator order, `batch` rows
+ * per step; params holds the initial values on entry and the trained ones
+ * on return; loss_hist (nullable) gets the mean loss of every epoch. */
+int efg_mlp_train(const float* feat, const uint8_t* label, size_t n, const float* norm, float* params, int epochs,
+                  int batch, float lr, uint64_t seed, float* loss_hist);
+/* Probe search parameters (0 = derived: pool max(k, init_num / 6), extend_to
+ * max(1, extend_to / 4), the search's epochs). */
+int efg_set_probe_params(efg_index* ix, int pool, int extend_to, int epochs);
+/* Traces of the probe search for nq host queries: per query and checkpoint
+ * (epochs + 1 of them) the 16 features and a label, the number of top-k ids
+ * that are also in plain_ids[query] (host, nq x k; capped at 254), 255 for
+ * checkpoints the search did not reach.  feat holds nq * (epochs + 1) * 16
+ * floats. */
+int efg_search_traces(efg_index* ix, int k, const float* queries, size_t nq, size_t q_stride,
+                      const int32_t* plain_ids, float* feat, uint8_t* label);
+/* Network used when the search mode is 1; a query stops at probability >= threshold. */
+int efg_set_stopping_network(efg_index* ix, const float* params, const float* norm, float threshold);
+/* Search mode for method 0: 0 = plain (default), 1 = probe search + network + full-search fallback. */
+int efg_set_search_mode(efg_index* ix, int mode);
+/* Queries of the last learned search that fell back, and running totals. */
+int efg_get_learned_stats(efg_index* ix, uint64_t* last_fallbacks, uint64_t* total_fallbacks,
+                          uint64_t* total_queries);
+
+/* Threads per query of the search kernels (no counterpart in the reference;
+ * launch tuning only, results do not change): 64, 128 or 256 forces one;
+ * 0 (default) uses the value tuned for the configuration, else one
+ * wavefront for pools up to 256, two up to 1024, four above. */
+int efg_set_search_threads(efg_index* ix, int threads);
+/* Times 64, 128 and 256 threads per query on nq host queries for the current
+ * search parameters and mode (learned mode: the probe and the full search)
+ * and keeps the fastest per configuration.  threads_out[0] (full search) and
+ * threads_out[1] (probe, 0 in plain mode) receive the choice (nullable). */
+int efg_autotune_search(efg_index* ix, int k, const float* queries, size_t nq, size_t q_stride, int* threads_out);
+
 /* getGraphSize / getGraphRow (base_index.hpp:494-501). */
 int efg_graph_size(const efg_index* ix, size_t* n);
 int efg_graph_row(const efg_index* ix, size_t row, uint32_t* out, size_t cap, size_t* len);

@@ -67,6 +67,12 @@ def lib():
             "orc_search": (ctypes.c_int, [ctypes.c_void_p, _f32p, _sz, _sz, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           _i32p, _f32p, _u64p]),
+            "orc_search_probe": (ctypes.c_int, [ctypes.c_void_p, _f32p, _sz, _sz] + [ctypes.c_int] * 9 +
+                                 [_f32p, _f32p, ctypes.c_float, _i32p, _i32p, _f32p, _f32p, _u8p, _u8p]),
+            "orc_mlp_logits": (None, [_f32p, _f32p, _f32p, _sz, _f32p]),
+            "orc_mlp_grad64": (ctypes.c_double, [_f32p, _f32p, _f32p, _u8p, _sz, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_double)]),
+            "orc_mlp_param_count": (ctypes.c_int, []),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -248,6 +254,30 @@ class Index:
         m = lib().orc_join_pairs(self.h, _p(out, _u64p), 64)
         return out[:m].tolist()
 
+    def search_probe(self, queries, k, epochs, init_num, extend_to, search_trees, p_pool, p_extend, p_epochs,
+                     mode, params=None, norm=None, zthr=0.0, plain=None):
+        """Learned stopping restated: mode 2 returns traces (features, labels);
+        mode 1 returns (ids, dists, stopped) of the probe search with the
+        network and the full-search fallback."""
+        q = np.ascontiguousarray(queries, np.float32)
+        nq = q.shape[0]
+        C = p_epochs + 1
+        ids = np.empty((nq, k), np.int32)
+        dists = np.empty((nq, k), np.float32)
+        feat = np.zeros((nq, C, 16), np.float32)
+        label = np.full((nq, C), 255, np.uint8)
+        stopped = np.zeros(nq, np.uint8)
+        w = np.ascontiguousarray(params if params is not None else np.zeros(5313), np.float32)
+        nm = np.ascontiguousarray(norm if norm is not None else np.zeros(32), np.float32)
+        pl = np.ascontiguousarray(plain if plain is not None else np.zeros((nq, k)), np.int32)
+        _check(lib().orc_search_probe(self.h, _p(q, _f32p), nq, q.shape[1], k, epochs, init_num, extend_to,
+                                      search_trees, p_pool, p_extend, p_epochs, mode, _p(w, _f32p), _p(nm, _f32p),
+                                      ctypes.c_float(zthr), _p(pl, _i32p), _p(ids, _i32p), _p(dists, _f32p),
+                                      _p(feat, _f32p), _p(label, _u8p), _p(stopped, _u8p)))
+        if mode == 2:
+            return feat, label
+        return ids, dists, stopped.astype(bool)
+
     def search(self, queries, k, epochs, init_num, extend_to, search_trees, method=0):
         q = np.ascontiguousarray(queries, np.float32)
         nq = q.shape[0]
@@ -259,3 +289,25 @@ class Index:
                                 search_trees, method, _p(ids, _i32p), _p(dists, _f32p), _p(nd, _u64p)))
         self.init_num_state = init.value
         return ids, dists, nd
+
+
+def mlp_logits(params, norm, feat):
+    """Logits of the stopping network (FP32, BF16-rounded parameters)."""
+    w = np.ascontiguousarray(params, np.float32)
+    nm = np.ascontiguousarray(norm, np.float32)
+    f = np.ascontiguousarray(feat, np.float32).reshape(-1, 16)
+    z = np.empty(f.shape[0], np.float32)
+    lib().orc_mlp_logits(_p(w, _f32p), _p(nm, _f32p), _p(f, _f32p), f.shape[0], _p(z, _f32p))
+    return z
+
+
+def mlp_grad64(params, norm, feat, label, round_params=True):
+    """(mean BCE loss, gradient) in float64."""
+    w = np.ascontiguousarray(params, np.float32)
+    nm = np.ascontiguousarray(norm, np.float32)
+    f = np.ascontiguousarray(feat, np.float32).reshape(-1, 16)
+    y = np.ascontiguousarray(label, np.uint8)
+    g = np.empty(lib().orc_mlp_param_count(), np.float64)
+    loss = lib().orc_mlp_grad64(_p(w, _f32p), _p(nm, _f32p), _p(f, _f32p), _p(y, _u8p), f.shape[0],
+                                1 if round_params else 0, g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return loss, g

@@ -774,6 +774,211 @@ void search_nnexp(const Index& ix, const float* q, uint64_t qid, const SearchArg
   }
 }
 
+// ---------------------------------------------------------------------------
+// Learned stopping (a form beside the plain search; no reference code).  A
+// probe search (method 0 with a small pool) is checkpointed after the seeds
+// and after every epoch that expands something.  At a checkpoint the
+// pool-state features below are computed; in trace mode they are recorded
+// with a label (top-k ids also in the plain search's top k), in learned mode a
+// network 16 -> 64 -> 64 -> 1 (ReLU, BF16 parameters, FP32 fused
+// multiply-adds in input order) decides: stop when its logit reaches zthr,
+// otherwise the query runs the full plain search from scratch.
+// ---------------------------------------------------------------------------
+constexpr int kNF = 16, kNH = 64;
+constexpr int kW1 = 0, kB1 = kW1 + kNF * kNH, kW2 = kB1 + kNH, kB2 = kW2 + kNH * kNH, kW3 = kB2 + kNH,
+              kB3 = kW3 + kNH, kNP = kB3 + 1;
+
+inline float round_bf16(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u = ((u + 0x7fffu + ((u >> 16) & 1u)) >> 16) << 16;
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+
+// Features of a checkpoint: e; best and k-th distance; k-th/best ratios to the
+// best, the extend_to-th and the last pool entry; relative gains of the k-th
+// and best distances over the last 1..3 checkpoints; ids measured; ids
+// expanded in the last epoch; top-k ids new since the last checkpoint;
+// unexpanded ids among the first extend_to.
+void checkpoint_features(int e, float d1, float dk, float dE, float dR, const float* dkh, const float* d1h,
+                         float visited, float expanded, float changed, float frontier, float* f) {
+  auto ratio = [](float a, float b) { return b > 0.f ? a / b : 1.f; };
+  auto gain = [](float before, float now) { return before > 0.f ? (before - now) / before : 0.f; };
+  f[0] = float(e);
+  f[1] = d1;
+  f[2] = dk;
+  f[3] = ratio(d1, dk);
+  f[4] = ratio(dk, dE);
+  f[5] = ratio(dk, dR);
+  for (int j = 0; j < 3; ++j) {
+    f[6 + j] = e > j ? gain(dkh[j], dk) : 0.f;
+    f[9 + j] = e > j ? gain(d1h[j], d1) : 0.f;
+  }
+  f[12] = visited;
+  f[13] = expanded;
+  f[14] = changed;
+  f[15] = frontier;
+}
+
+// Logit on BF16-rounded parameters (w holds the rounded values).
+float net_logit(const float* w, const float* norm, const float* f) {
+  float x[kNF], h1[kNH];
+  for (int i = 0; i < kNF; ++i) x[i] = (f[i] - norm[i]) * norm[kNF + i];
+  for (int j = 0; j < kNH; ++j) {
+    float a = w[kB1 + j];
+    for (int i = 0; i < kNF; ++i) a = std::fmaf(w[kW1 + i * kNH + j], x[i], a);
+    h1[j] = a > 0.f ? a : 0.f;
+  }
+  float z = w[kB3];
+  for (int j = 0; j < kNH; ++j) {
+    float a = w[kB2 + j];
+    for (int i = 0; i < kNH; ++i) a = std::fmaf(w[kW2 + i * kNH + j], h1[i], a);
+    z = std::fmaf(w[kW3 + j], a > 0.f ? a : 0.f, z);
+  }
+  return z;
+}
+
+struct ProbeArgs {
+  SearchArgs full, probe;
+  int mode;             // 1 learned, 2 trace
+  const float* w;       // mode 1: rounded parameters
+  const float* norm;    // mode 1
+  float zthr;           // mode 1
+  const int32_t* plain; // mode 2: this query's plain top k
+};
+
+// Returns whether the network stopped the probe (mode 1).  feat/label (mode
+// 2) hold probe.epochs + 1 checkpoints.
+bool search_probe(const Index& ix, const float* q, uint64_t qid, const ProbeArgs& pa, int32_t* out_ids,
+                  float* out_d, float* feat, uint8_t* label) {
+  const SearchArgs& a = pa.probe;
+  const size_t n = ix.n;
+  const int K = a.K;
+  std::vector<uint8_t> visited(n, 0), expanded(n, 0);
+  unsigned lsize = unsigned(a.init_num * 2 / (5 * a.trees) + 1);
+  std::vector<uint32_t> pool;
+  seed_pool(ix, q, lsize, a.init_num, a.trees, qid, visited, pool);
+  size_t R = size_t(std::max(a.extend_to, a.K));
+  std::vector<std::pair<float, uint32_t>> res;
+  for (uint32_t id : pool) res.push_back({l2_avx(q, ix.row(id), ix.d), id});
+  std::partial_sort(res.begin(), res.begin() + R, res.end());
+  res.resize(R);
+  size_t nvis = pool.size();
+  std::vector<int32_t> prevk(K, -1);
+  float dkh[3] = {0, 0, 0}, d1h[3] = {0, 0, 0};
+  auto ckpt = [&](int e, int nexp_last) -> bool {
+    const size_t ER = std::min(size_t(a.extend_to), R);
+    int fr = 0;
+    for (size_t j = 0; j < ER; ++j) fr += expanded[res[j].second] ? 0 : 1;
+    int ch = K;
+    if (e > 0) {
+      ch = 0;
+      for (int j = 0; j < K; ++j) ch += std::find(prevk.begin(), prevk.end(), int32_t(res[j].second)) == prevk.end();
+    }
+    for (int j = 0; j < K; ++j) prevk[j] = int32_t(res[j].second);
+    float f[kNF];
+    checkpoint_features(e, res[0].first, res[K - 1].first, res[(ER > 0 ? ER : 1) - 1].first, res[R - 1].first,
+                        dkh, d1h, float(nvis), float(nexp_last), float(ch), float(fr), f);
+    dkh[2] = dkh[1]; dkh[1] = dkh[0]; dkh[0] = res[K - 1].first;
+    d1h[2] = d1h[1]; d1h[1] = d1h[0]; d1h[0] = res[0].first;
+    if (pa.mode == 2) {
+      int match = 0;
+      for (int j = 0; j < K; ++j) match += std::find(pa.plain, pa.plain + K, int32_t(res[j].second)) != pa.plain + K;
+      std::memcpy(feat + size_t(e) * kNF, f, sizeof f);
+      label[e] = uint8_t(std::min(match, 254));
+      return false;
+    }
+    return net_logit(pa.w, pa.norm, f) >= pa.zthr;
+  };
+  bool stopped = ckpt(0, 0);
+  std::vector<uint32_t> ids;
+  for (int it = 0; it < a.epochs && !stopped; ++it) {
+    ids.clear();
+    int nexp = 0;
+    for (int j = 0; j < a.extend_to; ++j) {
+      uint32_t pj = res[j].second;
+      if (expanded[pj]) continue;
+      expanded[pj] = 1;
+      ++nexp;
+      for (uint64_t g = ix.goff[pj]; g < ix.goff[pj + 1]; ++g) {
+        uint32_t id = ix.gids[g];
+        if (visited[id]) continue;
+        visited[id] = 1;
+        ids.push_back(id);
+      }
+    }
+    if (nexp == 0) break;
+    for (uint32_t id : ids) res.push_back({l2_avx(q, ix.row(id), ix.d), id});
+    nvis += ids.size();
+    std::partial_sort(res.begin(), res.begin() + R, res.end());
+    res.resize(R);
+    stopped = ckpt(it + 1, nexp);
+  }
+  if (pa.mode == 1 && !stopped) {
+    uint64_t nd = 0;
+    search_nnexp(ix, q, qid, pa.full, out_ids, out_d, nd);
+    return false;
+  }
+  for (int i = 0; i < K; ++i) {
+    out_ids[i] = int32_t(res[i].second);
+    if (out_d) out_d[i] = res[i].first;
+  }
+  return stopped;
+}
+
+// Mean binary cross-entropy on the logit and its gradient, in double, with
+// the parameters as given (the caller rounds them when it wants the BF16
+// network); inputs standardised in float as the network does.
+double net_loss_grad64(const float* w, const float* norm, const float* feat, const uint8_t* label, size_t n,
+                       double* grad) {
+  if (grad) std::fill(grad, grad + kNP, 0.0);
+  double loss = 0.0;
+  for (size_t r = 0; r < n; ++r) {
+    double x[kNF], z1[kNH], h1[kNH], z2[kNH], h2[kNH];
+    for (int i = 0; i < kNF; ++i) x[i] = double((feat[r * kNF + i] - norm[i]) * norm[kNF + i]);
+    for (int j = 0; j < kNH; ++j) {
+      double a = w[kB1 + j];
+      for (int i = 0; i < kNF; ++i) a += double(w[kW1 + i * kNH + j]) * x[i];
+      z1[j] = a;
+      h1[j] = a > 0 ? a : 0;
+    }
+    for (int j = 0; j < kNH; ++j) {
+      double a = w[kB2 + j];
+      for (int i = 0; i < kNH; ++i) a += double(w[kW2 + i * kNH + j]) * h1[i];
+      z2[j] = a;
+      h2[j] = a > 0 ? a : 0;
+    }
+    double z = w[kB3];
+    for (int j = 0; j < kNH; ++j) z += double(w[kW3 + j]) * h2[j];
+    const double y = label[r] ? 1.0 : 0.0;
+    loss += std::max(z, 0.0) - z * y + std::log1p(std::exp(-std::fabs(z)));
+    if (!grad) continue;
+    const double dz = (1.0 / (1.0 + std::exp(-z)) - y) / double(n);
+    double d2[kNH], d1[kNH];
+    grad[kB3] += dz;
+    for (int j = 0; j < kNH; ++j) {
+      grad[kW3 + j] += dz * h2[j];
+      d2[j] = z2[j] > 0 ? dz * double(w[kW3 + j]) : 0.0;
+      grad[kB2 + j] += d2[j];
+    }
+    for (int i = 0; i < kNH; ++i) {
+      double a = 0;
+      for (int j = 0; j < kNH; ++j) {
+        grad[kW2 + i * kNH + j] += h1[i] * d2[j];
+        a += double(w[kW2 + i * kNH + j]) * d2[j];
+      }
+      d1[i] = z1[i] > 0 ? a : 0.0;
+    }
+    for (int j = 0; j < kNH; ++j) {
+      grad[kB1 + j] += d1[j];
+      for (int i = 0; i < kNF; ++i) grad[kW1 + i * kNH + j] += x[i] * d1[j];
+    }
+  }
+  return loss / double(n);
+}
+
 // Search, method 1: getNeighbors_kgraph (kdtreeub_index.hpp:496-654).  The
 // seeds of one epoch are ordered by (distance, pool rank) where the reference
 // calls the unstable std::sort on distance alone (588).
@@ -1052,6 +1257,58 @@ size_t orc_join_pairs(void* h, uint64_t* out, size_t cap) {
   for (size_t i = 0; i < m; ++i) out[i] = ix->join_pairs_per_iter[i];
   return ix->join_pairs_per_iter.size();
 }
+
+// Probe search, trace mode (2) or learned mode (1), see search_probe.
+int orc_search_probe(void* h, const float* q, size_t nq, size_t qstride, int k, int epochs, int init_num,
+                     int extend_to, int search_trees, int p_pool, int p_extend, int p_epochs, int mode,
+                     const float* params, const float* norm, float zthr, const int32_t* plain, int32_t* ids,
+                     float* dists, float* feat, uint8_t* label, uint8_t* stopped) {
+  Index* ix = static_cast<Index*>(h);
+  if (extend_to > init_num) extend_to = init_num;
+  if (p_extend > p_pool) p_extend = p_pool;
+  if (search_trees <= 0 || search_trees > ix->T) { g_err = "wrong tree number"; return -4; }
+  if (mode != 1 && mode != 2) { g_err = "mode must be 1 or 2"; return -2; }
+  ProbeArgs pa{};
+  pa.full = SearchArgs{epochs, init_num, extend_to, search_trees, k};
+  pa.probe = SearchArgs{p_epochs, p_pool, p_extend, search_trees, k};
+  pa.mode = mode;
+  std::vector<float> w(kNP);
+  if (mode == 1)
+    for (int i = 0; i < kNP; ++i) w[i] = round_bf16(params[i]);
+  pa.w = w.data();
+  pa.norm = norm;
+  pa.zthr = zthr;
+  const size_t C = size_t(p_epochs) + 1;
+  if (mode == 2) std::fill(label, label + nq * C, uint8_t(255));
+  size_t qs = (ix->d + 7) & ~size_t(7);
+  std::vector<float> qpad(nq * qs, 0.f);
+  for (size_t i = 0; i < nq; ++i) std::memcpy(&qpad[i * qs], q + i * qstride, ix->d * sizeof(float));
+#pragma omp parallel for schedule(dynamic, 8)
+  for (long long qi = 0; qi < (long long)nq; ++qi) {
+    ProbeArgs a2 = pa;
+    a2.plain = plain ? plain + size_t(qi) * k : nullptr;
+    bool st = search_probe(*ix, &qpad[size_t(qi) * qs], uint64_t(qi), a2, ids + size_t(qi) * k,
+                           dists ? dists + size_t(qi) * k : nullptr, feat ? feat + size_t(qi) * C * kNF : nullptr,
+                           label ? label + size_t(qi) * C : nullptr);
+    if (stopped) stopped[qi] = st ? 1 : 0;
+  }
+  return 0;
+}
+
+// Network logits (FP32, BF16-rounded parameters) and the FP64 loss/gradient.
+void orc_mlp_logits(const float* params, const float* norm, const float* feat, size_t n, float* z) {
+  std::vector<float> w(kNP);
+  for (int i = 0; i < kNP; ++i) w[i] = round_bf16(params[i]);
+  for (size_t r = 0; r < n; ++r) z[r] = net_logit(w.data(), norm, feat + r * kNF);
+}
+double orc_mlp_grad64(const float* params, const float* norm, const float* feat, const uint8_t* label, size_t n,
+                      int round_params, double* grad) {
+  std::vector<float> w(params, params + kNP);
+  if (round_params)
+    for (auto& v : w) v = round_bf16(v);
+  return net_loss_grad64(w.data(), norm, feat, label, n, grad);
+}
+int orc_mlp_param_count() { return kNP; }
 
 // knnSearch with setSearchParams semantics (base_index.hpp:153-161):
 // extend_to is clamped to init_num; method 0 = nnexp, 1 = kgraph (which

@@ -1,0 +1,152 @@
+"""GPU parity of the learned stopping form.
+
+The network kernels are checked against the oracle: logits bit for bit
+(same BF16 parameters, same FP32 fused multiply-add order), gradients within
+FP32 tolerance of the oracle's FP64 gradient.  Probe-search traces and the
+learned search itself (probe + network + full-search fallback) must equal
+the oracle's restatement bit for bit.  The last test shows what the trained
+network gains over the plain search.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from efanna_amd import learned
+from tests import helpers
+from tests.test_gpu_search import _gpu_index
+
+pytestmark = pytest.mark.gpu
+
+FULL = (4, 800, 100, 8)  # epochs, pool, extend_to, trees of the full search
+PROBE = (200, 60, 6)     # pool, extend_to, epochs of the probe search
+K = 10
+NTEST = 200             # the first queries are held out; the rest train
+AGREE = 0.9             # positive label: at least 9 of the probe's top 10 are in the plain top 10
+
+
+@pytest.fixture(scope="module")
+def setup():
+    base, q, oix = helpers.oracle_index("mixture", nq=2200)
+    gix = _gpu_index(base, oix.trees(), oix.graph())
+    gix.setSearchParams(*FULL, -1, 0)
+    gix.setProbeParams(*PROBE)
+    return base, q, oix, gix
+
+
+@pytest.fixture(scope="module")
+def trained(setup):
+    base, q, oix, gix = setup
+    w, norm, hist, pos = learned.train_for_index(gix, q[NTEST:], K, agree=AGREE, epochs=20, batch=256, lr=1e-3,
+                                                 seed=5)
+    return w, norm, hist, pos
+
+
+def _zthr(thr):
+    t = float(np.float32(thr))
+    return float(np.float32(math.log(t / (1.0 - t))))
+
+
+def _rand_net(seed):
+    rng = np.random.default_rng(seed)
+    w = learned.init_params(seed) + rng.normal(0, 0.05, learned.param_count()).astype(np.float32)
+    feat = rng.normal(0, 3, size=(777, 16)).astype(np.float32)
+    feat[:, 0] = rng.integers(0, 5, 777)
+    feat[:, 1:3] = np.abs(feat[:, 1:3]) * 1e5
+    norm = learned.feature_norm(feat)
+    return w, norm, feat
+
+
+def test_logits_bitexact(cuda):
+    w, norm, feat = _rand_net(1)
+    zg = learned.logits(w, norm, feat)
+    zo = oracle.mlp_logits(w, norm, feat)
+    np.testing.assert_array_equal(zg.view(np.uint32), zo.view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [1, 64, 700])
+def test_gradient_matches_oracle(cuda, n):
+    w, norm, feat = _rand_net(2)
+    y = (np.random.default_rng(3).random(777) < 0.6).astype(np.uint8)
+    gg, lg = learned.gradient(w, norm, feat[:n], y[:n])
+    lo, go = oracle.mlp_grad64(w, norm, feat[:n], y[:n], round_params=True)
+    assert abs(lg - lo) <= 1e-5 * max(1.0, abs(lo))
+    scale = np.abs(go).max()
+    np.testing.assert_allclose(gg, go, rtol=2e-3, atol=2e-5 * scale)
+
+
+def test_traces_match_oracle(cuda, setup):
+    base, q, oix, gix = setup
+    qt = q[NTEST:NTEST + 600]
+    gix.setSearchMode(0)
+    gix.knnSearch(K, qt)
+    plain, _ = gix.results()
+    fg, lg = gix.searchTraces(K, qt, plain)
+    fo, lo = oix.search_probe(qt, K, *FULL, *PROBE, mode=2, plain=plain)
+    np.testing.assert_array_equal(lg, lo)
+    reached = lo != 255
+    np.testing.assert_array_equal(fg[reached].view(np.uint32), fo[reached].view(np.uint32))
+    assert reached[:, 0].all()
+    assert (lo[reached] <= K).all()
+    assert 0 < (lo == K).sum() < reached.sum()
+
+
+def test_training_deterministic_and_learns(cuda, setup, trained):
+    w, norm, hist, pos = trained
+    assert hist[-1] < hist[0]
+    base, q, oix, gix = setup
+    gix.knnSearch(K, q[NTEST:NTEST + 300])
+    plain, _ = gix.results()
+    feat, label = gix.searchTraces(K, q[NTEST:NTEST + 300], plain)
+    x, y = learned.traces_to_examples(feat, label, K, 1.0)
+    w2, h2 = learned.train(x, y, norm, learned.init_params(9), epochs=3, batch=256, seed=4)
+    w3, h3 = learned.train(x, y, norm, learned.init_params(9), epochs=3, batch=256, seed=4)
+    np.testing.assert_array_equal(w2, w3)
+    np.testing.assert_array_equal(h2, h3)
+
+
+@pytest.mark.parametrize("thr", [0.5, 0.9, 0.99])
+def test_learned_search_matches_oracle(cuda, setup, trained, thr):
+    base, q, oix, gix = setup
+    w, norm, _, _ = trained
+    qt = q[:NTEST]
+    gix.setStoppingNetwork(w, norm, thr)
+    gix.setSearchMode(1)
+    gix.knnSearch(K, qt)
+    ids, d = gix.results()
+    st = gix.learned_stats()
+    gix.setSearchMode(0)
+    oids, od, stopped = oix.search_probe(qt, K, *FULL, *PROBE, mode=1, params=w, norm=norm, zthr=_zthr(thr))
+    np.testing.assert_array_equal(ids, oids)
+    np.testing.assert_array_equal(d.view(np.uint32), od.view(np.uint32))
+    assert st["last_fallbacks"] == int((~stopped).sum())
+    if thr == 0.9:
+        assert 0 < stopped.sum() < NTEST
+
+
+def test_learned_gain_over_plain(cuda, setup, trained):
+    """At the 0.9 threshold the learned rule measures at most 3/4 of the plain
+    search's distances and loses at most 0.03 recall@10 against exact kNN
+    (its label tolerates one of ten neighbours differing from the plain
+    search's)."""
+    base, q, oix, gix = setup
+    w, norm, _, _ = trained
+    qt = q[:NTEST]
+    gt, _ = oracle.exact_knn(base, qt, K)
+    gix.setSearchMode(0)
+    gix.search_work()  # counters restart
+    gix.knnSearch(K, qt)
+    plain_ids, _ = gix.results()
+    plain_work = gix.search_work()[0]
+    gix.setStoppingNetwork(w, norm, 0.9)
+    gix.setSearchMode(1)
+    gix.knnSearch(K, qt)
+    ids, _ = gix.results()
+    learned_work = gix.search_work()[0]
+    gix.setSearchMode(0)
+    r_plain, r_learned = helpers.recall(plain_ids, gt, K), helpers.recall(ids, gt, K)
+    print(f"plain {plain_work / NTEST:.1f} dist/q recall {r_plain:.4f}; "
+          f"learned {learned_work / NTEST:.1f} dist/q recall {r_learned:.4f}")
+    assert r_learned >= r_plain - 0.03
+    assert learned_work <= 0.75 * plain_work

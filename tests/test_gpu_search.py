@@ -209,3 +209,29 @@ def test_search_bf16_rows_float_data(cuda):
     gt, _ = oracle.exact_knn(base, q, 10)
     oids, _, _ = oix.search(q, 10, 4, 400, 60, 8, 0)
     assert helpers.recall(ids, gt, 10) >= helpers.recall(oids, gt, 10) - 0.01
+
+
+@pytest.mark.parametrize("threads", [64, 128, 256])
+def test_search_threads_per_query_same_results(cuda, threads):
+    # launch shape only: every width returns the oracle's results and counts
+    base, q, oix = helpers.oracle_index("mixture", n=20_000, nq=200, centres=50)
+    gix = _gpu_index(base, oix.trees(), oix.graph())
+    gix.setSearchThreads(threads)
+    for epochs, pool, extend, trees in [(4, 120, 20, 8), (4, 600, 100, 8), (2, 1000, 60, 4)]:
+        if pool > 16 * threads:
+            continue
+        gids, gd, oids, od, ond = _search_both(oix, gix, q, 10, epochs, pool, extend, trees)
+        np.testing.assert_array_equal(gids, oids)
+        np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32))
+
+
+def test_autotune_keeps_results(cuda):
+    base, q, oix = helpers.oracle_index("mixture", n=20_000, nq=200, centres=50)
+    gix = _gpu_index(base, oix.trees(), oix.graph())
+    gix.setSearchParams(4, 300, 50, 8, -1, 0)
+    full, probe = gix.autotune(10, q)
+    assert full in (64, 128, 256) and probe == 0
+    gix.knnSearch(10, q)
+    gids, _ = gix.results()
+    oids, _, _ = oix.search(q, 10, 4, 300, 50, 8, 0)
+    np.testing.assert_array_equal(gids, oids)
