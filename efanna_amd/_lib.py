@@ -51,6 +51,8 @@ _SIGS = {
     "efg_load_index": (c_int, [c_vp, ctypes.c_char_p]),
     "efg_save_results": (c_int, [c_vp, ctypes.c_char_p]),
     "efg_exact_knn_device": (c_int, [c_vp, c_sz, c_sz, c_sz, c_vp, c_sz, c_sz, c_int, c_int, c_vp, c_vp, c_vp]),
+    "efg_exact_knn_device_prec": (c_int, [c_vp, c_sz, c_sz, c_sz, c_vp, c_sz, c_sz, c_int, c_int, c_int, c_vp, c_vp,
+                                          c_vp]),
     "efg_tree_shape": (c_int, [c_vp, c_intp, c_szp]),
     "efg_get_trees": (c_int, [c_vp, c_i32p, c_f32p, c_i32p, c_i32p, c_u32p, c_u32p, c_i32p, c_u32p]),
     "efg_set_trees": (c_int, [c_vp, c_int, c_sz, c_i32p, c_f32p, c_i32p, c_i32p, c_u32p, c_u32p, c_i32p, c_u32p]),

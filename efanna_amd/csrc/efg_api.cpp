@@ -758,6 +758,23 @@ int efg_exact_knn_device(const float* d_base, size_t n, size_t d, size_t base_st
   return EFG_OK;
 }
 
+int efg_exact_knn_device_prec(const float* d_base, size_t n, size_t d, size_t base_stride, const float* d_queries,
+                              size_t nq, size_t q_stride, int k, int exclude_self, int precision, int32_t* d_ids,
+                              float* d_dists, void* stream) {
+  if (precision == 0)
+    return efg_exact_knn_device(d_base, n, d, base_stride, d_queries, nq, q_stride, k, exclude_self, d_ids, d_dists,
+                                stream);
+  if (precision != 1 && precision != 2) return fail(EFG_ERR_ARG, "precision must be 0 (FP32), 1 (BF16) or 2 (FP16)");
+  if (nq == 0) return EFG_OK;
+  if (!d_base || !d_queries || !d_ids || q_stride < d) return fail(EFG_ERR_ARG, "invalid arguments");
+  if (nq >= (size_t(1) << 31) || n >= (size_t(1) << 31)) return fail(EFG_ERR_ARG, "too large");
+  std::string err;
+  int rc = exact_knn_mfma(d_base, n, int(d), base_stride, d_queries, int(nq), q_stride, k, exclude_self, precision,
+                          d_ids, d_dists, static_cast<hipStream_t>(stream), &err);
+  if (rc) return fail(rc, err);
+  return EFG_OK;
+}
+
 int efg_tree_shape(const efg_index* ix, int* trees, size_t* nodes) {
   if (!ix) return fail(EFG_ERR_ARG, "null index");
   if (trees) *trees = ix->T;

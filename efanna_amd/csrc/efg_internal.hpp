@@ -172,6 +172,9 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
 // k_exact.hip
 int exact_knn(const float* X, size_t n, int d, size_t x_stride, const float* Q, int nq, size_t q_stride, int k,
               int exclude_self, int32_t* ids, float* dists, hipStream_t st, std::string* err);
+// precision 1 = BF16, 2 = FP16 MFMA tiles, then FP32 re-rank of the best 2k
+int exact_knn_mfma(const float* X, size_t n, int d, size_t x_stride, const float* Q, int nq, size_t q_stride, int k,
+                   int exclude_self, int precision, int32_t* ids, float* dists, hipStream_t st, std::string* err);
 // Times the search with 64, 128 and 256 threads per query on the host queries
 // (a sample of them) for the current parameters and mode; keeps the fastest.
 int search_autotune(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int* chosen,

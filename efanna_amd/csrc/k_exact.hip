@@ -11,6 +11,8 @@
 // bit-identical.  Candidates under the per-query k-th key go through a small
 // LDS buffer into a sorted per-query top-k list.
 // k_exact_merge: one workgroup per query merges the per-chunk lists.
+#include <algorithm>
+
 #include "efg_device.hpp"
 #include "efg_internal.hpp"
 
@@ -140,6 +142,200 @@ __global__ void __launch_bounds__(kBlock) k_exact_merge(const uint64_t* __restri
   }
 }
 
+// ---------------------------------------------------------------------------
+// Precision form: distances as ||q||^2 + ||x||^2 - 2 q.x with the dot products
+// on BF16 (or FP16) MFMA tiles, FP32 accumulation; the best kc = 2k per query
+// are then re-ranked by the direct FP32 difference in the reference's order.
+// On data whose values are exact in the 16-bit type and whose sums stay below
+// 2^24 (e.g. integers 0..255 with d <= 128) every step is exact and the
+// result equals the FP32 search.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+constexpr int kMP = 128, kMK = 64;  // base rows per tile, elements per LDS stage
+constexpr int kMKP = kMK + 8;        // padded row (halves): 16 B shift per row
+constexpr int kDS = kMP + 1;         // padded distance-tile row (floats)
+
+// 16-bit copy of rows (n x D floats, stride ld) padded to Dk (multiple of
+// kMK) with zeros, and the FP32 squared norm of the rounded row.
+template <bool kF16>
+__global__ void k_pack16(const float* __restrict__ X, size_t n, int d, size_t ld, int Dk,
+                         uint16_t* __restrict__ out, float* __restrict__ norm) {
+  const size_t r = size_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= n) return;
+  float acc = 0.f;
+  for (int e = lane; e < Dk; e += 64) {
+    const float v = e < d ? X[r * ld + e] : 0.f;
+    uint16_t h;
+    float back;
+    if (kF16) {
+      const _Float16 x = _Float16(v);
+      __builtin_memcpy(&h, &x, 2);
+      back = float(x);
+    } else {
+      const uint32_t u = __float_as_uint(v);
+      h = uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+      back = __uint_as_float(uint32_t(h) << 16);
+    }
+    out[r * Dk + e] = h;
+    acc = __builtin_fmaf(back, back, acc);
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) norm[r] = acc;
+}
+
+// One workgroup per (32 kQB queries, chunk of base rows).  For every kMP-row
+// base tile the (32 kQB) x 128 dot products are 32 x 32 MFMA blocks, kQB per
+// wavefront (block b = w kQB + a: query block b / 4, base block b % 4), over
+// the elements in stages of kMK through LDS; distances go to an LDS tile and
+// one thread per query keeps its kc smallest (distance, id) keys.
+template <bool kF16, int kQB>
+__global__ void __launch_bounds__(kBlock) k_exact_mfma(const uint16_t* __restrict__ Xh, const float* __restrict__ xn,
+                                                       size_t n, const uint16_t* __restrict__ Qh,
+                                                       const float* __restrict__ qn, int nq, int Dk, int kc,
+                                                       int exclude_self, size_t chunk, uint64_t* __restrict__ partial,
+                                                       int nchunks) {
+  constexpr int kMQ = 32 * kQB;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  uint16_t* qs = reinterpret_cast<uint16_t*>(smem_raw);  // [kMQ][kMKP]
+  uint16_t* xs = qs + kMQ * kMKP;                         // [kMP][kMKP]
+  float* dt = reinterpret_cast<float*>(xs + kMP * kMKP);  // [kMQ][kDS]
+  uint64_t* topk = reinterpret_cast<uint64_t*>(dt + kMQ * kDS);  // [kMQ][kc]
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int q0 = blockIdx.x * kMQ;
+  const size_t p0 = size_t(blockIdx.y) * chunk;
+  const size_t p1 = (p0 + chunk < n) ? p0 + chunk : n;
+  for (int i = tid; i < kMQ * kc; i += kBlock) topk[i] = kKeyMax;
+  const int qb = (w * kQB) / 4, pb = (w * kQB) % 4;  // query block; first base block (kQB consecutive)
+  const int r32 = lane & 31, kh = (lane >> 5) * 8;
+  for (size_t pt = p0; pt < p1; pt += kMP) {
+    f32x16_t acc[kQB];
+#pragma unroll
+    for (int a = 0; a < kQB; ++a) acc[a] = f32x16_t{};
+    for (int k0 = 0; k0 < Dk; k0 += kMK) {
+      __syncthreads();  // the previous stage (or the last tile's selection) is done with LDS
+      for (int i = tid; i < (kMQ + kMP) * (kMK / 8); i += kBlock) {
+        const int row = i / (kMK / 8), c8 = (i % (kMK / 8)) * 8;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        uint16_t* dst;
+        if (row < kMQ) {
+          const int qg = q0 + row;
+          if (qg < nq) v = *reinterpret_cast<const uint4*>(Qh + size_t(qg) * Dk + k0 + c8);
+          dst = qs + row * kMKP + c8;
+        } else {
+          const size_t pg = pt + (row - kMQ);
+          if (pg < p1) v = *reinterpret_cast<const uint4*>(Xh + pg * Dk + k0 + c8);
+          dst = xs + (row - kMQ) * kMKP + c8;
+        }
+        *reinterpret_cast<uint4*>(dst) = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < kMK; ks += 16) {
+        const uint4 av = *reinterpret_cast<const uint4*>(qs + (qb * 32 + r32) * kMKP + ks + kh);
+#pragma unroll
+        for (int a = 0; a < kQB; ++a) {
+          const uint4 bv = *reinterpret_cast<const uint4*>(xs + ((pb + a) * 32 + r32) * kMKP + ks + kh);
+          if (kF16) {
+            f16x8_t x, y;
+            __builtin_memcpy(&x, &av, 16);
+            __builtin_memcpy(&y, &bv, 16);
+            acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_f16(x, y, acc[a], 0, 0, 0);
+          } else {
+            bf16x8_t x, y;
+            __builtin_memcpy(&x, &av, 16);
+            __builtin_memcpy(&y, &bv, 16);
+            acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, acc[a], 0, 0, 0);
+          }
+        }
+      }
+    }
+    // element i of lane l: query row 8 (i / 4) + 4 (l / 32) + i % 4 of the
+    // block, base column l % 32
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int ql = qb * 32 + 8 * (i / 4) + 4 * (lane >> 5) + (i & 3);
+      const int qg = q0 + ql;
+      const float qv = qg < nq ? qn[qg] : 0.f;
+#pragma unroll
+      for (int a = 0; a < kQB; ++a) {
+        const int pl = (pb + a) * 32 + r32;
+        const size_t pg = pt + pl;
+        const float xv = pg < p1 ? xn[pg] : 0.f;
+        const float dd = __builtin_fmaf(-2.f, acc[a][i], qv + xv);
+        dt[ql * kDS + pl] = dd > 0.f ? dd : 0.f;
+      }
+    }
+    __syncthreads();
+    if (tid < kMQ) {
+      const int qg = q0 + tid;
+      uint64_t* tk = topk + size_t(tid) * kc;
+      uint64_t thr = tk[kc - 1];
+      const int m = int(p1 - pt < size_t(kMP) ? p1 - pt : size_t(kMP));
+      for (int j = 0; j < m && qg < nq; ++j) {
+        const size_t pg = pt + j;
+        if (exclude_self && pg == size_t(qg)) continue;
+        const uint64_t key = make_key(dt[tid * kDS + j], uint32_t(pg));
+        if (key >= thr) continue;
+        int z = kc - 1;
+        while (z > 0 && tk[z - 1] > key) {
+          tk[z] = tk[z - 1];
+          --z;
+        }
+        tk[z] = key;
+        thr = tk[kc - 1];
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < kMQ * kc; i += kBlock) {
+    const int ql = i / kc, j = i % kc;
+    const int qg = q0 + ql;
+    if (qg < nq) partial[(size_t(qg) * nchunks + blockIdx.y) * kc + j] = topk[i];
+  }
+}
+
+// One workgroup per query: the kc best keys over the chunks, re-measured by
+// the direct FP32 difference (reference order), sorted, first k written.
+__global__ void __launch_bounds__(kBlock) k_exact_rerank(const uint64_t* __restrict__ partial, int nchunks, int kc,
+                                                         const float* __restrict__ X, size_t x_stride,
+                                                         const float* __restrict__ Q, size_t q_stride, int d, int D,
+                                                         int nq, int k, int32_t* __restrict__ ids,
+                                                         float* __restrict__ dists) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  uint64_t* buf = reinterpret_cast<uint64_t*>(smem_raw);
+  float* qrow = reinterpret_cast<float*>(buf + next_pow2(nchunks * kc < 2 ? 2 : nchunks * kc));
+  const int m = nchunks * kc;
+  const int p2 = next_pow2(m < 2 ? 2 : m);
+  const int pc = next_pow2(kc < 2 ? 2 : kc);
+  const int pr = threadIdx.x >> 1, h = threadIdx.x & 1;
+  for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+    for (int i = threadIdx.x; i < p2; i += kBlock) buf[i] = (i < m) ? partial[size_t(q) * m + i] : kKeyMax;
+    for (int i = threadIdx.x; i < D; i += kBlock) qrow[i] = i < d ? Q[size_t(q) * q_stride + i] : 0.f;
+    __syncthreads();
+    block_bitonic_sort(buf, p2);
+    for (int i = pr; i < kc; i += kBlock / 2) {
+      const uint64_t key = buf[i];
+      float dd = 3.402823466e38f;
+      if (key != kKeyMax) dd = l2_pair_lanes(qrow, X + size_t(key_id(key)) * x_stride, D, h);
+      if (h) buf[i] = key != kKeyMax ? make_key(dd, key_id(key)) : kKeyMax;
+    }
+    for (int i = kc + threadIdx.x; i < pc; i += kBlock) buf[i] = kKeyMax;
+    __syncthreads();
+    block_bitonic_sort(buf, pc);
+    for (int j = threadIdx.x; j < k; j += kBlock) {
+      const uint64_t key = buf[j];
+      const bool ok = key != kKeyMax;
+      ids[size_t(q) * k + j] = ok ? int32_t(key_id(key)) : -1;
+      if (dists) dists[size_t(q) * k + j] = ok ? key_dist(key) : 3.402823466e38f;
+    }
+    __syncthreads();
+  }
+}
+
 namespace {
 template <int MQ, int MP>
 size_t tile_lds(int D, int k) {
@@ -147,6 +343,60 @@ size_t tile_lds(int D, int k) {
   return size_t(D) * (QT + PT) * 4 + size_t(QT) * k * 8 + size_t(QT) * PT * 8 + QT * 4;
 }
 }  // namespace
+
+int exact_knn_mfma(const float* X, size_t n, int d, size_t x_stride, const float* Q, int nq, size_t q_stride, int k,
+                   int exclude_self, int precision, int32_t* ids, float* dists, hipStream_t st, std::string* err) {
+  if (k <= 0 || k > 512 || nq <= 0 || n == 0 || d <= 0) { *err = "exact_knn: bad arguments"; return -2; }
+  if (n >= (size_t(1) << 32)) { *err = "exact_knn: too many base rows"; return -2; }
+  const int D = (d + 7) & ~7;
+  if (x_stride < size_t(D)) { *err = "exact_knn: base rows must be padded to a multiple of 8"; return -2; }
+  const int Dk = (d + kMK - 1) / kMK * kMK;
+  const int kc = int(std::min<size_t>(n, size_t(2 * k)));  // candidates re-ranked per query
+  auto lds_for = [&](int qb) {
+    return size_t(32 * qb + kMP) * kMKP * 2 + size_t(32 * qb) * kDS * 4 + size_t(32 * qb) * kc * 8;
+  };
+  const size_t lds_cap = 150 * 1024;
+  const int qb = lds_for(2) <= lds_cap ? 2 : 1;
+  const size_t lds = lds_for(qb);
+  if (lds > lds_cap) { *err = "exact_knn: k too large for the MFMA tiles"; return -2; }
+  const size_t chunk = 65536;
+  const int nchunks = int((n + chunk - 1) / chunk);
+  const int m = nchunks * kc;
+  const size_t lds_r = size_t(next_pow2(m < 2 ? 2 : m)) * 8 + size_t(D) * 4;
+  if (lds_r > lds_cap) { *err = "exact_knn: too many candidates for the re-rank"; return -2; }
+  uint16_t *xh = nullptr, *qh = nullptr;
+  float *xn = nullptr, *qn = nullptr;
+  uint64_t* partial = nullptr;
+  if (hipMallocAsync(reinterpret_cast<void**>(&xh), n * Dk * 2, st) != hipSuccess ||
+      hipMallocAsync(reinterpret_cast<void**>(&qh), size_t(nq) * Dk * 2, st) != hipSuccess ||
+      hipMallocAsync(reinterpret_cast<void**>(&xn), n * 4, st) != hipSuccess ||
+      hipMallocAsync(reinterpret_cast<void**>(&qn), size_t(nq) * 4, st) != hipSuccess ||
+      hipMallocAsync(reinterpret_cast<void**>(&partial), size_t(nq) * m * 8, st) != hipSuccess) {
+    *err = "exact_knn: out of device memory";
+    return -9;
+  }
+  const bool f16 = precision == 2;
+  auto pack = f16 ? &k_pack16<true> : &k_pack16<false>;
+  pack<<<unsigned((n + 3) / 4), 256, 0, st>>>(X, n, d, x_stride, Dk, xh, xn);
+  pack<<<unsigned((nq + 3) / 4), 256, 0, st>>>(Q, size_t(nq), d, q_stride, Dk, qh, qn);
+  auto kern = f16 ? (qb == 2 ? &k_exact_mfma<true, 2> : &k_exact_mfma<true, 1>)
+                  : (qb == 2 ? &k_exact_mfma<false, 2> : &k_exact_mfma<false, 1>);
+  hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  dim3 grid(unsigned((nq + 32 * qb - 1) / (32 * qb)), unsigned(nchunks));
+  kern<<<grid, kBlock, lds, st>>>(xh, xn, n, qh, qn, nq, Dk, kc, exclude_self, chunk, partial, nchunks);
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_exact_rerank), hipFuncAttributeMaxDynamicSharedMemorySize,
+                      int(lds_r));
+  k_exact_rerank<<<unsigned(nq < 4096 ? nq : 4096), kBlock, lds_r, st>>>(partial, nchunks, kc, X, x_stride, Q,
+                                                                        q_stride, d, D, nq, k, ids, dists);
+  hipFreeAsync(partial, st);
+  hipFreeAsync(qn, st);
+  hipFreeAsync(xn, st);
+  hipFreeAsync(qh, st);
+  hipFreeAsync(xh, st);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { *err = std::string("exact_knn: ") + hipGetErrorString(e); return -8; }
+  return 0;
+}
 
 int exact_knn(const float* X, size_t n, int d, size_t x_stride, const float* Q, int nq, size_t q_stride, int k,
               int exclude_self, int32_t* ids, float* dists, hipStream_t st, std::string* err) {

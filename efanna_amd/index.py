@@ -359,6 +359,7 @@ class FIndex:
 
 
 def exact_knn_device(base_ptr, n, d, base_stride, q_ptr, nq, q_stride, k, exclude_self, ids_ptr, d_ptr=None,
-                     stream=None):
-    check(lib().efg_exact_knn_device(base_ptr, n, d, base_stride, q_ptr, nq, q_stride, k, int(exclude_self),
-                                     ids_ptr, d_ptr, stream))
+                     stream=None, precision=0):
+    """Exact kNN over device rows; precision 0 FP32, 1 BF16 / 2 FP16 MFMA tiles + FP32 re-rank of the best 2k."""
+    check(lib().efg_exact_knn_device_prec(base_ptr, n, d, base_stride, q_ptr, nq, q_stride, k, int(exclude_self),
+                                          int(precision), ids_ptr, d_ptr, stream))

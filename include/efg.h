@@ -181,6 +181,15 @@ int efg_exact_knn_device(const float* d_base, size_t n, size_t d, size_t base_st
                          size_t nq, size_t q_stride, int k, int exclude_self, int32_t* d_ids, float* d_dists,
                          void* stream);
 
+/* Exact kNN, precision form (no counterpart in the reference): dot products
+ * on MFMA tiles in BF16 (precision 1) or FP16 (precision 2) with FP32
+ * accumulation, distances as |q|^2 + |x|^2 - 2 q.x, the best 2k per query
+ * re-ranked by the direct FP32 difference (the reference's order);
+ * precision 0 runs efg_exact_knn_device.  Same arguments otherwise; k <= 512. */
+int efg_exact_knn_device_prec(const float* d_base, size_t n, size_t d, size_t base_stride, const float* d_queries,
+                              size_t nq, size_t q_stride, int k, int exclude_self, int precision, int32_t* d_ids,
+                              float* d_dists, void* stream);
+
 /* ---- inspection (tests, tools) ------------------------------------------ */
 /* Trees as flat arrays over all nodes (leaves: dim = left = right = -1). */
 int efg_tree_shape(const efg_index* ix, int* trees, size_t* nodes);

@@ -129,14 +129,27 @@ def insert_into_knn(ids, dists, K, nid, ndist):
     return r, ids, dists
 
 
+def _pad8(a: np.ndarray) -> np.ndarray:
+    """Rows zero-padded to a multiple of 8 floats (the reference Matrix stride)."""
+    a = np.ascontiguousarray(a, np.float32)
+    d = a.shape[-1]
+    D = (d + 7) & ~7
+    if D == d:
+        return a
+    out = np.zeros(a.shape[:-1] + (D,), np.float32)
+    out[..., :d] = a
+    return out
+
+
 def exact_knn(base: np.ndarray, queries: np.ndarray, k: int, exclude_self: bool = False):
-    base = np.ascontiguousarray(base, np.float32)
-    queries = np.ascontiguousarray(queries, np.float32)
-    n, d = base.shape
+    n, d = np.asarray(base).shape
+    base = _pad8(base)
+    queries = _pad8(queries)
+    D = base.shape[1]
     nq = queries.shape[0]
     ids = np.empty((nq, k), np.int32)
     dists = np.empty((nq, k), np.float32)
-    lib().orc_exact_knn(_p(base, _f32p), n, d, d, _p(queries, _f32p), nq, d, k, int(exclude_self),
+    lib().orc_exact_knn(_p(base, _f32p), n, d, D, _p(queries, _f32p), nq, D, k, int(exclude_self),
                         _p(ids, _i32p), _p(dists, _f32p))
     return ids, dists
 

@@ -152,11 +152,11 @@ def test_insert_into_knn_hand_cases():
 # exact kNN (matrix.hpp:57-68)
 # --------------------------------------------------------------------------
 @pytest.mark.parametrize("exclude", [False, True])
-def test_exact_knn_brute_force(exclude):
-    base, q = helpers.uniform(n=3000, d=32, nq=50)
+@pytest.mark.parametrize("d", [32, 100])  # 100: rows padded to a multiple of 8 by the wrapper
+def test_exact_knn_brute_force(exclude, d):
     rng = np.random.default_rng(0)
-    base = rng.integers(0, 8, size=base.shape).astype(np.float32)  # many ties
-    qq = base[:40] if exclude else rng.integers(0, 8, size=(40, 32)).astype(np.float32)
+    base = rng.integers(0, 8, size=(3000, d)).astype(np.float32)  # many ties
+    qq = base[:40] if exclude else rng.integers(0, 8, size=(40, d)).astype(np.float32)
     ids, ds = oracle.exact_knn(base, qq, 15, exclude_self=exclude)
     d64 = ((qq[:, None, :].astype(np.int64) - base[None].astype(np.int64)) ** 2).sum(-1)
     for i in range(qq.shape[0]):
