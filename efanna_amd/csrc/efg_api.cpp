@@ -418,6 +418,35 @@ int efg_set_search_precision(efg_index* ix, int precision) {
   return EFG_OK;
 }
 
+int efg_knn_search_batch_device(efg_index* ix, const float* d_queries, size_t nq, size_t q_stride, const int* pool,
+                                const int* extend, const int* k, int out_stride, int32_t* d_ids, float* d_dists,
+                                int tune, void* stream) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (nq && (!d_queries || !pool || !k || !d_ids)) return fail(EFG_ERR_ARG, "null buffer");
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ix->stream;
+  return efg::search_batch(ix, d_queries, nq, q_stride, pool, extend, k, out_stride, d_ids, d_dists, tune != 0, st);
+}
+
+int efg_knn_search_batch(efg_index* ix, const float* queries, size_t nq, size_t q_stride, const int* pool,
+                         const int* extend, const int* k, int out_stride, int32_t* ids, float* dists, int tune) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (nq == 0) return EFG_OK;
+  if (!queries || !pool || !k || !ids || q_stride < ix->d || out_stride <= 0) return fail(EFG_ERR_ARG, "invalid arguments");
+  efg::DevBuf<float> dq, dd;
+  efg::DevBuf<int32_t> di;
+  const size_t D = ix->D, m = nq * size_t(out_stride);
+  if (!dq.alloc(nq * D) || !di.alloc(m) || !dd.alloc(m)) return fail(EFG_ERR_MEMORY, "out of device memory");
+  hipStream_t st = ix->stream;
+  hipMemsetAsync(dq.p, 0, nq * D * 4, st);
+  if (int rc = check_hip(hipMemcpy2DAsync(dq.p, D * 4, queries, q_stride * 4, ix->d * 4, nq, hipMemcpyHostToDevice, st),
+                         "copy"))
+    return rc;
+  if (int rc = efg::search_batch(ix, dq.p, nq, D, pool, extend, k, out_stride, di.p, dd.p, tune != 0, st)) return rc;
+  hipMemcpyAsync(ids, di.p, m * 4, hipMemcpyDeviceToHost, st);
+  if (dists) hipMemcpyAsync(dists, dd.p, m * 4, hipMemcpyDeviceToHost, st);
+  return check_hip(hipStreamSynchronize(st), "batch search");
+}
+
 int efg_set_search_threads(efg_index* ix, int threads) {
   if (!ix) return fail(EFG_ERR_ARG, "null index");
   if (threads != 0 && threads != 64 && threads != 128 && threads != 256)

@@ -70,6 +70,7 @@ struct SearchCfg {
   int kb;       // key buffer (seeds, then pool / new / merge areas)
   int stop;     // 0 plain; 1 probe search stopped by the network; 2 probe search writing traces
   int threads;  // threads per query (one workgroup): 64, 128 or 256
+  int out_stride;  // results per query row in the output (>= K)
   int logcap;   // visited-set capacity (log2)
   uint64_t seed;
   int method;   // 0 nnexp, 1 kgraph
@@ -100,6 +101,8 @@ struct Workspace {
   DevBuf<float> qbuf;
   DevBuf<int32_t> ids;
   DevBuf<float> dists;
+  DevBuf<int32_t> blist;  // batched search: queries per class, concatenated
+  DevBuf<int> bcount;     // batched search: queries per class
 };
 
 }  // namespace efg
@@ -175,6 +178,11 @@ int exact_knn(const float* X, size_t n, int d, size_t x_stride, const float* Q, 
 // precision 1 = BF16, 2 = FP16 MFMA tiles, then FP32 re-rank of the best 2k
 int exact_knn_mfma(const float* X, size_t n, int d, size_t x_stride, const float* Q, int nq, size_t q_stride, int k,
                    int exclude_self, int precision, int32_t* ids, float* dists, hipStream_t st, std::string* err);
+// Heterogeneous batch: per-query pool (init_num), extend_to (nullable: pool / 6)
+// and k (host arrays); results in rows of out_stride.  With tune, every
+// (pool, extend_to, k) class first times 64/128/256 threads per query.
+int search_batch(efg_index* ix, const float* dq, size_t nq, size_t q_stride, const int* pool, const int* ext,
+                 const int* kq, int out_stride, int32_t* d_ids, float* d_dists, bool tune, hipStream_t st);
 // Times the search with 64, 128 and 256 threads per query on the host queries
 // (a sample of them) for the current parameters and mode; keeps the fastest.
 int search_autotune(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int* chosen,

@@ -993,12 +993,12 @@ __global__ void __launch_bounds__(kThreads) k_search_nnexp(SearchCfg c, LearnArg
       for (int j = tid; j < c.K; j += kThreads) {
         if (kMethod == 0) {
           uint64_t k = P[j];
-          out_ids[size_t(qi) * c.K + j] = int32_t(key_id(k));
-          if (out_d) out_d[size_t(qi) * c.K + j] = key_dist(k);
+          out_ids[size_t(qi) * c.out_stride + j] = int32_t(key_id(k));
+          if (out_d) out_d[size_t(qi) * c.out_stride + j] = key_dist(k);
         } else {
           const bool ok = j < kg_valid;
-          out_ids[size_t(qi) * c.K + j] = ok ? int32_t(ri[j]) : -1;
-          if (out_d) out_d[size_t(qi) * c.K + j] = ok ? rd[j] : 3.402823466e38f;
+          out_ids[size_t(qi) * c.out_stride + j] = ok ? int32_t(ri[j]) : -1;
+          if (out_d) out_d[size_t(qi) * c.out_stride + j] = ok ? rd[j] : 3.402823466e38f;
         }
       }
       if (tid == 0) {
@@ -1146,6 +1146,7 @@ int make_cfg(efg_index* ix, int k, int init, int E, int epochs, int trees, int m
   if (c.logcap < 10) c.logcap = 10;
   c.seed = ix->bp.seed;
   c.stop = stop;
+  c.out_stride = k;
   *out = c;
   if (init_eff) *init_eff = init;
   return EFG_OK;
@@ -1435,6 +1436,113 @@ int search_autotune(efg_index* ix, int k, const float* dq, size_t nq, size_t q_s
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (ncfg == 1 && chosen) chosen[1] = 0;
+  return rc;
+}
+
+namespace {
+// Query classes of a heterogeneous batch: (pool, extend_to, k) -> queries.
+struct BatchClass {
+  int pool, ext, k;
+  std::vector<int32_t> q;
+};
+int batch_classes(const efg_index* ix, size_t nq, const int* pool, const int* ext, const int* kq, int out_stride,
+                  std::vector<BatchClass>* out) {
+  std::map<std::array<int, 3>, size_t> where;
+  for (size_t i = 0; i < nq; ++i) {
+    const int p = pool[i], k = kq[i];
+    const int e = ext ? ext[i] : std::max(1, p / 6);
+    if (p <= 0 || k <= 0 || e < 0) return fail(EFG_ERR_ARG, "invalid per-query parameters");
+    if (k > out_stride) return fail(EFG_ERR_ARG, "k above the output stride");
+    auto key = std::array<int, 3>{p, std::min(e, p), k};
+    auto it = where.find(key);
+    if (it == where.end()) {
+      it = where.emplace(key, out->size()).first;
+      out->push_back(BatchClass{key[0], key[1], key[2], {}});
+    }
+    (*out)[it->second].q.push_back(int32_t(i));
+  }
+  (void)ix;
+  return EFG_OK;
+}
+}  // namespace
+
+__global__ void k_fill_results(int32_t* __restrict__ ids, float* __restrict__ d, size_t m) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  ids[i] = -1;
+  if (d) d[i] = 3.402823466e38f;
+}
+
+int search_batch(efg_index* ix, const float* dq, size_t nq, size_t q_stride, const int* pool, const int* ext,
+                 const int* kq, int out_stride, int32_t* d_ids, float* d_dists, bool tune, hipStream_t st) {
+  if (int rc = search_prepare(ix, nq, q_stride, st)) return rc;
+  if (nq == 0) return EFG_OK;
+  if (out_stride <= 0) return fail(EFG_ERR_ARG, "invalid output stride");
+  const size_t m = nq * size_t(out_stride);
+  k_fill_results<<<unsigned((m + 255) / 256), 256, 0, st>>>(d_ids, d_dists, m);
+  if (ix->s_method != 0) return fail(EFG_ERR_METHOD, "batched search runs method 0");
+  std::vector<BatchClass> cls;
+  if (int rc = batch_classes(ix, nq, pool, ext, kq, out_stride, &cls)) return rc;
+  // class configurations and their query lists (concatenated on the device)
+  std::vector<SearchCfg> cfg(cls.size());
+  std::vector<int32_t> lists;
+  std::vector<int> counts, offs;
+  int max_nleaf = 0;
+  for (size_t i = 0; i < cls.size(); ++i) {
+    if (int rc = make_cfg(ix, cls[i].k, cls[i].pool, cls[i].ext, ix->s_epochs, ix->s_trees, 0, 0, &cfg[i], nullptr))
+      return rc;
+    cfg[i].out_stride = out_stride;
+    max_nleaf = std::max(max_nleaf, cfg[i].nleaf);
+    offs.push_back(int(lists.size()));
+    counts.push_back(int(cls[i].q.size()));
+    lists.insert(lists.end(), cls[i].q.begin(), cls[i].q.end());
+  }
+  efg::Workspace& w = ix->ws;
+  if (!w.leaves.ensure(nq * size_t(max_nleaf)) || !w.blist.ensure(lists.size()) || !w.bcount.ensure(counts.size()))
+    return fail(EFG_ERR_MEMORY, "out of device memory (batch)");
+  hipMemcpyAsync(w.blist.p, lists.data(), lists.size() * 4, hipMemcpyHostToDevice, st);
+  hipMemcpyAsync(w.bcount.p, counts.data(), counts.size() * 4, hipMemcpyHostToDevice, st);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (tune) {
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+  }
+  int rc = EFG_OK;
+  for (size_t i = 0; i < cls.size() && rc == EFG_OK; ++i) {
+    SearchCfg c = cfg[i];
+    const int32_t* ql = w.blist.p + offs[i];
+    const int* cnt = w.bcount.p + i;
+    if (tune) {  // time 64/128/256 threads per query on this class, keep the fastest
+      float best = 0.f;
+      int best_t = 0;
+      for (int t : {64, 128, 256}) {
+        if (c.init_num > kKPT * t) continue;
+        set_threads(c, t);
+        float sum = 0.f;
+        for (int rep = 0; rep < 3 && rc == EFG_OK; ++rep) {
+          hipEventRecord(e0, st);
+          rc = run_pass(ix, c, LearnArgs{}, false, dq, nq, q_stride, d_ids, d_dists, nullptr, ql, cnt, nullptr, st,
+                        false);
+          hipEventRecord(e1, st);
+          if (rc == EFG_OK) rc = check_hip(hipEventSynchronize(e1), "batch tuning");
+          float ms = 0.f;
+          hipEventElapsedTime(&ms, e0, e1);
+          if (rep > 0) sum += ms;
+        }
+        if (rc == EFG_OK && (best_t == 0 || sum < best)) {
+          best = sum;
+          best_t = t;
+        }
+      }
+      if (rc == EFG_OK) ix->tuned[{c.init_num, c.E, c.epochs, c.K, c.T, 0}] = best_t;
+    }
+    if (rc == EFG_OK) {
+      set_threads(c, choose_threads(ix, c));
+      rc = run_pass(ix, c, LearnArgs{}, false, dq, nq, q_stride, d_ids, d_dists, nullptr, ql, cnt, nullptr, st);
+    }
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
   return rc;
 }
 

@@ -152,6 +152,35 @@ class FIndex:
         """0 = FP32 rows; 1 = BF16 rows with FP32 re-rank of the best 2k."""
         check(lib().efg_set_search_precision(self.h, int(precision)))
 
+    # --- heterogeneous batch (no counterpart in the reference)
+    def knnSearchBatch(self, query, pool, k, extend=None, out_stride=None, tune=False):
+        """Per-query pool (init_num), k and extend_to (default pool // 6); returns (ids, dists)
+        with out_stride columns (default max k), -1 / FLT_MAX past each query's k."""
+        q = query.data if isinstance(query, Matrix) else np.ascontiguousarray(query, np.float32)
+        nq = q.shape[0]
+        pool = np.ascontiguousarray(pool, np.int32)
+        k = np.ascontiguousarray(k, np.int32)
+        assert pool.shape == (nq,) and k.shape == (nq,)
+        ext = None if extend is None else np.ascontiguousarray(extend, np.int32)
+        stride = int(out_stride or (int(k.max()) if nq else 1))
+        ids = np.empty((nq, stride), np.int32)
+        d = np.empty((nq, stride), np.float32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        check(lib().efg_knn_search_batch(self.h, _p(q, _lib.c_f32p), nq, q.shape[1], _p(pool, ip),
+                                         _p(ext, ip) if ext is not None else None, _p(k, ip), stride,
+                                         _p(ids, _lib.c_i32p), _p(d, _lib.c_f32p), int(tune)))
+        return ids, d
+
+    def knn_search_batch_device(self, q_ptr, nq, q_stride, pool, k, out_stride, ids_ptr, d_ptr=None, extend=None,
+                                tune=False, stream=None):
+        pool = np.ascontiguousarray(pool, np.int32)
+        k = np.ascontiguousarray(k, np.int32)
+        ext = None if extend is None else np.ascontiguousarray(extend, np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        check(lib().efg_knn_search_batch_device(self.h, q_ptr, nq, q_stride, _p(pool, ip),
+                                                _p(ext, ip) if ext is not None else None, _p(k, ip), int(out_stride),
+                                                ids_ptr, d_ptr, int(tune), stream))
+
     # --- launch tuning (no counterpart in the reference)
     def setSearchThreads(self, threads: int):
         """Threads per query: 64, 128, 256, or 0 for tuned / rule of thumb."""

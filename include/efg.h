@@ -145,6 +145,21 @@ int efg_set_search_mode(efg_index* ix, int mode);
 int efg_get_learned_stats(efg_index* ix, uint64_t* last_fallbacks, uint64_t* total_fallbacks,
                           uint64_t* total_queries);
 
+/* Heterogeneous batch (no counterpart in the reference; the reference runs
+ * one configuration per knnSearch): query q runs method 0 with init_num
+ * pool[q], extend_to extend[q] (nullable: max(1, pool[q] / 6)) and k[q], the
+ * epochs and trees of setSearchParams.  Queries are grouped by (pool,
+ * extend_to, k) and each group runs with its own threads per query (tuned
+ * first when tune != 0).  Row q of ids/dists (out_stride entries, >= every
+ * k[q]) holds the k[q] results, then -1 / FLT_MAX.  pool, extend, k are host
+ * arrays; the _device variant takes device queries and outputs and is
+ * ordered on `stream`. */
+int efg_knn_search_batch(efg_index* ix, const float* queries, size_t nq, size_t q_stride, const int* pool,
+                         const int* extend, const int* k, int out_stride, int32_t* ids, float* dists, int tune);
+int efg_knn_search_batch_device(efg_index* ix, const float* d_queries, size_t nq, size_t q_stride, const int* pool,
+                                const int* extend, const int* k, int out_stride, int32_t* d_ids, float* d_dists,
+                                int tune, void* stream);
+
 /* Threads per query of the search kernels (no counterpart in the reference;
  * launch tuning only, results do not change): 64, 128 or 256 forces one;
  * 0 (default) uses the value tuned for the configuration, else one

@@ -235,3 +235,27 @@ def test_autotune_keeps_results(cuda):
     gids, _ = gix.results()
     oids, _, _ = oix.search(q, 10, 4, 300, 50, 8, 0)
     np.testing.assert_array_equal(gids, oids)
+
+
+@pytest.mark.parametrize("tune", [False, True])
+def test_batch_heterogeneous_matches_oracle(cuda, tune):
+    """Per-query pool, extend_to and k in one call: every row equals the
+    oracle's search with that query's parameters."""
+    base, q, oix = helpers.oracle_index("mixture", n=20_000, nq=200, centres=50)
+    gix = _gpu_index(base, oix.trees(), oix.graph())
+    epochs, trees = 4, 8
+    gix.setSearchParams(epochs, 300, 50, trees, -1, 0)
+    rng = np.random.default_rng(11)
+    pools = rng.choice([60, 120, 240, 480], size=q.shape[0]).astype(np.int32)
+    ks = rng.choice([1, 10, 50], size=q.shape[0]).astype(np.int32)
+    ks = np.minimum(ks, pools)
+    ext = np.maximum(1, pools // 6).astype(np.int32)
+    ext[::5] = pools[::5] // 3  # some explicit extend_to values
+    ids, d = gix.knnSearchBatch(q, pools, ks, extend=ext, out_stride=64, tune=tune)
+    assert ids.shape == (q.shape[0], 64)
+    for (p, e, k) in set(zip(pools.tolist(), ext.tolist(), ks.tolist())):
+        sel = np.where((pools == p) & (ext == e) & (ks == k))[0]
+        oids, od, _ = oix.search(q[sel], k, epochs, p, e, trees, 0)
+        np.testing.assert_array_equal(ids[sel, :k], oids)
+        np.testing.assert_array_equal(d[sel, :k].view(np.uint32), od.view(np.uint32))
+        assert (ids[sel, k:] == -1).all()
