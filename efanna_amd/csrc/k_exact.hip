@@ -359,7 +359,11 @@ int exact_knn_mfma(const float* X, size_t n, int d, size_t x_stride, const float
   const int qb = lds_for(2) <= lds_cap ? 2 : 1;
   const size_t lds = lds_for(qb);
   if (lds > lds_cap) { *err = "exact_knn: k too large for the MFMA tiles"; return -2; }
-  const size_t chunk = 65536;
+  // base chunks per query tile: enough workgroups to fill the GPU, no more
+  // (each chunk adds kc candidates per query to the re-rank)
+  const size_t qtiles = (size_t(nq) + 32 * qb - 1) / (32 * qb);
+  const size_t want = std::max<size_t>(1, std::min<size_t>((2048 + qtiles - 1) / qtiles, 16384 / size_t(kc)));
+  const size_t chunk = std::max<size_t>(kMP, ((n + want - 1) / want + kMP - 1) / kMP * kMP);
   const int nchunks = int((n + chunk - 1) / chunk);
   const int m = nchunks * kc;
   const size_t lds_r = size_t(next_pow2(m < 2 ? 2 : m)) * 8 + size_t(D) * 4;

@@ -549,11 +549,19 @@ __device__ inline uint32_t join_task_count(uint32_t na, uint32_t nb, int BR) {
   return nI * nJ - nI * (nI - 1) / 2;
 }
 
+// Tasks per node; *rows accumulates the rows joined, sum of |new| + |old|
+// over the nodes with a non-empty new list (one atomic per wavefront).
 __global__ void k_join_count(uint32_t n, const uint64_t* __restrict__ new_off, const uint64_t* __restrict__ old_off,
-                             int BR, int32_t* __restrict__ cnt) {
+                             int BR, int32_t* __restrict__ cnt, unsigned long long* __restrict__ rows) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  cnt[p] = int32_t(join_task_count(uint32_t(new_off[p + 1] - new_off[p]), uint32_t(old_off[p + 1] - old_off[p]), BR));
+  unsigned long long r = 0;
+  if (p < n) {
+    const uint32_t na = uint32_t(new_off[p + 1] - new_off[p]), nb = uint32_t(old_off[p + 1] - old_off[p]);
+    cnt[p] = int32_t(join_task_count(na, nb, BR));
+    r = na ? na + nb : 0;
+  }
+  for (int o = 32; o > 0; o >>= 1) r += (unsigned long long)__shfl_down((long long)r, o);
+  if ((threadIdx.x & 63) == 0 && r) atomicAdd(rows, r);
 }
 
 __global__ void k_join_tasks(uint32_t n, const uint64_t* __restrict__ new_off, const uint64_t* __restrict__ old_off,
@@ -1123,6 +1131,7 @@ int gpu_merge_init(efg_index* ix) {
   bs.iter = 0;
   bs.ready = true;
   ix->counters[3] = 0;
+  ix->counters[6] = 0;
   ix->join_pairs.clear();
   if (err_depth < p.merge_level) {
     char buf[160];
@@ -1146,7 +1155,7 @@ int gpu_refine_iter(efg_index* ix) {
   DevBuf<int32_t> toff, sums, scal;
   DevBuf<unsigned long long> ctr;
   if (!worst.alloc(n) || !tcount.alloc(n) || !cursor.alloc(n) || !toff.alloc(n + 1) || !scal.alloc(4) ||
-      !ctr.alloc(2))
+      !ctr.alloc(3))
     return fail(EFG_ERR_MEMORY, "out of device memory (join)");
   k_worst<<<grid_for(n), 256, 0, st>>>(bs.pool.p, bs.fill.p, n, L, worst.p);
   // staging capacity: two candidates per pair at most
@@ -1162,7 +1171,8 @@ int gpu_refine_iter(efg_index* ix) {
   const size_t lds_join = size_t(2) * BR * (D + 4) * 4;
   DevBuf<int32_t> tcnt, tskoff;
   if (!tcnt.alloc(n) || !tskoff.alloc(n + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (join tasks)");
-  k_join_count<<<grid_for(n), 256, 0, st>>>(n, bs.new_off.p, bs.old_off.p, BR, tcnt.p);
+  hipMemsetAsync(ctr.p + 2, 0, 8, st);
+  k_join_count<<<grid_for(n), 256, 0, st>>>(n, bs.new_off.p, bs.old_off.p, BR, tcnt.p, ctr.p + 2);
   if (int rc = device_exclusive_scan(tcnt.p, int(n), tskoff.p, scal.p + 2, sums, st)) return rc;
   int ntask = 0;
   hipMemcpyAsync(&ntask, scal.p + 2, 4, hipMemcpyDeviceToHost, st);
@@ -1196,6 +1206,11 @@ int gpu_refine_iter(efg_index* ix) {
     if (attempt == 2) return fail(EFG_ERR_STATE, "join staging did not converge");
   }
   const unsigned long long nst = h_ctr[0];
+  {
+    unsigned long long rows = 0;
+    hipMemcpy(&rows, ctr.p + 2, 8, hipMemcpyDeviceToHost);
+    ix->counters[6] += rows;
+  }
   ix->join_pairs.push_back(h_ctr[1]);
   ix->counters[3] += h_ctr[1];
   if (nst >= (1ull << 31)) return fail(EFG_ERR_STATE, "too many join candidates for one iteration");

@@ -355,8 +355,11 @@ struct LearnArgs {
   const int32_t* plain_ids;  // stop 2: [query][K]
 };
 
+// The BF16-row variant is held to 128 VGPRs (four wavefronts per SIMD, which
+// its LDS allows); the others keep their registers for loads in flight.
 template <bool kGlobal, int kMethod, bool kBf16, int kStop, int kThreads>
-__global__ void __launch_bounds__(kThreads) k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kBf16 ? 4 : 1)))
+k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
                                                          size_t q_stride,
                                                          const float* __restrict__ X, const uint16_t* __restrict__ Xb,
                                                          const int32_t* __restrict__ leaves,
@@ -464,12 +467,19 @@ __global__ void __launch_bounds__(kThreads) k_search_nnexp(SearchCfg c, LearnArg
 
   // distances for keys a[from, to) whose low 32 bits hold ids
   auto distances = [&](uint64_t* a, int from, int to) {
+    if (kBf16) {
+      for (int i = from + pr; i < to; i += kThreads / 2) {
+        const uint32_t id = key_id(a[i]);
+        const float dd = l2_pair_lanes_bf16(s.q, Xb + size_t(id) * c.D, c.D, h);
+        if (h) a[i] = make_key(dd, id);
+      }
+    } else {
 #pragma unroll 2
-    for (int i = from + pr; i < to; i += kThreads / 2) {
-      uint32_t id = key_id(a[i]);
-      float dd = kBf16 ? l2_pair_lanes_bf16(s.q, Xb + size_t(id) * c.D, c.D, h)
-                       : l2_pair_lanes(s.q, X + size_t(id) * c.D, c.D, h);
-      if (h) a[i] = make_key(dd, id);
+      for (int i = from + pr; i < to; i += kThreads / 2) {  // two rows in flight per lane pair
+        const uint32_t id = key_id(a[i]);
+        const float dd = l2_pair_lanes(s.q, X + size_t(id) * c.D, c.D, h);
+        if (h) a[i] = make_key(dd, id);
+      }
     }
   };
   // merge N[0, m) (unique, unsorted, disjoint from P) into the sorted pool P[0, R)

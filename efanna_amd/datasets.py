@@ -51,6 +51,23 @@ def gaussian_mixture(n: int, d: int, nq: int, centres: int = 1000, sigma: float 
     return base, q
 
 
+def mixture_queries(d: int, nq: int, centres: int = 1000, sigma: float = 20.0, hi: float = 255.0,
+                    rounded: bool = True, seed: int = 0, query_seed: int = 1, uniform_query_frac: float = 0.0):
+    """Extra queries from the mixture of ``gaussian_mixture(..., seed)``: the
+    centres are drawn again from ``seed``, the queries from an independent
+    generator seeded with ``query_seed`` (uniform ones last)."""
+    c = np.random.default_rng(seed).uniform(0.0, hi, size=(centres, d)).astype(np.float32)
+    rng = np.random.default_rng(query_seed)
+    nu = int(round(nq * uniform_query_frac))
+    q = _mixture_rows(rng, c, nq - nu, sigma, 0.0, hi, rounded)
+    if nu:
+        u = rng.uniform(0.0, hi, size=(nu, d)).astype(np.float32)
+        if rounded:
+            np.rint(u, out=u)
+        q = np.concatenate([q, u])
+    return q
+
+
 def workload(name: str, n: int | None = None, nq: int | None = None):
     """Named workloads with their sizes; n/nq override for scaled-down tests."""
     if name == "uniform10k":

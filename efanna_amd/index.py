@@ -334,7 +334,7 @@ class FIndex:
         c = np.zeros(8, np.uint64)
         check(lib().efg_get_counters(self.h, _p(c, _lib.c_u64p), 8))
         return dict(search_dist=int(c[0]), merge=int(c[1]), init=int(c[2]), join=int(c[3]), overflow=int(c[4]),
-                    overflow2=int(c[5]))
+                    overflow2=int(c[5]), join_rows=int(c[6]))
 
     def join_pairs(self):
         out = np.zeros(64, np.uint64)
