@@ -1510,8 +1510,13 @@ int search_batch(efg_index* ix, const float* dq, size_t nq, size_t q_stride, con
   efg::Workspace& w = ix->ws;
   if (!w.leaves.ensure(nq * size_t(max_nleaf)) || !w.blist.ensure(lists.size()) || !w.bcount.ensure(counts.size()))
     return fail(EFG_ERR_MEMORY, "out of device memory (batch)");
-  hipMemcpyAsync(w.blist.p, lists.data(), lists.size() * 4, hipMemcpyHostToDevice, st);
-  hipMemcpyAsync(w.bcount.p, counts.data(), counts.size() * 4, hipMemcpyHostToDevice, st);
+  // pageable sources: the calls return once the host data is staged
+  if (int rc = check_hip(hipMemcpyAsync(w.blist.p, lists.data(), lists.size() * 4, hipMemcpyHostToDevice, st),
+                         "copy query lists"))
+    return rc;
+  if (int rc = check_hip(hipMemcpyAsync(w.bcount.p, counts.data(), counts.size() * 4, hipMemcpyHostToDevice, st),
+                         "copy query counts"))
+    return rc;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (tune) {
     hipEventCreate(&e0);
