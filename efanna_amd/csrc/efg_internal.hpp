@@ -88,6 +88,13 @@ struct BuildState {
   DevBuf<uint64_t> new_off, old_off;  // CSR join lists
   DevBuf<uint32_t> new_ids, old_ids;
   size_t new_n = 0, old_n = 0;
+  // join scratch, grown as needed and released with the finished graph:
+  // task headers, task row ids (64 per task), staged candidates, buckets
+  DevBuf<uint4> jhdr;
+  DevBuf<uint32_t> jtids;
+  DevBuf<unsigned char> jstage;
+  DevBuf<uint64_t> jbkey;
+  DevBuf<uint32_t> jbsrc;
   int iter = 0;
   bool ready = false;
 };

@@ -38,7 +38,6 @@ namespace efg {
 namespace {
 
 constexpr uint32_t kFirst = 0x80000000u;  // staging src flag: first endpoint of the pair
-constexpr int kJoinMaxBR = 32;            // rows per join block at most
 
 __device__ inline float l2_thread(const float* __restrict__ a, const float* __restrict__ b, int D) {
   // single-thread form of the reference's AVX order
@@ -531,200 +530,385 @@ __global__ void k_pair_bound(uint32_t n, const uint64_t* __restrict__ new_off, c
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
 }
 
+// A kept candidate staged for its target's bucket.
 struct Stage {
-  uint32_t target;
-  uint32_t src;  // joining node | kFirst
+  uint32_t target;  // kNoRow: an unused staging slot
+  uint32_t src;     // joining node | kFirst
   uint64_t key;
 };
 
 // Join tasks.  The rows of a joining node are its new list followed by its
-// old list, cut into blocks of BR rows.  Every pair the reference measures is
-// a pair of positions p < q with p new (new x new pairs once per unordered
+// old list, cut into blocks of kJBR rows.  Every pair the reference measures
+// is a pair of positions p < q with p new (new x new pairs once per unordered
 // position pair, new x old pairs always have p < q), so task (node, I, J),
 // I <= J, I a block holding new rows, covers all pairs with p in block I and
 // q in block J.
-__device__ inline uint32_t join_task_count(uint32_t na, uint32_t nb, int BR) {
+constexpr int kJBR = 32;
+constexpr uint32_t kNoRow = 0xffffffffu;
+__device__ inline uint32_t join_task_count(uint32_t na, uint32_t nb) {
   if (na == 0) return 0;
-  const uint32_t nI = (na + BR - 1) / BR, nJ = (na + nb + BR - 1) / BR;
+  const uint32_t nI = (na + kJBR - 1) / kJBR, nJ = (na + nb + kJBR - 1) / kJBR;
   return nI * nJ - nI * (nI - 1) / 2;
 }
 
 // Tasks per node; *rows accumulates the rows joined, sum of |new| + |old|
 // over the nodes with a non-empty new list (one atomic per wavefront).
 __global__ void k_join_count(uint32_t n, const uint64_t* __restrict__ new_off, const uint64_t* __restrict__ old_off,
-                             int BR, int32_t* __restrict__ cnt, unsigned long long* __restrict__ rows) {
+                             int32_t* __restrict__ cnt, unsigned long long* __restrict__ rows) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long r = 0;
   if (p < n) {
     const uint32_t na = uint32_t(new_off[p + 1] - new_off[p]), nb = uint32_t(old_off[p + 1] - old_off[p]);
-    cnt[p] = int32_t(join_task_count(na, nb, BR));
+    cnt[p] = int32_t(join_task_count(na, nb));
     r = na ? na + nb : 0;
   }
   for (int o = 32; o > 0; o >>= 1) r += (unsigned long long)__shfl_down((long long)r, o);
   if ((threadIdx.x & 63) == 0 && r) atomicAdd(rows, r);
 }
 
-__global__ void k_join_tasks(uint32_t n, const uint64_t* __restrict__ new_off, const uint64_t* __restrict__ old_off,
-                             int BR, const int32_t* __restrict__ toff, uint64_t* __restrict__ tasks) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+// Task records, one wavefront per node: header (node, sizeA | newA << 8 |
+// sizeB << 16 | newB << 24, I == J) and the rows' ids, slots [0, 32) block I,
+// [32, 64) block J (unused when I == J), kNoRow past the blocks' ends.
+__global__ void k_join_tasks(uint32_t n, const uint64_t* __restrict__ new_off, const uint32_t* __restrict__ new_ids,
+                             const uint64_t* __restrict__ old_off, const uint32_t* __restrict__ old_ids,
+                             const int32_t* __restrict__ toff, uint4* __restrict__ hdr, uint32_t* __restrict__ tids) {
+  const uint32_t p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
   if (p >= n) return;
-  const uint32_t na = uint32_t(new_off[p + 1] - new_off[p]), nb = uint32_t(old_off[p + 1] - old_off[p]);
+  const uint64_t a0 = new_off[p], b0 = old_off[p];
+  const int na = int(new_off[p + 1] - a0), m = na + int(old_off[p + 1] - b0);
   if (na == 0) return;
-  const uint32_t nI = (na + BR - 1) / BR, nJ = (na + nb + BR - 1) / BR;
-  uint64_t* out = tasks + toff[p];
-  for (uint32_t I = 0; I < nI; ++I)
-    for (uint32_t J = I; J < nJ; ++J) *out++ = (uint64_t(p) << 32) | (uint64_t(I) << 16) | J;
+  const int nI = (na + kJBR - 1) / kJBR, nJ = (m + kJBR - 1) / kJBR;
+  size_t t = size_t(toff[p]);
+  for (int I = 0; I < nI; ++I)
+    for (int J = I; J < nJ; ++J, ++t) {
+      const int sizeA = min(kJBR, m - I * kJBR), newA = min(kJBR, na - I * kJBR);
+      const int sizeB = min(kJBR, m - J * kJBR), newB = max(0, min(sizeB, na - J * kJBR));
+      const int side = lane >> 5, r = lane & 31;
+      const int size = side ? sizeB : sizeA, pos = (side ? J : I) * kJBR + r;
+      uint32_t id = kNoRow;
+      if (r < size && !(side && I == J)) id = pos < na ? new_ids[a0 + pos] : old_ids[b0 + pos - na];
+      tids[t * 64 + lane] = id;
+      if (lane == 0)
+        hdr[t] = make_uint4(p, uint32_t(sizeA) | uint32_t(newA) << 8 | uint32_t(sizeB) << 16 | uint32_t(newB) << 24,
+                            I == J ? 1u : 0u, 0u);
+    }
 }
 
-// Staging slots are reserved per wavefront in runs of at least kStageRun, so
-// the shared counter sees one atomic per run.  The slots a wavefront leaves
-// unused at its end are marked with target kNoTarget and skipped by the
-// bucketing.
-constexpr unsigned long long kStageRun = 512;
-constexpr uint32_t kNoTarget = 0xffffffffu;
-__device__ inline void stage_put(Stage* stage, unsigned long long cap, unsigned long long base,
-                                 unsigned long long nb, unsigned long long rem, int c, Stage v,
-                                 uint32_t* tcount) {
-  const unsigned long long at = (unsigned long long)c < rem ? base + c : nb + c;
-  if (at < cap) {
-    stage[at] = v;
-    atomicAdd(&tcount[v.target], 1u);
+// Pair k of a task -> positions (p, q) in blocks I and J.  I != J: all
+// newA x sizeB; I == J: q > p, row p holding sizeA - 1 - p pairs.
+__device__ inline void join_cell(int k, bool same, int sizeA, int sizeB, int& p, int& q) {
+  if (!same) {
+    p = k / sizeB;
+    q = k - p * sizeB;
+    return;
   }
-}
-__device__ inline void stage_abandon(Stage* stage, unsigned long long cap, unsigned long long from,
-                                     unsigned long long to, int lane) {
-  for (unsigned long long i = from + lane; i < to && i < cap; i += 64) stage[i].target = kNoTarget;
+  // largest p with start(p) = p (sizeA - 1) - p (p - 1) / 2 <= k
+  int lo = 0, hi = sizeA - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mid * (sizeA - 1) - mid * (mid - 1) / 2 <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  p = lo;
+  q = p + 1 + (k - (p * (sizeA - 1) - p * (p - 1) / 2));
 }
 
-// Squared L2 of two LDS rows, one thread, in the reference's AVX order.
-__device__ inline float l2_lds(const float* a, const float* b, int D) { return l2_thread(a, b, D); }
+// Endpoints of pair k: slots and ids, the first endpoint (whose candidate
+// carries the kFirst credit tag) being the lower id of a new x new pair.
+// False for a new x new pair of one id, which the reference skips.
+__device__ inline bool join_ends(int k, bool same, int sizeA, int newA, int sizeB, int newB,
+                                 const uint32_t* ids_s, int& sa, int& sb, uint32_t& a, uint32_t& b) {
+  int p, q;
+  join_cell(k, same, sizeA, sizeB, p, q);
+  sa = p;
+  sb = same ? q : 32 + q;
+  a = ids_s[sa];
+  b = ids_s[sb];
+  if (q < (same ? newA : newB)) {  // new x new: once, first endpoint = lower id
+    if (a == b) return false;
+    if (b < a) {
+      const uint32_t t2 = a;
+      a = b;
+      b = t2;
+      const int s2 = sa;
+      sa = sb;
+      sb = s2;
+    }
+  }
+  return true;
+}
 
-// One workgroup per task: both blocks' rows in LDS (row stride D + 4 floats,
-// so 16 consecutive rows start on distinct bank groups), one thread per pair,
-// the targets' worst keys in LDS.  A candidate is kept when it beats its
-// target's worst key; kept candidates are staged {target, joining node |
-// kFirst for the first endpoint, key} with one reservation per wavefront.
-__global__ void __launch_bounds__(kBlock) k_join_tiles(const float* __restrict__ X, int D, int BR, uint32_t ntask,
-                                                       const uint64_t* __restrict__ tasks,
-                                                       const uint64_t* __restrict__ new_off,
-                                                       const uint32_t* __restrict__ new_ids,
-                                                       const uint64_t* __restrict__ old_off,
-                                                       const uint32_t* __restrict__ old_ids,
-                                                       const uint64_t* __restrict__ worst,
-                                                       Stage* __restrict__ stage, unsigned long long cap,
-                                                       unsigned long long* __restrict__ nstage,
-                                                       uint32_t* __restrict__ tcount,
-                                                       unsigned long long* __restrict__ npairs) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ uint32_t ids_s[2][kJoinMaxBR];
-  __shared__ uint64_t w_s[2][kJoinMaxBR];
-  const int RS = D + 4;
-  float* rows0 = reinterpret_cast<float*>(smem_raw);
-  float* rows1 = rows0 + size_t(BR) * RS;
+// Logical column l of the reference's accumulation order (the D % 32 tail
+// first, then [0, D - D % 32)) -> physical column of a padded row.
+__device__ inline int join_col(int l, int D) {
+  const int DR = D & 31;
+  return l < DR ? D - DR + l : l - DR;
+}
+
+// Load k of this thread for the rows of one chunk (DC logical columns of up
+// to 64 rows, float4 i = threadIdx.x + k * kBlock is row i / (DC / 4)).
+__device__ __forceinline__ bool join_slot(int k, int DC4, int w4, int sizeA, int sizeB, bool same, int& r, int& c4) {
+  const int i = threadIdx.x + k * kBlock;
+  r = i / DC4;
+  c4 = i - r * DC4;
+  return r < 64 && c4 < w4 && ((r < 32) ? r < sizeA : (!same && r - 32 < sizeB));
+}
+__device__ __forceinline__ float4 join_load(const float* __restrict__ X, int D, int DC, const uint4& h,
+                                            const uint32_t* ids, int c, int k) {
+  const int sizeA = int(h.y & 0xff), sizeB = int((h.y >> 16) & 0xff), l0 = c * DC;
+  int r, c4;
+  const bool in = join_slot(k, DC >> 2, (min(DC, D - l0)) >> 2, sizeA, sizeB, h.z != 0, r, c4);
+  // out-of-range slots load row 0, column 0 (never stored)
+  const uint32_t id = in ? ids[r] : 0u;
+  return *reinterpret_cast<const float4*>(X + size_t(id) * D + (in ? join_col(l0 + 4 * c4, D) : 0));
+}
+
+// Staging slots are reserved per workgroup in runs of at least kStageRun, so
+// the shared counter sees one atomic per run; a task's candidates go to
+// consecutive slots, grouped by target row.  The slots a workgroup leaves
+// unused at its end get target kNoRow.
+constexpr unsigned long long kStageRun = 4096;
+
+// One workgroup walks a sequence of (task, chunk) steps.  The two blocks'
+// rows of one chunk of DC = 4 kJPF logical columns... sit in LDS (row stride
+// DC + 4); one thread per pair keeps the reference's eight FMA chains for up
+// to four pairs.  The next step's rows are loaded into registers while this
+// step computes, from row ids staged in LDS one task ahead (their global
+// loads are issued two tasks ahead).  Tasks are split into eight contiguous
+// ranges, one per XCD (the workgroup id modulo 8), so a node's tasks share
+// that XCD's L2.  After a task's last chunk its kept candidates (key below
+// the target's worst) are counted per target row in LDS; the counts go to the
+// targets' totals with one atomic per (task, target row) whose result is not
+// waited for, and the candidates to the workgroup's staging run.
+constexpr int kJPairs = kJBR * kJBR / kBlock;  // pairs per thread at most
+template <int kJPF>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_join_tiles(const float* __restrict__ X, int D, uint32_t ntask, const uint4* __restrict__ hdr,
+             const uint32_t* __restrict__ tids, const uint64_t* __restrict__ worst, Stage* __restrict__ stage,
+             unsigned long long cap, unsigned long long* __restrict__ nstage, uint32_t* __restrict__ tcount,
+             unsigned long long* __restrict__ npairs) {
+  constexpr int kDC = kJPF * 16;  // columns per chunk at most
+  extern __shared__ __align__(16) float4 rows_s[];  // [64][(DC + 4) / 4]
+  __shared__ uint32_t ids_s[2][64];
+  __shared__ uint4 hdr_s[2];
+  __shared__ uint64_t w_s[64];
+  __shared__ uint32_t cnt_s[64], off_s[64];
+  __shared__ unsigned long long run_s[2];  // this task's slots: i < run_s[1] at run_s[0] + i, else past a new run
+  __shared__ unsigned long long newrun_s;
   const int tid = threadIdx.x, lane = tid & 63;
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int DC = min(D, kDC), RS4 = (DC + 4) >> 2, DC4 = DC >> 2;
+  const int nch = (D + DC - 1) / DC;
+  const uint32_t xcd = blockIdx.x & 7u, ns = gridDim.x >> 3;
+  const uint32_t lo = uint32_t(uint64_t(ntask) * xcd / 8), hi = uint32_t(uint64_t(ntask) * (xcd + 1) / 8);
   unsigned long long pairs_local = 0;
-  unsigned long long wbase = 0, wend = 0;  // this wavefront's reserved staging run
-  for (uint32_t t = blockIdx.x; t < ntask; t += gridDim.x) {
-    const uint64_t task = tasks[t];
-    const uint32_t nn = uint32_t(task >> 32);
-    const int I = int((task >> 16) & 0xffff), J = int(task & 0xffff);
-    const uint64_t a0 = new_off[nn], b0 = old_off[nn];
-    const int na = int(new_off[nn + 1] - a0), m = na + int(old_off[nn + 1] - b0);
-    const bool same = I == J;
-    const int sizeA = min(BR, m - I * BR), newA = min(BR, na - I * BR), sizeB = min(BR, m - J * BR);
-    if (tid < sizeA + (same ? 0 : sizeB)) {
-      const int side = tid < sizeA ? 0 : 1, r = side ? tid - sizeA : tid;
-      const int pos = (side ? J : I) * BR + r;
-      const uint32_t id = pos < na ? new_ids[a0 + pos] : old_ids[b0 + pos - na];
-      ids_s[side][r] = id;
-      w_s[side][r] = worst[id];
+  unsigned long long wbase = 0, wend = 0;  // staging run (wave 0, lane 63)
+  uint32_t t = lo + blockIdx.x / 8;
+  // prologue: ids of the first task in LDS, of the second in registers
+  uint32_t reg_id = kNoRow;
+  uint4 reg_hdr = make_uint4(0, 0, 0, 0);
+  if (t < hi && tid < 64) {
+    ids_s[0][tid] = tids[size_t(t) * 64 + tid];
+    if (tid == 0) hdr_s[0] = hdr[t];
+    if (t + ns < hi) {
+      reg_id = tids[size_t(t + ns) * 64 + tid];
+      reg_hdr = hdr[t + ns];
+    }
+  }
+  __syncthreads();
+  float4 pf[kJPF];
+  if (t < hi) {
+    const uint4 h0 = hdr_s[0];
+#pragma unroll
+    for (int k = 0; k < kJPF; ++k) pf[k] = join_load(X, D, DC, h0, ids_s[0], 0, k);
+  }
+  float acc[kJPairs][8];
+  int cur = 0, c = 0;
+  while (t < hi) {
+    const uint4 h = hdr_s[cur];
+    const bool same = h.z != 0;
+    const int sizeA = int(h.y & 0xff), newA = int((h.y >> 8) & 0xff), sizeB = int((h.y >> 16) & 0xff),
+              newB = int(h.y >> 24);
+    if (c == 0 && tid < 64) {
+      const uint32_t id = ids_s[cur][tid];
+      w_s[tid] = id != kNoRow ? worst[id] : 0ull;
+      cnt_s[tid] = 0;
+      // the next task's ids into LDS, the one after into registers
+      ids_s[cur ^ 1][tid] = reg_id;
+      if (tid == 0) hdr_s[cur ^ 1] = reg_hdr;
+      const uint32_t t2 = t + 2 * ns;
+      if (t2 < hi && t2 > t) {
+        reg_id = tids[size_t(t2) * 64 + tid];
+        reg_hdr = hdr[t2];
+      }
+    }
+    {
+      const int w4 = (min(DC, D - c * DC)) >> 2;
+#pragma unroll
+      for (int k = 0; k < kJPF; ++k) {
+        int r, c4;
+        if (join_slot(k, DC4, w4, sizeA, sizeB, same, r, c4)) rows_s[r * RS4 + c4] = pf[k];
+      }
     }
     __syncthreads();
-    const int D4 = D >> 2, nrows = sizeA + (same ? 0 : sizeB);
-    for (int i = tid; i < nrows * D4; i += kBlock) {
-      const int r = i / D4, c = i - r * D4;
-      const int side = r < sizeA ? 0 : 1, rr = side ? r - sizeA : r;
-      float* dst = (side ? rows1 : rows0) + size_t(rr) * RS + 4 * c;
-      *reinterpret_cast<float4*>(dst) = reinterpret_cast<const float4*>(X + size_t(ids_s[side][rr]) * D)[c];
+    // next step's rows
+    const bool last = c + 1 == nch;
+    const uint32_t tn = last ? t + ns : t;
+    const int cn = last ? 0 : c + 1, nb = last ? cur ^ 1 : cur;
+    if (tn < hi && tn >= t) {
+      const uint4 hn = hdr_s[nb];
+#pragma unroll
+      for (int k = 0; k < kJPF; ++k) pf[k] = join_load(X, D, DC, hn, ids_s[nb], cn, k);
     }
-    __syncthreads();
-    const float* rowsB = same ? rows0 : rows1;
-    const int sB = same ? 0 : 1;
-    const int cells = newA * sizeB;
-    for (int k0 = 0; k0 < cells; k0 += kBlock) {
-      const int k = k0 + tid;
-      bool valid = k < cells;
-      int p = 0, q = 0;
-      uint32_t a = 0, b = 0;
-      uint64_t wa = 0, wb = 0;
-      if (valid) {
-        p = k / sizeB;
-        q = k - p * sizeB;
-        if (same && q <= p) valid = false;
-        a = ids_s[0][p];
-        b = ids_s[sB][q];
-        wa = w_s[0][p];
-        wb = w_s[sB][q];
-        if (J * BR + q < na) {  // new x new: once, first endpoint = lower id
-          if (a == b) valid = false;
-          if (b < a) {
-            uint32_t t2 = a; a = b; b = t2;
-            uint64_t w2 = wa; wa = wb; wb = w2;
+    const int cells = same ? newA * sizeA - newA * (newA + 1) / 2 : newA * sizeB;
+    if (c == 0) {
+#pragma unroll
+      for (int j = 0; j < kJPairs; ++j)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[j][e] = 0.f;
+    }
+    const int nb8 = (min(DC, D - c * DC)) >> 3;
+    const float4* rowsB = rows_s + (same ? 0 : 32 * RS4);
+#pragma unroll
+    for (int j = 0; j < kJPairs; ++j) {
+      const int k = tid + j * kBlock;
+      if (k < cells) {
+        int p, q;
+        join_cell(k, same, sizeA, sizeB, p, q);
+        const float4* a = rows_s + p * RS4;
+        const float4* b = rowsB + q * RS4;
+#pragma unroll 1
+        for (int e8 = 0; e8 < nb8; ++e8) {
+          const float4 x0 = a[2 * e8], x1 = a[2 * e8 + 1];
+          const float4 y0 = b[2 * e8], y1 = b[2 * e8 + 1];
+          float d;
+          d = x0.x - y0.x; acc[j][0] = __builtin_fmaf(d, d, acc[j][0]);
+          d = x0.y - y0.y; acc[j][1] = __builtin_fmaf(d, d, acc[j][1]);
+          d = x0.z - y0.z; acc[j][2] = __builtin_fmaf(d, d, acc[j][2]);
+          d = x0.w - y0.w; acc[j][3] = __builtin_fmaf(d, d, acc[j][3]);
+          d = x1.x - y1.x; acc[j][4] = __builtin_fmaf(d, d, acc[j][4]);
+          d = x1.y - y1.y; acc[j][5] = __builtin_fmaf(d, d, acc[j][5]);
+          d = x1.z - y1.z; acc[j][6] = __builtin_fmaf(d, d, acc[j][6]);
+          d = x1.w - y1.w; acc[j][7] = __builtin_fmaf(d, d, acc[j][7]);
+        }
+      }
+    }
+    if (last) {
+      // candidates, up to two per pair: the final distance and the
+      // candidate's index among its target row's candidates (LDS count)
+      const uint32_t* ids = ids_s[cur];
+      float dist[kJPairs];
+      uint32_t li[kJPairs];  // first endpoint's index | second's << 16; 0xffff = none
+#pragma unroll
+      for (int j = 0; j < kJPairs; ++j) {
+        li[j] = 0xffffffffu;
+        float sdist = acc[j][0];
+#pragma unroll
+        for (int e = 1; e < 8; ++e) sdist = sdist + acc[j][e];
+        dist[j] = sdist;
+        const int k = tid + j * kBlock;
+        int sa, sb;
+        uint32_t a, b;
+        if (k < cells && join_ends(k, same, sizeA, newA, sizeB, newB, ids, sa, sb, a, b)) {
+          ++pairs_local;
+          uint32_t la = 0xffffu, lb = 0xffffu;
+          if (make_key(sdist, b) < w_s[sa]) la = atomicAdd(&cnt_s[sa], 1u);
+          if (make_key(sdist, a) < w_s[sb]) lb = atomicAdd(&cnt_s[sb], 1u);
+          li[j] = la | lb << 16;
+        }
+      }
+      __syncthreads();
+      if (tid < 64) {
+        // totals per target (result not waited for), offsets within the task
+        const uint32_t cnt = cnt_s[tid];
+        if (cnt) atomicAdd(&tcount[ids[tid]], cnt);
+        uint32_t x = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = uint32_t(__shfl_up(int(x), o));
+          if (lane >= o) x += y;
+        }
+        off_s[tid] = x - cnt;
+        // this task's slots: the rest of the run, then a new run if needed
+        const unsigned long long tot = x, rem = wend - wbase;
+        unsigned long long nr = 0;
+        const bool grow = tot > rem;
+        if (grow && tid == 63) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
+        if (tid == 63) {
+          run_s[0] = wbase;
+          run_s[1] = rem;
+          newrun_s = nr;
+          if (grow) {
+            wbase = nr + (tot - rem);
+            wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
+          } else {
+            wbase += tot;
           }
         }
       }
-      bool ca = false, cb = false;
-      uint64_t ka = 0, kb = 0;
-      if (valid) {
-        const float dist = l2_lds(rows0 + size_t(p) * RS, rowsB + size_t(q) * RS, D);
-        ++pairs_local;
-        ka = make_key(dist, b);
-        kb = make_key(dist, a);
-        ca = ka < wa;
-        cb = kb < wb;
-      }
-      const uint64_t ma = __ballot(ca), mb = __ballot(cb);
-      const int na_ = __popcll(ma), tot = na_ + __popcll(mb);
-      if (tot) {
-        // candidate c of this step goes to slot c of the wavefront's run, and
-        // past its end into a freshly reserved run
-        const unsigned long long rem = wend - wbase;
-        unsigned long long nb = 0;
-        if ((unsigned long long)tot > rem) {
-          const unsigned long long want = (unsigned long long)tot - rem > kStageRun ? tot - rem : kStageRun;
-          if (lane == 0) nb = atomicAdd(nstage, want);
-          nb = (unsigned long long)__shfl((long long)nb, 0);
-          const unsigned long long base = wbase;
-          wbase = nb + (tot - rem);
-          wend = nb + want;
-          nb -= rem;  // slot of candidate c >= rem is nb + c
-          if (ca) stage_put(stage, cap, base, nb, rem, __popcll(ma & lt), Stage{a, nn | kFirst, ka}, tcount);
-          if (cb) stage_put(stage, cap, base, nb, rem, na_ + __popcll(mb & lt), Stage{b, nn, kb}, tcount);
-        } else {
-          const unsigned long long base = wbase;
-          wbase += tot;
-          if (ca) stage_put(stage, cap, base, 0, rem, __popcll(ma & lt), Stage{a, nn | kFirst, ka}, tcount);
-          if (cb) stage_put(stage, cap, base, 0, rem, na_ + __popcll(mb & lt), Stage{b, nn, kb}, tcount);
+      __syncthreads();
+      const unsigned long long r0 = run_s[0], rem = run_s[1], nr = newrun_s;
+      const uint32_t nn = h.x;
+#pragma unroll
+      for (int j = 0; j < kJPairs; ++j) {
+        if (li[j] == 0xffffffffu) continue;
+        int sa, sb;
+        uint32_t a, b;
+        join_ends(tid + j * kBlock, same, sizeA, newA, sizeB, newB, ids, sa, sb, a, b);
+        const uint32_t la = li[j] & 0xffffu, lb = li[j] >> 16;
+        if (la != 0xffffu) {
+          const unsigned long long i = off_s[sa] + la, at = i < rem ? r0 + i : nr + (i - rem);
+          if (at < cap) stage[at] = Stage{a, nn | kFirst, make_key(dist[j], b)};
+        }
+        if (lb != 0xffffu) {
+          const unsigned long long i = off_s[sb] + lb, at = i < rem ? r0 + i : nr + (i - rem);
+          if (at < cap) stage[at] = Stage{b, nn, make_key(dist[j], a)};
         }
       }
     }
     __syncthreads();
+    if (last) cur ^= 1;
+    t = tn;
+    c = cn;
   }
-  stage_abandon(stage, cap, wbase, wend, lane);
+  // unused rest of the staging run
+  if (tid < 64) {
+    const unsigned long long from = (unsigned long long)__shfl((long long)wbase, 63),
+                             to = (unsigned long long)__shfl((long long)wend, 63);
+    for (unsigned long long i = from + lane; i < to && i < cap; i += 64) stage[i].target = kNoRow;
+  }
   for (int o = 32; o > 0; o >>= 1) pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
   if (lane == 0 && pairs_local) atomicAdd(npairs, pairs_local);
 }
 
-__global__ void k_bucket(const Stage* __restrict__ stage, unsigned long long m, const int32_t* __restrict__ toff,
-                         uint32_t* __restrict__ cursor, uint64_t* __restrict__ bkey, uint32_t* __restrict__ bsrc) {
-  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  Stage s = stage[i];
-  if (s.target == kNoTarget) return;
-  uint32_t at = atomicAdd(&cursor[s.target], 1u);
-  bkey[size_t(toff[s.target]) + at] = s.key;
-  bsrc[size_t(toff[s.target]) + at] = s.src;
+// Staged candidates into their targets' buckets.  Consecutive slots with one
+// target (a task's candidates for one row) form a segment within each
+// wavefront; the segment's first lane reserves its bucket slots with one
+// atomic.
+__global__ void k_join_scatter(const Stage* __restrict__ stage, unsigned long long m, const int32_t* __restrict__ toff,
+                               uint32_t* __restrict__ cursor, uint64_t* __restrict__ bkey,
+                               uint32_t* __restrict__ bsrc) {
+  const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  Stage v{kNoRow, 0u, 0ull};
+  if (i < m) v = stage[i];
+  const bool valid = v.target != kNoRow;
+  const uint32_t prev = uint32_t(__shfl_up(int(v.target), 1));
+  const bool lead = valid && (lane == 0 || prev != v.target);
+  const uint64_t leads = __ballot(lead), breaks = __ballot(lead || !valid);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  const int my_lead = 63 - __clzll(leads & upto);  // valid lanes have a leader at or below
+  uint32_t base = 0;
+  if (lead) {
+    const uint64_t after = lane == 63 ? 0ull : (breaks & ~((2ull << lane) - 1));
+    const int end = after ? __ffsll((long long)after) - 1 : 64;
+    base = atomicAdd(&cursor[v.target], uint32_t(end - lane));
+  }
+  base = uint32_t(__shfl(int(base), valid ? my_lead : lane));
+  if (valid) {
+    const size_t o = size_t(toff[v.target]) + base + uint32_t(lane - my_lead);
+    bkey[o] = v.key;
+    bsrc[o] = v.src;
+  }
 }
 
 // Candidate tags: existing entries carry their flag (0/1); candidates carry
@@ -1158,83 +1342,78 @@ int gpu_refine_iter(efg_index* ix) {
       !ctr.alloc(3))
     return fail(EFG_ERR_MEMORY, "out of device memory (join)");
   k_worst<<<grid_for(n), 256, 0, st>>>(bs.pool.p, bs.fill.p, n, L, worst.p);
-  // staging capacity: two candidates per pair at most
-  hipMemsetAsync(ctr.p, 0, 16, st);
+  hipMemsetAsync(ctr.p, 0, 24, st);
   k_pair_bound<<<grid_for(n), 256, 0, st>>>(n, bs.new_off.p, bs.old_off.p, ctr.p);
-  unsigned long long bound = 0;
-  hipMemcpyAsync(&bound, ctr.p, 8, hipMemcpyDeviceToHost, st);
-  if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
-  DevBuf<Stage> stage;
-  // join tasks over blocks of BR rows; two blocks of padded rows in LDS
-  int BR = kJoinMaxBR;
-  while (BR > 4 && size_t(2) * BR * (D + 4) * 4 > 64 * 1024) BR >>= 1;
-  const size_t lds_join = size_t(2) * BR * (D + 4) * 4;
   DevBuf<int32_t> tcnt, tskoff;
   if (!tcnt.alloc(n) || !tskoff.alloc(n + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (join tasks)");
-  hipMemsetAsync(ctr.p + 2, 0, 8, st);
-  k_join_count<<<grid_for(n), 256, 0, st>>>(n, bs.new_off.p, bs.old_off.p, BR, tcnt.p, ctr.p + 2);
+  k_join_count<<<grid_for(n), 256, 0, st>>>(n, bs.new_off.p, bs.old_off.p, tcnt.p, ctr.p + 2);
   if (int rc = device_exclusive_scan(tcnt.p, int(n), tskoff.p, scal.p + 2, sums, st)) return rc;
+  unsigned long long bound = 0, rows = 0;
   int ntask = 0;
+  hipMemcpyAsync(&bound, ctr.p, 8, hipMemcpyDeviceToHost, st);
+  hipMemcpyAsync(&rows, ctr.p + 2, 8, hipMemcpyDeviceToHost, st);
   hipMemcpyAsync(&ntask, scal.p + 2, 4, hipMemcpyDeviceToHost, st);
   if (int rc = check_hip(hipStreamSynchronize(st), "join tasks")) return rc;
-  DevBuf<uint64_t> tasks;
-  if (!tasks.alloc(size_t(ntask) + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (join tasks)");
-  k_join_tasks<<<grid_for(n), 256, 0, st>>>(n, bs.new_off.p, bs.old_off.p, BR, tskoff.p, tasks.p);
-  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_join_tiles), hipFuncAttributeMaxDynamicSharedMemorySize,
-                      int(lds_join));
+  ix->counters[6] += rows;
   int dev = 0, ncu = 256;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  const int per_cu = std::max(1, std::min(8, int((160 * 1024) / (lds_join + 1024))));
-  const size_t join_grid = std::min<size_t>(size_t(ntask), size_t(ncu) * per_cu);
-  // two candidates per pair at most, plus one partly used run per wavefront
-  size_t cap = size_t(2 * bound) + join_grid * (kBlock / 64) * kStageRun + 1024;
+  // eight task ranges, one per XCD: a multiple of eight workgroups
+  const size_t join_grid = size_t((ncu * 4 + 7) / 8) * 8;
+  // two candidates per pair at most, plus one partly used run per workgroup
+  const size_t cap = size_t(2 * bound) + join_grid * kStageRun + 1024;
+  if (!bs.jhdr.ensure(size_t(ntask) + 1) || !bs.jtids.ensure((size_t(ntask) + 1) * 64) ||
+      !bs.jstage.ensure(cap * sizeof(Stage)))
+    return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
+  Stage* stage = reinterpret_cast<Stage*>(bs.jstage.p);
+  k_join_tasks<<<unsigned((size_t(n) * 64 + 255) / 256), 256, 0, st>>>(n, bs.new_off.p, bs.new_ids.p, bs.old_off.p,
+                                                                      bs.old_ids.p, tskoff.p, bs.jhdr.p, bs.jtids.p);
+  // one chunk of at most 64 columns of both blocks' rows in LDS
+  const int DC = std::min(D, 64);
+  const size_t lds_join = size_t(64) * (DC + 4) * 4;
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_join_tiles<4>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                      int(lds_join));
   unsigned long long h_ctr[2] = {0, 0};
-  for (int attempt = 0; attempt < 3; ++attempt) {
-    if (!stage.alloc(cap)) return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
-    hipMemsetAsync(ctr.p, 0, 16, st);
-    hipMemsetAsync(tcount.p, 0, size_t(n) * 4, st);
-    if (ntask) {
-      k_join_tiles<<<unsigned(join_grid), kBlock, lds_join, st>>>(ix->data.p, D, BR, uint32_t(ntask), tasks.p, bs.new_off.p,
-                                                   bs.new_ids.p, bs.old_off.p, bs.old_ids.p, worst.p, stage.p,
-                                                   (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
-    }
-    hipMemcpyAsync(h_ctr, ctr.p, 16, hipMemcpyDeviceToHost, st);
-    if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
-    if (h_ctr[0] <= cap) break;
-    cap = size_t(h_ctr[0]) + size_t(h_ctr[0]) / 8 + 1024;
-    if (attempt == 2) return fail(EFG_ERR_STATE, "join staging did not converge");
-  }
+  hipMemsetAsync(ctr.p, 0, 16, st);
+  hipMemsetAsync(tcount.p, 0, size_t(n) * 4, st);
+  if (ntask)
+    k_join_tiles<4><<<unsigned(join_grid), kBlock, lds_join, st>>>(ix->data.p, D, uint32_t(ntask), bs.jhdr.p,
+                                                                   bs.jtids.p, worst.p, stage,
+                                                                   (unsigned long long)cap, ctr.p, tcount.p,
+                                                                   ctr.p + 1);
+  hipMemcpyAsync(h_ctr, ctr.p, 16, hipMemcpyDeviceToHost, st);
+  if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
   const unsigned long long nst = h_ctr[0];
-  {
-    unsigned long long rows = 0;
-    hipMemcpy(&rows, ctr.p + 2, 8, hipMemcpyDeviceToHost);
-    ix->counters[6] += rows;
-  }
+  // the capacity covers every candidate and every partly used run
+  if (nst > cap) return fail(EFG_ERR_STATE, "join staging overflow");
   ix->join_pairs.push_back(h_ctr[1]);
   ix->counters[3] += h_ctr[1];
   if (nst >= (1ull << 31)) return fail(EFG_ERR_STATE, "too many join candidates for one iteration");
   if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(tcount.p), int(n), toff.p, scal.p, sums, st)) return rc;
-  DevBuf<uint64_t> bkey;
-  DevBuf<uint32_t> bsrc;
-  if (!bkey.alloc(nst + 1) || !bsrc.alloc(nst + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (buckets)");
+  int nkept = 0;
+  hipMemcpyAsync(&nkept, scal.p, 4, hipMemcpyDeviceToHost, st);
+  if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
+  if (!bs.jbkey.ensure(size_t(nkept) + 1) || !bs.jbsrc.ensure(size_t(nkept) + 1))
+    return fail(EFG_ERR_MEMORY, "out of device memory (buckets)");
+  uint64_t* bkey = bs.jbkey.p;
+  uint32_t* bsrc = bs.jbsrc.p;
   hipMemsetAsync(cursor.p, 0, size_t(n) * 4, st);
-  if (nst) k_bucket<<<unsigned((nst + 255) / 256), 256, 0, st>>>(stage.p, nst, toff.p, cursor.p, bkey.p, bsrc.p);
+  if (nst) k_join_scatter<<<unsigned((nst + 255) / 256), 256, 0, st>>>(stage, nst, toff.p, cursor.p, bkey, bsrc);
   hipMemsetAsync(bs.found.p, 0, n, st);
   {
     // one wavefront per target; the LDS window holds at least 2L entries
     const unsigned grid = unsigned(std::min<size_t>((n + 3) / 4, size_t(ncu) * 8));
     if (2 * L <= 256)
-      k_merge_wave<256><<<grid, kBlock, 0, st>>>(n, L, p.check, toff.p, bkey.p, bsrc.p, bs.pool.p, bs.flag.p,
+      k_merge_wave<256><<<grid, kBlock, 0, st>>>(n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
                                                  bs.fill.p, bs.found.p);
     else if (2 * L <= 512)
-      k_merge_wave<512><<<grid, kBlock, 0, st>>>(n, L, p.check, toff.p, bkey.p, bsrc.p, bs.pool.p, bs.flag.p,
+      k_merge_wave<512><<<grid, kBlock, 0, st>>>(n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
                                                  bs.fill.p, bs.found.p);
     else if (2 * L <= 1024)
-      k_merge_wave<1024><<<grid * 2, 128, 0, st>>>(n, L, p.check, toff.p, bkey.p, bsrc.p, bs.pool.p, bs.flag.p,
+      k_merge_wave<1024><<<grid * 2, 128, 0, st>>>(n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
                                                    bs.fill.p, bs.found.p);
     else
-      k_merge_wave<2048><<<grid * 2, 128, 0, st>>>(n, L, p.check, toff.p, bkey.p, bsrc.p, bs.pool.p, bs.flag.p,
+      k_merge_wave<2048><<<grid * 2, 128, 0, st>>>(n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
                                                    bs.fill.p, bs.found.p);
     if (int rc = check_hip(hipStreamSynchronize(st), "join merge")) return rc;
   }
@@ -1303,6 +1482,11 @@ int gpu_finalize_graph(efg_index* ix) {
   if (!ix->goff.alloc(n + 1) || !ix->gids.alloc(size_t(n) * K)) return fail(EFG_ERR_MEMORY, "out of device memory (graph)");
   k_graph_out<<<grid_for(n + 1), 256, 0, ix->stream>>>(n, bs.L, K, bs.pool.p, bs.fill.p, ix->goff.p, ix->gids.p);
   if (int rc = check_hip(hipStreamSynchronize(ix->stream), "graph")) return rc;
+  bs.jhdr.reset();
+  bs.jtids.reset();
+  bs.jstage.reset();
+  bs.jbkey.reset();
+  bs.jbsrc.reset();
   ix->nnz = size_t(n) * K;
   ix->max_deg = K;
   return EFG_OK;

@@ -87,6 +87,30 @@ def test_build_stages_match_oracle(cuda):
     np.testing.assert_array_equal(g.graph_matrix(), o.graph_matrix())
 
 
+@pytest.mark.parametrize("d", [100, 200])
+def test_build_float_data_long_lists_matches_oracle(cuda, d):
+    # non-integer rows (every FMA rounds), d % 32 != 0 (tail chunk first),
+    # more than 64 columns (several column chunks per join task) and lists
+    # longer than one 32-row block (tasks with I != J)
+    base, _ = helpers.mixture(n=6000, nq=1, d=d, centres=30, sigma=0.05, hi=1.0, rounded=False)
+    g, o = _pair(base, trees=4, ml=6, epochs=5, check=40, L=60, K=20, S=12)
+    g.build_stage(0)
+    o.build_trees()
+    g.build_stage(1)
+    o.merge_init()
+    _cmp_state(g, o)
+    longest = 0
+    for it in range(5):
+        g.build_stage(2)
+        o.refine_iter()
+        _cmp_state(g, o)
+        on, _ = g.lists(0)
+        oo, _ = g.lists(1)
+        longest = max(longest, int(np.max(np.diff(on) + np.diff(oo))))
+    assert longest > 64
+    assert g.join_pairs() == o.join_pairs()
+
+
 def test_build_index_mixture_matches_oracle(cuda):
     base, _ = helpers.mixture(n=30_000, nq=1)
     g, o = _pair(base, trees=8, ml=8, epochs=6)
