@@ -335,34 +335,41 @@ __global__ void __launch_bounds__(kBlock) k_replay(const float* __restrict__ X, 
   }
 }
 
-// Merge distances as leaf tiles: one wavefront per leaf bucket stages the
-// leaf's member rows and, PB at a time, its probers' rows in LDS (stride
-// D + 4) and writes dist[record * LM + member] for every record of the
+// Logical column l of the reference's accumulation order (the D % 32 tail
+// first, then [0, D - D % 32)) -> physical column of a padded row.
+__device__ inline int acc_col(int l, int D) {
+  const int DR = D & 31;
+  return l < DR ? D - DR + l : l - DR;
+}
+
+// Merge distances as leaf tiles: one wavefront per leaf bucket takes the
+// bucket's records PB at a time (PB x members <= 256 pairs, four per lane)
+// and, one chunk of DC logical columns after the other, stages the leaf's
+// member rows and the batch's prober rows in LDS (row stride DC + 4); each
+// lane keeps the reference's eight FMA chains for its pairs across the
+// chunks.  It writes dist[record * LM + member] for every record of the
 // bucket (sorted order), and pslot[pos * nlev + depth(X) - ml] = (record,
 // leaf) for the prober side of the replay.  Squared L2 is symmetric bit for
 // bit, so one value serves both endpoints.
-__global__ void k_leaf_dist(const float* __restrict__ X, int D, const uint2* __restrict__ range,
-                            const uint32_t* __restrict__ leaf_t, const int32_t* __restrict__ boff, int M,
-                            const uint64_t* __restrict__ rec, int LM, int PB, int ml, int nlev,
-                            float* __restrict__ dist, uint2* __restrict__ pslot) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ uint32_t pids[4][8];  // prober ids of the batch, per wavefront (PB <= 8, at most 4 wavefronts)
-  const int RS = D + 4, D4 = D >> 2;
+__global__ void __launch_bounds__(kBlock) k_leaf_dist(const float* __restrict__ X, int D, int DC,
+                                                      const uint2* __restrict__ range,
+                                                      const uint32_t* __restrict__ leaf_t,
+                                                      const int32_t* __restrict__ boff, int M,
+                                                      const uint64_t* __restrict__ rec, int LM, int PB, int ml,
+                                                      int nlev, float* __restrict__ dist, uint2* __restrict__ pslot) {
+  extern __shared__ __align__(16) float4 tiles_s[];
+  __shared__ uint32_t pids[kBlock / 64][64];  // prober ids of the batch (PB <= 64)
+  const int RS4 = (DC + 4) >> 2, nch = (D + DC - 1) / DC;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  float* rows_m = reinterpret_cast<float*>(smem_raw) + size_t(w) * (LM + PB) * RS;
-  float* rows_p = rows_m + size_t(LM) * RS;
+  float4* rows_m = tiles_s + size_t(w) * (LM + PB) * RS4;
+  float4* rows_p = rows_m + size_t(LM) * RS4;
   for (int l = blockIdx.x * nw + w; l < M; l += gridDim.x * nw) {
     const int b0 = boff[l], R = boff[l + 1] - b0;
     if (R == 0) continue;
     const uint2 r = range[l];
     const int m = int(r.y - r.x);
-    for (int i = lane; i < m * D4; i += 64) {
-      const int rr = i / D4, c = i - rr * D4;
-      *reinterpret_cast<float4*>(rows_m + size_t(rr) * RS + 4 * c) =
-          reinterpret_cast<const float4*>(X + size_t(leaf_t[r.x + rr]) * D)[c];
-    }
     for (int rb = 0; rb < R; rb += PB) {
-      const int nb = min(PB, R - rb);
+      const int nb = min(PB, R - rb), cells = nb * m;
       if (lane < nb) {
         const uint64_t v = rec[b0 + rb + lane];
         const uint32_t pos = uint32_t(v & 0xffffffffu);
@@ -370,16 +377,55 @@ __global__ void k_leaf_dist(const float* __restrict__ X, int D, const uint2* __r
         pids[w][lane] = leaf_t[pos];
         pslot[size_t(pos) * nlev + (dep - ml)] = make_uint2(uint32_t(b0 + rb + lane), uint32_t(l));
       }
-      wave_sync();
-      for (int i = lane; i < nb * D4; i += 64) {
-        const int rr = i / D4, c = i - rr * D4;
-        *reinterpret_cast<float4*>(rows_p + size_t(rr) * RS + 4 * c) =
-            reinterpret_cast<const float4*>(X + size_t(pids[w][rr]) * D)[c];
+      float acc[4][8];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[j][e] = 0.f;
+      for (int c = 0; c < nch; ++c) {
+        const int l0 = c * DC, w4 = (min(DC, D - l0)) >> 2;
+        wave_sync();
+        for (int i = lane; i < (m + nb) * w4; i += 64) {
+          const int rr = i / w4, c4 = i - rr * w4;
+          const uint32_t id = rr < m ? leaf_t[r.x + rr] : pids[w][rr - m];
+          rows_m[(rr < m ? rr : LM + rr - m) * RS4 + c4] =
+              *reinterpret_cast<const float4*>(X + size_t(id) * D + acc_col(l0 + 4 * c4, D));
+        }
+        wave_sync();
+        const int nb8 = w4 >> 1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int t = lane + 64 * j;
+          if (t < cells) {
+            const int b = t / m, jj = t - b * m;
+            const float4* x = rows_m + jj * RS4;
+            const float4* y = rows_p + b * RS4;
+#pragma unroll 1
+            for (int e8 = 0; e8 < nb8; ++e8) {
+              const float4 x0 = x[2 * e8], x1 = x[2 * e8 + 1], y0 = y[2 * e8], y1 = y[2 * e8 + 1];
+              float d;
+              d = x0.x - y0.x; acc[j][0] = __builtin_fmaf(d, d, acc[j][0]);
+              d = x0.y - y0.y; acc[j][1] = __builtin_fmaf(d, d, acc[j][1]);
+              d = x0.z - y0.z; acc[j][2] = __builtin_fmaf(d, d, acc[j][2]);
+              d = x0.w - y0.w; acc[j][3] = __builtin_fmaf(d, d, acc[j][3]);
+              d = x1.x - y1.x; acc[j][4] = __builtin_fmaf(d, d, acc[j][4]);
+              d = x1.y - y1.y; acc[j][5] = __builtin_fmaf(d, d, acc[j][5]);
+              d = x1.z - y1.z; acc[j][6] = __builtin_fmaf(d, d, acc[j][6]);
+              d = x1.w - y1.w; acc[j][7] = __builtin_fmaf(d, d, acc[j][7]);
+            }
+          }
+        }
       }
-      wave_sync();
-      for (int t = lane; t < nb * m; t += 64) {
-        const int b = t / m, j = t - b * m;
-        dist[size_t(b0 + rb + b) * LM + j] = l2_thread(rows_m + size_t(j) * RS, rows_p + size_t(b) * RS, D);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int t = lane + 64 * j;
+        if (t < cells) {
+          const int b = t / m, jj = t - b * m;
+          float sdist = acc[j][0];
+#pragma unroll
+          for (int e = 1; e < 8; ++e) sdist = sdist + acc[j][e];
+          dist[size_t(b0 + rb + b) * LM + jj] = sdist;
+        }
       }
       wave_sync();
     }
@@ -639,12 +685,6 @@ __device__ inline bool join_ends(int k, bool same, int sizeA, int newA, int size
   return true;
 }
 
-// Logical column l of the reference's accumulation order (the D % 32 tail
-// first, then [0, D - D % 32)) -> physical column of a padded row.
-__device__ inline int join_col(int l, int D) {
-  const int DR = D & 31;
-  return l < DR ? D - DR + l : l - DR;
-}
 
 // Load k of this thread for the rows of one chunk (DC logical columns of up
 // to 64 rows, float4 i = threadIdx.x + k * kBlock is row i / (DC / 4)).
@@ -661,7 +701,7 @@ __device__ __forceinline__ float4 join_load(const float* __restrict__ X, int D, 
   const bool in = join_slot(k, DC >> 2, (min(DC, D - l0)) >> 2, sizeA, sizeB, h.z != 0, r, c4);
   // out-of-range slots load row 0, column 0 (never stored)
   const uint32_t id = in ? ids[r] : 0u;
-  return *reinterpret_cast<const float4*>(X + size_t(id) * D + (in ? join_col(l0 + 4 * c4, D) : 0));
+  return *reinterpret_cast<const float4*>(X + size_t(id) * D + (in ? acc_col(l0 + 4 * c4, D) : 0));
 }
 
 // Staging slots are reserved per workgroup in runs of at least kStageRun, so
@@ -1233,9 +1273,11 @@ int gpu_merge_init(efg_index* ix) {
   hipMemcpyAsync(h4, scal.p, 16, hipMemcpyDeviceToHost, st);
   if (int rc = check_hip(hipStreamSynchronize(st), "merge")) return rc;
   const int err_depth = h4[0], max_depth = h4[2], LM = h4[3];
-  // distance tiles: LDS per wavefront (LM + PB) rows of D + 4 floats
-  const int PB = std::max(1, std::min(8, 64 / std::max(1, LM)));
-  const size_t wave_lds = size_t(LM + PB) * (D + 4) * 4;
+  // distance tiles: per wavefront (LM + PB) rows of one chunk of DC columns
+  // in LDS, PB probers per batch with PB x LM <= 256 pairs
+  const int DCm = std::min(D, 64);
+  const int PB = std::max(1, std::min(64, 256 / std::max(1, LM)));
+  const size_t wave_lds = size_t(LM + PB) * (DCm + 4) * 4;
   const int tile_waves = int(std::min<size_t>(4, (64 * 1024) / std::max<size_t>(wave_lds, 1)));
   const bool tiles = LM > 0 && tile_waves >= 1;
   const int nlev = std::max(1, max_depth - p.merge_level + 1);
@@ -1281,7 +1323,7 @@ int gpu_merge_init(efg_index* ix) {
     if (tiles) {
       if (!mdist.ensure(size_t(total) * LM + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (merge distances)");
       k_leaf_dist<<<unsigned(ncu_m * 8), 64 * tile_waves, wave_lds * tile_waves, st>>>(
-          ix->data.p, D, ix->range.p, lt, boff.p, M, rec.p, LM, PB, p.merge_level, nlev, mdist.p, pslot.p);
+          ix->data.p, D, DCm, ix->range.p, lt, boff.p, M, rec.p, LM, PB, p.merge_level, nlev, mdist.p, pslot.p);
       k_replay_dist<<<grid_for(n), kBlock, lds, st>>>(ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la, n,
                                                       p.merge_level, S, boff.p, rec.p, mdist.p, pslot.p, LM, nlev,
                                                       hkey.p, hsize.p, extra.p, ecount.p, nd.p);
