@@ -2,7 +2,8 @@
 //
 // Stage 1, k_descend: one thread per (query, tree) walks its tree depth first,
 // query side first, and records the first `lsize` leaves (getSearchNodeList,
-// kdtreeub_index.hpp:332-346).  The walk is stackless (parent links).
+// kdtreeub_index.hpp:332-346), with the pending far children on a short
+// LDS stack and parent links past its depth.
 //
 // Stage 2, query order: queries are ordered by the position of their first
 // leaf in tree 0 (a counting sort), so queries that read the same base rows
@@ -40,28 +41,46 @@
 
 namespace efg {
 
-__global__ void k_descend(const float* __restrict__ Q, size_t q_stride, int nq, int T, int lsize,
-                          const int32_t* __restrict__ roots, const SNode* __restrict__ sn,
-                          int32_t* __restrict__ leaves, const int32_t* __restrict__ qlist,
-                          const int* __restrict__ nlist) {
+// The far children passed on the way down are kept on a per-thread stack in
+// LDS (the last kDescStack of them), so the next leaf is reached from the
+// deepest pending far child without walking back up; past the stack's depth
+// the walk climbs the parent links as before.
+constexpr int kDescStack = 16;
+__global__ void __launch_bounds__(256) k_descend(const float* __restrict__ Q, size_t q_stride, int nq, int T,
+                                                 int lsize, const int32_t* __restrict__ roots,
+                                                 const SNode* __restrict__ sn, int32_t* __restrict__ leaves,
+                                                 const int32_t* __restrict__ qlist, const int* __restrict__ nlist) {
+  __shared__ int32_t stk[kDescStack][256];
   const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int nw = qlist ? *nlist : nq;  // queries of the list only, when given
   if (gid >= size_t(nw) * T) return;
+  const int tid = threadIdx.x;
   const int qi = qlist ? qlist[gid / T] : int(gid / T), t = int(gid % T);
   const float* q = Q + size_t(qi) * q_stride;
   int32_t* out = leaves + size_t(qi) * size_t(lsize) * T;  // [ni][t]
+  int top = 0, bottom = 0;  // stack entries [bottom, top), slot i % kDescStack
   int node = roots[t];
   SNode cur = sn[node];
   int cnt = 0;
-  while (cur.left >= 0) {  // first leaf, query side first
-    node = (q[cur.dim] < cur.val) ? cur.left : cur.right;
-    cur = sn[node];
-  }
   for (;;) {
+    while (cur.left >= 0) {  // down to a leaf, query side first
+      const bool left = q[cur.dim] < cur.val;
+      stk[top % kDescStack][tid] = left ? cur.right : cur.left;
+      if (++top - bottom > kDescStack) ++bottom;
+      node = left ? cur.left : cur.right;
+      cur = sn[node];
+    }
     out[size_t(cnt) * T + t] = node;
     if (++cnt == lsize) break;
+    if (top > bottom) {  // deepest pending far child
+      --top;
+      node = stk[top % kDescStack][tid];
+      cur = sn[node];
+      continue;
+    }
+    // stack exhausted: up to the first ancestor whose second child is unvisited
     bool more = false;
-    while (cur.parent >= 0) {  // up to the first ancestor whose second child is unvisited
+    while (cur.parent >= 0) {
       const int p = cur.parent;
       const SNode pd = sn[p];
       const int first = (q[pd.dim] < pd.val) ? pd.left : pd.right;
@@ -75,10 +94,6 @@ __global__ void k_descend(const float* __restrict__ Q, size_t q_stride, int nq, 
       cur = pd;
     }
     if (!more) break;
-    while (cur.left >= 0) {
-      node = (q[cur.dim] < cur.val) ? cur.left : cur.right;
-      cur = sn[node];
-    }
   }
   for (; cnt < lsize; ++cnt) out[size_t(cnt) * T + t] = -1;
 }
