@@ -1264,7 +1264,11 @@ int run_pass(efg_index* ix, const SearchCfg& c, const LearnArgs& la, bool bf16, 
     if (c2.logcap < c.logcap) c2.logcap = c.logcap;
     if (c2.logcap > 26) c2.logcap = 26;
     size_t lds2 = search_lds_bytes(c2, true);
-    unsigned grid2 = 128;
+    // as many workgroups as LDS allows, with at most 1 GiB of tables
+    const int per_cu2 = std::max(1, std::min(8, int(lds_cap / std::max<size_t>(lds2, 1))));
+    const size_t tab_bytes = size_t(4) << c2.logcap;
+    const unsigned grid2 = unsigned(std::min<size_t>(size_t(ncu) * per_cu2,
+                                                     std::max<size_t>(128, (size_t(1) << 30) / tab_bytes)));
     if (!w.gtab.ensure(size_t(grid2) << c2.logcap)) return fail(EFG_ERR_MEMORY, "out of device memory (search table)");
     auto kern2 = pick(true);
     hipFuncSetAttribute(reinterpret_cast<const void*>(kern2), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2));
