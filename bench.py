@@ -245,6 +245,18 @@ def main() -> int:
         torch.cuda.synchronize()
         b_ms = (time.perf_counter() - t0) * 1e3 / bsteps
         bres = bids.cpu().numpy()
+        # the same batch captured into a HIP graph and replayed
+        plan = gix.batch_plan(qbd.data_ptr(), nb, d, pools, ks, 100, bids.data_ptr(), bd.data_ptr(),
+                              stream=stream.cuda_stream)
+        plan.run(stream.cuda_stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(bsteps):
+            plan.run(stream.cuda_stream)
+        torch.cuda.synchronize()
+        g_ms = (time.perf_counter() - t0) * 1e3 / bsteps
+        graph_same = bool((bids.cpu().numpy() == bres).all())
+        plan.close()
         t = time.perf_counter()
         bgt = ground_truth(qbd, nb, 100)
         bgt_s = time.perf_counter() - t
@@ -254,6 +266,8 @@ def main() -> int:
             per_k[str(kk)] = round(rec(bres[sel], bgt[sel], kk), 5)
         hits = sum(len(set(a[:kk].tolist()) & set(b[:kk].tolist())) for a, b, kk in zip(bres, bgt, ks.tolist()))
         forms["batched"] = {"queries": nb, "ms_per_batch": round(b_ms, 3), "qps": round(nb / (b_ms / 1e3), 1),
+                            "graph_ms_per_batch": round(g_ms, 3), "graph_qps": round(nb / (g_ms / 1e3), 1),
+                            "graph_identical": graph_same,
                             "recall_at_k": round(hits / float(ks.sum()), 5), "recall_at_k_by_k": per_k,
                             "pools": [200, 400, 800, 1600], "ks": [1, 10, 100], "uniform_query_frac": 0.3,
                             "extend": "pool // 6", "ground_truth_s": round(bgt_s, 3)}

@@ -71,6 +71,10 @@ _SIGS = {
     "efg_knn_search_batch": (c_int, [c_vp, c_f32p, c_sz, c_sz, c_intp, c_intp, c_intp, c_int, c_i32p, c_f32p, c_int]),
     "efg_knn_search_batch_device": (c_int, [c_vp, c_vp, c_sz, c_sz, c_intp, c_intp, c_intp, c_int, c_vp, c_vp, c_int,
                                             c_vp]),
+    "efg_batch_plan_create": (c_int, [c_vp, c_vp, c_sz, c_sz, c_intp, c_intp, c_intp, c_int, c_vp, c_vp, c_int,
+                                      c_vp, ctypes.POINTER(c_vp)]),
+    "efg_batch_plan_run": (c_int, [c_vp, c_vp]),
+    "efg_batch_plan_destroy": (c_int, [c_vp]),
     "efg_set_search_threads": (c_int, [c_vp, c_int]),
     "efg_autotune_search": (c_int, [c_vp, c_int, c_f32p, c_sz, c_sz, c_intp]),
     "efg_mlp_param_count": (c_int, []),

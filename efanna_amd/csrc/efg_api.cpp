@@ -427,6 +427,27 @@ int efg_knn_search_batch_device(efg_index* ix, const float* d_queries, size_t nq
   return efg::search_batch(ix, d_queries, nq, q_stride, pool, extend, k, out_stride, d_ids, d_dists, tune != 0, st);
 }
 
+int efg_batch_plan_create(efg_index* ix, const float* d_queries, size_t nq, size_t q_stride, const int* pool,
+                          const int* extend, const int* k, int out_stride, int32_t* d_ids, float* d_dists, int tune,
+                          void* stream, efg_batch_plan** plan) {
+  if (!ix || !plan) return fail(EFG_ERR_ARG, "null argument");
+  *plan = nullptr;
+  if (!d_queries || !pool || !k || !d_ids) return fail(EFG_ERR_ARG, "null buffer");
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ix->stream;
+  return efg::batch_plan_create(ix, d_queries, nq, q_stride, pool, extend, k, out_stride, d_ids, d_dists, tune != 0,
+                                st, plan);
+}
+
+int efg_batch_plan_run(efg_batch_plan* plan, void* stream) {
+  if (!plan) return fail(EFG_ERR_ARG, "null plan");
+  return efg::batch_plan_run(plan, static_cast<hipStream_t>(stream));
+}
+
+int efg_batch_plan_destroy(efg_batch_plan* plan) {
+  efg::batch_plan_destroy(plan);
+  return EFG_OK;
+}
+
 int efg_knn_search_batch(efg_index* ix, const float* queries, size_t nq, size_t q_stride, const int* pool,
                          const int* extend, const int* k, int out_stride, int32_t* ids, float* dists, int tune) {
   if (!ix) return fail(EFG_ERR_ARG, "null index");

@@ -190,6 +190,14 @@ int exact_knn_mfma(const float* X, size_t n, int d, size_t x_stride, const float
 // (pool, extend_to, k) class first times 64/128/256 threads per query.
 int search_batch(efg_index* ix, const float* dq, size_t nq, size_t q_stride, const int* pool, const int* ext,
                  const int* kq, int out_stride, int32_t* d_ids, float* d_dists, bool tune, hipStream_t st);
+// The same batch captured into a HIP graph (after one eager run, and tuning
+// when asked); the graph refers to dq, d_ids and d_dists and to the plan's
+// own workspace.
+int batch_plan_create(efg_index* ix, const float* dq, size_t nq, size_t q_stride, const int* pool, const int* ext,
+                      const int* kq, int out_stride, int32_t* d_ids, float* d_dists, bool tune, hipStream_t st,
+                      efg_batch_plan** out);
+int batch_plan_run(efg_batch_plan* pl, hipStream_t st);
+void batch_plan_destroy(efg_batch_plan* pl);
 // Times the search with 64, 128 and 256 threads per query on the host queries
 // (a sample of them) for the current parameters and mode; keeps the fastest.
 int search_autotune(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int* chosen,

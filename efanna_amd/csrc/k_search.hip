@@ -1186,11 +1186,17 @@ using SearchKernel = void (*)(SearchCfg, LearnArgs, const float*, size_t, const 
 // qlist is null; *nlist_dev of them otherwise): descent, then the search
 // kernel with the visited table in LDS, then the queries whose table
 // overflowed again with a table in global memory.
-int run_pass(efg_index* ix, const SearchCfg& c, const LearnArgs& la, bool bf16, const float* dq, size_t nq,
-             size_t q_stride, int32_t* d_ids, float* d_dists, uint64_t* d_ndist, const int32_t* qlist,
-             const int* nlist_dev, hipEvent_t* ev, hipStream_t st, bool count_work = true) {
+__global__ void k_zero_i32(int* p, int m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) p[i] = 0;
+}
+
+// (The pass clears its counters with a kernel, not a memset, so that a
+// captured batch replays correctly.)
+int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const LearnArgs& la, bool bf16,
+                const float* dq, size_t nq, size_t q_stride, int32_t* d_ids, float* d_dists, uint64_t* d_ndist,
+                const int32_t* qlist, const int* nlist_dev, hipEvent_t* ev, hipStream_t st, bool count_work) {
   unsigned long long* work = count_work ? ix->work.p : nullptr;
-  efg::Workspace& w = ix->ws;
   const int nb_q = 4096;
   if (!w.leaves.ensure(nq * size_t(c.nleaf)) || !w.overflow.ensure(2 * nq) || !w.nover.ensure(2) ||
       !w.ndist.ensure(3) || !w.qorder.ensure(nq) || !w.qkey.ensure(nq) || !w.qhist.ensure(nb_q + 1) ||
@@ -1200,7 +1206,7 @@ int run_pass(efg_index* ix, const SearchCfg& c, const LearnArgs& la, bool bf16, 
   const size_t lds_cap = 150 * 1024;
   const bool table_in_lds = lds <= lds_cap;
   if (!table_in_lds && search_lds_bytes(c, true) > lds_cap) return fail(EFG_ERR_ARG, "pool too large for one workgroup");
-  if (int rc = check_hip(hipMemsetAsync(w.nover.p, 0, 2 * sizeof(int), st), "memset")) return rc;
+  k_zero_i32<<<1, 64, 0, st>>>(w.nover.p, 2);
   if (ev) hipEventRecord(ev[0], st);
   {
     size_t threads = nq * size_t(c.T);
@@ -1286,6 +1292,13 @@ int run_pass(efg_index* ix, const SearchCfg& c, const LearnArgs& la, bool bf16, 
   }
   if (ev) hipEventRecord(ev[3], st);
   return check_hip(hipGetLastError(), "search launch");
+}
+
+int run_pass(efg_index* ix, const SearchCfg& c, const LearnArgs& la, bool bf16, const float* dq, size_t nq,
+             size_t q_stride, int32_t* d_ids, float* d_dists, uint64_t* d_ndist, const int32_t* qlist,
+             const int* nlist_dev, hipEvent_t* ev, hipStream_t st, bool count_work = true) {
+  return run_pass_ws(ix, ix->ws, c, la, bf16, dq, nq, q_stride, d_ids, d_dists, d_ndist, qlist, nlist_dev, ev, st,
+                     count_work);
 }
 
 // Probe search parameters: set explicitly, else pool = max(k, init_num / 6),
@@ -1502,31 +1515,39 @@ __global__ void k_fill_results(int32_t* __restrict__ ids, float* __restrict__ d,
   if (d) d[i] = 3.402823466e38f;
 }
 
-int search_batch(efg_index* ix, const float* dq, size_t nq, size_t q_stride, const int* pool, const int* ext,
-                 const int* kq, int out_stride, int32_t* d_ids, float* d_dists, bool tune, hipStream_t st) {
-  if (int rc = search_prepare(ix, nq, q_stride, st)) return rc;
-  if (nq == 0) return EFG_OK;
-  if (out_stride <= 0) return fail(EFG_ERR_ARG, "invalid output stride");
-  const size_t m = nq * size_t(out_stride);
-  k_fill_results<<<unsigned((m + 255) / 256), 256, 0, st>>>(d_ids, d_dists, m);
+// A heterogeneous batch after its classes are formed, their query lists are
+// on the device and their threads per query are chosen.
+struct BatchSetup {
+  std::vector<SearchCfg> cfg;
+  std::vector<int> offs;
+  size_t nq = 0;
+  int out_stride = 0;
+};
+
+int batch_setup(efg_index* ix, efg::Workspace& w, const float* dq, size_t nq, size_t q_stride, const int* pool,
+                const int* ext, const int* kq, int out_stride, int32_t* d_ids, float* d_dists, bool tune,
+                hipStream_t st, BatchSetup* b) {
   if (ix->s_method != 0) return fail(EFG_ERR_METHOD, "batched search runs method 0");
+  if (out_stride <= 0) return fail(EFG_ERR_ARG, "invalid output stride");
   std::vector<BatchClass> cls;
   if (int rc = batch_classes(ix, nq, pool, ext, kq, out_stride, &cls)) return rc;
   // class configurations and their query lists (concatenated on the device)
-  std::vector<SearchCfg> cfg(cls.size());
+  b->cfg.assign(cls.size(), SearchCfg{});
+  b->offs.clear();
+  b->nq = nq;
+  b->out_stride = out_stride;
   std::vector<int32_t> lists;
-  std::vector<int> counts, offs;
+  std::vector<int> counts;
   int max_nleaf = 0;
   for (size_t i = 0; i < cls.size(); ++i) {
-    if (int rc = make_cfg(ix, cls[i].k, cls[i].pool, cls[i].ext, ix->s_epochs, ix->s_trees, 0, 0, &cfg[i], nullptr))
+    if (int rc = make_cfg(ix, cls[i].k, cls[i].pool, cls[i].ext, ix->s_epochs, ix->s_trees, 0, 0, &b->cfg[i], nullptr))
       return rc;
-    cfg[i].out_stride = out_stride;
-    max_nleaf = std::max(max_nleaf, cfg[i].nleaf);
-    offs.push_back(int(lists.size()));
+    b->cfg[i].out_stride = out_stride;
+    max_nleaf = std::max(max_nleaf, b->cfg[i].nleaf);
+    b->offs.push_back(int(lists.size()));
     counts.push_back(int(cls[i].q.size()));
     lists.insert(lists.end(), cls[i].q.begin(), cls[i].q.end());
   }
-  efg::Workspace& w = ix->ws;
   if (!w.leaves.ensure(nq * size_t(max_nleaf)) || !w.blist.ensure(lists.size()) || !w.bcount.ensure(counts.size()))
     return fail(EFG_ERR_MEMORY, "out of device memory (batch)");
   // pageable sources: the calls return once the host data is staged
@@ -1536,17 +1557,13 @@ int search_batch(efg_index* ix, const float* dq, size_t nq, size_t q_stride, con
   if (int rc = check_hip(hipMemcpyAsync(w.bcount.p, counts.data(), counts.size() * 4, hipMemcpyHostToDevice, st),
                          "copy query counts"))
     return rc;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (tune) {
+  int rc = EFG_OK;
+  if (tune) {  // time 64/128/256 threads per query on each class, keep the fastest
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-  }
-  int rc = EFG_OK;
-  for (size_t i = 0; i < cls.size() && rc == EFG_OK; ++i) {
-    SearchCfg c = cfg[i];
-    const int32_t* ql = w.blist.p + offs[i];
-    const int* cnt = w.bcount.p + i;
-    if (tune) {  // time 64/128/256 threads per query on this class, keep the fastest
+    for (size_t i = 0; i < b->cfg.size() && rc == EFG_OK; ++i) {
+      SearchCfg c = b->cfg[i];
       float best = 0.f;
       int best_t = 0;
       for (int t : {64, 128, 256}) {
@@ -1555,8 +1572,8 @@ int search_batch(efg_index* ix, const float* dq, size_t nq, size_t q_stride, con
         float sum = 0.f;
         for (int rep = 0; rep < 3 && rc == EFG_OK; ++rep) {
           hipEventRecord(e0, st);
-          rc = run_pass(ix, c, LearnArgs{}, false, dq, nq, q_stride, d_ids, d_dists, nullptr, ql, cnt, nullptr, st,
-                        false);
+          rc = run_pass_ws(ix, w, c, LearnArgs{}, false, dq, nq, q_stride, d_ids, d_dists, nullptr,
+                           w.blist.p + b->offs[i], w.bcount.p + i, nullptr, st, false);
           hipEventRecord(e1, st);
           if (rc == EFG_OK) rc = check_hip(hipEventSynchronize(e1), "batch tuning");
           float ms = 0.f;
@@ -1570,14 +1587,92 @@ int search_batch(efg_index* ix, const float* dq, size_t nq, size_t q_stride, con
       }
       if (rc == EFG_OK) ix->tuned[{c.init_num, c.E, c.epochs, c.K, c.T, 0}] = best_t;
     }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+  }
+  for (auto& c : b->cfg) set_threads(c, choose_threads(ix, c));
+  return rc;
+}
+
+// The batch's launches only (no host synchronisation, no memset), so the
+// sequence can be captured into a graph.
+int batch_launch(efg_index* ix, efg::Workspace& w, const BatchSetup& b, const float* dq, size_t q_stride,
+                 int32_t* d_ids, float* d_dists, hipStream_t st) {
+  const size_t m = b.nq * size_t(b.out_stride);
+  k_fill_results<<<unsigned((m + 255) / 256), 256, 0, st>>>(d_ids, d_dists, m);
+  for (size_t i = 0; i < b.cfg.size(); ++i)
+    if (int rc = run_pass_ws(ix, w, b.cfg[i], LearnArgs{}, false, dq, b.nq, q_stride, d_ids, d_dists, nullptr,
+                             w.blist.p + b.offs[i], w.bcount.p + i, nullptr, st, true))
+      return rc;
+  return EFG_OK;
+}
+
+int search_batch(efg_index* ix, const float* dq, size_t nq, size_t q_stride, const int* pool, const int* ext,
+                 const int* kq, int out_stride, int32_t* d_ids, float* d_dists, bool tune, hipStream_t st) {
+  if (int rc = search_prepare(ix, nq, q_stride, st)) return rc;
+  if (nq == 0) return EFG_OK;
+  BatchSetup b;
+  if (int rc = batch_setup(ix, ix->ws, dq, nq, q_stride, pool, ext, kq, out_stride, d_ids, d_dists, tune, st, &b))
+    return rc;
+  return batch_launch(ix, ix->ws, b, dq, q_stride, d_ids, d_dists, st);
+}
+
+}  // namespace efg
+
+// A batch plan: the launch sequence of one heterogeneous batch captured into
+// a HIP graph, with a workspace of its own so that other searches on the
+// index cannot move the buffers the graph refers to.
+struct efg_batch_plan {
+  efg_index* ix = nullptr;
+  efg::Workspace ws;
+  efg::BatchSetup setup;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace efg {
+
+int batch_plan_create(efg_index* ix, const float* dq, size_t nq, size_t q_stride, const int* pool, const int* ext,
+                      const int* kq, int out_stride, int32_t* d_ids, float* d_dists, bool tune, hipStream_t st,
+                      efg_batch_plan** out) {
+  *out = nullptr;
+  if (nq == 0) return fail(EFG_ERR_ARG, "empty batch");
+  if (int rc = search_prepare(ix, nq, q_stride, st)) return rc;
+  auto* pl = new efg_batch_plan();
+  pl->ix = ix;
+  int rc = batch_setup(ix, pl->ws, dq, nq, q_stride, pool, ext, kq, out_stride, d_ids, d_dists, tune, st, &pl->setup);
+  // one eager run sizes the plan's workspace before the capture
+  if (rc == EFG_OK) rc = batch_launch(ix, pl->ws, pl->setup, dq, q_stride, d_ids, d_dists, st);
+  if (rc == EFG_OK) rc = check_hip(hipStreamSynchronize(st), "batch plan");
+  hipStream_t cs = nullptr;
+  if (rc == EFG_OK) rc = check_hip(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream");
+  if (rc == EFG_OK) {
+    rc = check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
     if (rc == EFG_OK) {
-      set_threads(c, choose_threads(ix, c));
-      rc = run_pass(ix, c, LearnArgs{}, false, dq, nq, q_stride, d_ids, d_dists, nullptr, ql, cnt, nullptr, st);
+      const int lrc = batch_launch(ix, pl->ws, pl->setup, dq, q_stride, d_ids, d_dists, cs);
+      const int erc = check_hip(hipStreamEndCapture(cs, &pl->graph), "end capture");
+      rc = lrc ? lrc : erc;
     }
   }
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
-  return rc;
+  if (rc == EFG_OK) rc = check_hip(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0), "instantiate");
+  if (cs) hipStreamDestroy(cs);
+  if (rc != EFG_OK) {
+    batch_plan_destroy(pl);
+    return rc;
+  }
+  *out = pl;
+  return EFG_OK;
+}
+
+int batch_plan_run(efg_batch_plan* pl, hipStream_t st) {
+  return check_hip(hipGraphLaunch(pl->exec, st), "batch plan launch");
+}
+
+void batch_plan_destroy(efg_batch_plan* pl) {
+  if (!pl) return;
+  if (pl->exec) hipGraphExecDestroy(pl->exec);
+  if (pl->graph) hipGraphDestroy(pl->graph);
+  delete pl;
 }
 
 int search_traces(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, const int32_t* d_plain,

@@ -97,6 +97,35 @@ def _p(a, t):
     return a.ctypes.data_as(t)
 
 
+class BatchPlan:
+    """Captured heterogeneous batch (efg_batch_plan_*); destroy before the index."""
+
+    def __init__(self, index, q_ptr, nq, q_stride, pool, k, out_stride, ids_ptr, d_ptr, extend, tune, stream):
+        pool = np.ascontiguousarray(pool, np.int32)
+        k = np.ascontiguousarray(k, np.int32)
+        ext = None if extend is None else np.ascontiguousarray(extend, np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        self._index = index
+        self.h = ctypes.c_void_p()
+        check(lib().efg_batch_plan_create(index.h, q_ptr, nq, q_stride, _p(pool, ip),
+                                          _p(ext, ip) if ext is not None else None, _p(k, ip), int(out_stride),
+                                          ids_ptr, d_ptr, int(tune), stream, ctypes.byref(self.h)))
+
+    def run(self, stream=None):
+        check(lib().efg_batch_plan_run(self.h, stream))
+
+    def close(self):
+        if self.h:
+            lib().efg_batch_plan_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class FIndex:
     def __init__(self, features, distance: Distance, params: IndexParams):
         if not isinstance(distance, _L2_CLASSES):
@@ -180,6 +209,12 @@ class FIndex:
         check(lib().efg_knn_search_batch_device(self.h, q_ptr, nq, q_stride, _p(pool, ip),
                                                 _p(ext, ip) if ext is not None else None, _p(k, ip), int(out_stride),
                                                 ids_ptr, d_ptr, int(tune), stream))
+
+    def batch_plan(self, q_ptr, nq, q_stride, pool, k, out_stride, ids_ptr, d_ptr=None, extend=None, tune=False,
+                   stream=None):
+        """A heterogeneous batch captured into a HIP graph over these device buffers: BatchPlan.run()
+        replays it (write the next queries into q_ptr first)."""
+        return BatchPlan(self, q_ptr, nq, q_stride, pool, k, out_stride, ids_ptr, d_ptr, extend, tune, stream)
 
     # --- launch tuning (no counterpart in the reference)
     def setSearchThreads(self, threads: int):

@@ -36,6 +36,7 @@ extern "C" {
 #define EFG_ERR_MEMORY -9  /* device allocation failed */
 
 typedef struct efg_index efg_index;
+typedef struct efg_batch_plan efg_batch_plan;
 
 /* KDTreeUbIndexParams (algorithm/kdtreeub_index.hpp:14-35) + IndexParams
  * (general/params.hpp:19-33).  `seed` keys the counter-based generator that
@@ -159,6 +160,19 @@ int efg_knn_search_batch(efg_index* ix, const float* queries, size_t nq, size_t 
 int efg_knn_search_batch_device(efg_index* ix, const float* d_queries, size_t nq, size_t q_stride, const int* pool,
                                 const int* extend, const int* k, int out_stride, int32_t* d_ids, float* d_dists,
                                 int tune, void* stream);
+/* A heterogeneous batch captured into a HIP graph (no counterpart in the
+ * reference), for serving loops that search batch after batch of the same
+ * shape: create runs the batch once (tuning first when tune != 0) on
+ * `stream`, then captures its launches; run replays them on `stream`.  The
+ * graph refers to d_queries, d_ids and d_dists: write the next batch's
+ * queries into d_queries before each run.  pool, extend and k are read at
+ * creation only.  A plan uses a workspace of its own; it must be destroyed
+ * before its index, and runs of one plan must not overlap. */
+int efg_batch_plan_create(efg_index* ix, const float* d_queries, size_t nq, size_t q_stride, const int* pool,
+                          const int* extend, const int* k, int out_stride, int32_t* d_ids, float* d_dists, int tune,
+                          void* stream, efg_batch_plan** plan);
+int efg_batch_plan_run(efg_batch_plan* plan, void* stream);
+int efg_batch_plan_destroy(efg_batch_plan* plan);
 
 /* Threads per query of the search kernels (no counterpart in the reference;
  * launch tuning only, results do not change): 64, 128 or 256 forces one;

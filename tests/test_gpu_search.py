@@ -259,3 +259,40 @@ def test_batch_heterogeneous_matches_oracle(cuda, tune):
         np.testing.assert_array_equal(ids[sel, :k], oids)
         np.testing.assert_array_equal(d[sel, :k].view(np.uint32), od.view(np.uint32))
         assert (ids[sel, k:] == -1).all()
+
+
+def test_batch_plan_graph_replays_match_eager(cuda):
+    """The batch captured into a HIP graph: every replay, with eager searches
+    and synchronisations in between and new queries written into the same
+    buffer, equals the eager batch on those queries."""
+    import torch
+    base, q, oix = helpers.oracle_index("mixture", n=20_000, nq=200, centres=50)
+    gix = _gpu_index(base, oix.trees(), oix.graph())
+    gix.setSearchParams(4, 300, 50, 8, -1, 0)
+    nq = q.shape[0]
+    rng = np.random.default_rng(12)
+    pools = rng.choice([60, 120, 240], size=nq).astype(np.int32)
+    ks = rng.choice([1, 10, 40], size=nq).astype(np.int32)
+    _, q2 = helpers.mixture(n=20_000, nq=2 * nq, centres=50)
+    batches = [q, q2[nq:], q2[:nq]]
+    stride = 40
+    st = torch.cuda.Stream()
+    qd = torch.from_numpy(np.ascontiguousarray(q)).to(cuda)
+    ids = torch.empty((nq, stride), dtype=torch.int32, device=cuda)
+    dd = torch.empty((nq, stride), dtype=torch.float32, device=cuda)
+    plan = gix.batch_plan(qd.data_ptr(), nq, q.shape[1], pools, ks, stride, ids.data_ptr(), dd.data_ptr(),
+                          stream=st.cuda_stream)
+    try:
+        for rep in range(6):
+            qb = batches[rep % len(batches)]
+            ei, ed = gix.knnSearchBatch(qb, pools, ks, out_stride=stride)  # eager, on the host path
+            with torch.cuda.stream(st):
+                qd.copy_(torch.from_numpy(np.ascontiguousarray(qb)).to(cuda))
+            plan.run(st.cuda_stream)
+            st.synchronize()
+            np.testing.assert_array_equal(ids.cpu().numpy(), ei, err_msg=f"replay {rep}")
+            np.testing.assert_array_equal(dd.cpu().numpy().view(np.uint32), ed.view(np.uint32))
+            gix.knnSearch(10, qb)  # other eager work on the index between replays
+            torch.cuda.synchronize()
+    finally:
+        plan.close()
