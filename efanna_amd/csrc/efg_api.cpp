@@ -418,6 +418,13 @@ int efg_set_search_precision(efg_index* ix, int precision) {
   return EFG_OK;
 }
 
+int efg_set_build_precision(efg_index* ix, int precision) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (precision < 0 || precision > 2) return fail(EFG_ERR_ARG, "precision must be 0 (FP32), 1 (BF16) or 2 (FP16)");
+  ix->b_precision = precision;
+  return EFG_OK;
+}
+
 int efg_knn_search_batch_device(efg_index* ix, const float* d_queries, size_t nq, size_t q_stride, const int* pool,
                                 const int* extend, const int* k, int out_stride, int32_t* d_ids, float* d_dists,
                                 int tune, void* stream) {

@@ -15,6 +15,7 @@
 
 #include "efg_device.hpp"
 #include "efg_internal.hpp"
+#include "efg_mfma.hpp"
 
 namespace efg {
 
@@ -150,42 +151,9 @@ __global__ void __launch_bounds__(kBlock) k_exact_merge(const uint64_t* __restri
 // 2^24 (e.g. integers 0..255 with d <= 128) every step is exact and the
 // result equals the FP32 search.
 // ---------------------------------------------------------------------------
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
-typedef float f32x16_t __attribute__((ext_vector_type(16)));
-
 constexpr int kMP = 128, kMK = 64;  // base rows per tile, elements per LDS stage
 constexpr int kMKP = kMK + 8;        // padded row (halves): 16 B shift per row
 constexpr int kDS = kMP + 1;         // padded distance-tile row (floats)
-
-// 16-bit copy of rows (n x D floats, stride ld) padded to Dk (multiple of
-// kMK) with zeros, and the FP32 squared norm of the rounded row.
-template <bool kF16>
-__global__ void k_pack16(const float* __restrict__ X, size_t n, int d, size_t ld, int Dk,
-                         uint16_t* __restrict__ out, float* __restrict__ norm) {
-  const size_t r = size_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (r >= n) return;
-  float acc = 0.f;
-  for (int e = lane; e < Dk; e += 64) {
-    const float v = e < d ? X[r * ld + e] : 0.f;
-    uint16_t h;
-    float back;
-    if (kF16) {
-      const _Float16 x = _Float16(v);
-      __builtin_memcpy(&h, &x, 2);
-      back = float(x);
-    } else {
-      const uint32_t u = __float_as_uint(v);
-      h = uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
-      back = __uint_as_float(uint32_t(h) << 16);
-    }
-    out[r * Dk + e] = h;
-    acc = __builtin_fmaf(back, back, acc);
-  }
-  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-  if (lane == 0) norm[r] = acc;
-}
 
 // One workgroup per (32 kQB queries, chunk of base rows).  For every kMP-row
 // base tile the (32 kQB) x 128 dot products are 32 x 32 MFMA blocks, kQB per

@@ -32,6 +32,7 @@
 
 #include "efg_device.hpp"
 #include "efg_internal.hpp"
+#include "efg_mfma.hpp"
 
 namespace efg {
 
@@ -660,13 +661,12 @@ __device__ inline void join_cell(int k, bool same, int sizeA, int sizeB, int& p,
   q = p + 1 + (k - (p * (sizeA - 1) - p * (p - 1) / 2));
 }
 
-// Endpoints of pair k: slots and ids, the first endpoint (whose candidate
-// carries the kFirst credit tag) being the lower id of a new x new pair.
-// False for a new x new pair of one id, which the reference skips.
-__device__ inline bool join_ends(int k, bool same, int sizeA, int newA, int sizeB, int newB,
-                                 const uint32_t* ids_s, int& sa, int& sb, uint32_t& a, uint32_t& b) {
-  int p, q;
-  join_cell(k, same, sizeA, sizeB, p, q);
+// Endpoints of the pair at positions (p, q) of blocks I and J: slots and
+// ids, the first endpoint (whose candidate carries the kFirst credit tag)
+// being the lower id of a new x new pair.  False for a new x new pair of one
+// id, which the reference skips.
+__device__ inline bool pair_ends(int p, int q, bool same, int newA, int newB, const uint32_t* ids_s, int& sa,
+                                 int& sb, uint32_t& a, uint32_t& b) {
   sa = p;
   sb = same ? q : 32 + q;
   a = ids_s[sa];
@@ -683,6 +683,19 @@ __device__ inline bool join_ends(int k, bool same, int sizeA, int newA, int size
     }
   }
   return true;
+}
+// Pair k of a task (join_cell order).
+__device__ inline bool join_ends(int k, bool same, int sizeA, int newA, int sizeB, int newB,
+                                 const uint32_t* ids_s, int& sa, int& sb, uint32_t& a, uint32_t& b) {
+  int p, q;
+  join_cell(k, same, sizeA, sizeB, p, q);
+  return pair_ends(p, q, same, newA, newB, ids_s, sa, sb, a, b);
+}
+// Pair (p, q) of a full 32 x 32 block, if the reference measures it.
+__device__ inline bool block_ends(int p, int q, bool same, int sizeA, int newA, int sizeB, int newB,
+                                  const uint32_t* ids_s, int& sa, int& sb, uint32_t& a, uint32_t& b) {
+  if (p >= newA || q >= (same ? sizeA : sizeB) || (same && q <= p)) return false;
+  return pair_ends(p, q, same, newA, newB, ids_s, sa, sb, a, b);
 }
 
 
@@ -951,6 +964,302 @@ __global__ void k_join_scatter(const Stage* __restrict__ stage, unsigned long lo
   }
 }
 
+// ---------------------------------------------------------------------------
+// Precision form of the build: the merge's leaf tiles and the join's pair
+// tiles as |x|^2 + |y|^2 - 2 x.y with the dot products on 16-bit MFMA blocks
+// and FP32 accumulation.  Every row is split into two 16-bit parts, x = hi +
+// lo (hi = x rounded, lo = the rest rounded), and x.y is taken as hi.hi' +
+// (hi.lo' + lo.hi'): about 22 significant bits with FP16 parts, 16 with BF16
+// ones, instead of 11 or 8 for a single 16-bit copy, whose error at |x|^2 >>
+// neighbour distances would reorder the neighbours.  The two cross terms are
+// summed in separate accumulators, so a pair's distance does not depend on
+// which block holds which row and the batched merge still sees one key per
+// pair.  The norms are FP32 sums of the FP32 rows.  The merged S-sets and the
+// first 2K entries of the final pools are re-ranked by the direct FP32
+// difference in the reference's order.  On integer rows 0..255 with d <= 128
+// lo is zero and every sum is exact, so the build equals the FP32 build bit
+// for bit.
+// ---------------------------------------------------------------------------
+
+// Split rows: hi[r][e], lo[r][e] (Dk a multiple of 16, zero padded) and the
+// FP32 squared norm of row r; one wavefront per row.
+template <bool kF16>
+__global__ void k_pack_split(const float* __restrict__ X, size_t n, int d, int D, int Dk, uint16_t* __restrict__ hi,
+                             uint16_t* __restrict__ lo, float* __restrict__ norm) {
+  const size_t r = size_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= n) return;
+  float acc = 0.f;
+  for (int e = lane; e < Dk; e += 64) {
+    const float v = e < d ? X[r * D + e] : 0.f;
+    uint16_t h, l;
+    if (kF16) {
+      const _Float16 x = _Float16(v);
+      const _Float16 y = _Float16(v - float(x));
+      __builtin_memcpy(&h, &x, 2);
+      __builtin_memcpy(&l, &y, 2);
+    } else {
+      const uint32_t u = __float_as_uint(v);
+      h = uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+      const float rest = v - __uint_as_float(uint32_t(h) << 16);
+      const uint32_t w = __float_as_uint(rest);
+      l = uint16_t((w + 0x7fffu + ((w >> 16) & 1u)) >> 16);
+    }
+    hi[r * Dk + e] = h;
+    lo[r * Dk + e] = l;
+    acc = __builtin_fmaf(v, v, acc);
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) norm[r] = acc;
+}
+
+template <bool kF16>
+__device__ __forceinline__ f32x16_t mfma16(const uint4& a, const uint4& b, const f32x16_t& c) {
+  if (kF16) {
+    f16x8_t x, y;
+    __builtin_memcpy(&x, &a, 16);
+    __builtin_memcpy(&y, &b, 16);
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(x, y, c, 0, 0, 0);
+  } else {
+    bf16x8_t x, y;
+    __builtin_memcpy(&x, &a, 16);
+    __builtin_memcpy(&y, &b, 16);
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, c, 0, 0, 0);
+  }
+}
+
+// One 32 x 32 block of split dot products: this lane feeds A row (lane % 32)
+// from row ra and B row (lane % 32) from row rb, columns 8 (lane / 32) .. + 7
+// of every 16; element l of the result is A row 8 (l / 4) + 4 (lane / 32) +
+// l % 4 against B row lane % 32.
+template <bool kF16>
+__device__ __forceinline__ f32x16_t mfma_split(const uint16_t* __restrict__ hi, const uint16_t* __restrict__ lo,
+                                               size_t ra, size_t rb, int Dk) {
+  const int kh = (threadIdx.x & 32) ? 8 : 0;
+  const uint16_t *ah = hi + ra * Dk + kh, *al = lo + ra * Dk + kh, *bh = hi + rb * Dk + kh, *bl = lo + rb * Dk + kh;
+  f32x16_t hh = f32x16_t{}, x1 = f32x16_t{}, x2 = f32x16_t{};
+#pragma unroll 2
+  for (int k = 0; k < Dk; k += 16) {
+    const uint4 a0 = *reinterpret_cast<const uint4*>(ah + k), a1 = *reinterpret_cast<const uint4*>(al + k);
+    const uint4 b0 = *reinterpret_cast<const uint4*>(bh + k), b1 = *reinterpret_cast<const uint4*>(bl + k);
+    hh = mfma16<kF16>(a0, b0, hh);
+    x1 = mfma16<kF16>(a0, b1, x1);
+    x2 = mfma16<kF16>(a1, b0, x2);
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) hh[i] = hh[i] + (x1[i] + x2[i]);
+  return hh;
+}
+
+// Merge leaf tiles on MFMA: one wavefront per (leaf bucket, 32 records),
+// members in blocks of 32 (LM <= 64); writes the same dist / pslot entries
+// as k_leaf_dist.
+template <bool kF16>
+__global__ void __launch_bounds__(kBlock) k_leaf_mfma(const uint16_t* __restrict__ xhi, const uint16_t* __restrict__ xlo,
+                                                      const float* __restrict__ xn, int Dk,
+                                                      const uint2* __restrict__ range,
+                                                      const uint32_t* __restrict__ leaf_t,
+                                                      const int32_t* __restrict__ boff, int M,
+                                                      const uint64_t* __restrict__ rec, int LM, int ml, int nlev,
+                                                      float* __restrict__ dist, uint2* __restrict__ pslot) {
+  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  const int r32 = lane & 31;
+  for (int l = blockIdx.x * nw + (threadIdx.x >> 6); l < M; l += gridDim.x * nw) {
+    const int b0 = boff[l], R = boff[l + 1] - b0;
+    if (R == 0) continue;
+    const uint2 r = range[l];
+    const int m = int(r.y - r.x);
+    for (int rb = 0; rb < R; rb += 32) {
+      const int nb = min(32, R - rb);
+      uint32_t pid = 0;  // prober of A row r32
+      if (r32 < nb) {
+        const uint64_t v = rec[b0 + rb + r32];
+        const uint32_t pos = uint32_t(v & 0xffffffffu);
+        pid = leaf_t[pos];
+        if (lane < 32) pslot[size_t(pos) * nlev + (63 - int(v >> 32) - ml)] = make_uint2(uint32_t(b0 + rb + r32), uint32_t(l));
+      }
+      for (int mb = 0; mb < m; mb += 32) {
+        const uint32_t mid = (mb + r32 < m) ? leaf_t[r.x + mb + r32] : 0u;  // member of B row r32
+        const f32x16_t acc = mfma_split<kF16>(xhi, xlo, pid, mid, Dk);
+        const float pn = xn[pid], mn = xn[mid];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int p = 8 * (i / 4) + 4 * (lane >> 5) + (i & 3);
+          const float pnp = __shfl(pn, p);
+          if (p < nb && mb + r32 < m) {
+            const float dd = __builtin_fmaf(-2.f, acc[i], pnp + mn);
+            dist[size_t(b0 + rb + p) * LM + mb + r32] = dd > 0.f ? dd : 0.f;
+          }
+        }
+      }
+    }
+  }
+}
+
+// Join tasks on MFMA: one wavefront per task (the task records of
+// k_join_tasks), the 32 x 32 block of dot products in one accumulator; then
+// the candidates as in k_join_tiles, counted per target row in the
+// wavefront's LDS slice, staged into per-wavefront runs.
+template <bool kF16>
+__global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict__ xhi, const uint16_t* __restrict__ xlo,
+                                                      const float* __restrict__ xn, int Dk, uint32_t ntask,
+                                                      const uint4* __restrict__ hdr,
+                                                      const uint32_t* __restrict__ tids,
+                                                      const uint64_t* __restrict__ worst, Stage* __restrict__ stage,
+                                                      unsigned long long cap, unsigned long long* __restrict__ nstage,
+                                                      uint32_t* __restrict__ tcount,
+                                                      unsigned long long* __restrict__ npairs) {
+  constexpr int kW = kBlock / 64;
+  __shared__ uint32_t ids_s[kW][64];
+  __shared__ uint64_t w_s[kW][64];
+  __shared__ uint32_t cnt_s[kW][64];
+  __shared__ uint32_t off_s[kW][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r32 = lane & 31;
+  const uint32_t xcd = blockIdx.x & 7u, nsl = (gridDim.x >> 3) * kW;
+  const uint32_t lo = uint32_t(uint64_t(ntask) * xcd / 8), hi = uint32_t(uint64_t(ntask) * (xcd + 1) / 8);
+  unsigned long long pairs_local = 0;
+  unsigned long long wbase = 0, wend = 0;  // staging run (lane 63)
+  uint32_t* ids = ids_s[wv];
+  for (uint32_t t = lo + (blockIdx.x >> 3) * kW + wv; t < hi; t += nsl) {
+    const uint4 h = hdr[t];
+    const bool same = h.z != 0;
+    const int sizeA = int(h.y & 0xff), newA = int((h.y >> 8) & 0xff), sizeB = int((h.y >> 16) & 0xff),
+              newB = int(h.y >> 24);
+    const uint32_t id = tids[size_t(t) * 64 + lane];
+    ids[lane] = id;
+    w_s[wv][lane] = id != kNoRow ? worst[id] : 0ull;
+    cnt_s[wv][lane] = 0;
+    const uint32_t ia = uint32_t(__shfl(int(id), r32)), ib = uint32_t(__shfl(int(id), same ? r32 : 32 + r32));
+    const uint32_t ra = ia != kNoRow ? ia : 0u, rbw = ib != kNoRow ? ib : 0u;
+    const f32x16_t acc = mfma_split<kF16>(xhi, xlo, ra, rbw, Dk);
+    const float na = xn[ra], nb = xn[rbw];
+    wave_sync();
+    float dist[16];
+    uint32_t li[16];  // first endpoint's index | second's << 16; ~0u: no pair
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int p = 8 * (i / 4) + 4 * (lane >> 5) + (i & 3), q = r32;
+      const float nap = __shfl(na, p);
+      const float dd = __builtin_fmaf(-2.f, acc[i], nap + nb);
+      dist[i] = dd > 0.f ? dd : 0.f;
+      li[i] = ~0u;
+      int sa, sb;
+      uint32_t a, b;
+      if (block_ends(p, q, same, sizeA, newA, sizeB, newB, ids, sa, sb, a, b)) {
+        ++pairs_local;
+        uint32_t la = 0xffffu, lb = 0xffffu;
+        if (make_key(dist[i], b) < w_s[wv][sa]) la = atomicAdd(&cnt_s[wv][sa], 1u);
+        if (make_key(dist[i], a) < w_s[wv][sb]) lb = atomicAdd(&cnt_s[wv][sb], 1u);
+        li[i] = la | lb << 16;
+      }
+    }
+    wave_sync();
+    // totals per target (result not waited for), offsets within the task
+    const uint32_t cnt = cnt_s[wv][lane];
+    if (cnt) atomicAdd(&tcount[id], cnt);
+    uint32_t x = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = uint32_t(__shfl_up(int(x), o));
+      if (lane >= o) x += y;
+    }
+    off_s[wv][lane] = x - cnt;
+    const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
+    // this task's slots: the rest of the run, then a new run if needed
+    const unsigned long long rem = wend - wbase;
+    unsigned long long nr = 0;
+    if (tot > rem) {
+      if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
+      nr = (unsigned long long)__shfl((long long)nr, 0);
+    }
+    const unsigned long long r0 = wbase;
+    if (tot > rem) {
+      wbase = nr + (tot - rem);
+      wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
+    } else {
+      wbase += tot;
+    }
+    wave_sync();
+    const uint32_t nn = h.x;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      if (li[i] == ~0u) continue;
+      const int p = 8 * (i / 4) + 4 * (lane >> 5) + (i & 3), q = r32;
+      int sa, sb;
+      uint32_t a, b;
+      block_ends(p, q, same, sizeA, newA, sizeB, newB, ids, sa, sb, a, b);
+      const uint32_t la = li[i] & 0xffffu, lb = li[i] >> 16;
+      if (la != 0xffffu) {
+        const unsigned long long j = off_s[wv][sa] + la, at = j < rem ? r0 + j : nr + (j - rem);
+        if (at < cap) stage[at] = Stage{a, nn | kFirst, make_key(dist[i], b)};
+      }
+      if (lb != 0xffffu) {
+        const unsigned long long j = off_s[wv][sb] + lb, at = j < rem ? r0 + j : nr + (j - rem);
+        if (at < cap) stage[at] = Stage{b, nn, make_key(dist[i], a)};
+      }
+    }
+    wave_sync();
+  }
+  for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
+  for (int o = 32; o > 0; o >>= 1) pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
+  if (lane == 0 && pairs_local) atomicAdd(npairs, pairs_local);
+}
+
+// FP32 re-rank of the merged S-sets: each point's keys get the direct FP32
+// distance (reference order) and are sorted again.
+__global__ void k_rerank_sets(const float* __restrict__ X, int D, uint32_t n, int S, uint64_t* __restrict__ hkey,
+                              const uint32_t* __restrict__ hsize) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint64_t* h = hkey + size_t(p) * S;
+  const int sz = int(hsize[p]);
+  for (int j = 0; j < sz; ++j) {
+    const uint32_t id = key_id(h[j]);
+    const uint64_t k = make_key(l2_thread(X + size_t(p) * D, X + size_t(id) * D, D), id);
+    int i = j;
+    while (i > 0 && h[i - 1] > k) {
+      h[i] = h[i - 1];
+      --i;
+    }
+    h[i] = k;
+  }
+}
+
+// FP32 re-rank of the final pools: the first min(fill, 2K) entries of every
+// pool get their direct FP32 distances (a lane pair per entry, the
+// reference's order); k_sort_pools then orders that prefix (keys with their
+// flags).
+__global__ void k_rerank_pool(const float* __restrict__ X, int D, uint32_t n, int L, int R,
+                              uint64_t* __restrict__ pool, const uint32_t* __restrict__ fill) {
+  const size_t g = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 1;
+  const int h = threadIdx.x & 1;
+  const uint32_t p = uint32_t(g / R), j = uint32_t(g % R);
+  const bool in = g < size_t(n) * R && j < fill[p < n ? p : 0];
+  const uint32_t id = in ? key_id(pool[size_t(p) * L + j]) : 0u;
+  const float dd = l2_pair_lanes(X + size_t(in ? p : 0) * D, X + size_t(id) * D, D, h);
+  if (in && h) pool[size_t(p) * L + j] = make_key(dd, id);
+}
+__global__ void k_sort_pools(uint32_t n, int L, int R, uint64_t* __restrict__ pool, uint8_t* __restrict__ flag,
+                             const uint32_t* __restrict__ fill) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint64_t* k = pool + size_t(p) * L;
+  uint8_t* f = flag + size_t(p) * L;
+  const int m = min(int(fill[p]), R);
+  for (int j = 1; j < m; ++j) {  // nearly sorted already: insertion sort
+    const uint64_t v = k[j];
+    const uint8_t fv = f[j];
+    int i = j;
+    while (i > 0 && k[i - 1] > v) {
+      k[i] = k[i - 1];
+      f[i] = f[i - 1];
+      --i;
+    }
+    k[i] = v;
+    f[i] = fv;
+  }
+}
+
 // Candidate tags: existing entries carry their flag (0/1); candidates carry
 // 1<<31 | (not first endpoint) << 30 | joining node, so duplicates resolve to
 // the existing entry, else to the lowest first-endpoint joining node.
@@ -965,15 +1274,23 @@ __device__ inline uint32_t cand_tag(uint32_t s) {
 // duplicates dropped, and the first L kept.  An entry beyond the L-th unique
 // key of a subset is beyond it in the union, so rounds give the one-shot
 // result.
-template <int kCap>
+//
+// kById (precision form of the build): a pair's approximate distance may
+// differ from the FP32 distance its id already carries in the pool (from the
+// merge's re-ranked sets or the initial top-up), so duplicates are dropped by
+// id, the first in (key, tag) order staying, through a per-wavefront LDS hash
+// of the window's ids holding the smallest position of each.
+template <int kCap, bool kById>
 __global__ void __launch_bounds__(kBlock) k_merge_wave(uint32_t n, int L, int CK, const int32_t* __restrict__ toff,
                                                        const uint64_t* __restrict__ bkey,
                                                        const uint32_t* __restrict__ bsrc,
                                                        uint64_t* __restrict__ pool, uint8_t* __restrict__ flag,
                                                        uint32_t* __restrict__ fill, uint8_t* __restrict__ credit) {
   constexpr int kW = kCap >= 1024 ? 2 : 4;  // wavefronts per workgroup
+  constexpr int kHS = kById ? 2 * kCap : 1;  // id hash slots per wavefront
   __shared__ uint64_t ks[kW][kCap];
   __shared__ uint32_t ts[kW][kCap];
+  __shared__ uint32_t hid[kW][kHS], hpos[kW][kHS];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint64_t* k = ks[w];
   uint32_t* tg = ts[w];
@@ -1014,7 +1331,27 @@ __global__ void __launch_bounds__(kBlock) k_merge_wave(uint32_t n, int L, int CK
           wave_sync();
         }
       }
-      // drop duplicates (the first of equal keys stays), compact to the front
+      if (kById) {  // smallest position of every id of the window
+        uint32_t* hi_ = hid[w];
+        uint32_t* hp = hpos[w];
+        for (int i = lane; i < kHS; i += 64) {
+          hi_[i] = 0xffffffffu;
+          hp[i] = 0xffffffffu;
+        }
+        wave_sync();
+        for (int i = lane; i < m; i += 64) {
+          const uint32_t id = key_id(k[i]);
+          uint32_t h = (id * 2654435761u) & uint32_t(kHS - 1);
+          for (;;) {
+            const uint32_t prev = atomicCAS(&hi_[h], 0xffffffffu, id);
+            if (prev == 0xffffffffu || prev == id) break;
+            h = (h + 1) & uint32_t(kHS - 1);
+          }
+          atomicMin(&hp[h], uint32_t(i));
+        }
+        wave_sync();
+      }
+      // drop duplicates (the first of equal keys, or of equal ids, stays), compact to the front
       int kept = 0;
       for (int b0 = 0; b0 < m && kept < L; b0 += 64) {
         const int i = b0 + lane;
@@ -1024,7 +1361,14 @@ __global__ void __launch_bounds__(kBlock) k_merge_wave(uint32_t n, int L, int CK
         if (i < m) {
           kv = k[i];
           tv = tg[i];
-          first = i == 0 || k[i - 1] != kv;
+          if (kById) {
+            const uint32_t id = key_id(kv);
+            uint32_t h = (id * 2654435761u) & uint32_t(kHS - 1);
+            while (hid[w][h] != id) h = (h + 1) & uint32_t(kHS - 1);
+            first = hpos[w][h] == uint32_t(i);
+          } else {
+            first = i == 0 || k[i - 1] != kv;
+          }
         }
         const uint64_t fm = __ballot(first);
         const int at = kept + __popcll(fm & lt);
@@ -1235,6 +1579,32 @@ double now_ms() {
 
 int grid_for(size_t n) { return int((n + 255) / 256); }
 
+// The split rows of the build's precision form (made once per build).
+int ensure_split(efg_index* ix, hipStream_t st) {
+  if (ix->b_precision == 0) return EFG_OK;
+  if (ix->split_hi.p && ix->split_kind == ix->b_precision) return EFG_OK;
+  const int Dk = int((ix->d + 15) / 16 * 16);
+  if (!ix->split_hi.alloc(ix->n * size_t(Dk)) || !ix->split_lo.alloc(ix->n * size_t(Dk)) ||
+      !ix->split_norm.alloc(ix->n))
+    return fail(EFG_ERR_MEMORY, "out of device memory (split rows)");
+  const unsigned grid = unsigned((ix->n + 3) / 4);
+  if (ix->b_precision == 2)
+    k_pack_split<true><<<grid, 256, 0, st>>>(ix->data.p, ix->n, int(ix->d), int(ix->D), Dk, ix->split_hi.p,
+                                             ix->split_lo.p, ix->split_norm.p);
+  else
+    k_pack_split<false><<<grid, 256, 0, st>>>(ix->data.p, ix->n, int(ix->d), int(ix->D), Dk, ix->split_hi.p,
+                                              ix->split_lo.p, ix->split_norm.p);
+  ix->split_Dk = Dk;
+  ix->split_kind = ix->b_precision;
+  return check_hip(hipGetLastError(), "split rows");
+}
+void release_split(efg_index* ix) {
+  ix->split_hi.reset();
+  ix->split_lo.reset();
+  ix->split_norm.reset();
+  ix->split_kind = 0;
+}
+
 }  // namespace
 
 int gpu_merge_init(efg_index* ix) {
@@ -1279,7 +1649,10 @@ int gpu_merge_init(efg_index* ix) {
   const int PB = std::max(1, std::min(64, 256 / std::max(1, LM)));
   const size_t wave_lds = size_t(LM + PB) * (DCm + 4) * 4;
   const int tile_waves = int(std::min<size_t>(4, (64 * 1024) / std::max<size_t>(wave_lds, 1)));
-  const bool tiles = LM > 0 && tile_waves >= 1;
+  // precision form: leaf tiles on MFMA (leaves of at most 64 members)
+  const bool mfma_leaf = ix->b_precision > 0 && LM > 0 && LM <= 64;
+  if (int rc = ensure_split(ix, st)) return rc;
+  const bool tiles = LM > 0 && (tile_waves >= 1 || mfma_leaf);
   const int nlev = std::max(1, max_depth - p.merge_level + 1);
   DevBuf<float> mdist;
   DevBuf<uint2> pslot;
@@ -1322,8 +1695,19 @@ int gpu_merge_init(efg_index* ix) {
     k_sort_big<<<unsigned(ncu_m * 2), kBlock, size_t(8192) * 8, st>>>(boff.p, bigl.p, nbig.p, rec.p);
     if (tiles) {
       if (!mdist.ensure(size_t(total) * LM + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (merge distances)");
-      k_leaf_dist<<<unsigned(ncu_m * 8), 64 * tile_waves, wave_lds * tile_waves, st>>>(
-          ix->data.p, D, DCm, ix->range.p, lt, boff.p, M, rec.p, LM, PB, p.merge_level, nlev, mdist.p, pslot.p);
+      if (mfma_leaf) {
+        if (ix->b_precision == 2)
+          k_leaf_mfma<true><<<unsigned(ncu_m * 8), kBlock, 0, st>>>(
+              ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, ix->range.p, lt, boff.p, M, rec.p, LM,
+              p.merge_level, nlev, mdist.p, pslot.p);
+        else
+          k_leaf_mfma<false><<<unsigned(ncu_m * 8), kBlock, 0, st>>>(
+              ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, ix->range.p, lt, boff.p, M, rec.p, LM,
+              p.merge_level, nlev, mdist.p, pslot.p);
+      } else {
+        k_leaf_dist<<<unsigned(ncu_m * 8), 64 * tile_waves, wave_lds * tile_waves, st>>>(
+            ix->data.p, D, DCm, ix->range.p, lt, boff.p, M, rec.p, LM, PB, p.merge_level, nlev, mdist.p, pslot.p);
+      }
       k_replay_dist<<<grid_for(n), kBlock, lds, st>>>(ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la, n,
                                                       p.merge_level, S, boff.p, rec.p, mdist.p, pslot.p, LM, nlev,
                                                       hkey.p, hsize.p, extra.p, ecount.p, nd.p);
@@ -1333,6 +1717,8 @@ int gpu_merge_init(efg_index* ix) {
                                                  ecount.p, nd.p);
     }
   }
+  // precision form: the merged sets get their FP32 distances back
+  if (mfma_leaf) k_rerank_sets<<<grid_for(n), 256, 0, st>>>(ix->data.p, D, n, S, hkey.p, hsize.p);
   // init
   bs.L = L;
   if (!bs.pool.alloc(size_t(n) * L) || !bs.flag.alloc(size_t(n) * L) || !bs.fill.alloc(n) || !bs.range.alloc(n) ||
@@ -1403,7 +1789,9 @@ int gpu_refine_iter(efg_index* ix) {
   // eight task ranges, one per XCD: a multiple of eight workgroups
   const size_t join_grid = size_t((ncu * 4 + 7) / 8) * 8;
   // two candidates per pair at most, plus one partly used run per workgroup
-  const size_t cap = size_t(2 * bound) + join_grid * kStageRun + 1024;
+  // (per wavefront on the MFMA path, ncu * 8 workgroups of four)
+  const size_t cap16 = size_t(2 * bound) + size_t(ncu) * 8 * (kBlock / 64) * kStageRun + 1024;
+  const size_t cap = ix->b_precision > 0 ? cap16 : size_t(2 * bound) + join_grid * kStageRun + 1024;
   if (!bs.jhdr.ensure(size_t(ntask) + 1) || !bs.jtids.ensure((size_t(ntask) + 1) * 64) ||
       !bs.jstage.ensure(cap * sizeof(Stage)))
     return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
@@ -1418,11 +1806,24 @@ int gpu_refine_iter(efg_index* ix) {
   unsigned long long h_ctr[2] = {0, 0};
   hipMemsetAsync(ctr.p, 0, 16, st);
   hipMemsetAsync(tcount.p, 0, size_t(n) * 4, st);
-  if (ntask)
+  if (ntask && ix->b_precision > 0) {
+    // precision form: one wavefront per task on MFMA
+    if (int rc = ensure_split(ix, st)) return rc;
+    const unsigned g16 = unsigned(size_t(ncu) * 8);
+    if (ix->b_precision == 2)
+      k_join_mfma<true><<<g16, kBlock, 0, st>>>(ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk,
+                                                uint32_t(ntask), bs.jhdr.p, bs.jtids.p, worst.p, stage,
+                                                (unsigned long long)cap16, ctr.p, tcount.p, ctr.p + 1);
+    else
+      k_join_mfma<false><<<g16, kBlock, 0, st>>>(ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk,
+                                                 uint32_t(ntask), bs.jhdr.p, bs.jtids.p, worst.p, stage,
+                                                 (unsigned long long)cap16, ctr.p, tcount.p, ctr.p + 1);
+  } else if (ntask) {
     k_join_tiles<4><<<unsigned(join_grid), kBlock, lds_join, st>>>(ix->data.p, D, uint32_t(ntask), bs.jhdr.p,
                                                                    bs.jtids.p, worst.p, stage,
                                                                    (unsigned long long)cap, ctr.p, tcount.p,
                                                                    ctr.p + 1);
+  }
   hipMemcpyAsync(h_ctr, ctr.p, 16, hipMemcpyDeviceToHost, st);
   if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
   const unsigned long long nst = h_ctr[0];
@@ -1444,19 +1845,25 @@ int gpu_refine_iter(efg_index* ix) {
   hipMemsetAsync(bs.found.p, 0, n, st);
   {
     // one wavefront per target; the LDS window holds at least 2L entries
+    const bool byid = ix->b_precision > 0;
     const unsigned grid = unsigned(std::min<size_t>((n + 3) / 4, size_t(ncu) * 8));
     if (2 * L <= 256)
-      k_merge_wave<256><<<grid, kBlock, 0, st>>>(n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
+      (byid ? k_merge_wave<256, true> : k_merge_wave<256, false>)<<<grid, kBlock, 0, st>>>(
+          n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
                                                  bs.fill.p, bs.found.p);
     else if (2 * L <= 512)
-      k_merge_wave<512><<<grid, kBlock, 0, st>>>(n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
+      (byid ? k_merge_wave<512, true> : k_merge_wave<512, false>)<<<grid, kBlock, 0, st>>>(
+          n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
                                                  bs.fill.p, bs.found.p);
     else if (2 * L <= 1024)
-      k_merge_wave<1024><<<grid * 2, 128, 0, st>>>(n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
+      (byid ? k_merge_wave<1024, true> : k_merge_wave<1024, false>)<<<grid * 2, 128, 0, st>>>(
+          n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
                                                    bs.fill.p, bs.found.p);
+    else if (byid)
+      return fail(EFG_ERR_ARG, "the build's precision form supports pools of up to 512 entries");
     else
-      k_merge_wave<2048><<<grid * 2, 128, 0, st>>>(n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
-                                                   bs.fill.p, bs.found.p);
+      k_merge_wave<2048, false><<<grid * 2, 128, 0, st>>>(n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
+                                                          bs.fill.p, bs.found.p);
     if (int rc = check_hip(hipStreamSynchronize(st), "join merge")) return rc;
   }
   if (int rc = check_hip(hipStreamSynchronize(st), "join merge")) return rc;
@@ -1522,8 +1929,16 @@ int gpu_finalize_graph(efg_index* ix) {
   const uint32_t n = uint32_t(ix->n);
   const int K = ix->bp.K;
   if (!ix->goff.alloc(n + 1) || !ix->gids.alloc(size_t(n) * K)) return fail(EFG_ERR_MEMORY, "out of device memory (graph)");
+  if (ix->b_precision > 0) {  // precision form: FP32 re-rank of the first 2K entries of every pool
+    const int R = std::min(bs.L, 2 * K);
+    const size_t m = size_t(n) * R * 2;
+    k_rerank_pool<<<unsigned((m + 255) / 256), 256, 0, ix->stream>>>(ix->data.p, int(ix->D), n, bs.L, R, bs.pool.p,
+                                                                     bs.fill.p);
+    k_sort_pools<<<grid_for(n), 256, 0, ix->stream>>>(n, bs.L, R, bs.pool.p, bs.flag.p, bs.fill.p);
+  }
   k_graph_out<<<grid_for(n + 1), 256, 0, ix->stream>>>(n, bs.L, K, bs.pool.p, bs.fill.p, ix->goff.p, ix->gids.p);
   if (int rc = check_hip(hipStreamSynchronize(ix->stream), "graph")) return rc;
+  release_split(ix);
   bs.jhdr.reset();
   bs.jtids.reset();
   bs.jstage.reset();

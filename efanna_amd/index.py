@@ -177,6 +177,10 @@ class FIndex:
         check(lib().efg_set_search_params(self.h, epochs, init_num, extend_to, search_trees, search_lv,
                                           search_method))
 
+    def setBuildPrecision(self, precision: int):
+        """0 = FP32 tiles; 1 = BF16 / 2 = FP16 MFMA merge and join tiles, FP32 re-rank of sets and pools."""
+        check(lib().efg_set_build_precision(self.h, int(precision)))
+
     def setSearchPrecision(self, precision: int):
         """0 = FP32 rows; 1 = BF16 rows with FP32 re-rank of the best 2k."""
         check(lib().efg_set_search_precision(self.h, int(precision)))

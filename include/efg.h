@@ -107,6 +107,16 @@ int efg_knn_search_device(efg_index* ix, int k, const float* d_queries, size_t n
  * results.  Applies to method 0. */
 int efg_set_search_precision(efg_index* ix, int precision);
 
+/* Build precision (no counterpart in the reference; a form beside the plain
+ * FP32 build): 0 = FP32 distance tiles (default, bit-exact with the
+ * reference's arithmetic); 1 = BF16, 2 = FP16 MFMA tiles with FP32
+ * accumulation for the merge's leaf tiles (leaves of at most 64 points) and
+ * the join's pair tiles, as |x|^2 + |y|^2 - 2 x.y; the merged sets and the
+ * final pools are then re-ranked by the direct FP32 difference.  On rows
+ * exact in the 16-bit type whose sums stay below 2^24 (integers 0..255 with
+ * d <= 128) the build equals the FP32 build.  Applies to later builds. */
+int efg_set_build_precision(efg_index* ix, int precision);
+
 /* Learned stopping (no counterpart in the reference; a form beside the plain
  * search).  A small network (16 pool-state features -> 64 -> 64 -> 1 logit,
  * ReLU, 5,313 parameters laid out W1[16][64] | b1[64] | W2[64][64] | b2[64] |

@@ -126,3 +126,66 @@ def test_merge_level_error(cuda):
     with pytest.raises(EfgError) as e:
         g.buildIndex()
     assert e.value.code == -3 and "merge level deeper than tree" in e.value.msg
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_build_precision_form_integer_data_equals_fp32_build(cuda, precision):
+    """BF16 / FP16 MFMA merge and join tiles: on integers 0..255 with d = 128
+    every distance is exact, so the build equals the FP32 build bit for bit
+    (graph, pools and per-iteration join pairs)."""
+    base, _ = helpers.mixture(n=30_000, nq=1)
+    g, _o = _pair(base, trees=8, ml=8, epochs=6)
+    g.buildIndex()
+    h, _o2 = _pair(base, trees=8, ml=8, epochs=6)
+    h.setBuildPrecision(precision)
+    h.buildIndex()
+    np.testing.assert_array_equal(h.graph_matrix(), g.graph_matrix())
+    assert h.join_pairs() == g.join_pairs()
+    kg, fg, ng, _ = g.pools()
+    kh, fh, nh, _ = h.pools()
+    np.testing.assert_array_equal(nh, ng)
+    np.testing.assert_array_equal(kh, kg)
+
+
+def test_build_precision_form_float_data(cuda):
+    """On float rows the BF16 tiles change the build slightly: the graph's
+    recall against exact kNN stays within 0.01 of the FP32 build's, and the
+    final pools hold exact FP32 distances in ascending order."""
+    base, _ = helpers.mixture(n=8000, nq=1, d=200, centres=30, sigma=0.05, hi=1.0, rounded=False)
+    K = 20
+    g, _o = _pair(base, trees=4, ml=4, epochs=6, check=40, L=40, K=K, S=12)
+    g.buildIndex()
+    h, _o2 = _pair(base, trees=4, ml=4, epochs=6, check=40, L=40, K=K, S=12)
+    h.setBuildPrecision(1)
+    h.buildIndex()
+    gt, _ = oracle.exact_knn(base, base, K, exclude_self=True)
+    rg = helpers.recall(g.graph_matrix(), gt, K)
+    rh = helpers.recall(h.graph_matrix(), gt, K)
+    assert rh >= rg - 0.01, (rh, rg)
+    kh, _, nh, _ = h.pools()
+    L = kh.shape[1]
+    rows = np.random.default_rng(1).choice(base.shape[0], 200, replace=False)
+    for p in rows:
+        keys = kh[p, :nh[p]]
+        assert (np.diff(keys.astype(np.uint64)) > 0).all()
+        for key in keys[:5]:
+            i = int(key & 0xFFFFFFFF)
+            dist = np.uint32(int(key) >> 32).view(np.float32)
+            assert dist == np.float32(oracle.l2(base[p], base[i]))
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_build_precision_form_pools_unique(cuda, precision):
+    """Split 16-bit tiles give one distance per pair whichever block holds
+    which row, so the batched merge never keeps one id twice."""
+    base, _ = helpers.mixture(n=8000, nq=1, d=200, centres=30, sigma=0.05, hi=1.0, rounded=False)
+    h, _o = _pair(base, trees=4, ml=4, epochs=3, check=40, L=40, K=20, S=12)
+    h.setBuildPrecision(precision)
+    h.build_stage(0)
+    h.build_stage(1)
+    for it in range(3):
+        h.build_stage(2)
+        kh, _, nh, _ = h.pools()
+        for p in range(0, base.shape[0], 7):
+            ids = (kh[p, :nh[p]] & np.uint64(0xFFFFFFFF)).astype(np.int64)
+            assert len(np.unique(ids)) == len(ids), (it, p)
