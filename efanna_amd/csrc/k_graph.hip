@@ -971,10 +971,10 @@ __global__ void k_join_scatter(const Stage* __restrict__ stage, unsigned long lo
 // lo (hi = x rounded, lo = the rest rounded), and x.y is taken as hi.hi' +
 // (hi.lo' + lo.hi'): about 22 significant bits with FP16 parts, 16 with BF16
 // ones, instead of 11 or 8 for a single 16-bit copy, whose error at |x|^2 >>
-// neighbour distances would reorder the neighbours.  The two cross terms are
-// summed in separate accumulators, so a pair's distance does not depend on
-// which block holds which row and the batched merge still sees one key per
-// pair.  The norms are FP32 sums of the FP32 rows.  The merged S-sets and the
+// neighbour distances would reorder the neighbours.  A pair's approximate
+// distance may differ between tasks (block roles, accumulation order) and
+// from its FP32 distance, so the pool merge drops duplicates by id in this
+// form.  The norms are FP32 sums of the FP32 rows.  The merged S-sets and the
 // first 2K entries of the final pools are re-ranked by the direct FP32
 // difference in the reference's order.  On integer rows 0..255 with d <= 128
 // lo is zero and every sum is exact, so the build equals the FP32 build bit
@@ -1037,18 +1037,16 @@ __device__ __forceinline__ f32x16_t mfma_split(const uint16_t* __restrict__ hi, 
                                                size_t ra, size_t rb, int Dk) {
   const int kh = (threadIdx.x & 32) ? 8 : 0;
   const uint16_t *ah = hi + ra * Dk + kh, *al = lo + ra * Dk + kh, *bh = hi + rb * Dk + kh, *bl = lo + rb * Dk + kh;
-  f32x16_t hh = f32x16_t{}, x1 = f32x16_t{}, x2 = f32x16_t{};
+  f32x16_t acc = f32x16_t{};
 #pragma unroll 2
   for (int k = 0; k < Dk; k += 16) {
     const uint4 a0 = *reinterpret_cast<const uint4*>(ah + k), a1 = *reinterpret_cast<const uint4*>(al + k);
     const uint4 b0 = *reinterpret_cast<const uint4*>(bh + k), b1 = *reinterpret_cast<const uint4*>(bl + k);
-    hh = mfma16<kF16>(a0, b0, hh);
-    x1 = mfma16<kF16>(a0, b1, x1);
-    x2 = mfma16<kF16>(a1, b0, x2);
+    acc = mfma16<kF16>(a0, b0, acc);
+    acc = mfma16<kF16>(a0, b1, acc);
+    acc = mfma16<kF16>(a1, b0, acc);
   }
-#pragma unroll
-  for (int i = 0; i < 16; ++i) hh[i] = hh[i] + (x1[i] + x2[i]);
-  return hh;
+  return acc;
 }
 
 // Merge leaf tiles on MFMA: one wavefront per (leaf bucket, 32 records),
