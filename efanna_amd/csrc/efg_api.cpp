@@ -122,6 +122,9 @@ int upload_graph(efg_index* ix, const std::vector<uint64_t>& off, const std::vec
   if (rc) return EFG_ERR_DEVICE;
   ix->nnz = ids.size();
   ix->max_deg = maxd;
+  bool regular = true;
+  for (size_t i = 0; i <= ix->n && regular; ++i) regular = off[i] == uint64_t(i) * uint64_t(maxd);
+  ix->reg_deg = regular ? maxd : 0;
   return EFG_OK;
 }
 

@@ -75,6 +75,7 @@ struct SearchCfg {
   uint64_t seed;
   int method;   // 0 nnexp, 1 kgraph
   int sorted;   // method 1: seeds reduced to the best epochs * extend_to
+  int deg;      // graph rows all of this degree at offsets row * deg (0: read the offsets)
 };
 
 // Device state of the NN-descent build (see k_graph.hip).
@@ -143,6 +144,7 @@ struct efg_index {
   // graph (CSR)
   size_t nnz = 0;
   int max_deg = 0;
+  int reg_deg = 0;  // degree of every row when the offsets are row * degree, else 0
   efg::DevBuf<uint64_t> goff;
   efg::DevBuf<uint32_t> gids;
   // build

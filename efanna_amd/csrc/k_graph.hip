@@ -1944,6 +1944,7 @@ int gpu_finalize_graph(efg_index* ix) {
   bs.jbsrc.reset();
   ix->nnz = size_t(n) * K;
   ix->max_deg = K;
+  ix->reg_deg = K;
   return EFG_OK;
 }
 

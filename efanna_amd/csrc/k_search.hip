@@ -932,34 +932,42 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       if (nexp == 0) break;
       nexp_total += uint64_t(nexp);
       int nnb = 0;
-      for (int b0 = 0; b0 < nexp; b0 += kThreads) {
-        int j = b0 + tid;
-        int deg = 0;
-        if (j < nexp) {
-          uint32_t e = s.explist[j];
-          deg = int(goff[e + 1] - goff[e]);
+      if (c.deg > 0) {  // regular graph: the neighbours of expansion j are at (explist[j] * deg, + deg)
+        nnb = nexp * c.deg;
+      } else {
+        for (int b0 = 0; b0 < nexp; b0 += kThreads) {
+          int j = b0 + tid;
+          int deg = 0;
+          if (j < nexp) {
+            uint32_t e = s.explist[j];
+            deg = int(goff[e + 1] - goff[e]);
+          }
+          int tot;
+          int ex = block_exclusive_scan(deg, s.scan, &tot);
+          if (j < nexp) s.expoff[j] = nnb + ex;
+          nnb += tot;
         }
-        int tot;
-        int ex = block_exclusive_scan(deg, s.scan, &tot);
-        if (j < nexp) s.expoff[j] = nnb + ex;
-        nnb += tot;
+        if (tid == 0) s.expoff[nexp] = nnb;
       }
-      if (tid == 0) {
-        s.expoff[nexp] = nnb;
-        s.misc[0] = 0;
-      }
+      if (tid == 0) s.misc[0] = 0;
       __syncthreads();
       int m = 0;
       for (int b0 = 0; b0 < nnb; b0 += kThreads) {
         int sq = b0 + tid;
         if (sq < nnb) {
-          int lo = 0, hi = nexp;
-          while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (s.expoff[mid] <= sq) lo = mid; else hi = mid;
+          uint32_t id;
+          if (c.deg > 0) {
+            const int j = sq / c.deg;
+            id = gids[size_t(s.explist[j]) * c.deg + (sq - j * c.deg)];
+          } else {
+            int lo = 0, hi = nexp;
+            while (hi - lo > 1) {
+              int mid = (lo + hi) >> 1;
+              if (s.expoff[mid] <= sq) lo = mid; else hi = mid;
+            }
+            uint32_t e = s.explist[lo];
+            id = gids[goff[e] + uint64_t(sq - s.expoff[lo])];
           }
-          uint32_t e = s.explist[lo];
-          uint32_t id = gids[goff[e] + uint64_t(sq - s.expoff[lo])];
           if (tab_insert(s.tab, c.logcap, id)) N[atomicAdd(&s.misc[0], 1)] = uint64_t(id);
         }
         __syncthreads();
@@ -1170,6 +1178,7 @@ int make_cfg(efg_index* ix, int k, int init, int E, int epochs, int trees, int m
   c.logcap = ceil_log2(size_t(init + 512) * 4 / 3 + 1);
   if (c.logcap < 10) c.logcap = 10;
   c.seed = ix->bp.seed;
+  c.deg = ix->reg_deg;
   c.stop = stop;
   c.out_stride = k;
   *out = c;

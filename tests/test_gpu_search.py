@@ -296,3 +296,23 @@ def test_batch_plan_graph_replays_match_eager(cuda):
             torch.cuda.synchronize()
     finally:
         plan.close()
+
+
+def test_search_irregular_graph_matches_oracle(cuda):
+    """Rows of different degrees (offsets not row * degree) take the CSR
+    path of the neighbour gather; a K-regular graph takes the direct one."""
+    base, q, oix = helpers.oracle_index("mixture", n=20_000, nq=200, centres=50)
+    off, ids = oix.graph()
+    off = off.astype(np.int64)
+    deg = np.diff(off)
+    keep = deg - (np.arange(deg.size) % 3 == 0)  # every third row loses its last neighbour
+    noff = np.concatenate([[0], np.cumsum(keep)]).astype(np.uint64)
+    nids = np.concatenate([ids[off[i]:off[i] + keep[i]] for i in range(deg.size)]).astype(np.uint32)
+    o2 = oracle.Index(base, trees=8, ml=8, epochs=8, check=25, L=30, K=10, S=10)
+    o2.set_trees(oix.trees())
+    o2.set_graph(noff, nids)
+    gix = _gpu_index(base, oix.trees(), (noff, nids))
+    gids, gd, oids, od, ond = _search_both(o2, gix, q, 10, 4, 300, 50, 8)
+    np.testing.assert_array_equal(gids, oids)
+    np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32))
+    assert gix.counters()["search_dist"] == int(ond.sum())
