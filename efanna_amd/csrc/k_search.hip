@@ -737,6 +737,24 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       const int t = lo % c.T;
       return leaf_lists[size_t(t) * c.n + s.lstart[lo] + (sq - s.lpos[lo])];
     };
+    // A chunk that cannot fill the pool admits every id not yet in it, in
+    // any order: method 0 ranks the pool by key, so only the chunk that
+    // fills it needs the first occurrences in sequence order (seed_chunk).
+    // Its count alternates between misc[6] and misc[7], each cleared while
+    // the other is in use.
+    auto seed_fast = [&](bool valid, uint32_t id, int par) {
+      if (valid && tab_insert(s.tab, c.logcap, id)) s.buf[p + atomicAdd(&s.misc[6 + par], 1)] = uint64_t(id);
+      __syncthreads();
+      const int cnt = s.misc[6 + par];
+      if (tid == 0) s.misc[7 - par] = 0;
+      p += cnt;
+    };
+    if (tid == 0) {
+      s.misc[6] = 0;
+      s.misc[7] = 0;
+    }
+    __syncthreads();
+    int par = 0;
     // the next chunk's ids are loaded before this chunk is processed
     uint32_t id_next = tid < total ? seed_id(tid) : 0u;
     for (int b0 = 0; b0 < total && p < c.init_num; b0 += kThreads) {
@@ -745,7 +763,12 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       const bool valid = sq < total;
       const uint32_t id = id_next;
       if (sq + kThreads < total) id_next = seed_id(sq + kThreads);
-      seed_chunk(valid, id, uint32_t(sq));
+      if (kMethod == 0 && p + kThreads <= c.init_num) {
+        seed_fast(valid, id, par);
+        par ^= 1;
+      } else {
+        seed_chunk(valid, id, uint32_t(sq));
+      }
     }
     for (uint32_t r0 = 0; p < c.init_num; r0 += kThreads) {
       uint32_t id = uint32_t(hash4(c.seed, ST_SEARCH, uint64_t(qi), r0 + tid) % uint64_t(c.n));
