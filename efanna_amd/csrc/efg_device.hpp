@@ -98,6 +98,36 @@ __device__ inline float l2_pair_lanes(const float* __restrict__ a, const float* 
   return s1;
 }
 
+// Block-wide bitonic sort of buf[0, n), n a power of two no larger than the
+// block: one key per thread in registers; partners within a wavefront are
+// exchanged by shuffles, only steps across wavefronts (j >= 64) go through
+// LDS.  Ascending; threads >= n hold the largest key.
+__device__ inline void block_bitonic_sort_reg(uint64_t* buf, int n) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  uint64_t v = tid < n ? buf[tid] : ~0ull;
+  const int m = n < 64 ? 64 : n;  // sort at least a wavefront's worth (padding sorts last)
+  __syncthreads();
+  for (int k = 2; k <= m; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      uint64_t w;
+      if (j >= 64) {
+        if (tid < m) buf[tid] = v;
+        __syncthreads();
+        w = (tid < m) ? buf[tid ^ j] : v;
+        __syncthreads();
+      } else {
+        const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(v)), j)), hi = uint32_t(__shfl_xor(int(uint32_t(v >> 32)), j));
+        w = uint64_t(hi) << 32 | lo;
+      }
+      const bool lower = (tid & j) == 0, asc = (tid & k) == 0;
+      v = (lower == asc) ? (v < w ? v : w) : (v < w ? w : v);
+    }
+  }
+  (void)nt;
+  if (tid < n) buf[tid] = v;
+  __syncthreads();
+}
+
 // Block-wide bitonic sort of n (power of two) keys in LDS, ascending.
 __device__ inline void block_bitonic_sort(uint64_t* buf, int n) {
   for (int k = 2; k <= n; k <<= 1) {

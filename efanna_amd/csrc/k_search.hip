@@ -800,7 +800,8 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       }
       for (int i = R + tid; i < Rp; i += kThreads) P[i] = kKeyMax;
       __syncthreads();
-      block_bitonic_sort(P, Rp);
+      if (Rp <= kThreads) block_bitonic_sort_reg(P, Rp);
+      else block_bitonic_sort(P, Rp);
     }
     uint64_t ndq = uint64_t(c.init_num);
     bool ovf = false;
@@ -1026,7 +1027,8 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       }
       for (int i = rr + tid; i < p2; i += kThreads) O[i] = kKeyMax;
       __syncthreads();
-      block_bitonic_sort(O, p2);
+      if (p2 <= kThreads) block_bitonic_sort_reg(O, p2);
+      else block_bitonic_sort(O, p2);
       for (int i = tid; i < rr; i += kThreads) P[i] = O[i];
       __syncthreads();
       ndq += uint64_t(rr);
