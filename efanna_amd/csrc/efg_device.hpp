@@ -104,9 +104,9 @@ __device__ inline float l2_pair_lanes(const float* __restrict__ a, const float* 
 // LDS.  Ascending; threads >= n hold the largest key.
 __device__ inline void block_bitonic_sort_reg(uint64_t* buf, int n) {
   const int tid = threadIdx.x, nt = blockDim.x;
+  __syncthreads();  // the callers' writes to buf are visible
   uint64_t v = tid < n ? buf[tid] : ~0ull;
   const int m = n < 64 ? 64 : n;  // sort at least a wavefront's worth (padding sorts last)
-  __syncthreads();
   for (int k = 2; k <= m; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
       uint64_t w;

@@ -499,6 +499,35 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
   };
   // merge N[0, m) (unique, unsorted, disjoint from P) into the sorted pool P[0, R)
   auto merge_new = [&](int m) {
+    const int mp = next_pow2(m < 2 ? 2 : m);
+    if (mp <= kThreads) {
+      // sort N, then every key's place is its index plus its rank in the
+      // other sorted array (binary search)
+      for (int i = m + tid; i < mp; i += kThreads) N[i] = kKeyMax;
+      block_bitonic_sort_reg(N, mp);
+      for (int i = tid; i < m; i += kThreads) {
+        const uint64_t key = N[i];
+        int lo = 0, hi = R;  // pool keys below key
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (P[mid] < key) lo = mid + 1; else hi = mid;
+        }
+        if (lo + i < R) O[lo + i] = key;
+      }
+      for (int i = tid; i < R; i += kThreads) {
+        const uint64_t key = P[i];
+        int lo = 0, hi = m;  // new keys below key
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (N[mid] < key) lo = mid + 1; else hi = mid;
+        }
+        if (i + lo < R) O[i + lo] = key;
+      }
+      __syncthreads();
+      for (int i = tid; i < R; i += kThreads) P[i] = O[i];
+      __syncthreads();
+      return;
+    }
     for (int i = tid; i < m; i += kThreads) {
       uint64_t key = N[i];
       int lo = 0, hi = R;  // number of pool keys below key
