@@ -54,6 +54,15 @@ struct alignas(32) SNode {
   int32_t pad;
 };
 
+// The same nodes, same positions, reduced to what the descent reads: the
+// left child of an internal node is the next node (pre-order), dim < 0 marks
+// a leaf.
+struct alignas(16) DNode {
+  int32_t dim;
+  float val;
+  int32_t right, parent;
+};
+
 // Search launch configuration (all sizes in elements).
 struct SearchCfg {
   int n;        // base rows
@@ -139,6 +148,7 @@ struct efg_index {
   efg::DevBuf<uint32_t> leaf;    // T x n
   std::vector<int8_t> leaf_dim;  // value written as DivDim of a leaf (0 or -1)
   efg::DevBuf<efg::SNode> snode;  // search layout of the trees
+  efg::DevBuf<efg::DNode> dnode;  // descent copy of snode
   efg::DevBuf<int32_t> sroots;
   bool snode_valid = false;
   // graph (CSR)

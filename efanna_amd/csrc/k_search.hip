@@ -44,56 +44,97 @@ namespace efg {
 // The far children passed on the way down are kept on a per-thread stack in
 // LDS (the last kDescStack of them), so the next leaf is reached from the
 // deepest pending far child without walking back up; past the stack's depth
-// the walk climbs the parent links as before.
+// the walk climbs the parent links.  Nodes are read from the 16-byte descent
+// copy: the left child of an internal node is the next node, and the node
+// says which of its children are leaves, so a leaf is recorded from its
+// parent (or its stack entry, bit 31) without being read.  Each trip round
+// the loop reads one internal node for every lane, so the lanes of a
+// wavefront, which walk different trees, advance together.  With kLdsQ the
+// workgroup's query rows are staged in LDS first, so each step waits on one
+// global load (the node), not two.
 constexpr int kDescStack = 16;
-__global__ void __launch_bounds__(256) k_descend(const float* __restrict__ Q, size_t q_stride, int nq, int T,
-                                                 int lsize, const int32_t* __restrict__ roots,
-                                                 const SNode* __restrict__ sn, int32_t* __restrict__ leaves,
-                                                 const int32_t* __restrict__ qlist, const int* __restrict__ nlist) {
-  __shared__ int32_t stk[kDescStack][256];
-  const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int nw = qlist ? *nlist : nq;  // queries of the list only, when given
-  if (gid >= size_t(nw) * T) return;
+constexpr int kDescThreads = 256;
+constexpr size_t kDescLdsQ = 64 * 1024;  // largest staged query block
+constexpr uint32_t kDescLeaf = 0x80000000u;
+
+// dim word of an internal descent node: split dimension in bits 0-23, left
+// child is a leaf in bit 24, right child in bit 25; -1 marks a leaf
+__device__ __forceinline__ uint32_t desc_left(int32_t node, int32_t dimw) {
+  return uint32_t(node + 1) | (uint32_t(dimw >> 24 & 1) << 31);
+}
+__device__ __forceinline__ uint32_t desc_right(const DNode& c) {
+  return uint32_t(c.right) | (uint32_t(c.dim >> 25 & 1) << 31);
+}
+
+template <bool kLdsQ>
+__global__ void __launch_bounds__(kDescThreads)
+    k_descend(const float* __restrict__ Q, size_t q_stride, int nq, int d, int T, int lsize,
+              const int32_t* __restrict__ roots, const DNode* __restrict__ dn, int32_t* __restrict__ leaves,
+              const int32_t* __restrict__ qlist, const int* __restrict__ nlist) {
+  __shared__ uint32_t stk[kDescStack][kDescThreads];
+  extern __shared__ float qrows[];  // kLdsQ: rows of list positions [p0, p1]
   const int tid = threadIdx.x;
+  const size_t gid = size_t(blockIdx.x) * kDescThreads + tid;
+  const int nw = qlist ? *nlist : nq;  // queries of the list only, when given
+  const size_t nthr = size_t(nw) * T;
+  const size_t p0 = size_t(blockIdx.x) * kDescThreads / T;
+  if (kLdsQ) {
+    if (size_t(blockIdx.x) * kDescThreads >= nthr) return;  // whole workgroup idle
+    const size_t last = std::min(nthr, size_t(blockIdx.x + 1) * kDescThreads) - 1;
+    const int rows = int(last / T - p0 + 1);
+    for (int e = tid; e < rows * d; e += kDescThreads) {
+      const int r = e / d, j = e - r * d;
+      const size_t p = p0 + r;
+      const int qr = qlist ? qlist[p] : int(p);
+      qrows[e] = Q[size_t(qr) * q_stride + j];
+    }
+    __syncthreads();
+  }
+  if (gid >= nthr) return;
   const int qi = qlist ? qlist[gid / T] : int(gid / T), t = int(gid % T);
-  const float* q = Q + size_t(qi) * q_stride;
+  const float* q = kLdsQ ? qrows + (gid / T - p0) * d : Q + size_t(qi) * q_stride;
   int32_t* out = leaves + size_t(qi) * size_t(lsize) * T;  // [ni][t]
-  int top = 0, bottom = 0;  // stack entries [bottom, top), slot i % kDescStack
-  int node = roots[t];
-  SNode cur = sn[node];
   int cnt = 0;
+  int node = roots[t];
+  DNode cur = dn[node];
+  uint32_t e = cur.dim < 0 ? uint32_t(node) | kDescLeaf : uint32_t(node);
+  int top = 0, bottom = 0;  // stack entries [bottom, top), slot i % kDescStack
   for (;;) {
-    while (cur.left >= 0) {  // down to a leaf, query side first
-      const bool left = q[cur.dim] < cur.val;
-      stk[top % kDescStack][tid] = left ? cur.right : cur.left;
+    if (!(e & kDescLeaf)) {  // internal node (read): one step down, query side first
+      const bool left = q[cur.dim & 0xFFFFFF] < cur.val;
+      const uint32_t l = desc_left(node, cur.dim), r = desc_right(cur);
+      stk[top % kDescStack][tid] = left ? r : l;
       if (++top - bottom > kDescStack) ++bottom;
-      node = left ? cur.left : cur.right;
-      cur = sn[node];
+      e = left ? l : r;
     }
-    out[size_t(cnt) * T + t] = node;
-    if (++cnt == lsize) break;
-    if (top > bottom) {  // deepest pending far child
-      --top;
-      node = stk[top % kDescStack][tid];
-      cur = sn[node];
-      continue;
-    }
-    // stack exhausted: up to the first ancestor whose second child is unvisited
-    bool more = false;
-    while (cur.parent >= 0) {
-      const int p = cur.parent;
-      const SNode pd = sn[p];
-      const int first = (q[pd.dim] < pd.val) ? pd.left : pd.right;
-      if (node == first) {
-        node = (first == pd.left) ? pd.right : pd.left;
-        cur = sn[node];
-        more = true;
-        break;
+    while (e & kDescLeaf) {  // leaf: record it, then the next entry
+      node = int(e & ~kDescLeaf);
+      out[size_t(cnt) * T + t] = node;
+      if (++cnt == lsize) break;
+      if (top > bottom) {  // deepest pending far child
+        --top;
+        e = stk[top % kDescStack][tid];
+        continue;
       }
-      node = p;
-      cur = pd;
+      // stack exhausted: up to the first ancestor whose second child is unvisited
+      bool more = false;
+      int p = dn[node].parent;
+      while (p >= 0) {
+        const DNode pd = dn[p];
+        const bool pleft = q[pd.dim & 0xFFFFFF] < pd.val;
+        if ((node == p + 1) == pleft) {  // node was the side taken first
+          e = pleft ? desc_right(pd) : desc_left(p, pd.dim);
+          more = true;
+          break;
+        }
+        node = p;
+        p = pd.parent;
+      }
+      if (!more) break;
     }
-    if (!more) break;
+    if (cnt == lsize || (e & kDescLeaf)) break;
+    node = int(e);
+    cur = dn[node];
   }
   for (; cnt < lsize; ++cnt) out[size_t(cnt) * T + t] = -1;
 }
@@ -1171,10 +1212,21 @@ int build_search_layout(efg_index* ix) {
       out[out[i].right].parent = int32_t(i);
     }
   }
+  if (ix->d >= (size_t(1) << 24)) return fail(EFG_ERR_ARG, "dimension too large for the search trees");
+  std::vector<DNode> dn(out.size());
+  for (size_t i = 0; i < out.size(); ++i) {
+    const SNode& o = out[i];
+    dn[i].dim = o.left >= 0 ? o.dim | int32_t(out[o.left].left < 0) << 24 | int32_t(out[o.right].left < 0) << 25 : -1;
+    dn[i].val = out[i].val;
+    dn[i].right = out[i].right;
+    dn[i].parent = out[i].parent;
+  }
   std::vector<int32_t> sroots(ix->T);
   for (int t = 0; t < ix->T; ++t) sroots[t] = nid[roots[t]];
-  if (!ix->snode.alloc(out.size()) || !ix->sroots.alloc(ix->T)) return fail(EFG_ERR_MEMORY, "out of device memory (search trees)");
+  if (!ix->snode.alloc(out.size()) || !ix->dnode.alloc(out.size()) || !ix->sroots.alloc(ix->T))
+    return fail(EFG_ERR_MEMORY, "out of device memory (search trees)");
   rc |= check_hip(hipMemcpy(ix->snode.p, out.data(), out.size() * sizeof(SNode), hipMemcpyHostToDevice), "copy");
+  rc |= check_hip(hipMemcpy(ix->dnode.p, dn.data(), dn.size() * sizeof(DNode), hipMemcpyHostToDevice), "copy");
   rc |= check_hip(hipMemcpy(ix->sroots.p, sroots.data(), ix->T * 4, hipMemcpyHostToDevice), "copy");
   if (rc) return EFG_ERR_DEVICE;
   ix->snode_valid = true;
@@ -1273,8 +1325,14 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
   if (ev) hipEventRecord(ev[0], st);
   {
     size_t threads = nq * size_t(c.T);
-    k_descend<<<unsigned((threads + 255) / 256), 256, 0, st>>>(dq, q_stride, int(nq), c.T, c.lsize, ix->sroots.p,
-                                                               ix->snode.p, w.leaves.p, qlist, nlist_dev);
+    const unsigned grid = unsigned((threads + kDescThreads - 1) / kDescThreads);
+    const size_t qbytes = size_t((kDescThreads - 1) / c.T + 2) * size_t(c.d) * sizeof(float);
+    if (qbytes <= kDescLdsQ)
+      k_descend<true><<<grid, kDescThreads, qbytes, st>>>(dq, q_stride, int(nq), c.d, c.T, c.lsize, ix->sroots.p,
+                                                          ix->dnode.p, w.leaves.p, qlist, nlist_dev);
+    else
+      k_descend<false><<<grid, kDescThreads, 0, st>>>(dq, q_stride, int(nq), c.d, c.T, c.lsize, ix->sroots.p,
+                                                      ix->dnode.p, w.leaves.p, qlist, nlist_dev);
     if (!qlist) {  // query order by first leaf
       int shift = std::max(0, ceil_log2(ix->n + 1) - 12);
       hipMemsetAsync(w.qhist.p, 0, size_t(nb_q + 1) * 4, st);

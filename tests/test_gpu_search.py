@@ -316,3 +316,58 @@ def test_search_irregular_graph_matches_oracle(cuda):
     np.testing.assert_array_equal(gids, oids)
     np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32))
     assert gix.counters()["search_dist"] == int(ond.sum())
+
+
+def _caterpillar_trees(base, leaf=2):
+    """Two search trees over `base`.  Tree 0: each internal node splits off
+    the `leaf` rows with the largest first coordinate as its right child (a
+    leaf), so the tree is n / leaf levels deep and a query that goes left at
+    the top has far more pending far children than a short stack holds.
+    Tree 1: a single leaf holding every row."""
+    n = base.shape[0]
+    order = np.argsort(base[:, 0], kind="stable").astype(np.uint32)
+    m = n // leaf - 1  # internal nodes of tree 0
+    size = 2 * m + 1 + 1
+    dim = np.zeros(size, np.int32)
+    val = np.zeros(size, np.float32)
+    left = np.full(size, -1, np.int32)
+    right = np.full(size, -1, np.int32)
+    start = np.zeros(size, np.uint32)
+    end = np.zeros(size, np.uint32)
+    for k in range(m):  # internal node k covers positions [0, n - k * leaf)
+        hi = n - k * leaf
+        val[k] = base[order[hi - leaf], 0]
+        left[k] = k + 1 if k + 1 < m else 2 * m
+        right[k] = m + k
+        end[k] = hi
+        start[m + k], end[m + k] = hi - leaf, hi
+    end[2 * m] = n - m * leaf  # last left leaf: [0, n - m * leaf)
+    end[2 * m + 1] = n  # tree 1
+    leaves = np.stack([order, np.arange(n, dtype=np.uint32)])
+    return dict(dim=dim, val=val, left=left, right=right, start=start, end=end,
+                roots=np.array([0, 2 * m + 1], np.int32), leaf=leaves)
+
+
+def test_search_deep_tree_parent_walk(cuda):
+    """Seed leaves past the descent stack's depth: on the caterpillar tree the
+    first leaf is 99 levels down, so after the 16 deepest far children the
+    descent climbs parent links for every further leaf, and the one-leaf tree
+    records its root and stops."""
+    rng = np.random.default_rng(5)
+    n, d = 200, 8
+    base = rng.random((n, d), dtype=np.float32)
+    q = rng.random((30, d), dtype=np.float32)
+    q[:10, 0] *= 0.05  # left at the top of tree 0: the deepest walks
+    trees = _caterpillar_trees(base)
+    oix = oracle.Index(base, trees=2, ml=1, epochs=1, S=2, K=4, L=8, check=4)
+    oix.set_trees(trees)
+    gt, _ = oracle.exact_knn(base, base, 6, exclude_self=True)
+    off = np.arange(n + 1, dtype=np.uint64) * 6
+    ids = gt.astype(np.uint32).ravel()
+    oix.set_graph(off, ids)
+    gix = _gpu_index(base, trees, (off, ids))
+    # pool 200 over 2 trees: 41 leaves per tree, 25 of them past the stack
+    gids, gd, oids, od, ond = _search_both(oix, gix, q, 10, 2, 200, 40, 2)
+    np.testing.assert_array_equal(gids, oids)
+    np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32))
+    assert gix.counters()["search_dist"] == int(ond.sum())
