@@ -70,6 +70,16 @@ int upload_trees(efg_index* ix, const HostTrees& h) {
   ix->T = h.T;
   ix->M = M;
   ix->snode_valid = false;
+  {  // tree 0's leaf list holds every row once (the join's task order)
+    std::vector<uint8_t> seen(ix->n, 0);
+    bool perm = true;
+    for (size_t i = 0; i < ix->n && perm; ++i) {
+      const uint32_t x = h.leaf[i];
+      perm = x < ix->n && !seen[x];
+      if (perm) seen[x] = 1;
+    }
+    ix->leaf0_perm = perm;
+  }
   ix->leaf_dim = h.leaf_dim;
   if (ix->leaf_dim.size() != M) ix->leaf_dim.assign(M, -1);
   return EFG_OK;

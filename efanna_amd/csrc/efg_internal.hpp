@@ -146,6 +146,7 @@ struct efg_index {
   efg::DevBuf<uint2> range;      // [start, end)
   efg::DevBuf<int32_t> roots;
   efg::DevBuf<uint32_t> leaf;    // T x n
+  bool leaf0_perm = false;       // tree 0's leaf list holds every row once
   std::vector<int8_t> leaf_dim;  // value written as DivDim of a leaf (0 or -1)
   efg::DevBuf<efg::SNode> snode;  // search layout of the trees
   efg::DevBuf<efg::DNode> dnode;  // descent copy of snode

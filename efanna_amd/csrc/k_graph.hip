@@ -598,35 +598,45 @@ __device__ inline uint32_t join_task_count(uint32_t na, uint32_t nb) {
   return nI * nJ - nI * (nI - 1) / 2;
 }
 
-// Tasks per node; *rows accumulates the rows joined, sum of |new| + |old|
+// Tasks per node, the nodes taken in the order of tree 0's leaf list
+// (order[i] = node at position i; node i when order is null), so that nodes of one leaf, whose lists
+// share rows, get neighbouring task ranges and run at about the same time on
+// one XCD; nodes with at most node_rows rows get none (the whole-node
+// kernel joins them); *rows accumulates the rows joined, sum of |new| + |old|
 // over the nodes with a non-empty new list (one atomic per wavefront).
-__global__ void k_join_count(uint32_t n, const uint64_t* __restrict__ new_off, const uint64_t* __restrict__ old_off,
-                             int32_t* __restrict__ cnt, unsigned long long* __restrict__ rows) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_join_count(uint32_t n, const uint32_t* __restrict__ order, const uint64_t* __restrict__ new_off,
+                             const uint64_t* __restrict__ old_off, uint32_t node_rows, int32_t* __restrict__ cnt,
+                             unsigned long long* __restrict__ rows) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long r = 0;
-  if (p < n) {
+  if (i < n) {
+    const uint32_t p = order ? order[i] : i;
     const uint32_t na = uint32_t(new_off[p + 1] - new_off[p]), nb = uint32_t(old_off[p + 1] - old_off[p]);
-    cnt[p] = int32_t(join_task_count(na, nb));
+    cnt[i] = na + nb <= node_rows ? 0 : int32_t(join_task_count(na, nb));
     r = na ? na + nb : 0;
   }
   for (int o = 32; o > 0; o >>= 1) r += (unsigned long long)__shfl_down((long long)r, o);
   if ((threadIdx.x & 63) == 0 && r) atomicAdd(rows, r);
 }
 
-// Task records, one wavefront per node: header (node, sizeA | newA << 8 |
-// sizeB << 16 | newB << 24, I == J) and the rows' ids, slots [0, 32) block I,
-// [32, 64) block J (unused when I == J), kNoRow past the blocks' ends.
-__global__ void k_join_tasks(uint32_t n, const uint64_t* __restrict__ new_off, const uint32_t* __restrict__ new_ids,
-                             const uint64_t* __restrict__ old_off, const uint32_t* __restrict__ old_ids,
+// Task records, one wavefront per node (position i of the order): header
+// (node, sizeA | newA << 8 | sizeB << 16 | newB << 24, I == J) and the rows'
+// ids, slots [0, 32) block I, [32, 64) block J (unused when I == J), kNoRow
+// past the blocks' ends.  Nodes of at most node_rows rows have no tasks
+// (k_join_count gave them none).
+__global__ void k_join_tasks(uint32_t n, const uint32_t* __restrict__ order, const uint64_t* __restrict__ new_off,
+                             const uint32_t* __restrict__ new_ids, const uint64_t* __restrict__ old_off,
+                             const uint32_t* __restrict__ old_ids, uint32_t node_rows,
                              const int32_t* __restrict__ toff, uint4* __restrict__ hdr, uint32_t* __restrict__ tids) {
-  const uint32_t p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (p >= n) return;
+  if (i >= n) return;
+  const uint32_t p = order ? order[i] : i;
   const uint64_t a0 = new_off[p], b0 = old_off[p];
   const int na = int(new_off[p + 1] - a0), m = na + int(old_off[p + 1] - b0);
-  if (na == 0) return;
+  if (na == 0 || uint32_t(m) <= node_rows) return;
   const int nI = (na + kJBR - 1) / kJBR, nJ = (m + kJBR - 1) / kJBR;
-  size_t t = size_t(toff[p]);
+  size_t t = size_t(toff[i]);
   for (int I = 0; I < nI; ++I)
     for (int J = I; J < nJ; ++J, ++t) {
       const int sizeA = min(kJBR, m - I * kJBR), newA = min(kJBR, na - I * kJBR);
@@ -981,7 +991,7 @@ __global__ void k_join_scatter(const Stage* __restrict__ stage, unsigned long lo
 // for bit.
 // ---------------------------------------------------------------------------
 
-// Split rows: hi[r][e], lo[r][e] (Dk a multiple of 16, zero padded) and the
+// Split rows: hi[r][e], lo[r][e] (Dk a multiple of 64, zero padded) and the
 // FP32 squared norm of row r; one wavefront per row.
 template <bool kF16>
 __global__ void k_pack_split(const float* __restrict__ X, size_t n, int d, int D, int Dk, uint16_t* __restrict__ hi,
@@ -1028,23 +1038,67 @@ __device__ __forceinline__ f32x16_t mfma16(const uint4& a, const uint4& b, const
   }
 }
 
-// One 32 x 32 block of split dot products: this lane feeds A row (lane % 32)
-// from row ra and B row (lane % 32) from row rb, columns 8 (lane / 32) .. + 7
-// of every 16; element l of the result is A row 8 (l / 4) + 4 (lane / 32) +
-// l % 4 against B row lane % 32.
-template <bool kF16>
+// One 32 x 32 block of split dot products: lane (r, h) = (lane % 32,
+// lane / 32) feeds A row r from row ra and B row r from row rb.  Columns go
+// in chunks of 64; within a chunk the lane holds the 32 consecutive columns
+// from 32 h as four 16-byte pieces, piece j feeding k-step j, so k-step j
+// takes columns 8 j .. 8 j + 7 and 32 + 8 j .. 32 + 8 j + 7 of the chunk:
+// one fixed order of the columns for both operands, which leaves every
+// product and the dot product's terms unchanged, while each row is read
+// in whole 128-byte lines.  The next chunk is loaded while this one is
+// multiplied.  Element l of the result is A row 8 (l / 4) + 4 (lane / 32) +
+// l % 4 against B row lane % 32.  Dk is a multiple of 64; with kSame the B
+// rows are the A rows (rb is not read).
+template <bool kF16, bool kSame>
 __device__ __forceinline__ f32x16_t mfma_split(const uint16_t* __restrict__ hi, const uint16_t* __restrict__ lo,
                                                size_t ra, size_t rb, int Dk) {
-  const int kh = (threadIdx.x & 32) ? 8 : 0;
-  const uint16_t *ah = hi + ra * Dk + kh, *al = lo + ra * Dk + kh, *bh = hi + rb * Dk + kh, *bl = lo + rb * Dk + kh;
+  const int kh = (threadIdx.x & 32) ? 32 : 0;
+  const uint4* ah = reinterpret_cast<const uint4*>(hi + ra * Dk + kh);
+  const uint4* al = reinterpret_cast<const uint4*>(lo + ra * Dk + kh);
+  const uint4* bh = reinterpret_cast<const uint4*>(hi + rb * Dk + kh);
+  const uint4* bl = reinterpret_cast<const uint4*>(lo + rb * Dk + kh);
+  const int nch = Dk >> 6;
+  uint4 a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    a0[j] = ah[j];
+    a1[j] = al[j];
+    if (!kSame) {
+      b0[j] = bh[j];
+      b1[j] = bl[j];
+    }
+  }
   f32x16_t acc = f32x16_t{};
-#pragma unroll 2
-  for (int k = 0; k < Dk; k += 16) {
-    const uint4 a0 = *reinterpret_cast<const uint4*>(ah + k), a1 = *reinterpret_cast<const uint4*>(al + k);
-    const uint4 b0 = *reinterpret_cast<const uint4*>(bh + k), b1 = *reinterpret_cast<const uint4*>(bl + k);
-    acc = mfma16<kF16>(a0, b0, acc);
-    acc = mfma16<kF16>(a0, b1, acc);
-    acc = mfma16<kF16>(a1, b0, acc);
+#pragma unroll 1
+  for (int c = 0; c < nch; ++c) {
+    const int o = (c + 1 < nch ? c + 1 : c) * 8;  // next chunk (the last one again at the end)
+    uint4 n0[4], n1[4], m0[4], m1[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      n0[j] = ah[o + j];
+      n1[j] = al[o + j];
+      if (!kSame) {
+        m0[j] = bh[o + j];
+        m1[j] = bl[o + j];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint4& y0 = kSame ? a0[j] : b0[j];
+      const uint4& y1 = kSame ? a1[j] : b1[j];
+      acc = mfma16<kF16>(a0[j], y0, acc);
+      acc = mfma16<kF16>(a0[j], y1, acc);
+      acc = mfma16<kF16>(a1[j], y0, acc);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      a0[j] = n0[j];
+      a1[j] = n1[j];
+      if (!kSame) {
+        b0[j] = m0[j];
+        b1[j] = m1[j];
+      }
+    }
   }
   return acc;
 }
@@ -1078,7 +1132,7 @@ __global__ void __launch_bounds__(kBlock) k_leaf_mfma(const uint16_t* __restrict
       }
       for (int mb = 0; mb < m; mb += 32) {
         const uint32_t mid = (mb + r32 < m) ? leaf_t[r.x + mb + r32] : 0u;  // member of B row r32
-        const f32x16_t acc = mfma_split<kF16>(xhi, xlo, pid, mid, Dk);
+        const f32x16_t acc = mfma_split<kF16, false>(xhi, xlo, pid, mid, Dk);
         const float pn = xn[pid], mn = xn[mid];
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
@@ -1130,7 +1184,7 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
     cnt_s[wv][lane] = 0;
     const uint32_t ia = uint32_t(__shfl(int(id), r32)), ib = uint32_t(__shfl(int(id), same ? r32 : 32 + r32));
     const uint32_t ra = ia != kNoRow ? ia : 0u, rbw = ib != kNoRow ? ib : 0u;
-    const f32x16_t acc = mfma_split<kF16>(xhi, xlo, ra, rbw, Dk);
+    const f32x16_t acc = same ? mfma_split<kF16, true>(xhi, xlo, ra, ra, Dk) : mfma_split<kF16, false>(xhi, xlo, ra, rbw, Dk);
     const float na = xn[ra], nb = xn[rbw];
     wave_sync();
     float dist[16];
@@ -1194,6 +1248,218 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
       if (lb != 0xffffu) {
         const unsigned long long j = off_s[wv][sb] + lb, at = j < rem ? r0 + j : nr + (j - rem);
         if (at < cap) stage[at] = Stage{b, nn, make_key(dist[i], a)};
+      }
+    }
+    wave_sync();
+  }
+  for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
+  for (int o = 32; o > 0; o >>= 1) pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
+  if (lane == 0 && pairs_local) atomicAdd(npairs, pairs_local);
+}
+
+// Three 32 x 32 blocks of split dot products of one node's rows, loaded once:
+// A = rows ra (positions 0..31), B = rows rb (positions 32..63); t00 = A.A,
+// t01 = A.B, t11 = B.B, columns in the order of mfma_split.
+template <bool kF16>
+__device__ __forceinline__ void mfma_split3(const uint16_t* __restrict__ hi, const uint16_t* __restrict__ lo,
+                                            size_t ra, size_t rb, int Dk, f32x16_t& t00, f32x16_t& t01,
+                                            f32x16_t& t11) {
+  const int kh = (threadIdx.x & 32) ? 32 : 0;
+  const uint4* ah = reinterpret_cast<const uint4*>(hi + ra * Dk + kh);
+  const uint4* al = reinterpret_cast<const uint4*>(lo + ra * Dk + kh);
+  const uint4* bh = reinterpret_cast<const uint4*>(hi + rb * Dk + kh);
+  const uint4* bl = reinterpret_cast<const uint4*>(lo + rb * Dk + kh);
+  const int nch = Dk >> 6;
+  uint4 a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    a0[j] = ah[j];
+    a1[j] = al[j];
+    b0[j] = bh[j];
+    b1[j] = bl[j];
+  }
+  t00 = f32x16_t{};
+  t01 = f32x16_t{};
+  t11 = f32x16_t{};
+#pragma unroll 1
+  for (int c = 0; c < nch; ++c) {
+    const int o = (c + 1 < nch ? c + 1 : c) * 8;  // next chunk (the last one again at the end)
+    uint4 n0[4], n1[4], m0[4], m1[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      n0[j] = ah[o + j];
+      n1[j] = al[o + j];
+      m0[j] = bh[o + j];
+      m1[j] = bl[o + j];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      t00 = mfma16<kF16>(a0[j], a0[j], t00);
+      t01 = mfma16<kF16>(a0[j], b0[j], t01);
+      t11 = mfma16<kF16>(b0[j], b0[j], t11);
+      t00 = mfma16<kF16>(a0[j], a1[j], t00);
+      t01 = mfma16<kF16>(a0[j], b1[j], t01);
+      t11 = mfma16<kF16>(b0[j], b1[j], t11);
+      t00 = mfma16<kF16>(a1[j], a0[j], t00);
+      t01 = mfma16<kF16>(a1[j], b0[j], t01);
+      t11 = mfma16<kF16>(b1[j], b0[j], t11);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      a0[j] = n0[j];
+      a1[j] = n1[j];
+      b0[j] = m0[j];
+      b1[j] = m1[j];
+    }
+  }
+}
+
+// Positions P < Q of a node's rows (new list, then old list): the pair the
+// reference measures when P is new; slots and ids of the two endpoints, the
+// first endpoint being the lower id of a new x new pair (as pair_ends).
+__device__ inline bool node_ends(int P, int Q, int na, int m, const uint32_t* ids, int& sa, int& sb, uint32_t& a,
+                                 uint32_t& b) {
+  if (P >= na || Q >= m || Q <= P) return false;
+  sa = P;
+  sb = Q;
+  a = ids[P];
+  b = ids[Q];
+  if (Q < na) {
+    if (a == b) return false;
+    if (b < a) {
+      const uint32_t t2 = a;
+      a = b;
+      b = t2;
+      sa = Q;
+      sb = P;
+    }
+  }
+  return true;
+}
+
+// Whole-node join on MFMA (precision form) for the nodes with at most 64
+// rows: one wavefront per node, each row read once (the block tasks read a
+// node's rows once per task), the node's pairs from the blocks (0,0), and
+// for more than 32 rows (0,1) and (1,1).  Nodes are taken in the order of
+// tree 0's leaf list (order, null: node order), one contiguous range per
+// XCD.  Candidates are counted, offset and staged as in k_join_mfma.
+template <bool kF16>
+__global__ void __launch_bounds__(kBlock) k_join_node_mfma(const uint16_t* __restrict__ xhi,
+                                                           const uint16_t* __restrict__ xlo,
+                                                           const float* __restrict__ xn, int Dk, uint32_t n,
+                                                           const uint32_t* __restrict__ order,
+                                                           const uint64_t* __restrict__ new_off,
+                                                           const uint32_t* __restrict__ new_ids,
+                                                           const uint64_t* __restrict__ old_off,
+                                                           const uint32_t* __restrict__ old_ids,
+                                                           const uint64_t* __restrict__ worst, Stage* __restrict__ stage,
+                                                           unsigned long long cap,
+                                                           unsigned long long* __restrict__ nstage,
+                                                           uint32_t* __restrict__ tcount,
+                                                           unsigned long long* __restrict__ npairs) {
+  constexpr int kW = kBlock / 64;
+  __shared__ uint32_t ids_s[kW][64];
+  __shared__ uint64_t w_s[kW][64];
+  __shared__ uint32_t cnt_s[kW][64];
+  __shared__ uint32_t off_s[kW][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r32 = lane & 31;
+  const uint32_t xcd = blockIdx.x & 7u, nsl = (gridDim.x >> 3) * kW;
+  const uint32_t lo = uint32_t(uint64_t(n) * xcd / 8), hi = uint32_t(uint64_t(n) * (xcd + 1) / 8);
+  unsigned long long pairs_local = 0;
+  unsigned long long wbase = 0, wend = 0;  // staging run
+  uint32_t* ids = ids_s[wv];
+  for (uint32_t i = lo + (blockIdx.x >> 3) * kW + wv; i < hi; i += nsl) {
+    const uint32_t nd = order ? order[i] : i;
+    const uint64_t a0 = new_off[nd], b0 = old_off[nd];
+    const int na = int(new_off[nd + 1] - a0), m = na + int(old_off[nd + 1] - b0);
+    if (na == 0 || m > 64) continue;  // no pairs / the block tasks' nodes
+    const uint32_t id = lane < m ? (lane < na ? new_ids[a0 + lane] : old_ids[b0 + lane - na]) : kNoRow;
+    ids[lane] = id;
+    w_s[wv][lane] = id != kNoRow ? worst[id] : 0ull;
+    cnt_s[wv][lane] = 0;
+    const uint32_t ia = uint32_t(__shfl(int(id), r32)), ib = uint32_t(__shfl(int(id), 32 + r32));
+    const uint32_t ra = ia != kNoRow ? ia : 0u, rb = ib != kNoRow ? ib : 0u;
+    const int nt = m > 32 ? 3 : 1;  // blocks (0,0) [, (0,1), (1,1)]
+    f32x16_t t[3];
+    if (nt == 3) {
+      mfma_split3<kF16>(xhi, xlo, ra, rb, Dk, t[0], t[1], t[2]);
+    } else {
+      t[0] = mfma_split<kF16, true>(xhi, xlo, ra, ra, Dk);
+      t[1] = f32x16_t{};
+      t[2] = f32x16_t{};
+    }
+    const float nra = xn[ra], nrb = xn[rb];
+    wave_sync();
+    float dist[3][16];
+    uint32_t li[3][16];  // first endpoint's index | second's << 16; ~0u: no pair
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const int pb = b == 2 ? 32 : 0, qb = b == 0 ? 0 : 32;  // position offsets of the A and B rows
+      const float nq = b == 0 ? nra : nrb;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int p = 8 * (e / 4) + 4 * (lane >> 5) + (e & 3);
+        const float np = __shfl(b == 2 ? nrb : nra, p);
+        const float dd = __builtin_fmaf(-2.f, t[b][e], np + nq);
+        dist[b][e] = dd > 0.f ? dd : 0.f;
+        li[b][e] = ~0u;
+        int sa, sb;
+        uint32_t x, y;
+        if (b < nt && node_ends(pb + p, qb + r32, na, m, ids, sa, sb, x, y)) {
+          ++pairs_local;
+          uint32_t la = 0xffffu, lb = 0xffffu;
+          if (make_key(dist[b][e], y) < w_s[wv][sa]) la = atomicAdd(&cnt_s[wv][sa], 1u);
+          if (make_key(dist[b][e], x) < w_s[wv][sb]) lb = atomicAdd(&cnt_s[wv][sb], 1u);
+          li[b][e] = la | lb << 16;
+        }
+      }
+    }
+    wave_sync();
+    // totals per target (result not waited for), offsets within the node
+    const uint32_t cnt = cnt_s[wv][lane];
+    if (cnt) atomicAdd(&tcount[id], cnt);
+    uint32_t x = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = uint32_t(__shfl_up(int(x), o));
+      if (lane >= o) x += y;
+    }
+    off_s[wv][lane] = x - cnt;
+    const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
+    // this node's slots: the rest of the run, then a new run if needed
+    const unsigned long long rem = wend - wbase;
+    unsigned long long nr = 0;
+    if (tot > rem) {
+      if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
+      nr = (unsigned long long)__shfl((long long)nr, 0);
+    }
+    const unsigned long long r0 = wbase;
+    if (tot > rem) {
+      wbase = nr + (tot - rem);
+      wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
+    } else {
+      wbase += tot;
+    }
+    wave_sync();
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const int pb = b == 2 ? 32 : 0, qb = b == 0 ? 0 : 32;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        if (li[b][e] == ~0u) continue;
+        const int p = 8 * (e / 4) + 4 * (lane >> 5) + (e & 3);
+        int sa, sb;
+        uint32_t xa, xb;
+        node_ends(pb + p, qb + r32, na, m, ids, sa, sb, xa, xb);
+        const uint32_t la = li[b][e] & 0xffffu, lb = li[b][e] >> 16;
+        if (la != 0xffffu) {
+          const unsigned long long j = off_s[wv][sa] + la, at = j < rem ? r0 + j : nr + (j - rem);
+          if (at < cap) stage[at] = Stage{xa, nd | kFirst, make_key(dist[b][e], xb)};
+        }
+        if (lb != 0xffffu) {
+          const unsigned long long j = off_s[wv][sb] + lb, at = j < rem ? r0 + j : nr + (j - rem);
+          if (at < cap) stage[at] = Stage{xb, nd, make_key(dist[b][e], xa)};
+        }
       }
     }
     wave_sync();
@@ -1581,7 +1847,7 @@ int grid_for(size_t n) { return int((n + 255) / 256); }
 int ensure_split(efg_index* ix, hipStream_t st) {
   if (ix->b_precision == 0) return EFG_OK;
   if (ix->split_hi.p && ix->split_kind == ix->b_precision) return EFG_OK;
-  const int Dk = int((ix->d + 15) / 16 * 16);
+  const int Dk = int((ix->d + 63) / 64 * 64);  // whole 64-column chunks (zero padded)
   if (!ix->split_hi.alloc(ix->n * size_t(Dk)) || !ix->split_lo.alloc(ix->n * size_t(Dk)) ||
       !ix->split_norm.alloc(ix->n))
     return fail(EFG_ERR_MEMORY, "out of device memory (split rows)");
@@ -1772,7 +2038,11 @@ int gpu_refine_iter(efg_index* ix) {
   k_pair_bound<<<grid_for(n), 256, 0, st>>>(n, bs.new_off.p, bs.old_off.p, ctr.p);
   DevBuf<int32_t> tcnt, tskoff;
   if (!tcnt.alloc(n) || !tskoff.alloc(n + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (join tasks)");
-  k_join_count<<<grid_for(n), 256, 0, st>>>(n, bs.new_off.p, bs.old_off.p, tcnt.p, ctr.p + 2);
+  // tree 0's leaf list (a permutation of the nodes) orders the tasks
+  const uint32_t* order = ix->leaf0_perm ? ix->leaf.p : nullptr;
+  // precision form: nodes of at most 64 rows go to the whole-node kernel
+  const uint32_t node_rows = ix->b_precision > 0 ? 64u : 0u;
+  k_join_count<<<grid_for(n), 256, 0, st>>>(n, order, bs.new_off.p, bs.old_off.p, node_rows, tcnt.p, ctr.p + 2);
   if (int rc = device_exclusive_scan(tcnt.p, int(n), tskoff.p, scal.p + 2, sums, st)) return rc;
   unsigned long long bound = 0, rows = 0;
   int ntask = 0;
@@ -1787,15 +2057,16 @@ int gpu_refine_iter(efg_index* ix) {
   // eight task ranges, one per XCD: a multiple of eight workgroups
   const size_t join_grid = size_t((ncu * 4 + 7) / 8) * 8;
   // two candidates per pair at most, plus one partly used run per workgroup
-  // (per wavefront on the MFMA path, ncu * 8 workgroups of four)
-  const size_t cap16 = size_t(2 * bound) + size_t(ncu) * 8 * (kBlock / 64) * kStageRun + 1024;
+  // (per wavefront on the MFMA path, ncu * 8 workgroups of four, two kernels)
+  const size_t cap16 = size_t(2 * bound) + 2 * size_t(ncu) * 8 * (kBlock / 64) * kStageRun + 1024;
   const size_t cap = ix->b_precision > 0 ? cap16 : size_t(2 * bound) + join_grid * kStageRun + 1024;
   if (!bs.jhdr.ensure(size_t(ntask) + 1) || !bs.jtids.ensure((size_t(ntask) + 1) * 64) ||
       !bs.jstage.ensure(cap * sizeof(Stage)))
     return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
   Stage* stage = reinterpret_cast<Stage*>(bs.jstage.p);
-  k_join_tasks<<<unsigned((size_t(n) * 64 + 255) / 256), 256, 0, st>>>(n, bs.new_off.p, bs.new_ids.p, bs.old_off.p,
-                                                                      bs.old_ids.p, tskoff.p, bs.jhdr.p, bs.jtids.p);
+  k_join_tasks<<<unsigned((size_t(n) * 64 + 255) / 256), 256, 0, st>>>(n, order, bs.new_off.p, bs.new_ids.p,
+                                                                      bs.old_off.p, bs.old_ids.p, node_rows,
+                                                                      tskoff.p, bs.jhdr.p, bs.jtids.p);
   // one chunk of at most 64 columns of both blocks' rows in LDS
   const int DC = std::min(D, 64);
   const size_t lds_join = size_t(64) * (DC + 4) * 4;
@@ -1804,18 +2075,19 @@ int gpu_refine_iter(efg_index* ix) {
   unsigned long long h_ctr[2] = {0, 0};
   hipMemsetAsync(ctr.p, 0, 16, st);
   hipMemsetAsync(tcount.p, 0, size_t(n) * 4, st);
-  if (ntask && ix->b_precision > 0) {
-    // precision form: one wavefront per task on MFMA
+  if (ix->b_precision > 0) {
+    // precision form on MFMA: one wavefront per node of at most 64 rows,
+    // then one per block task of the larger nodes
     if (int rc = ensure_split(ix, st)) return rc;
     const unsigned g16 = unsigned(size_t(ncu) * 8);
-    if (ix->b_precision == 2)
-      k_join_mfma<true><<<g16, kBlock, 0, st>>>(ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk,
-                                                uint32_t(ntask), bs.jhdr.p, bs.jtids.p, worst.p, stage,
-                                                (unsigned long long)cap16, ctr.p, tcount.p, ctr.p + 1);
-    else
-      k_join_mfma<false><<<g16, kBlock, 0, st>>>(ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk,
-                                                 uint32_t(ntask), bs.jhdr.p, bs.jtids.p, worst.p, stage,
-                                                 (unsigned long long)cap16, ctr.p, tcount.p, ctr.p + 1);
+    const bool f16 = ix->b_precision == 2;
+    (f16 ? k_join_node_mfma<true> : k_join_node_mfma<false>)<<<g16, kBlock, 0, st>>>(
+        ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, n, order, bs.new_off.p, bs.new_ids.p,
+        bs.old_off.p, bs.old_ids.p, worst.p, stage, (unsigned long long)cap16, ctr.p, tcount.p, ctr.p + 1);
+    if (ntask)
+      (f16 ? k_join_mfma<true> : k_join_mfma<false>)<<<g16, kBlock, 0, st>>>(
+          ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, uint32_t(ntask), bs.jhdr.p, bs.jtids.p,
+          worst.p, stage, (unsigned long long)cap16, ctr.p, tcount.p, ctr.p + 1);
   } else if (ntask) {
     k_join_tiles<4><<<unsigned(join_grid), kBlock, lds_join, st>>>(ix->data.p, D, uint32_t(ntask), bs.jhdr.p,
                                                                    bs.jtids.p, worst.p, stage,

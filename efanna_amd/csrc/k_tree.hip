@@ -496,6 +496,7 @@ int gpu_build_trees(efg_index* ix) {
   ix->T = T;
   ix->M = size_t(M);
   ix->snode_valid = false;
+  ix->leaf0_perm = true;  // every leaf list is a permutation of the rows
   // DivDim written for leaves in the trees file: leaves the DFS phase made
   // (and roots) carry -1, leaves left behind by the BFS phase carry 0
   // (zero-initialised nodes, kdtreeub_index.hpp:1249, 1142-1143).
