@@ -105,6 +105,26 @@ struct BuildState {
   DevBuf<unsigned char> jstage;
   DevBuf<uint64_t> jbkey;
   DevBuf<uint32_t> jbsrc;
+  // per-iteration scratch of the join and the list update (n or n + 1
+  // entries, the reverse lists as needed), kept across iterations so that
+  // no iteration allocates or frees device memory once they have grown
+  struct IterScratch {
+    DevBuf<uint64_t> worst;
+    DevBuf<uint32_t> tcount, cursor;
+    DevBuf<int32_t> toff, sums, scal, tcnt, tskoff;
+    DevBuf<unsigned long long> ctr;
+    DevBuf<uint32_t> fnew, fold, rcn, rco, rcurn, rcuro, rszn, rszo, ridn, rido;
+    DevBuf<int32_t> roffn, roffo, snew, sold, onew, oold;
+    void release() {
+      for (DevBuf<uint32_t>* b : {&tcount, &cursor, &fnew, &fold, &rcn, &rco, &rcurn, &rcuro, &rszn, &rszo, &ridn, &rido})
+        b->reset();
+      for (DevBuf<int32_t>* b : {&toff, &sums, &scal, &tcnt, &tskoff, &roffn, &roffo, &snew, &sold, &onew, &oold})
+        b->reset();
+      worst.reset();
+      ctr.reset();
+    }
+  };
+  IterScratch it;
   int iter = 0;
   bool ready = false;
 };

@@ -708,6 +708,28 @@ __device__ inline bool block_ends(int p, int q, bool same, int sizeA, int newA, 
   return pair_ends(p, q, same, newA, newB, ids_s, sa, sb, a, b);
 }
 
+// Positions P < Q of a node's rows (new list, then old list): the pair the
+// reference measures when P is new; slots and ids of the two endpoints, the
+// first endpoint being the lower id of a new x new pair (as pair_ends).
+__device__ inline bool node_ends(int P, int Q, int na, int m, const uint32_t* ids, int& sa, int& sb, uint32_t& a,
+                                 uint32_t& b) {
+  if (P >= na || Q >= m || Q <= P) return false;
+  sa = P;
+  sb = Q;
+  a = ids[P];
+  b = ids[Q];
+  if (Q < na) {
+    if (a == b) return false;
+    if (b < a) {
+      const uint32_t t2 = a;
+      a = b;
+      b = t2;
+      sa = Q;
+      sb = P;
+    }
+  }
+  return true;
+}
 
 // Load k of this thread for the rows of one chunk (DC logical columns of up
 // to 64 rows, float4 i = threadIdx.x + k * kBlock is row i / (DC / 4)).
@@ -939,6 +961,191 @@ k_join_tiles(const float* __restrict__ X, int D, uint32_t ntask, const uint4* __
                              to = (unsigned long long)__shfl((long long)wend, 63);
     for (unsigned long long i = from + lane; i < to && i < cap; i += 64) stage[i].target = kNoRow;
   }
+  for (int o = 32; o > 0; o >>= 1) pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
+  if (lane == 0 && pairs_local) atomicAdd(npairs, pairs_local);
+}
+
+// Whole-node join in FP32 for the nodes with at most 32 rows: one wavefront
+// per node (the block tasks give such a node a whole workgroup and leave
+// most of its threads idle).  The node's rows go through the wavefront's own
+// LDS slice one chunk of DC logical columns at a time (the next chunk loaded
+// into registers meanwhile); each lane keeps the reference's eight FMA
+// chains for four pairs (P < Q, P new, in join_cell order) at a time, a
+// second pass over the chunks taking pairs 256 .. 495, so every distance
+// equals the block tasks' and the reference's.  Nodes are taken in the
+// order of tree 0's leaf list (order, null: node order), one contiguous
+// range per XCD; candidates are counted, offset and staged as in
+// k_join_node_mfma.
+constexpr int kNodeRows = 32;
+constexpr int kNodePairs = 8;  // per lane: ceil(32 * 31 / 2 / 64)
+constexpr int kNodeDC = 64;
+constexpr int kNodeRS = kNodeDC + 4;  // LDS row stride (floats)
+constexpr int kNodeLd = kNodeRows * kNodeDC / 4 / 64;  // float4 loads per lane and chunk (8)
+__device__ __forceinline__ float4 node_load(const float* __restrict__ X, int D, const uint32_t* ids, int m, int l0,
+                                            int w4, int k) {
+  const int i = (threadIdx.x & 63) + 64 * k, r = i / (kNodeDC / 4), c4 = i - r * (kNodeDC / 4);
+  const bool in = r < m && c4 < w4;
+  // out-of-range slots load row 0, column 0 (never stored)
+  return *reinterpret_cast<const float4*>(X + size_t(in ? ids[r] : 0u) * D + (in ? acc_col(l0 + 4 * c4, D) : 0));
+}
+// Distances of this lane's pairs k0 + lane + 64 j, j < 4 (k < cells).
+__device__ __forceinline__ void node_pairs4(const float* __restrict__ X, int D, const uint32_t* ids, int m, int cells,
+                                            int k0, float* rows, const int* pq, float* out) {
+  const int lane = threadIdx.x & 63;
+  const int nch = (D + kNodeDC - 1) / kNodeDC;
+  float acc[4][8];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) acc[j][e] = 0.f;
+  float4 pf[kNodeLd];
+#pragma unroll
+  for (int k = 0; k < kNodeLd; ++k) pf[k] = node_load(X, D, ids, m, 0, min(kNodeDC, D) >> 2, k);
+#pragma unroll 1
+  for (int c = 0; c < nch; ++c) {
+    const int w4 = min(kNodeDC, D - c * kNodeDC) >> 2;
+#pragma unroll
+    for (int k = 0; k < kNodeLd; ++k) {
+      const int ii = lane + 64 * k, r = ii / (kNodeDC / 4), c4 = ii - r * (kNodeDC / 4);
+      if (r < m && c4 < w4) *reinterpret_cast<float4*>(rows + r * kNodeRS + 4 * c4) = pf[k];
+    }
+    wave_sync();
+    if (c + 1 < nch) {
+      const int w4n = min(kNodeDC, D - (c + 1) * kNodeDC) >> 2;
+#pragma unroll
+      for (int k = 0; k < kNodeLd; ++k) pf[k] = node_load(X, D, ids, m, (c + 1) * kNodeDC, w4n, k);
+    }
+    const int nb8 = w4 >> 1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (k0 + lane + 64 * j >= cells) continue;
+      const float4* a = reinterpret_cast<const float4*>(rows + (pq[j] & 0xff) * kNodeRS);
+      const float4* b = reinterpret_cast<const float4*>(rows + (pq[j] >> 8) * kNodeRS);
+#pragma unroll 1
+      for (int e8 = 0; e8 < nb8; ++e8) {
+        const float4 x0 = a[2 * e8], x1 = a[2 * e8 + 1];
+        const float4 y0 = b[2 * e8], y1 = b[2 * e8 + 1];
+        float d;
+        d = x0.x - y0.x; acc[j][0] = __builtin_fmaf(d, d, acc[j][0]);
+        d = x0.y - y0.y; acc[j][1] = __builtin_fmaf(d, d, acc[j][1]);
+        d = x0.z - y0.z; acc[j][2] = __builtin_fmaf(d, d, acc[j][2]);
+        d = x0.w - y0.w; acc[j][3] = __builtin_fmaf(d, d, acc[j][3]);
+        d = x1.x - y1.x; acc[j][4] = __builtin_fmaf(d, d, acc[j][4]);
+        d = x1.y - y1.y; acc[j][5] = __builtin_fmaf(d, d, acc[j][5]);
+        d = x1.z - y1.z; acc[j][6] = __builtin_fmaf(d, d, acc[j][6]);
+        d = x1.w - y1.w; acc[j][7] = __builtin_fmaf(d, d, acc[j][7]);
+      }
+    }
+    wave_sync();  // the slice is rewritten by the next chunk
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    float sdist = acc[j][0];
+#pragma unroll
+    for (int e = 1; e < 8; ++e) sdist = sdist + acc[j][e];
+    out[j] = sdist;
+  }
+}
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_join_node_f32(const float* __restrict__ X, int D, uint32_t n, const uint32_t* __restrict__ order,
+                const uint64_t* __restrict__ new_off, const uint32_t* __restrict__ new_ids,
+                const uint64_t* __restrict__ old_off, const uint32_t* __restrict__ old_ids,
+                const uint64_t* __restrict__ worst, Stage* __restrict__ stage, unsigned long long cap,
+                unsigned long long* __restrict__ nstage, uint32_t* __restrict__ tcount,
+                unsigned long long* __restrict__ npairs) {
+  constexpr int kW = kBlock / 64;
+  __shared__ __align__(16) float rows_s[kW][kNodeRows * kNodeRS];
+  __shared__ uint32_t ids_s[kW][64];
+  __shared__ uint64_t w_s[kW][64];
+  __shared__ uint32_t cnt_s[kW][64];
+  __shared__ uint32_t off_s[kW][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t xcd = blockIdx.x & 7u, nsl = (gridDim.x >> 3) * kW;
+  const uint32_t lo = uint32_t(uint64_t(n) * xcd / 8), hi = uint32_t(uint64_t(n) * (xcd + 1) / 8);
+  unsigned long long pairs_local = 0;
+  unsigned long long wbase = 0, wend = 0;  // staging run
+  uint32_t* ids = ids_s[wv];
+  float* rows = rows_s[wv];
+  for (uint32_t i = lo + (blockIdx.x >> 3) * kW + wv; i < hi; i += nsl) {
+    const uint32_t nd = order ? order[i] : i;
+    const uint64_t a0 = new_off[nd], b0 = old_off[nd];
+    const int na = int(new_off[nd + 1] - a0), m = na + int(old_off[nd + 1] - b0);
+    if (na == 0 || m > kNodeRows) continue;  // no pairs / the block tasks' nodes
+    const uint32_t id = lane < m ? (lane < na ? new_ids[a0 + lane] : old_ids[b0 + lane - na]) : kNoRow;
+    ids[lane] = id;
+    w_s[wv][lane] = id != kNoRow ? worst[id] : 0ull;
+    cnt_s[wv][lane] = 0;
+    wave_sync();
+    const int cells = na * m - na * (na + 1) / 2;
+    int pq[kNodePairs];  // P | Q << 8 of this lane's pairs
+#pragma unroll
+    for (int j = 0; j < kNodePairs; ++j) {
+      int P = 0, Q = 0;
+      if (lane + 64 * j < cells) join_cell(lane + 64 * j, true, m, m, P, Q);
+      pq[j] = P | Q << 8;
+    }
+    float dist[kNodePairs];
+    node_pairs4(X, D, ids, m, cells, 0, rows, pq, dist);
+    if (cells > 256) node_pairs4(X, D, ids, m, cells, 256, rows, pq + 4, dist + 4);
+    uint32_t li[kNodePairs];  // first endpoint's index | second's << 16; ~0u: no pair
+#pragma unroll
+    for (int j = 0; j < kNodePairs; ++j) {
+      li[j] = ~0u;
+      int sa, sb;
+      uint32_t x, y;
+      if (lane + 64 * j < cells && node_ends(pq[j] & 0xff, pq[j] >> 8, na, m, ids, sa, sb, x, y)) {
+        ++pairs_local;
+        uint32_t la = 0xffffu, lb = 0xffffu;
+        if (make_key(dist[j], y) < w_s[wv][sa]) la = atomicAdd(&cnt_s[wv][sa], 1u);
+        if (make_key(dist[j], x) < w_s[wv][sb]) lb = atomicAdd(&cnt_s[wv][sb], 1u);
+        li[j] = la | lb << 16;
+      }
+    }
+    wave_sync();
+    // totals per target (result not waited for), offsets within the node
+    const uint32_t cnt = cnt_s[wv][lane];
+    if (cnt) atomicAdd(&tcount[id], cnt);
+    uint32_t x = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = uint32_t(__shfl_up(int(x), o));
+      if (lane >= o) x += y;
+    }
+    off_s[wv][lane] = x - cnt;
+    const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
+    // this node's slots: the rest of the run, then a new run if needed
+    const unsigned long long rem = wend - wbase;
+    unsigned long long nr = 0;
+    if (tot > rem) {
+      if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
+      nr = (unsigned long long)__shfl((long long)nr, 0);
+    }
+    const unsigned long long r0 = wbase;
+    if (tot > rem) {
+      wbase = nr + (tot - rem);
+      wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
+    } else {
+      wbase += tot;
+    }
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < kNodePairs; ++j) {
+      if (li[j] == ~0u) continue;
+      int sa, sb;
+      uint32_t xa, xb;
+      node_ends(pq[j] & 0xff, pq[j] >> 8, na, m, ids, sa, sb, xa, xb);
+      const uint32_t la = li[j] & 0xffffu, lb = li[j] >> 16;
+      if (la != 0xffffu) {
+        const unsigned long long q = off_s[wv][sa] + la, at = q < rem ? r0 + q : nr + (q - rem);
+        if (at < cap) stage[at] = Stage{xa, nd | kFirst, make_key(dist[j], xb)};
+      }
+      if (lb != 0xffffu) {
+        const unsigned long long q = off_s[wv][sb] + lb, at = q < rem ? r0 + q : nr + (q - rem);
+        if (at < cap) stage[at] = Stage{xb, nd, make_key(dist[j], xa)};
+      }
+    }
+    wave_sync();
+  }
+  for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
   for (int o = 32; o > 0; o >>= 1) pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
   if (lane == 0 && pairs_local) atomicAdd(npairs, pairs_local);
 }
@@ -1312,29 +1519,6 @@ __device__ __forceinline__ void mfma_split3(const uint16_t* __restrict__ hi, con
       b1[j] = m1[j];
     }
   }
-}
-
-// Positions P < Q of a node's rows (new list, then old list): the pair the
-// reference measures when P is new; slots and ids of the two endpoints, the
-// first endpoint being the lower id of a new x new pair (as pair_ends).
-__device__ inline bool node_ends(int P, int Q, int na, int m, const uint32_t* ids, int& sa, int& sb, uint32_t& a,
-                                 uint32_t& b) {
-  if (P >= na || Q >= m || Q <= P) return false;
-  sa = P;
-  sb = Q;
-  a = ids[P];
-  b = ids[Q];
-  if (Q < na) {
-    if (a == b) return false;
-    if (b < a) {
-      const uint32_t t2 = a;
-      a = b;
-      b = t2;
-      sa = Q;
-      sb = P;
-    }
-  }
-  return true;
 }
 
 // Whole-node join on MFMA (precision form) for the nodes with at most 64
@@ -2026,22 +2210,24 @@ int gpu_refine_iter(efg_index* ix) {
   const int L = bs.L, S = p.S, D = int(ix->D);
   hipStream_t st = ix->stream;
   double t0 = now_ms();
-  DevBuf<uint64_t> worst;
-  DevBuf<uint32_t> tcount, cursor;
-  DevBuf<int32_t> toff, sums, scal;
-  DevBuf<unsigned long long> ctr;
-  if (!worst.alloc(n) || !tcount.alloc(n) || !cursor.alloc(n) || !toff.alloc(n + 1) || !scal.alloc(4) ||
-      !ctr.alloc(3))
+  BuildState::IterScratch& sc = bs.it;
+  DevBuf<uint64_t>& worst = sc.worst;
+  DevBuf<uint32_t>&tcount = sc.tcount, &cursor = sc.cursor;
+  DevBuf<int32_t>&toff = sc.toff, &sums = sc.sums, &scal = sc.scal;
+  DevBuf<unsigned long long>& ctr = sc.ctr;
+  if (!worst.ensure(n) || !tcount.ensure(n) || !cursor.ensure(n) || !toff.ensure(n + 1) || !scal.ensure(4) ||
+      !ctr.ensure(3))
     return fail(EFG_ERR_MEMORY, "out of device memory (join)");
   k_worst<<<grid_for(n), 256, 0, st>>>(bs.pool.p, bs.fill.p, n, L, worst.p);
   hipMemsetAsync(ctr.p, 0, 24, st);
   k_pair_bound<<<grid_for(n), 256, 0, st>>>(n, bs.new_off.p, bs.old_off.p, ctr.p);
-  DevBuf<int32_t> tcnt, tskoff;
-  if (!tcnt.alloc(n) || !tskoff.alloc(n + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (join tasks)");
+  DevBuf<int32_t>&tcnt = sc.tcnt, &tskoff = sc.tskoff;
+  if (!tcnt.ensure(n) || !tskoff.ensure(n + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (join tasks)");
   // tree 0's leaf list (a permutation of the nodes) orders the tasks
   const uint32_t* order = ix->leaf0_perm ? ix->leaf.p : nullptr;
-  // precision form: nodes of at most 64 rows go to the whole-node kernel
-  const uint32_t node_rows = ix->b_precision > 0 ? 64u : 0u;
+  // nodes of at most 64 (precision form) or kNodeRows (FP32) rows go to the
+  // whole-node kernels, the others to the block tasks
+  const uint32_t node_rows = ix->b_precision > 0 ? 64u : uint32_t(kNodeRows);
   k_join_count<<<grid_for(n), 256, 0, st>>>(n, order, bs.new_off.p, bs.old_off.p, node_rows, tcnt.p, ctr.p + 2);
   if (int rc = device_exclusive_scan(tcnt.p, int(n), tskoff.p, scal.p + 2, sums, st)) return rc;
   unsigned long long bound = 0, rows = 0;
@@ -2059,7 +2245,10 @@ int gpu_refine_iter(efg_index* ix) {
   // two candidates per pair at most, plus one partly used run per workgroup
   // (per wavefront on the MFMA path, ncu * 8 workgroups of four, two kernels)
   const size_t cap16 = size_t(2 * bound) + 2 * size_t(ncu) * 8 * (kBlock / 64) * kStageRun + 1024;
-  const size_t cap = ix->b_precision > 0 ? cap16 : size_t(2 * bound) + join_grid * kStageRun + 1024;
+  const size_t node_grid = size_t(ncu) * 8;  // whole-node FP32 kernel: workgroups of four wavefronts
+  const size_t cap = ix->b_precision > 0
+                         ? cap16
+                         : size_t(2 * bound) + (join_grid + node_grid * (kBlock / 64)) * kStageRun + 1024;
   if (!bs.jhdr.ensure(size_t(ntask) + 1) || !bs.jtids.ensure((size_t(ntask) + 1) * 64) ||
       !bs.jstage.ensure(cap * sizeof(Stage)))
     return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
@@ -2088,11 +2277,15 @@ int gpu_refine_iter(efg_index* ix) {
       (f16 ? k_join_mfma<true> : k_join_mfma<false>)<<<g16, kBlock, 0, st>>>(
           ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, uint32_t(ntask), bs.jhdr.p, bs.jtids.p,
           worst.p, stage, (unsigned long long)cap16, ctr.p, tcount.p, ctr.p + 1);
-  } else if (ntask) {
-    k_join_tiles<4><<<unsigned(join_grid), kBlock, lds_join, st>>>(ix->data.p, D, uint32_t(ntask), bs.jhdr.p,
-                                                                   bs.jtids.p, worst.p, stage,
-                                                                   (unsigned long long)cap, ctr.p, tcount.p,
-                                                                   ctr.p + 1);
+  } else {
+    k_join_node_f32<<<unsigned(node_grid), kBlock, 0, st>>>(ix->data.p, D, n, order, bs.new_off.p, bs.new_ids.p,
+                                                            bs.old_off.p, bs.old_ids.p, worst.p, stage,
+                                                            (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
+    if (ntask)
+      k_join_tiles<4><<<unsigned(join_grid), kBlock, lds_join, st>>>(ix->data.p, D, uint32_t(ntask), bs.jhdr.p,
+                                                                     bs.jtids.p, worst.p, stage,
+                                                                     (unsigned long long)cap, ctr.p, tcount.p,
+                                                                     ctr.p + 1);
   }
   hipMemcpyAsync(h_ctr, ctr.p, 16, hipMemcpyDeviceToHost, st);
   if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
@@ -2140,11 +2333,13 @@ int gpu_refine_iter(efg_index* ix) {
   double t1 = now_ms();
   // ---- update(100)
   bs.iter += 1;
-  DevBuf<uint32_t> fnew, fold, rcn, rco, rcurn, rcuro, rszn, rszo, ridn, rido;
-  DevBuf<int32_t> roffn, roffo, snew, sold, onew, oold;
-  if (!fnew.alloc(n) || !fold.alloc(n) || !rcn.alloc(n) || !rco.alloc(n) || !rcurn.alloc(n) || !rcuro.alloc(n) ||
-      !rszn.alloc(n) || !rszo.alloc(n) || !roffn.alloc(n + 1) || !roffo.alloc(n + 1) || !snew.alloc(n + 1) ||
-      !sold.alloc(n + 1) || !onew.alloc(n + 1) || !oold.alloc(n + 1))
+  DevBuf<uint32_t>&fnew = sc.fnew, &fold = sc.fold, &rcn = sc.rcn, &rco = sc.rco, &rcurn = sc.rcurn,
+                   &rcuro = sc.rcuro, &rszn = sc.rszn, &rszo = sc.rszo, &ridn = sc.ridn, &rido = sc.rido;
+  DevBuf<int32_t>&roffn = sc.roffn, &roffo = sc.roffo, &snew = sc.snew, &sold = sc.sold, &onew = sc.onew,
+                  &oold = sc.oold;
+  if (!fnew.ensure(n) || !fold.ensure(n) || !rcn.ensure(n) || !rco.ensure(n) || !rcurn.ensure(n) ||
+      !rcuro.ensure(n) || !rszn.ensure(n) || !rszo.ensure(n) || !roffn.ensure(n + 1) || !roffo.ensure(n + 1) ||
+      !snew.ensure(n + 1) || !sold.ensure(n + 1) || !onew.ensure(n + 1) || !oold.ensure(n + 1))
     return fail(EFG_ERR_MEMORY, "out of device memory (update)");
   k_range<<<grid_for(n), 256, 0, st>>>(bs.pool.p, bs.flag.p, bs.fill.p, bs.found.p, n, L, S, bs.range.p, bs.radiusM.p);
   hipMemsetAsync(rcn.p, 0, size_t(n) * 4, st);
@@ -2156,7 +2351,7 @@ int gpu_refine_iter(efg_index* ix) {
   int rtot[2] = {0, 0};
   hipMemcpyAsync(rtot, scal.p, 8, hipMemcpyDeviceToHost, st);
   if (int rc = check_hip(hipStreamSynchronize(st), "update")) return rc;
-  if (!ridn.alloc(size_t(rtot[0]) + 1) || !rido.alloc(size_t(rtot[1]) + 1))
+  if (!ridn.ensure(size_t(rtot[0]) + 1) || !rido.ensure(size_t(rtot[1]) + 1))
     return fail(EFG_ERR_MEMORY, "out of device memory (reverse lists)");
   hipMemsetAsync(rcurn.p, 0, size_t(n) * 4, st);
   hipMemsetAsync(rcuro.p, 0, size_t(n) * 4, st);
@@ -2214,6 +2409,7 @@ int gpu_finalize_graph(efg_index* ix) {
   bs.jstage.reset();
   bs.jbkey.reset();
   bs.jbsrc.reset();
+  bs.it.release();
   ix->nnz = size_t(n) * K;
   ix->max_deg = K;
   ix->reg_deg = K;
