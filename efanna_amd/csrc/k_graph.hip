@@ -1717,11 +1717,12 @@ __device__ inline uint32_t cand_tag(uint32_t s) {
 
 // Per target: pool' = best L of (pool + candidates) by (key, tag), one
 // wavefront per target, in rounds: the running list (at most L entries,
-// sorted, unique) plus the next
-// kCap - L candidates are sorted by (key, tag) in the wavefront's LDS window,
-// duplicates dropped, and the first L kept.  An entry beyond the L-th unique
-// key of a subset is beyond it in the union, so rounds give the one-shot
-// result.
+// sorted, unique) plus the next candidates at or below its worst key (every
+// candidate while it holds fewer than L), up to kCap entries, are sorted by
+// (key, tag) in the wavefront's LDS window, duplicates dropped, and the
+// first L kept.  An entry beyond the L-th unique key of a subset is beyond it
+// in the union, so rounds give the one-shot result, and a candidate above a
+// full list's worst key can be skipped.
 //
 // kById (precision form of the build): a pair's approximate distance may
 // differ from the FP32 distance its id already carries in the pool (from the
@@ -1752,13 +1753,35 @@ __global__ void __launch_bounds__(kBlock) k_merge_wave(uint32_t n, int L, int CK
       tg[i] = flag[size_t(x) * L + i];
     }
     for (int base = c0; base < c1;) {
-      const int take = min(c1 - base, kCap - cur), m = cur + take, p2 = next_pow2(m < 2 ? 2 : m);
-      for (int i = cur + lane; i < p2; i += 64) {
-        const bool in = i < m;
-        k[i] = in ? bkey[base + i - cur] : kKeyMax;
-        tg[i] = in ? cand_tag(bsrc[base + i - cur]) : 0xffffffffu;
+      // the window: the pool, then the next candidates at or below the pool's
+      // worst key (all of them while the pool has room); a candidate above a
+      // full pool's worst key cannot be among the best L distinct keys
+      const uint64_t wk = cur >= L ? k[L - 1] : kKeyMax;
+      int m = cur;
+      while (base < c1 && m < kCap) {
+        const int i = base + lane;
+        const uint64_t v = i < c1 ? bkey[i] : kKeyMax;
+        const bool keep = i < c1 && v <= wk;
+        const uint64_t km = __ballot(keep);
+        const int room = kCap - m, rank = __popcll(km & lt), nk = __popcll(km);
+        if (keep && rank < room) {
+          k[m + rank] = v;
+          tg[m + rank] = cand_tag(bsrc[i]);
+        }
+        if (nk > room) {  // the kept candidate of rank `room` opens the next window
+          base += __ffsll((long long)__ballot(keep && rank == room)) - 1;
+          m = kCap;
+        } else {
+          base += 64;
+          m += nk;
+        }
       }
-      base += take;
+      if (m == cur) break;  // no candidate left that could enter the pool
+      const int p2 = next_pow2(m < 2 ? 2 : m);
+      for (int i = m + lane; i < p2; i += 64) {
+        k[i] = kKeyMax;
+        tg[i] = 0xffffffffu;
+      }
       wave_sync();
       for (int kk = 2; kk <= p2; kk <<= 1) {
         for (int j = kk >> 1; j > 0; j >>= 1) {
