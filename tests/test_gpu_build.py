@@ -189,3 +189,33 @@ def test_build_precision_form_pools_unique(cuda, precision):
         for p in range(0, base.shape[0], 7):
             ids = (kh[p, :nh[p]] & np.uint64(0xFFFFFFFF)).astype(np.int64)
             assert len(np.unique(ids)) == len(ids), (it, p)
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_build_precision_form_long_lists_equals_fp32_build(cuda, precision):
+    """Lists of more than 64 rows: the MFMA block tasks (beside the whole-node
+    MFMA kernel) give the FP32 build's pools and graph bit for bit on integer
+    rows."""
+    base, _ = helpers.mixture(n=20_000, nq=1)
+    kw = dict(trees=4, ml=6, epochs=4, check=40, L=80, K=40, S=20)
+    g, _o = _pair(base, **kw)
+    g.build_stage(0)
+    g.build_stage(1)
+    h, _o2 = _pair(base, **kw)
+    h.setBuildPrecision(precision)
+    h.build_stage(0)
+    h.build_stage(1)
+    longest = 0
+    for it in range(4):
+        g.build_stage(2)
+        h.build_stage(2)
+        on, _ = g.lists(0)
+        oo, _ = g.lists(1)
+        longest = max(longest, int(np.max(np.diff(on) + np.diff(oo))))
+        kg, fg, ng, _ = g.pools()
+        kh, fh, nh, _ = h.pools()
+        np.testing.assert_array_equal(nh, ng)
+        np.testing.assert_array_equal(kh, kg)
+        np.testing.assert_array_equal(fh, fg)
+    assert longest > 64
+    assert h.join_pairs() == g.join_pairs()
