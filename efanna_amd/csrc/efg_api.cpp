@@ -391,8 +391,74 @@ int efg_build_stage(efg_index* ix, int stage) {
   }
 }
 
+int efg_hash_set_codes(efg_index* ix, int tables, int codelen, int lenshift, int upbits, int radius,
+                       const uint64_t* base_codes, const uint64_t* query_codes, size_t nq) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  return hash_set_codes(ix, tables, codelen, lenshift, upbits, radius, base_codes, query_codes, nq);
+}
+
+// Code files as LoadCode32/64 read them (hashing_index.hpp:206-321): file
+// "<prefix>_<t>" for t = 1..tables holds int32 1, int32 code length, int32
+// count, then count codes of 4 bytes (code length <= 32) or 8 bytes.  The
+// code length given here picks the width, the file's length is used after.
+static int read_codes(const char* prefix, int tables, int width_bits, int* file_len, std::vector<uint64_t>& out,
+                      size_t* count) {
+  out.clear();
+  for (int t = 0; t < tables; ++t) {
+    const std::string path = std::string(prefix) + "_" + std::to_string(t + 1);
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return fail(EFG_ERR_IO, ("open file " + std::string(prefix) + " error").c_str());
+    int32_t hdr[3];
+    if (std::fread(hdr, 4, 3, f) != 3) {
+      std::fclose(f);
+      return fail(EFG_ERR_IO, ("Codefile  " + std::to_string(t) + " error!").c_str());
+    }
+    if (hdr[0] != 1 || hdr[2] < 0 || (t > 0 && size_t(hdr[2]) != *count)) {
+      std::fclose(f);
+      return fail(EFG_ERR_IO, ("Codefile  " + std::to_string(t) + " error!").c_str());
+    }
+    *file_len = hdr[1];
+    *count = size_t(hdr[2]);
+    const size_t m = size_t(hdr[2]);
+    const size_t w = width_bits <= 32 ? 4 : 8;
+    std::vector<unsigned char> raw(m * w);
+    const size_t got = m ? std::fread(raw.data(), w, m, f) : 0;
+    std::fclose(f);
+    if (got != m) return fail(EFG_ERR_IO, ("Codefile  " + std::to_string(t) + " error!").c_str());
+    for (size_t i = 0; i < m; ++i) {
+      uint64_t v = 0;
+      if (w == 4) {
+        uint32_t u;
+        std::memcpy(&u, &raw[i * 4], 4);
+        v = u;
+      } else {
+        std::memcpy(&v, &raw[i * 8], 8);
+      }
+      out.push_back(v);
+    }
+  }
+  return EFG_OK;
+}
+
+int efg_hash_load_codes(efg_index* ix, int tables, int codelen, int lenshift, int upbits, int radius,
+                        const char* base_prefix, const char* query_prefix) {
+  if (!ix || !base_prefix || !query_prefix) return fail(EFG_ERR_ARG, "null argument");
+  if (tables < 1) return fail(EFG_ERR_ARG, "Total hash table num error!");
+  if (codelen > 64 || codelen < 1) return fail(EFG_ERR_ARG, "code length not supported yet!");
+  std::vector<uint64_t> base, query;
+  int len_b = codelen, len_q = codelen;
+  size_t nb = 0, nq = 0;
+  if (int rc = read_codes(base_prefix, tables, codelen, &len_b, base, &nb)) return rc;
+  if (nb != ix->n) return fail(EFG_ERR_IO, "base code count differs from the data rows");
+  // the query file's width follows the base file's code length, as the
+  // reference reads it after the base file has set codelength
+  if (int rc = read_codes(query_prefix, tables, len_b, &len_q, query, &nq)) return rc;
+  return hash_set_codes(ix, tables, len_q, lenshift, upbits, radius, base.data(), query.data(), nq);
+}
+
 int efg_build_index(efg_index* ix) {
   if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (ix->hashing) return hash_build(ix);  // hashing index: the tables (hashing_index.hpp:874-889)
   const efg_build_params& p = ix->bp;
   if (p.K <= 0 || p.S <= 0 || p.L <= 0 || p.check <= 0 || p.trees <= 0 || p.epochs < 0 || p.merge_level < 0)
     return fail(EFG_ERR_ARG, "invalid build parameters");

@@ -172,6 +172,24 @@ struct efg_index {
   efg::DevBuf<efg::DNode> dnode;  // descent copy of snode
   efg::DevBuf<int32_t> sroots;
   bool snode_valid = false;
+  // hashing index (IEH seeds, hashing_index.hpp): the seed sequence of a
+  // query is the buckets of its code XOR each Hamming-ball mask, mask outer,
+  // table inner.  Each table's rows are grouped by code (codes ascending, ids
+  // ascending inside a bucket); bucket b of table t is a range [start, end)
+  // of that table's list, kept in an SNode so the search kernel reads it as
+  // it reads a leaf.
+  bool hashing = false;
+  int h_T = 0, h_bits = 0, h_radius = 0, h_nmask = 0;
+  size_t h_nq = 0;                      // queries with codes
+  std::vector<uint64_t> h_base;         // T x n codes, shifted (until the tables are built)
+  std::vector<uint64_t> h_query;        // T x h_nq codes, shifted
+  bool h_built = false;
+  efg::DevBuf<uint64_t> h_ucode;        // per table the distinct codes, ascending, tables concatenated
+  efg::DevBuf<uint32_t> h_uoff;         // T + 1: first bucket of each table
+  efg::DevBuf<efg::SNode> h_nodes;      // bucket ranges (start, end)
+  efg::DevBuf<uint32_t> h_lists;        // T x n rows grouped by bucket
+  efg::DevBuf<uint64_t> h_masks;        // h_nmask masks in the reference's order
+  efg::DevBuf<uint64_t> h_qcodes;       // T x h_nq query codes
   // graph (CSR)
   size_t nnz = 0;
   int max_deg = 0;
@@ -213,6 +231,13 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int check_hip(hipError_t e, const char* what);
 
+// k_hash.hip
+constexpr int kHashSlots = 4096;  // largest masks x tables per query
+int hash_set_codes(efg_index* ix, int tables, int codelen, int lenshift, int upbits, int radius, const uint64_t* base,
+                   const uint64_t* query, size_t nq);
+int hash_build(efg_index* ix);
+int hash_probe(efg_index* ix, int nleaf, size_t nq, int32_t* leaves, const int32_t* qlist, const int* nlist,
+               hipStream_t st);
 // k_search.hip
 int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int32_t* d_ids, float* d_dists,
                   uint64_t* d_ndist, hipStream_t st);

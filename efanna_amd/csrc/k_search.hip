@@ -833,25 +833,32 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       const bool valid = sq < total;
       const uint32_t id = id_next;
       if (sq + kThreads < total) id_next = seed_id(sq + kThreads);
-      if (kMethod == 0 && p + kThreads <= c.init_num) {
+      if (kMethod != 1 && p + kThreads <= c.init_num) {
         seed_fast(valid, id, par);
         par ^= 1;
       } else {
         seed_chunk(valid, id, uint32_t(sq));
       }
     }
-    for (uint32_t r0 = 0; p < c.init_num; r0 += kThreads) {
+    // the hash-only search tops up to K only (hashing_index.hpp:983-991)
+    // and measures the candidates it has: the ids a chunk adds past K are
+    // left out of the count
+    const int pseq = p;
+    for (uint32_t r0 = 0; p < (kMethod == 2 ? min(c.K, c.init_num) : c.init_num); r0 += kThreads) {
       uint32_t id = uint32_t(hash4(c.seed, ST_SEARCH, uint64_t(qi), r0 + tid) % uint64_t(c.n));
       seed_chunk(true, id, r0 + tid);
     }
-    int inserted = c.init_num;
+    const int ninit = kMethod == 2 ? max(pseq, min(c.K, c.init_num)) : c.init_num;
+    int inserted = ninit;
     uint64_t nexp_total = 0;
     const uint64_t seeds_read = uint64_t(seq_read);
-    distances(s.buf, 0, c.init_num);
+    distances(s.buf, 0, ninit);
+    if (kMethod == 2)  // slots past the candidates sort last
+      for (int i = ninit + tid; i < c.init_num; i += kThreads) s.buf[i] = kKeyMax;
     __syncthreads();
     // best R seeds: select the R-th key, gather the R keys, sort them
     // (method 1 without sorting keeps the seed order, kdtreeub_index.hpp:567-570)
-    if (kMethod == 0 || c.sorted) {
+    if (kMethod == 0 || kMethod == 2 || c.sorted) {
       const uint64_t thr = (R < c.init_num) ? block_select(s.buf, c.init_num, R, s.hist, s.misc) : kKeyMax;
       uint64_t mine[kKPT];
       int nm = 0;
@@ -873,7 +880,7 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       if (Rp <= kThreads) block_bitonic_sort_reg(P, Rp);
       else block_bitonic_sort(P, Rp);
     }
-    uint64_t ndq = uint64_t(c.init_num);
+    uint64_t ndq = uint64_t(ninit);
     bool ovf = false;
     bool stopped = false;
     if (kStop) stopped = checkpoint(qi, 0, 0, inserted);
@@ -1119,7 +1126,7 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       if (kStop == 0 && tid == 0) overflow[atomicAdd(noverflow, 1)] = qi;
     } else {
       for (int j = tid; j < c.K; j += kThreads) {
-        if (kMethod == 0) {
+        if (kMethod != 1) {
           uint64_t k = P[j];
           out_ids[size_t(qi) * c.out_stride + j] = int32_t(key_id(k));
           if (out_d) out_d[size_t(qi) * c.out_stride + j] = key_dist(k);
@@ -1323,20 +1330,27 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
   if (!table_in_lds && search_lds_bytes(c, true) > lds_cap) return fail(EFG_ERR_ARG, "pool too large for one workgroup");
   k_zero_i32<<<1, 64, 0, st>>>(w.nover.p, 2);
   if (ev) hipEventRecord(ev[0], st);
+  // seed slots: bucket probes on a hashing index, tree descent otherwise
+  const SNode* snp = ix->hashing ? ix->h_nodes.p : ix->snode.p;
+  const uint32_t* lists = ix->hashing ? ix->h_lists.p : ix->leaf.p;
   {
-    size_t threads = nq * size_t(c.T);
-    const unsigned grid = unsigned((threads + kDescThreads - 1) / kDescThreads);
-    const size_t qbytes = size_t((kDescThreads - 1) / c.T + 2) * size_t(c.d) * sizeof(float);
-    if (qbytes <= kDescLdsQ)
-      k_descend<true><<<grid, kDescThreads, qbytes, st>>>(dq, q_stride, int(nq), c.d, c.T, c.lsize, ix->sroots.p,
-                                                          ix->dnode.p, w.leaves.p, qlist, nlist_dev);
-    else
-      k_descend<false><<<grid, kDescThreads, 0, st>>>(dq, q_stride, int(nq), c.d, c.T, c.lsize, ix->sroots.p,
-                                                      ix->dnode.p, w.leaves.p, qlist, nlist_dev);
-    if (!qlist) {  // query order by first leaf
+    if (ix->hashing) {
+      hash_probe(ix, c.nleaf, nq, w.leaves.p, qlist, nlist_dev, st);
+    } else {
+      size_t threads = nq * size_t(c.T);
+      const unsigned grid = unsigned((threads + kDescThreads - 1) / kDescThreads);
+      const size_t qbytes = size_t((kDescThreads - 1) / c.T + 2) * size_t(c.d) * sizeof(float);
+      if (qbytes <= kDescLdsQ)
+        k_descend<true><<<grid, kDescThreads, qbytes, st>>>(dq, q_stride, int(nq), c.d, c.T, c.lsize, ix->sroots.p,
+                                                            ix->dnode.p, w.leaves.p, qlist, nlist_dev);
+      else
+        k_descend<false><<<grid, kDescThreads, 0, st>>>(dq, q_stride, int(nq), c.d, c.T, c.lsize, ix->sroots.p,
+                                                        ix->dnode.p, w.leaves.p, qlist, nlist_dev);
+    }
+    if (!qlist) {  // query order by first leaf (bucket)
       int shift = std::max(0, ceil_log2(ix->n + 1) - 12);
       hipMemsetAsync(w.qhist.p, 0, size_t(nb_q + 1) * 4, st);
-      k_qkey<<<unsigned((nq + 255) / 256), 256, 0, st>>>(w.leaves.p, c.nleaf, int(nq), ix->snode.p, shift, w.qkey.p,
+      k_qkey<<<unsigned((nq + 255) / 256), 256, 0, st>>>(w.leaves.p, c.nleaf, int(nq), snp, shift, w.qkey.p,
                                                          w.qhist.p);
       if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(w.qhist.p), nb_q, w.qoff.p, w.qoff.p + nb_q,
                                          w.scan_sums, st))
@@ -1355,6 +1369,7 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
 #define EFG_K(G, M, B, S, T) &k_search_nnexp<G, M, B, S, T>
 #define EFG_KT(G, M, B, S) (tq == 64 ? EFG_K(G, M, B, S, 64) : tq == 128 ? EFG_K(G, M, B, S, 128) : EFG_K(G, M, B, S, 256))
     if (c.method == 1) return global ? EFG_K(true, 1, false, 0, 256) : EFG_K(false, 1, false, 0, 256);
+    if (c.method == 2) return global ? EFG_K(true, 2, false, 0, 256) : EFG_K(false, 2, false, 0, 256);
     if (c.stop == 1) return global ? nullptr : EFG_KT(false, 0, false, 1);
     if (c.stop == 2) return global ? nullptr : EFG_KT(false, 0, false, 2);
     if (bf16) return global ? EFG_K(true, 0, true, 0, 256) : EFG_K(false, 0, true, 0, 256);
@@ -1371,7 +1386,7 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
     auto kern = pick(false);
     hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     kern<<<unsigned(grid), unsigned(tq), lds, st>>>(c, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p,
-                                                    ix->snode.p, ix->leaf.p, ix->goff.p, ix->gids.p, d_ids, d_dists,
+                                                    snp, lists, ix->goff.p, ix->gids.p, d_ids, d_dists,
                                                     d_ndist, w.ndist.p, order, int(nq), nlist_dev, qlist ? 0 : 1,
                                                     nullptr, w.overflow.p, w.nover.p, work);
   } else if (c.stop) {
@@ -1400,13 +1415,13 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
     auto kern2 = pick(true);
     hipFuncSetAttribute(reinterpret_cast<const void*>(kern2), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2));
     if (table_in_lds) {
-      kern2<<<grid2, kBlock, lds2, st>>>(c2, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, ix->snode.p,
-                                         ix->leaf.p, ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
+      kern2<<<grid2, kBlock, lds2, st>>>(c2, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, snp,
+                                         lists, ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
                                          w.overflow.p, 0, w.nover.p, 0, w.gtab.p, w.overflow.p + nq, w.nover.p + 1,
                                          work);
     } else {
-      kern2<<<grid2, kBlock, lds2, st>>>(c2, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, ix->snode.p,
-                                         ix->leaf.p, ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
+      kern2<<<grid2, kBlock, lds2, st>>>(c2, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, snp,
+                                         lists, ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
                                          order, int(nq), nlist_dev, 0, w.gtab.p, w.overflow.p + nq, w.nover.p + 1,
                                          work);
     }
@@ -1435,7 +1450,7 @@ void probe_params(const efg_index* ix, int k, int* pool, int* ext, int* ep) {
 // (a wavefront for pools up to 256, two up to 1024, else four); the seed
 // keys held in registers need init_num <= 16 * threads.
 int choose_threads(const efg_index* ix, const SearchCfg& c) {
-  if (c.method == 1) return 256;
+  if (c.method != 0) return 256;  // kgraph and the hash-only search run 256 threads per query
   int t = 0;
   if (ix->s_threads) {
     t = ix->s_threads;
@@ -1451,13 +1466,18 @@ int choose_threads(const efg_index* ix, const SearchCfg& c) {
 int search_prepare(efg_index* ix, size_t nq, size_t q_stride, hipStream_t st) {
   if (!ix->s_set) return fail(EFG_ERR_STATE, "search parameters not set (setSearchParams)");
   if (ix->s_method != 0 && ix->s_method != 1) return fail(EFG_ERR_METHOD, "no such searching method");
-  if (ix->T == 0) return fail(EFG_ERR_STATE, "no trees: build or load trees first");
-  if (ix->nnz == 0) return fail(EFG_ERR_STATE, "no graph: build or load a graph first");
-  if (ix->s_trees <= 0) return fail(EFG_ERR_ARG, "search tree number must be positive");
-  if (ix->s_trees > ix->T) return fail(EFG_ERR_TREES, "wrong tree number");
+  if (ix->hashing) {  // hash tables instead of trees; the graph is optional
+    if (!ix->h_built) return fail(EFG_ERR_STATE, "no hash tables: build the index first");
+    if (nq > ix->h_nq) return fail(EFG_ERR_ARG, "more queries than query codes");
+  } else {
+    if (ix->T == 0) return fail(EFG_ERR_STATE, "no trees: build or load trees first");
+    if (ix->nnz == 0) return fail(EFG_ERR_STATE, "no graph: build or load a graph first");
+    if (ix->s_trees <= 0) return fail(EFG_ERR_ARG, "search tree number must be positive");
+    if (ix->s_trees > ix->T) return fail(EFG_ERR_TREES, "wrong tree number");
+  }
   if (nq > (size_t(1) << 31)) return fail(EFG_ERR_ARG, "too many queries");
   if (q_stride < ix->d) return fail(EFG_ERR_ARG, "query stride below the dimension");
-  if (!ix->snode_valid) {
+  if (!ix->hashing && !ix->snode_valid) {
     if (int rc = check_hip(hipStreamSynchronize(st), "sync")) return rc;
     if (int rc = build_search_layout(ix)) return rc;
   }
@@ -1477,11 +1497,19 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
                   uint64_t* d_ndist, hipStream_t st) {
   if (int rc = search_prepare(ix, nq, q_stride, st)) return rc;
   if (nq == 0) return EFG_OK;
-  const int method = ix->s_method;
+  // a hashing index numbers the methods the other way round (0 = kgraph,
+  // 1 = nnexp, hashing_index.hpp:902-924) and without a graph runs the
+  // hash-only search (method 2 here)
+  const int method = !ix->hashing ? ix->s_method : (ix->nnz == 0 ? 2 : 1 - ix->s_method);
   SearchCfg c;
   int init_eff = 0;
-  if (int rc = make_cfg(ix, k, ix->s_init, ix->s_extend, ix->s_epochs, ix->s_trees, method, 0, &c, &init_eff))
+  if (int rc = make_cfg(ix, k, ix->s_init, method == 2 ? 0 : ix->s_extend, method == 2 ? 0 : ix->s_epochs,
+                        ix->hashing ? ix->h_T : ix->s_trees, method, 0, &c, &init_eff))
     return rc;
+  if (ix->hashing) {  // one seed slot per (mask, table)
+    c.lsize = ix->h_nmask;
+    c.nleaf = c.T * c.lsize;
+  }
   if (method == 1) ix->s_init = init_eff;
   const bool bf16 = method == 0 && ix->s_precision == 1;
   set_threads(c, bf16 ? 256 : choose_threads(ix, c));
@@ -1490,7 +1518,7 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
     const size_t tot = ix->n * ix->D;
     k_pack_bf16<<<unsigned((tot + 255) / 256), 256, 0, st>>>(ix->data.p, int(ix->D), ix->n, ix->data_bf16.p);
   }
-  const bool learned = method == 0 && ix->s_mode == 1;
+  const bool learned = method == 0 && ix->s_mode == 1 && !ix->hashing;
   if (learned && !ix->net_set) return fail(EFG_ERR_STATE, "no stopping network set");
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   const bool prof = ix->profiling && ix->prof_events.size() < 4 * 4096;
@@ -1532,6 +1560,7 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
 
 int search_autotune(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int* chosen,
                     hipStream_t st) {
+  if (ix->hashing) return fail(EFG_ERR_STATE, "the tuner runs on kd-tree indexes");
   if (int rc = search_prepare(ix, nq, q_stride, st)) return rc;
   if (nq == 0) return fail(EFG_ERR_ARG, "no queries to tune on");
   if (ix->s_method != 0) {
@@ -1648,6 +1677,7 @@ struct BatchSetup {
 int batch_setup(efg_index* ix, efg::Workspace& w, const float* dq, size_t nq, size_t q_stride, const int* pool,
                 const int* ext, const int* kq, int out_stride, int32_t* d_ids, float* d_dists, bool tune,
                 hipStream_t st, BatchSetup* b) {
+  if (ix->hashing) return fail(EFG_ERR_STATE, "batched search runs on kd-tree indexes");
   if (ix->s_method != 0) return fail(EFG_ERR_METHOD, "batched search runs method 0");
   if (out_stride <= 0) return fail(EFG_ERR_ARG, "invalid output stride");
   std::vector<BatchClass> cls;
@@ -1798,6 +1828,7 @@ void batch_plan_destroy(efg_batch_plan* pl) {
 
 int search_traces(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, const int32_t* d_plain,
                   float* d_feat, uint8_t* d_label, hipStream_t st) {
+  if (ix->hashing) return fail(EFG_ERR_STATE, "learned stopping runs on kd-tree indexes");
   if (int rc = search_prepare(ix, nq, q_stride, st)) return rc;
   if (nq == 0) return EFG_OK;
   int pp, pe, pep;

@@ -80,3 +80,23 @@ def workload(name: str, n: int | None = None, nq: int | None = None):
     if name == "mixture1m_d960":
         return gaussian_mixture(n or 1_000_000, 960, nq or 10_000, centres=500, sigma=0.05, hi=1.0, rounded=False)
     raise KeyError(name)
+
+
+def lsh_codes(base, queries, tables: int, bits: int, seed: int = 0):
+    """Hash codes for a hashing index: per table `bits` random hyperplanes
+    through the base mean, bit b set when the row lies on the positive side
+    of hyperplane b.  Returns (tables x n, tables x nq) uint64 codes, the
+    layout of the reference's code files (one file per table)."""
+    base = np.asarray(base, np.float32)
+    queries = np.asarray(queries, np.float32)
+    mean = base.mean(axis=0, dtype=np.float64).astype(np.float32)
+    rng = np.random.default_rng(seed)
+    out = [np.zeros((tables, x.shape[0]), np.uint64) for x in (base, queries)]
+    for t in range(tables):
+        r = rng.standard_normal((base.shape[1], bits)).astype(np.float32)
+        for codes, x in zip(out, (base, queries)):
+            for c0 in range(0, x.shape[0], 65536):
+                s = (x[c0:c0 + 65536] - mean) @ r > 0
+                w = (s.astype(np.uint64) << np.arange(bits, dtype=np.uint64)).sum(axis=1, dtype=np.uint64)
+                codes[t, c0:c0 + 65536] = w
+    return out[0], out[1]

@@ -177,6 +177,23 @@ class FIndex:
         check(lib().efg_set_search_params(self.h, epochs, init_num, extend_to, search_trees, search_lv,
                                           search_method))
 
+    def setHashCodes(self, base_codes, query_codes, codelen, upbits, radius, lenshift=0):
+        """Make this a hashing index (IEH seeds, hashing_index.hpp): tables x n
+        base codes and tables x nq query codes (unshifted), as the reference's
+        code files hold them; buildIndex() then builds the hash tables."""
+        b = np.ascontiguousarray(base_codes, np.uint64)
+        q = np.ascontiguousarray(query_codes, np.uint64)
+        if b.ndim != 2 or q.ndim != 2 or b.shape[0] != q.shape[0]:
+            raise ValueError("codes must be (tables, n) and (tables, nq)")
+        check(lib().efg_hash_set_codes(self.h, b.shape[0], int(codelen), int(lenshift), int(upbits), int(radius),
+                                       _p(b, _lib.c_u64p), _p(q, _lib.c_u64p), q.shape[1]))
+
+    def loadHashCodes(self, tables, codelen, upbits, radius, base_prefix, query_prefix, lenshift=0):
+        """HASHINGIndexParams(codelen, tables, upbits, radius, base, query, lenshift):
+        code files "<prefix>_<t>", t = 1..tables."""
+        check(lib().efg_hash_load_codes(self.h, int(tables), int(codelen), int(lenshift), int(upbits), int(radius),
+                                        str(base_prefix).encode(), str(query_prefix).encode()))
+
     def setBuildPrecision(self, precision: int):
         """0 = FP32 tiles; 1 = BF16 / 2 = FP16 MFMA merge and join tiles, FP32 re-rank of sets and pools."""
         check(lib().efg_set_build_precision(self.h, int(precision)))

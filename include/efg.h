@@ -99,6 +99,33 @@ int efg_get_results(const efg_index* ix, size_t* nq, int* k, int32_t* ids, float
 int efg_knn_search_device(efg_index* ix, int k, const float* d_queries, size_t nq, size_t q_stride, int32_t* d_ids,
                           float* d_dists, uint64_t* d_ndist, void* stream);
 
+/* Hashing index (IEH seeds): replaces HASHINGIndex's constructor code loading
+ * (algorithm/hashing_index.hpp:66-197, LoadCode32/64 206-321).  Turns the index
+ * into a hashing index: `tables` hash tables of codes of `codelen` bits,
+ * shifted right by `lenshift` (at most 2^(codelen - lenshift) - 1 each),
+ * `upbits` < codelen - lenshift (checked as the reference checks it), Hamming
+ * radius `radius` (clamped as the reference clamps it; masks x tables must
+ * stay at or below 4096).  base_codes: tables x n codes, table-major;
+ * query_codes: tables x nq, query i of a search uses codes column i.  After
+ * efg_build_index (the tables, buildIndexImpl 874-889) efg_knn_search
+ * seeds every query from the buckets of its code XOR each Hamming-ball mask,
+ * mask outer, table inner (getNeighborsIEH32/64_*, 1087-1630):
+ *   no graph loaded           -> the K nearest of the first init_num
+ *                                candidates (getNeighbors32/64, 929-1085)
+ *   graph, search_method == 1 -> nnexp expansion (getNeighborsIEH*_nnexp)
+ *   graph, search_method == 0 -> kgraph expansion (getNeighborsIEH*_kgraph)
+ * search_trees is not used.  Random seeds beyond the buckets come from the
+ * counter-based generator (the reference calls rand()). */
+int efg_hash_set_codes(efg_index* ix, int tables, int codelen, int lenshift, int upbits, int radius,
+                       const uint64_t* base_codes, const uint64_t* query_codes, size_t nq);
+/* The same from the reference's code files "<prefix>_<t>", t = 1..tables:
+ * int32 1, int32 code length, int32 count, then count codes of 4 bytes
+ * (codelen <= 32) or 8 bytes (HASHINGIndexParams(codelen, TableNum,
+ * UpperBits, HashRadius, BaseCodeFile, QueryCodeFile, codelenShift),
+ * hashing_index.hpp:19-46). */
+int efg_hash_load_codes(efg_index* ix, int tables, int codelen, int lenshift, int upbits, int radius,
+                        const char* base_prefix, const char* query_prefix);
+
 /* Search precision (no counterpart in the reference; a form beside the plain
  * FP32 search): 0 = FP32 rows (default, bit-exact with the reference's
  * arithmetic); 1 = BF16 copy of the rows for the pool and expansion

@@ -73,6 +73,13 @@ def lib():
             "orc_mlp_grad64": (ctypes.c_double, [_f32p, _f32p, _f32p, _u8p, _sz, ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_double)]),
             "orc_mlp_param_count": (ctypes.c_int, []),
+            "orc_hash_set": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u64p, _u64p,
+                                            _sz]),
+            "orc_hash_masks": (_sz, [ctypes.c_void_p, _u64p, _sz]),
+            "orc_hash_seeds": (_sz, [ctypes.c_void_p, _sz, ctypes.c_int, ctypes.c_int, _u32p]),
+            "orc_hash_search": (ctypes.c_int, [ctypes.c_void_p, _f32p, _sz, _sz, ctypes.c_int, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, _i32p,
+                                               _f32p, _u64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -300,6 +307,38 @@ class Index:
         init = ctypes.c_int(int(init_num))
         _check(lib().orc_search(self.h, _p(q, _f32p), nq, q.shape[1], k, epochs, ctypes.byref(init), extend_to,
                                 search_trees, method, _p(ids, _i32p), _p(dists, _f32p), _p(nd, _u64p)))
+        self.init_num_state = init.value
+        return ids, dists, nd
+
+    # ---- hashing index (IEH seeds) ---------------------------------------
+    def set_hash(self, base_codes, query_codes, bits, radius):
+        """Tables from T x n base codes and T x nq query codes (already
+        shifted), code length `bits`, Hamming radius `radius`."""
+        b = np.ascontiguousarray(base_codes, np.uint64)
+        qc = np.ascontiguousarray(query_codes, np.uint64)
+        T = b.shape[0]
+        _check(lib().orc_hash_set(self.h, T, int(bits), int(radius), _p(b, _u64p), _p(qc, _u64p), qc.shape[1]))
+
+    def hash_masks(self):
+        m = lib().orc_hash_masks(self.h, None, 0)
+        out = np.empty(m, np.uint64)
+        lib().orc_hash_masks(self.h, _p(out, _u64p), m)
+        return out
+
+    def hash_seeds(self, qi, init_num, topup_to=-1):
+        out = np.empty(max(init_num, topup_to, 1), np.uint32)
+        m = lib().orc_hash_seeds(self.h, int(qi), int(init_num), int(topup_to), _p(out, _u32p))
+        return out[:m]
+
+    def hash_search(self, queries, k, epochs, init_num, extend_to, method=1):
+        q = np.ascontiguousarray(queries, np.float32)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.int32)
+        dists = np.empty((nq, k), np.float32)
+        nd = np.empty(nq, np.uint64)
+        init = ctypes.c_int(int(init_num))
+        _check(lib().orc_hash_search(self.h, _p(q, _f32p), nq, q.shape[1], k, epochs, ctypes.byref(init),
+                                     extend_to, method, _p(ids, _i32p), _p(dists, _f32p), _p(nd, _u64p)))
         self.init_num_state = init.value
         return ids, dists, nd
 

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 #include <omp.h>
@@ -161,7 +162,22 @@ struct Params {
   int join_mode = 0;  // 0 = batched (the HIP library's semantics), 1 = sequential (reference order)
 };
 
+// Hashing index (IEH seeds, hashing_index.hpp): per table the members of
+// each code in id order (BuildHashTable32/64, 323-347 / 629-655: points are
+// appended in id order), the Hamming-ball masks in radius order
+// (generateMask32/64, 349-627 / 657-872: radius r = the r-bit combinations of
+// the code's bits in lexicographic order, i < j < k ...), the query codes.
+struct HashState {
+  int T = 0, bits = 0, radius = 0;
+  std::vector<std::unordered_map<uint64_t, std::vector<uint32_t>>> tab;
+  std::vector<uint64_t> masks;
+  std::vector<uint64_t> qcodes;  // T x nq
+  size_t nq = 0;
+};
+
 struct Index {
+  HashState hs;
+  bool hashing = false;
   size_t n = 0, d = 0, stride = 0;
   std::vector<float> data;  // n x stride, zero padded (samples/*.cc load_data)
   Params p;
@@ -706,25 +722,60 @@ struct SearchArgs {
 };
 
 // Seed pool common to both methods (kdtreeub_index.hpp:387-417 / 521-550).
+// On a hashing index the seeds come from the buckets of the query's code
+// XOR each mask, mask outer, table inner (hashing_index.hpp:1108-1130).  The
+// random top-up runs to topup_to ids (init_num; K for the hash-only search,
+// 1005-1014).
 void seed_pool(const Index& ix, const float* q, unsigned lsize, int init_num, int trees, uint64_t qid,
-               std::vector<uint8_t>& visited, std::vector<uint32_t>& pool) {
-  std::vector<std::vector<int32_t>> cand(trees);
-  for (int i = 0; i < trees; ++i) search_node_list(ix, ix.roots[i], q, lsize, cand[i]);
+               std::vector<uint8_t>& visited, std::vector<uint32_t>& pool, int topup_to = -1) {
   unsigned p = 0;
   const unsigned want = unsigned(init_num);
   pool.assign(want, 0);
-  for (unsigned ni = 0; ni < lsize && p < want; ++ni) {
-    for (int i = 0; i < trees && p < want; ++i) {
-      if (ni >= cand[i].size()) continue;
-      const Node& lf = ix.nodes[cand[i][ni]];
-      const uint32_t* ll = &ix.leaf[size_t(i) * ix.n];
-      for (uint32_t j = lf.start; j < lf.end && p < want; ++j) {
-        uint32_t nn = ll[j];
-        if (visited[nn]) continue;
-        visited[nn] = 1;
-        pool[p++] = nn;
+  if (ix.hashing) {
+    const HashState& hs = ix.hs;
+    for (size_t j = 0; j < hs.masks.size() && p < want; ++j) {
+      for (int h = 0; h < hs.T && p < want; ++h) {
+        const uint64_t code = hs.qcodes[size_t(h) * hs.nq + qid] ^ hs.masks[j];
+        auto it = hs.tab[h].find(code);
+        if (it == hs.tab[h].end()) continue;
+        for (size_t k = 0; k < it->second.size() && p < want; ++k) {
+          const uint32_t nn = it->second[k];
+          if (visited[nn]) continue;
+          visited[nn] = 1;
+          pool[p++] = nn;
+        }
       }
     }
+  } else {
+    std::vector<std::vector<int32_t>> cand(trees);
+    for (int i = 0; i < trees; ++i) search_node_list(ix, ix.roots[i], q, lsize, cand[i]);
+    for (unsigned ni = 0; ni < lsize && p < want; ++ni) {
+      for (int i = 0; i < trees && p < want; ++i) {
+        if (ni >= cand[i].size()) continue;
+        const Node& lf = ix.nodes[cand[i][ni]];
+        const uint32_t* ll = &ix.leaf[size_t(i) * ix.n];
+        for (uint32_t j = lf.start; j < lf.end && p < want; ++j) {
+          uint32_t nn = ll[j];
+          if (visited[nn]) continue;
+          visited[nn] = 1;
+          pool[p++] = nn;
+        }
+      }
+    }
+  }
+  if (topup_to >= 0 && p < unsigned(topup_to)) {
+    // hash-only search: top up to K and keep the p candidates
+    uint64_t r = 0;
+    while (p < unsigned(topup_to)) {
+      uint32_t nn = uint32_t(hash4(ix.p.seed, ST_SEARCH, qid, r++) % ix.n);
+      if (visited[nn]) continue;
+      visited[nn] = 1;
+      pool[p++] = nn;
+    }
+  }
+  if (topup_to >= 0) {
+    pool.resize(p);
+    return;
   }
   uint64_t r = 0;
   while (p < want) {
@@ -977,6 +1028,45 @@ double net_loss_grad64(const float* w, const float* norm, const float* feat, con
     }
   }
   return loss / double(n);
+}
+
+// Hash-only search (hashing_index.hpp:929-1005 getNeighbors32, 1007-1085
+// getNeighbors64; no graph loaded): the first init_num distinct bucket
+// members, a random top-up only below K, and the K nearest of them.
+void search_hash_only(const Index& ix, const float* q, uint64_t qid, const SearchArgs& a, int32_t* out_ids,
+                      float* out_d, uint64_t& ndist) {
+  std::vector<uint8_t> visited(ix.n, 0);
+  std::vector<uint32_t> pool;
+  seed_pool(ix, q, 0, a.init_num, 0, qid, visited, pool, a.K);
+  std::vector<std::pair<float, uint32_t>> res;
+  for (uint32_t id : pool) res.push_back({l2_avx(q, ix.row(id), ix.d), id});
+  ndist += pool.size();
+  std::partial_sort(res.begin(), res.begin() + a.K, res.end());
+  for (int i = 0; i < a.K; ++i) {
+    out_ids[i] = int32_t(res[i].second);
+    if (out_d) out_d[i] = res[i].first;
+  }
+}
+
+// Hamming-ball masks (generateMask32/64): 0, then for r = 1..radius the
+// r-bit combinations of bits 0..bits-1 in lexicographic order of (i < j <
+// ...), as the reference's nested loops emit them.
+void gen_masks(int bits, int radius, std::vector<uint64_t>& out) {
+  out.assign(1, 0ull);
+  for (int r = 1; r <= radius && r <= bits; ++r) {
+    std::vector<int> idx(r);
+    for (int i = 0; i < r; ++i) idx[i] = i;
+    for (;;) {
+      uint64_t m = 0;
+      for (int i : idx) m |= 1ull << i;
+      out.push_back(m);
+      int i = r - 1;
+      while (i >= 0 && idx[i] == bits - r + i) --i;
+      if (i < 0) break;
+      ++idx[i];
+      for (int j = i + 1; j < r; ++j) idx[j] = idx[j - 1] + 1;
+    }
+  }
 }
 
 // Search, method 1: getNeighbors_kgraph (kdtreeub_index.hpp:496-654).  The
@@ -1309,6 +1399,97 @@ double orc_mlp_grad64(const float* params, const float* norm, const float* feat,
   return net_loss_grad64(w.data(), norm, feat, label, n, grad);
 }
 int orc_mlp_param_count() { return kNP; }
+
+// Hashing index: T tables of base codes (T x n) and query codes (T x nq),
+// already shifted right by the code-length shift; bits = the code length
+// used (codelength - lenshift), radius = the Hamming radius.
+int orc_hash_set(void* h, int T, int bits, int radius, const uint64_t* base_codes, const uint64_t* query_codes,
+                 size_t nq) {
+  Index* ix = static_cast<Index*>(h);
+  if (T <= 0 || bits <= 0 || bits > 64 || radius < 0) { g_err = "invalid hashing parameters"; return -2; }
+  HashState& hs = ix->hs;
+  hs.T = T;
+  hs.bits = bits;
+  hs.radius = radius;
+  hs.nq = nq;
+  hs.tab.assign(T, {});
+  for (int t = 0; t < T; ++t)
+    for (size_t i = 0; i < ix->n; ++i) hs.tab[t][base_codes[size_t(t) * ix->n + i]].push_back(uint32_t(i));
+  hs.qcodes.assign(query_codes, query_codes + size_t(T) * nq);
+  gen_masks(bits, radius, hs.masks);
+  ix->hashing = true;
+  return 0;
+}
+size_t orc_hash_masks(void* h, uint64_t* out, size_t cap) {
+  const HashState& hs = static_cast<Index*>(h)->hs;
+  for (size_t i = 0; i < hs.masks.size() && i < cap; ++i) out[i] = hs.masks[i];
+  return hs.masks.size();
+}
+// The seed ids of query qi before the distances (the bucket sequence's
+// first init_num distinct members, then the random top-up to topup_to ids
+// when topup_to >= 0, else to init_num).
+size_t orc_hash_seeds(void* h, size_t qi, int init_num, int topup_to, uint32_t* out) {
+  Index* ix = static_cast<Index*>(h);
+  std::vector<uint8_t> visited(ix->n, 0);
+  std::vector<uint32_t> pool;
+  seed_pool(*ix, nullptr, 0, init_num, 0, uint64_t(qi), visited, pool, topup_to);
+  std::memcpy(out, pool.data(), pool.size() * 4);
+  return pool.size();
+}
+
+// knnSearch on a hashing index (hashing_index.hpp:891-927): without a graph
+// the hash-only search; with one, method 0 = kgraph and 1 = nnexp (the
+// hashing index numbers them the other way round from the kd-tree index).
+int orc_hash_search(void* h, const float* q, size_t nq, size_t qstride, int k, int epochs, int* init_num_io,
+                    int extend_to, int method, int32_t* ids, float* dists, uint64_t* ndist) {
+  Index* ix = static_cast<Index*>(h);
+  if (!ix->hashing) { g_err = "no hash tables"; return -3; }
+  if (nq > ix->hs.nq) { g_err = "more queries than query codes"; return -2; }
+  int init_num = *init_num_io;
+  if (extend_to > init_num) extend_to = init_num;
+  if (init_num > int(ix->n) || k > std::max(extend_to, init_num)) { g_err = "pool larger than the data set"; return -2; }
+  SearchArgs a{epochs, init_num, extend_to, ix->hs.T, k};
+  std::vector<float> qpad;
+  size_t qs = (ix->d + 7) & ~size_t(7);
+  qpad.assign(nq * qs, 0.f);
+  for (size_t i = 0; i < nq; ++i) std::memcpy(&qpad[i * qs], q + i * qstride, ix->d * sizeof(float));
+  const bool graph = ix->goff.size() == ix->n + 1;
+  if (!graph) {
+#pragma omp parallel for schedule(dynamic, 8)
+    for (long long qi = 0; qi < (long long)nq; ++qi) {
+      uint64_t nd = 0;
+      search_hash_only(*ix, &qpad[size_t(qi) * qs], uint64_t(qi), a, ids + size_t(qi) * k,
+                       dists ? dists + size_t(qi) * k : nullptr, nd);
+      if (ndist) ndist[qi] = nd;
+    }
+  } else if (method == 1) {
+#pragma omp parallel for schedule(dynamic, 8)
+    for (long long qi = 0; qi < (long long)nq; ++qi) {
+      uint64_t nd = 0;
+      search_nnexp(*ix, &qpad[size_t(qi) * qs], uint64_t(qi), a, ids + size_t(qi) * k,
+                   dists ? dists + size_t(qi) * k : nullptr, nd);
+      if (ndist) ndist[qi] = nd;
+    }
+  } else if (method == 0) {
+    bool sorted = true;
+    unsigned pool_size = unsigned(epochs) * unsigned(extend_to);
+    if (pool_size >= unsigned(init_num)) { init_num = int(pool_size); sorted = false; }
+    if (init_num > int(ix->n)) { g_err = "pool larger than the data set"; return -2; }
+    *init_num_io = init_num;
+    a.init_num = init_num;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (long long qi = 0; qi < (long long)nq; ++qi) {
+      uint64_t nd = 0;
+      search_kgraph(*ix, &qpad[size_t(qi) * qs], uint64_t(qi), a, init_num, sorted, 0, ids + size_t(qi) * k,
+                    dists ? dists + size_t(qi) * k : nullptr, nd);
+      if (ndist) ndist[qi] = nd;
+    }
+  } else {
+    g_err = "no such searching method";
+    return -5;
+  }
+  return 0;
+}
 
 // knnSearch with setSearchParams semantics (base_index.hpp:153-161):
 // extend_to is clamped to init_num; method 0 = nnexp, 1 = kgraph (which
