@@ -15,6 +15,9 @@ measures the other forms on the same index:
                 16-64-64-1 network trained on the GPU from traces of 200,000
                 other queries of the mixture, full search when p < 0.9
   method1       getNeighbors_kgraph at the same parameters
+  hash_seeds    hashing index (IEH): seeds from 4 tables of 16-bit LSH codes,
+                Hamming radius 2, nnexp over the built graph (and the
+                hash-only search without a graph)
   batched       100,000 queries (70 % mixture, 30 % uniform), per-query pool
                 in {200, 400, 800, 1600} and k in {1, 10, 100}, one call
 `value` is the fastest form whose recall@10 is at least 0.95.  Ground truth:
@@ -220,6 +223,45 @@ def main() -> int:
         gix.setSearchParams(s_epochs, s_pool, s_extend, s_trees, -1, 0)
         forms["method1"] = {"ms_per_step": round(m1_ms, 4), "qps": round(nq / (m1_ms / 1e3), 1),
                             "recall_at_10": round(rec(res_m1), 5)}
+
+        # ---- hashing index (IEH seeds): 4 tables of 16-bit LSH codes, Hamming
+        # radius 2; the hash-only search, then nnexp over the same graph
+        t0 = time.perf_counter()
+        hbits, htab, hrad = 16, 4, 2
+        hbc, hqc = datasets.lsh_codes(base, q, htab, hbits, seed=3)
+        codes_s = time.perf_counter() - t0
+        hix = FIndex(base, L2DistanceAVX(), KDTreeUbIndexParams(True, 8, 8, 8))
+        hix.setHashCodes(hbc, hqc, hbits, upbits=8, radius=hrad)
+        del hbc, hqc
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hix.buildIndex()
+        hb_s = time.perf_counter() - t0
+
+        def htimed(steps):
+            for _ in range(args.warmup):
+                hix.knn_search_device(k, qd.data_ptr(), nq, d, ids.data_ptr(), dd.data_ptr(), None, stream.cuda_stream)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                hix.knn_search_device(k, qd.data_ptr(), nq, d, ids.data_ptr(), dd.data_ptr(), None, stream.cuda_stream)
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t1) * 1e3 / steps
+
+        hix.setSearchParams(0, s_pool, s_extend, 0, -1, 1)
+        ho_ms = htimed(max(1, args.steps // 4))
+        res_ho = ids.cpu().numpy()
+        hix.set_graph(*gix.graph())
+        hix.setSearchParams(s_epochs, s_pool, s_extend, 0, -1, 1)
+        hn_ms = htimed(max(1, args.steps // 4))
+        res_hn = ids.cpu().numpy()
+        forms["hash_seeds"] = {
+            "ms_per_step": round(hn_ms, 4), "qps": round(nq / (hn_ms / 1e3), 1), "recall_at_10": round(rec(res_hn), 5),
+            "codes": f"{htab} tables x {hbits}-bit LSH (random hyperplanes), Hamming radius {hrad}",
+            "hash_only": {"ms_per_step": round(ho_ms, 4), "qps": round(nq / (ho_ms / 1e3), 1),
+                          "recall_at_10": round(rec(res_ho), 5)},
+            "table_build_s": round(hb_s, 3), "codes_s": round(codes_s, 2)}
+        del hix
 
         # ---- heterogeneous batch
         nb = args.batch_queries

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../efg.h"
+#include "algorithm/hashing_index.hpp"
 #include "algorithm/kdtreeub_index.hpp"
 #include "general/distance.hpp"
 #include "general/matrix.hpp"
@@ -29,7 +30,6 @@ class FIndex {
  public:
   FIndex(const Matrix<DataType>& features, Distance<DataType>* d, const IndexParams& params) : params_(params) {
     (void)d;  // every L2 variant maps to the device's squared L2
-    if (params.init_index_type != KDTREE_UB) throw std::runtime_error("only the KDTREE_UB index is available");
     efg_build_params bp;
     int trees = extra(params, "trees", 4);
     efg_default_build_params(&bp, trees);
@@ -44,6 +44,15 @@ class FIndex {
     bp.seed = params.seed;
     check(efg_create(features.data(), features.get_rows(), features.get_cols(), features.get_stride(), &bp, &ix_),
           true);
+    if (params.init_index_type == HASHING) {  // code files (HASHINGIndex's constructor)
+      ExtraParamsMap::const_iterator b = params.extra_params.find("bcfile"), q = params.extra_params.find("qcfile");
+      if (b == params.extra_params.end() || q == params.extra_params.end())
+        throw std::runtime_error("error: no base or query code file");
+      check(efg_hash_load_codes(ix_, extra(params, "tablenum", 0), extra(params, "codelen", 0),
+                                extra(params, "lenshift", 0), extra(params, "upbits", 0), extra(params, "radius", 0),
+                                b->second.str_pt, q->second.str_pt),
+            true);
+    }
   }
   virtual ~FIndex() { efg_destroy(ix_); }
   FIndex(const FIndex&) = delete;

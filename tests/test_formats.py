@@ -60,8 +60,12 @@ int main() {
   L2DistanceAVX<float> l2;
   float x = l2.compare(m.get_row(0), m.get_row(1), 5);
   KDTreeUbIndexParams p(true, 4, 2, 2, 10, 12, 5, 0, 4);
-  (void)p; (void)x;
+  char* bf = (char*)"base_codes";
+  char* qf = (char*)"query_codes";
+  HASHINGIndexParams hp(32, 4, 16, 2, bf, qf, 0);
+  (void)p; (void)x; (void)hp;
   if (0) { FIndex<float> ix(m, &l2, p); ix.buildIndex(); ix.setSearchParams(2, 10, 5, 4); ix.knnSearch(5, m); }
+  if (0) { FIndex<float> ih(m, &l2, hp); ih.buildIndex(); ih.setSearchParams(2, 10, 5, 0, -1, 1); ih.knnSearch(5, m); }
   return 0;
 }
 """)
