@@ -190,6 +190,7 @@ struct efg_index {
   efg::DevBuf<uint32_t> h_lists;        // T x n rows grouped by bucket
   efg::DevBuf<uint64_t> h_masks;        // h_nmask masks in the reference's order
   efg::DevBuf<uint64_t> h_qcodes;       // T x h_nq query codes
+  efg::DevBuf<int32_t> h_dense;         // codes of at most kHashDenseBits bits: T x 2^bits bucket index or -1
   // graph (CSR)
   size_t nnz = 0;
   int max_deg = 0;
@@ -232,7 +233,8 @@ int fail(int code, const std::string& msg);
 int check_hip(hipError_t e, const char* what);
 
 // k_hash.hip
-constexpr int kHashSlots = 4096;  // largest masks x tables per query
+constexpr int kHashSlots = 4096;     // largest masks x tables per query
+constexpr int kHashDenseBits = 22;   // direct bucket lookup up to this code length
 int hash_set_codes(efg_index* ix, int tables, int codelen, int lenshift, int upbits, int radius, const uint64_t* base,
                    const uint64_t* query, size_t nq);
 int hash_build(efg_index* ix);

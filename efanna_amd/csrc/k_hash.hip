@@ -10,7 +10,8 @@
 // distinct codes of each table in ascending order, and one SNode (start,
 // end) per bucket, so the search kernel reads a bucket as it reads a leaf of
 // a tree: slot j * T + t of a query is its bucket in table t for mask j.
-// k_hash_probe finds the buckets by binary search over the table's codes.
+// k_hash_probe finds the buckets in a dense table indexed by code (codes of
+// at most 22 bits) or by binary search over the table's codes.
 #include <algorithm>
 #include <numeric>
 #include <vector>
@@ -54,11 +55,13 @@ int radius_cap(int bits) {
 }
 
 // Slot li = j * T + t of query list position w: the bucket of table t holding
-// code(query, t) XOR mask j, or -1.
+// code(query, t) XOR mask j, or -1.  Codes of at most kHashDenseBits bits
+// index a dense table (one load); longer ones are found by binary search.
 __global__ void k_hash_probe(const uint64_t* __restrict__ qcodes, size_t qn, int T,
                              const uint64_t* __restrict__ masks, const uint64_t* __restrict__ ucode,
-                             const uint32_t* __restrict__ uoff, int nleaf, int nq, int32_t* __restrict__ leaves,
-                             const int32_t* __restrict__ qlist, const int* __restrict__ nlist) {
+                             const uint32_t* __restrict__ uoff, const int32_t* __restrict__ dense, int bits,
+                             int nleaf, int nq, int32_t* __restrict__ leaves, const int32_t* __restrict__ qlist,
+                             const int* __restrict__ nlist) {
   const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int nw = qlist ? *nlist : nq;
   if (gid >= size_t(nw) * nleaf) return;
@@ -66,6 +69,10 @@ __global__ void k_hash_probe(const uint64_t* __restrict__ qcodes, size_t qn, int
   const int qi = qlist ? qlist[w] : w;
   const int j = li / T, t = li - j * T;
   const uint64_t code = qcodes[size_t(t) * qn + qi] ^ masks[j];
+  if (dense) {  // code < 2^bits: query codes are checked against the same bound
+    leaves[size_t(qi) * nleaf + li] = dense[(size_t(t) << bits) + code];
+    return;
+  }
   uint32_t lo = uoff[t], hi = uoff[t + 1];
   const uint32_t end = hi;
   while (lo < hi) {
@@ -145,6 +152,17 @@ int hash_build(efg_index* ix) {
     }
   }
   uoff[T] = uint32_t(ucode.size());
+  std::vector<int32_t> dense;
+  if (ix->h_bits <= kHashDenseBits) {  // bucket index by code
+    dense.assign(T << ix->h_bits, -1);
+    for (size_t t = 0; t < T; ++t)
+      for (uint32_t b = uoff[t]; b < uoff[t + 1]; ++b) dense[(t << ix->h_bits) + ucode[b]] = int32_t(b);
+    if (!ix->h_dense.alloc(dense.size())) return fail(EFG_ERR_MEMORY, "out of device memory (hash tables)");
+    if (int rc = check_hip(hipMemcpy(ix->h_dense.p, dense.data(), dense.size() * 4, hipMemcpyHostToDevice), "copy"))
+      return rc;
+  } else {
+    ix->h_dense.reset();
+  }
   if (!ix->h_lists.alloc(lists.size()) || !ix->h_ucode.alloc(ucode.size()) || !ix->h_uoff.alloc(uoff.size()) ||
       !ix->h_nodes.alloc(nodes.size()) || !ix->h_qcodes.alloc(std::max<size_t>(ix->h_query.size(), 1)))
     return fail(EFG_ERR_MEMORY, "out of device memory (hash tables)");
@@ -165,9 +183,9 @@ int hash_build(efg_index* ix) {
 int hash_probe(efg_index* ix, int nleaf, size_t nq, int32_t* leaves, const int32_t* qlist, const int* nlist,
                hipStream_t st) {
   const size_t threads = nq * size_t(nleaf);
-  k_hash_probe<<<unsigned((threads + 255) / 256), 256, 0, st>>>(ix->h_qcodes.p, ix->h_nq, ix->h_T, ix->h_masks.p,
-                                                                ix->h_ucode.p, ix->h_uoff.p, nleaf, int(nq), leaves,
-                                                                qlist, nlist);
+  k_hash_probe<<<unsigned((threads + 255) / 256), 256, 0, st>>>(
+      ix->h_qcodes.p, ix->h_nq, ix->h_T, ix->h_masks.p, ix->h_ucode.p, ix->h_uoff.p,
+      ix->h_bits <= kHashDenseBits ? ix->h_dense.p : nullptr, ix->h_bits, nleaf, int(nq), leaves, qlist, nlist);
   return EFG_OK;
 }
 

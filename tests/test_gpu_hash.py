@@ -40,7 +40,8 @@ def _both(o, g, q, k, epochs, pool, extend, method):
     return gi, gd, oi, od, ond
 
 
-@pytest.mark.parametrize("bits,radius,pool,k", [(12, 2, 100, 10), (20, 1, 50, 10), (12, 0, 40, 5)])
+# codes of up to 22 bits use the dense bucket table, longer ones the binary search
+@pytest.mark.parametrize("bits,radius,pool,k", [(12, 2, 100, 10), (20, 1, 50, 10), (12, 0, 40, 5), (30, 1, 60, 10)])
 def test_hash_only_search_matches_oracle(cuda, bits, radius, pool, k):
     base, q, o, g = _pair(bits=bits, radius=radius)
     gi, gd, oi, od, ond = _both(o, g, q, k, 4, pool, 20, 1)
@@ -59,8 +60,9 @@ def test_hash_graph_search_matches_oracle(cuda, method):
     assert g.counters()["search_dist"] == int(ond.sum())
 
 
-def test_hash_64bit_codes_and_shift(cuda):
-    base, q, o, g = _pair(bits=40, radius=1, lenshift=18, graph_k=8)
+@pytest.mark.parametrize("lenshift", [18, 4])
+def test_hash_64bit_codes_and_shift(cuda, lenshift):
+    base, q, o, g = _pair(bits=40, radius=1, lenshift=lenshift, graph_k=8)
     gi, gd, oi, od, ond = _both(o, g, q, 8, 3, 120, 30, 1)
     np.testing.assert_array_equal(gi, oi)
     np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32))
