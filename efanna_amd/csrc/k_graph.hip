@@ -2028,26 +2028,52 @@ __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK,
 // ---------------------------------------------------------------------------
 // update
 // ---------------------------------------------------------------------------
+// The update kernels give each node a group of G lanes (G a power of two,
+// at most 64; 64 / G nodes per wavefront), so that a node's pool entries are
+// read in one coalesced pass and counted with ballots over the group.
+template <int G>
+__device__ __forceinline__ uint64_t group_mask() {
+  const int lane = threadIdx.x & 63;
+  return (G == 64 ? ~0ull : ((1ull << G) - 1)) << (lane & ~(G - 1));
+}
+
+// Range (base_index.hpp:383-393): for found nodes, advance over at most
+// min(Range + S, fill) entries until S new ones are seen; then radiusM =
+// distance of entry Range - 1 (394).
+template <int G>
 __global__ void k_range(const uint64_t* __restrict__ pool, const uint8_t* __restrict__ flag,
                         const uint32_t* __restrict__ fill, const uint8_t* __restrict__ found, uint32_t n, int L, int S,
                         uint32_t* __restrict__ rng, float* __restrict__ radiusM) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint32_t p = uint32_t(t / G);
+  const int l0 = int(t % G), lane = threadIdx.x & 63;
+  if (p >= n) return;  // whole groups leave together
+  const uint64_t seg = group_mask<G>();
   uint32_t R = rng[p];
   if (found[p]) {
-    uint32_t maxl = R + uint32_t(S) < fill[p] ? R + uint32_t(S) : fill[p];
-    uint32_t c = 0, l = 0;
-    while (l < maxl && c < uint32_t(S)) {
-      if (flag[size_t(p) * L + l]) ++c;
-      ++l;
+    const uint32_t maxl = R + uint32_t(S) < fill[p] ? R + uint32_t(S) : fill[p];
+    uint32_t c = 0, nr = maxl;
+    for (int lb = 0; lb < int(maxl); lb += G) {  // maxl is the same across the group
+      const int l = lb + l0;
+      const bool f = uint32_t(l) < maxl && flag[size_t(p) * L + l] != 0;
+      uint64_t m = (__ballot(f) & seg) >> (lane & ~(G - 1));
+      const uint32_t cnt = uint32_t(__popcll(m));
+      if (c + cnt >= uint32_t(S)) {  // the S-th new entry lies in this block
+        for (uint32_t i = 1; i < uint32_t(S) - c; ++i) m &= m - 1;
+        nr = uint32_t(lb + __ffsll((long long)m));
+        break;
+      }
+      c += cnt;
     }
-    R = l;
-    rng[p] = R;
+    R = nr;
+    if (l0 == 0) rng[p] = R;
   }
-  radiusM[p] = key_dist(pool[size_t(p) * L + R - 1]);
+  if (l0 == 0) radiusM[p] = key_dist(pool[size_t(p) * L + R - 1]);
 }
 
 // pass 0: forward counts and reverse counts; pass 1: reverse fill
+// (base_index.hpp:396-420)
+template <int G>
 __global__ void k_lists(const uint64_t* __restrict__ pool, const uint8_t* __restrict__ flag,
                         const uint32_t* __restrict__ rng, const float* __restrict__ radiusM, uint32_t n, int L, int pass,
                         uint32_t* __restrict__ fcnt_new, uint32_t* __restrict__ fcnt_old,
@@ -2055,17 +2081,25 @@ __global__ void k_lists(const uint64_t* __restrict__ pool, const uint8_t* __rest
                         const int32_t* __restrict__ roff_new, const int32_t* __restrict__ roff_old,
                         uint32_t* __restrict__ rcur_new, uint32_t* __restrict__ rcur_old,
                         uint32_t* __restrict__ rids_new, uint32_t* __restrict__ rids_old) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint32_t p = uint32_t(t / G);
+  const int l0 = int(t % G);
   if (p >= n) return;
+  const uint64_t seg = group_mask<G>();
   const uint32_t R = rng[p];
-  uint32_t fn = 0, fo = 0;
-  for (uint32_t l = 0; l < R; ++l) {
-    uint64_t e = pool[size_t(p) * L + l];
-    uint32_t o = key_id(e);
-    float dist = key_dist(e);
-    bool isnew = flag[size_t(p) * L + l] != 0;
-    bool rev = dist > radiusM[o];
-    if (isnew) ++fn; else ++fo;
+  uint32_t fn = 0;
+  for (int lb = 0; lb < int(R); lb += G) {
+    const int l = lb + l0;
+    const bool act = uint32_t(l) < R;
+    bool isnew = false, rev = false;
+    uint32_t o = 0;
+    if (act) {
+      const uint64_t e = pool[size_t(p) * L + l];
+      o = key_id(e);
+      isnew = flag[size_t(p) * L + l] != 0;
+      rev = key_dist(e) > radiusM[o];
+    }
+    fn += uint32_t(__popcll(__ballot(act && isnew) & seg));
     if (!rev) continue;
     if (pass == 0) {
       atomicAdd(isnew ? &rcnt_new[o] : &rcnt_old[o], 1u);
@@ -2075,9 +2109,9 @@ __global__ void k_lists(const uint64_t* __restrict__ pool, const uint8_t* __rest
       rids_old[size_t(roff_old[o]) + atomicAdd(&rcur_old[o], 1u)] = p;
     }
   }
-  if (pass == 0) {
+  if (pass == 0 && l0 == 0) {
     fcnt_new[p] = fn;
-    fcnt_old[p] = fo;
+    fcnt_old[p] = R - fn;
   }
 }
 
@@ -2151,8 +2185,9 @@ __global__ void k_list_sizes(uint32_t n, const uint32_t* __restrict__ fnew, cons
   sold[p] = int32_t(fold[p] + rold[p]);
 }
 
-// Final join lists: forward part from the pool (new flags cleared), then the
-// sampled reverse part.
+// Final join lists: forward part from the pool (new flags cleared) in pool
+// order, then the sampled reverse part; a group of G lanes per node.
+template <int G>
 __global__ void k_list_fill(uint32_t n, int L, const uint64_t* __restrict__ pool, uint8_t* __restrict__ flag,
                             const uint32_t* __restrict__ rng, const int32_t* __restrict__ onew,
                             const int32_t* __restrict__ oold, const int32_t* __restrict__ roff_new,
@@ -2161,24 +2196,43 @@ __global__ void k_list_fill(uint32_t n, int L, const uint64_t* __restrict__ pool
                             const uint32_t* __restrict__ rold_sz, uint64_t* __restrict__ new_off,
                             uint32_t* __restrict__ new_ids, uint64_t* __restrict__ old_off,
                             uint32_t* __restrict__ old_ids) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint32_t p = uint32_t(t / G);
+  const int l0 = int(t % G), lane = threadIdx.x & 63;
   if (p > n) return;
-  new_off[p] = uint64_t(onew[p]);
-  old_off[p] = uint64_t(oold[p]);
+  if (l0 == 0) {
+    new_off[p] = uint64_t(onew[p]);
+    old_off[p] = uint64_t(oold[p]);
+  }
   if (p == n) return;
+  const uint64_t seg = group_mask<G>();
+  const uint64_t below = seg & ((1ull << lane) - 1);
   uint32_t wn = uint32_t(onew[p]), wo = uint32_t(oold[p]);
   const uint32_t R = rng[p];
-  for (uint32_t l = 0; l < R; ++l) {
-    uint32_t o = key_id(pool[size_t(p) * L + l]);
-    if (flag[size_t(p) * L + l]) {
-      new_ids[wn++] = o;
-      flag[size_t(p) * L + l] = 0;
-    } else {
-      old_ids[wo++] = o;
+  for (int lb = 0; lb < int(R); lb += G) {
+    const int l = lb + l0;
+    const bool act = uint32_t(l) < R;
+    bool nw = false;
+    uint32_t o = 0;
+    if (act) {
+      o = key_id(pool[size_t(p) * L + l]);
+      nw = flag[size_t(p) * L + l] != 0;
     }
+    const uint64_t mn = __ballot(act && nw) & seg, mo = __ballot(act && !nw) & seg;
+    if (act) {
+      if (nw) {
+        new_ids[wn + uint32_t(__popcll(mn & below))] = o;
+        flag[size_t(p) * L + l] = 0;
+      } else {
+        old_ids[wo + uint32_t(__popcll(mo & below))] = o;
+      }
+    }
+    wn += uint32_t(__popcll(mn));
+    wo += uint32_t(__popcll(mo));
   }
-  for (uint32_t i = 0; i < rnew_sz[p]; ++i) new_ids[wn++] = rnew_ids[size_t(roff_new[p]) + i];
-  for (uint32_t i = 0; i < rold_sz[p]; ++i) old_ids[wo++] = rold_ids[size_t(roff_old[p]) + i];
+  const uint32_t rn = rnew_sz[p], ro = rold_sz[p];
+  for (uint32_t i = uint32_t(l0); i < rn; i += G) new_ids[wn + i] = rnew_ids[size_t(roff_new[p]) + i];
+  for (uint32_t i = uint32_t(l0); i < ro; i += G) old_ids[wo + i] = rold_ids[size_t(roff_old[p]) + i];
 }
 
 __global__ void k_graph_out(uint32_t n, int L, int K, const uint64_t* __restrict__ pool,
@@ -2523,10 +2577,16 @@ int gpu_refine_iter(efg_index* ix) {
       !rcuro.ensure(n) || !rszn.ensure(n) || !rszo.ensure(n) || !roffn.ensure(n + 1) || !roffo.ensure(n + 1) ||
       !snew.ensure(n + 1) || !sold.ensure(n + 1) || !onew.ensure(n + 1) || !oold.ensure(n + 1))
     return fail(EFG_ERR_MEMORY, "out of device memory (update)");
-  k_range<<<grid_for(n), 256, 0, st>>>(bs.pool.p, bs.flag.p, bs.fill.p, bs.found.p, n, L, S, bs.range.p, bs.radiusM.p);
+  // a group of G lanes per node: the pool length rounded up to a power of
+  // two, 16 to 64
+  const int G = L <= 16 ? 16 : (L <= 32 ? 32 : 64);
+  const unsigned ggrid = unsigned((size_t(n) * G + 255) / 256), ggrid1 = unsigned((size_t(n + 1) * G + 255) / 256);
+#define EFG_G(kern) (G == 16 ? kern<16> : (G == 32 ? kern<32> : kern<64>))
+  EFG_G(k_range)<<<ggrid, 256, 0, st>>>(bs.pool.p, bs.flag.p, bs.fill.p, bs.found.p, n, L, S, bs.range.p,
+                                        bs.radiusM.p);
   hipMemsetAsync(rcn.p, 0, size_t(n) * 4, st);
   hipMemsetAsync(rco.p, 0, size_t(n) * 4, st);
-  k_lists<<<grid_for(n), 256, 0, st>>>(bs.pool.p, bs.flag.p, bs.range.p, bs.radiusM.p, n, L, 0, fnew.p, fold.p, rcn.p,
+  EFG_G(k_lists)<<<ggrid, 256, 0, st>>>(bs.pool.p, bs.flag.p, bs.range.p, bs.radiusM.p, n, L, 0, fnew.p, fold.p, rcn.p,
                                        rco.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
   if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(rcn.p), int(n), roffn.p, scal.p, sums, st)) return rc;
   if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(rco.p), int(n), roffo.p, scal.p + 1, sums, st)) return rc;
@@ -2537,7 +2597,7 @@ int gpu_refine_iter(efg_index* ix) {
     return fail(EFG_ERR_MEMORY, "out of device memory (reverse lists)");
   hipMemsetAsync(rcurn.p, 0, size_t(n) * 4, st);
   hipMemsetAsync(rcuro.p, 0, size_t(n) * 4, st);
-  k_lists<<<grid_for(n), 256, 0, st>>>(bs.pool.p, bs.flag.p, bs.range.p, bs.radiusM.p, n, L, 1, nullptr, nullptr,
+  EFG_G(k_lists)<<<ggrid, 256, 0, st>>>(bs.pool.p, bs.flag.p, bs.range.p, bs.radiusM.p, n, L, 1, nullptr, nullptr,
                                        nullptr, nullptr, roffn.p, roffo.p, rcurn.p, rcuro.p, ridn.p, rido.p);
   const int rcap = 100;
   const int schunk = 4096;
@@ -2558,9 +2618,10 @@ int gpu_refine_iter(efg_index* ix) {
   if (int rc = check_hip(hipStreamSynchronize(st), "update")) return rc;
   if (!bs.new_ids.ensure(size_t(ltot[0]) + 1) || !bs.old_ids.ensure(size_t(ltot[1]) + 1))
     return fail(EFG_ERR_MEMORY, "out of device memory (join lists)");
-  k_list_fill<<<grid_for(n + 1), 256, 0, st>>>(n, L, bs.pool.p, bs.flag.p, bs.range.p, onew.p, oold.p, roffn.p,
+  EFG_G(k_list_fill)<<<ggrid1, 256, 0, st>>>(n, L, bs.pool.p, bs.flag.p, bs.range.p, onew.p, oold.p, roffn.p,
                                                ridn.p, rszn.p, roffo.p, rido.p, rszo.p, bs.new_off.p, bs.new_ids.p,
                                                bs.old_off.p, bs.old_ids.p);
+#undef EFG_G
   bs.new_n = size_t(ltot[0]);
   bs.old_n = size_t(ltot[1]);
   if (int rc = check_hip(hipStreamSynchronize(st), "update")) return rc;
