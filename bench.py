@@ -11,9 +11,10 @@ One step = one search of all queries (inputs resident in HBM).  The graph
 build is timed once before the steps.  Beside the plain FP32 search the run
 measures the other forms on the same index:
   bf16_rows     BF16 copy of the rows, FP32 re-rank of the best 2k
-  learned_stop  probe search (pool 300, extend 60, 4 epochs) stopped by the
+  learned_stop  probe search (pool 500, extend 80, 2 epochs) stopped by the
                 16-64-64-1 network trained on the GPU from traces of 200,000
-                other queries of the mixture, full search when p < 0.9
+                other queries of the mixture, full search when p < 0.9; a
+                plain search at the probe's parameters is reported beside it
   method1       getNeighbors_kgraph at the same parameters
   hash_seeds    hashing index (IEH): seeds from 4 tables of 16-bit LSH codes,
                 Hamming radius 2, nnexp over the built graph (and the
@@ -188,7 +189,10 @@ def main() -> int:
 
     if not args.no_forms:
         # ---- learned stopping: traces of other queries of the mixture, GPU training, probe + fallback
-        probe = (300, 60, 4)
+        # probe parameters from a sweep on this workload: pools of 300-600
+        # with 1-4 epochs; smaller probes leave the network unsure for many
+        # queries, whose full searches then cost more than the probe saves
+        probe = (500, 80, 2)
         t = time.perf_counter()
         qtr = datasets.mixture_queries(d, args.train_queries, centres=1000, sigma=20.0, hi=255.0, rounded=True,
                                        seed=0, query_seed=2024)
@@ -207,6 +211,12 @@ def main() -> int:
         res_ls = ids.cpu().numpy()
         st = gix.learned_stats()
         gix.setSearchMode(0)
+        # the same probe without the network: plain search at its parameters
+        gix.setSearchParams(probe[2], probe[0], probe[1], s_trees, -1, 0)
+        gix.autotune(k, q)
+        pp_ms = timed(args.steps)
+        res_pp = ids.cpu().numpy()
+        gix.setSearchParams(s_epochs, s_pool, s_extend, s_trees, -1, 0)
         forms["learned_stop"] = {"ms_per_step": round(ls_ms, 4), "qps": round(nq / (ls_ms / 1e3), 1),
                                  "recall_at_10": round(rec(res_ls), 5), "fallbacks": st["last_fallbacks"],
                                  "probe": {"pool": probe[0], "extend": probe[1], "epochs": probe[2]},
@@ -214,7 +224,10 @@ def main() -> int:
                                  "threshold": 0.9, "train_queries": int(args.train_queries),
                                  "examples": int(len(y)), "positive_frac": round(float(y.mean()), 4),
                                  "loss_first_last": [round(float(hist[0]), 5), round(float(hist[-1]), 5)],
-                                 "traces_and_training_s": round(train_s, 2)}
+                                 "traces_and_training_s": round(train_s, 2),
+                                 "plain_at_probe_params": {"ms_per_step": round(pp_ms, 4),
+                                                           "qps": round(nq / (pp_ms / 1e3), 1),
+                                                           "recall_at_10": round(rec(res_pp), 5)}}
 
         # ---- method 1 (kgraph) at the same parameters
         gix.setSearchParams(s_epochs, s_pool, s_extend, s_trees, -1, 1)

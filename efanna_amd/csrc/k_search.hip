@@ -386,8 +386,6 @@ __host__ __device__ inline size_t search_lds_bytes(const SearchCfg& c, bool glob
     b += a16(sizeof(int32_t) * c.K);            // top k ids of the last checkpoint
     b += a16(sizeof(float) * (kFeat + 2 * kHid + 8 + kHid + 1 + 2 * kFeat));  // features, hidden units,
                                                                              // history, W3 | b3, norm
-    if (c.stop == 1) b += a16(sizeof(uint32_t) * (kHid / 2) * kHid);            // W2, packed BF16 pairs
-
   }
   if (c.method == 1) {
     const int kc = c.K + c.E + 1;
@@ -411,8 +409,148 @@ struct LearnArgs {
   const int32_t* plain_ids;  // stop 2: [query][K]
 };
 
+// Probe-search checkpoint: features of the sorted pool P (R entries); stop 2
+// writes them with the label, stop 1 evaluates the network and returns true
+// when its logit reaches la.zthr.  e: checkpoint index, nexp_last: ids
+// expanded in the last epoch, visited: ids measured so far.  Called by every
+// thread of the workgroup after a barrier; wavefront 0 does all the work
+// (ballot counts, one lane per hidden unit, layer inputs passed between lanes
+// with readlane, sums in the same order as the training forward pass), and
+// one barrier publishes the decision.  Kept out of line so that the search
+// loop around it is register-allocated as in the plain kernel.
+template <int kStop>
+__device__ __noinline__ bool ckpt_eval(int cE, int cK, int cepochs, int clogcap, const uint32_t* __restrict__ wpack,
+                                       float zthr, float* tr_feat, uint8_t* tr_label, const int32_t* plain_ids,
+                                       const uint32_t* tab, int* misc, const uint64_t* P, int R, int32_t* prevk,
+                                       float* ckf, int qi, int e, int nexp_last, int visited) {
+  const int tid = threadIdx.x;
+  auto rl = [](float x, int j) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j)); };
+  if (tid < 64) {
+    const int lane = tid;
+    // stop 1: lane j loads the packed W1 column and biases of hidden unit j
+    // now; the loads complete while the pool statistics are gathered
+    constexpr int kW1Words = kFeat / 2;
+    uint32_t wreg[kW1Words + 1];
+    if (kStop == 1) {
+#pragma unroll
+      for (int i = 0; i < kW1Words; ++i) wreg[i] = wpack[i * kHid + lane];
+      wreg[kW1Words] = wpack[(kPackWords - 1) * kHid + lane];
+    }
+    const int ER = cE < R ? cE : R;
+    int fr = 0;  // unexpanded ids among the first extend_to
+    for (int b0 = 0; b0 < ER; b0 += 64) {
+      const int j = b0 + lane;
+      bool v = false;
+      if (j < ER) {
+        bool pres;
+        const int sl = tab_find(tab, clogcap, key_id(P[j]), &pres);
+        v = (tab[sl] & kExp) == 0;
+      }
+      fr += __popcll(__ballot(v));
+    }
+    int ch = cK;  // top k ids that were not there at the last checkpoint
+    if (e > 0) {
+      ch = 0;
+      for (int b0 = 0; b0 < cK; b0 += 64) {
+        const int j = b0 + lane;
+        bool v = false;
+        if (j < cK) {
+          const int32_t id = int32_t(key_id(P[j]));
+          v = true;
+          for (int t2 = 0; t2 < cK; ++t2)
+            if (prevk[t2] == id) {
+              v = false;
+              break;
+            }
+        }
+        ch += __popcll(__ballot(v));
+      }
+    }
+    int match = 0;  // stop 2: top k ids that are in the plain result's top k
+    if (kStop == 2) {
+      const int32_t* pl = plain_ids + size_t(qi) * cK;
+      for (int b0 = 0; b0 < cK; b0 += 64) {
+        const int j = b0 + lane;
+        bool v = false;
+        if (j < cK) {
+          const int32_t id = int32_t(key_id(P[j]));
+          for (int t2 = 0; t2 < cK; ++t2)
+            if (pl[t2] == id) {
+              v = true;
+              break;
+            }
+        }
+        match += __popcll(__ballot(v));
+      }
+    }
+    wave_sync();  // prevk is read before it is replaced
+    for (int j = lane; j < cK; j += 64) prevk[j] = int32_t(key_id(P[j]));
+    float* hist = ckf + kFeat + 2 * kHid;  // k-th [0..2] and best [3..5] distance, latest first
+    if (lane == 0) {
+      CkptState st;
+      st.e = e;
+      st.d1 = key_dist(P[0]);
+      st.dk = key_dist(P[cK - 1]);
+      st.dE = key_dist(P[(ER > 0 ? ER : 1) - 1]);
+      st.dR = key_dist(P[R - 1]);
+      for (int j = 0; j < 3; ++j) {
+        st.dk_hist[j] = hist[j];
+        st.d1_hist[j] = hist[3 + j];
+      }
+      st.visited = float(visited);
+      st.expanded = float(nexp_last);
+      st.changed = float(ch);
+      st.frontier = float(fr);
+      ckpt_features(st, ckf);
+      hist[2] = hist[1];
+      hist[1] = hist[0];
+      hist[0] = st.dk;
+      hist[5] = hist[4];
+      hist[4] = hist[3];
+      hist[3] = st.d1;
+    }
+    wave_sync();
+    if (kStop == 2) {
+      const size_t slot = size_t(qi) * (cepochs + 1) + e;
+      if (lane < kFeat) tr_feat[slot * kFeat + lane] = ckf[lane];
+      if (lane == 0) tr_label[slot] = uint8_t(match < 254 ? match : 254);
+    } else {
+      const float* w3s = ckf + kFeat + 2 * kHid + 8;  // W3 | b3
+      const float* nrm = w3s + kHid + 1;               // feature mean | 1 / deviation
+      const uint32_t* __restrict__ w2 = wpack + (kFeat / 2) * kHid;
+      auto lo = [](uint32_t v) { return __uint_as_float(v << 16); };
+      auto hi = [](uint32_t v) { return __uint_as_float(v & 0xffff0000u); };
+      float a = lo(wreg[kW1Words]);
+#pragma unroll
+      for (int i = 0; i < kFeat; ++i)
+        a = __builtin_fmaf((i & 1) ? hi(wreg[i / 2]) : lo(wreg[i / 2]), mlp_input(ckf, nrm, i), a);
+      const float h1 = a > 0.f ? a : 0.f;
+      a = hi(wreg[kW1Words]);
+#pragma unroll 8
+      for (int i = 0; i < kHid; i += 2) {
+        const uint32_t v = w2[(i / 2) * kHid + lane];
+        a = __builtin_fmaf(lo(v), rl(h1, i), a);
+        a = __builtin_fmaf(hi(v), rl(h1, i + 1), a);
+      }
+      const float h2 = a > 0.f ? a : 0.f;
+      const float w3 = w3s[lane];
+      float z = w3s[kHid];
+#pragma unroll 8
+      for (int j = 0; j < kHid; ++j)
+        z = __builtin_fmaf(rl(w3, j), rl(h2, j), z);
+      if (lane == 0) misc[7] = z >= zthr ? 1 : 0;
+    }
+  }
+  __syncthreads();  // wavefront 0 has read P and the table before they change
+  if (kStop == 2) return false;
+  const bool stop = misc[7] != 0;
+  __syncthreads();  // misc[7] is read before it is reused
+  return stop;
+}
+
 // The BF16-row variant is held to 128 VGPRs (four wavefronts per SIMD, which
-// its LDS allows); the others keep their registers for loads in flight.
+// its LDS allows); the others keep their registers for loads in flight (a cap
+// of four wavefronts made the plain kernel 40 % slower at pool 500).
 template <bool kGlobal, int kMethod, bool kBf16, int kStop, int kThreads>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kBf16 ? 4 : 1)))
 k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
@@ -460,7 +598,6 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
   // learned stopping working arrays (after the common layout)
   int32_t* prevk = nullptr;
   float* ckf = nullptr;  // features [kFeat] | h1 [kHid] | h2 [kHid] | history [8] | W3 | b3 | norm
-  uint32_t* w2s = nullptr;  // stop 1: W2 rows (2i, 2i+1) of hidden unit j at [i * kHid + j]
   if (kStop) {
     SearchCfg c0 = c;
     c0.stop = 0;
@@ -473,7 +610,6 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     };
     prevk = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * c.K));
     ckf = reinterpret_cast<float*>(take(sizeof(float) * (kFeat + 2 * kHid + 8 + kHid + 1 + 2 * kFeat)));
-    if (kStop == 1) w2s = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * (kHid / 2) * kHid));
   }
   float* w3s = ckf ? ckf + kFeat + 2 * kHid + 8 : nullptr;  // W3 | b3
   float* nrm = w3s ? w3s + kHid + 1 : nullptr;              // feature mean | 1 / deviation
@@ -481,7 +617,6 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     const float* w3 = reinterpret_cast<const float*>(la.wpack + kPackLen);
     for (int i = threadIdx.x; i <= kHid; i += kThreads) w3s[i] = w3[i];
     for (int i = threadIdx.x; i < 2 * kFeat; i += kThreads) nrm[i] = la.norm[i];
-    for (int i = threadIdx.x; i < (kHid / 2) * kHid; i += kThreads) w2s[i] = la.wpack[(kFeat / 2) * kHid + i];
     __syncthreads();
   }
   // method 1 (kgraph) working arrays
@@ -591,136 +726,10 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     __syncthreads();
   };
 
-  // Probe-search checkpoint (all threads): features of the sorted pool P;
-  // stop 2 writes them with the label, stop 1 evaluates the network and
-  // returns true when its logit reaches la.zthr.  e: checkpoint index,
-  // nexp_last: ids expanded in the last epoch, visited: ids measured so far.
+  // Probe-search checkpoint (all threads), out of line (see ckpt_eval).
   auto checkpoint = [&](int qi, int e, int nexp_last, int visited) -> bool {
-    // stop 1: thread j < 64 loads its packed W1 column and biases now (W2 is
-    // in LDS); the loads complete while the pool statistics are gathered
-    constexpr int kW1Words = kFeat / 2;
-    uint32_t wreg[kW1Words + 1];
-    if (kStop == 1 && tid < kHid) {
-#pragma unroll
-      for (int i = 0; i < kW1Words; ++i) wreg[i] = la.wpack[i * kHid + tid];
-      wreg[kW1Words] = la.wpack[(kPackWords - 1) * kHid + tid];
-    }
-    const int ER = c.E < R ? c.E : R;
-    int fr = 0;  // unexpanded ids among the first extend_to
-    for (int b0 = 0; b0 < ER; b0 += kThreads) {
-      const int j = b0 + tid;
-      int v = 0;
-      if (j < ER) {
-        bool pres;
-        const int sl = tab_find(s.tab, c.logcap, key_id(P[j]), &pres);
-        v = (s.tab[sl] & kExp) ? 0 : 1;
-      }
-      int tot;
-      block_exclusive_scan(v, s.scan, &tot);
-      fr += tot;
-    }
-    int ch = c.K;  // top k ids that were not there at the last checkpoint
-    if (e > 0) {
-      ch = 0;
-      for (int b0 = 0; b0 < c.K; b0 += kThreads) {
-        const int j = b0 + tid;
-        int v = 0;
-        if (j < c.K) {
-          const int32_t id = int32_t(key_id(P[j]));
-          v = 1;
-          for (int t2 = 0; t2 < c.K; ++t2)
-            if (prevk[t2] == id) {
-              v = 0;
-              break;
-            }
-        }
-        int tot;
-        block_exclusive_scan(v, s.scan, &tot);
-        ch += tot;
-      }
-    }
-    int match = 0;  // stop 2: top k ids that are in the plain result's top k
-    if (kStop == 2) {
-      const int32_t* pl = la.plain_ids + size_t(qi) * c.K;
-      for (int b0 = 0; b0 < c.K; b0 += kThreads) {
-        const int j = b0 + tid;
-        int v = 0;
-        if (j < c.K) {
-          const int32_t id = int32_t(key_id(P[j]));
-          for (int t2 = 0; t2 < c.K; ++t2)
-            if (pl[t2] == id) {
-              v = 1;
-              break;
-            }
-        }
-        int tot;
-        block_exclusive_scan(v, s.scan, &tot);
-        match += tot;
-      }
-    }
-    for (int j = tid; j < c.K; j += kThreads) prevk[j] = int32_t(key_id(P[j]));
-    float* hist = ckf + kFeat + 2 * kHid;  // k-th [0..2] and best [3..5] distance, latest first
-    if (tid == 0) {
-      CkptState st;
-      st.e = e;
-      st.d1 = key_dist(P[0]);
-      st.dk = key_dist(P[c.K - 1]);
-      st.dE = key_dist(P[(ER > 0 ? ER : 1) - 1]);
-      st.dR = key_dist(P[R - 1]);
-      for (int j = 0; j < 3; ++j) {
-        st.dk_hist[j] = hist[j];
-        st.d1_hist[j] = hist[3 + j];
-      }
-      st.visited = float(visited);
-      st.expanded = float(nexp_last);
-      st.changed = float(ch);
-      st.frontier = float(fr);
-      ckpt_features(st, ckf);
-      hist[2] = hist[1];
-      hist[1] = hist[0];
-      hist[0] = st.dk;
-      hist[5] = hist[4];
-      hist[4] = hist[3];
-      hist[3] = st.d1;
-    }
-    __syncthreads();
-    if (kStop == 2) {
-      const size_t slot = size_t(qi) * (c.epochs + 1) + e;
-      if (tid < kFeat) la.tr_feat[slot * kFeat + tid] = ckf[tid];
-      if (tid == 0) la.tr_label[slot] = uint8_t(match < 254 ? match : 254);
-      return false;
-    }
-    float* h1 = ckf + kFeat;
-    float* h2 = h1 + kHid;
-    auto lo = [](uint32_t v) { return __uint_as_float(v << 16); };
-    auto hi = [](uint32_t v) { return __uint_as_float(v & 0xffff0000u); };
-    if (tid < kHid) {
-      float a = lo(wreg[kW1Words]);
-#pragma unroll
-      for (int i = 0; i < kFeat; ++i)
-        a = __builtin_fmaf((i & 1) ? hi(wreg[i / 2]) : lo(wreg[i / 2]), mlp_input(ckf, nrm, i), a);
-      h1[tid] = a > 0.f ? a : 0.f;
-    }
-    __syncthreads();
-    if (tid < kHid) {
-      float a = hi(wreg[kW1Words]);
-      for (int i = 0; i < kHid; i += 2) {
-        const uint32_t v = w2s[(i / 2) * kHid + tid];
-        a = __builtin_fmaf(lo(v), h1[i], a);
-        a = __builtin_fmaf(hi(v), h1[i + 1], a);
-      }
-      h2[tid] = a > 0.f ? a : 0.f;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      float z = w3s[kHid];
-      for (int j = 0; j < kHid; ++j) z = __builtin_fmaf(w3s[j], h2[j], z);
-      s.misc[7] = z >= la.zthr ? 1 : 0;
-    }
-    __syncthreads();
-    const bool stop = s.misc[7] != 0;
-    __syncthreads();
-    return stop;
+    return ckpt_eval<kStop>(c.E, c.K, c.epochs, c.logcap, la.wpack, la.zthr, la.tr_feat, la.tr_label, la.plain_ids,
+                            s.tab, s.misc, P, R, prevk, ckf, qi, e, nexp_last, visited);
   };
 
   // work distribution: slices of the query order per XCD

@@ -87,6 +87,28 @@ def test_build_stages_match_oracle(cuda):
     np.testing.assert_array_equal(g.graph_matrix(), o.graph_matrix())
 
 
+@pytest.mark.parametrize("L,K,check,S", [(12, 8, 12, 8), (80, 20, 40, 12)])
+def test_build_pool_lengths_match_oracle(cuda, L, K, check, S):
+    """Pool lengths on either side of the update kernels' lane groups (16
+    lanes per node for L = 12; 64 lanes and two passes for L = 80) and of the
+    register pool merge (L = 80 takes the LDS window merge)."""
+    base, _ = helpers.uniform(n=4000)
+    g, o = _pair(base, trees=4, ml=5, epochs=5, check=check, L=L, K=K, S=S)
+    g.build_stage(0)
+    o.build_trees()
+    g.build_stage(1)
+    o.merge_init()
+    _cmp_state(g, o)
+    for it in range(5):
+        g.build_stage(2)
+        o.refine_iter()
+        _cmp_state(g, o)
+    assert g.join_pairs() == o.join_pairs()
+    g.build_stage(3)
+    o.finalize()
+    np.testing.assert_array_equal(g.graph_matrix(), o.graph_matrix())
+
+
 @pytest.mark.parametrize("d", [100, 200])
 def test_build_float_data_long_lists_matches_oracle(cuda, d):
     # non-integer rows (every FMA rounds), d % 32 != 0 (tail chunk first),
