@@ -11,7 +11,7 @@ One step = one search of all queries (inputs resident in HBM).  The graph
 build is timed once before the steps.  Beside the plain FP32 search the run
 measures the other forms on the same index:
   bf16_rows     BF16 copy of the rows, FP32 re-rank of the best 2k
-  learned_stop  probe search (pool 500, extend 80, 2 epochs) stopped by the
+  learned_stop  probe search (pool 450, extend 75, 2 epochs) stopped by the
                 16-64-64-1 network trained on the GPU from traces of 200,000
                 other queries of the mixture, full search when p < 0.9; a
                 plain search at the probe's parameters is reported beside it
@@ -189,10 +189,10 @@ def main() -> int:
 
     if not args.no_forms:
         # ---- learned stopping: traces of other queries of the mixture, GPU training, probe + fallback
-        # probe parameters from a sweep on this workload: pools of 300-600
-        # with 1-4 epochs; smaller probes leave the network unsure for many
+        # probe parameters from a sweep on this workload (pools of 300-600,
+        # 1-4 epochs): smaller probes leave the network unsure for many
         # queries, whose full searches then cost more than the probe saves
-        probe = (500, 80, 2)
+        probe = (450, 75, 2)
         t = time.perf_counter()
         qtr = datasets.mixture_queries(d, args.train_queries, centres=1000, sigma=20.0, hi=255.0, rounded=True,
                                        seed=0, query_seed=2024)
