@@ -158,8 +158,10 @@ constexpr int kDS = kMP + 1;         // padded distance-tile row (floats)
 // One workgroup per (32 kQB queries, chunk of base rows).  For every kMP-row
 // base tile the (32 kQB) x 128 dot products are 32 x 32 MFMA blocks, kQB per
 // wavefront (block b = w kQB + a: query block b / 4, base block b % 4), over
-// the elements in stages of kMK through LDS; distances go to an LDS tile and
-// one thread per query keeps its kc smallest (distance, id) keys.
+// the elements in stages of kMK through LDS; distances go to an LDS tile.
+// All threads then mark the tile's entries below their query's k-th key
+// (kBlock / (32 kQB) threads per query), and one thread per query inserts
+// only those into its kc smallest (distance, id) keys.
 template <bool kF16, int kQB>
 __global__ void __launch_bounds__(kBlock) k_exact_mfma(const uint16_t* __restrict__ Xh, const float* __restrict__ xn,
                                                        size_t n, const uint16_t* __restrict__ Qh,
@@ -172,6 +174,9 @@ __global__ void __launch_bounds__(kBlock) k_exact_mfma(const uint16_t* __restric
   uint16_t* xs = qs + kMQ * kMKP;                         // [kMP][kMKP]
   float* dt = reinterpret_cast<float*>(xs + kMP * kMKP);  // [kMQ][kDS]
   uint64_t* topk = reinterpret_cast<uint64_t*>(dt + kMQ * kDS);  // [kMQ][kc]
+  uint32_t* smask = reinterpret_cast<uint32_t*>(topk + kMQ * kc);  // [kBlock] survivor masks
+  constexpr int kSub = kBlock / kMQ;                                // threads per query in the survivor scan
+  static_assert(kMP / kSub <= 32, "one 32-bit mask per thread");
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int q0 = blockIdx.x * kMQ;
   const size_t p0 = size_t(blockIdx.y) * chunk;
@@ -238,23 +243,46 @@ __global__ void __launch_bounds__(kBlock) k_exact_mfma(const uint16_t* __restric
       }
     }
     __syncthreads();
-    if (tid < kMQ) {
-      const int qg = q0 + tid;
+    const int m = int(p1 - pt < size_t(kMP) ? p1 - pt : size_t(kMP));
+    {
+      // survivors: the kSub threads of a query check the columns sub, sub +
+      // kSub, ... against the query's threshold at the start of the tile
+      // and record them in bit masks (bit t: column sub + t kSub)
+      const int ql = tid / kSub, sub = tid % kSub;
+      const int qg = q0 + ql;
+      const uint64_t thr = topk[size_t(ql) * kc + kc - 1];
+      uint32_t mask = 0;
+      if (qg < nq) {
+        for (int t = 0; sub + t * kSub < m; ++t) {
+          const int j = sub + t * kSub;
+          const size_t pg = pt + j;
+          if (exclude_self && pg == size_t(qg)) continue;
+          if (make_key(dt[ql * kDS + j], uint32_t(pg)) < thr) mask |= 1u << t;
+        }
+      }
+      smask[tid] = mask;
+    }
+    __syncthreads();
+    // one thread per query inserts its survivors; the kc smallest keys do
+    // not depend on the order of insertion (keys are distinct)
+    if (tid < kMQ && q0 + tid < nq) {
       uint64_t* tk = topk + size_t(tid) * kc;
       uint64_t thr = tk[kc - 1];
-      const int m = int(p1 - pt < size_t(kMP) ? p1 - pt : size_t(kMP));
-      for (int j = 0; j < m && qg < nq; ++j) {
-        const size_t pg = pt + j;
-        if (exclude_self && pg == size_t(qg)) continue;
-        const uint64_t key = make_key(dt[tid * kDS + j], uint32_t(pg));
-        if (key >= thr) continue;
-        int z = kc - 1;
-        while (z > 0 && tk[z - 1] > key) {
-          tk[z] = tk[z - 1];
-          --z;
+      for (int sub = 0; sub < kSub; ++sub) {
+        uint32_t mask = smask[tid * kSub + sub];
+        while (mask) {
+          const int j = sub + (__ffs(int(mask)) - 1) * kSub;
+          mask &= mask - 1;
+          const uint64_t key = make_key(dt[tid * kDS + j], uint32_t(pt + j));
+          if (key >= thr) continue;
+          int z = kc - 1;
+          while (z > 0 && tk[z - 1] > key) {
+            tk[z] = tk[z - 1];
+            --z;
+          }
+          tk[z] = key;
+          thr = tk[kc - 1];
         }
-        tk[z] = key;
-        thr = tk[kc - 1];
       }
     }
   }
@@ -321,7 +349,8 @@ int exact_knn_mfma(const float* X, size_t n, int d, size_t x_stride, const float
   const int Dk = (d + kMK - 1) / kMK * kMK;
   const int kc = int(std::min<size_t>(n, size_t(2 * k)));  // candidates re-ranked per query
   auto lds_for = [&](int qb) {
-    return size_t(32 * qb + kMP) * kMKP * 2 + size_t(32 * qb) * kDS * 4 + size_t(32 * qb) * kc * 8;
+    return size_t(32 * qb + kMP) * kMKP * 2 + size_t(32 * qb) * kDS * 4 + size_t(32 * qb) * kc * 8 +
+           size_t(kBlock) * 4;
   };
   const size_t lds_cap = 150 * 1024;
   const int qb = lds_for(2) <= lds_cap ? 2 : 1;
