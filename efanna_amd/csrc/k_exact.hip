@@ -159,9 +159,9 @@ constexpr int kDS = kMP + 1;         // padded distance-tile row (floats)
 // base tile the (32 kQB) x 128 dot products are 32 x 32 MFMA blocks, kQB per
 // wavefront (block b = w kQB + a: query block b / 4, base block b % 4), over
 // the elements in stages of kMK through LDS; distances go to an LDS tile.
-// All threads then mark the tile's entries below their query's k-th key
+// All threads then mark the tile's entries below their query's kc-th key
 // (kBlock / (32 kQB) threads per query), and one thread per query inserts
-// only those into its kc smallest (distance, id) keys.
+// only those into a max-heap of its kc smallest (distance, id) keys.
 template <bool kF16, int kQB>
 __global__ void __launch_bounds__(kBlock) k_exact_mfma(const uint16_t* __restrict__ Xh, const float* __restrict__ xn,
                                                        size_t n, const uint16_t* __restrict__ Qh,
@@ -184,26 +184,49 @@ __global__ void __launch_bounds__(kBlock) k_exact_mfma(const uint16_t* __restric
   for (int i = tid; i < kMQ * kc; i += kBlock) topk[i] = kKeyMax;
   const int qb = (w * kQB) / 4, pb = (w * kQB) % 4;  // query block; first base block (kQB consecutive)
   const int r32 = lane & 31, kh = (lane >> 5) * 8;
+  // A stage is (kMQ query rows + kMP base rows) x kMK elements, kLoads 16-byte
+  // pieces per thread.  The next stage's pieces are loaded into registers
+  // while this stage's MFMAs (and, at a tile's end, the selection) run.
+  constexpr int kPieces = (kMQ + kMP) * (kMK / 8), kLoads = (kPieces + kBlock - 1) / kBlock;
+  uint4 pre[kLoads];
+  auto load_stage = [&](size_t pt_, int k0_) {
+#pragma unroll
+    for (int u = 0; u < kLoads; ++u) {
+      const int i = tid + u * kBlock;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (i < kPieces) {
+        const int row = i / (kMK / 8), c8 = (i % (kMK / 8)) * 8;
+        if (row < kMQ) {
+          const int qg = q0 + row;
+          if (qg < nq) v = *reinterpret_cast<const uint4*>(Qh + size_t(qg) * Dk + k0_ + c8);
+        } else {
+          const size_t pg = pt_ + (row - kMQ);
+          if (pg < p1) v = *reinterpret_cast<const uint4*>(Xh + pg * Dk + k0_ + c8);
+        }
+      }
+      pre[u] = v;
+    }
+  };
+  if (p0 < p1) load_stage(p0, 0);
   for (size_t pt = p0; pt < p1; pt += kMP) {
     f32x16_t acc[kQB];
 #pragma unroll
     for (int a = 0; a < kQB; ++a) acc[a] = f32x16_t{};
     for (int k0 = 0; k0 < Dk; k0 += kMK) {
       __syncthreads();  // the previous stage (or the last tile's selection) is done with LDS
-      for (int i = tid; i < (kMQ + kMP) * (kMK / 8); i += kBlock) {
-        const int row = i / (kMK / 8), c8 = (i % (kMK / 8)) * 8;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        uint16_t* dst;
-        if (row < kMQ) {
-          const int qg = q0 + row;
-          if (qg < nq) v = *reinterpret_cast<const uint4*>(Qh + size_t(qg) * Dk + k0 + c8);
-          dst = qs + row * kMKP + c8;
-        } else {
-          const size_t pg = pt + (row - kMQ);
-          if (pg < p1) v = *reinterpret_cast<const uint4*>(Xh + pg * Dk + k0 + c8);
-          dst = xs + (row - kMQ) * kMKP + c8;
+#pragma unroll
+      for (int u = 0; u < kLoads; ++u) {
+        const int i = tid + u * kBlock;
+        if (i < kPieces) {
+          const int row = i / (kMK / 8), c8 = (i % (kMK / 8)) * 8;
+          uint16_t* dst = row < kMQ ? qs + row * kMKP + c8 : xs + (row - kMQ) * kMKP + c8;
+          *reinterpret_cast<uint4*>(dst) = pre[u];
         }
-        *reinterpret_cast<uint4*>(dst) = v;
+      }
+      {  // the next stage: the next elements of this tile, else the next tile's first
+        const bool last = k0 + kMK >= Dk;
+        const size_t npt = last ? pt + kMP : pt;
+        if (npt < p1) load_stage(npt, last ? 0 : k0 + kMK);
       }
       __syncthreads();
 #pragma unroll
@@ -250,7 +273,7 @@ __global__ void __launch_bounds__(kBlock) k_exact_mfma(const uint16_t* __restric
       // and record them in bit masks (bit t: column sub + t kSub)
       const int ql = tid / kSub, sub = tid % kSub;
       const int qg = q0 + ql;
-      const uint64_t thr = topk[size_t(ql) * kc + kc - 1];
+      const uint64_t thr = topk[size_t(ql) * kc];  // the heap's largest key
       uint32_t mask = 0;
       if (qg < nq) {
         for (int t = 0; sub + t * kSub < m; ++t) {
@@ -263,11 +286,12 @@ __global__ void __launch_bounds__(kBlock) k_exact_mfma(const uint16_t* __restric
       smask[tid] = mask;
     }
     __syncthreads();
-    // one thread per query inserts its survivors; the kc smallest keys do
-    // not depend on the order of insertion (keys are distinct)
+    // one thread per query inserts its survivors into the query's max-heap
+    // of its kc smallest keys (the re-rank sorts them; the kc smallest keys
+    // do not depend on the order of insertion, keys being distinct)
     if (tid < kMQ && q0 + tid < nq) {
       uint64_t* tk = topk + size_t(tid) * kc;
-      uint64_t thr = tk[kc - 1];
+      uint64_t thr = tk[0];
       for (int sub = 0; sub < kSub; ++sub) {
         uint32_t mask = smask[tid * kSub + sub];
         while (mask) {
@@ -275,13 +299,17 @@ __global__ void __launch_bounds__(kBlock) k_exact_mfma(const uint16_t* __restric
           mask &= mask - 1;
           const uint64_t key = make_key(dt[tid * kDS + j], uint32_t(pt + j));
           if (key >= thr) continue;
-          int z = kc - 1;
-          while (z > 0 && tk[z - 1] > key) {
-            tk[z] = tk[z - 1];
-            --z;
+          int z = 0;  // the root is replaced and sifted down
+          for (;;) {
+            const int l = 2 * z + 1;
+            if (l >= kc) break;
+            const int cz = (l + 1 < kc && tk[l + 1] > tk[l]) ? l + 1 : l;
+            if (tk[cz] <= key) break;
+            tk[z] = tk[cz];
+            z = cz;
           }
           tk[z] = key;
-          thr = tk[kc - 1];
+          thr = tk[0];
         }
       }
     }
