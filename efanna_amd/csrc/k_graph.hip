@@ -1874,29 +1874,35 @@ __device__ __forceinline__ bool kt_less(uint64_t ka, uint32_t ta, uint64_t kb, u
   return ka < kb || (ka == kb && ta < tb);
 }
 
-// Pool merge for pools of at most 64 entries (FP32 build): the result of
+// Pool merge for pools of at most kLMax entries (FP32 build): the result of
 // k_merge_wave<*, false> (best L distinct keys of pool + candidates, the
 // lowest tag staying among equal keys), reached with less work.  Rounds take
 // up to 64 candidates at or below the running list's worst key, one per lane;
 // they are sorted by (key, tag) in registers (bitonic network over shuffles)
-// and merged with the sorted running list by ranks: a candidate's place is
-// its sorted index plus the number of list entries at or before it (binary
-// search), a list entry's place its index plus the number of candidates
-// before it.  Exactly equal entries put the list's first.  Then duplicates
-// are dropped and the first L kept.  One wavefront per target.
+// and merged with the sorted running list by ranks into a second LDS buffer:
+// a candidate's place is its sorted index plus the number of list entries at
+// or before it (binary search), a list entry's place its index plus the
+// number of candidates before it.  Exactly equal entries put the list's
+// first.  Then duplicates are dropped and the first L copied back as the new
+// running list.  One wavefront per target.
+template <int kLMax>
 __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK, const int32_t* __restrict__ toff,
                                                       const uint64_t* __restrict__ bkey,
                                                       const uint32_t* __restrict__ bsrc, uint64_t* __restrict__ pool,
                                                       uint8_t* __restrict__ flag, uint32_t* __restrict__ fill,
                                                       uint8_t* __restrict__ credit) {
   constexpr int kW = kBlock / 64;
-  __shared__ uint64_t ws_k[kW][128];
-  __shared__ uint32_t ws_t[kW][128];
-  __shared__ uint64_t cs_k[kW][64];
+  __shared__ uint64_t ls_k[kW][kLMax];  // running list
+  __shared__ uint32_t ls_t[kW][kLMax];
+  __shared__ uint64_t ms_k[kW][kLMax + 64];  // merged window
+  __shared__ uint32_t ms_t[kW][kLMax + 64];
+  __shared__ uint64_t cs_k[kW][64];  // candidates of the round
   __shared__ uint32_t cs_t[kW][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint64_t* P = ws_k[w];
-  uint32_t* PT = ws_t[w];
+  uint64_t* A = ls_k[w];
+  uint32_t* AT = ls_t[w];
+  uint64_t* B = ms_k[w];
+  uint32_t* BT = ms_t[w];
   uint64_t* C = cs_k[w];
   uint32_t* CT = cs_t[w];
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -1904,13 +1910,13 @@ __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK,
     const int c0 = toff[x], c1 = toff[x + 1];
     if (c0 == c1) continue;
     int cur = int(fill[x]);
-    if (lane < cur) {
-      P[lane] = pool[size_t(x) * L + lane];
-      PT[lane] = flag[size_t(x) * L + lane];
+    for (int i = lane; i < cur; i += 64) {
+      A[i] = pool[size_t(x) * L + i];
+      AT[i] = flag[size_t(x) * L + i];
     }
     wave_sync();
     for (int base = c0; base < c1;) {
-      const uint64_t wk = cur >= L ? P[L - 1] : kKeyMax;
+      const uint64_t wk = cur >= L ? A[L - 1] : kKeyMax;
       int m = 0;
       while (base < c1 && m < 64) {
         const int i = base + lane;
@@ -1952,39 +1958,32 @@ __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK,
       wave_sync();  // every lane has read C before the sorted copy replaces it
       C[lane] = ck;
       CT[lane] = ct;
-      const uint64_t pk = lane < cur ? P[lane] : kKeyMax;
-      const uint32_t pt = lane < cur ? PT[lane] : 0xffffffffu;
       wave_sync();
-      int cpos = lane, ppos = lane;
       if (lane < m) {  // list entries at or before the candidate
         int lo = 0, hi = cur;
         while (lo < hi) {
           const int mid = (lo + hi) >> 1;
-          if (!kt_less(ck, ct, P[mid], PT[mid])) lo = mid + 1;
+          if (!kt_less(ck, ct, A[mid], AT[mid])) lo = mid + 1;
           else hi = mid;
         }
-        cpos += lo;
+        B[lane + lo] = ck;
+        BT[lane + lo] = ct;
       }
-      if (lane < cur) {  // candidates strictly before the list entry
+      for (int i = lane; i < cur; i += 64) {  // candidates strictly before the list entry
+        const uint64_t pk = A[i];
+        const uint32_t pt = AT[i];
         int lo = 0, hi = m;
         while (lo < hi) {
           const int mid = (lo + hi) >> 1;
           if (kt_less(C[mid], CT[mid], pk, pt)) lo = mid + 1;
           else hi = mid;
         }
-        ppos += lo;
-      }
-      wave_sync();  // the list is read before the merged window overwrites it
-      if (lane < cur) {
-        P[ppos] = pk;
-        PT[ppos] = pt;
-      }
-      if (lane < m) {
-        P[cpos] = ck;
-        PT[cpos] = ct;
+        B[i + lo] = pk;
+        BT[i + lo] = pt;
       }
       wave_sync();
-      // drop duplicates (the first of equal keys stays), compact to the front
+      // drop duplicates (the first of equal keys stays); the first L kept
+      // become the running list
       const int mm = cur + m;
       int kept = 0;
       for (int b0 = 0; b0 < mm && kept < L; b0 += 64) {
@@ -1993,31 +1992,30 @@ __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK,
         uint64_t kv = 0;
         uint32_t tv = 0;
         if (i < mm) {
-          kv = P[i];
-          tv = PT[i];
-          first = i == 0 || P[i - 1] != kv;
+          kv = B[i];
+          tv = BT[i];
+          first = i == 0 || B[i - 1] != kv;
         }
         const uint64_t fm = __ballot(first);
         const int at = kept + __popcll(fm & lt);
-        wave_sync();
-        if (first && at < L) {  // at <= i: only this lane's own slot or earlier ones are written
-          P[at] = kv;
-          PT[at] = tv;
+        if (first && at < L) {
+          A[at] = kv;
+          AT[at] = tv;
         }
-        wave_sync();
         kept += __popcll(fm);
       }
+      wave_sync();
       cur = kept < L ? kept : L;
     }
-    if (lane < L) {
-      if (lane < cur) {
-        const uint32_t tv = PT[lane];
-        pool[size_t(x) * L + lane] = P[lane];
-        flag[size_t(x) * L + lane] = (tv & 0x80000000u) ? 1 : uint8_t(tv & 1u);
-        if ((tv & 0xc0000000u) == 0x80000000u && lane < CK) credit[tv & 0x3fffffffu] = 1;
+    for (int i = lane; i < L; i += 64) {
+      if (i < cur) {
+        const uint32_t tv = AT[i];
+        pool[size_t(x) * L + i] = A[i];
+        flag[size_t(x) * L + i] = (tv & 0x80000000u) ? 1 : uint8_t(tv & 1u);
+        if ((tv & 0xc0000000u) == 0x80000000u && i < CK) credit[tv & 0x3fffffffu] = 1;
       } else {
-        pool[size_t(x) * L + lane] = kKeyMax;
-        flag[size_t(x) * L + lane] = 0;
+        pool[size_t(x) * L + i] = kKeyMax;
+        flag[size_t(x) * L + i] = 0;
       }
     }
     if (lane == 0) fill[x] = uint32_t(cur);
@@ -2543,9 +2541,12 @@ int gpu_refine_iter(efg_index* ix) {
     // one wavefront per target; the LDS window holds at least 2L entries
     const bool byid = ix->b_precision > 0;
     const unsigned grid = unsigned(std::min<size_t>((n + 3) / 4, size_t(ncu) * 8));
-    if (!byid && L <= 64)
-      k_merge_reg<<<grid, kBlock, 0, st>>>(n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p, bs.fill.p,
-                                           bs.found.p);
+    if (!byid && L <= 256)
+      (L <= 64 ? k_merge_reg<64> : k_merge_reg<256>)<<<grid, kBlock, 0, st>>>(
+          n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p, bs.fill.p, bs.found.p);
+    else if (!byid && L <= 512)
+      k_merge_reg<512><<<grid, kBlock, 0, st>>>(n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p, bs.fill.p,
+                                                bs.found.p);
     else if (2 * L <= 256)
       (byid ? k_merge_wave<256, true> : k_merge_wave<256, false>)<<<grid, kBlock, 0, st>>>(
           n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,

@@ -87,11 +87,11 @@ def test_build_stages_match_oracle(cuda):
     np.testing.assert_array_equal(g.graph_matrix(), o.graph_matrix())
 
 
-@pytest.mark.parametrize("L,K,check,S", [(12, 8, 12, 8), (80, 20, 40, 12)])
+@pytest.mark.parametrize("L,K,check,S", [(12, 8, 12, 8), (80, 20, 40, 12), (300, 20, 40, 12)])
 def test_build_pool_lengths_match_oracle(cuda, L, K, check, S):
     """Pool lengths on either side of the update kernels' lane groups (16
-    lanes per node for L = 12; 64 lanes and two passes for L = 80) and of the
-    register pool merge (L = 80 takes the LDS window merge)."""
+    lanes per node for L = 12; 64 lanes and several passes above 64) and of
+    the register pool merge's list buffers (64, 256 and 512 entries)."""
     base, _ = helpers.uniform(n=4000)
     g, o = _pair(base, trees=4, ml=5, epochs=5, check=check, L=L, K=K, S=S)
     g.build_stage(0)
