@@ -1884,8 +1884,11 @@ __device__ __forceinline__ bool kt_less(uint64_t ka, uint32_t ta, uint64_t kb, u
 // or before it (binary search), a list entry's place its index plus the
 // number of candidates before it.  Exactly equal entries put the list's
 // first.  Then duplicates are dropped and the first L copied back as the new
-// running list.  One wavefront per target.
-template <int kLMax>
+// running list.  One wavefront per target.  kById (precision form of the
+// build): duplicates are dropped by id, the first in (key, tag) order
+// staying, through an LDS hash of the window's ids holding the smallest
+// position of each (as in k_merge_wave).
+template <int kLMax, bool kById>
 __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK, const int32_t* __restrict__ toff,
                                                       const uint64_t* __restrict__ bkey,
                                                       const uint32_t* __restrict__ bsrc, uint64_t* __restrict__ pool,
@@ -1898,6 +1901,9 @@ __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK,
   __shared__ uint32_t ms_t[kW][kLMax + 64];
   __shared__ uint64_t cs_k[kW][64];  // candidates of the round
   __shared__ uint32_t cs_t[kW][64];
+  // id hash slots: at least 1.6 (kLMax + 64), a power of two
+  constexpr int kHS = kById ? (kLMax <= 64 ? 256 : (kLMax <= 256 ? 512 : 1024)) : 1;
+  __shared__ uint32_t hid_s[kW][kHS], hpos_s[kW][kHS];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint64_t* A = ls_k[w];
   uint32_t* AT = ls_t[w];
@@ -1982,9 +1988,29 @@ __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK,
         BT[i + lo] = pt;
       }
       wave_sync();
-      // drop duplicates (the first of equal keys stays); the first L kept
-      // become the running list
       const int mm = cur + m;
+      if (kById) {  // smallest position of every id of the window
+        uint32_t* hi_ = hid_s[w];
+        uint32_t* hp = hpos_s[w];
+        for (int i = lane; i < kHS; i += 64) {
+          hi_[i] = 0xffffffffu;
+          hp[i] = 0xffffffffu;
+        }
+        wave_sync();
+        for (int i = lane; i < mm; i += 64) {
+          const uint32_t id = key_id(B[i]);
+          uint32_t h = (id * 2654435761u) & uint32_t(kHS - 1);
+          for (;;) {
+            const uint32_t prev = atomicCAS(&hi_[h], 0xffffffffu, id);
+            if (prev == 0xffffffffu || prev == id) break;
+            h = (h + 1) & uint32_t(kHS - 1);
+          }
+          atomicMin(&hp[h], uint32_t(i));
+        }
+        wave_sync();
+      }
+      // drop duplicates (the first of equal keys, or of equal ids, stays);
+      // the first L kept become the running list
       int kept = 0;
       for (int b0 = 0; b0 < mm && kept < L; b0 += 64) {
         const int i = b0 + lane;
@@ -1994,7 +2020,14 @@ __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK,
         if (i < mm) {
           kv = B[i];
           tv = BT[i];
-          first = i == 0 || B[i - 1] != kv;
+          if (kById) {
+            const uint32_t id = key_id(kv);
+            uint32_t h = (id * 2654435761u) & uint32_t(kHS - 1);
+            while (hid_s[w][h] != id) h = (h + 1) & uint32_t(kHS - 1);
+            first = hpos_s[w][h] == uint32_t(i);
+          } else {
+            first = i == 0 || B[i - 1] != kv;
+          }
         }
         const uint64_t fm = __ballot(first);
         const int at = kept + __popcll(fm & lt);
@@ -2541,12 +2574,11 @@ int gpu_refine_iter(efg_index* ix) {
     // one wavefront per target; the LDS window holds at least 2L entries
     const bool byid = ix->b_precision > 0;
     const unsigned grid = unsigned(std::min<size_t>((n + 3) / 4, size_t(ncu) * 8));
-    if (!byid && L <= 256)
-      (L <= 64 ? k_merge_reg<64> : k_merge_reg<256>)<<<grid, kBlock, 0, st>>>(
+    if (L <= 512)
+      (L <= 64 ? (byid ? k_merge_reg<64, true> : k_merge_reg<64, false>)
+               : L <= 256 ? (byid ? k_merge_reg<256, true> : k_merge_reg<256, false>)
+                          : (byid ? k_merge_reg<512, true> : k_merge_reg<512, false>))<<<grid, kBlock, 0, st>>>(
           n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p, bs.fill.p, bs.found.p);
-    else if (!byid && L <= 512)
-      k_merge_reg<512><<<grid, kBlock, 0, st>>>(n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p, bs.fill.p,
-                                                bs.found.p);
     else if (2 * L <= 256)
       (byid ? k_merge_wave<256, true> : k_merge_wave<256, false>)<<<grid, kBlock, 0, st>>>(
           n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
