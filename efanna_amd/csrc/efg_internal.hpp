@@ -114,12 +114,15 @@ struct BuildState {
     DevBuf<int32_t> toff, sums, scal, tcnt, tskoff;
     DevBuf<unsigned long long> ctr;
     DevBuf<uint32_t> fnew, fold, rcn, rco, rcurn, rcuro, rszn, rszo, ridn, rido;
-    DevBuf<int32_t> roffn, roffo, snew, sold, onew, oold;
+    DevBuf<int32_t> roffn, roffo, snew, sold, onew, oold, bign, bigo;
+    DevBuf<int> nbig;
     void release() {
       for (DevBuf<uint32_t>* b : {&tcount, &cursor, &fnew, &fold, &rcn, &rco, &rcurn, &rcuro, &rszn, &rszo, &ridn, &rido})
         b->reset();
-      for (DevBuf<int32_t>* b : {&toff, &sums, &scal, &tcnt, &tskoff, &roffn, &roffo, &snew, &sold, &onew, &oold})
+      for (DevBuf<int32_t>* b :
+           {&toff, &sums, &scal, &tcnt, &tskoff, &roffn, &roffo, &snew, &sold, &onew, &oold, &bign, &bigo})
         b->reset();
+      nbig.reset();
       worst.reset();
       ctr.reset();
     }

@@ -2146,22 +2146,35 @@ __global__ void k_lists(const uint64_t* __restrict__ pool, const uint8_t* __rest
   }
 }
 
-// Reverse lists above `cap` keep the `cap` members with the smallest
-// generator value (hash of iteration, owner, member); one workgroup per list.
-__global__ void __launch_bounds__(kBlock) k_sample(uint32_t n, const int32_t* __restrict__ roff,
-                                                   uint32_t* __restrict__ rids, uint32_t* __restrict__ rsize, int cap,
-                                                   uint64_t seed, uint64_t stream, uint64_t iter, int chunk) {
+// Reverse-list sizes after sampling (min(size, cap)); the lists above cap
+// are appended to big[] for k_sample.  One thread per node, both kinds.
+__global__ void k_rev_sizes(uint32_t n, const int32_t* __restrict__ roffn, const int32_t* __restrict__ roffo, int cap,
+                            uint32_t* __restrict__ rszn, uint32_t* __restrict__ rszo, int32_t* __restrict__ bign,
+                            int32_t* __restrict__ bigo, int* __restrict__ nbig) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int c = roffn[p + 1] - roffn[p];
+  rszn[p] = uint32_t(c < cap ? c : cap);
+  if (c > cap) bign[atomicAdd(&nbig[0], 1)] = int32_t(p);
+  c = roffo[p + 1] - roffo[p];
+  rszo[p] = uint32_t(c < cap ? c : cap);
+  if (c > cap) bigo[atomicAdd(&nbig[1], 1)] = int32_t(p);
+}
+
+// Reverse lists above `cap` (listed in big[0, *nbig)) keep the `cap` members
+// with the smallest generator value (hash of iteration, owner, member); one
+// workgroup per list.
+__global__ void __launch_bounds__(kBlock) k_sample(const int32_t* __restrict__ big, const int* __restrict__ nbig,
+                                                   const int32_t* __restrict__ roff, uint32_t* __restrict__ rids,
+                                                   int cap, uint64_t seed, uint64_t stream, uint64_t iter, int chunk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   uint64_t* k = reinterpret_cast<uint64_t*>(smem_raw);
   uint32_t* v = reinterpret_cast<uint32_t*>(k + chunk);
   const int tid = threadIdx.x;
-  for (uint32_t o = blockIdx.x; o < n; o += gridDim.x) {
+  const int nb = *nbig;
+  for (int bi = blockIdx.x; bi < nb; bi += gridDim.x) {
+    const uint32_t o = uint32_t(big[bi]);
     const int c0 = roff[o], c1 = roff[o + 1];
-    const int c = c1 - c0;
-    if (c <= cap) {
-      if (tid == 0) rsize[o] = uint32_t(c);
-      continue;
-    }
     int cur = 0;
     for (int base = c0; base < c1;) {
       int take = c1 - base;
@@ -2202,7 +2215,6 @@ __global__ void __launch_bounds__(kBlock) k_sample(uint32_t n, const int32_t* __
       cur = cap;
     }
     for (int i = tid; i < cap; i += kBlock) rids[c0 + i] = v[i];
-    if (tid == 0) rsize[o] = uint32_t(cap);
     __syncthreads();
   }
 }
@@ -2637,11 +2649,16 @@ int gpu_refine_iter(efg_index* ix) {
   const size_t lds_s = size_t(schunk) * 12;
   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sample), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_s));
   {
+    if (!sc.bign.ensure(n) || !sc.bigo.ensure(n) || !sc.nbig.ensure(2))
+      return fail(EFG_ERR_MEMORY, "out of device memory (update)");
+    hipMemsetAsync(sc.nbig.p, 0, 2 * sizeof(int), st);
+    k_rev_sizes<<<grid_for(n), 256, 0, st>>>(n, roffn.p, roffo.p, rcap, rszn.p, rszo.p, sc.bign.p, sc.bigo.p,
+                                             sc.nbig.p);
     unsigned grid = unsigned(std::min<size_t>(n, size_t(ncu) * 32));
-    k_sample<<<grid, kBlock, lds_s, st>>>(n, roffn.p, ridn.p, rszn.p, rcap, p.seed, ST_RNN_NEW, uint64_t(bs.iter),
-                                          schunk);
-    k_sample<<<grid, kBlock, lds_s, st>>>(n, roffo.p, rido.p, rszo.p, rcap, p.seed, ST_RNN_OLD, uint64_t(bs.iter),
-                                          schunk);
+    k_sample<<<grid, kBlock, lds_s, st>>>(sc.bign.p, sc.nbig.p, roffn.p, ridn.p, rcap, p.seed, ST_RNN_NEW,
+                                          uint64_t(bs.iter), schunk);
+    k_sample<<<grid, kBlock, lds_s, st>>>(sc.bigo.p, sc.nbig.p + 1, roffo.p, rido.p, rcap, p.seed, ST_RNN_OLD,
+                                          uint64_t(bs.iter), schunk);
   }
   k_list_sizes<<<grid_for(n), 256, 0, st>>>(n, fnew.p, rszn.p, fold.p, rszo.p, snew.p, sold.p);
   if (int rc = device_exclusive_scan(snew.p, int(n), onew.p, scal.p, sums, st)) return rc;
