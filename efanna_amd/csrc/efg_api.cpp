@@ -504,6 +504,13 @@ int efg_set_build_precision(efg_index* ix, int precision) {
   return EFG_OK;
 }
 
+int efg_set_join_staging_margin(efg_index* ix, double margin) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (!(margin > 0.0)) return fail(EFG_ERR_ARG, "staging margin must be positive");
+  ix->b_stage_margin = margin;
+  return EFG_OK;
+}
+
 int efg_knn_search_batch_device(efg_index* ix, const float* d_queries, size_t nq, size_t q_stride, const int* pool,
                                 const int* extend, const int* k, int out_stride, int32_t* d_ids, float* d_dists,
                                 int tune, void* stream) {

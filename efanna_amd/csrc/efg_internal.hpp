@@ -129,6 +129,7 @@ struct BuildState {
   };
   IterScratch it;
   int iter = 0;
+  double keep_frac = 1.0;  // join candidates kept / 2 x pair bound, last iteration (sizes the staging)
   bool ready = false;
 };
 
@@ -158,6 +159,7 @@ struct efg_index {
   efg::DevBuf<uint16_t> data_bf16;  // BF16 copy, lane-split rows (precision form)
   int s_precision = 0;              // 0 FP32 rows, 1 BF16 rows + FP32 re-rank
   int b_precision = 0;              // build: 0 FP32 tiles, 1 BF16 / 2 FP16 MFMA tiles + FP32 re-rank
+  double b_stage_margin = 1.25;     // join staging: margin over the last iteration's kept fraction
   efg::DevBuf<uint16_t> split_hi, split_lo;  // build precision form: rows as hi + lo 16-bit parts (n x Dk)
   efg::DevBuf<float> split_norm;             // FP32 squared norms of the rows
   int split_Dk = 0, split_kind = 0;

@@ -2465,9 +2465,11 @@ int gpu_merge_init(efg_index* ix) {
   bs.new_n = size_t(n) * 2 * S;
   bs.old_n = 0;
   bs.iter = 0;
+  bs.keep_frac = 1.0;
   bs.ready = true;
   ix->counters[3] = 0;
   ix->counters[6] = 0;
+  ix->counters[7] = 0;
   ix->join_pairs.clear();
   if (err_depth < p.merge_level) {
     char buf[160];
@@ -2518,17 +2520,19 @@ int gpu_refine_iter(efg_index* ix) {
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   // eight task ranges, one per XCD: a multiple of eight workgroups
   const size_t join_grid = size_t((ncu * 4 + 7) / 8) * 8;
-  // two candidates per pair at most, plus one partly used run per workgroup
-  // (per wavefront on the MFMA path, ncu * 8 workgroups of four, two kernels)
-  const size_t cap16 = size_t(2 * bound) + 2 * size_t(ncu) * 8 * (kBlock / 64) * kStageRun + 1024;
   const size_t node_grid = size_t(ncu) * 8;  // whole-node FP32 kernel: workgroups of four wavefronts
-  const size_t cap = ix->b_precision > 0
-                         ? cap16
-                         : size_t(2 * bound) + (join_grid + node_grid * (kBlock / 64)) * kStageRun + 1024;
+  // staging: the kept candidates (two per pair at most) plus one partly used
+  // run per workgroup (per wavefront on the MFMA path, ncu * 8 workgroups of
+  // four, two kernels).  It is sized from the last iteration's kept fraction
+  // with a margin; when the candidates do not fit, the join kernels (which do
+  // not change the pools) run again with room for every reservation counted.
+  const size_t slack = ix->b_precision > 0 ? 2 * size_t(ncu) * 8 * (kBlock / 64) * kStageRun + 1024
+                                           : (join_grid + node_grid * (kBlock / 64)) * kStageRun + 1024;
+  const size_t full = size_t(2 * bound) + slack;
+  size_t cap = std::min(full, size_t(double(2 * bound) * std::min(1.0, ix->b_stage_margin * bs.keep_frac)) + slack);
   if (!bs.jhdr.ensure(size_t(ntask) + 1) || !bs.jtids.ensure((size_t(ntask) + 1) * 64) ||
       !bs.jstage.ensure(cap * sizeof(Stage)))
     return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
-  Stage* stage = reinterpret_cast<Stage*>(bs.jstage.p);
   k_join_tasks<<<unsigned((size_t(n) * 64 + 255) / 256), 256, 0, st>>>(n, order, bs.new_off.p, bs.new_ids.p,
                                                                       bs.old_off.p, bs.old_ids.p, node_rows,
                                                                       tskoff.p, bs.jhdr.p, bs.jtids.p);
@@ -2538,36 +2542,45 @@ int gpu_refine_iter(efg_index* ix) {
   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_join_tiles<4>), hipFuncAttributeMaxDynamicSharedMemorySize,
                       int(lds_join));
   unsigned long long h_ctr[2] = {0, 0};
-  hipMemsetAsync(ctr.p, 0, 16, st);
-  hipMemsetAsync(tcount.p, 0, size_t(n) * 4, st);
-  if (ix->b_precision > 0) {
-    // precision form on MFMA: one wavefront per node of at most 64 rows,
-    // then one per block task of the larger nodes
-    if (int rc = ensure_split(ix, st)) return rc;
-    const unsigned g16 = unsigned(size_t(ncu) * 8);
-    const bool f16 = ix->b_precision == 2;
-    (f16 ? k_join_node_mfma<true> : k_join_node_mfma<false>)<<<g16, kBlock, 0, st>>>(
-        ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, n, order, bs.new_off.p, bs.new_ids.p,
-        bs.old_off.p, bs.old_ids.p, worst.p, stage, (unsigned long long)cap16, ctr.p, tcount.p, ctr.p + 1);
-    if (ntask)
-      (f16 ? k_join_mfma<true> : k_join_mfma<false>)<<<g16, kBlock, 0, st>>>(
-          ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, uint32_t(ntask), bs.jhdr.p, bs.jtids.p,
-          worst.p, stage, (unsigned long long)cap16, ctr.p, tcount.p, ctr.p + 1);
-  } else {
-    k_join_node_f32<<<unsigned(node_grid), kBlock, 0, st>>>(ix->data.p, D, n, order, bs.new_off.p, bs.new_ids.p,
-                                                            bs.old_off.p, bs.old_ids.p, worst.p, stage,
-                                                            (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
-    if (ntask)
-      k_join_tiles<4><<<unsigned(join_grid), kBlock, lds_join, st>>>(ix->data.p, D, uint32_t(ntask), bs.jhdr.p,
-                                                                     bs.jtids.p, worst.p, stage,
-                                                                     (unsigned long long)cap, ctr.p, tcount.p,
-                                                                     ctr.p + 1);
+  for (int attempt = 0;; ++attempt) {
+    Stage* stage = reinterpret_cast<Stage*>(bs.jstage.p);
+    hipMemsetAsync(ctr.p, 0, 16, st);
+    hipMemsetAsync(tcount.p, 0, size_t(n) * 4, st);
+    if (ix->b_precision > 0) {
+      // precision form on MFMA: one wavefront per node of at most 64 rows,
+      // then one per block task of the larger nodes
+      if (int rc = ensure_split(ix, st)) return rc;
+      const unsigned g16 = unsigned(size_t(ncu) * 8);
+      const bool f16 = ix->b_precision == 2;
+      (f16 ? k_join_node_mfma<true> : k_join_node_mfma<false>)<<<g16, kBlock, 0, st>>>(
+          ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, n, order, bs.new_off.p, bs.new_ids.p,
+          bs.old_off.p, bs.old_ids.p, worst.p, stage, (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
+      if (ntask)
+        (f16 ? k_join_mfma<true> : k_join_mfma<false>)<<<g16, kBlock, 0, st>>>(
+            ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, uint32_t(ntask), bs.jhdr.p, bs.jtids.p,
+            worst.p, stage, (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
+    } else {
+      k_join_node_f32<<<unsigned(node_grid), kBlock, 0, st>>>(ix->data.p, D, n, order, bs.new_off.p, bs.new_ids.p,
+                                                              bs.old_off.p, bs.old_ids.p, worst.p, stage,
+                                                              (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
+      if (ntask)
+        k_join_tiles<4><<<unsigned(join_grid), kBlock, lds_join, st>>>(ix->data.p, D, uint32_t(ntask), bs.jhdr.p,
+                                                                       bs.jtids.p, worst.p, stage,
+                                                                       (unsigned long long)cap, ctr.p, tcount.p,
+                                                                       ctr.p + 1);
+    }
+    hipMemcpyAsync(h_ctr, ctr.p, 16, hipMemcpyDeviceToHost, st);
+    if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
+    // every candidate reserved a slot, written or not: a second run fits in
+    // the reservations counted plus one partly used run per workgroup
+    if (h_ctr[0] <= cap) break;
+    if (attempt > 0 || cap >= full) return fail(EFG_ERR_STATE, "join staging overflow");
+    cap = std::min(full, size_t(h_ctr[0]) + slack);
+    ix->counters[7] += 1;
+    if (!bs.jstage.ensure(cap * sizeof(Stage))) return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
   }
-  hipMemcpyAsync(h_ctr, ctr.p, 16, hipMemcpyDeviceToHost, st);
-  if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
   const unsigned long long nst = h_ctr[0];
-  // the capacity covers every candidate and every partly used run
-  if (nst > cap) return fail(EFG_ERR_STATE, "join staging overflow");
+  Stage* stage = reinterpret_cast<Stage*>(bs.jstage.p);
   ix->join_pairs.push_back(h_ctr[1]);
   ix->counters[3] += h_ctr[1];
   if (nst >= (1ull << 31)) return fail(EFG_ERR_STATE, "too many join candidates for one iteration");
@@ -2575,6 +2588,7 @@ int gpu_refine_iter(efg_index* ix) {
   int nkept = 0;
   hipMemcpyAsync(&nkept, scal.p, 4, hipMemcpyDeviceToHost, st);
   if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
+  bs.keep_frac = bound ? double(nkept) / double(2 * bound) : 1.0;
   if (!bs.jbkey.ensure(size_t(nkept) + 1) || !bs.jbsrc.ensure(size_t(nkept) + 1))
     return fail(EFG_ERR_MEMORY, "out of device memory (buckets)");
   uint64_t* bkey = bs.jbkey.p;

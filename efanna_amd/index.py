@@ -198,6 +198,10 @@ class FIndex:
         """0 = FP32 tiles; 1 = BF16 / 2 = FP16 MFMA merge and join tiles, FP32 re-rank of sets and pools."""
         check(lib().efg_set_build_precision(self.h, int(precision)))
 
+    def setJoinStagingMargin(self, margin: float):
+        """Join staging room: margin x the last iteration's kept candidate fraction (default 1.25)."""
+        check(lib().efg_set_join_staging_margin(self.h, float(margin)))
+
     def setSearchPrecision(self, precision: int):
         """0 = FP32 rows; 1 = BF16 rows with FP32 re-rank of the best 2k."""
         check(lib().efg_set_search_precision(self.h, int(precision)))
@@ -390,7 +394,7 @@ class FIndex:
         c = np.zeros(8, np.uint64)
         check(lib().efg_get_counters(self.h, _p(c, _lib.c_u64p), 8))
         return dict(search_dist=int(c[0]), merge=int(c[1]), init=int(c[2]), join=int(c[3]), overflow=int(c[4]),
-                    overflow2=int(c[5]), join_rows=int(c[6]))
+                    overflow2=int(c[5]), join_rows=int(c[6]), join_reruns=int(c[7]))
 
     def join_pairs(self):
         out = np.zeros(64, np.uint64)

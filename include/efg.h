@@ -144,6 +144,14 @@ int efg_set_search_precision(efg_index* ix, int precision);
  * d <= 128) the build equals the FP32 build.  Applies to later builds. */
 int efg_set_build_precision(efg_index* ix, int precision);
 
+/* Join staging of the build (no counterpart in the reference; sizing only):
+ * each NN-descent iteration reserves room for `margin` times the fraction of
+ * candidates the previous iteration kept (at most every pair's two), and
+ * joins again with room for all of them when they do not fit (counter 7),
+ * besides one partly used run of 4096 slots per wavefront.  margin > 0;
+ * default 1.25.  Results do not depend on it. */
+int efg_set_join_staging_margin(efg_index* ix, double margin);
+
 /* Learned stopping (no counterpart in the reference; a form beside the plain
  * search).  A small network (16 pool-state features -> 64 -> 64 -> 1 logit,
  * ReLU, 5,313 parameters laid out W1[16][64] | b1[64] | W2[64][64] | b2[64] |
@@ -270,7 +278,9 @@ int efg_set_graph(efg_index* ix, const uint64_t* off, const uint32_t* ids);
 /* Device pointer of the padded base rows and their stride (floats). */
 int efg_device_data(const efg_index* ix, const float** d_data, size_t* stride);
 /* Counters: [0] distances in the last search, [1] merge distances, [2] init
- * distances, [3] join distances (pairs), [4] last search overflow queries. */
+ * distances, [3] join distances (pairs), [4] last search overflow queries,
+ * [5] their second overflow, [6] join rows, [7] join iterations whose
+ * candidates outgrew the staging estimate and were joined again. */
 int efg_get_counters(const efg_index* ix, uint64_t* out, int n);
 /* Per-iteration join pair counts of the last build. */
 int efg_get_join_pairs(const efg_index* ix, uint64_t* out, int cap, int* n);

@@ -109,6 +109,27 @@ def test_build_pool_lengths_match_oracle(cuda, L, K, check, S):
     np.testing.assert_array_equal(g.graph_matrix(), o.graph_matrix())
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_build_join_staging_reruns_keep_results(cuda, precision):
+    """A staging margin far below the kept candidate fraction makes the
+    larger iterations join again with room for every candidate (the staging
+    always has one 4096-slot run per wavefront besides, so this takes a few
+    hundred thousand rows).  Graph and join pairs equal the default build's
+    (the FP32 build equals the oracle in the tests above; the precision form
+    equals it on integer rows)."""
+    base, _ = helpers.mixture(n=300_000, nq=1)
+    g, _o = _pair(base, trees=8, ml=8, epochs=6)
+    g.buildIndex()
+    h, _o2 = _pair(base, trees=8, ml=8, epochs=6)
+    h.setBuildPrecision(precision)
+    h.setJoinStagingMargin(0.01)
+    h.buildIndex()
+    assert g.counters()["join_reruns"] == 0
+    assert h.counters()["join_reruns"] > 0
+    np.testing.assert_array_equal(h.graph_matrix(), g.graph_matrix())
+    assert h.join_pairs() == g.join_pairs()
+
+
 @pytest.mark.parametrize("d", [100, 200])
 def test_build_float_data_long_lists_matches_oracle(cuda, d):
     # non-integer rows (every FMA rounds), d % 32 != 0 (tail chunk first),
