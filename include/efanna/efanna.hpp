@@ -29,7 +29,13 @@ class FIndex {
 
  public:
   FIndex(const Matrix<DataType>& features, Distance<DataType>* d, const IndexParams& params) : params_(params) {
-    (void)d;  // every L2 variant maps to the device's squared L2
+    // The four L2 classes (L2Distance, L2DistanceSSE, L2DistanceAVX,
+    // L2DistanceAVXr4) map to the device's squared L2; any other distance
+    // has no device kernel and is refused.
+    if (dynamic_cast<L2DistanceAVX<DataType>*>(d) == nullptr) {
+      status_ = EFG_ERR_ARG;
+      throw std::invalid_argument("unsupported distance: the MI355X library computes squared L2 only");
+    }
     efg_build_params bp;
     int trees = extra(params, "trees", 4);
     efg_default_build_params(&bp, trees);

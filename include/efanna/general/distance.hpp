@@ -8,6 +8,13 @@
 #include <cmath>
 #include <cstddef>
 
+// Allocation alignment for row buffers handed to Matrix (reference:
+// general/distance.hpp:10-20, used by the samples' memalign calls).  Rows
+// here always have the round8 stride, so 32-byte alignment suits every build.
+#ifndef KGRAPH_MATRIX_ALIGN
+#define KGRAPH_MATRIX_ALIGN 32
+#endif
+
 namespace efanna {
 
 template <typename T>

@@ -45,6 +45,12 @@ def lib():
             "orc_new": (ctypes.c_void_p, [_f32p, _sz, _sz, _sz]),
             "orc_free": (None, [ctypes.c_void_p]),
             "orc_set_params": (None, [ctypes.c_void_p] + [ctypes.c_int] * 9 + [ctypes.c_uint64, ctypes.c_int]),
+            "orc_gen_random": (None, [ctypes.c_uint64, ctypes.c_uint32, _u32p, ctypes.c_uint, ctypes.c_uint]),
+            "orc_keep_rev": (None, [ctypes.c_void_p, ctypes.c_int]),
+            "orc_rev_total": (_sz, [ctypes.c_void_p, ctypes.c_int]),
+            "orc_get_rev": (None, [ctypes.c_void_p, ctypes.c_int, _u64p, _u32p]),
+            "orc_search_seeds": (ctypes.c_int, [ctypes.c_void_p, _f32p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                                _u32p]),
             "orc_build_trees": (ctypes.c_int, [ctypes.c_void_p]),
             "orc_merge_init": (ctypes.c_int, [ctypes.c_void_p]),
             "orc_refine_iter": (ctypes.c_int, [ctypes.c_void_p]),
@@ -127,6 +133,13 @@ def l2(a: np.ndarray, b: np.ndarray) -> float:
     aa[:d] = a
     bb[:d] = b
     return float(lib().orc_l2(_p(aa, _f32p), _p(bb, _f32p), d))
+
+
+def gen_random(seed, node, size, N):
+    """GenRandom (kdtreeub_index.hpp:1121-1135) with the oracle's generator."""
+    out = np.empty(size, np.uint32)
+    lib().orc_gen_random(seed, node, _p(out, _u32p), size, N)
+    return out
 
 
 def insert_into_knn(ids, dists, K, nid, ndist):
@@ -257,6 +270,25 @@ class Index:
         ids = np.empty(tot, np.uint32)
         lib().orc_get_lists(self.h, which, _p(off, _u64p), _p(ids, _u32p))
         return off, ids
+
+    def keep_reverse_lists(self, on=True):
+        """Record the reverse lists of each update before and after the cap."""
+        lib().orc_keep_rev(self.h, int(on))
+
+    def reverse_lists(self, which):
+        """which: 0 new before sampling, 1 old before, 2 new after, 3 old after (CSR, each row sorted)."""
+        tot = lib().orc_rev_total(self.h, which)
+        off = np.empty(self.n + 1, np.uint64)
+        ids = np.empty(tot, np.uint32)
+        lib().orc_get_rev(self.h, which, _p(off, _u64p), _p(ids, _u32p))
+        return off, ids
+
+    def search_seeds(self, query, qid, init_num, search_trees):
+        """Seed pool of one query (method 0) in sequence order."""
+        q = np.ascontiguousarray(query, np.float32)
+        out = np.empty(init_num, np.uint32)
+        _check(lib().orc_search_seeds(self.h, _p(q, _f32p), qid, init_num, search_trees, _p(out, _u32p)))
+        return out
 
     def found(self):
         f = np.empty(self.n, np.uint8)

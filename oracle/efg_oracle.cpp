@@ -201,6 +201,10 @@ struct Index {
   // counters
   uint64_t n_dist_merge = 0, n_dist_init = 0, n_dist_join = 0, n_pairs_join = 0;
   std::vector<uint64_t> join_pairs_per_iter;
+  // test hook: reverse lists of the last update before ([0] new, [1] old)
+  // and after sampling ([2] new, [3] old), each sorted
+  bool keep_rev = false;
+  std::vector<std::vector<uint32_t>> rev[4];
 
   const float* row(size_t i) const { return &data[i * stride]; }
 };
@@ -670,6 +674,12 @@ void update(Index& ix, unsigned paramL) {
     v.resize(paramL);
     for (unsigned i = 0; i < paramL; ++i) v[i] = k[i].second;
   };
+  if (ix.keep_rev) {
+    ix.rev[0] = rnew;
+    ix.rev[1] = rold;
+    for (int w = 0; w < 2; ++w)
+      for (auto& v : ix.rev[w]) std::sort(v.begin(), v.end());
+  }
 #pragma omp parallel for schedule(dynamic, 256)
   for (long long ii = 0; ii < (long long)n; ++ii) {
     uint32_t o = uint32_t(ii);
@@ -679,6 +689,10 @@ void update(Index& ix, unsigned paramL) {
     sample(rold[o], ST_RNN_OLD, o);
     std::sort(rold[o].begin(), rold[o].end());
     ix.nn_old[o].insert(ix.nn_old[o].end(), rold[o].begin(), rold[o].end());
+  }
+  if (ix.keep_rev) {
+    ix.rev[2] = rnew;
+    ix.rev[3] = rold;
   }
 }
 
@@ -1218,6 +1232,29 @@ void orc_set_params(void* h, int trees, int trees_build, int ml, int epochs, int
   ix->p.join_mode = join_mode;
 }
 
+// Test hooks for the random-fill branches: GenRandom (kdtreeub_index.hpp:
+// 1121-1135), the search seed pool with its top-up (387-417), and the
+// reverse lists around the cap of 100 (base_index.hpp:421-436).
+void orc_gen_random(uint64_t seed, uint32_t node, uint32_t* out, unsigned size, unsigned N) {
+  gen_random(seed, node, out, size, N);
+}
+void orc_keep_rev(void* h, int on) { static_cast<Index*>(h)->keep_rev = on != 0; }
+// which: 0 new before sampling, 1 old before, 2 new after, 3 old after.
+size_t orc_rev_total(void* h, int which) {
+  size_t t = 0;
+  for (auto& v : static_cast<Index*>(h)->rev[which & 3]) t += v.size();
+  return t;
+}
+void orc_get_rev(void* h, int which, uint64_t* off, uint32_t* ids) {
+  Index* ix = static_cast<Index*>(h);
+  const auto& r = ix->rev[which & 3];
+  off[0] = 0;
+  for (size_t i = 0; i < r.size(); ++i) {
+    std::memcpy(ids + off[i], r[i].data(), r[i].size() * 4);
+    off[i + 1] = off[i] + r[i].size();
+  }
+}
+
 int orc_build_trees(void* h) { return build_trees(*static_cast<Index*>(h)); }
 int orc_merge_init(void* h) { return merge_and_init(*static_cast<Index*>(h)); }
 // One refine iteration (join + update(100)): base_index.hpp:443-448.
@@ -1488,6 +1525,21 @@ int orc_hash_search(void* h, const float* q, size_t nq, size_t qstride, int k, i
     g_err = "no such searching method";
     return -5;
   }
+  return 0;
+}
+
+// Seed pool of one query in sequence order (method 0), init_num ids.
+int orc_search_seeds(void* h, const float* q, uint64_t qid, int init_num, int search_trees, uint32_t* out) {
+  Index* ix = static_cast<Index*>(h);
+  if (search_trees <= 0 || search_trees > ix->T) { g_err = "wrong tree number"; return -4; }
+  if (init_num <= 0 || init_num > int(ix->n)) { g_err = "pool larger than the data set"; return -2; }
+  std::vector<float> qpad((ix->d + 7) & ~size_t(7), 0.f);
+  std::memcpy(qpad.data(), q, ix->d * sizeof(float));
+  std::vector<uint8_t> visited(ix->n, 0);
+  std::vector<uint32_t> pool;
+  unsigned lsize = unsigned(init_num * 2 / (5 * search_trees) + 1);
+  seed_pool(*ix, qpad.data(), lsize, init_num, search_trees, qid, visited, pool);
+  std::memcpy(out, pool.data(), pool.size() * 4);
   return 0;
 }
 

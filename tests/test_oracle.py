@@ -436,8 +436,9 @@ def test_graph_recall_matches_reference_10k():
         o.build_index()
         g = o.graph_matrix()
         rec[mode] = helpers.recall(g, gt, 10)
-    assert 0.889 <= rec[1] <= 0.901, rec
-    assert abs(rec[0] - 0.895) <= 0.0075, rec
+    # both forms within 0.005 of the reference's midpoint 0.895
+    assert abs(rec[1] - 0.895) <= 0.005, rec
+    assert abs(rec[0] - 0.895) <= 0.005, rec
     assert abs(rec[0] - rec[1]) <= 0.005, rec
 
 
