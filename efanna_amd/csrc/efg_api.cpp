@@ -1007,7 +1007,7 @@ int efg_get_counters(const efg_index* ix, uint64_t* out, int n) {
     unsigned long long nd = 0;
     int nv[2] = {0, 0};
     hipMemcpy(&nd, ix->ws.ndist.p, 8, hipMemcpyDeviceToHost);
-    if (ix->ws.nover.p) hipMemcpy(nv, ix->ws.nover.p, 8, hipMemcpyDeviceToHost);
+    if (ix->ws.nover_sum.p) hipMemcpy(nv, ix->ws.nover_sum.p, 8, hipMemcpyDeviceToHost);
     c[0] = nd;
     c[4] = uint64_t(nv[0]);
     c[5] = uint64_t(nv[1]);

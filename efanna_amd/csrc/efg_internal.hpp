@@ -137,6 +137,7 @@ struct Workspace {
   DevBuf<int32_t> leaves, overflow, qorder, qkey, qoff, scan_sums;
   DevBuf<uint32_t> qhist;
   DevBuf<int> nover;
+  DevBuf<int> nover_sum;  // [overflow queries, second overflows] summed over the passes of one search or batch
   DevBuf<unsigned long long> ndist;
   DevBuf<uint32_t> gtab;
   DevBuf<float> qbuf;

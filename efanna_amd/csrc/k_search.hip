@@ -1133,6 +1133,13 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       acc_nd += ndq;
     } else if (ovf) {
       if (kStop == 0 && tid == 0) overflow[atomicAdd(noverflow, 1)] = qi;
+      if (kGlobal) {
+        // outgrew the largest global table as well: no result, the row says so
+        for (int j = tid; j < c.K; j += kThreads) {
+          out_ids[size_t(qi) * c.out_stride + j] = -1;
+          if (out_d) out_d[size_t(qi) * c.out_stride + j] = 3.402823466e38f;
+        }
+      }
     } else {
       for (int j = tid; j < c.K; j += kThreads) {
         if (kMethod != 1) {
@@ -1322,6 +1329,11 @@ __global__ void k_zero_i32(int* p, int m) {
   if (i < m) p[i] = 0;
 }
 
+__global__ void k_add_i32(int* __restrict__ acc, const int* __restrict__ v, int m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) acc[i] += v[i];
+}
+
 // (The pass clears its counters with a kernel, not a memset, so that a
 // captured batch replays correctly.)
 int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const LearnArgs& la, bool bf16,
@@ -1330,8 +1342,8 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
   unsigned long long* work = count_work ? ix->work.p : nullptr;
   const int nb_q = 4096;
   if (!w.leaves.ensure(nq * size_t(c.nleaf)) || !w.overflow.ensure(2 * nq) || !w.nover.ensure(2) ||
-      !w.ndist.ensure(3) || !w.qorder.ensure(nq) || !w.qkey.ensure(nq) || !w.qhist.ensure(nb_q + 1) ||
-      !w.qoff.ensure(nb_q + 1))
+      !w.nover_sum.ensure(2) || !w.ndist.ensure(3) || !w.qorder.ensure(nq) || !w.qkey.ensure(nq) ||
+      !w.qhist.ensure(nb_q + 1) || !w.qoff.ensure(nb_q + 1))
     return fail(EFG_ERR_MEMORY, "out of device memory (search workspace)");
   size_t lds = search_lds_bytes(c, false);
   const size_t lds_cap = 150 * 1024;
@@ -1436,6 +1448,7 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
     }
   }
   if (ev) hipEventRecord(ev[3], st);
+  k_add_i32<<<1, 64, 0, st>>>(w.nover_sum.p, w.nover.p, 2);
   return check_hip(hipGetLastError(), "search launch");
 }
 
@@ -1491,8 +1504,10 @@ int search_prepare(efg_index* ix, size_t nq, size_t q_stride, hipStream_t st) {
     if (int rc = build_search_layout(ix)) return rc;
   }
   efg::Workspace& w = ix->ws;
-  if (!w.ndist.ensure(3)) return fail(EFG_ERR_MEMORY, "out of device memory (search workspace)");
+  if (!w.ndist.ensure(3) || !w.nover_sum.ensure(2))
+    return fail(EFG_ERR_MEMORY, "out of device memory (search workspace)");
   if (int rc = check_hip(hipMemsetAsync(w.ndist.p, 0, 3 * sizeof(unsigned long long), st), "memset")) return rc;
+  k_zero_i32<<<1, 64, 0, st>>>(w.nover_sum.p, 2);
   if (!ix->work.p) {
     if (!ix->work.ensure(3)) return fail(EFG_ERR_MEMORY, "out of device memory (counters)");
     if (int rc = check_hip(hipMemsetAsync(ix->work.p, 0, 3 * sizeof(unsigned long long), st), "memset")) return rc;
@@ -1760,6 +1775,8 @@ int batch_launch(efg_index* ix, efg::Workspace& w, const BatchSetup& b, const fl
                  int32_t* d_ids, float* d_dists, hipStream_t st) {
   const size_t m = b.nq * size_t(b.out_stride);
   k_fill_results<<<unsigned((m + 255) / 256), 256, 0, st>>>(d_ids, d_dists, m);
+  if (!w.nover_sum.ensure(2)) return fail(EFG_ERR_MEMORY, "out of device memory (batch)");
+  k_zero_i32<<<1, 64, 0, st>>>(w.nover_sum.p, 2);
   for (size_t i = 0; i < b.cfg.size(); ++i)
     if (int rc = run_pass_ws(ix, w, b.cfg[i], LearnArgs{}, false, dq, b.nq, q_stride, d_ids, d_dists, nullptr,
                              w.blist.p + b.offs[i], w.bcount.p + i, nullptr, st, true))
