@@ -159,7 +159,8 @@ struct Node {
 struct Params {
   int trees = 4, trees_build = 4, ml = 4, epochs = 4, check = 25, L = 30, K = 10, S = 10, tns = 10;
   uint64_t seed = 1998;
-  int join_mode = 0;  // 0 = batched (the HIP library's semantics), 1 = sequential (reference order)
+  int join_mode = 0;  // 0 = batched (the HIP library's semantics), 1 = sequential (reference order),
+                      // 2 = parallel with the reference's locking (CPU baseline)
 };
 
 // Hashing index (IEH seeds, hashing_index.hpp): per table the members of
@@ -607,6 +608,54 @@ void join_sequential(Index& ix) {
     };
     const auto nw = ix.nn_new[ni];
     const auto od = ix.nn_old[ni];
+    for (uint32_t i : nw) {
+      for (uint32_t j : nw)
+        if (i < j) cb(i, j);
+      for (uint32_t j : od) cb(i, j);
+    }
+    ix.found[ni] = uu > 0;
+  }
+  ix.n_pairs_join += pairs;
+  ix.n_dist_join += pairs;
+  ix.join_pairs_per_iter.push_back(pairs);
+}
+
+// Parallel form with the reference's threading (join_mode 2): OpenMP over
+// the joining nodes (dynamic schedule of 100, base_index.hpp:331), each
+// insert an unlocked radius check and then InsertIntoKnn under the target's
+// lock (Neighbor::insert 80-93).  Like the reference it is not deterministic
+// under several threads; it is the CPU baseline's form.
+void join_parallel(Index& ix) {
+  const size_t n = ix.n;
+  const unsigned Lp = unsigned(ix.p.L), CK = unsigned(ix.p.check);
+  uint64_t pairs = 0;
+  Striped locks;
+  auto insert = [&](uint32_t x, uint32_t id, float dist) -> unsigned {
+    float r;
+    __atomic_load(&ix.radius[x], &r, __ATOMIC_RELAXED);
+    if (dist > r) return Lp + 1;
+    std::lock_guard<std::mutex> g(locks.at(x));
+    Point* pl = &ix.pool[size_t(x) * (Lp + 1)];
+    unsigned& L = ix.fill[x];
+    unsigned l = insert_into_knn(pl, L, Point{id, dist, true});
+    if (l <= L) {
+      if (L + 1 < Lp + 1) ++L;
+      else __atomic_store(&ix.radius[x], &pl[L - 1].dist, __ATOMIC_RELAXED);
+    }
+    return l;
+  };
+#pragma omp parallel for schedule(dynamic, 100) reduction(+ : pairs)
+  for (long long ni = 0; ni < (long long)n; ++ni) {
+    unsigned uu = 0;
+    auto cb = [&](uint32_t i, uint32_t j) {
+      float dist = l2_avx(ix.row(i), ix.row(j), ix.d);
+      ++pairs;
+      unsigned r = insert(i, j, dist);
+      if (r < CK) ++uu;
+      insert(j, i, dist);
+    };
+    const auto& nw = ix.nn_new[ni];
+    const auto& od = ix.nn_old[ni];
     for (uint32_t i : nw) {
       for (uint32_t j : nw)
         if (i < j) cb(i, j);
@@ -1261,6 +1310,7 @@ int orc_merge_init(void* h) { return merge_and_init(*static_cast<Index*>(h)); }
 int orc_refine_iter(void* h) {
   Index* ix = static_cast<Index*>(h);
   if (ix->p.join_mode == 1) join_sequential(*ix);
+  else if (ix->p.join_mode == 2) join_parallel(*ix);
   else join_batched(*ix);
   ix->iter++;
   update(*ix, 100);
