@@ -45,6 +45,7 @@ _SIGS = {
     "efg_knn_search": (c_int, [c_vp, c_int, c_f32p, c_sz, c_sz]),
     "efg_get_results": (c_int, [c_vp, c_szp, c_intp, c_i32p, c_f32p]),
     "efg_knn_search_device": (c_int, [c_vp, c_int, c_vp, c_sz, c_sz, c_vp, c_vp, c_vp, c_vp]),
+    "efg_knn_search_device_noalloc": (c_int, [c_vp, c_int, c_vp, c_sz, c_sz, c_vp, c_vp, c_vp]),
     "efg_graph_size": (c_int, [c_vp, c_szp]),
     "efg_graph_row": (c_int, [c_vp, c_sz, c_u32p, c_sz, c_szp]),
     "efg_save_graph": (c_int, [c_vp, ctypes.c_char_p]),

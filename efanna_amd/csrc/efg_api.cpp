@@ -725,6 +725,13 @@ int efg_knn_search_device(efg_index* ix, int k, const float* d_queries, size_t n
   return search_device(ix, k, d_queries, nq, q_stride, d_ids, d_dists, d_ndist, st);
 }
 
+int efg_knn_search_device_noalloc(efg_index* ix, int k, const float* d_queries, size_t nq, size_t q_stride,
+                                  int32_t* d_ids, float* d_dists, void* stream) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ix->stream;
+  return search_device_noalloc(ix, k, d_queries, nq, q_stride, d_ids, d_dists, st);
+}
+
 int efg_knn_search(efg_index* ix, int k, const float* queries, size_t nq, size_t q_stride) {
   if (!ix) return fail(EFG_ERR_ARG, "null index");
   if (nq && (!queries || q_stride < ix->d)) return fail(EFG_ERR_ARG, "invalid query matrix");

@@ -12,23 +12,32 @@
 
 namespace efg {
 
-// Owning device buffer.
+// Owning device buffer, or a non-owning view of memory held elsewhere (a
+// view never allocates: ensure() only checks that it is large enough).
 template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  bool owned = true;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { reset(); }
   void reset() {
-    if (p) hipFree(p);
+    if (p && owned) hipFree(p);
     p = nullptr;
     n = 0;
+  }
+  void view(T* ptr, size_t count) {
+    reset();
+    owned = false;
+    p = ptr;
+    n = count;
   }
   // grow-only (contents not preserved)
   bool ensure(size_t count) {
     if (count <= n && p) return true;
+    if (!owned) return false;
     reset();
     if (count == 0) count = 1;
     if (hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)) != hipSuccess) {
@@ -249,6 +258,10 @@ int hash_probe(efg_index* ix, int nleaf, size_t nq, int32_t* leaves, const int32
 // k_search.hip
 int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int32_t* d_ids, float* d_dists,
                   uint64_t* d_ndist, hipStream_t st);
+// The same search with every scratch buffer in the library's fixed device
+// array (no device allocation); see efg_knn_search_device_noalloc.
+int search_device_noalloc(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int32_t* d_ids,
+                          float* d_dists, hipStream_t st);
 // k_exact.hip
 int exact_knn(const float* X, size_t n, int d, size_t x_stride, const float* Q, int nq, size_t q_stride, int k,
               int exclude_self, int32_t* ids, float* dists, hipStream_t st, std::string* err);

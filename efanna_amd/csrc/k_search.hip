@@ -33,6 +33,8 @@
 //     the pool).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "efg_device.hpp"
@@ -1320,6 +1322,18 @@ using SearchKernel = void (*)(SearchCfg, LearnArgs, const float*, size_t, const 
                               int32_t*, float*, uint64_t*, unsigned long long*, const int32_t*, int, const int*, int,
                               uint32_t*, int32_t*, int*, unsigned long long*);
 
+// Capacity (log2 entries) of a global-memory visited table: the worst-case
+// visited set of a query (seeds plus every expansion), at most 2^26.
+int global_logcap(const efg_index* ix, const SearchCfg& c) {
+  size_t worst = size_t(c.init_num) + size_t(c.epochs) * size_t(c.E + c.K + 1) *
+                                          size_t(ix->max_deg > 0 ? ix->max_deg : 1) + 512;
+  if (worst > ix->n + 512) worst = ix->n + 512;
+  int l = ceil_log2(worst * 4 / 3 + 1);
+  if (l < c.logcap) l = c.logcap;
+  if (l > 26) l = 26;
+  return l;
+}
+
 // One search pass over the queries of qlist (all nq, in leaf order, when
 // qlist is null; *nlist_dev of them otherwise): descent, then the search
 // kernel with the visited table in LDS, then the queries whose table
@@ -1370,13 +1384,13 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
     }
     if (!qlist) {  // query order by first leaf (bucket)
       int shift = std::max(0, ceil_log2(ix->n + 1) - 12);
-      hipMemsetAsync(w.qhist.p, 0, size_t(nb_q + 1) * 4, st);
+      k_zero_i32<<<(nb_q + 1 + 255) / 256, 256, 0, st>>>(reinterpret_cast<int*>(w.qhist.p), nb_q + 1);
       k_qkey<<<unsigned((nq + 255) / 256), 256, 0, st>>>(w.leaves.p, c.nleaf, int(nq), snp, shift, w.qkey.p,
                                                          w.qhist.p);
       if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(w.qhist.p), nb_q, w.qoff.p, w.qoff.p + nb_q,
                                          w.scan_sums, st))
         return rc;
-      hipMemsetAsync(w.qhist.p, 0, size_t(nb_q + 1) * 4, st);
+      k_zero_i32<<<(nb_q + 1 + 255) / 256, 256, 0, st>>>(reinterpret_cast<int*>(w.qhist.p), nb_q + 1);
       k_qscatter<<<unsigned((nq + 255) / 256), 256, 0, st>>>(w.qkey.p, int(nq), w.qoff.p, w.qhist.p, w.qorder.p);
     }
   }
@@ -1420,19 +1434,16 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
     // memory
     SearchCfg c2 = c;
     set_threads(c2, 256);
-    size_t worst = size_t(c.init_num) + size_t(c.epochs) * size_t(c.E + c.K + 1) *
-                                            size_t(ix->max_deg > 0 ? ix->max_deg : 1) + 512;
-    if (worst > ix->n + 512) worst = ix->n + 512;
-    c2.logcap = ceil_log2(worst * 4 / 3 + 1);
-    if (c2.logcap < c.logcap) c2.logcap = c.logcap;
-    if (c2.logcap > 26) c2.logcap = 26;
+    c2.logcap = global_logcap(ix, c);
     size_t lds2 = search_lds_bytes(c2, true);
     // as many workgroups as LDS allows, with at most 1 GiB of tables
     const int per_cu2 = std::max(1, std::min(8, int(lds_cap / std::max<size_t>(lds2, 1))));
     const size_t tab_bytes = size_t(4) << c2.logcap;
-    const unsigned grid2 = unsigned(std::min<size_t>(size_t(ncu) * per_cu2,
-                                                     std::max<size_t>(128, (size_t(1) << 30) / tab_bytes)));
-    if (!w.gtab.ensure(size_t(grid2) << c2.logcap)) return fail(EFG_ERR_MEMORY, "out of device memory (search table)");
+    unsigned grid2 = unsigned(std::min<size_t>(size_t(ncu) * per_cu2,
+                                               std::max<size_t>(128, (size_t(1) << 30) / tab_bytes)));
+    if (!w.gtab.owned) grid2 = unsigned(std::min<size_t>(grid2, w.gtab.n >> c2.logcap));  // tables that fit a view
+    if (grid2 == 0 || !w.gtab.ensure(size_t(grid2) << c2.logcap))
+      return fail(EFG_ERR_MEMORY, "out of device memory (search table)");
     auto kern2 = pick(true);
     hipFuncSetAttribute(reinterpret_cast<const void*>(kern2), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2));
     if (table_in_lds) {
@@ -1580,6 +1591,102 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
   if (prof)
     for (auto& e : ev) ix->prof_events.push_back(e);
   return EFG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Search without device allocation (efg_knn_search_device_noalloc).  Every
+// scratch buffer of the pass (leaf slots, query order, overflow lists,
+// counters, global visited tables) is cut from one file-scope device array.
+// Host code never names the array itself: its address on the current device
+// comes from hipGetSymbolAddress, taken once per device and kept.  Calls on
+// one GPU share the array, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(EFG_NOALLOC_SCRATCH_BYTES);
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+int scratch_address(unsigned char** out) {
+  int dev = 0;
+  if (int rc = check_hip(hipGetDevice(&dev), "current device")) return rc;
+  if (dev < 0 || dev >= kMaxDevices) return fail(EFG_ERR_DEVICE, "device index out of range");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    if (int rc = check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "scratch address")) return rc;
+    g_scratch[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch[dev];
+  return EFG_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+}  // namespace
+
+int search_device_noalloc(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int32_t* d_ids,
+                          float* d_dists, hipStream_t st) {
+  // the checks of search_prepare, without its allocations
+  if (!ix->s_set) return fail(EFG_ERR_STATE, "search parameters not set (setSearchParams)");
+  if (ix->s_method != 0 && ix->s_method != 1) return fail(EFG_ERR_METHOD, "no such searching method");
+  if (ix->hashing) return fail(EFG_ERR_STATE, "the allocation-free search runs on kd-tree indexes");
+  if (ix->T == 0) return fail(EFG_ERR_STATE, "no trees: build or load trees first");
+  if (ix->nnz == 0) return fail(EFG_ERR_STATE, "no graph: build or load a graph first");
+  if (ix->s_trees <= 0) return fail(EFG_ERR_ARG, "search tree number must be positive");
+  if (ix->s_trees > ix->T) return fail(EFG_ERR_TREES, "wrong tree number");
+  if (ix->s_mode != 0) return fail(EFG_ERR_STATE, "learned stopping allocates: use efg_knn_search_device");
+  if (nq > (size_t(1) << 31)) return fail(EFG_ERR_ARG, "too many queries");
+  if (q_stride < ix->d) return fail(EFG_ERR_ARG, "query stride below the dimension");
+  if (nq && (!dq || !d_ids)) return fail(EFG_ERR_ARG, "null query or result buffer");
+  if (!ix->snode_valid)
+    return fail(EFG_ERR_STATE, "search trees not laid out yet: run one efg_knn_search(_device) on the index first");
+  SearchCfg c;
+  int init_eff = 0;
+  if (int rc = make_cfg(ix, k, ix->s_init, ix->s_extend, ix->s_epochs, ix->s_trees, ix->s_method, 0, &c, &init_eff))
+    return rc;
+  if (ix->s_method == 1) ix->s_init = init_eff;  // as the allocating search (kdtreeub_index.hpp:510-514)
+  const bool bf16 = ix->s_method == 0 && ix->s_precision == 1;
+  if (bf16 && !ix->data_bf16.p)
+    return fail(EFG_ERR_STATE, "BF16 rows not made yet: run one search with this precision on the index first");
+  set_threads(c, bf16 ? 256 : choose_threads(ix, c));
+  if (nq == 0) return EFG_OK;
+  // scratch layout (256-byte aligned pieces); the global visited tables take the rest
+  const int nb_q = 4096;
+  const size_t o_ndist = 0, o_nover = o_ndist + 256, o_nsum = o_nover + 256, o_sums = o_nsum + 256;
+  const size_t o_hist = o_sums + align256(64 * 4), o_qoff = o_hist + align256((nb_q + 1) * 4);
+  const size_t o_qorder = o_qoff + align256((nb_q + 1) * 4), o_qkey = o_qorder + align256(nq * 4);
+  const size_t o_over = o_qkey + align256(nq * 4), o_leaves = o_over + align256(2 * nq * 4);
+  const size_t o_gtab = o_leaves + align256(nq * size_t(c.nleaf) * 4);
+  const size_t table = size_t(4) << global_logcap(ix, c);
+  if (o_gtab > kScratchBytes || kScratchBytes - o_gtab < table) {
+    const size_t per_q = 4 * (size_t(c.nleaf) + 4);      // leaf slots, order, key, two overflow entries
+    const size_t fixed = o_qorder + 4 * 256 + table;   // counters, histogram, alignment, one table
+    const size_t room = kScratchBytes > fixed ? (kScratchBytes - fixed) / per_q : 0;
+    return fail(EFG_ERR_ARG, "batch too large for the fixed search scratch of " + std::to_string(kScratchBytes >> 20) +
+                                 " MiB: at most " + std::to_string(room) +
+                                 " queries per call at these search parameters");
+  }
+  unsigned char* base = nullptr;
+  if (int rc = scratch_address(&base)) return rc;
+  efg::Workspace w;  // views of the scratch: nothing below allocates
+  w.ndist.view(reinterpret_cast<unsigned long long*>(base + o_ndist), 3);
+  w.nover.view(reinterpret_cast<int*>(base + o_nover), 2);
+  w.nover_sum.view(reinterpret_cast<int*>(base + o_nsum), 2);
+  w.scan_sums.view(reinterpret_cast<int32_t*>(base + o_sums), 64);
+  w.qhist.view(reinterpret_cast<uint32_t*>(base + o_hist), nb_q + 1);
+  w.qoff.view(reinterpret_cast<int32_t*>(base + o_qoff), nb_q + 1);
+  w.qorder.view(reinterpret_cast<int32_t*>(base + o_qorder), nq);
+  w.qkey.view(reinterpret_cast<int32_t*>(base + o_qkey), nq);
+  w.overflow.view(reinterpret_cast<int32_t*>(base + o_over), 2 * nq);
+  w.leaves.view(reinterpret_cast<int32_t*>(base + o_leaves), nq * size_t(c.nleaf));
+  w.gtab.view(reinterpret_cast<uint32_t*>(base + o_gtab), (kScratchBytes - o_gtab) / 4);
+  // counters start at zero (a fill kernel, not a memset, so that callers may capture the call)
+  k_zero_i32<<<1, 64, 0, st>>>(reinterpret_cast<int*>(base + o_ndist), 6);
+  k_zero_i32<<<1, 64, 0, st>>>(w.nover_sum.p, 2);
+  return run_pass_ws(ix, w, c, LearnArgs{}, bf16, dq, nq, q_stride, d_ids, d_dists, nullptr, nullptr, nullptr,
+                     nullptr, st, false);
 }
 
 int search_autotune(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int* chosen,

@@ -219,6 +219,25 @@ int efg_batch_plan_create(efg_index* ix, const float* d_queries, size_t nq, size
 int efg_batch_plan_run(efg_batch_plan* plan, void* stream);
 int efg_batch_plan_destroy(efg_batch_plan* plan);
 
+/* Search without device allocation (no counterpart in the reference).  For
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  The same search, with
+ * the same results, as efg_knn_search_device (method 0 or 1, FP32 or BF16
+ * rows; kd-tree index, plain mode), ordered on `stream`, but every scratch
+ * buffer lives in one fixed device array of EFG_NOALLOC_SCRATCH_BYTES per GPU,
+ * reserved when the library is loaded and held until the process ends; the
+ * call allocates and frees nothing.  Limits: nq * (search_trees * leaves per
+ * tree + 4) * 4 bytes plus one global visited table must fit the array (about
+ * 30,000 queries at 16 trees and pool 1200); larger batches are refused with
+ * EFG_ERR_ARG.  The index must have searched once with the allocating entry
+ * points (that lays out the search trees, and the BF16 rows for precision 1),
+ * else EFG_ERR_STATE.  Calls on one GPU share the array and its last contents:
+ * calls that overlap on one GPU race.  The work counters of
+ * efg_get_counters / efg_get_search_work do not see these calls. */
+#define EFG_NOALLOC_SCRATCH_BYTES (64u << 20)
+int efg_knn_search_device_noalloc(efg_index* ix, int k, const float* d_queries, size_t nq, size_t q_stride,
+                                  int32_t* d_ids, float* d_dists, void* stream);
+
 /* Threads per query of the search kernels (no counterpart in the reference;
  * launch tuning only, results do not change): 64, 128 or 256 forces one;
  * 0 (default) uses the value tuned for the configuration, else one
