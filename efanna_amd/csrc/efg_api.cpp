@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <fstream>
 #include <string>
 #include <vector>
@@ -27,6 +29,74 @@ int fail(int code, const std::string& msg) {
 int check_hip(hipError_t e, const char* what) {
   if (e == hipSuccess) return EFG_OK;
   return fail(EFG_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+struct MemCache {
+  std::mutex mu;
+  // per device: free blocks by size class, and blocks released since the
+  // device was last synchronised
+  std::map<int, std::multimap<size_t, void*>> ready;
+  std::map<int, std::vector<std::pair<size_t, void*>>> pending;
+};
+MemCache& mem_cache() {
+  static MemCache* c = new MemCache();  // never destroyed: blocks live until the process ends
+  return *c;
+}
+// size classes: multiples of 256 B up to 1 MiB, then eight classes per power of two
+size_t size_class(size_t b) {
+  if (b <= (size_t(1) << 20)) return (b + 255) & ~size_t(255);
+  int e = 63 - __builtin_clzll(b);
+  size_t step = size_t(1) << (e - 3);
+  return (b + step - 1) & ~(step - 1);
+}
+}  // namespace
+
+void* dev_alloc(size_t bytes) {
+  const size_t cls = size_class(bytes);
+  int dev = 0;
+  hipGetDevice(&dev);
+  MemCache& c = mem_cache();
+  std::lock_guard<std::mutex> g(c.mu);
+  auto& rd = c.ready[dev];
+  auto& pd = c.pending[dev];
+  for (int round = 0; round < 2; ++round) {
+    auto it = rd.find(cls);
+    if (it != rd.end()) {
+      void* p = it->second;
+      rd.erase(it);
+      return p;
+    }
+    bool fits = false;
+    for (auto& b : pd) fits = fits || b.first == cls;
+    if (!fits || round) break;
+    // released blocks of this class exist: once the device has finished what
+    // was submitted before their release, they are free to reuse
+    if (hipDeviceSynchronize() != hipSuccess) return nullptr;
+    for (auto& b : pd) rd.emplace(b.first, b.second);
+    pd.clear();
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, cls) == hipSuccess) return p;
+  // out of memory: give every cached block back and try again
+  (void)hipGetLastError();
+  if (hipDeviceSynchronize() != hipSuccess) return nullptr;
+  for (auto& b : pd) hipFree(b.second);
+  pd.clear();
+  for (auto& b : rd) hipFree(b.second);
+  rd.clear();
+  if (hipMalloc(&p, cls) == hipSuccess) return p;
+  (void)hipGetLastError();
+  return nullptr;
+}
+
+void dev_release(void* p, size_t bytes) {
+  if (!p) return;
+  int dev = 0;
+  hipGetDevice(&dev);
+  MemCache& c = mem_cache();
+  std::lock_guard<std::mutex> g(c.mu);
+  c.pending[dev].emplace_back(size_class(bytes), p);
 }
 
 namespace {
@@ -82,6 +152,7 @@ int upload_trees(efg_index* ix, const HostTrees& h) {
   }
   ix->leaf_dim = h.leaf_dim;
   if (ix->leaf_dim.size() != M) ix->leaf_dim.assign(M, -1);
+  ix->leaf_dim_lazy = false;
   return EFG_OK;
 }
 
@@ -112,7 +183,20 @@ int download_trees(const efg_index* ix, HostTrees& h) {
     h.start[i] = rg[i].x;
     h.end[i] = rg[i].y;
   }
-  h.leaf_dim = ix->leaf_dim;
+  if (ix->leaf_dim_lazy) {
+    // DivDim written for leaves (kdtreeub_index.hpp:1249, 1142-1143): leaves
+    // the DFS phase made (and roots) carry -1, leaves left behind by the BFS
+    // phase carry 0
+    h.leaf_dim.assign(M, -1);
+    const int Lb = ix->tree_lb;
+    for (size_t i = size_t(ix->T); i < M; ++i) {
+      const int lv = ix->node_level[i];
+      const bool dfs_made = Lb >= 0 && (lv > Lb || (lv == Lb && rg[i].y - rg[i].x > uint32_t(ix->tree_S)));
+      h.leaf_dim[i] = dfs_made ? -1 : 0;
+    }
+  } else {
+    h.leaf_dim = ix->leaf_dim;
+  }
   return EFG_OK;
 }
 

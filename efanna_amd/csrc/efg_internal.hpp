@@ -12,6 +12,15 @@
 
 namespace efg {
 
+// Device memory cache behind DevBuf.  Freed blocks are kept per device and
+// size class instead of going back to the runtime (hipFree synchronises the
+// device and unmaps; hipMalloc maps again).  A freed block is reused only
+// after the device has been synchronised once since its release, so no block
+// is handed out while an earlier kernel may still use it.  On an allocation
+// failure the cache is emptied and the allocation retried.
+void* dev_alloc(size_t bytes);
+void dev_release(void* p, size_t bytes);
+
 // Owning device buffer, or a non-owning view of memory held elsewhere (a
 // view never allocates: ensure() only checks that it is large enough).
 template <class T>
@@ -24,7 +33,7 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { reset(); }
   void reset() {
-    if (p && owned) hipFree(p);
+    if (p && owned) dev_release(p, n * sizeof(T));
     p = nullptr;
     n = 0;
   }
@@ -40,10 +49,8 @@ struct DevBuf {
     if (!owned) return false;
     reset();
     if (count == 0) count = 1;
-    if (hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)) != hipSuccess) {
-      p = nullptr;
-      return false;
-    }
+    p = static_cast<T*>(dev_alloc(count * sizeof(T)));
+    if (!p) return false;
     n = count;
     return true;
   }
@@ -183,6 +190,11 @@ struct efg_index {
   efg::DevBuf<uint32_t> leaf;    // T x n
   bool leaf0_perm = false;       // tree 0's leaf list holds every row once
   std::vector<int8_t> leaf_dim;  // value written as DivDim of a leaf (0 or -1)
+  // after a device tree build leaf_dim is derived when the trees are read
+  // back (saving): node levels, the BFS->DFS switch level and S
+  bool leaf_dim_lazy = false;
+  std::vector<int8_t> node_level;
+  int tree_lb = -1, tree_S = 0;
   efg::DevBuf<efg::SNode> snode;  // search layout of the trees
   efg::DevBuf<efg::DNode> dnode;  // descent copy of snode
   efg::DevBuf<int32_t> sroots;

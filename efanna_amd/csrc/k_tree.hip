@@ -500,16 +500,13 @@ int gpu_build_trees(efg_index* ix) {
   // DivDim written for leaves in the trees file: leaves the DFS phase made
   // (and roots) carry -1, leaves left behind by the BFS phase carry 0
   // (zero-initialised nodes, kdtreeub_index.hpp:1249, 1142-1143).
-  ix->leaf_dim.assign(size_t(M), -1);
-  if (Lb != 0) {
-    std::vector<uint2> rg(M);
-    hipMemcpy(rg.data(), ix->range.p, size_t(M) * sizeof(uint2), hipMemcpyDeviceToHost);
-    for (int32_t i = T; i < M; ++i) {
-      int lv = level_of[i];
-      bool dfs_made = (Lb >= 0) && (lv > Lb || (lv == Lb && rg[i].y - rg[i].x > uint32_t(p.S)));
-      ix->leaf_dim[i] = dfs_made ? -1 : 0;
-    }
-  }
+  // (derived from the node levels when the trees are read back, so the
+  // build does not copy the node ranges to the host)
+  ix->leaf_dim.clear();
+  ix->leaf_dim_lazy = Lb != 0;
+  ix->node_level = std::move(level_of);
+  ix->tree_lb = Lb;
+  ix->tree_S = p.S;
   ix->s_trees = T;
   return EFG_OK;
 }
