@@ -55,6 +55,8 @@ _SIGS = {
     "efg_save_index": (c_int, [c_vp, ctypes.c_char_p]),
     "efg_load_index": (c_int, [c_vp, ctypes.c_char_p]),
     "efg_save_results": (c_int, [c_vp, ctypes.c_char_p]),
+    "efg_save_image": (c_int, [c_vp, ctypes.c_char_p]),
+    "efg_load_image": (c_int, [c_vp, ctypes.c_char_p]),
     "efg_exact_knn_device": (c_int, [c_vp, c_sz, c_sz, c_sz, c_vp, c_sz, c_sz, c_int, c_int, c_vp, c_vp, c_vp]),
     "efg_exact_knn_device_prec": (c_int, [c_vp, c_sz, c_sz, c_sz, c_vp, c_sz, c_sz, c_int, c_int, c_int, c_vp, c_vp,
                                           c_vp]),

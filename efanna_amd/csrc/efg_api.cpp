@@ -963,6 +963,16 @@ int efg_load_index(efg_index* ix, const char* path) {
   return upload_graph(ix, off, ids);
 }
 
+int efg_save_image(efg_index* ix, const char* path) {
+  if (!ix || !path) return fail(EFG_ERR_ARG, "null argument");
+  return save_image(ix, path);
+}
+
+int efg_load_image(efg_index* ix, const char* path) {
+  if (!ix || !path) return fail(EFG_ERR_ARG, "null argument");
+  return load_image(ix, path);
+}
+
 // saveResults (base_index.hpp:136-151): per query int k, then k int ids.
 // Rows report their valid entries only (method 1 can find fewer than k).
 int efg_save_results(const efg_index* ix, const char* path) {

@@ -267,6 +267,9 @@ int hash_set_codes(efg_index* ix, int tables, int codelen, int lenshift, int upb
 int hash_build(efg_index* ix);
 int hash_probe(efg_index* ix, int nleaf, size_t nq, int32_t* leaves, const int32_t* qlist, const int* nlist,
                hipStream_t st);
+// efg_image.cpp: the index image (device arrays of a built index in one file)
+int save_image(efg_index* ix, const char* path);
+int load_image(efg_index* ix, const char* path);
 // k_search.hip
 int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_stride, int32_t* d_ids, float* d_dists,
                   uint64_t* d_ndist, hipStream_t st);

@@ -329,6 +329,13 @@ class FIndex:
     def loadIndex(self, fn):
         check(lib().efg_load_index(self.h, str(fn).encode()))
 
+    def saveImage(self, fn):
+        """Device-loadable index image (trees + graph as laid out in device memory)."""
+        check(lib().efg_save_image(self.h, str(fn).encode()))
+
+    def loadImage(self, fn):
+        check(lib().efg_load_image(self.h, str(fn).encode()))
+
     def saveResults(self, fn):
         check(lib().efg_save_results(self.h, str(fn).encode()))
 

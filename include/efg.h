@@ -266,6 +266,20 @@ int efg_save_index(const efg_index* ix, const char* path);
 int efg_load_index(efg_index* ix, const char* path);
 int efg_save_results(const efg_index* ix, const char* path);
 
+/* Index image: saveIndex / loadIndex (kdtreeub_index.hpp:129-137, 701-825)
+ * as one device-loadable file.  The image holds the arrays the search reads
+ * as they sit in device memory (depth-first search trees, descent nodes, leaf
+ * lists, CSR graph) and the build-order trees, in 4 KiB-aligned sections with
+ * checksums; loading streams the file through pinned host memory straight
+ * into HBM, with no parsing or re-layout, and a device pass range-checks
+ * every id and node link before the index takes the arrays (a damaged file
+ * gives EFG_ERR_FORMAT and leaves the index unchanged).  The index must hold
+ * the same data (rows and dimension) as the one that saved the image.  The
+ * reference formats (efg_save_index, efg_save_trees, efg_save_graph) stay the
+ * interchange formats; images are for this library's own fast reloads. */
+int efg_save_image(efg_index* ix, const char* path);
+int efg_load_image(efg_index* ix, const char* path);
+
 /* Exact k nearest neighbours by brute force over device rows (the oracle
  * the reference only has as Matrix::brute_force_search, general/matrix.hpp:57-68):
  * k smallest (distance, id), ties to the lower id; exclude_self skips row q for

@@ -83,3 +83,46 @@ def trees_bytes(K, num, dim, trees, leaf):
             out += struct.pack("<ifQQQQ", r["dim"], r["val"], r["start"], r["end"], r.get("lp", 0), r.get("rp", 0))
     out += np.asarray(leaf, "<u4").tobytes()
     return bytes(out)
+
+
+# ---- index image (efg_save_image / efg_load_image) --------------------------
+# header: 8-byte magic, u32 version, u32 sections, u64 n d D M nnz, u32 T,
+# 8 x i32 (reg_deg, max_deg, leaf0_perm, tree_lb, tree_S, lazy, pad), u64
+# seed, u64 check; then per section u32 id, u32 element size, u64 count,
+# u64 offset, u64 check; sections at 4096-byte aligned offsets.
+IMAGE_HEADER = struct.Struct("<8sII5QI7iQQ")
+IMAGE_SECTION = struct.Struct("<IIQQQ")
+M64 = (1 << 64) - 1
+
+
+def image_hash(data: bytes) -> int:
+    """The image's 64-bit multiply-xor checksum, written independently."""
+    h = 0x9E3779B97F4A7C15
+    n8 = len(data) // 8
+    words = struct.unpack_from(f"<{n8}Q", data) if n8 else ()
+    for w in words:
+        h = ((h ^ w) * 0xFF51AFD7ED558CCD) & M64
+        h ^= h >> 29
+    for b in data[8 * n8:]:
+        h = ((h ^ b) * 0x100000001B3) & M64
+    return h
+
+
+def read_image(data: bytes):
+    hdr = IMAGE_HEADER.unpack_from(data, 0)
+    nsec = hdr[2]
+    secs = [IMAGE_SECTION.unpack_from(data, IMAGE_HEADER.size + i * IMAGE_SECTION.size) for i in range(nsec)]
+    return hdr, secs
+
+
+def resign_image(data: bytearray):
+    """Recompute every section checksum and the header checksum in place."""
+    hdr, secs = read_image(bytes(data))
+    table = b""
+    for i, (sid, elem, count, off, _) in enumerate(secs):
+        chk = image_hash(bytes(data[off:off + elem * count]))
+        rec = IMAGE_SECTION.pack(sid, elem, count, off, chk)
+        data[IMAGE_HEADER.size + i * IMAGE_SECTION.size:IMAGE_HEADER.size + (i + 1) * IMAGE_SECTION.size] = rec
+        table += rec
+    h0 = IMAGE_HEADER.pack(*hdr[:-1], 0)
+    data[:IMAGE_HEADER.size] = IMAGE_HEADER.pack(*hdr[:-1], image_hash(h0) ^ image_hash(table))

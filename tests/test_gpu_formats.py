@@ -143,3 +143,112 @@ def test_cli_pipeline_matches_oracle(tmp_path):
     out = run(os.path.join(BIN, "evaluate"), tmp_path / "res.ivecs", tmp_path / "gt.ivecs", 10)
     rec = float(out.strip().split()[-1])
     assert abs(rec - helpers.recall(oids, gt, 10)) < 1e-6
+
+
+# --------------------------------------------------------------------------
+# index image (efg_save_image / efg_load_image)
+# --------------------------------------------------------------------------
+def _image_pair(tmp_path):
+    base, q = helpers.mixture(n=20_000, nq=100, centres=50)
+    p = KDTreeUbIndexParams(True, 8, 6, 5, 25, 30, 10, 4, 10)
+    a = FIndex(base, L2DistanceAVX(), p)
+    a.buildIndex()
+    a.setSearchParams(4, 300, 50, 8, -1, 0)
+    a.knnSearch(10, q)
+    ra = a.results()
+    a.saveImage(tmp_path / "ix.img")
+    return base, q, p, a, ra
+
+
+def test_image_round_trip(tmp_path):
+    base, q, p, a, (ia, da) = _image_pair(tmp_path)
+    b = FIndex(base, L2DistanceAVX(), p)
+    b.loadImage(tmp_path / "ix.img")
+    b.setSearchParams(4, 300, 50, 8, -1, 0)
+    b.knnSearch(10, q)
+    ib, db = b.results()
+    np.testing.assert_array_equal(ib, ia)
+    np.testing.assert_array_equal(db.view(np.uint32), da.view(np.uint32))
+    ta, tb = a.trees(), b.trees()
+    for key in ta:
+        np.testing.assert_array_equal(ta[key], tb[key], err_msg=key)
+    np.testing.assert_array_equal(a.graph_matrix(), b.graph_matrix())
+    # the reference formats written from the reloaded index are the same bytes
+    a.saveTrees(tmp_path / "a.trees")
+    b.saveTrees(tmp_path / "b.trees")
+    ta_ = formats.read_trees((tmp_path / "a.trees").read_bytes())
+    tb_ = formats.read_trees((tmp_path / "b.trees").read_bytes())
+    assert [[{k: v for k, v in r.items() if k not in ("lp", "rp")} for r in t["nodes"]] for t in ta_["trees"]] == \
+        [[{k: v for k, v in r.items() if k not in ("lp", "rp")} for r in t["nodes"]] for t in tb_["trees"]]
+    np.testing.assert_array_equal(ta_["leaf"], tb_["leaf"])
+    a.saveGraph(tmp_path / "a.graph")
+    b.saveGraph(tmp_path / "b.graph")
+    assert (tmp_path / "a.graph").read_bytes() == (tmp_path / "b.graph").read_bytes()
+    # layout: the independent parser finds the documented sections, checksums included
+    data = (tmp_path / "ix.img").read_bytes()
+    hdr, secs = formats.read_image(data)
+    assert hdr[0] == b"EFGIMG01" and hdr[3] == base.shape[0] and len(secs) == 12
+    assert [s[0] for s in secs] == list(range(1, 13))
+    for sid, elem, count, off, chk in secs:
+        assert off % 4096 == 0 and formats.image_hash(data[off:off + elem * count]) == chk
+
+
+def test_image_reloads_index_from_reference_files(tmp_path):
+    """Trees and graph loaded from the reference formats, saved as an image
+    and reloaded, search the same."""
+    base, q, p, a, (ia, da) = _image_pair(tmp_path)
+    a.saveTrees(tmp_path / "t.trees")
+    a.saveGraph(tmp_path / "t.graph")
+    b = FIndex(base, L2DistanceAVX(), p)
+    b.loadTrees(tmp_path / "t.trees")
+    b.loadGraph(tmp_path / "t.graph")
+    b.saveImage(tmp_path / "b.img")
+    c = FIndex(base, L2DistanceAVX(), p)
+    c.loadImage(tmp_path / "b.img")
+    c.setSearchParams(4, 300, 50, 8, -1, 0)
+    c.knnSearch(10, q)
+    ic, dc = c.results()
+    np.testing.assert_array_equal(ic, ia)
+    np.testing.assert_array_equal(dc.view(np.uint32), da.view(np.uint32))
+
+
+def test_image_refusals(tmp_path):
+    base, q, p, a, (ia, _) = _image_pair(tmp_path)
+    good = (tmp_path / "ix.img").read_bytes()
+    hdr, secs = formats.read_image(good)
+    gids = secs[9]  # graph ids
+    b = FIndex(base, L2DistanceAVX(), p)
+    b.loadImage(tmp_path / "ix.img")
+    b.setSearchParams(4, 300, 50, 8, -1, 0)
+
+    def refused(data, what):
+        (tmp_path / "bad.img").write_bytes(bytes(data))
+        with pytest.raises(EfgError) as e:
+            b.loadImage(tmp_path / "bad.img")
+        assert e.value.code == -7 and what in e.value.msg, e.value.msg
+
+    flipped = bytearray(good)
+    flipped[gids[3] + 40] ^= 1
+    refused(flipped, "checksum")
+    refused(good[:gids[3] + 100], "inconsistent")
+    refused(b"EFGIMG02" + good[8:], "not an index image")
+    # an id out of range with valid checksums: the device range check refuses it
+    tampered = bytearray(good)
+    struct_at = gids[3] + 4 * 123
+    tampered[struct_at:struct_at + 4] = np.uint32(base.shape[0] + 7).tobytes()
+    formats.resign_image(tampered)
+    refused(tampered, "out of range")
+    # a tree link out of range, re-signed
+    tampered = bytearray(good)
+    sn = secs[5]  # search nodes: 32 bytes each, right child at byte 12
+    tampered[sn[3] + 12:sn[3] + 16] = np.int32(10 ** 8).tobytes()
+    formats.resign_image(tampered)
+    refused(tampered, "out of range")
+    # other data: rows differ
+    other = FIndex(base[:15_000], L2DistanceAVX(), p)
+    with pytest.raises(EfgError) as e:
+        other.loadImage(tmp_path / "ix.img")
+    assert e.value.code == -7 and "other data" in e.value.msg
+    # after the refusals the index still holds the good image
+    b.knnSearch(10, q)
+    np.testing.assert_array_equal(b.results()[0], ia)
