@@ -180,6 +180,7 @@ struct efg_index {
   efg::DevBuf<uint16_t> split_hi, split_lo;  // build precision form: rows as hi + lo 16-bit parts (n x Dk)
   efg::DevBuf<float> split_norm;             // FP32 squared norms of the rows
   int split_Dk = 0, split_kind = 0;
+  bool split_lo_used = true;                 // some low part is non-zero (else the tiles skip them)
   // trees
   int T = 0;
   size_t M = 0;

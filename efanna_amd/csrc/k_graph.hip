@@ -1202,11 +1202,12 @@ __global__ void k_join_scatter(const Stage* __restrict__ stage, unsigned long lo
 // FP32 squared norm of row r; one wavefront per row.
 template <bool kF16>
 __global__ void k_pack_split(const float* __restrict__ X, size_t n, int d, int D, int Dk, uint16_t* __restrict__ hi,
-                             uint16_t* __restrict__ lo, float* __restrict__ norm) {
+                             uint16_t* __restrict__ lo, float* __restrict__ norm, int* __restrict__ lo_any) {
   const size_t r = size_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (r >= n) return;
   float acc = 0.f;
+  bool nz = false;
   for (int e = lane; e < Dk; e += 64) {
     const float v = e < d ? X[r * D + e] : 0.f;
     uint16_t h, l;
@@ -1224,10 +1225,12 @@ __global__ void k_pack_split(const float* __restrict__ X, size_t n, int d, int D
     }
     hi[r * Dk + e] = h;
     lo[r * Dk + e] = l;
+    nz = nz || (l & 0x7fffu) != 0;  // a low part of +-0 adds nothing
     acc = __builtin_fmaf(v, v, acc);
   }
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
   if (lane == 0) norm[r] = acc;
+  if (__ballot(nz) && lane == 0) atomicOr(lo_any, 1);
 }
 
 template <bool kF16>
@@ -1256,7 +1259,11 @@ __device__ __forceinline__ f32x16_t mfma16(const uint4& a, const uint4& b, const
 // multiplied.  Element l of the result is A row 8 (l / 4) + 4 (lane / 32) +
 // l % 4 against B row lane % 32.  Dk is a multiple of 64; with kSame the B
 // rows are the A rows (rb is not read).
-template <bool kF16, bool kSame>
+//
+// kLo = false: every low part of the rows is zero (rows exact in the 16-bit
+// type), so hi.lo' and lo.hi' are exact zeros; the low parts are neither
+// read nor multiplied, which leaves every accumulator unchanged.
+template <bool kF16, bool kSame, bool kLo = true>
 __device__ __forceinline__ f32x16_t mfma_split(const uint16_t* __restrict__ hi, const uint16_t* __restrict__ lo,
                                                size_t ra, size_t rb, int Dk) {
   const int kh = (threadIdx.x & 32) ? 32 : 0;
@@ -1269,10 +1276,10 @@ __device__ __forceinline__ f32x16_t mfma_split(const uint16_t* __restrict__ hi, 
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     a0[j] = ah[j];
-    a1[j] = al[j];
+    if (kLo) a1[j] = al[j];
     if (!kSame) {
       b0[j] = bh[j];
-      b1[j] = bl[j];
+      if (kLo) b1[j] = bl[j];
     }
   }
   f32x16_t acc = f32x16_t{};
@@ -1283,10 +1290,10 @@ __device__ __forceinline__ f32x16_t mfma_split(const uint16_t* __restrict__ hi, 
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       n0[j] = ah[o + j];
-      n1[j] = al[o + j];
+      if (kLo) n1[j] = al[o + j];
       if (!kSame) {
         m0[j] = bh[o + j];
-        m1[j] = bl[o + j];
+        if (kLo) m1[j] = bl[o + j];
       }
     }
 #pragma unroll
@@ -1294,16 +1301,18 @@ __device__ __forceinline__ f32x16_t mfma_split(const uint16_t* __restrict__ hi, 
       const uint4& y0 = kSame ? a0[j] : b0[j];
       const uint4& y1 = kSame ? a1[j] : b1[j];
       acc = mfma16<kF16>(a0[j], y0, acc);
-      acc = mfma16<kF16>(a0[j], y1, acc);
-      acc = mfma16<kF16>(a1[j], y0, acc);
+      if (kLo) {
+        acc = mfma16<kF16>(a0[j], y1, acc);
+        acc = mfma16<kF16>(a1[j], y0, acc);
+      }
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       a0[j] = n0[j];
-      a1[j] = n1[j];
+      if (kLo) a1[j] = n1[j];
       if (!kSame) {
         b0[j] = m0[j];
-        b1[j] = m1[j];
+        if (kLo) b1[j] = m1[j];
       }
     }
   }
@@ -1313,7 +1322,7 @@ __device__ __forceinline__ f32x16_t mfma_split(const uint16_t* __restrict__ hi, 
 // Merge leaf tiles on MFMA: one wavefront per (leaf bucket, 32 records),
 // members in blocks of 32 (LM <= 64); writes the same dist / pslot entries
 // as k_leaf_dist.
-template <bool kF16>
+template <bool kF16, bool kLo>
 __global__ void __launch_bounds__(kBlock) k_leaf_mfma(const uint16_t* __restrict__ xhi, const uint16_t* __restrict__ xlo,
                                                       const float* __restrict__ xn, int Dk,
                                                       const uint2* __restrict__ range,
@@ -1339,7 +1348,7 @@ __global__ void __launch_bounds__(kBlock) k_leaf_mfma(const uint16_t* __restrict
       }
       for (int mb = 0; mb < m; mb += 32) {
         const uint32_t mid = (mb + r32 < m) ? leaf_t[r.x + mb + r32] : 0u;  // member of B row r32
-        const f32x16_t acc = mfma_split<kF16, false>(xhi, xlo, pid, mid, Dk);
+        const f32x16_t acc = mfma_split<kF16, false, kLo>(xhi, xlo, pid, mid, Dk);
         const float pn = xn[pid], mn = xn[mid];
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
@@ -1359,7 +1368,7 @@ __global__ void __launch_bounds__(kBlock) k_leaf_mfma(const uint16_t* __restrict
 // k_join_tasks), the 32 x 32 block of dot products in one accumulator; then
 // the candidates as in k_join_tiles, counted per target row in the
 // wavefront's LDS slice, staged into per-wavefront runs.
-template <bool kF16>
+template <bool kF16, bool kLo>
 __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict__ xhi, const uint16_t* __restrict__ xlo,
                                                       const float* __restrict__ xn, int Dk, uint32_t ntask,
                                                       const uint4* __restrict__ hdr,
@@ -1391,7 +1400,8 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
     cnt_s[wv][lane] = 0;
     const uint32_t ia = uint32_t(__shfl(int(id), r32)), ib = uint32_t(__shfl(int(id), same ? r32 : 32 + r32));
     const uint32_t ra = ia != kNoRow ? ia : 0u, rbw = ib != kNoRow ? ib : 0u;
-    const f32x16_t acc = same ? mfma_split<kF16, true>(xhi, xlo, ra, ra, Dk) : mfma_split<kF16, false>(xhi, xlo, ra, rbw, Dk);
+    const f32x16_t acc = same ? mfma_split<kF16, true, kLo>(xhi, xlo, ra, ra, Dk)
+                                : mfma_split<kF16, false, kLo>(xhi, xlo, ra, rbw, Dk);
     const float na = xn[ra], nb = xn[rbw];
     wave_sync();
     float dist[16];
@@ -1467,7 +1477,7 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
 // Three 32 x 32 blocks of split dot products of one node's rows, loaded once:
 // A = rows ra (positions 0..31), B = rows rb (positions 32..63); t00 = A.A,
 // t01 = A.B, t11 = B.B, columns in the order of mfma_split.
-template <bool kF16>
+template <bool kF16, bool kLo>
 __device__ __forceinline__ void mfma_split3(const uint16_t* __restrict__ hi, const uint16_t* __restrict__ lo,
                                             size_t ra, size_t rb, int Dk, f32x16_t& t00, f32x16_t& t01,
                                             f32x16_t& t11) {
@@ -1481,9 +1491,11 @@ __device__ __forceinline__ void mfma_split3(const uint16_t* __restrict__ hi, con
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     a0[j] = ah[j];
-    a1[j] = al[j];
     b0[j] = bh[j];
-    b1[j] = bl[j];
+    if (kLo) {
+      a1[j] = al[j];
+      b1[j] = bl[j];
+    }
   }
   t00 = f32x16_t{};
   t01 = f32x16_t{};
@@ -1495,28 +1507,34 @@ __device__ __forceinline__ void mfma_split3(const uint16_t* __restrict__ hi, con
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       n0[j] = ah[o + j];
-      n1[j] = al[o + j];
       m0[j] = bh[o + j];
-      m1[j] = bl[o + j];
+      if (kLo) {
+        n1[j] = al[o + j];
+        m1[j] = bl[o + j];
+      }
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       t00 = mfma16<kF16>(a0[j], a0[j], t00);
       t01 = mfma16<kF16>(a0[j], b0[j], t01);
       t11 = mfma16<kF16>(b0[j], b0[j], t11);
-      t00 = mfma16<kF16>(a0[j], a1[j], t00);
-      t01 = mfma16<kF16>(a0[j], b1[j], t01);
-      t11 = mfma16<kF16>(b0[j], b1[j], t11);
-      t00 = mfma16<kF16>(a1[j], a0[j], t00);
-      t01 = mfma16<kF16>(a1[j], b0[j], t01);
-      t11 = mfma16<kF16>(b1[j], b0[j], t11);
+      if (kLo) {
+        t00 = mfma16<kF16>(a0[j], a1[j], t00);
+        t01 = mfma16<kF16>(a0[j], b1[j], t01);
+        t11 = mfma16<kF16>(b0[j], b1[j], t11);
+        t00 = mfma16<kF16>(a1[j], a0[j], t00);
+        t01 = mfma16<kF16>(a1[j], b0[j], t01);
+        t11 = mfma16<kF16>(b1[j], b0[j], t11);
+      }
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       a0[j] = n0[j];
-      a1[j] = n1[j];
       b0[j] = m0[j];
-      b1[j] = m1[j];
+      if (kLo) {
+        a1[j] = n1[j];
+        b1[j] = m1[j];
+      }
     }
   }
 }
@@ -1527,7 +1545,7 @@ __device__ __forceinline__ void mfma_split3(const uint16_t* __restrict__ hi, con
 // for more than 32 rows (0,1) and (1,1).  Nodes are taken in the order of
 // tree 0's leaf list (order, null: node order), one contiguous range per
 // XCD.  Candidates are counted, offset and staged as in k_join_mfma.
-template <bool kF16>
+template <bool kF16, bool kLo>
 __global__ void __launch_bounds__(kBlock) k_join_node_mfma(const uint16_t* __restrict__ xhi,
                                                            const uint16_t* __restrict__ xlo,
                                                            const float* __restrict__ xn, int Dk, uint32_t n,
@@ -1567,9 +1585,9 @@ __global__ void __launch_bounds__(kBlock) k_join_node_mfma(const uint16_t* __res
     const int nt = m > 32 ? 3 : 1;  // blocks (0,0) [, (0,1), (1,1)]
     f32x16_t t[3];
     if (nt == 3) {
-      mfma_split3<kF16>(xhi, xlo, ra, rb, Dk, t[0], t[1], t[2]);
+      mfma_split3<kF16, kLo>(xhi, xlo, ra, rb, Dk, t[0], t[1], t[2]);
     } else {
-      t[0] = mfma_split<kF16, true>(xhi, xlo, ra, ra, Dk);
+      t[0] = mfma_split<kF16, true, kLo>(xhi, xlo, ra, ra, Dk);
       t[1] = f32x16_t{};
       t[2] = f32x16_t{};
     }
@@ -2312,12 +2330,21 @@ int ensure_split(efg_index* ix, hipStream_t st) {
       !ix->split_norm.alloc(ix->n))
     return fail(EFG_ERR_MEMORY, "out of device memory (split rows)");
   const unsigned grid = unsigned((ix->n + 3) / 4);
+  DevBuf<int> lo_any;
+  if (!lo_any.alloc(1)) return fail(EFG_ERR_MEMORY, "out of device memory (split rows)");
+  hipMemsetAsync(lo_any.p, 0, sizeof(int), st);
   if (ix->b_precision == 2)
     k_pack_split<true><<<grid, 256, 0, st>>>(ix->data.p, ix->n, int(ix->d), int(ix->D), Dk, ix->split_hi.p,
-                                             ix->split_lo.p, ix->split_norm.p);
+                                             ix->split_lo.p, ix->split_norm.p, lo_any.p);
   else
     k_pack_split<false><<<grid, 256, 0, st>>>(ix->data.p, ix->n, int(ix->d), int(ix->D), Dk, ix->split_hi.p,
-                                              ix->split_lo.p, ix->split_norm.p);
+                                              ix->split_lo.p, ix->split_norm.p, lo_any.p);
+  // rows exact in the 16-bit type leave every low part zero: the tiles then
+  // neither read nor multiply them
+  int any = 1;
+  hipMemcpyAsync(&any, lo_any.p, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (int rc = check_hip(hipStreamSynchronize(st), "split rows")) return rc;
+  ix->split_lo_used = any != 0;
   ix->split_Dk = Dk;
   ix->split_kind = ix->b_precision;
   return check_hip(hipGetLastError(), "split rows");
@@ -2420,14 +2447,12 @@ int gpu_merge_init(efg_index* ix) {
     if (tiles) {
       if (!mdist.ensure(size_t(total) * LM + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (merge distances)");
       if (mfma_leaf) {
-        if (ix->b_precision == 2)
-          k_leaf_mfma<true><<<unsigned(ncu_m * 8), kBlock, 0, st>>>(
-              ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, ix->range.p, lt, boff.p, M, rec.p, LM,
-              p.merge_level, nlev, mdist.p, pslot.p);
-        else
-          k_leaf_mfma<false><<<unsigned(ncu_m * 8), kBlock, 0, st>>>(
-              ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, ix->range.p, lt, boff.p, M, rec.p, LM,
-              p.merge_level, nlev, mdist.p, pslot.p);
+        const bool f16 = ix->b_precision == 2, lo = ix->split_lo_used;
+        auto kern = f16 ? (lo ? k_leaf_mfma<true, true> : k_leaf_mfma<true, false>)
+                        : (lo ? k_leaf_mfma<false, true> : k_leaf_mfma<false, false>);
+        kern<<<unsigned(ncu_m * 8), kBlock, 0, st>>>(ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk,
+                                                     ix->range.p, lt, boff.p, M, rec.p, LM, p.merge_level, nlev,
+                                                     mdist.p, pslot.p);
       } else {
         k_leaf_dist<<<unsigned(ncu_m * 8), 64 * tile_waves, wave_lds * tile_waves, st>>>(
             ix->data.p, D, DCm, ix->range.p, lt, boff.p, M, rec.p, LM, PB, p.merge_level, nlev, mdist.p, pslot.p);
@@ -2552,11 +2577,14 @@ int gpu_refine_iter(efg_index* ix) {
       if (int rc = ensure_split(ix, st)) return rc;
       const unsigned g16 = unsigned(size_t(ncu) * 8);
       const bool f16 = ix->b_precision == 2;
-      (f16 ? k_join_node_mfma<true> : k_join_node_mfma<false>)<<<g16, kBlock, 0, st>>>(
+      const bool lo = ix->split_lo_used;
+      (f16 ? (lo ? k_join_node_mfma<true, true> : k_join_node_mfma<true, false>)
+           : (lo ? k_join_node_mfma<false, true> : k_join_node_mfma<false, false>))<<<g16, kBlock, 0, st>>>(
           ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, n, order, bs.new_off.p, bs.new_ids.p,
           bs.old_off.p, bs.old_ids.p, worst.p, stage, (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
       if (ntask)
-        (f16 ? k_join_mfma<true> : k_join_mfma<false>)<<<g16, kBlock, 0, st>>>(
+        (f16 ? (lo ? k_join_mfma<true, true> : k_join_mfma<true, false>)
+             : (lo ? k_join_mfma<false, true> : k_join_mfma<false, false>))<<<g16, kBlock, 0, st>>>(
             ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, uint32_t(ntask), bs.jhdr.p, bs.jtids.p,
             worst.p, stage, (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
     } else {
