@@ -449,9 +449,14 @@ int efg_create(const float* data, size_t n, size_t d, size_t row_stride, const e
 int efg_destroy(efg_index* ix) {
   if (!ix) return EFG_OK;
   if (ix->stream) hipStreamSynchronize(ix->stream);
+  for (hipStream_t s : ix->bstreams) hipStreamSynchronize(s);
   for (hipEvent_t e : ix->prof_events) hipEventDestroy(e);
+  if (ix->bfork) hipEventDestroy(ix->bfork);
+  for (hipEvent_t e : ix->bjoin) hipEventDestroy(e);
+  std::vector<hipStream_t> aux = ix->bstreams;
   hipStream_t st = ix->stream;
   delete ix;
+  for (hipStream_t s : aux) hipStreamDestroy(s);
   if (st) hipStreamDestroy(st);
   return EFG_OK;
 }

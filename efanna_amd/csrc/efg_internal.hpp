@@ -6,6 +6,7 @@
 #include <string>
 #include <array>
 #include <map>
+#include <memory>
 #include <vector>
 
 #include "efg.h"
@@ -252,6 +253,13 @@ struct efg_index {
   // threads per query: forced (64/128/256), else tuned per configuration, else a rule of thumb
   int s_threads = 0;
   std::map<std::array<int, 6>, int> tuned;  // (init_num, extend_to, epochs, k, trees, stop) -> threads
+  // batched search: the classes run on parallel streams (default priority),
+  // forked from and joined to the caller's stream with events; one
+  // workspace per class for the eager batches (plans keep their own)
+  std::vector<hipStream_t> bstreams;
+  hipEvent_t bfork = nullptr;
+  std::vector<hipEvent_t> bjoin;
+  std::vector<std::unique_ptr<efg::Workspace>> bws;
 };
 
 namespace efg {

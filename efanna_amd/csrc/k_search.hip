@@ -1345,7 +1345,7 @@ __global__ void k_zero_i32(int* p, int m) {
 
 __global__ void k_add_i32(int* __restrict__ acc, const int* __restrict__ v, int m) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m) acc[i] += v[i];
+  if (i < m) atomicAdd(acc + i, v[i]);  // batch classes on parallel streams share the sum
 }
 
 // (The pass clears its counters with a kernel, not a memset, so that a
@@ -1800,7 +1800,7 @@ __global__ void k_fill_results(int32_t* __restrict__ ids, float* __restrict__ d,
 // on the device and their threads per query are chosen.
 struct BatchSetup {
   std::vector<SearchCfg> cfg;
-  std::vector<int> offs;
+  std::vector<int> offs, counts;
   size_t nq = 0;
   int out_stride = 0;
 };
@@ -1828,6 +1828,7 @@ int batch_setup(efg_index* ix, efg::Workspace& w, const float* dq, size_t nq, si
     max_nleaf = std::max(max_nleaf, b->cfg[i].nleaf);
     b->offs.push_back(int(lists.size()));
     counts.push_back(int(cls[i].q.size()));
+    b->counts.push_back(int(cls[i].q.size()));
     lists.insert(lists.end(), cls[i].q.begin(), cls[i].q.end());
   }
   if (!w.leaves.ensure(nq * size_t(max_nleaf)) || !w.blist.ensure(lists.size()) || !w.bcount.ensure(counts.size()))
@@ -1876,18 +1877,65 @@ int batch_setup(efg_index* ix, efg::Workspace& w, const float* dq, size_t nq, si
   return rc;
 }
 
+// Parallel streams of the batched search (default priority), created once
+// per index, and the events that fork them from and join them to the
+// caller's stream.
+constexpr int kBatchStreams = 4;
+int batch_streams(efg_index* ix) {
+  if (!ix->bstreams.empty()) return EFG_OK;
+  for (int i = 0; i < kBatchStreams; ++i) {
+    hipStream_t s = nullptr;
+    if (int rc = check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "batch stream")) return rc;
+    ix->bstreams.push_back(s);
+    hipEvent_t e = nullptr;
+    if (int rc = check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "batch event")) return rc;
+    ix->bjoin.push_back(e);
+  }
+  return check_hip(hipEventCreateWithFlags(&ix->bfork, hipEventDisableTiming), "batch event");
+}
+
 // The batch's launches only (no host synchronisation, no memset), so the
-// sequence can be captured into a graph.
-int batch_launch(efg_index* ix, efg::Workspace& w, const BatchSetup& b, const float* dq, size_t q_stride,
-                 int32_t* d_ids, float* d_dists, hipStream_t st) {
+// sequence can be captured into a graph.  Each (pool, extend_to, k) class
+// runs its passes with a workspace of its own (cws[i]) on one of the
+// parallel streams, so the classes' passes overlap instead of each draining
+// the GPU before the next begins; in a captured graph they become parallel
+// branches between the fork and the join.
+int batch_launch(efg_index* ix, efg::Workspace& w, std::vector<std::unique_ptr<efg::Workspace>>& cws,
+                 const BatchSetup& b, const float* dq, size_t q_stride, int32_t* d_ids, float* d_dists,
+                 hipStream_t st) {
   const size_t m = b.nq * size_t(b.out_stride);
   k_fill_results<<<unsigned((m + 255) / 256), 256, 0, st>>>(d_ids, d_dists, m);
   if (!w.nover_sum.ensure(2)) return fail(EFG_ERR_MEMORY, "out of device memory (batch)");
   k_zero_i32<<<1, 64, 0, st>>>(w.nover_sum.p, 2);
-  for (size_t i = 0; i < b.cfg.size(); ++i)
-    if (int rc = run_pass_ws(ix, w, b.cfg[i], LearnArgs{}, false, dq, b.nq, q_stride, d_ids, d_dists, nullptr,
-                             w.blist.p + b.offs[i], w.bcount.p + i, nullptr, st, true))
+  const size_t nc = b.cfg.size();
+  if (nc == 1) {
+    return run_pass_ws(ix, w, b.cfg[0], LearnArgs{}, false, dq, b.nq, q_stride, d_ids, d_dists, nullptr, w.blist.p,
+                       w.bcount.p, nullptr, st, true);
+  }
+  if (int rc = batch_streams(ix)) return rc;
+  while (cws.size() < nc) cws.emplace_back(new efg::Workspace());
+  if (int rc = check_hip(hipEventRecord(ix->bfork, st), "batch fork")) return rc;
+  const int ns = int(std::min<size_t>(nc, ix->bstreams.size()));
+  for (int j = 0; j < ns; ++j)
+    if (int rc = check_hip(hipStreamWaitEvent(ix->bstreams[j], ix->bfork, 0), "batch fork")) return rc;
+  // largest classes first, dealt round-robin over the streams
+  std::vector<size_t> order(nc);
+  for (size_t i = 0; i < nc; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) {
+    return size_t(b.cfg[a].init_num) * size_t(b.counts[a]) > size_t(b.cfg[c].init_num) * size_t(b.counts[c]);
+  });
+  for (size_t r = 0; r < nc; ++r) {
+    const size_t i = order[r];
+    efg::Workspace& wi = *cws[i];
+    wi.nover_sum.view(w.nover_sum.p, 2);  // the classes add into the batch's sum
+    if (int rc = run_pass_ws(ix, wi, b.cfg[i], LearnArgs{}, false, dq, b.nq, q_stride, d_ids, d_dists, nullptr,
+                             w.blist.p + b.offs[i], w.bcount.p + i, nullptr, ix->bstreams[r % ns], true))
       return rc;
+  }
+  for (int j = 0; j < ns; ++j) {
+    if (int rc = check_hip(hipEventRecord(ix->bjoin[j], ix->bstreams[j]), "batch join")) return rc;
+    if (int rc = check_hip(hipStreamWaitEvent(st, ix->bjoin[j], 0), "batch join")) return rc;
+  }
   return EFG_OK;
 }
 
@@ -1898,7 +1946,7 @@ int search_batch(efg_index* ix, const float* dq, size_t nq, size_t q_stride, con
   BatchSetup b;
   if (int rc = batch_setup(ix, ix->ws, dq, nq, q_stride, pool, ext, kq, out_stride, d_ids, d_dists, tune, st, &b))
     return rc;
-  return batch_launch(ix, ix->ws, b, dq, q_stride, d_ids, d_dists, st);
+  return batch_launch(ix, ix->ws, ix->bws, b, dq, q_stride, d_ids, d_dists, st);
 }
 
 }  // namespace efg
@@ -1909,6 +1957,7 @@ int search_batch(efg_index* ix, const float* dq, size_t nq, size_t q_stride, con
 struct efg_batch_plan {
   efg_index* ix = nullptr;
   efg::Workspace ws;
+  std::vector<std::unique_ptr<efg::Workspace>> cws;  // one per class
   efg::BatchSetup setup;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1926,14 +1975,14 @@ int batch_plan_create(efg_index* ix, const float* dq, size_t nq, size_t q_stride
   pl->ix = ix;
   int rc = batch_setup(ix, pl->ws, dq, nq, q_stride, pool, ext, kq, out_stride, d_ids, d_dists, tune, st, &pl->setup);
   // one eager run sizes the plan's workspace before the capture
-  if (rc == EFG_OK) rc = batch_launch(ix, pl->ws, pl->setup, dq, q_stride, d_ids, d_dists, st);
+  if (rc == EFG_OK) rc = batch_launch(ix, pl->ws, pl->cws, pl->setup, dq, q_stride, d_ids, d_dists, st);
   if (rc == EFG_OK) rc = check_hip(hipStreamSynchronize(st), "batch plan");
   hipStream_t cs = nullptr;
   if (rc == EFG_OK) rc = check_hip(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream");
   if (rc == EFG_OK) {
     rc = check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
     if (rc == EFG_OK) {
-      const int lrc = batch_launch(ix, pl->ws, pl->setup, dq, q_stride, d_ids, d_dists, cs);
+      const int lrc = batch_launch(ix, pl->ws, pl->cws, pl->setup, dq, q_stride, d_ids, d_dists, cs);
       const int erc = check_hip(hipStreamEndCapture(cs, &pl->graph), "end capture");
       rc = lrc ? lrc : erc;
     }
