@@ -157,6 +157,7 @@ struct Workspace {
   DevBuf<int> nover_sum;  // [overflow queries, second overflows] summed over the passes of one search or batch
   DevBuf<unsigned long long> ndist;
   DevBuf<uint32_t> gtab;
+  DevBuf<uint32_t> spill;  // per-workgroup tables that queries outgrowing the LDS table move into
   DevBuf<float> qbuf;
   DevBuf<int32_t> ids;
   DevBuf<float> dists;

@@ -194,6 +194,14 @@ __device__ inline bool tab_insert(uint32_t* tab, int logcap, uint32_t id) {
   }
 }
 
+// Insert a table entry as it is (id with its expanded flag), for moving a
+// visited set into a larger table; ids are unique, so no duplicate check.
+__device__ inline void tab_put_raw(uint32_t* tab, int logcap, uint32_t v) {
+  const uint32_t mask = (1u << logcap) - 1;
+  uint32_t s = hslot(v & ~kExp, logcap);
+  while (atomicCAS(&tab[s], kEmpty, v) != kEmpty) s = (s + 1) & mask;
+}
+
 struct Smem {
   float* q;
   uint64_t* buf;  // seeds; then pool P[Rp] | new N[new_buf] | merge output O[Rp]
@@ -553,7 +561,7 @@ __device__ __noinline__ bool ckpt_eval(int cE, int cK, int cepochs, int clogcap,
 // The BF16-row variant is held to 128 VGPRs (four wavefronts per SIMD, which
 // its LDS allows); the others keep their registers for loads in flight (a cap
 // of four wavefronts made the plain kernel 40 % slower at pool 500).
-template <bool kGlobal, int kMethod, bool kBf16, int kStop, int kThreads>
+template <bool kGlobal, int kMethod, bool kBf16, int kStop, int kThreads, bool kSpill = false>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kBf16 ? 4 : 1)))
 k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
                                                          size_t q_stride,
@@ -569,7 +577,8 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
                                                          const int* __restrict__ nlist_dev, int xcd_slices,
                                                          uint32_t* __restrict__ gtab, int32_t* __restrict__ overflow,
                                                          int* __restrict__ noverflow,
-                                                         unsigned long long* __restrict__ work) {
+                                                         unsigned long long* __restrict__ work,
+                                                         uint32_t* __restrict__ spill, int spill_logcap) {
   const int nlist = nlist_dev ? *nlist_dev : nlist_host;
   constexpr int kCtab = ctab_slots(kThreads), kNB = new_buf(kThreads);
   constexpr int kCtabLog = kThreads == 64 ? 7 : (kThreads == 128 ? 8 : 9);
@@ -650,8 +659,15 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     nwl = reinterpret_cast<uint64_t*>(take(8 * 64));
   }
   const int tid = threadIdx.x;
-  const int cap = 1 << c.logcap;
-  const int limit = cap - cap / 4;  // keep the table under 3/4 load
+  // the visited table of the current query: the LDS (or global-pass)
+  // table; method 0 moves a query that is about to outgrow the LDS table
+  // into this workgroup's spill table in global memory (when given) and
+  // continues there instead of handing it to the global-table pass
+  uint32_t* const tab0 = s.tab;
+  int logcap = c.logcap;
+  int cap = 1 << logcap;
+  int limit = cap - cap / 4;  // keep the table under 3/4 load
+  unsigned long long acc_spill = 0;
   const int pr = tid >> 1, h = tid & 1;  // lane pair for distances
   const int R = c.R, Rp = c.Rp;
   uint64_t* P = s.buf;
@@ -748,6 +764,12 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     const int qi = qlist ? qlist[w] : w;
     const float* qg = Q + size_t(qi) * q_stride;
     for (int i = tid; i < c.D; i += kThreads) s.q[i] = (i < c.d) ? qg[i] : 0.f;
+    if (kSpill && s.tab != tab0) {  // the last query moved to the spill table
+      s.tab = tab0;
+      logcap = c.logcap;
+      cap = 1 << logcap;
+      limit = cap - cap / 4;
+    }
     for (int i = tid; i < cap; i += kThreads) s.tab[i] = kEmpty;
     // leaf sequence: sizes and prefix in (rank, tree) order
     int run = 0;
@@ -780,7 +802,7 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       bool present = false;
       int cs = -1;
       if (valid) {
-        tab_find(s.tab, c.logcap, id, &present);
+        tab_find(s.tab, logcap, id, &present);
         if (!present) {
           uint32_t sl = hslot(id, kCtabLog);
           for (;;) {
@@ -802,7 +824,7 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       int cnt;
       int rank = block_exclusive_scan(first, s.scan, &cnt);
       if (first && p + rank < c.init_num) {
-        tab_insert(s.tab, c.logcap, id);
+        tab_insert(s.tab, logcap, id);
         s.buf[p + rank] = uint64_t(id);
       }
       __syncthreads();
@@ -824,7 +846,7 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     // Its count alternates between misc[6] and misc[7], each cleared while
     // the other is in use.
     auto seed_fast = [&](bool valid, uint32_t id, int par) {
-      if (valid && tab_insert(s.tab, c.logcap, id)) s.buf[p + atomicAdd(&s.misc[6 + par], 1)] = uint64_t(id);
+      if (valid && tab_insert(s.tab, logcap, id)) s.buf[p + atomicAdd(&s.misc[6 + par], 1)] = uint64_t(id);
       __syncthreads();
       const int cnt = s.misc[6 + par];
       if (tid == 0) s.misc[7 - par] = 0;
@@ -916,7 +938,7 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
             uint64_t e = 0;
             if (j < c.E) {
               e = pool[size_t(it) * c.E + j];
-              take = tab_insert(s.tab, c.logcap, key_id(e));
+              take = tab_insert(s.tab, logcap, key_id(e));
             }
             const uint64_t mk = __ballot(take);
             if (take) {
@@ -960,7 +982,7 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
                   dupl = dupl || (o2 < lane && other == id);
                 }
                 bool isnew = false;
-                if (m < maxM && !dupl) isnew = tab_insert(s.tab, c.logcap, id);
+                if (m < maxM && !dupl) isnew = tab_insert(s.tab, logcap, id);
                 const uint64_t mk = __ballot(isnew);
                 const int nn = __popcll(mk);
                 ins += nn;
@@ -1029,7 +1051,7 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
         if (j < c.E) {
           id = key_id(P[j]);
           bool pres;
-          sl = tab_find(s.tab, c.logcap, id, &pres);
+          sl = tab_find(s.tab, logcap, id, &pres);
           take = (s.tab[sl] & kExp) ? 0 : 1;
         }
         int tot;
@@ -1080,12 +1102,29 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
             uint32_t e = s.explist[lo];
             id = gids[goff[e] + uint64_t(sq - s.expoff[lo])];
           }
-          if (tab_insert(s.tab, c.logcap, id)) N[atomicAdd(&s.misc[0], 1)] = uint64_t(id);
+          if (tab_insert(s.tab, logcap, id)) N[atomicAdd(&s.misc[0], 1)] = uint64_t(id);
         }
         __syncthreads();
         const int m2 = s.misc[0];
         inserted += m2 - m;
         m = m2;
+        if (kSpill && inserted + kThreads > limit && !kGlobal && kStop == 0 && spill && logcap < spill_logcap) {
+          // move the visited set into this workgroup's spill table and go on
+          uint32_t* g = spill + (size_t(blockIdx.x) << spill_logcap);
+          const int gcap = 1 << spill_logcap;
+          for (int i = tid; i < gcap; i += kThreads) g[i] = kEmpty;
+          __syncthreads();
+          for (int i = tid; i < cap; i += kThreads) {
+            const uint32_t v = s.tab[i];
+            if (v != kEmpty) tab_put_raw(g, spill_logcap, v);
+          }
+          __syncthreads();
+          s.tab = g;
+          logcap = spill_logcap;
+          cap = gcap;
+          limit = cap - cap / 4;
+          if (tid == 0) ++acc_spill;
+        }
         if (inserted + kThreads > limit) {
           ovf = true;
           break;
@@ -1165,6 +1204,7 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
   }
   // counters: one atomic per workgroup (same-address atomics per query would
   // serialise the whole grid)
+  if (tid == 0 && acc_spill) atomicAdd(noverflow + 2, int(acc_spill));
   if (tid == 0) {
     if (acc_nd | acc_seed | acc_exp) {
       atomicAdd(ndist_total, acc_nd);
@@ -1320,7 +1360,7 @@ int make_cfg(efg_index* ix, int k, int init, int E, int epochs, int trees, int m
 using SearchKernel = void (*)(SearchCfg, LearnArgs, const float*, size_t, const float*, const uint16_t*,
                               const int32_t*, const SNode*, const uint32_t*, const uint64_t*, const uint32_t*,
                               int32_t*, float*, uint64_t*, unsigned long long*, const int32_t*, int, const int*, int,
-                              uint32_t*, int32_t*, int*, unsigned long long*);
+                              uint32_t*, int32_t*, int*, unsigned long long*, uint32_t*, int);
 
 // Capacity (log2 entries) of a global-memory visited table: the worst-case
 // visited set of a query (seeds plus every expansion), at most 2^26.
@@ -1343,9 +1383,15 @@ __global__ void k_zero_i32(int* p, int m) {
   if (i < m) p[i] = 0;
 }
 
-__global__ void k_add_i32(int* __restrict__ acc, const int* __restrict__ v, int m) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m) atomicAdd(acc + i, v[i]);  // batch classes on parallel streams share the sum
+// acc[0] += queries that outgrew the LDS table (re-run in the global pass,
+// v[0], or moved to a spill table, v[2]); acc[1] += those that outgrew the
+// global table too (v[1]).  Atomic: batch classes on parallel streams share
+// the sum.
+__global__ void k_add_over(int* __restrict__ acc, const int* __restrict__ v) {
+  if (threadIdx.x == 0) {
+    atomicAdd(acc, v[0] + v[2]);
+    atomicAdd(acc + 1, v[1]);
+  }
 }
 
 // (The pass clears its counters with a kernel, not a memset, so that a
@@ -1355,7 +1401,7 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
                 const int32_t* qlist, const int* nlist_dev, hipEvent_t* ev, hipStream_t st, bool count_work) {
   unsigned long long* work = count_work ? ix->work.p : nullptr;
   const int nb_q = 4096;
-  if (!w.leaves.ensure(nq * size_t(c.nleaf)) || !w.overflow.ensure(2 * nq) || !w.nover.ensure(2) ||
+  if (!w.leaves.ensure(nq * size_t(c.nleaf)) || !w.overflow.ensure(2 * nq) || !w.nover.ensure(3) ||
       !w.nover_sum.ensure(2) || !w.ndist.ensure(3) || !w.qorder.ensure(nq) || !w.qkey.ensure(nq) ||
       !w.qhist.ensure(nb_q + 1) || !w.qoff.ensure(nb_q + 1))
     return fail(EFG_ERR_MEMORY, "out of device memory (search workspace)");
@@ -1363,7 +1409,7 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
   const size_t lds_cap = 150 * 1024;
   const bool table_in_lds = lds <= lds_cap;
   if (!table_in_lds && search_lds_bytes(c, true) > lds_cap) return fail(EFG_ERR_ARG, "pool too large for one workgroup");
-  k_zero_i32<<<1, 64, 0, st>>>(w.nover.p, 2);
+  k_zero_i32<<<1, 64, 0, st>>>(w.nover.p, 3);
   if (ev) hipEventRecord(ev[0], st);
   // seed slots: bucket probes on a hashing index, tree descent otherwise
   const SNode* snp = ix->hashing ? ix->h_nodes.p : ix->snode.p;
@@ -1400,6 +1446,12 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   const int32_t* order = qlist ? qlist : w.qorder.p;
   const int tq = c.threads;
+  // passes over query lists (batch classes, the learned form's full
+  // searches) move queries that outgrow the LDS table into a per-workgroup
+  // spill table and continue; the plain pass over all queries keeps its
+  // registers (the spill path costs a wave per SIMD) and hands such queries
+  // to the global-table pass.  Workspaces of fixed size never spill.
+  const bool use_spill = qlist != nullptr && c.method == 0 && c.stop == 0 && !bf16 && w.spill.owned;
   auto pick = [&](bool global) -> SearchKernel {
 #define EFG_K(G, M, B, S, T) &k_search_nnexp<G, M, B, S, T>
 #define EFG_KT(G, M, B, S) (tq == 64 ? EFG_K(G, M, B, S, 64) : tq == 128 ? EFG_K(G, M, B, S, 128) : EFG_K(G, M, B, S, 256))
@@ -1408,6 +1460,10 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
     if (c.stop == 1) return global ? nullptr : EFG_KT(false, 0, false, 1);
     if (c.stop == 2) return global ? nullptr : EFG_KT(false, 0, false, 2);
     if (bf16) return global ? EFG_K(true, 0, true, 0, 256) : EFG_K(false, 0, true, 0, 256);
+    if (!global && use_spill)
+      return tq == 64 ? &k_search_nnexp<false, 0, false, 0, 64, true>
+                      : tq == 128 ? &k_search_nnexp<false, 0, false, 0, 128, true>
+                                  : &k_search_nnexp<false, 0, false, 0, 256, true>;
     return global ? EFG_K(true, 0, false, 0, 256) : EFG_KT(false, 0, false, 0);
 #undef EFG_KT
 #undef EFG_K
@@ -1419,11 +1475,23 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
     size_t grid = size_t(ncu) * per_cu;
     if (grid > nq) grid = (nq + 7) & ~size_t(7);
     auto kern = pick(false);
+    // method 0 (plain): a spill table per workgroup, so that queries
+    // outgrowing the LDS table continue in place (not with a fixed-size
+    // workspace, which keeps the global-table pass)
+    uint32_t* spill = nullptr;
+    int spill_logcap = 0;
+    if (use_spill) {
+      spill_logcap = global_logcap(ix, c);
+      if (spill_logcap > c.logcap) {
+        if (!w.spill.ensure(grid << spill_logcap)) return fail(EFG_ERR_MEMORY, "out of device memory (spill tables)");
+        spill = w.spill.p;
+      }
+    }
     hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     kern<<<unsigned(grid), unsigned(tq), lds, st>>>(c, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p,
                                                     snp, lists, ix->goff.p, ix->gids.p, d_ids, d_dists,
                                                     d_ndist, w.ndist.p, order, int(nq), nlist_dev, qlist ? 0 : 1,
-                                                    nullptr, w.overflow.p, w.nover.p, work);
+                                                    nullptr, w.overflow.p, w.nover.p, work, spill, spill_logcap);
   } else if (c.stop) {
     return fail(EFG_ERR_ARG, "probe pool too large for one workgroup");
   }
@@ -1450,16 +1518,16 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
       kern2<<<grid2, kBlock, lds2, st>>>(c2, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, snp,
                                          lists, ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
                                          w.overflow.p, 0, w.nover.p, 0, w.gtab.p, w.overflow.p + nq, w.nover.p + 1,
-                                         work);
+                                         work, nullptr, 0);
     } else {
       kern2<<<grid2, kBlock, lds2, st>>>(c2, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, snp,
                                          lists, ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
                                          order, int(nq), nlist_dev, 0, w.gtab.p, w.overflow.p + nq, w.nover.p + 1,
-                                         work);
+                                         work, nullptr, 0);
     }
   }
   if (ev) hipEventRecord(ev[3], st);
-  k_add_i32<<<1, 64, 0, st>>>(w.nover_sum.p, w.nover.p, 2);
+  k_add_over<<<1, 64, 0, st>>>(w.nover_sum.p, w.nover.p);
   return check_hip(hipGetLastError(), "search launch");
 }
 
@@ -1672,7 +1740,7 @@ int search_device_noalloc(efg_index* ix, int k, const float* dq, size_t nq, size
   if (int rc = scratch_address(&base)) return rc;
   efg::Workspace w;  // views of the scratch: nothing below allocates
   w.ndist.view(reinterpret_cast<unsigned long long*>(base + o_ndist), 3);
-  w.nover.view(reinterpret_cast<int*>(base + o_nover), 2);
+  w.nover.view(reinterpret_cast<int*>(base + o_nover), 3);
   w.nover_sum.view(reinterpret_cast<int*>(base + o_nsum), 2);
   w.scan_sums.view(reinterpret_cast<int32_t*>(base + o_sums), 64);
   w.qhist.view(reinterpret_cast<uint32_t*>(base + o_hist), nb_q + 1);

@@ -312,11 +312,12 @@ int efg_set_graph(efg_index* ix, const uint64_t* off, const uint32_t* ids);
 int efg_device_data(const efg_index* ix, const float** d_data, size_t* stride);
 /* Counters: [0] distances in the last search, [1] merge distances, [2] init
  * distances, [3] join distances (pairs), [4] queries of the last search or
- * batch (all its passes) whose visited set outgrew the LDS table and ran
- * again with a table in global memory, [5] those that outgrew the largest
- * global table too (their result rows are -1 / FLT_MAX), [6] join rows, [7]
- * join iterations whose
- * candidates outgrew the staging estimate and were joined again. */
+ * batch (all its passes) whose visited set outgrew the LDS table (in a pass
+ * over a query list they move to a spill table in global memory and go on;
+ * otherwise they run again with a table in global memory), [5] those that
+ * outgrew the largest global table too (their result rows are -1 /
+ * FLT_MAX), [6] join rows, [7] join iterations whose candidates outgrew the
+ * staging estimate and were joined again. */
 int efg_get_counters(const efg_index* ix, uint64_t* out, int n);
 /* Per-iteration join pair counts of the last build. */
 int efg_get_join_pairs(const efg_index* ix, uint64_t* out, int cap, int* n);
