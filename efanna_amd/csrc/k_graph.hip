@@ -673,16 +673,20 @@ __device__ inline void join_cell(int k, bool same, int sizeA, int sizeB, int& p,
 
 // Endpoints of the pair at positions (p, q) of blocks I and J: slots and
 // ids, the first endpoint (whose candidate carries the kFirst credit tag)
-// being the lower id of a new x new pair.  False for a new x new pair of one
-// id, which the reference skips.
+// being the lower id of a new x new pair.  False for a pair of one id: the
+// reference skips it in new x new (base_index.hpp:99).  In new x old it
+// cannot arise under a symmetric distance (an id within a node's Range is
+// not beyond its radiusM, so it is not also on the node's reverse list), but
+// under the precision form's approximate keys d(x, y) and d(y, x) may differ
+// by a rounding, and the pair would put x into its own pool at distance 0.
 __device__ inline bool pair_ends(int p, int q, bool same, int newA, int newB, const uint32_t* ids_s, int& sa,
                                  int& sb, uint32_t& a, uint32_t& b) {
   sa = p;
   sb = same ? q : 32 + q;
   a = ids_s[sa];
   b = ids_s[sb];
+  if (a == b) return false;
   if (q < (same ? newA : newB)) {  // new x new: once, first endpoint = lower id
-    if (a == b) return false;
     if (b < a) {
       const uint32_t t2 = a;
       a = b;
@@ -710,7 +714,8 @@ __device__ inline bool block_ends(int p, int q, bool same, int sizeA, int newA, 
 
 // Positions P < Q of a node's rows (new list, then old list): the pair the
 // reference measures when P is new; slots and ids of the two endpoints, the
-// first endpoint being the lower id of a new x new pair (as pair_ends).
+// first endpoint being the lower id of a new x new pair (as pair_ends, and
+// like it false for a pair of one id).
 __device__ inline bool node_ends(int P, int Q, int na, int m, const uint32_t* ids, int& sa, int& sb, uint32_t& a,
                                  uint32_t& b) {
   if (P >= na || Q >= m || Q <= P) return false;
@@ -718,8 +723,8 @@ __device__ inline bool node_ends(int P, int Q, int na, int m, const uint32_t* id
   sb = Q;
   a = ids[P];
   b = ids[Q];
+  if (a == b) return false;
   if (Q < na) {
-    if (a == b) return false;
     if (b < a) {
       const uint32_t t2 = a;
       a = b;
@@ -1745,8 +1750,10 @@ __device__ inline uint32_t cand_tag(uint32_t s) {
 // kById (precision form of the build): a pair's approximate distance may
 // differ from the FP32 distance its id already carries in the pool (from the
 // merge's re-ranked sets or the initial top-up), so duplicates are dropped by
-// id, the first in (key, tag) order staying, through a per-wavefront LDS hash
-// of the window's ids holding the smallest position of each.
+// id through a per-wavefront LDS hash of the window's ids: an entry already
+// in the pool stays (as an equal key does in the FP32 form, where a
+// duplicate's key is the pool entry's), else the first candidate in (key,
+// tag) order.
 template <int kCap, bool kById>
 __global__ void __launch_bounds__(kBlock) k_merge_wave(uint32_t n, int L, int CK, const int32_t* __restrict__ toff,
                                                        const uint64_t* __restrict__ bkey,
@@ -1820,7 +1827,7 @@ __global__ void __launch_bounds__(kBlock) k_merge_wave(uint32_t n, int L, int CK
           wave_sync();
         }
       }
-      if (kById) {  // smallest position of every id of the window
+      if (kById) {  // per id of the window: its pool entry, else its first candidate
         uint32_t* hi_ = hid[w];
         uint32_t* hp = hpos[w];
         for (int i = lane; i < kHS; i += 64) {
@@ -1836,11 +1843,11 @@ __global__ void __launch_bounds__(kBlock) k_merge_wave(uint32_t n, int L, int CK
             if (prev == 0xffffffffu || prev == id) break;
             h = (h + 1) & uint32_t(kHS - 1);
           }
-          atomicMin(&hp[h], uint32_t(i));
+          atomicMin(&hp[h], (tg[i] & 0x80000000u) | uint32_t(i));  // a pool entry before any candidate
         }
         wave_sync();
       }
-      // drop duplicates (the first of equal keys, or of equal ids, stays), compact to the front
+      // drop duplicates (the first of equal keys stays; of equal ids, the pool entry or else the first), compact to the front
       int kept = 0;
       for (int b0 = 0; b0 < m && kept < L; b0 += 64) {
         const int i = b0 + lane;
@@ -1854,7 +1861,7 @@ __global__ void __launch_bounds__(kBlock) k_merge_wave(uint32_t n, int L, int CK
             const uint32_t id = key_id(kv);
             uint32_t h = (id * 2654435761u) & uint32_t(kHS - 1);
             while (hid[w][h] != id) h = (h + 1) & uint32_t(kHS - 1);
-            first = hpos[w][h] == uint32_t(i);
+            first = hpos[w][h] == ((tv & 0x80000000u) | uint32_t(i));
           } else {
             first = i == 0 || k[i - 1] != kv;
           }
@@ -1903,9 +1910,9 @@ __device__ __forceinline__ bool kt_less(uint64_t ka, uint32_t ta, uint64_t kb, u
 // number of candidates before it.  Exactly equal entries put the list's
 // first.  Then duplicates are dropped and the first L copied back as the new
 // running list.  One wavefront per target.  kById (precision form of the
-// build): duplicates are dropped by id, the first in (key, tag) order
-// staying, through an LDS hash of the window's ids holding the smallest
-// position of each (as in k_merge_wave).
+// build): duplicates are dropped by id, an entry already in the pool staying
+// (else the first in (key, tag) order), through an LDS hash of the window's
+// ids (as in k_merge_wave).
 template <int kLMax, bool kById>
 __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK, const int32_t* __restrict__ toff,
                                                       const uint64_t* __restrict__ bkey,
@@ -2007,7 +2014,7 @@ __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK,
       }
       wave_sync();
       const int mm = cur + m;
-      if (kById) {  // smallest position of every id of the window
+      if (kById) {  // per id of the window: its pool entry, else its first candidate
         uint32_t* hi_ = hid_s[w];
         uint32_t* hp = hpos_s[w];
         for (int i = lane; i < kHS; i += 64) {
@@ -2023,11 +2030,11 @@ __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK,
             if (prev == 0xffffffffu || prev == id) break;
             h = (h + 1) & uint32_t(kHS - 1);
           }
-          atomicMin(&hp[h], uint32_t(i));
+          atomicMin(&hp[h], (BT[i] & 0x80000000u) | uint32_t(i));  // a pool entry before any candidate
         }
         wave_sync();
       }
-      // drop duplicates (the first of equal keys, or of equal ids, stays);
+      // drop duplicates (the first of equal keys stays; of equal ids, the pool entry or else the first);
       // the first L kept become the running list
       int kept = 0;
       for (int b0 = 0; b0 < mm && kept < L; b0 += 64) {
@@ -2042,7 +2049,7 @@ __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK,
             const uint32_t id = key_id(kv);
             uint32_t h = (id * 2654435761u) & uint32_t(kHS - 1);
             while (hid_s[w][h] != id) h = (h + 1) & uint32_t(kHS - 1);
-            first = hpos_s[w][h] == uint32_t(i);
+            first = hpos_s[w][h] == ((tv & 0x80000000u) | uint32_t(i));
           } else {
             first = i == 0 || B[i - 1] != kv;
           }

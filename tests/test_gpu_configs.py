@@ -80,6 +80,13 @@ def test_d960_shape_bf16_build(cuda):
     h.setBuildPrecision(1)
     h.buildIndex()
     kh, _, nh, _ = h.pools()
+    # approximate keys may differ between d(x, y) and d(y, x); no row may
+    # still meet itself in a join pair and enter its own pool
+    n = base.shape[0]
+    live = np.arange(kh.shape[1])[None, :] < nh.astype(np.int64)[:, None]
+    own = (kh & np.uint64(0xFFFFFFFF)).astype(np.int64) == np.arange(n)[:, None]
+    assert not (own & live).any()
+    assert not (h.graph_matrix() == np.arange(n)[:, None]).any()
     rng = np.random.default_rng(4)
     for p in rng.choice(base.shape[0], 300, replace=False):
         keys = kh[p, :nh[p]]
