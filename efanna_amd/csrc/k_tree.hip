@@ -10,16 +10,19 @@
 // on (DFS) a node is split iff its size > TNS.
 //
 // Per level:
-//   k_split_stats  one workgroup per active node: mean and unnormalised
-//                  variance over the first min(101, count) ids of the node's
-//                  range (sequential sums per dimension), top-5 variance
-//                  dimensions, the choice among them from the counter-based
-//                  generator, cut value = mean.
+//   k_split_stats  a group of threads per active node: mean and
+//                  unnormalised variance over the first min(101, count) ids
+//                  of the node's range (sequential sums per dimension), top-5
+//                  variance dimensions, the choice among them from the
+//                  counter-based generator, cut value = mean; a node of at
+//                  most one id per thread of its group is partitioned and
+//                  split right there.
 //   k_part_count / k_part_node / k_part_scatter / k_part_copy
-//                  stable three-way partition (<, ==, > cut) of every active
-//                  range in blocks of 1024 ids: per-block class counts, a
-//                  per-node scan that also applies the balancing rule and
-//                  creates the children, then the scatter and copy back.
+//                  stable three-way partition (<, ==, > cut) of every larger
+//                  active range in blocks of 1024 ids: per-block class
+//                  counts, a per-node scan that also applies the balancing
+//                  rule and creates the children, then the scatter and copy
+//                  back.
 //   k_next_flags + scan + k_next_compact
 //                  children that stay active, in child order.
 #include <algorithm>
@@ -50,28 +53,75 @@ __global__ void k_init_roots(int4* node4, int32_t* parent, uint2* range, int32_t
   roots[t] = t;
 }
 
+__device__ inline int classify(float v, float cut) { return v < cut ? 0 : (v <= cut ? 1 : 2); }
+
+// Children of active node ni (slot a) after its partition into c0 ids below
+// the cut, c1 equal to it and the rest above: the balancing rule
+// (kdtreeub_index.hpp:943-950) and the two child nodes.
+__device__ inline void make_children(int ni, int a, int32_t base, uint2 r, int4 nd, uint32_t c0, uint32_t c1,
+                                     int4* __restrict__ node4, int32_t* __restrict__ parent,
+                                     uint2* __restrict__ range, int32_t* __restrict__ tree_of) {
+  const unsigned count = r.y - r.x;
+  const unsigned lim1 = c0, lim2 = c0 + c1;
+  unsigned index;
+  if (lim1 > count / 2) index = lim1;
+  else if (lim2 < count / 2) index = lim2;
+  else index = count / 2;
+  if (lim1 == count || lim2 == 0) index = count / 2;
+  const int32_t L = base + 2 * a, R = base + 2 * a + 1;
+  nd.z = L;
+  nd.w = R;
+  node4[ni] = nd;
+  node4[L] = make_int4(-1, 0, -1, -1);
+  node4[R] = make_int4(-1, 0, -1, -1);
+  parent[L] = ni;
+  parent[R] = ni;
+  range[L] = make_uint2(r.x, r.x + index);
+  range[R] = make_uint2(r.x + index, r.y);
+  tree_of[L] = tree_of[ni];
+  tree_of[R] = tree_of[ni];
+}
+
 // Split statistics: a group of `tpn` threads (a multiple of 64) per active
 // node, `kBlock / tpn` nodes per workgroup.  A thread owns dimensions k,
 // k + tpn, ...; the up to 101 sampled values of its dimension are loaded
 // kChunk at a time into registers (the row index is clamped, values past the
 // sample are never used) and summed in row order as the reference does; the
-// variance pass loads them again.  The first
-// wavefront of the group picks the top-5 variance dimensions by five arg-max
-// rounds over (variance descending, dimension ascending), which is the order
-// the reference's insertion keeps.
+// variance pass loads them again (from cache).  Row offsets are 32-bit when
+// the rows span fewer than 2^30 floats (fewer registers per load in flight;
+// the kernel is bound by the latency of its short per-node steps, so it
+// needs the occupancy).  The first wavefront of the group picks the top-5 variance
+// dimensions by five arg-max rounds over (variance descending, dimension
+// ascending), which is the order the reference's insertion keeps.
+//
+// A node of at most `small` ids (at most one per thread of its group) is
+// also partitioned here, in the order k_part_count / k_part_scatter give:
+// each thread classifies one id (the row was just read by the statistics),
+// wavefront ballots and per-wavefront counts give every id its place in the
+// stable three-way order, the ids are written back over the node's range
+// and the children are made.  The block partition kernels skip such nodes.
 constexpr int kSample = 101;
 constexpr int kChunk = 32;
-__global__ void __launch_bounds__(kBlock) k_split_stats(const float* __restrict__ X, int d, int D,
+template <bool kOff32>
+__device__ __forceinline__ auto row_off(uint32_t id, int D) {
+  if constexpr (kOff32) return id * uint32_t(D);
+  else return size_t(id) * size_t(D);
+}
+template <bool kOff32>
+__global__ void __launch_bounds__(kBlock, 5) k_split_stats(const float* __restrict__ X, int d, int D,
                                                         const int32_t* __restrict__ active, int na,
-                                                        const uint2* __restrict__ range,
-                                                        const int32_t* __restrict__ tree_of,
-                                                        const uint32_t* __restrict__ leaf, uint32_t n, uint64_t seed,
-                                                        int4* __restrict__ node4, int tpn) {
+                                                        uint2* __restrict__ range, int32_t* __restrict__ tree_of,
+                                                        uint32_t* __restrict__ leaf, uint32_t n, uint64_t seed,
+                                                        int4* __restrict__ node4, int32_t* __restrict__ parent,
+                                                        int tpn, int small, int32_t base) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int G = kBlock / tpn, g = threadIdx.x / tpn, lt = threadIdx.x - g * tpn;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   float* mean = reinterpret_cast<float*>(smem_raw) + size_t(g) * 2 * d;
   float* var = mean + d;
   __shared__ uint32_t ids_s[kBlock / 64][kSample];
+  __shared__ int cd_s[kBlock / 64];
+  __shared__ uint32_t wcnt[kBlock / 64][3];
   for (int a0 = blockIdx.x * G; a0 < na; a0 += gridDim.x * G) {
     const int a = a0 + g;
     const bool valid = a < na;
@@ -84,6 +134,9 @@ __global__ void __launch_bounds__(kBlock) k_split_stats(const float* __restrict_
     }
     const unsigned count = r.y - r.x;
     const int cnt = int(count < unsigned(kSample) ? count : unsigned(kSample));
+    const bool part = valid && count <= unsigned(small);  // the same for the whole group
+    uint32_t my_id = 0;
+    if (part && unsigned(lt) < count) my_id = leaf[size_t(t) * n + r.x + lt];
     if (valid)
       for (int j = lt; j < cnt; j += tpn) ids_s[g][j] = leaf[size_t(t) * n + r.x + j];
     __syncthreads();
@@ -96,7 +149,7 @@ __global__ void __launch_bounds__(kBlock) k_split_stats(const float* __restrict_
 #pragma unroll
           for (int j = 0; j < kChunk; ++j) {
             const int jj = j0 + j < cnt ? j0 + j : cnt - 1;
-            xv[j] = X[size_t(ids_s[g][jj]) * D + k];
+            xv[j] = X[row_off<kOff32>(ids_s[g][jj], D) + k];
           }
 #pragma unroll
           for (int j = 0; j < kChunk; ++j)
@@ -109,7 +162,7 @@ __global__ void __launch_bounds__(kBlock) k_split_stats(const float* __restrict_
 #pragma unroll
           for (int j = 0; j < kChunk; ++j) {
             const int jj = j0 + j < cnt ? j0 + j : cnt - 1;
-            xv[j] = X[size_t(ids_s[g][jj]) * D + k];
+            xv[j] = X[row_off<kOff32>(ids_s[g][jj], D) + k];
           }
 #pragma unroll
           for (int j = 0; j < kChunk; ++j)
@@ -153,10 +206,56 @@ __global__ void __launch_bounds__(kBlock) k_split_stats(const float* __restrict_
         int cd = top[0];
         for (int z = 1; z < 5; ++z)
           if (uint64_t(z) == rnd) cd = top[z];
-        int4 nd = node4[ni];
-        nd.x = cd;
-        nd.y = __float_as_int(mean[cd]);
-        node4[ni] = nd;
+        cd_s[g] = cd;
+        if (!part) {
+          int4 nd = node4[ni];
+          nd.x = cd;
+          nd.y = __float_as_int(mean[cd]);
+          node4[ni] = nd;
+        }
+      }
+    }
+    __syncthreads();
+    // small nodes: stable three-way partition (<, ==, > cut) of the ids
+    int c = 3;
+    uint64_t m0 = 0, m1 = 0, m2 = 0;
+    if (part) {
+      const int cd = cd_s[g];
+      const float cut = mean[cd];
+      if (unsigned(lt) < count) c = classify(X[size_t(my_id) * D + cd], cut);
+      m0 = __ballot(c == 0);
+      m1 = __ballot(c == 1);
+      m2 = __ballot(c == 2);
+      if (lane == 0) {
+        wcnt[wid][0] = uint32_t(__popcll(m0));
+        wcnt[wid][1] = uint32_t(__popcll(m1));
+        wcnt[wid][2] = uint32_t(__popcll(m2));
+      }
+    }
+    __syncthreads();
+    if (part) {
+      const int w0 = g * (tpn >> 6);
+      uint32_t p0 = 0, p1 = 0, p2 = 0, c0 = 0, c1 = 0;
+      for (int w = w0; w < w0 + (tpn >> 6); ++w) {
+        if (w < wid) {
+          p0 += wcnt[w][0];
+          p1 += wcnt[w][1];
+          p2 += wcnt[w][2];
+        }
+        c0 += wcnt[w][0];
+        c1 += wcnt[w][1];
+      }
+      const uint64_t below = (1ull << lane) - 1ull;
+      if (c != 3) {
+        const uint32_t dst = c == 0 ? p0 + uint32_t(__popcll(m0 & below))
+                                    : (c == 1 ? c0 + p1 + uint32_t(__popcll(m1 & below))
+                                              : c0 + c1 + p2 + uint32_t(__popcll(m2 & below)));
+        leaf[size_t(t) * n + r.x + dst] = my_id;
+      }
+      if (lt == 0) {
+        const int cd = cd_s[g];
+        make_children(ni, a, base, r, make_int4(cd, __float_as_int(mean[cd]), -1, -1), c0, c1, node4, parent,
+                      range, tree_of);
       }
     }
     __syncthreads();
@@ -172,8 +271,6 @@ __device__ inline int find_node(const int32_t* __restrict__ bstart, int na, int 
   }
   return lo;
 }
-
-__device__ inline int classify(float v, float cut) { return v < cut ? 0 : (v <= cut ? 1 : 2); }
 
 __global__ void __launch_bounds__(kBlock) k_part_count(const float* __restrict__ X, int D,
                                                        const int32_t* __restrict__ active, int na,
@@ -218,50 +315,56 @@ __global__ void __launch_bounds__(kBlock) k_part_count(const float* __restrict__
   }
 }
 
-// One thread per active node: per-block exclusive offsets, lim1/lim2, the
-// balancing rule (kdtreeub_index.hpp:943-950), children.
+// One wavefront per active node of more than `small` ids: class totals, then
+// per-block exclusive offsets (class bases [<][==][>] added) in rounds of 64
+// blocks, lim1/lim2, the balancing rule, children.
 __global__ void k_part_node(const int32_t* __restrict__ active, int na, const int32_t* __restrict__ bstart,
                             uint32_t* __restrict__ bcnt, int4* __restrict__ node4, int32_t* __restrict__ parent,
-                            uint2* __restrict__ range, int32_t* __restrict__ tree_of, int32_t base) {
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= na) return;
+                            uint2* __restrict__ range, int32_t* __restrict__ tree_of, int32_t base, int small) {
+  const int a = int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (a >= na) return;  // whole wavefronts
   const int ni = active[a];
-  const int b0 = bstart[a], b1 = bstart[a + 1];
-  uint32_t c0 = 0, c1 = 0, c2 = 0;
-  for (int b = b0; b < b1; ++b) {
-    uint32_t x0 = bcnt[size_t(b) * 3], x1 = bcnt[size_t(b) * 3 + 1], x2 = bcnt[size_t(b) * 3 + 2];
-    bcnt[size_t(b) * 3] = c0;
-    bcnt[size_t(b) * 3 + 1] = c1;
-    bcnt[size_t(b) * 3 + 2] = c2;
-    c0 += x0;
-    c1 += x1;
-    c2 += x2;
-  }
-  for (int b = b0; b < b1; ++b) {  // class bases: [<][==][>]
-    bcnt[size_t(b) * 3 + 1] += c0;
-    bcnt[size_t(b) * 3 + 2] += c0 + c1;
-  }
   const uint2 r = range[ni];
-  const unsigned count = r.y - r.x;
-  const unsigned lim1 = c0, lim2 = c0 + c1;
-  unsigned index;
-  if (lim1 > count / 2) index = lim1;
-  else if (lim2 < count / 2) index = lim2;
-  else index = count / 2;
-  if (lim1 == count || lim2 == 0) index = count / 2;
-  const int32_t L = base + 2 * a, R = base + 2 * a + 1;
-  int4 nd = node4[ni];
-  nd.z = L;
-  nd.w = R;
-  node4[ni] = nd;
-  node4[L] = make_int4(-1, 0, -1, -1);
-  node4[R] = make_int4(-1, 0, -1, -1);
-  parent[L] = ni;
-  parent[R] = ni;
-  range[L] = make_uint2(r.x, r.x + index);
-  range[R] = make_uint2(r.x + index, r.y);
-  tree_of[L] = tree_of[ni];
-  tree_of[R] = tree_of[ni];
+  if (r.y - r.x <= unsigned(small)) return;  // partitioned with its statistics
+  const int b0 = bstart[a], b1 = bstart[a + 1];
+  uint32_t c0 = 0, c1 = 0;
+  for (int b = b0 + lane; b < b1; b += 64) {
+    c0 += bcnt[size_t(b) * 3];
+    c1 += bcnt[size_t(b) * 3 + 1];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    c0 += __shfl_xor(c0, o);
+    c1 += __shfl_xor(c1, o);
+  }
+  uint32_t run0 = 0, run1 = 0, run2 = 0;
+  for (int bb = b0; bb < b1; bb += 64) {
+    const int b = bb + lane;
+    uint32_t x0 = 0, x1 = 0, x2 = 0;
+    if (b < b1) {
+      x0 = bcnt[size_t(b) * 3];
+      x1 = bcnt[size_t(b) * 3 + 1];
+      x2 = bcnt[size_t(b) * 3 + 2];
+    }
+    uint32_t s0 = x0, s1 = x1, s2 = x2;  // inclusive scans over the round
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y0 = __shfl_up(s0, o), y1 = __shfl_up(s1, o), y2 = __shfl_up(s2, o);
+      if (lane >= o) {
+        s0 += y0;
+        s1 += y1;
+        s2 += y2;
+      }
+    }
+    if (b < b1) {
+      bcnt[size_t(b) * 3] = run0 + s0 - x0;
+      bcnt[size_t(b) * 3 + 1] = c0 + run1 + s1 - x1;
+      bcnt[size_t(b) * 3 + 2] = c0 + c1 + run2 + s2 - x2;
+    }
+    run0 += __shfl(s0, 63);
+    run1 += __shfl(s1, 63);
+    run2 += __shfl(s2, 63);
+  }
+  if (lane == 0) make_children(ni, a, base, r, node4[ni], c0, c1, node4, parent, range, tree_of);
 }
 
 __global__ void __launch_bounds__(kBlock) k_part_scatter(const int32_t* __restrict__ active, int na,
@@ -322,13 +425,14 @@ __global__ void __launch_bounds__(kBlock) k_part_copy(const int32_t* __restrict_
   }
 }
 
-// blocks per active node
+// partition blocks per active node (none for a node of at most `small` ids)
 __global__ void k_blocks_per_node(const int32_t* __restrict__ active, int na, const uint2* __restrict__ range,
-                                  int32_t* __restrict__ nb) {
+                                  int small, int32_t* __restrict__ nb) {
   int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= na) return;
   uint2 r = range[active[a]];
-  nb[a] = int((r.y - r.x + kPartB - 1) / kPartB);
+  const uint32_t c = r.y - r.x;
+  nb[a] = c <= uint32_t(small) ? 0 : int((c + kPartB - 1) / kPartB);
 }
 
 // flags[i] = node (cands[i]) stays active: size > thr
@@ -436,10 +540,12 @@ int gpu_build_trees(efg_index* ix) {
   int Lb = -1, level = 0;
   // threads per node: the dimensions rounded up to wavefronts, at most a workgroup
   const int tpn = int(std::min<size_t>(kBlock, (ix->d + 63) / 64 * 64));
+  const int small = tpn;  // nodes of at most this many ids are partitioned with their statistics
   const size_t lds_stats = size_t(kBlock / tpn) * 2 * ix->d * sizeof(float);
   if (lds_stats > 150 * 1024) return fail(EFG_ERR_ARG, "dimension too large for the split statistics");
-  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_split_stats), hipFuncAttributeMaxDynamicSharedMemorySize,
-                      int(lds_stats));
+  const bool off32 = size_t(n) * ix->D < (size_t(1) << 30);
+  hipFuncSetAttribute(reinterpret_cast<const void*>(off32 ? &k_split_stats<true> : &k_split_stats<false>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_stats));
   int32_t h_total = 0;
   while (na > 0) {
     if (bfs && !(na < 1100)) {  // switch to DFS: keep nodes larger than TNS
@@ -461,18 +567,19 @@ int gpu_build_trees(efg_index* ix) {
     {
       const int per = kBlock / tpn;
       unsigned g = unsigned(std::min<int>((na + per - 1) / per, 65535 * 4));
-      k_split_stats<<<g, kBlock, lds_stats, st>>>(ix->data.p, int(ix->d), int(ix->D), act.p, na, ix->range.p,
-                                                   tree_of.p, ix->leaf.p, n, p.seed, ix->node4.p, tpn);
+      (off32 ? k_split_stats<true> : k_split_stats<false>)<<<g, kBlock, lds_stats, st>>>(
+          ix->data.p, int(ix->d), int(ix->D), act.p, na, ix->range.p, tree_of.p, ix->leaf.p, n, p.seed, ix->node4.p,
+          ix->parent.p, tpn, small, base);
     }
-    // blocks
-    k_blocks_per_node<<<(na + 255) / 256, 256, 0, st>>>(act.p, na, ix->range.p, nb.p);
+    // blocks of the nodes larger than `small`
+    k_blocks_per_node<<<(na + 255) / 256, 256, 0, st>>>(act.p, na, ix->range.p, small, nb.p);
     if (int rc = device_exclusive_scan(nb.p, na, bstart.p, scal.p + 1, sums, st)) return rc;
     {
       unsigned g = unsigned(size_t(ncu) * 16);
       k_part_count<<<g, kBlock, 0, st>>>(ix->data.p, int(ix->D), act.p, na, bstart.p, scal.p + 1, ix->range.p,
                                          tree_of.p, ix->node4.p, ix->leaf.p, n, cls.p, bcnt.p);
-      k_part_node<<<(na + 255) / 256, 256, 0, st>>>(act.p, na, bstart.p, bcnt.p, ix->node4.p, ix->parent.p,
-                                                    ix->range.p, tree_of.p, base);
+      k_part_node<<<unsigned((size_t(na) * 64 + 255) / 256), 256, 0, st>>>(
+          act.p, na, bstart.p, bcnt.p, ix->node4.p, ix->parent.p, ix->range.p, tree_of.p, base, small);
       k_part_scatter<<<g, kBlock, 0, st>>>(act.p, na, bstart.p, scal.p + 1, ix->range.p, tree_of.p, bcnt.p, cls.p,
                                            ix->leaf.p, n, tmpbuf.p);
       k_part_copy<<<g, kBlock, 0, st>>>(act.p, na, bstart.p, scal.p + 1, ix->range.p, tree_of.p, tmpbuf.p, n,
