@@ -1071,84 +1071,102 @@ k_join_node_f32(const float* __restrict__ X, int D, uint32_t n, const uint32_t* 
   unsigned long long wbase = 0, wend = 0;  // staging run
   uint32_t* ids = ids_s[wv];
   float* rows = rows_s[wv];
-  for (uint32_t i = lo + (blockIdx.x >> 3) * kW + wv; i < hi; i += nsl) {
-    const uint32_t nd = order ? order[i] : i;
-    const uint64_t a0 = new_off[nd], b0 = old_off[nd];
-    const int na = int(new_off[nd + 1] - a0), m = na + int(old_off[nd + 1] - b0);
-    if (na == 0 || m > kNodeRows) continue;  // no pairs / the block tasks' nodes
-    const uint32_t id = lane < m ? (lane < na ? new_ids[a0 + lane] : old_ids[b0 + lane - na]) : kNoRow;
-    ids[lane] = id;
-    w_s[wv][lane] = id != kNoRow ? worst[id] : 0ull;
-    cnt_s[wv][lane] = 0;
-    wave_sync();
-    const int cells = na * m - na * (na + 1) / 2;
-    int pq[kNodePairs];  // P | Q << 8 of this lane's pairs
+  // the wavefront's nodes i0, i0 + nsl, ... are read 64 at a time, one per
+  // lane; the nodes taken here are then worked through in order
+  for (uint32_t i0 = lo + (blockIdx.x >> 3) * kW + wv; i0 < hi; i0 += 64 * nsl) {
+    const uint32_t ij = i0 + uint32_t(lane) * nsl;
+    uint32_t ndj = 0;
+    uint64_t a0j = 0, b0j = 0;
+    int naj = 0, mj = 0;
+    if (ij < hi) {
+      ndj = order ? order[ij] : ij;
+      a0j = new_off[ndj];
+      b0j = old_off[ndj];
+      naj = int(new_off[ndj + 1] - a0j);
+      mj = naj + int(old_off[ndj + 1] - b0j);
+    }
+    // no pairs / the block tasks' nodes
+    uint64_t todo = __ballot(ij < hi && naj > 0 && mj <= kNodeRows);
+    while (todo) {
+      const int jn = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const uint32_t nd = uint32_t(__shfl(int(ndj), jn));
+      const uint64_t a0 = uint64_t(__shfl((long long)a0j, jn)), b0 = uint64_t(__shfl((long long)b0j, jn));
+      const int na = __shfl(naj, jn), m = __shfl(mj, jn);
+      const uint32_t id = lane < m ? (lane < na ? new_ids[a0 + lane] : old_ids[b0 + lane - na]) : kNoRow;
+      ids[lane] = id;
+      w_s[wv][lane] = id != kNoRow ? worst[id] : 0ull;
+      cnt_s[wv][lane] = 0;
+      wave_sync();
+      const int cells = na * m - na * (na + 1) / 2;
+      int pq[kNodePairs];  // P | Q << 8 of this lane's pairs
 #pragma unroll
-    for (int j = 0; j < kNodePairs; ++j) {
-      int P = 0, Q = 0;
-      if (lane + 64 * j < cells) join_cell(lane + 64 * j, true, m, m, P, Q);
-      pq[j] = P | Q << 8;
-    }
-    float dist[kNodePairs];
-    node_pairs4(X, D, ids, m, cells, 0, rows, pq, dist);
-    if (cells > 256) node_pairs4(X, D, ids, m, cells, 256, rows, pq + 4, dist + 4);
-    uint32_t li[kNodePairs];  // first endpoint's index | second's << 16; ~0u: no pair
+      for (int j = 0; j < kNodePairs; ++j) {
+        int P = 0, Q = 0;
+        if (lane + 64 * j < cells) join_cell(lane + 64 * j, true, m, m, P, Q);
+        pq[j] = P | Q << 8;
+      }
+      float dist[kNodePairs];
+      node_pairs4(X, D, ids, m, cells, 0, rows, pq, dist);
+      if (cells > 256) node_pairs4(X, D, ids, m, cells, 256, rows, pq + 4, dist + 4);
+      uint32_t li[kNodePairs];  // first endpoint's index | second's << 16; ~0u: no pair
 #pragma unroll
-    for (int j = 0; j < kNodePairs; ++j) {
-      li[j] = ~0u;
-      int sa, sb;
-      uint32_t x, y;
-      if (lane + 64 * j < cells && node_ends(pq[j] & 0xff, pq[j] >> 8, na, m, ids, sa, sb, x, y)) {
-        ++pairs_local;
-        uint32_t la = 0xffffu, lb = 0xffffu;
-        if (make_key(dist[j], y) < w_s[wv][sa]) la = atomicAdd(&cnt_s[wv][sa], 1u);
-        if (make_key(dist[j], x) < w_s[wv][sb]) lb = atomicAdd(&cnt_s[wv][sb], 1u);
-        li[j] = la | lb << 16;
+      for (int j = 0; j < kNodePairs; ++j) {
+        li[j] = ~0u;
+        int sa, sb;
+        uint32_t x, y;
+        if (lane + 64 * j < cells && node_ends(pq[j] & 0xff, pq[j] >> 8, na, m, ids, sa, sb, x, y)) {
+          ++pairs_local;
+          uint32_t la = 0xffffu, lb = 0xffffu;
+          if (make_key(dist[j], y) < w_s[wv][sa]) la = atomicAdd(&cnt_s[wv][sa], 1u);
+          if (make_key(dist[j], x) < w_s[wv][sb]) lb = atomicAdd(&cnt_s[wv][sb], 1u);
+          li[j] = la | lb << 16;
+        }
       }
-    }
-    wave_sync();
-    // totals per target (result not waited for), offsets within the node
-    const uint32_t cnt = cnt_s[wv][lane];
-    if (cnt) atomicAdd(&tcount[id], cnt);
-    uint32_t x = cnt;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = uint32_t(__shfl_up(int(x), o));
-      if (lane >= o) x += y;
-    }
-    off_s[wv][lane] = x - cnt;
-    const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
-    // this node's slots: the rest of the run, then a new run if needed
-    const unsigned long long rem = wend - wbase;
-    unsigned long long nr = 0;
-    if (tot > rem) {
-      if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
-      nr = (unsigned long long)__shfl((long long)nr, 0);
-    }
-    const unsigned long long r0 = wbase;
-    if (tot > rem) {
-      wbase = nr + (tot - rem);
-      wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
-    } else {
-      wbase += tot;
-    }
-    wave_sync();
+      wave_sync();
+      // totals per target (result not waited for), offsets within the node
+      const uint32_t cnt = cnt_s[wv][lane];
+      if (cnt) atomicAdd(&tcount[id], cnt);
+      uint32_t x = cnt;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = uint32_t(__shfl_up(int(x), o));
+        if (lane >= o) x += y;
+      }
+      off_s[wv][lane] = x - cnt;
+      const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
+      // this node's slots: the rest of the run, then a new run if needed
+      const unsigned long long rem = wend - wbase;
+      unsigned long long nr = 0;
+      if (tot > rem) {
+        if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
+        nr = (unsigned long long)__shfl((long long)nr, 0);
+      }
+      const unsigned long long r0 = wbase;
+      if (tot > rem) {
+        wbase = nr + (tot - rem);
+        wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
+      } else {
+        wbase += tot;
+      }
+      wave_sync();
 #pragma unroll
-    for (int j = 0; j < kNodePairs; ++j) {
-      if (li[j] == ~0u) continue;
-      int sa, sb;
-      uint32_t xa, xb;
-      node_ends(pq[j] & 0xff, pq[j] >> 8, na, m, ids, sa, sb, xa, xb);
-      const uint32_t la = li[j] & 0xffffu, lb = li[j] >> 16;
-      if (la != 0xffffu) {
-        const unsigned long long q = off_s[wv][sa] + la, at = q < rem ? r0 + q : nr + (q - rem);
-        if (at < cap) stage[at] = Stage{xa, nd | kFirst, make_key(dist[j], xb)};
+      for (int j = 0; j < kNodePairs; ++j) {
+        if (li[j] == ~0u) continue;
+        int sa, sb;
+        uint32_t xa, xb;
+        node_ends(pq[j] & 0xff, pq[j] >> 8, na, m, ids, sa, sb, xa, xb);
+        const uint32_t la = li[j] & 0xffffu, lb = li[j] >> 16;
+        if (la != 0xffffu) {
+          const unsigned long long q = off_s[wv][sa] + la, at = q < rem ? r0 + q : nr + (q - rem);
+          if (at < cap) stage[at] = Stage{xa, nd | kFirst, make_key(dist[j], xb)};
+        }
+        if (lb != 0xffffu) {
+          const unsigned long long q = off_s[wv][sb] + lb, at = q < rem ? r0 + q : nr + (q - rem);
+          if (at < cap) stage[at] = Stage{xb, nd, make_key(dist[j], xa)};
+        }
       }
-      if (lb != 0xffffu) {
-        const unsigned long long q = off_s[wv][sb] + lb, at = q < rem ? r0 + q : nr + (q - rem);
-        if (at < cap) stage[at] = Stage{xb, nd, make_key(dist[j], xa)};
-      }
+      wave_sync();
     }
-    wave_sync();
   }
   for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
   for (int o = 32; o > 0; o >>= 1) pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
@@ -1547,11 +1565,14 @@ __device__ __forceinline__ void mfma_split3(const uint16_t* __restrict__ hi, con
 // Whole-node join on MFMA (precision form) for the nodes with at most 64
 // rows: one wavefront per node, each row read once (the block tasks read a
 // node's rows once per task), the node's pairs from the blocks (0,0), and
-// for more than 32 rows (0,1) and (1,1).  Nodes are taken in the order of
-// tree 0's leaf list (order, null: node order), one contiguous range per
-// XCD.  Candidates are counted, offset and staged as in k_join_mfma.
-template <bool kF16, bool kLo>
-__global__ void __launch_bounds__(kBlock) k_join_node_mfma(const uint16_t* __restrict__ xhi,
+// for more than 32 rows (0,1) and (1,1).  kNT = 1 takes the nodes of at most
+// 32 rows, kNT = 3 those of 33 to 64 (two launches: the one-block form needs
+// a third of the registers, so more wavefronts stay resident).  Nodes are
+// taken in the order of tree 0's leaf list (order, null: node order), one
+// contiguous range per XCD.  Candidates are counted, offset and staged as in
+// k_join_mfma.
+template <bool kF16, bool kLo, int kNT>
+__global__ void __launch_bounds__(kBlock, kNT == 1 ? 4 : 3) k_join_node_mfma(const uint16_t* __restrict__ xhi,
                                                            const uint16_t* __restrict__ xlo,
                                                            const float* __restrict__ xn, int Dk, uint32_t n,
                                                            const uint32_t* __restrict__ order,
@@ -1576,100 +1597,121 @@ __global__ void __launch_bounds__(kBlock) k_join_node_mfma(const uint16_t* __res
   unsigned long long pairs_local = 0;
   unsigned long long wbase = 0, wend = 0;  // staging run
   uint32_t* ids = ids_s[wv];
-  for (uint32_t i = lo + (blockIdx.x >> 3) * kW + wv; i < hi; i += nsl) {
-    const uint32_t nd = order ? order[i] : i;
-    const uint64_t a0 = new_off[nd], b0 = old_off[nd];
-    const int na = int(new_off[nd + 1] - a0), m = na + int(old_off[nd + 1] - b0);
-    if (na == 0 || m > 64) continue;  // no pairs / the block tasks' nodes
-    const uint32_t id = lane < m ? (lane < na ? new_ids[a0 + lane] : old_ids[b0 + lane - na]) : kNoRow;
-    ids[lane] = id;
-    w_s[wv][lane] = id != kNoRow ? worst[id] : 0ull;
-    cnt_s[wv][lane] = 0;
-    const uint32_t ia = uint32_t(__shfl(int(id), r32)), ib = uint32_t(__shfl(int(id), 32 + r32));
-    const uint32_t ra = ia != kNoRow ? ia : 0u, rb = ib != kNoRow ? ib : 0u;
-    const int nt = m > 32 ? 3 : 1;  // blocks (0,0) [, (0,1), (1,1)]
-    f32x16_t t[3];
-    if (nt == 3) {
-      mfma_split3<kF16, kLo>(xhi, xlo, ra, rb, Dk, t[0], t[1], t[2]);
-    } else {
-      t[0] = mfma_split<kF16, true, kLo>(xhi, xlo, ra, ra, Dk);
-      t[1] = f32x16_t{};
-      t[2] = f32x16_t{};
+  // the wavefront's nodes i0, i0 + nsl, ... are read 64 at a time, one per
+  // lane; the nodes this launch takes are then worked through in order
+  for (uint32_t i0 = lo + (blockIdx.x >> 3) * kW + wv; i0 < hi; i0 += 64 * nsl) {
+    const uint32_t ij = i0 + uint32_t(lane) * nsl;
+    uint32_t ndj = 0;
+    uint64_t a0j = 0, b0j = 0;
+    int naj = 0, mj = 0;
+    if (ij < hi) {
+      ndj = order ? order[ij] : ij;
+      a0j = new_off[ndj];
+      b0j = old_off[ndj];
+      naj = int(new_off[ndj + 1] - a0j);
+      mj = naj + int(old_off[ndj + 1] - b0j);
     }
-    const float nra = xn[ra], nrb = xn[rb];
-    wave_sync();
-    float dist[3][16];
-    uint32_t li[3][16];  // first endpoint's index | second's << 16; ~0u: no pair
+    // no pairs / the block tasks' nodes / the other launch's nodes
+    uint64_t todo = __ballot(ij < hi && naj > 0 && mj <= 64 && (mj > 32) == (kNT == 3));
+    while (todo) {
+      const int jn = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const uint32_t nd = uint32_t(__shfl(int(ndj), jn));
+      const uint64_t a0 = uint64_t(__shfl((long long)a0j, jn)), b0 = uint64_t(__shfl((long long)b0j, jn));
+      const int na = __shfl(naj, jn), m = __shfl(mj, jn);
+      const uint32_t id = lane < m ? (lane < na ? new_ids[a0 + lane] : old_ids[b0 + lane - na]) : kNoRow;
+      ids[lane] = id;
+      w_s[wv][lane] = id != kNoRow ? worst[id] : 0ull;
+      cnt_s[wv][lane] = 0;
+      const uint32_t ia = uint32_t(__shfl(int(id), r32)), ib = uint32_t(__shfl(int(id), 32 + r32));
+      const uint32_t ra = ia != kNoRow ? ia : 0u, rb = ib != kNoRow ? ib : 0u;
+      f32x16_t t[kNT];  // blocks (0,0) [, (0,1), (1,1)]
+      if constexpr (kNT == 3) mfma_split3<kF16, kLo>(xhi, xlo, ra, rb, Dk, t[0], t[1], t[2]);
+      else t[0] = mfma_split<kF16, true, kLo>(xhi, xlo, ra, ra, Dk);
+      const float nra = xn[ra], nrb = xn[rb];
+      wave_sync();
+      // per pair: slot of the first endpoint's candidate | second's << 8 (0xff:
+      // none; a row meets at most 63 others), two pairs per register; the
+      // distances are formed again from the accumulators when staging
+      uint32_t li[kNT][8];
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      const int pb = b == 2 ? 32 : 0, qb = b == 0 ? 0 : 32;  // position offsets of the A and B rows
-      const float nq = b == 0 ? nra : nrb;
+      for (int b = 0; b < kNT; ++b) {
+        const int pb = b == 2 ? 32 : 0, qb = b == 0 ? 0 : 32;  // position offsets of the A and B rows
+        const float nq = b == 0 ? nra : nrb;
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int p = 8 * (e / 4) + 4 * (lane >> 5) + (e & 3);
-        const float np = __shfl(b == 2 ? nrb : nra, p);
-        const float dd = __builtin_fmaf(-2.f, t[b][e], np + nq);
-        dist[b][e] = dd > 0.f ? dd : 0.f;
-        li[b][e] = ~0u;
-        int sa, sb;
-        uint32_t x, y;
-        if (b < nt && node_ends(pb + p, qb + r32, na, m, ids, sa, sb, x, y)) {
-          ++pairs_local;
-          uint32_t la = 0xffffu, lb = 0xffffu;
-          if (make_key(dist[b][e], y) < w_s[wv][sa]) la = atomicAdd(&cnt_s[wv][sa], 1u);
-          if (make_key(dist[b][e], x) < w_s[wv][sb]) lb = atomicAdd(&cnt_s[wv][sb], 1u);
-          li[b][e] = la | lb << 16;
+        for (int e = 0; e < 16; ++e) {
+          const int p = 8 * (e / 4) + 4 * (lane >> 5) + (e & 3);
+          const float np = __shfl(b == 2 ? nrb : nra, p);
+          const float dd = __builtin_fmaf(-2.f, t[b][e], np + nq);
+          const float dist = dd > 0.f ? dd : 0.f;
+          uint32_t v = 0xffffu;
+          int sa, sb;
+          uint32_t x, y;
+          if (node_ends(pb + p, qb + r32, na, m, ids, sa, sb, x, y)) {
+            ++pairs_local;
+            uint32_t la = 0xffu, lb = 0xffu;
+            if (make_key(dist, y) < w_s[wv][sa]) la = atomicAdd(&cnt_s[wv][sa], 1u);
+            if (make_key(dist, x) < w_s[wv][sb]) lb = atomicAdd(&cnt_s[wv][sb], 1u);
+            v = la | lb << 8;
+          }
+          if (e & 1) li[b][e >> 1] |= v << 16;
+          else li[b][e >> 1] = v;
         }
       }
-    }
-    wave_sync();
-    // totals per target (result not waited for), offsets within the node
-    const uint32_t cnt = cnt_s[wv][lane];
-    if (cnt) atomicAdd(&tcount[id], cnt);
-    uint32_t x = cnt;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = uint32_t(__shfl_up(int(x), o));
-      if (lane >= o) x += y;
-    }
-    off_s[wv][lane] = x - cnt;
-    const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
-    // this node's slots: the rest of the run, then a new run if needed
-    const unsigned long long rem = wend - wbase;
-    unsigned long long nr = 0;
-    if (tot > rem) {
-      if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
-      nr = (unsigned long long)__shfl((long long)nr, 0);
-    }
-    const unsigned long long r0 = wbase;
-    if (tot > rem) {
-      wbase = nr + (tot - rem);
-      wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
-    } else {
-      wbase += tot;
-    }
-    wave_sync();
+      wave_sync();
+      // totals per target (result not waited for), offsets within the node
+      const uint32_t cnt = cnt_s[wv][lane];
+      if (cnt) atomicAdd(&tcount[id], cnt);
+      uint32_t x = cnt;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = uint32_t(__shfl_up(int(x), o));
+        if (lane >= o) x += y;
+      }
+      off_s[wv][lane] = x - cnt;
+      const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
+      // this node's slots: the rest of the run, then a new run if needed
+      const unsigned long long rem = wend - wbase;
+      unsigned long long nr = 0;
+      if (tot > rem) {
+        if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
+        nr = (unsigned long long)__shfl((long long)nr, 0);
+      }
+      const unsigned long long r0 = wbase;
+      if (tot > rem) {
+        wbase = nr + (tot - rem);
+        wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
+      } else {
+        wbase += tot;
+      }
+      wave_sync();
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      const int pb = b == 2 ? 32 : 0, qb = b == 0 ? 0 : 32;
+      for (int b = 0; b < kNT; ++b) {
+        const int pb = b == 2 ? 32 : 0, qb = b == 0 ? 0 : 32;
+        const float nq = b == 0 ? nra : nrb;
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        if (li[b][e] == ~0u) continue;
-        const int p = 8 * (e / 4) + 4 * (lane >> 5) + (e & 3);
-        int sa, sb;
-        uint32_t xa, xb;
-        node_ends(pb + p, qb + r32, na, m, ids, sa, sb, xa, xb);
-        const uint32_t la = li[b][e] & 0xffffu, lb = li[b][e] >> 16;
-        if (la != 0xffffu) {
-          const unsigned long long j = off_s[wv][sa] + la, at = j < rem ? r0 + j : nr + (j - rem);
-          if (at < cap) stage[at] = Stage{xa, nd | kFirst, make_key(dist[b][e], xb)};
-        }
-        if (lb != 0xffffu) {
-          const unsigned long long j = off_s[wv][sb] + lb, at = j < rem ? r0 + j : nr + (j - rem);
-          if (at < cap) stage[at] = Stage{xb, nd, make_key(dist[b][e], xa)};
+        for (int e = 0; e < 16; ++e) {
+          const int p = 8 * (e / 4) + 4 * (lane >> 5) + (e & 3);
+          const float np = __shfl(b == 2 ? nrb : nra, p);
+          const uint32_t v = (li[b][e >> 1] >> (16 * (e & 1))) & 0xffffu;
+          if (v == 0xffffu) continue;
+          const float dd = __builtin_fmaf(-2.f, t[b][e], np + nq);
+          const float dist = dd > 0.f ? dd : 0.f;
+          int sa, sb;
+          uint32_t xa, xb;
+          node_ends(pb + p, qb + r32, na, m, ids, sa, sb, xa, xb);
+          const uint32_t la = v & 0xffu, lb = v >> 8;
+          if (la != 0xffu) {
+            const unsigned long long j = off_s[wv][sa] + la, at = j < rem ? r0 + j : nr + (j - rem);
+            if (at < cap) stage[at] = Stage{xa, nd | kFirst, make_key(dist, xb)};
+          }
+          if (lb != 0xffu) {
+            const unsigned long long j = off_s[wv][sb] + lb, at = j < rem ? r0 + j : nr + (j - rem);
+            if (at < cap) stage[at] = Stage{xb, nd, make_key(dist, xa)};
+          }
         }
       }
+      wave_sync();
     }
-    wave_sync();
   }
   for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
   for (int o = 32; o > 0; o >>= 1) pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
@@ -1847,7 +1889,8 @@ __global__ void __launch_bounds__(kBlock) k_merge_wave(uint32_t n, int L, int CK
         }
         wave_sync();
       }
-      // drop duplicates (the first of equal keys stays; of equal ids, the pool entry or else the first), compact to the front
+      // drop duplicates (the first of equal keys stays; of equal ids, the pool
+      // entry or else the first), compact to the front
       int kept = 0;
       for (int b0 = 0; b0 < m && kept < L; b0 += 64) {
         const int i = b0 + lane;
@@ -2555,10 +2598,10 @@ int gpu_refine_iter(efg_index* ix) {
   const size_t node_grid = size_t(ncu) * 8;  // whole-node FP32 kernel: workgroups of four wavefronts
   // staging: the kept candidates (two per pair at most) plus one partly used
   // run per workgroup (per wavefront on the MFMA path, ncu * 8 workgroups of
-  // four, two kernels).  It is sized from the last iteration's kept fraction
+  // four, three kernels).  It is sized from the last iteration's kept fraction
   // with a margin; when the candidates do not fit, the join kernels (which do
   // not change the pools) run again with room for every reservation counted.
-  const size_t slack = ix->b_precision > 0 ? 2 * size_t(ncu) * 8 * (kBlock / 64) * kStageRun + 1024
+  const size_t slack = ix->b_precision > 0 ? 3 * size_t(ncu) * 8 * (kBlock / 64) * kStageRun + 1024
                                            : (join_grid + node_grid * (kBlock / 64)) * kStageRun + 1024;
   const size_t full = size_t(2 * bound) + slack;
   size_t cap = std::min(full, size_t(double(2 * bound) * std::min(1.0, ix->b_stage_margin * bs.keep_frac)) + slack);
@@ -2585,8 +2628,12 @@ int gpu_refine_iter(efg_index* ix) {
       const unsigned g16 = unsigned(size_t(ncu) * 8);
       const bool f16 = ix->b_precision == 2;
       const bool lo = ix->split_lo_used;
-      (f16 ? (lo ? k_join_node_mfma<true, true> : k_join_node_mfma<true, false>)
-           : (lo ? k_join_node_mfma<false, true> : k_join_node_mfma<false, false>))<<<g16, kBlock, 0, st>>>(
+      (f16 ? (lo ? k_join_node_mfma<true, true, 1> : k_join_node_mfma<true, false, 1>)
+           : (lo ? k_join_node_mfma<false, true, 1> : k_join_node_mfma<false, false, 1>))<<<g16, kBlock, 0, st>>>(
+          ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, n, order, bs.new_off.p, bs.new_ids.p,
+          bs.old_off.p, bs.old_ids.p, worst.p, stage, (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
+      (f16 ? (lo ? k_join_node_mfma<true, true, 3> : k_join_node_mfma<true, false, 3>)
+           : (lo ? k_join_node_mfma<false, true, 3> : k_join_node_mfma<false, false, 3>))<<<g16, kBlock, 0, st>>>(
           ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, n, order, bs.new_off.p, bs.new_ids.p,
           bs.old_off.p, bs.old_ids.p, worst.p, stage, (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
       if (ntask)
