@@ -183,6 +183,7 @@ struct efg_index {
   efg::DevBuf<float> split_norm;             // FP32 squared norms of the rows
   int split_Dk = 0, split_kind = 0;
   bool split_lo_used = true;                 // some low part is non-zero (else the tiles skip them)
+  bool split_exact = false;                  // every tile distance is exact (integer rows, small norms)
   // trees
   int T = 0;
   size_t M = 0;

@@ -1222,38 +1222,50 @@ __global__ void k_join_scatter(const Stage* __restrict__ stage, unsigned long lo
 // ---------------------------------------------------------------------------
 
 // Split rows: hi[r][e], lo[r][e] (Dk a multiple of 64, zero padded) and the
-// FP32 squared norm of row r; one wavefront per row.
+// FP32 squared norm of row r; one wavefront per row at a time.  stat[0]: some low part
+// is non-zero; stat[1]: some value is not an integer; stat[2]: the largest
+// squared norm (float bits, non-negative, so they order as integers).
 template <bool kF16>
 __global__ void k_pack_split(const float* __restrict__ X, size_t n, int d, int D, int Dk, uint16_t* __restrict__ hi,
-                             uint16_t* __restrict__ lo, float* __restrict__ norm, int* __restrict__ lo_any) {
-  const size_t r = size_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+                             uint16_t* __restrict__ lo, float* __restrict__ norm, int* __restrict__ stat) {
   const int lane = threadIdx.x & 63;
-  if (r >= n) return;
-  float acc = 0.f;
-  bool nz = false;
-  for (int e = lane; e < Dk; e += 64) {
-    const float v = e < d ? X[r * D + e] : 0.f;
-    uint16_t h, l;
-    if (kF16) {
-      const _Float16 x = _Float16(v);
-      const _Float16 y = _Float16(v - float(x));
-      __builtin_memcpy(&h, &x, 2);
-      __builtin_memcpy(&l, &y, 2);
-    } else {
-      const uint32_t u = __float_as_uint(v);
-      h = uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
-      const float rest = v - __uint_as_float(uint32_t(h) << 16);
-      const uint32_t w = __float_as_uint(rest);
-      l = uint16_t((w + 0x7fffu + ((w >> 16) & 1u)) >> 16);
+  const size_t nw = size_t(gridDim.x) * (blockDim.x / 64);
+  bool nz = false, frac = false;
+  int nmax = 0;
+  for (size_t r = size_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6); r < n; r += nw) {
+    float acc = 0.f;
+    for (int e = lane; e < Dk; e += 64) {
+      const float v = e < d ? X[r * D + e] : 0.f;
+      frac = frac || __builtin_truncf(v) != v;
+      uint16_t h, l;
+      if (kF16) {
+        const _Float16 x = _Float16(v);
+        const _Float16 y = _Float16(v - float(x));
+        __builtin_memcpy(&h, &x, 2);
+        __builtin_memcpy(&l, &y, 2);
+      } else {
+        const uint32_t u = __float_as_uint(v);
+        h = uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+        const float rest = v - __uint_as_float(uint32_t(h) << 16);
+        const uint32_t w = __float_as_uint(rest);
+        l = uint16_t((w + 0x7fffu + ((w >> 16) & 1u)) >> 16);
+      }
+      hi[r * Dk + e] = h;
+      lo[r * Dk + e] = l;
+      nz = nz || (l & 0x7fffu) != 0;  // a low part of +-0 adds nothing
+      acc = __builtin_fmaf(v, v, acc);
     }
-    hi[r * Dk + e] = h;
-    lo[r * Dk + e] = l;
-    nz = nz || (l & 0x7fffu) != 0;  // a low part of +-0 adds nothing
-    acc = __builtin_fmaf(v, v, acc);
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) norm[r] = acc;
+    nmax = max(nmax, __float_as_int(acc));
   }
-  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-  if (lane == 0) norm[r] = acc;
-  if (__ballot(nz) && lane == 0) atomicOr(lo_any, 1);
+  // one update per wavefront of each statistic
+  const bool anz = __ballot(nz) != 0, afrac = __ballot(frac) != 0;
+  if (lane == 0) {
+    if (anz) atomicOr(&stat[0], 1);
+    if (afrac) atomicOr(&stat[1], 1);
+    atomicMax(&stat[2], nmax);
+  }
 }
 
 template <bool kF16>
@@ -2379,22 +2391,31 @@ int ensure_split(efg_index* ix, hipStream_t st) {
   if (!ix->split_hi.alloc(ix->n * size_t(Dk)) || !ix->split_lo.alloc(ix->n * size_t(Dk)) ||
       !ix->split_norm.alloc(ix->n))
     return fail(EFG_ERR_MEMORY, "out of device memory (split rows)");
-  const unsigned grid = unsigned((ix->n + 3) / 4);
-  DevBuf<int> lo_any;
-  if (!lo_any.alloc(1)) return fail(EFG_ERR_MEMORY, "out of device memory (split rows)");
-  hipMemsetAsync(lo_any.p, 0, sizeof(int), st);
+  int dev = 0, ncu = 256;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const unsigned grid = unsigned(std::min<size_t>((ix->n + 3) / 4, size_t(ncu) * 16));  // a wavefront per row
+  DevBuf<int> stat;
+  if (!stat.alloc(3)) return fail(EFG_ERR_MEMORY, "out of device memory (split rows)");
+  hipMemsetAsync(stat.p, 0, 3 * sizeof(int), st);
   if (ix->b_precision == 2)
     k_pack_split<true><<<grid, 256, 0, st>>>(ix->data.p, ix->n, int(ix->d), int(ix->D), Dk, ix->split_hi.p,
-                                             ix->split_lo.p, ix->split_norm.p, lo_any.p);
+                                             ix->split_lo.p, ix->split_norm.p, stat.p);
   else
     k_pack_split<false><<<grid, 256, 0, st>>>(ix->data.p, ix->n, int(ix->d), int(ix->D), Dk, ix->split_hi.p,
-                                              ix->split_lo.p, ix->split_norm.p, lo_any.p);
+                                              ix->split_lo.p, ix->split_norm.p, stat.p);
+  int h[3] = {1, 1, 0x7f800000};
+  hipMemcpyAsync(h, stat.p, sizeof h, hipMemcpyDeviceToHost, st);
+  if (int rc = check_hip(hipStreamSynchronize(st), "split rows")) return rc;
   // rows exact in the 16-bit type leave every low part zero: the tiles then
   // neither read nor multiply them
-  int any = 1;
-  hipMemcpyAsync(&any, lo_any.p, sizeof(int), hipMemcpyDeviceToHost, st);
-  if (int rc = check_hip(hipStreamSynchronize(st), "split rows")) return rc;
-  ix->split_lo_used = any != 0;
+  ix->split_lo_used = h[0] != 0;
+  // integer rows exact in the 16-bit type with squared norms below 2^22: every
+  // partial sum of a dot product is an integer below 2^22 (Cauchy-Schwarz),
+  // every distance an integer below (|x| + |y|)^2 < 2^24, so the tiles'
+  // |x|^2 + |y|^2 - 2 x.y and the FP32 difference form (partial sums at most
+  // the distance) are both exact: equal keys, no re-rank and no id dedup
+  ix->split_exact = h[0] == 0 && h[1] == 0 && __builtin_bit_cast(float, h[2]) < 4194304.f;
   ix->split_Dk = Dk;
   ix->split_kind = ix->b_precision;
   return check_hip(hipGetLastError(), "split rows");
@@ -2517,7 +2538,7 @@ int gpu_merge_init(efg_index* ix) {
     }
   }
   // precision form: the merged sets get their FP32 distances back
-  if (mfma_leaf) k_rerank_sets<<<grid_for(n), 256, 0, st>>>(ix->data.p, D, n, S, hkey.p, hsize.p);
+  if (mfma_leaf && !ix->split_exact) k_rerank_sets<<<grid_for(n), 256, 0, st>>>(ix->data.p, D, n, S, hkey.p, hsize.p);
   // init
   bs.L = L;
   if (!bs.pool.alloc(size_t(n) * L) || !bs.flag.alloc(size_t(n) * L) || !bs.fill.alloc(n) || !bs.range.alloc(n) ||
@@ -2680,7 +2701,7 @@ int gpu_refine_iter(efg_index* ix) {
   hipMemsetAsync(bs.found.p, 0, n, st);
   {
     // one wavefront per target; the LDS window holds at least 2L entries
-    const bool byid = ix->b_precision > 0;
+    const bool byid = ix->b_precision > 0 && !ix->split_exact;  // approximate keys
     const unsigned grid = unsigned(std::min<size_t>((n + 3) / 4, size_t(ncu) * 8));
     if (L <= 512)
       (L <= 64 ? (byid ? k_merge_reg<64, true> : k_merge_reg<64, false>)
@@ -2783,7 +2804,7 @@ int gpu_finalize_graph(efg_index* ix) {
   const uint32_t n = uint32_t(ix->n);
   const int K = ix->bp.K;
   if (!ix->goff.alloc(n + 1) || !ix->gids.alloc(size_t(n) * K)) return fail(EFG_ERR_MEMORY, "out of device memory (graph)");
-  if (ix->b_precision > 0) {  // precision form: FP32 re-rank of the first 2K entries of every pool
+  if (ix->b_precision > 0 && !ix->split_exact) {  // precision form: FP32 re-rank of the first 2K entries of every pool
     const int R = std::min(bs.L, 2 * K);
     const size_t m = size_t(n) * R * 2;
     k_rerank_pool<<<unsigned((m + 255) / 256), 256, 0, ix->stream>>>(ix->data.p, int(ix->D), n, bs.L, R, bs.pool.p,
