@@ -2372,11 +2372,6 @@ __global__ void k_graph_out(uint32_t n, int L, int K, const uint64_t* __restrict
   }
 }
 
-__global__ void k_u64_from_i32(const int32_t* __restrict__ in, uint64_t* __restrict__ out, int m) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i <= m) out[i] = uint64_t(uint32_t(in[i]));
-}
-
 double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
