@@ -1730,58 +1730,68 @@ __global__ void __launch_bounds__(kBlock, kNT == 1 ? 4 : 3) k_join_node_mfma(con
   if (lane == 0 && pairs_local) atomicAdd(npairs, pairs_local);
 }
 
-// FP32 re-rank of the merged S-sets: each point's keys get the direct FP32
-// distance (reference order) and are sorted again.
-__global__ void k_rerank_sets(const float* __restrict__ X, int D, uint32_t n, int S, uint64_t* __restrict__ hkey,
-                              const uint32_t* __restrict__ hsize) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  uint64_t* h = hkey + size_t(p) * S;
-  const int sz = int(hsize[p]);
-  for (int j = 0; j < sz; ++j) {
-    const uint32_t id = key_id(h[j]);
-    const uint64_t k = make_key(l2_thread(X + size_t(p) * D, X + size_t(id) * D, D), id);
-    int i = j;
-    while (i > 0 && h[i - 1] > k) {
-      h[i] = h[i - 1];
-      --i;
-    }
-    h[i] = k;
-  }
-}
-
-// FP32 re-rank of the final pools: the first min(fill, 2K) entries of every
-// pool get their direct FP32 distances (a lane pair per entry, the
-// reference's order); k_sort_pools then orders that prefix (keys with their
-// flags).
+// FP32 re-rank of the final pools (the first min(fill, 2K) entries) and of
+// the merged S-sets (all of them): each entry gets its direct FP32 distance
+// (a lane pair per entry, the reference's order); k_sort_pools then orders
+// that prefix (keys, with their flags for the pools).  Points are taken in
+// `order` (tree 0's leaf list, or null: point order), so points re-ranked
+// together are close and share most of their neighbours' rows in the caches.
 __global__ void k_rerank_pool(const float* __restrict__ X, int D, uint32_t n, int L, int R,
-                              uint64_t* __restrict__ pool, const uint32_t* __restrict__ fill) {
+                              const uint32_t* __restrict__ order, uint64_t* __restrict__ pool,
+                              const uint32_t* __restrict__ fill) {
   const size_t g = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 1;
   const int h = threadIdx.x & 1;
-  const uint32_t p = uint32_t(g / R), j = uint32_t(g % R);
+  const uint32_t i = uint32_t(g / R), j = uint32_t(g % R);
+  const uint32_t p = order && i < n ? order[i] : i;
   const bool in = g < size_t(n) * R && j < fill[p < n ? p : 0];
   const uint32_t id = in ? key_id(pool[size_t(p) * L + j]) : 0u;
   const float dd = l2_pair_lanes(X + size_t(in ? p : 0) * D, X + size_t(id) * D, D, h);
   if (in && h) pool[size_t(p) * L + j] = make_key(dd, id);
 }
-__global__ void k_sort_pools(uint32_t n, int L, int R, uint64_t* __restrict__ pool, uint8_t* __restrict__ flag,
-                             const uint32_t* __restrict__ fill) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  uint64_t* k = pool + size_t(p) * L;
-  uint8_t* f = flag + size_t(p) * L;
-  const int m = min(int(fill[p]), R);
-  for (int j = 1; j < m; ++j) {  // nearly sorted already: insertion sort
-    const uint64_t v = k[j];
-    const uint8_t fv = f[j];
-    int i = j;
-    while (i > 0 && k[i - 1] > v) {
-      k[i] = k[i - 1];
-      f[i] = f[i - 1];
-      --i;
+// One wavefront per point: the prefix, padded to a power of two with the
+// largest key, is sorted in LDS by a bitonic network (flags move with their
+// keys).  R is at most 1024.
+template <bool kFlags>
+__global__ void __launch_bounds__(kBlock) k_sort_pools(uint32_t n, int L, int R, uint64_t* __restrict__ pool,
+                                                       uint8_t* __restrict__ flag, const uint32_t* __restrict__ fill) {
+  __shared__ uint64_t ks[kBlock / 64][1024];
+  __shared__ uint8_t fs[kBlock / 64][kFlags ? 1024 : 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (uint32_t p = blockIdx.x * (kBlock / 64) + w; p < n; p += gridDim.x * (kBlock / 64)) {
+    uint64_t* k = pool + size_t(p) * L;
+    const int m = min(int(fill[p]), R);
+    if (m < 2) continue;
+    const int p2 = next_pow2(m);
+    for (int j = lane; j < p2; j += 64) {
+      ks[w][j] = j < m ? k[j] : kKeyMax;
+      if (kFlags) fs[w][j] = j < m ? flag[size_t(p) * L + j] : 0;
     }
-    k[i] = v;
-    f[i] = fv;
+    wave_sync();
+    for (int kk = 2; kk <= p2; kk <<= 1) {
+      for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+        for (int t = lane; t < p2; t += 64) {
+          const int u = t ^ jj;
+          if (u > t) {
+            const uint64_t a = ks[w][t], b = ks[w][u];
+            if ((a > b) == ((t & kk) == 0)) {
+              ks[w][t] = b;
+              ks[w][u] = a;
+              if (kFlags) {
+                const uint8_t fa = fs[w][t];
+                fs[w][t] = fs[w][u];
+                fs[w][u] = fa;
+              }
+            }
+          }
+        }
+        wave_sync();
+      }
+    }
+    for (int j = lane; j < m; j += 64) {
+      k[j] = ks[w][j];
+      if (kFlags) flag[size_t(p) * L + j] = fs[w][j];
+    }
+    wave_sync();
   }
 }
 
@@ -2533,7 +2543,13 @@ int gpu_merge_init(efg_index* ix) {
     }
   }
   // precision form: the merged sets get their FP32 distances back
-  if (mfma_leaf && !ix->split_exact) k_rerank_sets<<<grid_for(n), 256, 0, st>>>(ix->data.p, D, n, S, hkey.p, hsize.p);
+  if (mfma_leaf && !ix->split_exact) {
+    const size_t m = size_t(n) * S * 2;
+    k_rerank_pool<<<unsigned((m + 255) / 256), 256, 0, st>>>(ix->data.p, D, n, S, S,
+                                                             ix->leaf0_perm ? ix->leaf.p : nullptr, hkey.p, hsize.p);
+    k_sort_pools<false><<<unsigned(std::min<size_t>((n + 3) / 4, size_t(ncu_m) * 16)), kBlock, 0, st>>>(
+        n, S, S, hkey.p, nullptr, hsize.p);
+  }
   // init
   bs.L = L;
   if (!bs.pool.alloc(size_t(n) * L) || !bs.flag.alloc(size_t(n) * L) || !bs.fill.alloc(n) || !bs.range.alloc(n) ||
@@ -2802,9 +2818,10 @@ int gpu_finalize_graph(efg_index* ix) {
   if (ix->b_precision > 0 && !ix->split_exact) {  // precision form: FP32 re-rank of the first 2K entries of every pool
     const int R = std::min(bs.L, 2 * K);
     const size_t m = size_t(n) * R * 2;
-    k_rerank_pool<<<unsigned((m + 255) / 256), 256, 0, ix->stream>>>(ix->data.p, int(ix->D), n, bs.L, R, bs.pool.p,
-                                                                     bs.fill.p);
-    k_sort_pools<<<grid_for(n), 256, 0, ix->stream>>>(n, bs.L, R, bs.pool.p, bs.flag.p, bs.fill.p);
+    k_rerank_pool<<<unsigned((m + 255) / 256), 256, 0, ix->stream>>>(
+        ix->data.p, int(ix->D), n, bs.L, R, ix->leaf0_perm ? ix->leaf.p : nullptr, bs.pool.p, bs.fill.p);
+    k_sort_pools<true><<<unsigned(std::min<size_t>((n + 3) / 4, 256 * 16)), kBlock, 0, ix->stream>>>(
+        n, bs.L, R, bs.pool.p, bs.flag.p, bs.fill.p);
   }
   k_graph_out<<<grid_for(n + 1), 256, 0, ix->stream>>>(n, bs.L, K, bs.pool.p, bs.fill.p, ix->goff.p, ix->gids.p);
   if (int rc = check_hip(hipStreamSynchronize(ix->stream), "graph")) return rc;
