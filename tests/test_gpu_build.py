@@ -217,10 +217,44 @@ def test_build_precision_form_float_data(cuda):
             assert dist == np.float32(oracle.l2(base[p], base[i]))
 
 
+def test_build_precision_form_exact_rows_large_norms(cuda):
+    """Rows exact in BF16 (multiples of 256 up to 65280) skip the low parts,
+    but their squared norms are far above 2^22, so the FP32 sums of the tiles
+    round and the build keeps its re-rank and id dedup: pools unique, no row
+    in its own pool, exact FP32 distances in the final pools, recall close to
+    the FP32 build's."""
+    base, _ = helpers.mixture(n=6000, nq=1, d=64, centres=30, sigma=20.0, hi=255.0, rounded=True)
+    base = base * np.float32(256.0)
+    K = 16
+    g, _o = _pair(base, trees=4, ml=4, epochs=6, check=32, L=32, K=K, S=10)
+    g.buildIndex()
+    h, _o2 = _pair(base, trees=4, ml=4, epochs=6, check=32, L=32, K=K, S=10)
+    h.setBuildPrecision(1)
+    h.buildIndex()
+    kh, _, nh, _ = h.pools()
+    n = base.shape[0]
+    live = np.arange(kh.shape[1])[None, :] < nh.astype(np.int64)[:, None]
+    ids = (kh & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    assert not ((ids == np.arange(n)[:, None]) & live).any()
+    for p in range(0, n, 11):
+        row = ids[p, :nh[p]]
+        assert len(np.unique(row)) == len(row)
+        keys = kh[p, :nh[p]]
+        assert (np.diff(keys.astype(np.uint64)) > 0).all()
+        for key in keys[:4]:
+            i = int(key & 0xFFFFFFFF)
+            assert np.uint32(int(key) >> 32).view(np.float32) == np.float32(oracle.l2(base[p], base[i]))
+    gt, _ = oracle.exact_knn(base, base, K, exclude_self=True)
+    rg = helpers.recall(g.graph_matrix(), gt, K)
+    rh = helpers.recall(h.graph_matrix(), gt, K)
+    assert rh >= rg - 0.01, (rh, rg)
+
+
 @pytest.mark.parametrize("precision", [1, 2])
 def test_build_precision_form_pools_unique(cuda, precision):
-    """Split 16-bit tiles give one distance per pair whichever block holds
-    which row, so the batched merge never keeps one id twice."""
+    """A pair's 16-bit-tile distance can depend on which block holds which
+    row, so the precision form's merge drops duplicate ids: no pool keeps one
+    id twice."""
     base, _ = helpers.mixture(n=8000, nq=1, d=200, centres=30, sigma=0.05, hi=1.0, rounded=False)
     h, _o = _pair(base, trees=4, ml=4, epochs=3, check=40, L=40, K=20, S=12)
     h.setBuildPrecision(precision)
