@@ -127,34 +127,46 @@ __device__ inline int32_t descend(const int4* __restrict__ node4, int32_t x, con
 
 // For each position: walk the ancestors at depth >= ml; where the position is
 // in the smaller child, probe the larger one.  Pass 0 counts records per leaf
-// reached, pass 1 writes (63 - depth(X)) << 32 | pos into the leaf's bucket.
+// reached and keeps the leaf of every (position, depth(X) - ml) slot in lam
+// (-1: no probe there); pass 1 reads it back instead of descending again and
+// writes (63 - depth(X)) << 32 | pos into the leaf's bucket.
 __global__ void k_probe(const float* __restrict__ X, int D, const int4* __restrict__ node4,
                         const int32_t* __restrict__ parent, const uint2* __restrict__ range,
                         const int8_t* __restrict__ depth, const uint32_t* __restrict__ leaf_t,
                         const int32_t* __restrict__ leaf_at_t, uint32_t n, int ml, int pass,
                         uint32_t* __restrict__ cnt, const int32_t* __restrict__ boff,
-                        uint32_t* __restrict__ cursor, uint64_t* __restrict__ rec) {
+                        uint32_t* __restrict__ cursor, uint64_t* __restrict__ rec, int32_t* __restrict__ lam_s,
+                        int nlev) {
   uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
   if (pos >= n) return;
-  const uint32_t p = leaf_t[pos];
-  const float* xp = X + size_t(p) * D;
   int32_t child = leaf_at_t[pos];
   int32_t x = parent[child];
+  if (pass == 1) {
+    while (x >= 0 && depth[x] >= ml) {
+      const int dx = depth[x];
+      const int32_t lam = lam_s[size_t(pos) * nlev + (dx - ml)];
+      if (lam >= 0) {
+        uint32_t at = atomicAdd(&cursor[lam], 1u);
+        rec[size_t(boff[lam]) + at] = (uint64_t(63 - dx) << 32) | pos;
+      }
+      x = parent[x];
+    }
+    return;
+  }
+  const uint32_t p = leaf_t[pos];
+  const float* xp = X + size_t(p) * D;
   while (x >= 0 && depth[x] >= ml) {
     const int4 nd = node4[x];
     const uint2 rl = range[nd.z], rr = range[nd.w];
     const uint32_t numL = rl.y - rl.x, numR = rr.y - rr.x;
     const int32_t small = (numL < numR) ? nd.z : nd.w;
+    int32_t lam = -1;
     if (child == small) {
       const int32_t big = (small == nd.z) ? nd.w : nd.z;
-      const int32_t lam = descend(node4, big, xp);
-      if (pass == 0) {
-        atomicAdd(&cnt[lam], 1u);
-      } else {
-        uint32_t at = atomicAdd(&cursor[lam], 1u);
-        rec[size_t(boff[lam]) + at] = (uint64_t(63 - depth[x]) << 32) | pos;
-      }
+      lam = descend(node4, big, xp);
+      atomicAdd(&cnt[lam], 1u);
     }
+    lam_s[size_t(pos) * nlev + (depth[x] - ml)] = lam;
     child = x;
     x = parent[x];
   }
@@ -2493,9 +2505,10 @@ int gpu_merge_init(efg_index* ix) {
   const size_t lds = size_t(kBlock) * S * 8;
   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replay), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sort_big), hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8);
-  DevBuf<int32_t> bigl;
+  DevBuf<int32_t> bigl, lam_s;
   DevBuf<int> nbig;
-  if (!bigl.alloc(M) || !nbig.alloc(1)) return fail(EFG_ERR_MEMORY, "out of device memory (merge)");
+  if (!bigl.alloc(M) || !nbig.alloc(1) || !lam_s.alloc(size_t(n) * nlev))
+    return fail(EFG_ERR_MEMORY, "out of device memory (merge)");
   int ncu_m = 256;
   {
     int dv = 0;
@@ -2507,7 +2520,7 @@ int gpu_merge_init(efg_index* ix) {
     const int32_t* la = leaf_at.p + size_t(t) * n;
     hipMemsetAsync(cnt.p, 0, size_t(M) * 4, st);
     k_probe<<<grid_for(n), 256, 0, st>>>(ix->data.p, D, ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la, n,
-                                         p.merge_level, 0, cnt.p, nullptr, nullptr, nullptr);
+                                         p.merge_level, 0, cnt.p, nullptr, nullptr, nullptr, lam_s.p, nlev);
     if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(cnt.p), M, boff.p, scal.p + 1, sums, st)) return rc;
     int total = 0;
     hipMemcpyAsync(&total, scal.p + 1, 4, hipMemcpyDeviceToHost, st);
@@ -2515,7 +2528,7 @@ int gpu_merge_init(efg_index* ix) {
     if (!rec.ensure(size_t(total) + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (merge records)");
     hipMemsetAsync(cursor.p, 0, size_t(M) * 4, st);
     k_probe<<<grid_for(n), 256, 0, st>>>(ix->data.p, D, ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la, n,
-                                         p.merge_level, 1, cnt.p, boff.p, cursor.p, rec.p);
+                                         p.merge_level, 1, cnt.p, boff.p, cursor.p, rec.p, lam_s.p, nlev);
     hipMemsetAsync(nbig.p, 0, sizeof(int), st);
     k_sort_buckets<<<unsigned(std::min<size_t>((M + 3) / 4, 65536)), kBlock, 0, st>>>(boff.p, M, rec.p, bigl.p,
                                                                                        nbig.p);
