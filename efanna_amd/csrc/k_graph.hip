@@ -179,9 +179,9 @@ __device__ inline uint64_t shfl_xor_u64(uint64_t v, int m) {
   return (uint64_t(hi) << 32) | lo;
 }
 
-// Sort every bucket ascending: one wavefront per bucket; up to 64 records in
-// registers (bitonic network across lanes), up to 1024 in LDS, longer ones
-// by one lane.
+// Sort every bucket ascending: one wavefront per bucket; up to 128 records
+// in registers (bitonic network across lanes, two records per lane above
+// 64), up to 1024 in LDS, longer ones by one lane.
 __global__ void __launch_bounds__(kBlock) k_sort_buckets(const int32_t* __restrict__ boff, int M,
                                                          uint64_t* __restrict__ rec, int32_t* __restrict__ big,
                                                          int* __restrict__ nbig) {
@@ -201,6 +201,31 @@ __global__ void __launch_bounds__(kBlock) k_sort_buckets(const int32_t* __restri
         }
       }
       if (lane < m) rec[b0 + lane] = v;
+    } else if (m <= 128) {  // two records per lane (e = lane + 64 h), registers only
+      uint64_t v[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) v[h] = lane + 64 * h < m ? rec[b0 + lane + 64 * h] : kKeyMax;
+      for (int k = 2; k <= 128; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          if (j == 64) {  // partner: the other record of this lane
+            const bool up = (lane & k) == 0;
+            const uint64_t mn = v[0] < v[1] ? v[0] : v[1], mx = v[0] < v[1] ? v[1] : v[0];
+            v[0] = up ? mn : mx;
+            v[1] = up ? mx : mn;
+          } else {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              const uint64_t o = shfl_xor_u64(v[h], j);
+              const bool up = ((lane + 64 * h) & k) == 0, lower = (lane & j) == 0;
+              const uint64_t mn = v[h] < o ? v[h] : o, mx = v[h] < o ? o : v[h];
+              v[h] = (lower == up) ? mn : mx;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (lane + 64 * h < m) rec[b0 + lane + 64 * h] = v[h];
     } else if (m <= 1024) {
       uint64_t* b = sh[wv];
       int p2 = next_pow2(m);
