@@ -480,7 +480,8 @@ __global__ void __launch_bounds__(kBlock) k_replay_dist(const int4* __restrict__
                                                         const uint2* __restrict__ pslot, int LM, int nlev,
                                                         uint64_t* __restrict__ hkey, uint32_t* __restrict__ hsize,
                                                         uint32_t* __restrict__ extra, uint32_t* __restrict__ ecount,
-                                                        unsigned long long* __restrict__ ndist) {
+                                                        unsigned long long* __restrict__ ndist,
+                                                        const int32_t* __restrict__ lam_s) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   uint64_t* h = reinterpret_cast<uint64_t*>(smem_raw) + size_t(threadIdx.x) * S;
   uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
@@ -495,15 +496,12 @@ __global__ void __launch_bounds__(kBlock) k_replay_dist(const int4* __restrict__
     const int j0 = int(pos - range[lf].x);
     int rb = boff[lf];
     const int re = boff[lf + 1];
-    int32_t child = lf;
-    int32_t x = parent[child];
-    while (x >= 0 && depth[x] >= ml) {
-      const int4 nd = node4[x];
-      const uint2 rl = range[nd.z], rr = range[nd.w];
-      const uint32_t numL = rl.y - rl.x, numR = rr.y - rr.x;
-      const int32_t small = (numL < numR) ? nd.z : nd.w;
-      if (child == small) {
-        const uint2 e = pslot[size_t(pos) * nlev + (depth[x] - ml)];
+    // the ancestors X from the leaf's parent up to depth ml, deepest first;
+    // the probe's counting pass recorded at each whether this position is in
+    // X's smaller child (its landing leaf, else -1), so no node is read here
+    for (int dx = int(depth[lf]) - 1; dx >= ml; --dx) {
+      if (lam_s[size_t(pos) * nlev + (dx - ml)] >= 0) {
+        const uint2 e = pslot[size_t(pos) * nlev + (dx - ml)];
         const uint2 r = range[e.y];
         const float* dr = dist + size_t(e.x) * LM;
         for (uint32_t i = r.x; i < r.y; ++i) {
@@ -511,15 +509,38 @@ __global__ void __launch_bounds__(kBlock) k_replay_dist(const int4* __restrict__
           heap_offer(h, sz, S, ex, ecnt, leaf_t[i], dr[i - r.x]);
         }
       } else {
-        const uint64_t want = uint64_t(63 - depth[x]);
-        while (rb < re && (rec[rb] >> 32) < want) ++rb;
-        while (rb < re && (rec[rb] >> 32) == want) {
-          heap_offer(h, sz, S, ex, ecnt, leaf_t[uint32_t(rec[rb] & 0xffffffffu)], dist[size_t(rb) * LM + j0]);
-          ++rb;
+        // this level's records of the leaf's bucket (sorted by level key,
+        // then position): [lo, hi) by binary search from rb, then offered in
+        // order, their loads issued eight at a time
+        const uint64_t want = uint64_t(63 - dx);
+        int lo = rb, hi = re;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if ((rec[mid] >> 32) < want) lo = mid + 1;
+          else hi = mid;
         }
+        int e1 = lo;
+        hi = re;
+        while (e1 < hi) {
+          const int mid = (e1 + hi) >> 1;
+          if ((rec[mid] >> 32) <= want) e1 = mid + 1;
+          else hi = mid;
+        }
+        for (int r0 = lo; r0 < e1; r0 += 8) {
+          uint32_t bid[8];
+          float bd[8];
+#pragma unroll
+          for (int b = 0; b < 8; ++b) {
+            const int r = r0 + b < e1 ? r0 + b : e1 - 1;
+            bid[b] = leaf_t[uint32_t(rec[r] & 0xffffffffu)];
+            bd[b] = dist[size_t(r) * LM + j0];
+          }
+#pragma unroll
+          for (int b = 0; b < 8; ++b)
+            if (r0 + b < e1) heap_offer(h, sz, S, ex, ecnt, bid[b], bd[b]);
+        }
+        rb = e1;
       }
-      child = x;
-      x = parent[x];
     }
     for (int j = 0; j < sz; ++j) hkey[size_t(p) * S + j] = h[j];
     hsize[p] = uint32_t(sz);
@@ -2573,7 +2594,7 @@ int gpu_merge_init(efg_index* ix) {
       }
       k_replay_dist<<<grid_for(n), kBlock, lds, st>>>(ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la, n,
                                                       p.merge_level, S, boff.p, rec.p, mdist.p, pslot.p, LM, nlev,
-                                                      hkey.p, hsize.p, extra.p, ecount.p, nd.p);
+                                                      hkey.p, hsize.p, extra.p, ecount.p, nd.p, lam_s.p);
     } else {
       k_replay<<<grid_for(n), kBlock, lds, st>>>(ix->data.p, D, ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt,
                                                  la, n, p.merge_level, S, boff.p, rec.p, hkey.p, hsize.p, extra.p,
