@@ -578,7 +578,8 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
                                                          uint32_t* __restrict__ gtab, int32_t* __restrict__ overflow,
                                                          int* __restrict__ noverflow,
                                                          unsigned long long* __restrict__ work,
-                                                         uint32_t* __restrict__ spill, int spill_logcap) {
+                                                         uint32_t* __restrict__ spill, int spill_logcap,
+                                                         int* __restrict__ qctr) {
   const int nlist = nlist_dev ? *nlist_dev : nlist_host;
   constexpr int kCtab = ctab_slots(kThreads), kNB = new_buf(kThreads);
   constexpr int kCtabLog = kThreads == 64 ? 7 : (kThreads == 128 ? 8 : 9);
@@ -750,17 +751,29 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
                             s.tab, s.misc, P, R, prevk, ckf, qi, e, nexp_last, visited);
   };
 
-  // work distribution: slices of the query order per XCD
-  int w0 = blockIdx.x, wstep = gridDim.x, wend = nlist;
+  // work distribution: the query order in slices, one per XCD (so queries
+  // that share rows share that XCD's L2).  A workgroup's first query is its
+  // index in the slice.  Workgroups of 256 threads draw every later one
+  // from the slice's counter, so those that drew cheap queries take more
+  // (pool 800, 2 epochs: 1.34 -> 1.02 ms per 10K queries); smaller ones step
+  // through the slice, since the counter's atomic, which a wavefront's
+  // loads wait behind, costs them more than the balance gains.
+  constexpr bool kDyn = kThreads == 256;
+  __shared__ int wnext_s;
+  int wbase = 0, wend = nlist, nwg = gridDim.x, wfirst = blockIdx.x;
+  int* wctr = qctr;
   if (xcd_slices && (gridDim.x & 7) == 0) {
-    const int x = blockIdx.x & 7, j = blockIdx.x >> 3;
-    w0 = int((long long)nlist * x / 8) + j;
+    const int x = blockIdx.x & 7;
+    wbase = int((long long)nlist * x / 8);
     wend = int((long long)nlist * (x + 1) / 8);
-    wstep = gridDim.x >> 3;
+    nwg = gridDim.x >> 3;
+    wfirst = blockIdx.x >> 3;
+    wctr = qctr + x;
   }
+  int wn = 0;  // thread 0: the next query, drawn while this one runs
   unsigned long long acc_nd = 0, acc_seed = 0, acc_exp = 0;  // thread 0: this workgroup's counters
   int acc_q = 0, acc_fb = 0;
-  for (int w = w0; w < wend; w += wstep) {
+  for (int w = wbase + wfirst; w < wend; w = kDyn ? wnext_s : w + nwg) {
     const int qi = qlist ? qlist[w] : w;
     const float* qg = Q + size_t(qi) * q_stride;
     for (int i = tid; i < c.D; i += kThreads) s.q[i] = (i < c.d) ? qg[i] : 0.f;
@@ -885,6 +898,9 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     int inserted = ninit;
     uint64_t nexp_total = 0;
     const uint64_t seeds_read = uint64_t(seq_read);
+    // the next query: the atomic is issued here so that the seed
+    // distances' loads, which complete after it, cover its latency
+    if (kDyn && tid == 0) wn = wbase + nwg + atomicAdd(wctr, 1);
     distances(s.buf, 0, ninit);
     if (kMethod == 2)  // slots past the candidates sort last
       for (int i = ninit + tid; i < c.init_num; i += kThreads) s.buf[i] = kKeyMax;
@@ -1200,6 +1216,9 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
         acc_exp += nexp_total;
       }
     }
+    // read after the barrier; written again only after the next query's
+    // barriers
+    if (kDyn && tid == 0) wnext_s = wn;
     __syncthreads();
   }
   // counters: one atomic per workgroup (same-address atomics per query would
@@ -1360,7 +1379,7 @@ int make_cfg(efg_index* ix, int k, int init, int E, int epochs, int trees, int m
 using SearchKernel = void (*)(SearchCfg, LearnArgs, const float*, size_t, const float*, const uint16_t*,
                               const int32_t*, const SNode*, const uint32_t*, const uint64_t*, const uint32_t*,
                               int32_t*, float*, uint64_t*, unsigned long long*, const int32_t*, int, const int*, int,
-                              uint32_t*, int32_t*, int*, unsigned long long*, uint32_t*, int);
+                              uint32_t*, int32_t*, int*, unsigned long long*, uint32_t*, int, int*);
 
 // Capacity (log2 entries) of a global-memory visited table: the worst-case
 // visited set of a query (seeds plus every expansion), at most 2^26.
@@ -1394,14 +1413,18 @@ __global__ void k_add_over(int* __restrict__ acc, const int* __restrict__ v) {
   }
 }
 
-// (The pass clears its counters with a kernel, not a memset, so that a
-// captured batch replays correctly.)
+// Counters of one pass (w.nover): [0] queries that outgrew the LDS table,
+// [1] those that outgrew the global table too, [2] queries moved to a spill
+// table, [4, 12) the query counters of the eight XCD slices, [12] the query
+// counter of the global-table pass.  (The pass clears them with a kernel,
+// not a memset, so that a captured batch replays correctly.)
+constexpr int kNoverInts = 16;
 int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const LearnArgs& la, bool bf16,
                 const float* dq, size_t nq, size_t q_stride, int32_t* d_ids, float* d_dists, uint64_t* d_ndist,
                 const int32_t* qlist, const int* nlist_dev, hipEvent_t* ev, hipStream_t st, bool count_work) {
   unsigned long long* work = count_work ? ix->work.p : nullptr;
   const int nb_q = 4096;
-  if (!w.leaves.ensure(nq * size_t(c.nleaf)) || !w.overflow.ensure(2 * nq) || !w.nover.ensure(3) ||
+  if (!w.leaves.ensure(nq * size_t(c.nleaf)) || !w.overflow.ensure(2 * nq) || !w.nover.ensure(kNoverInts) ||
       !w.nover_sum.ensure(2) || !w.ndist.ensure(3) || !w.qorder.ensure(nq) || !w.qkey.ensure(nq) ||
       !w.qhist.ensure(nb_q + 1) || !w.qoff.ensure(nb_q + 1))
     return fail(EFG_ERR_MEMORY, "out of device memory (search workspace)");
@@ -1409,7 +1432,7 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
   const size_t lds_cap = 150 * 1024;
   const bool table_in_lds = lds <= lds_cap;
   if (!table_in_lds && search_lds_bytes(c, true) > lds_cap) return fail(EFG_ERR_ARG, "pool too large for one workgroup");
-  k_zero_i32<<<1, 64, 0, st>>>(w.nover.p, 3);
+  k_zero_i32<<<1, 64, 0, st>>>(w.nover.p, kNoverInts);
   if (ev) hipEventRecord(ev[0], st);
   // seed slots: bucket probes on a hashing index, tree descent otherwise
   const SNode* snp = ix->hashing ? ix->h_nodes.p : ix->snode.p;
@@ -1469,12 +1492,19 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
 #undef EFG_K
   };
   if (table_in_lds) {
-    // workgroups per CU: LDS bound, at most 32 wavefronts
-    int per_cu = int(lds_cap / lds);
+    // workgroups per CU: as many as are resident at once (LDS and
+    // registers), at most 32 wavefronts; the workgroups draw queries from a
+    // counter, so a larger grid would only add workgroups that find none
+    auto kern = pick(false);
+    hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    int per_cu = int(lds_cap / lds), resident = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, reinterpret_cast<const void*>(kern), tq, lds) ==
+            hipSuccess &&
+        resident > 0)
+      per_cu = std::min(per_cu, resident);
     per_cu = std::max(1, std::min(per_cu, 32 / (tq / 64)));
     size_t grid = size_t(ncu) * per_cu;
     if (grid > nq) grid = (nq + 7) & ~size_t(7);
-    auto kern = pick(false);
     // method 0 (plain): a spill table per workgroup, so that queries
     // outgrowing the LDS table continue in place (not with a fixed-size
     // workspace, which keeps the global-table pass)
@@ -1487,11 +1517,11 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
         spill = w.spill.p;
       }
     }
-    hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     kern<<<unsigned(grid), unsigned(tq), lds, st>>>(c, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p,
                                                     snp, lists, ix->goff.p, ix->gids.p, d_ids, d_dists,
                                                     d_ndist, w.ndist.p, order, int(nq), nlist_dev, qlist ? 0 : 1,
-                                                    nullptr, w.overflow.p, w.nover.p, work, spill, spill_logcap);
+                                                    nullptr, w.overflow.p, w.nover.p, work, spill, spill_logcap,
+                                                    w.nover.p + 4);
   } else if (c.stop) {
     return fail(EFG_ERR_ARG, "probe pool too large for one workgroup");
   }
@@ -1504,26 +1534,31 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
     set_threads(c2, 256);
     c2.logcap = global_logcap(ix, c);
     size_t lds2 = search_lds_bytes(c2, true);
-    // as many workgroups as LDS allows, with at most 1 GiB of tables
-    const int per_cu2 = std::max(1, std::min(8, int(lds_cap / std::max<size_t>(lds2, 1))));
+    // as many workgroups as are resident at once (LDS and registers), with
+    // at most 1 GiB of tables
+    auto kern2 = pick(true);
+    hipFuncSetAttribute(reinterpret_cast<const void*>(kern2), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2));
+    int per_cu2 = std::max(1, std::min(8, int(lds_cap / std::max<size_t>(lds2, 1)))), resident2 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident2, reinterpret_cast<const void*>(kern2), kBlock, lds2) ==
+            hipSuccess &&
+        resident2 > 0)
+      per_cu2 = std::min(per_cu2, resident2);
     const size_t tab_bytes = size_t(4) << c2.logcap;
     unsigned grid2 = unsigned(std::min<size_t>(size_t(ncu) * per_cu2,
                                                std::max<size_t>(128, (size_t(1) << 30) / tab_bytes)));
     if (!w.gtab.owned) grid2 = unsigned(std::min<size_t>(grid2, w.gtab.n >> c2.logcap));  // tables that fit a view
     if (grid2 == 0 || !w.gtab.ensure(size_t(grid2) << c2.logcap))
       return fail(EFG_ERR_MEMORY, "out of device memory (search table)");
-    auto kern2 = pick(true);
-    hipFuncSetAttribute(reinterpret_cast<const void*>(kern2), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2));
     if (table_in_lds) {
       kern2<<<grid2, kBlock, lds2, st>>>(c2, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, snp,
                                          lists, ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
                                          w.overflow.p, 0, w.nover.p, 0, w.gtab.p, w.overflow.p + nq, w.nover.p + 1,
-                                         work, nullptr, 0);
+                                         work, nullptr, 0, w.nover.p + 12);
     } else {
       kern2<<<grid2, kBlock, lds2, st>>>(c2, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p, snp,
                                          lists, ix->goff.p, ix->gids.p, d_ids, d_dists, d_ndist, w.ndist.p,
                                          order, int(nq), nlist_dev, 0, w.gtab.p, w.overflow.p + nq, w.nover.p + 1,
-                                         work, nullptr, 0);
+                                         work, nullptr, 0, w.nover.p + 12);
     }
   }
   if (ev) hipEventRecord(ev[3], st);
@@ -1740,7 +1775,7 @@ int search_device_noalloc(efg_index* ix, int k, const float* dq, size_t nq, size
   if (int rc = scratch_address(&base)) return rc;
   efg::Workspace w;  // views of the scratch: nothing below allocates
   w.ndist.view(reinterpret_cast<unsigned long long*>(base + o_ndist), 3);
-  w.nover.view(reinterpret_cast<int*>(base + o_nover), 3);
+  w.nover.view(reinterpret_cast<int*>(base + o_nover), kNoverInts);
   w.nover_sum.view(reinterpret_cast<int*>(base + o_nsum), 2);
   w.scan_sums.view(reinterpret_cast<int32_t*>(base + o_sums), 64);
   w.qhist.view(reinterpret_cast<uint32_t*>(base + o_hist), nb_q + 1);
