@@ -95,6 +95,7 @@ _SIGS = {
     "efg_set_stopping_network": (c_int, [c_vp, c_f32p, c_f32p, ctypes.c_float]),
     "efg_set_search_mode": (c_int, [c_vp, c_int]),
     "efg_get_learned_stats": (c_int, [c_vp, c_u64p, c_u64p, c_u64p]),
+    "efg_get_learned_restarts": (c_int, [c_vp, c_u64p]),
 }
 
 _lib = None

@@ -776,8 +776,10 @@ int efg_set_stopping_network(efg_index* ix, const float* params, const float* no
 
 int efg_set_search_mode(efg_index* ix, int mode) {
   if (!ix) return fail(EFG_ERR_ARG, "null index");
-  if (mode != 0 && mode != 1) return fail(EFG_ERR_ARG, "search mode must be 0 (plain) or 1 (learned stopping)");
-  if (mode == 1 && !ix->net_set) return fail(EFG_ERR_STATE, "no stopping network set");
+  if (mode < 0 || mode > 2)
+    return fail(EFG_ERR_ARG, "search mode must be 0 (plain), 1 (learned stopping, restart) or 2 (learned stopping, "
+                             "continuation)");
+  if (mode != 0 && !ix->net_set) return fail(EFG_ERR_STATE, "no stopping network set");
   ix->s_mode = mode;
   return EFG_OK;
 }
@@ -785,14 +787,25 @@ int efg_set_search_mode(efg_index* ix, int mode) {
 int efg_get_learned_stats(efg_index* ix, uint64_t* last_fallbacks, uint64_t* total_fallbacks,
                           uint64_t* total_queries) {
   if (!ix) return fail(EFG_ERR_ARG, "null index");
-  int h[3] = {0, 0, 0};
+  int h[4] = {0, 0, 0, 0};
   if (ix->fb_count.p) {
     hipMemcpyAsync(h, ix->fb_count.p, sizeof(h), hipMemcpyDeviceToHost, ix->stream);
     if (int rc = efg::check_hip(hipStreamSynchronize(ix->stream), "copy")) return rc;
   }
-  if (last_fallbacks) *last_fallbacks = uint64_t(h[0]);
+  if (last_fallbacks) *last_fallbacks = uint64_t(h[3]);
   if (total_fallbacks) *total_fallbacks = uint64_t(h[1]);
   if (total_queries) *total_queries = uint64_t(h[2]);
+  return EFG_OK;
+}
+
+int efg_get_learned_restarts(efg_index* ix, uint64_t* last_restarts) {
+  if (!ix || !last_restarts) return fail(EFG_ERR_ARG, "null argument");
+  int h = 0;
+  if (ix->fb_count.p) {
+    hipMemcpyAsync(&h, ix->fb_count.p, sizeof(h), hipMemcpyDeviceToHost, ix->stream);
+    if (int rc = efg::check_hip(hipStreamSynchronize(ix->stream), "copy")) return rc;
+  }
+  *last_restarts = uint64_t(h);
   return EFG_OK;
 }
 

@@ -251,7 +251,9 @@ struct efg_index {
   efg::DevBuf<uint32_t> net_pack;                // BF16 parameters packed for the search kernel
   efg::DevBuf<float> net_norm;                   // feature mean | 1 / deviation
   efg::DevBuf<int32_t> fb_list;                  // queries handed to the full search
-  efg::DevBuf<int> fb_count;                     // [fallbacks of the last search, total fallbacks, total learned queries]
+  efg::DevBuf<int> fb_count;                     // [queries handed to the full search by the last search, total not
+                                                 //  stopped, total learned queries, not stopped in the last search]
+  bool fb_count_init = false;
   // threads per query: forced (64/128/256), else tuned per configuration, else a rule of thumb
   int s_threads = 0;
   std::map<std::array<int, 6>, int> tuned;  // (init_num, extend_to, epochs, k, trees, stop) -> threads

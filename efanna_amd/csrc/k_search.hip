@@ -168,6 +168,7 @@ constexpr uint32_t kExp = 0x80000000u;
 __host__ __device__ constexpr int ctab_slots(int threads) { return 2 * threads; }
 __host__ __device__ constexpr int new_buf(int threads) { return 2 * threads; }
 constexpr int kKPT = 16;   // seed keys per thread held in registers (init_num <= 4096)
+constexpr int kMiscInts = 12;  // per-query LDS scalars (Smem::misc)
 
 __device__ inline int tab_find(const uint32_t* tab, int logcap, uint32_t id, bool* present) {
   const uint32_t mask = (1u << logcap) - 1;
@@ -391,7 +392,7 @@ __host__ __device__ inline size_t search_lds_bytes(const SearchCfg& c, bool glob
   b += a16(sizeof(uint32_t) * ctab_slots(c.threads)) * 2;
   b += a16(sizeof(int32_t) * (c.nleaf + 1)) + a16(sizeof(int32_t) * c.nleaf);
   b += a16(sizeof(uint32_t) * c.E) + a16(sizeof(int32_t) * (c.E + 1));
-  b += a16(sizeof(uint32_t) * 256) + a16(sizeof(int) * 4) + a16(sizeof(int) * 8);
+  b += a16(sizeof(uint32_t) * 256) + a16(sizeof(int) * 4) + a16(sizeof(int) * kMiscInts);
   if (c.stop) {
     b += a16(sizeof(int32_t) * c.K);            // top k ids of the last checkpoint
     b += a16(sizeof(float) * (kFeat + 2 * kHid + 8 + kHid + 1 + 2 * kFeat));  // features, hidden units,
@@ -417,11 +418,12 @@ struct LearnArgs {
   float* tr_feat;            // stop 2: [query][checkpoint][kFeat]
   uint8_t* tr_label;         // stop 2: [query][checkpoint], top k ids found in plain_ids (254 at most), 255 = not reached
   const int32_t* plain_ids;  // stop 2: [query][K]
+  int p_pool, p_ext, p_epochs;  // stop 3: the probe phase's init_num, extend_to and epochs
 };
 
 // Probe-search checkpoint: features of the sorted pool P (R entries); stop 2
-// writes them with the label, stop 1 evaluates the network and returns true
-// when its logit reaches la.zthr.  e: checkpoint index, nexp_last: ids
+// writes them with the label, stop 1 and 3 evaluate the network and return
+// true when its logit reaches la.zthr.  e: checkpoint index, nexp_last: ids
 // expanded in the last epoch, visited: ids measured so far.  Called by every
 // thread of the workgroup after a barrier; wavefront 0 does all the work
 // (ballot counts, one lane per hidden unit, layer inputs passed between lanes
@@ -437,11 +439,11 @@ __device__ __noinline__ bool ckpt_eval(int cE, int cK, int cepochs, int clogcap,
   auto rl = [](float x, int j) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j)); };
   if (tid < 64) {
     const int lane = tid;
-    // stop 1: lane j loads the packed W1 column and biases of hidden unit j
-    // now; the loads complete while the pool statistics are gathered
+    // stop 1 and 3: lane j loads the packed W1 column and biases of hidden
+    // unit j now; the loads complete while the pool statistics are gathered
     constexpr int kW1Words = kFeat / 2;
     uint32_t wreg[kW1Words + 1];
-    if (kStop == 1) {
+    if (kStop != 2) {
 #pragma unroll
       for (int i = 0; i < kW1Words; ++i) wreg[i] = wpack[i * kHid + lane];
       wreg[kW1Words] = wpack[(kPackWords - 1) * kHid + lane];
@@ -562,7 +564,7 @@ __device__ __noinline__ bool ckpt_eval(int cE, int cK, int cepochs, int clogcap,
 // its LDS allows); the others keep their registers for loads in flight (a cap
 // of four wavefronts made the plain kernel 40 % slower at pool 500).
 template <bool kGlobal, int kMethod, bool kBf16, int kStop, int kThreads, bool kSpill = false>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kBf16 ? 4 : 1)))
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kBf16 || kStop == 3 ? 4 : 1)))
 k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
                                                          size_t q_stride,
                                                          const float* __restrict__ X, const uint16_t* __restrict__ Xb,
@@ -605,7 +607,7 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     s.expoff = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * (c.E + 1)));
     s.hist = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * 256));
     s.scan = reinterpret_cast<int*>(take(sizeof(int) * 4));
-    s.misc = reinterpret_cast<int*>(take(sizeof(int) * 8));
+    s.misc = reinterpret_cast<int*>(take(sizeof(int) * kMiscInts));
   }
   // learned stopping working arrays (after the common layout)
   int32_t* prevk = nullptr;
@@ -625,7 +627,7 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
   }
   float* w3s = ckf ? ckf + kFeat + 2 * kHid + 8 : nullptr;  // W3 | b3
   float* nrm = w3s ? w3s + kHid + 1 : nullptr;              // feature mean | 1 / deviation
-  if (kStop == 1) {
+  if (kStop == 1 || kStop == 3) {
     const float* w3 = reinterpret_cast<const float*>(la.wpack + kPackLen);
     for (int i = threadIdx.x; i <= kHid; i += kThreads) w3s[i] = w3[i];
     for (int i = threadIdx.x; i < 2 * kFeat; i += kThreads) nrm[i] = la.norm[i];
@@ -670,7 +672,12 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
   int limit = cap - cap / 4;  // keep the table under 3/4 load
   unsigned long long acc_spill = 0;
   const int pr = tid >> 1, h = tid & 1;  // lane pair for distances
-  const int R = c.R, Rp = c.Rp;
+  // the pool: R entries sorted in Rs slots.  The continuation form (stop 3)
+  // runs its probe phase with the probe's smaller pool in the same layout
+  // (N and O after the full Rp slots); the other forms keep them constant.
+  const int Rp = c.Rp;
+  std::conditional_t<kStop == 3, int, const int> R = c.R;
+  int Rs = c.Rp;  // stop 3 only
   uint64_t* P = s.buf;
   uint64_t* N = s.buf + Rp;
   uint64_t* O = s.buf + Rp + kNB;
@@ -695,11 +702,22 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
   // merge N[0, m) (unique, unsorted, disjoint from P) into the sorted pool P[0, R)
   auto merge_new = [&](int m) {
     const int mp = next_pow2(m < 2 ? 2 : m);
-    if (mp <= kThreads) {
+    if (mp <= kThreads || kStop == 3) {
       // sort N, then every key's place is its index plus its rank in the
-      // other sorted array (binary search)
+      // other sorted array (binary search); the continuation form also
+      // sorts a batch above the block this way (in LDS): its probe phase
+      // meets such batches at 256 threads
       for (int i = m + tid; i < mp; i += kThreads) N[i] = kKeyMax;
-      block_bitonic_sort_reg(N, mp);
+      if constexpr (kStop == 3) {
+        if (mp <= kThreads) {
+          block_bitonic_sort_reg(N, mp);
+        } else {
+          __syncthreads();
+          block_bitonic_sort(N, mp);
+        }
+      } else {
+        block_bitonic_sort_reg(N, mp);
+      }
       for (int i = tid; i < m; i += kThreads) {
         const uint64_t key = N[i];
         int lo = 0, hi = R;  // pool keys below key
@@ -747,8 +765,8 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
 
   // Probe-search checkpoint (all threads), out of line (see ckpt_eval).
   auto checkpoint = [&](int qi, int e, int nexp_last, int visited) -> bool {
-    return ckpt_eval<kStop>(c.E, c.K, c.epochs, c.logcap, la.wpack, la.zthr, la.tr_feat, la.tr_label, la.plain_ids,
-                            s.tab, s.misc, P, R, prevk, ckf, qi, e, nexp_last, visited);
+    return ckpt_eval<kStop>(kStop == 3 ? la.p_ext : c.E, c.K, c.epochs, c.logcap, la.wpack, la.zthr, la.tr_feat,
+                            la.tr_label, la.plain_ids, s.tab, s.misc, P, R, prevk, ckf, qi, e, nexp_last, visited);
   };
 
   // work distribution: the query order in slices, one per XCD (so queries
@@ -808,7 +826,17 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     const int total = run;
 
     // ---- seed pool -------------------------------------------------------
+    // The continuation form (stop 3) seeds in two phases: the probe's pool
+    // now; for a query the network does not stop, the rest of the full pool
+    // later (from the sequence position where this scan stopped, into the
+    // area after the pool).  Seeds go to sb[0, want).
     int p = 0;  // kept so far (uniform)
+    int want = kStop == 3 ? la.p_pool : 0;  // stop 3 only
+    uint64_t* sb = s.buf;                     // stop 3 only
+    if constexpr (kStop == 3) {  // the probe phase's pool
+      R = la.p_ext > c.K ? la.p_ext : c.K;
+      Rs = next_pow2(R < 2 ? 2 : R);
+    }
     auto seed_chunk = [&](bool valid, uint32_t id, uint32_t sq) {
       for (int i = tid; i < kCtab; i += kThreads) { s.ctab_id[i] = kEmpty; s.ctab_pos[i] = kEmpty; }
       __syncthreads();
@@ -836,12 +864,14 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       int first = (valid && !present && s.ctab_pos[cs] == sq) ? 1 : 0;
       int cnt;
       int rank = block_exclusive_scan(first, s.scan, &cnt);
-      if (first && p + rank < c.init_num) {
+      if (first && p + rank < (kStop == 3 ? want : c.init_num)) {
         tab_insert(s.tab, logcap, id);
-        s.buf[p + rank] = uint64_t(id);
+        (kStop == 3 ? sb : s.buf)[p + rank] = uint64_t(id);
+        if constexpr (kStop == 3)
+          if (p + rank == want - 1) s.misc[8] = int(sq) + 1;  // the position after the last seed
       }
       __syncthreads();
-      p = min(c.init_num, p + cnt);
+      p = min(kStop == 3 ? want : c.init_num, p + cnt);
     };
     int seq_read = 0;
     auto seed_id = [&](int sq) -> uint32_t {  // id at position sq of the leaf sequence
@@ -859,7 +889,7 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     // Its count alternates between misc[6] and misc[7], each cleared while
     // the other is in use.
     auto seed_fast = [&](bool valid, uint32_t id, int par) {
-      if (valid && tab_insert(s.tab, logcap, id)) s.buf[p + atomicAdd(&s.misc[6 + par], 1)] = uint64_t(id);
+      if (valid && tab_insert(s.tab, logcap, id)) (kStop == 3 ? sb : s.buf)[p + atomicAdd(&s.misc[6 + par], 1)] = uint64_t(id);
       __syncthreads();
       const int cnt = s.misc[6 + par];
       if (tid == 0) s.misc[7 - par] = 0;
@@ -871,30 +901,34 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     }
     __syncthreads();
     int par = 0;
+    int resume = 0;  // stop 3: the position after the last id taken or passed over
     // the next chunk's ids are loaded before this chunk is processed
     uint32_t id_next = tid < total ? seed_id(tid) : 0u;
-    for (int b0 = 0; b0 < total && p < c.init_num; b0 += kThreads) {
+    for (int b0 = 0; b0 < total && p < (kStop == 3 ? want : c.init_num); b0 += kThreads) {
       seq_read = min(total, b0 + kThreads);
       const int sq = b0 + tid;
       const bool valid = sq < total;
       const uint32_t id = id_next;
       if (sq + kThreads < total) id_next = seed_id(sq + kThreads);
-      if (kMethod != 1 && p + kThreads <= c.init_num) {
+      if constexpr (kStop == 3) resume = seq_read;
+      if (kMethod != 1 && p + kThreads <= (kStop == 3 ? want : c.init_num)) {
         seed_fast(valid, id, par);
         par ^= 1;
       } else {
         seed_chunk(valid, id, uint32_t(sq));
+        if constexpr (kStop == 3)
+          if (p >= want) resume = s.misc[8];
       }
     }
     // the hash-only search tops up to K only (hashing_index.hpp:983-991)
     // and measures the candidates it has: the ids a chunk adds past K are
     // left out of the count
     const int pseq = p;
-    for (uint32_t r0 = 0; p < (kMethod == 2 ? min(c.K, c.init_num) : c.init_num); r0 += kThreads) {
+    for (uint32_t r0 = 0; p < (kMethod == 2 ? min(c.K, c.init_num) : (kStop == 3 ? want : c.init_num)); r0 += kThreads) {
       uint32_t id = uint32_t(hash4(c.seed, ST_SEARCH, uint64_t(qi), r0 + tid) % uint64_t(c.n));
       seed_chunk(true, id, r0 + tid);
     }
-    const int ninit = kMethod == 2 ? max(pseq, min(c.K, c.init_num)) : c.init_num;
+    const int ninit = kMethod == 2 ? max(pseq, min(c.K, c.init_num)) : (kStop == 3 ? want : c.init_num);
     int inserted = ninit;
     uint64_t nexp_total = 0;
     const uint64_t seeds_read = uint64_t(seq_read);
@@ -908,26 +942,28 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     // best R seeds: select the R-th key, gather the R keys, sort them
     // (method 1 without sorting keeps the seed order, kdtreeub_index.hpp:567-570)
     if (kMethod == 0 || kMethod == 2 || c.sorted) {
-      const uint64_t thr = (R < c.init_num) ? block_select(s.buf, c.init_num, R, s.hist, s.misc) : kKeyMax;
+      const uint64_t thr = (R < (kStop == 3 ? want : c.init_num))
+                               ? block_select(s.buf, kStop == 3 ? want : c.init_num, R, s.hist, s.misc)
+                               : kKeyMax;
       uint64_t mine[kKPT];
       int nm = 0;
 #pragma unroll
       for (int j = 0; j < kKPT; ++j) {
         const int i = tid + j * kThreads;
-        mine[j] = (i < c.init_num) ? s.buf[i] : kKeyMax;
-        nm += (i < c.init_num && mine[j] <= thr) ? 1 : 0;
+        mine[j] = (i < (kStop == 3 ? want : c.init_num)) ? s.buf[i] : kKeyMax;
+        nm += (i < (kStop == 3 ? want : c.init_num) && mine[j] <= thr) ? 1 : 0;
       }
       int tot;
       int o = block_exclusive_scan(nm, s.scan, &tot);  // barriers inside: all reads precede the writes
 #pragma unroll
       for (int j = 0; j < kKPT; ++j) {
         const int i = tid + j * kThreads;
-        if (i < c.init_num && mine[j] <= thr) P[o++] = mine[j];
+        if (i < (kStop == 3 ? want : c.init_num) && mine[j] <= thr) P[o++] = mine[j];
       }
-      for (int i = R + tid; i < Rp; i += kThreads) P[i] = kKeyMax;
+      for (int i = R + tid; i < (kStop == 3 ? Rs : Rp); i += kThreads) P[i] = kKeyMax;
       __syncthreads();
-      if (Rp <= kThreads) block_bitonic_sort_reg(P, Rp);
-      else block_bitonic_sort(P, Rp);
+      if ((kStop == 3 ? Rs : Rp) <= kThreads) block_bitonic_sort_reg(P, kStop == 3 ? Rs : Rp);
+      else block_bitonic_sort(P, kStop == 3 ? Rs : Rp);
     }
     uint64_t ndq = uint64_t(ninit);
     bool ovf = false;
@@ -1056,15 +1092,17 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       ndq = uint64_t(c.init_num) + uint64_t(s.misc[3]);
     }
     // ---- epochs (method 0) ---------------------------------------------------
-    for (int ep = 0; kMethod == 0 && ep < c.epochs && !stopped; ++ep) {
+    // stop 3: the probe phase runs with the probe's extend_to and epochs
+    int cE = kStop == 3 ? la.p_ext : 0, cEp = kStop == 3 ? la.p_epochs : 0;  // stop 3 only
+    for (int ep = 0; kMethod == 0 && ep < (kStop == 3 ? cEp : c.epochs) && !stopped; ++ep) {
       // expansion list: unexpanded ids among the first extend_to (448-451)
       int nexp = 0;
-      for (int b0 = 0; b0 < c.E; b0 += kThreads) {
+      for (int b0 = 0; b0 < (kStop == 3 ? cE : c.E); b0 += kThreads) {
         int j = b0 + tid;
         int take = 0;
         uint32_t id = 0;
         int sl = -1;
-        if (j < c.E) {
+        if (j < (kStop == 3 ? cE : c.E)) {
           id = key_id(P[j]);
           bool pres;
           sl = tab_find(s.tab, logcap, id, &pres);
@@ -1159,6 +1197,178 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       if (ovf) break;
       if (kStop) stopped = checkpoint(qi, ep + 1, nexp, inserted);
     }
+    // stop 3: a query the network did not stop continues in place with the
+    // rest of the full pool (ids from the sequence position where the probe's
+    // scan stopped, then the random top-up, measured and taken with the
+    // probe's pool into the best R) and the full search's extend_to and
+    // epochs, keeping its visited table (with the expanded flags)
+    bool restart = false;  // stop 3: no room to continue in place; the full search runs instead
+    if constexpr (kStop == 3) {
+      if (!stopped && !ovf) {
+        const int R1 = R;
+        p = 0;
+        want = min(c.init_num - la.p_pool, c.n - inserted);
+        if (want + R1 < c.R || inserted + want + kThreads > limit) restart = true;
+      }
+      if (!stopped && !ovf && !restart) {
+        const int R1 = R;
+        sb = s.buf + Rp;
+        if (tid == 0) {
+          s.misc[6] = 0;
+          s.misc[7] = 0;
+        }
+        __syncthreads();
+        int par2 = 0;
+        uint32_t nxt = resume + tid < total ? seed_id(resume + tid) : 0u;
+        for (int b0 = resume; b0 < total && p < want; b0 += kThreads) {
+          const int sq = b0 + tid;
+          const bool valid = sq < total;
+          const uint32_t id = nxt;
+          if (sq + kThreads < total) nxt = seed_id(sq + kThreads);
+          if (p + kThreads <= want) {
+            seed_fast(valid, id, par2);
+            par2 ^= 1;
+          } else {
+            seed_chunk(valid, id, uint32_t(sq));
+          }
+        }
+        for (uint32_t r0 = 0; p < want; r0 += kThreads) {
+          uint32_t id = uint32_t(hash4(c.seed, ST_SEARCH, uint64_t(qi), r0 + tid) % uint64_t(c.n));
+          seed_chunk(true, id, r0 + tid);
+        }
+        inserted += want;
+        ndq += uint64_t(want);
+        distances(sb, 0, want);
+        for (int i = tid; i < R1; i += kThreads) sb[want + i] = P[i];
+        __syncthreads();
+        // best R of the new seeds and the probe's pool
+        R = c.R;
+        Rs = c.Rp;
+        const int cnt = want + R1;
+        const uint64_t thr = (R < cnt) ? block_select(sb, cnt, R, s.hist, s.misc) : kKeyMax;
+        uint64_t mine[kKPT];
+        int nm = 0;
+#pragma unroll
+        for (int j = 0; j < kKPT; ++j) {
+          const int i = tid + j * kThreads;
+          mine[j] = (i < cnt) ? sb[i] : kKeyMax;
+          nm += (i < cnt && mine[j] <= thr) ? 1 : 0;
+        }
+        int tot;
+        int o = block_exclusive_scan(nm, s.scan, &tot);
+#pragma unroll
+        for (int j = 0; j < kKPT; ++j) {
+          const int i = tid + j * kThreads;
+          if (i < cnt && mine[j] <= thr) P[o++] = mine[j];
+        }
+        for (int i = R + tid; i < Rs; i += kThreads) P[i] = kKeyMax;
+        __syncthreads();
+        if (Rs <= kThreads) block_bitonic_sort_reg(P, Rs);
+        else block_bitonic_sort(P, Rs);
+        for (int ep = 0; ep < c.epochs; ++ep) {
+          // expansion list: unexpanded ids among the first extend_to (448-451)
+          int nexp = 0;
+          for (int b0 = 0; b0 < c.E; b0 += kThreads) {
+            int j = b0 + tid;
+            int take = 0;
+            uint32_t id = 0;
+            int sl = -1;
+            if (j < c.E) {
+              id = key_id(P[j]);
+              bool pres;
+              sl = tab_find(s.tab, logcap, id, &pres);
+              take = (s.tab[sl] & kExp) ? 0 : 1;
+            }
+            int tot;
+            int rk = block_exclusive_scan(take, s.scan, &tot);
+            if (take) {
+              s.tab[sl] |= kExp;
+              s.explist[nexp + rk] = id;
+            }
+            nexp += tot;
+          }
+          __syncthreads();
+          if (nexp == 0) break;
+          nexp_total += uint64_t(nexp);
+          int nnb = 0;
+          if (c.deg > 0) {  // regular graph: the neighbours of expansion j are at (explist[j] * deg, + deg)
+            nnb = nexp * c.deg;
+          } else {
+            for (int b0 = 0; b0 < nexp; b0 += kThreads) {
+              int j = b0 + tid;
+              int deg = 0;
+              if (j < nexp) {
+                uint32_t e = s.explist[j];
+                deg = int(goff[e + 1] - goff[e]);
+              }
+              int tot;
+              int ex = block_exclusive_scan(deg, s.scan, &tot);
+              if (j < nexp) s.expoff[j] = nnb + ex;
+              nnb += tot;
+            }
+            if (tid == 0) s.expoff[nexp] = nnb;
+          }
+          if (tid == 0) s.misc[0] = 0;
+          __syncthreads();
+          int m = 0;
+          for (int b0 = 0; b0 < nnb; b0 += kThreads) {
+            int sq = b0 + tid;
+            if (sq < nnb) {
+              uint32_t id;
+              if (c.deg > 0) {
+                const int j = sq / c.deg;
+                id = gids[size_t(s.explist[j]) * c.deg + (sq - j * c.deg)];
+              } else {
+                int lo = 0, hi = nexp;
+                while (hi - lo > 1) {
+                  int mid = (lo + hi) >> 1;
+                  if (s.expoff[mid] <= sq) lo = mid; else hi = mid;
+                }
+                uint32_t e = s.explist[lo];
+                id = gids[goff[e] + uint64_t(sq - s.expoff[lo])];
+              }
+              if (tab_insert(s.tab, logcap, id)) N[atomicAdd(&s.misc[0], 1)] = uint64_t(id);
+            }
+            __syncthreads();
+            const int m2 = s.misc[0];
+            inserted += m2 - m;
+            m = m2;
+            if (kSpill && inserted + kThreads > limit && !kGlobal && kStop == 0 && spill && logcap < spill_logcap) {
+              // move the visited set into this workgroup's spill table and go on
+              uint32_t* g = spill + (size_t(blockIdx.x) << spill_logcap);
+              const int gcap = 1 << spill_logcap;
+              for (int i = tid; i < gcap; i += kThreads) g[i] = kEmpty;
+              __syncthreads();
+              for (int i = tid; i < cap; i += kThreads) {
+                const uint32_t v = s.tab[i];
+                if (v != kEmpty) tab_put_raw(g, spill_logcap, v);
+              }
+              __syncthreads();
+              s.tab = g;
+              logcap = spill_logcap;
+              cap = gcap;
+              limit = cap - cap / 4;
+              if (tid == 0) ++acc_spill;
+            }
+            if (inserted + kThreads > limit) {
+              ovf = true;
+              break;
+            }
+            const bool last = b0 + kThreads >= nnb;
+            if (m > 0 && (last || m + kThreads > kNB)) {
+              distances(N, 0, m);
+              ndq += uint64_t(m);
+              __syncthreads();
+              merge_new(m);
+              m = 0;
+              if (tid == 0) s.misc[0] = 0;
+              __syncthreads();
+            }
+          }
+          if (ovf) break;
+        }
+      }
+    }
     if (kBf16 && kMethod == 0 && !ovf) {
       // re-rank the best min(2K, R) by direct FP32 difference
       const int rr = (2 * c.K < c.R) ? 2 * c.K : c.R;
@@ -1178,13 +1388,19 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     }
     // stop 1: queries the network did not stop (or whose visited set
     // overflowed) go to the full search, which writes their results
-    const bool to_full = kStop == 1 && (ovf || !stopped);
+    // stop 3: those that overflowed or found no room to continue
+    const bool to_full = (kStop == 1 && (ovf || !stopped)) || (kStop == 3 && (ovf || restart));
     if (kStop == 1 && tid == 0) {
       ++acc_q;
       if (to_full) {
         la.fb_list[atomicAdd(la.fb_count, 1)] = qi;
         ++acc_fb;
       }
+    }
+    if (kStop == 3 && tid == 0) {
+      ++acc_q;
+      if (to_full) la.fb_list[atomicAdd(la.fb_count, 1)] = qi;
+      if (!stopped) ++acc_fb;  // not stopped: continued (or run from scratch)
     }
     if (to_full) {
       acc_nd += ndq;
@@ -1235,9 +1451,10 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
         atomicAdd(work + 2, acc_exp);
       }
     }
-    if (kStop == 1 && acc_q) {
+    if ((kStop == 1 || kStop == 3) && acc_q) {
       atomicAdd(la.fb_count + 1, acc_fb);
       atomicAdd(la.fb_count + 2, acc_q);
+      atomicAdd(la.fb_count + 3, acc_fb);
     }
   }
 }
@@ -1482,6 +1699,7 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
     if (c.method == 2) return global ? EFG_K(true, 2, false, 0, 256) : EFG_K(false, 2, false, 0, 256);
     if (c.stop == 1) return global ? nullptr : EFG_KT(false, 0, false, 1);
     if (c.stop == 2) return global ? nullptr : EFG_KT(false, 0, false, 2);
+    if (c.stop == 3) return global ? nullptr : EFG_KT(false, 0, false, 3);
     if (bf16) return global ? EFG_K(true, 0, true, 0, 256) : EFG_K(false, 0, true, 0, 256);
     if (!global && use_spill)
       return tq == 64 ? &k_search_nnexp<false, 0, false, 0, 64, true>
@@ -1582,6 +1800,21 @@ void probe_params(const efg_index* ix, int k, int* pool, int* ext, int* ep) {
   *ep = ix->lp_epochs > 0 ? ix->lp_epochs : ix->s_epochs;
 }
 
+// The continuation form's probe must fit inside the full search: its pool
+// and extend_to at most the search's, and max(extend_to, k) at most its pool.
+int probe_within(const SearchCfg& c, int pp, int pe) {
+  if (pp > c.init_num || pe > c.E || std::max(pe, c.K) > pp)
+    return fail(EFG_ERR_ARG, "probe pool and extend_to must lie within the search's, max(extend_to, k) within the probe pool");
+  return EFG_OK;
+}
+
+// Key buffer of the continuation form: the pool's Rp slots, then the rest of
+// the full pool's seeds and the probe's pool together.
+void cfg_continue(SearchCfg& c, int pp, int pe) {
+  const int r1 = std::max(pe, c.K);
+  c.kb = std::max(c.kb, c.Rp + (c.init_num - pp) + r1);
+}
+
 // Threads per query for a configuration: forced, tuned, or by pool size
 // (a wavefront for pools up to 256, two up to 1024, else four); the seed
 // keys held in registers need init_num <= 16 * threads.
@@ -1656,7 +1889,7 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
     const size_t tot = ix->n * ix->D;
     k_pack_bf16<<<unsigned((tot + 255) / 256), 256, 0, st>>>(ix->data.p, int(ix->D), ix->n, ix->data_bf16.p);
   }
-  const bool learned = method == 0 && ix->s_mode == 1 && !ix->hashing;
+  const bool learned = method == 0 && ix->s_mode != 0 && !ix->hashing;
   if (learned && !ix->net_set) return fail(EFG_ERR_STATE, "no stopping network set");
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   const bool prof = ix->profiling && ix->prof_events.size() < 4 * 4096;
@@ -1668,27 +1901,44 @@ int search_device(efg_index* ix, int k, const float* dq, size_t nq, size_t q_str
                           prof ? ev : nullptr, st))
       return rc;
   } else {
-    // probe search stopped by the network; the rest runs the full search
+    // probe search stopped by the network; mode 1: the rest runs the full
+    // search from scratch; mode 2: the rest continues in place into the
+    // full search (one pass), and only queries without room to continue
+    // run the full search
     int pp, pe, pep;
     probe_params(ix, k, &pp, &pe, &pep);
     SearchCfg cp;
-    if (int rc = make_cfg(ix, k, pp, pe, pep, ix->s_trees, 0, 1, &cp, nullptr)) return rc;
-    set_threads(cp, choose_threads(ix, cp));
-    if (!ix->fb_count.p) {
-      if (!ix->fb_count.ensure(3)) return fail(EFG_ERR_MEMORY, "out of device memory");
-      hipMemsetAsync(ix->fb_count.p, 0, 3 * sizeof(int), st);
+    if (ix->s_mode == 2) {
+      if (int rc = probe_within(c, pp, pe)) return rc;
+      cp = c;
+      cp.stop = 3;
+      set_threads(cp, choose_threads(ix, cp));
+      cfg_continue(cp, pp, pe);
+      la.p_pool = pp;
+      la.p_ext = pe;
+      la.p_epochs = pep;
+    } else {
+      if (int rc = make_cfg(ix, k, pp, pe, pep, ix->s_trees, 0, 1, &cp, nullptr)) return rc;
+      set_threads(cp, choose_threads(ix, cp));
+    }
+    if (!ix->fb_count.ensure(4)) return fail(EFG_ERR_MEMORY, "out of device memory");
+    if (!ix->fb_count_init) {
+      hipMemsetAsync(ix->fb_count.p, 0, 4 * sizeof(int), st);
+      ix->fb_count_init = true;
     }
     if (!ix->fb_list.ensure(nq)) return fail(EFG_ERR_MEMORY, "out of device memory");
     hipMemsetAsync(ix->fb_count.p, 0, sizeof(int), st);
+    hipMemsetAsync(ix->fb_count.p + 3, 0, sizeof(int), st);
     la.wpack = ix->net_pack.p;
     la.norm = ix->net_norm.p;
     la.zthr = ix->net_zthr;
     la.fb_list = ix->fb_list.p;
     la.fb_count = ix->fb_count.p;
-    if (int rc = run_pass(ix, cp, la, false, dq, nq, q_stride, d_ids, d_dists, d_ndist, nullptr, nullptr, nullptr, st))
+    if (int rc = run_pass(ix, cp, la, false, dq, nq, q_stride, d_ids, d_dists, d_ndist, nullptr, nullptr,
+                          ix->s_mode == 2 && prof ? ev : nullptr, st))
       return rc;
     if (int rc = run_pass(ix, c, LearnArgs{}, false, dq, nq, q_stride, d_ids, d_dists, d_ndist, ix->fb_list.p,
-                          ix->fb_count.p, prof ? ev : nullptr, st))
+                          ix->fb_count.p, ix->s_mode == 1 && prof ? ev : nullptr, st))
       return rc;
   }
   if (prof)
@@ -1801,23 +2051,30 @@ int search_autotune(efg_index* ix, int k, const float* dq, size_t nq, size_t q_s
     if (chosen) chosen[0] = chosen[1] = 256;
     return EFG_OK;
   }
-  const bool learned = ix->s_mode == 1;
+  const bool learned = ix->s_mode != 0;
   if (learned && !ix->net_set) return fail(EFG_ERR_STATE, "no stopping network set");
-  // the configurations this mode runs: the full search, and the probe when learned
+  // the configurations this mode runs: the full search, and the probe (mode
+  // 1) or the probe with its continuation (mode 2) when learned
   SearchCfg cfg[2];
   int ncfg = 0;
   if (int rc = make_cfg(ix, k, ix->s_init, ix->s_extend, ix->s_epochs, ix->s_trees, 0, 0, &cfg[ncfg++], nullptr))
     return rc;
+  int pp = 0, pe = 0, pep = 0;
   if (learned) {
-    int pp, pe, pep;
     probe_params(ix, k, &pp, &pe, &pep);
-    if (int rc = make_cfg(ix, k, pp, pe, pep, ix->s_trees, 0, 1, &cfg[ncfg++], nullptr)) return rc;
+    if (ix->s_mode == 2) {
+      if (int rc = probe_within(cfg[0], pp, pe)) return rc;
+      cfg[ncfg] = cfg[0];
+      cfg[ncfg++].stop = 3;
+    } else if (int rc = make_cfg(ix, k, pp, pe, pep, ix->s_trees, 0, 1, &cfg[ncfg++], nullptr)) {
+      return rc;
+    }
     if (!ix->fb_list.ensure(nq)) return fail(EFG_ERR_MEMORY, "out of device memory");
   }
   DevBuf<int32_t> ids;
   DevBuf<float> dd;
   DevBuf<int> tcount;  // fallback counters of the tuning runs (zeroed before each)
-  if (!ids.alloc(nq * size_t(k)) || !dd.alloc(nq * size_t(k)) || !tcount.alloc(3))
+  if (!ids.alloc(nq * size_t(k)) || !dd.alloc(nq * size_t(k)) || !tcount.alloc(4))
     return fail(EFG_ERR_MEMORY, "out of device memory");
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
@@ -1826,21 +2083,25 @@ int search_autotune(efg_index* ix, int k, const float* dq, size_t nq, size_t q_s
   for (int ci = 0; ci < ncfg && rc == EFG_OK; ++ci) {
     SearchCfg c = cfg[ci];
     LearnArgs la{};
-    if (c.stop == 1) {
+    if (c.stop != 0) {
       la.wpack = ix->net_pack.p;
       la.norm = ix->net_norm.p;
       la.zthr = ix->net_zthr;
       la.fb_list = ix->fb_list.p;
       la.fb_count = tcount.p;
+      la.p_pool = pp;
+      la.p_ext = pe;
+      la.p_epochs = pep;
     }
     float best = 0.f;
     int best_t = 0;
     for (int t : {64, 128, 256}) {
       if (c.init_num > kKPT * t) continue;
       set_threads(c, t);
+      if (c.stop == 3) cfg_continue(c, pp, pe);
       float ms_sum = 0.f;
       for (int rep = 0; rep < 3 && rc == EFG_OK; ++rep) {
-        hipMemsetAsync(tcount.p, 0, 3 * sizeof(int), st);
+        hipMemsetAsync(tcount.p, 0, 4 * sizeof(int), st);
         hipEventRecord(e0, st);
         rc = run_pass(ix, c, la, false, dq, nq, q_stride, ids.p, dd.p, nullptr, nullptr, nullptr, nullptr, st, false);
         hipEventRecord(e1, st);

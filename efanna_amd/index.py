@@ -281,13 +281,22 @@ class FIndex:
                                              float(threshold)))
 
     def setSearchMode(self, mode: int):
-        """0 = plain search; 1 = probe search stopped by the network, full search as fallback."""
+        """0 = plain search; 1 = probe search stopped by the network, the full
+        search from scratch for the rest; 2 = the same probe, the rest continue
+        in place into the full search."""
         check(lib().efg_set_search_mode(self.h, int(mode)))
 
     def learned_stats(self):
+        """last_fallbacks: queries of the last learned search the network did
+        not stop; last_restarts: those of them that ran the full search from
+        scratch; running totals."""
         v = [ctypes.c_uint64() for _ in range(3)]
         check(lib().efg_get_learned_stats(self.h, *[ctypes.byref(x) for x in v]))
-        return dict(zip(["last_fallbacks", "total_fallbacks", "total_queries"], [x.value for x in v]))
+        r = ctypes.c_uint64()
+        check(lib().efg_get_learned_restarts(self.h, ctypes.byref(r)))
+        out = dict(zip(["last_fallbacks", "total_fallbacks", "total_queries"], [x.value for x in v]))
+        out["last_restarts"] = r.value
+        return out
 
     def getSearchParams(self):
         v = [ctypes.c_int() for _ in range(5)]

@@ -158,8 +158,13 @@ int efg_set_join_staging_margin(efg_index* ix, double margin);
  * W3[64] | b3, evaluated on BF16 parameters with FP32 accumulation) stands in
  * for the fixed epoch/pool stopping rule of method 0.  A probe search with a
  * small pool evaluates it after the seeds and after every epoch; a query
- * stops when the network's probability reaches the threshold, otherwise it
- * runs the full plain search (setSearchParams) from scratch. */
+ * stops when the network's probability reaches the threshold.  Otherwise,
+ * in search mode 1 it runs the full plain search (setSearchParams) from
+ * scratch; in search mode 2 it continues in place: the rest of the full pool
+ * (the next ids of the full search's leaf sequence not yet measured, from
+ * where the probe stopped) joins the probe's pool, and the full search's
+ * epochs follow with the probe's visited set.  Mode 2 needs the probe's pool
+ * and extend_to within the search's. */
 int efg_mlp_param_count(void);
 /* Initial parameters from the generator (uniform Glorot weights, zero biases). */
 int efg_mlp_init(float* params, uint64_t seed);
@@ -185,11 +190,16 @@ int efg_search_traces(efg_index* ix, int k, const float* queries, size_t nq, siz
                       const int32_t* plain_ids, float* feat, uint8_t* label);
 /* Network used when the search mode is 1; a query stops at probability >= threshold. */
 int efg_set_stopping_network(efg_index* ix, const float* params, const float* norm, float threshold);
-/* Search mode for method 0: 0 = plain (default), 1 = probe search + network + full-search fallback. */
+/* Search mode for method 0: 0 = plain (default), 1 = probe search + network + full search from scratch,
+ * 2 = probe search + network + continuation in place. */
 int efg_set_search_mode(efg_index* ix, int mode);
-/* Queries of the last learned search that fell back, and running totals. */
+/* Queries of the last learned search that the network did not stop, and running totals. */
 int efg_get_learned_stats(efg_index* ix, uint64_t* last_fallbacks, uint64_t* total_fallbacks,
                           uint64_t* total_queries);
+/* Queries of the last learned search that ran the full search from scratch:
+ * in mode 1 every query not stopped; in mode 2 those whose visited set
+ * outgrew the workgroup's table or with too few rows left to continue. */
+int efg_get_learned_restarts(efg_index* ix, uint64_t* last_restarts);
 
 /* Heterogeneous batch (no counterpart in the reference; the reference runs
  * one configuration per knnSearch): query q runs method 0 with init_num

@@ -310,7 +310,8 @@ class Index:
                      mode, params=None, norm=None, zthr=0.0, plain=None):
         """Learned stopping restated: mode 2 returns traces (features, labels);
         mode 1 returns (ids, dists, stopped) of the probe search with the
-        network and the full-search fallback."""
+        network and the full search from scratch for the queries it does not
+        stop; mode 3 the same with those queries continued in place."""
         q = np.ascontiguousarray(queries, np.float32)
         nq = q.shape[0]
         C = p_epochs + 1

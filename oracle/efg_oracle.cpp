@@ -956,24 +956,80 @@ float net_logit(const float* w, const float* norm, const float* f) {
 
 struct ProbeArgs {
   SearchArgs full, probe;
-  int mode;             // 1 learned, 2 trace
+  int mode;             // 1 learned (restart), 2 trace, 3 learned (continuation)
   const float* w;       // mode 1: rounded parameters
   const float* norm;    // mode 1
   float zthr;           // mode 1
   const int32_t* plain; // mode 2: this query's plain top k
 };
 
-// Returns whether the network stopped the probe (mode 1).  feat/label (mode
-// 2) hold probe.epochs + 1 checkpoints.
+// The leaf sequence of a query: the ids of the first lsize leaves of every
+// tree, leaf rank outer, tree inner (the order seed_pool walks, without
+// dedup or bound).
+void leaf_sequence(const Index& ix, const float* q, unsigned lsize, int trees, std::vector<uint32_t>& seq) {
+  std::vector<std::vector<int32_t>> cand(trees);
+  for (int i = 0; i < trees; ++i) search_node_list(ix, ix.roots[i], q, lsize, cand[i]);
+  seq.clear();
+  for (unsigned ni = 0; ni < lsize; ++ni)
+    for (int i = 0; i < trees; ++i) {
+      if (ni >= cand[i].size()) continue;
+      const Node& lf = ix.nodes[cand[i][ni]];
+      const uint32_t* ll = &ix.leaf[size_t(i) * ix.n];
+      for (uint32_t j = lf.start; j < lf.end; ++j) seq.push_back(ll[j]);
+    }
+}
+
+// The next `want` ids not yet visited, from position `from` of the sequence,
+// then from the generator's draws 0, 1, 2, ... (the search top-up); marks
+// them visited.  *resume: the position after the last id taken from the
+// sequence (its end when the sequence ran out).
+void take_seeds(const Index& ix, const std::vector<uint32_t>& seq, size_t from, int want, uint64_t qid,
+                std::vector<uint8_t>& visited, std::vector<uint32_t>& out, size_t* resume) {
+  out.clear();
+  size_t pos = from;
+  for (; pos < seq.size() && int(out.size()) < want; ++pos) {
+    const uint32_t nn = seq[pos];
+    if (visited[nn]) continue;
+    visited[nn] = 1;
+    out.push_back(nn);
+  }
+  if (resume) *resume = pos;
+  uint64_t r = 0;
+  while (int(out.size()) < want) {
+    const uint32_t nn = uint32_t(hash4(ix.p.seed, ST_SEARCH, qid, r++) % ix.n);
+    if (visited[nn]) continue;
+    visited[nn] = 1;
+    out.push_back(nn);
+  }
+}
+
+// Returns whether the network stopped the probe (modes 1 and 3).  feat/label
+// (mode 2) hold probe.epochs + 1 checkpoints.
+//
+// Mode 3 (continuation): the probe takes its seeds from the full search's
+// leaf sequence.  A query the network does not stop continues in place: the
+// next min(full.init_num - probe.init_num, n - visited) ids not yet visited,
+// from where the probe's scan stopped (then the top-up draws from 0), are
+// measured and taken with the probe's pool into the best max(full.extend_to,
+// K); the full search's epochs follow with the same visited and expanded
+// flags.  When those ids and the probe's pool are fewer than that pool, the
+// query runs the full search from scratch instead.
 bool search_probe(const Index& ix, const float* q, uint64_t qid, const ProbeArgs& pa, int32_t* out_ids,
                   float* out_d, float* feat, uint8_t* label) {
   const SearchArgs& a = pa.probe;
   const size_t n = ix.n;
   const int K = a.K;
   std::vector<uint8_t> visited(n, 0), expanded(n, 0);
-  unsigned lsize = unsigned(a.init_num * 2 / (5 * a.trees) + 1);
-  std::vector<uint32_t> pool;
-  seed_pool(ix, q, lsize, a.init_num, a.trees, qid, visited, pool);
+  std::vector<uint32_t> pool, seq;
+  size_t resume = 0;
+  if (pa.mode == 3) {
+    const SearchArgs& f = pa.full;
+    leaf_sequence(ix, q, unsigned(f.init_num * 2 / (5 * f.trees) + 1), f.trees, seq);
+    take_seeds(ix, seq, 0, a.init_num, qid, visited, pool, &resume);
+  } else {
+    unsigned lsize = unsigned(a.init_num * 2 / (5 * a.trees) + 1);
+    seed_pool(ix, q, lsize, a.init_num, a.trees, qid, visited, pool);
+  }
   size_t R = size_t(std::max(a.extend_to, a.K));
   std::vector<std::pair<float, uint32_t>> res;
   for (uint32_t id : pool) res.push_back({l2_avx(q, ix.row(id), ix.d), id});
@@ -1033,6 +1089,43 @@ bool search_probe(const Index& ix, const float* q, uint64_t qid, const ProbeArgs
   if (pa.mode == 1 && !stopped) {
     uint64_t nd = 0;
     search_nnexp(ix, q, qid, pa.full, out_ids, out_d, nd);
+    return false;
+  }
+  if (pa.mode == 3 && !stopped) {
+    const SearchArgs& f = pa.full;
+    const size_t Rf = size_t(std::max(f.extend_to, f.K));
+    const int want = std::min(f.init_num - a.init_num, int(n - nvis));
+    if (size_t(want) + R < Rf) {
+      uint64_t nd = 0;
+      search_nnexp(ix, q, qid, f, out_ids, out_d, nd);
+      return false;
+    }
+    std::vector<uint32_t> more;
+    take_seeds(ix, seq, resume, want, qid, visited, more, nullptr);
+    for (uint32_t id : more) res.push_back({l2_avx(q, ix.row(id), ix.d), id});
+    std::partial_sort(res.begin(), res.begin() + Rf, res.end());
+    res.resize(Rf);
+    for (int it = 0; it < f.epochs; ++it) {
+      ids.clear();
+      for (int j = 0; j < f.extend_to; ++j) {
+        uint32_t pj = res[j].second;
+        if (expanded[pj]) continue;
+        expanded[pj] = 1;
+        for (uint64_t g = ix.goff[pj]; g < ix.goff[pj + 1]; ++g) {
+          uint32_t id = ix.gids[g];
+          if (visited[id]) continue;
+          visited[id] = 1;
+          ids.push_back(id);
+        }
+      }
+      for (uint32_t id : ids) res.push_back({l2_avx(q, ix.row(id), ix.d), id});
+      std::partial_sort(res.begin(), res.begin() + Rf, res.end());
+      res.resize(Rf);
+    }
+    for (int i = 0; i < K; ++i) {
+      out_ids[i] = int32_t(res[i].second);
+      if (out_d) out_d[i] = res[i].first;
+    }
     return false;
   }
   for (int i = 0; i < K; ++i) {
@@ -1444,13 +1537,17 @@ int orc_search_probe(void* h, const float* q, size_t nq, size_t qstride, int k, 
   if (extend_to > init_num) extend_to = init_num;
   if (p_extend > p_pool) p_extend = p_pool;
   if (search_trees <= 0 || search_trees > ix->T) { g_err = "wrong tree number"; return -4; }
-  if (mode != 1 && mode != 2) { g_err = "mode must be 1 or 2"; return -2; }
+  if (mode < 1 || mode > 3) { g_err = "mode must be 1, 2 or 3"; return -2; }
+  if (mode == 3 && (p_pool > init_num || p_extend > extend_to || std::max(p_extend, k) > p_pool)) {
+    g_err = "probe pool and extend_to must lie within the search's, max(extend_to, k) within the probe pool";
+    return -2;
+  }
   ProbeArgs pa{};
   pa.full = SearchArgs{epochs, init_num, extend_to, search_trees, k};
   pa.probe = SearchArgs{p_epochs, p_pool, p_extend, search_trees, k};
   pa.mode = mode;
   std::vector<float> w(kNP);
-  if (mode == 1)
+  if (mode != 2)
     for (int i = 0; i < kNP; ++i) w[i] = round_bf16(params[i]);
   pa.w = w.data();
   pa.norm = norm;

@@ -3,9 +3,10 @@
 The network kernels are checked against the oracle: logits bit for bit
 (same BF16 parameters, same FP32 fused multiply-add order), gradients within
 FP32 tolerance of the oracle's FP64 gradient.  Probe-search traces and the
-learned search itself (probe + network + full-search fallback) must equal
-the oracle's restatement bit for bit.  The last test shows what the trained
-network gains over the plain search.
+learned search itself (probe + network + full search from scratch, mode 1;
+probe + network + continuation in place, mode 2) must equal the oracle's
+restatement bit for bit.  The last tests show what the trained network gains
+over the plain search and where it sits against its ablations.
 """
 import math
 
@@ -24,6 +25,7 @@ PROBE = (200, 60, 6)     # pool, extend_to, epochs of the probe search
 K = 10
 NTEST = 200             # the first queries are held out; the rest train
 AGREE = 0.9             # positive label: at least 9 of the probe's top 10 are in the plain top 10
+PROBE_C = (200, 60, 2)  # probe of the continuation form (within FULL)
 
 
 @pytest.fixture(scope="module")
@@ -40,6 +42,20 @@ def trained(setup):
     base, q, oix, gix = setup
     w, norm, hist, pos = learned.train_for_index(gix, q[NTEST:], K, agree=AGREE, epochs=20, batch=256, lr=1e-3,
                                                  seed=5)
+    return w, norm, hist, pos
+
+
+@pytest.fixture(scope="module")
+def trained_c(setup):
+    """A network for the continuation form: traces of its probe, positive
+    when the probe's top 10 equals the plain top 10."""
+    base, q, oix, gix = setup
+    gix.setProbeParams(*PROBE_C)
+    try:
+        w, norm, hist, pos = learned.train_for_index(gix, q[NTEST:], K, agree=1.0, epochs=20, batch=256, lr=1e-3,
+                                                     seed=6)
+    finally:
+        gix.setProbeParams(*PROBE)
     return w, norm, hist, pos
 
 
@@ -196,3 +212,94 @@ def test_learned_against_ablations(cuda, setup, trained):
     # the best fixed setting at or above the learned form's recall, in work
     at = [wk for name, wk, rc in rows if name.startswith("fixed") and rc >= learned_r]
     print("best fixed setting at or above the learned recall:", min(at) if at else None, "dist/q")
+
+
+@pytest.mark.parametrize("thr", [0.3, 0.7, 0.99])
+def test_continuation_matches_oracle(cuda, setup, trained_c, thr):
+    """Mode 2: ids and distance bits equal the oracle's continuation form
+    (mode 3); no query needed the full search from scratch."""
+    base, q, oix, gix = setup
+    w, norm, _, _ = trained_c
+    qt = q[:NTEST]
+    gix.setProbeParams(*PROBE_C)
+    gix.setStoppingNetwork(w, norm, thr)
+    gix.setSearchMode(2)
+    try:
+        gix.knnSearch(K, qt)
+        ids, d = gix.results()
+        st = gix.learned_stats()
+    finally:
+        gix.setSearchMode(0)
+        gix.setProbeParams(*PROBE)
+    oids, od, stopped = oix.search_probe(qt, K, *FULL, *PROBE_C, mode=3, params=w, norm=norm, zthr=_zthr(thr))
+    assert st["last_restarts"] == 0
+    np.testing.assert_array_equal(ids, oids)
+    np.testing.assert_array_equal(d.view(np.uint32), od.view(np.uint32))
+    assert st["last_fallbacks"] == int((~stopped).sum())
+    if thr == 0.7:
+        assert 0 < stopped.sum() < NTEST
+
+
+def test_continuation_refuses_probe_outside_search(cuda, setup, trained_c):
+    base, q, oix, gix = setup
+    w, norm, _, _ = trained_c
+    gix.setStoppingNetwork(w, norm, 0.5)
+    gix.setSearchMode(2)
+    try:
+        for probe in ((FULL[1] + 100, 60, 2), (200, FULL[2] + 10, 2)):
+            gix.setProbeParams(*probe)
+            with pytest.raises(Exception):
+                gix.knnSearch(K, q[:10])
+    finally:
+        gix.setSearchMode(0)
+        gix.setProbeParams(*PROBE)
+
+
+def test_continuation_against_ablations(cuda, setup, trained_c):
+    """The continuation form against the plain search, the probe without the
+    network and fixed settings, in distance evaluations per query and
+    recall@10 against exact kNN.  A higher threshold stops fewer queries, so
+    the queries continued and the work grow with it; a query that continues
+    keeps the probe's visited set and pool and runs the rest of the full
+    search, so with (nearly) every query continued the recall is the plain
+    search's within 0.01.  The printed table places the form among the fixed
+    settings (the bench line records the same at 1M x 128, where the
+    queries differ more in difficulty than in this small mixture)."""
+    base, q, oix, gix = setup
+    w, norm, _, _ = trained_c
+    qt = q[:NTEST]
+    gt, _ = oracle.exact_knn(base, qt, K)
+
+    def run(epochs, pool, extend, mode=0, thr=0.5):
+        gix.setSearchParams(epochs, pool, extend, FULL[3], -1, 0)
+        gix.setStoppingNetwork(w, norm, thr)
+        gix.setProbeParams(*PROBE_C)
+        gix.setSearchMode(mode)
+        try:
+            gix.search_work()
+            gix.knnSearch(K, qt)
+            ids, _ = gix.results()
+            work = gix.search_work()[0] / NTEST
+            st = gix.learned_stats()
+        finally:
+            gix.setSearchMode(0)
+            gix.setProbeParams(*PROBE)
+        return work, helpers.recall(ids, gt, K), (st["last_fallbacks"] if mode else None)
+
+    full_w, full_r, _ = run(*FULL[:3])
+    probe_w, probe_r, _ = run(PROBE_C[2], PROBE_C[0], PROBE_C[1])
+    rows = [("full", full_w, full_r, None), ("probe, no network", probe_w, probe_r, None)]
+    thrs = (0.3, 0.5, 0.7, 0.9, 0.999)
+    for thr in thrs:
+        rows.append((f"continuation p>={thr}",) + run(*FULL[:3], mode=2, thr=thr))
+    for pool, ext, ep in ((300, 100, 2), (400, 66, 2), (600, 100, 2)):
+        rows.append((f"fixed {pool}/{ext}/{ep}",) + run(ep, pool, ext))
+    gix.setSearchParams(*FULL, -1, 0)
+    for name, wk, rc, fb in rows:
+        print(f"{name:24s} {wk:8.1f} dist/q  recall {rc:.4f}" + (f"  continued {fb}" if fb is not None else ""))
+    cont = [(wk, rc, fb) for name, wk, rc, fb in rows if name.startswith("continuation")]
+    assert all(a[2] <= b[2] for a, b in zip(cont, cont[1:]))
+    assert all(a[0] <= b[0] + 1e-9 for a, b in zip(cont, cont[1:]))
+    wk, rc, fb = cont[-1]
+    if fb == NTEST:
+        assert rc >= full_r - 0.01

@@ -119,3 +119,68 @@ def test_learned_search_extremes():
     assert not stopped.any()
     fids, _, _ = oix.search(q, k, *full, 0)
     np.testing.assert_array_equal(ids, fids)
+
+
+def test_continuation_pins():
+    """The continuation form (mode 3) against what its definition fixes.
+
+    (a) A network that always stops returns the probe's seeds-only result.
+    (b) With the probe's extend_to equal to the search's and no probe epoch,
+        a network that never stops returns the plain search's result: the
+        continuation takes the next ids of the leaf sequence, so the seeds
+        are the plain search's first init_num ids, and the best R of them
+        that lie among the probe's seeds are within the probe's best R.
+    (c) With probe epochs, a network that never stops continues every query:
+        the result is a valid top k (distinct ids, sorted true distances),
+        its recall stays within 0.01 of the plain search's, and it differs
+        from the restart form (mode 1) on some query, so the continuation
+        path is the one taken.
+    A planted mistake in the oracle (the continuation skipping one sequence
+    position, or dropping the probe's pool) fails (b)."""
+    base, q, oix = helpers.oracle_index("mixture", n=20_000, nq=200, centres=50)
+    full, k = (4, 300, 50, 8), 10
+    norm = np.concatenate([np.zeros(16), np.ones(16)]).astype(np.float32)
+    w = np.zeros(NP, np.float32)
+    w[B3] = 4.0
+    probe = (100, 20, 3)
+    ids, _, stopped = oix.search_probe(q, k, *full, *probe, mode=3, params=w, norm=norm, zthr=2.0)
+    assert stopped.all()
+    pids, _, _ = oix.search(q, k, 0, probe[0], probe[1], full[3], 0)
+    np.testing.assert_array_equal(ids, pids)
+    w[B3] = -4.0
+    fids, fd, _ = oix.search(q, k, *full, 0)
+    for pp in (60, 150, 300):
+        ids, d, stopped = oix.search_probe(q, k, *full, pp, full[2], 0, mode=3, params=w, norm=norm, zthr=2.0)
+        assert not stopped.any()
+        np.testing.assert_array_equal(ids, fids)
+        np.testing.assert_array_equal(d.view(np.uint32), fd.view(np.uint32))
+    # no epoch and k = extend_to: the whole pool, the best 50 of the first 300 seeds
+    seeds0 = (0, 300, 50, 8)
+    sids, _, _ = oix.search(q, 50, *seeds0, 0)
+    for pp in (60, 150):
+        ids, _, _ = oix.search_probe(q, 50, *seeds0, pp, 50, 0, mode=3, params=w, norm=norm, zthr=2.0)
+        np.testing.assert_array_equal(ids, sids)
+    ids, d, stopped = oix.search_probe(q, k, *full, *probe, mode=3, params=w, norm=norm, zthr=2.0)
+    assert not stopped.any()
+    for row, dr, qq in zip(ids, d, q):
+        assert len(set(row.tolist())) == k
+        assert (np.diff(dr) >= 0).all()
+        true = np.array([oracle.l2(qq, base[i]) for i in row], np.float32)
+        np.testing.assert_array_equal(dr.view(np.uint32), true.view(np.uint32))
+    gt, _ = oracle.exact_knn(base, q, k)
+    assert helpers.recall(ids, gt, k) >= helpers.recall(fids, gt, k) - 0.01
+    rids, _, _ = oix.search_probe(q, k, *full, *probe, mode=1, params=w, norm=norm, zthr=2.0)
+    np.testing.assert_array_equal(rids, fids)
+    assert (ids != rids).any(axis=1).sum() > 0
+
+
+def test_continuation_refuses_probe_outside_search():
+    base, q, oix = helpers.oracle_index("mixture", n=20_000, nq=10, centres=50)
+    w = np.zeros(NP, np.float32)
+    norm = np.concatenate([np.zeros(16), np.ones(16)]).astype(np.float32)
+    for probe in ((400, 20, 2), (100, 60, 2), (5, 5, 2)):
+        try:
+            oix.search_probe(q, 10, 4, 300, 50, 8, *probe, mode=3, params=w, norm=norm, zthr=0.0)
+        except oracle.OracleError:
+            continue
+        raise AssertionError(f"probe {probe} accepted")
