@@ -421,6 +421,32 @@ def main() -> int:
         g_ms = (time.perf_counter() - t0) * 1e3 / bsteps
         graph_same = bool((bids.cpu().numpy() == bres).all())
         plan.close()
+        # the one-pass form: all classes in one or two persistent kernels
+        # (shape tuned on the held-out batch), eager and as a graph
+        gix.setBatchForm(True)
+        qb_tune = torch.from_numpy(datasets.mixture_queries(d, nb, centres=1000, sigma=20.0, hi=255.0, rounded=True,
+                                                            seed=0, query_seed=78, uniform_query_frac=0.3)).to(dev)
+        bstep(qb_tune.data_ptr(), tune=True)
+        del qb_tune
+        bstep(qbd.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(bsteps):
+            bstep(qbd.data_ptr())
+        torch.cuda.synchronize()
+        b1_ms = (time.perf_counter() - t0) * 1e3 / bsteps
+        one_same = bool((bids.cpu().numpy() == bres).all())
+        plan = gix.batch_plan(qbd.data_ptr(), nb, d, pools, ks, 100, bids.data_ptr(), bd.data_ptr(),
+                              stream=stream.cuda_stream)
+        plan.run(stream.cuda_stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(bsteps):
+            plan.run(stream.cuda_stream)
+        torch.cuda.synchronize()
+        g1_ms = (time.perf_counter() - t0) * 1e3 / bsteps
+        plan.close()
+        gix.setBatchForm(False)
         t = time.perf_counter()
         bgt = ground_truth(qbd, nb, 100)
         bgt_s = time.perf_counter() - t
@@ -432,6 +458,8 @@ def main() -> int:
         forms["batched"] = {"queries": nb, "ms_per_batch": round(b_ms, 3), "qps": round(nb / (b_ms / 1e3), 1),
                             "graph_ms_per_batch": round(g_ms, 3), "graph_qps": round(nb / (g_ms / 1e3), 1),
                             "graph_identical": graph_same,
+                            "one_pass": {"ms_per_batch": round(b1_ms, 3), "graph_ms_per_batch": round(g1_ms, 3),
+                                         "qps": round(nb / (b1_ms / 1e3), 1), "identical": one_same},
                             "recall_at_k": round(hits / float(ks.sum()), 5), "recall_at_k_by_k": per_k,
                             "pools": [200, 400, 800, 1600], "ks": [1, 10, 100], "uniform_query_frac": 0.3,
                             "extend": "pool // 6", "ground_truth_s": round(bgt_s, 3)}

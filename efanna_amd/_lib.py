@@ -96,6 +96,7 @@ _SIGS = {
     "efg_set_search_mode": (c_int, [c_vp, c_int]),
     "efg_get_learned_stats": (c_int, [c_vp, c_u64p, c_u64p, c_u64p]),
     "efg_get_learned_restarts": (c_int, [c_vp, c_u64p]),
+    "efg_set_batch_form": (c_int, [c_vp, c_int]),
 }
 
 _lib = None

@@ -630,6 +630,13 @@ int efg_batch_plan_destroy(efg_batch_plan* plan) {
   return EFG_OK;
 }
 
+int efg_set_batch_form(efg_index* ix, int one_pass) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (one_pass != 0 && one_pass != 1) return fail(EFG_ERR_ARG, "batch form must be 0 or 1");
+  ix->b_mode = one_pass;
+  return EFG_OK;
+}
+
 int efg_knn_search_batch(efg_index* ix, const float* queries, size_t nq, size_t q_stride, const int* pool,
                          const int* extend, const int* k, int out_stride, int32_t* ids, float* dists, int tune) {
   if (!ix) return fail(EFG_ERR_ARG, "null index");

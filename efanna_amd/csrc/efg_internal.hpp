@@ -163,6 +163,12 @@ struct Workspace {
   DevBuf<float> dists;
   DevBuf<int32_t> blist;  // batched search: queries per class, concatenated
   DevBuf<int> bcount;     // batched search: queries per class
+  // batched search in one pass: class configurations, the class and the
+  // leaves per tree of every query, the order the workgroups draw queries in
+  DevBuf<SearchCfg> bcfg;
+  DevBuf<uint8_t> bcls;
+  DevBuf<int> blsize;
+  DevBuf<int32_t> border;
 };
 
 }  // namespace efg
@@ -264,6 +270,12 @@ struct efg_index {
   hipEvent_t bfork = nullptr;
   std::vector<hipEvent_t> bjoin;
   std::vector<std::unique_ptr<efg::Workspace>> bws;
+  // batched search form: 0 one pass per class on the parallel streams
+  // (default), 1 one pass over all classes; the one-pass shape chosen by the
+  // tuner: classes with pools up to b_split in a group at b_tsmall threads
+  // per query, the rest at b_tlarge (b_split 0: one group)
+  int b_mode = 0;
+  int b_split = 0, b_tsmall = 128, b_tlarge = 256;
 };
 
 namespace efg {

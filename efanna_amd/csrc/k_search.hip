@@ -72,7 +72,8 @@ template <bool kLdsQ>
 __global__ void __launch_bounds__(kDescThreads)
     k_descend(const float* __restrict__ Q, size_t q_stride, int nq, int d, int T, int lsize,
               const int32_t* __restrict__ roots, const DNode* __restrict__ dn, int32_t* __restrict__ leaves,
-              const int32_t* __restrict__ qlist, const int* __restrict__ nlist) {
+              const int32_t* __restrict__ qlist, const int* __restrict__ nlist, const int* __restrict__ qlsize,
+              size_t lstride) {
   __shared__ uint32_t stk[kDescStack][kDescThreads];
   extern __shared__ float qrows[];  // kLdsQ: rows of list positions [p0, p1]
   const int tid = threadIdx.x;
@@ -95,7 +96,9 @@ __global__ void __launch_bounds__(kDescThreads)
   if (gid >= nthr) return;
   const int qi = qlist ? qlist[gid / T] : int(gid / T), t = int(gid % T);
   const float* q = kLdsQ ? qrows + (gid / T - p0) * d : Q + size_t(qi) * q_stride;
-  int32_t* out = leaves + size_t(qi) * size_t(lsize) * T;  // [ni][t]
+  // a batch of classes: leaves per tree per query (qlsize), rows of lstride slots
+  if (qlsize) lsize = qlsize[qi];
+  int32_t* out = leaves + size_t(qi) * (qlsize ? lstride : size_t(lsize) * T);  // [ni][t]
   int cnt = 0;
   int node = roots[t];
   DNode cur = dn[node];
@@ -419,6 +422,10 @@ struct LearnArgs {
   uint8_t* tr_label;         // stop 2: [query][checkpoint], top k ids found in plain_ids (254 at most), 255 = not reached
   const int32_t* plain_ids;  // stop 2: [query][K]
   int p_pool, p_ext, p_epochs;  // stop 3: the probe phase's init_num, extend_to and epochs
+  // batched search in one pass (kBatch): the configuration of every class and
+  // the class of every query
+  const SearchCfg* ccfg;
+  const uint8_t* qcls;
 };
 
 // Probe-search checkpoint: features of the sorted pool P (R entries); stop 2
@@ -563,7 +570,7 @@ __device__ __noinline__ bool ckpt_eval(int cE, int cK, int cepochs, int clogcap,
 // The BF16-row variant is held to 128 VGPRs (four wavefronts per SIMD, which
 // its LDS allows); the others keep their registers for loads in flight (a cap
 // of four wavefronts made the plain kernel 40 % slower at pool 500).
-template <bool kGlobal, int kMethod, bool kBf16, int kStop, int kThreads, bool kSpill = false>
+template <bool kGlobal, int kMethod, bool kBf16, int kStop, int kThreads, bool kSpill = false, bool kBatch = false>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kBf16 || kStop == 3 ? 4 : 1)))
 k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
                                                          size_t q_stride,
@@ -676,7 +683,8 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
   // runs its probe phase with the probe's smaller pool in the same layout
   // (N and O after the full Rp slots); the other forms keep them constant.
   const int Rp = c.Rp;
-  std::conditional_t<kStop == 3, int, const int> R = c.R;
+  const int lstride = c.nleaf;  // kBatch: leaf slots per query (the classes' largest)
+  std::conditional_t<kStop == 3 || kBatch, int, const int> R = c.R;
   int Rs = c.Rp;  // stop 3 only
   uint64_t* P = s.buf;
   uint64_t* N = s.buf + Rp;
@@ -793,6 +801,14 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
   int acc_q = 0, acc_fb = 0;
   for (int w = wbase + wfirst; w < wend; w = kDyn ? wnext_s : w + nwg) {
     const int qi = qlist ? qlist[w] : w;
+    if constexpr (kBatch) {  // this query's class: pool, extend_to, k, leaves, table size
+      c = la.ccfg[__builtin_amdgcn_readfirstlane(int(la.qcls[__builtin_amdgcn_readfirstlane(qi)]))];
+      R = c.R;
+      Rs = c.Rp;
+      logcap = c.logcap;
+      cap = 1 << logcap;
+      limit = cap - cap / 4;
+    }
     const float* qg = Q + size_t(qi) * q_stride;
     for (int i = tid; i < c.D; i += kThreads) s.q[i] = (i < c.d) ? qg[i] : 0.f;
     if (kSpill && s.tab != tab0) {  // the last query moved to the spill table
@@ -808,7 +824,7 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       int li = b0 + tid;
       int sz = 0;
       if (li < c.nleaf) {
-        int nd = leaves[size_t(qi) * c.nleaf + li];
+        int nd = leaves[size_t(qi) * (kBatch ? lstride : c.nleaf) + li];
         int st = 0;
         if (nd >= 0) {
           st = int(sn[nd].start);
@@ -960,10 +976,10 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
         const int i = tid + j * kThreads;
         if (i < (kStop == 3 ? want : c.init_num) && mine[j] <= thr) P[o++] = mine[j];
       }
-      for (int i = R + tid; i < (kStop == 3 ? Rs : Rp); i += kThreads) P[i] = kKeyMax;
+      for (int i = R + tid; i < (kStop == 3 || kBatch ? Rs : Rp); i += kThreads) P[i] = kKeyMax;
       __syncthreads();
-      if ((kStop == 3 ? Rs : Rp) <= kThreads) block_bitonic_sort_reg(P, kStop == 3 ? Rs : Rp);
-      else block_bitonic_sort(P, kStop == 3 ? Rs : Rp);
+      if ((kStop == 3 || kBatch ? Rs : Rp) <= kThreads) block_bitonic_sort_reg(P, kStop == 3 || kBatch ? Rs : Rp);
+      else block_bitonic_sort(P, kStop == 3 || kBatch ? Rs : Rp);
     }
     uint64_t ndq = uint64_t(ninit);
     bool ovf = false;
@@ -1405,8 +1421,11 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     if (to_full) {
       acc_nd += ndq;
     } else if (ovf) {
-      if (kStop == 0 && tid == 0) overflow[atomicAdd(noverflow, 1)] = qi;
-      if (kGlobal) {
+      // kBatch: the query outgrew its spill table, sized as the global
+      // pass's would be, so it has no result either (the row says so)
+      if (kStop == 0 && !kBatch && tid == 0) overflow[atomicAdd(noverflow, 1)] = qi;
+      if (kBatch && tid == 0) atomicAdd(noverflow + 1, 1);
+      if (kGlobal || kBatch) {
         // outgrew the largest global table as well: no result, the row says so
         for (int j = tid; j < c.K; j += kThreads) {
           out_ids[size_t(qi) * c.out_stride + j] = -1;
@@ -1663,10 +1682,10 @@ int run_pass_ws(efg_index* ix, efg::Workspace& w, const SearchCfg& c, const Lear
       const size_t qbytes = size_t((kDescThreads - 1) / c.T + 2) * size_t(c.d) * sizeof(float);
       if (qbytes <= kDescLdsQ)
         k_descend<true><<<grid, kDescThreads, qbytes, st>>>(dq, q_stride, int(nq), c.d, c.T, c.lsize, ix->sroots.p,
-                                                            ix->dnode.p, w.leaves.p, qlist, nlist_dev);
+                                                            ix->dnode.p, w.leaves.p, qlist, nlist_dev, nullptr, 0);
       else
         k_descend<false><<<grid, kDescThreads, 0, st>>>(dq, q_stride, int(nq), c.d, c.T, c.lsize, ix->sroots.p,
-                                                        ix->dnode.p, w.leaves.p, qlist, nlist_dev);
+                                                        ix->dnode.p, w.leaves.p, qlist, nlist_dev, nullptr, 0);
     }
     if (!qlist) {  // query order by first leaf (bucket)
       int shift = std::max(0, ceil_log2(ix->n + 1) - 12);
@@ -2160,14 +2179,202 @@ __global__ void k_fill_results(int32_t* __restrict__ ids, float* __restrict__ d,
   if (d) d[i] = 3.402823466e38f;
 }
 
+// One persistent pass over a group of a batch's classes: the largest sizes
+// over the group (its LDS layout), its spill-table size and its queries'
+// range in the draw order.
+struct FusedGroup {
+  SearchCfg c0{};
+  int spill_logcap = 0;
+  int off = 0, count = 0;
+};
+
 // A heterogeneous batch after its classes are formed, their query lists are
-// on the device and their threads per query are chosen.
+// on the device and their threads per query are chosen.  One-pass form
+// (fused): the classes' configurations, each query's class and leaves per
+// tree, and the draw order are on the device; the classes run in one or two
+// groups, each one persistent pass.
 struct BatchSetup {
   std::vector<SearchCfg> cfg;
   std::vector<int> offs, counts;
   size_t nq = 0;
   int out_stride = 0;
+  bool fused = false;
+  std::vector<FusedGroup> groups;
+  int lstride = 0;  // leaf slots per query (the classes' largest)
 };
+
+// Shape of the one-pass form: the classes with pools up to `split` form one
+// group at `t_small` threads per query, the others another at `t_large`
+// (split 0: all in one group at t_large).
+struct FusedShape {
+  int split = 0, t_small = 128, t_large = 256;
+};
+
+SearchKernel fused_kernel(int tq) {
+  return tq == 64 ? &k_search_nnexp<false, 0, false, 0, 64, true, true>
+                  : tq == 128 ? &k_search_nnexp<false, 0, false, 0, 128, true, true>
+                              : &k_search_nnexp<false, 0, false, 0, 256, true, true>;
+}
+
+// One group's launches (no host synchronisation, no memset, so a plan can
+// capture them): a descent of the group's queries with each query's leaves
+// per tree, then one persistent search kernel whose workgroups draw the
+// group's queries from the order in w.border, each query with its class's
+// configuration; a query about to outgrow the LDS visited table continues in
+// the workgroup's spill table.  wg holds the group's counters and spill
+// tables, w the batch's shared arrays.
+int fused_launch_group(efg_index* ix, efg::Workspace& w, efg::Workspace& wg, const BatchSetup& b, size_t g,
+                       const float* dq, size_t q_stride, int32_t* d_ids, float* d_dists, hipStream_t st) {
+  const FusedGroup& fg = b.groups[g];
+  const SearchCfg& c0 = fg.c0;
+  if (fg.count == 0) return EFG_OK;
+  if (!wg.nover.ensure(kNoverInts) || !wg.ndist.ensure(3) || !wg.overflow.ensure(1))
+    return fail(EFG_ERR_MEMORY, "out of device memory (batch)");
+  k_zero_i32<<<1, 64, 0, st>>>(wg.nover.p, kNoverInts);
+  const int32_t* order = w.border.p + fg.off;
+  const size_t threads = size_t(fg.count) * size_t(c0.T);
+  const unsigned dgrid = unsigned((threads + kDescThreads - 1) / kDescThreads);
+  const size_t qbytes = size_t((kDescThreads - 1) / c0.T + 2) * size_t(c0.d) * sizeof(float);
+  if (qbytes <= kDescLdsQ)
+    k_descend<true><<<dgrid, kDescThreads, qbytes, st>>>(dq, q_stride, int(b.nq), c0.d, c0.T, c0.lsize, ix->sroots.p,
+                                                         ix->dnode.p, w.leaves.p, order, w.bcount.p + g, w.blsize.p,
+                                                         size_t(b.lstride));
+  else
+    k_descend<false><<<dgrid, kDescThreads, 0, st>>>(dq, q_stride, int(b.nq), c0.d, c0.T, c0.lsize, ix->sroots.p,
+                                                     ix->dnode.p, w.leaves.p, order, w.bcount.p + g, w.blsize.p,
+                                                     size_t(b.lstride));
+  const int tq = c0.threads;
+  SearchKernel kern = fused_kernel(tq);
+  const size_t lds = search_lds_bytes(c0, false);
+  hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  int dev = 0, ncu = 256;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  int per_cu = int(150 * 1024 / lds), resident = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, reinterpret_cast<const void*>(kern), tq, lds) ==
+          hipSuccess &&
+      resident > 0)
+    per_cu = std::min(per_cu, resident);
+  per_cu = std::max(1, std::min(per_cu, 32 / (tq / 64)));
+  size_t grid = size_t(ncu) * per_cu;
+  if (grid > size_t(fg.count)) grid = (size_t(fg.count) + 7) & ~size_t(7);
+  if (!wg.spill.ensure(grid << fg.spill_logcap)) return fail(EFG_ERR_MEMORY, "out of device memory (spill tables)");
+  LearnArgs la{};
+  la.ccfg = w.bcfg.p;
+  la.qcls = w.bcls.p;
+  kern<<<unsigned(grid), unsigned(tq), lds, st>>>(c0, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p,
+                                                  ix->snode.p, ix->leaf.p, ix->goff.p, ix->gids.p, d_ids, d_dists,
+                                                  nullptr, wg.ndist.p, order, fg.count, nullptr, 1, nullptr,
+                                                  wg.overflow.p, wg.nover.p, ix->work.p, wg.spill.p,
+                                                  fg.spill_logcap, wg.nover.p + 4);
+  k_add_over<<<1, 64, 0, st>>>(w.nover_sum.p, wg.nover.p);
+  return check_hip(hipGetLastError(), "batch launch");
+}
+
+// The one-pass form's configurations and device arrays for the classes
+// `cls` in shape `sh`: each class at its group's threads per query; per
+// group the largest sizes, the spill-table size and the draw order (the
+// eight XCD slices of a group each take every eighth query of every class,
+// classes by pool size, largest first, so that every slice ends on small
+// queries).  Fails (EFG_ERR_ARG) when a class does not fit its group's
+// threads or a group's largest configuration one workgroup's LDS.
+int fused_setup(efg_index* ix, efg::Workspace& w, const std::vector<BatchClass>& cls, const FusedShape& sh,
+                BatchSetup* b, hipStream_t st) {
+  std::vector<SearchCfg> cc = b->cfg;
+  std::vector<int> grp(cc.size());
+  b->lstride = 0;
+  for (size_t i = 0; i < cc.size(); ++i) {
+    grp[i] = (sh.split > 0 && cc[i].init_num <= sh.split) ? 0 : 1;
+    const int t = grp[i] == 0 ? sh.t_small : sh.t_large;
+    if (cc[i].init_num > kKPT * t) return fail(EFG_ERR_ARG, "pool too large for the threads per query");
+    set_threads(cc[i], t);
+    b->lstride = std::max(b->lstride, cc[i].nleaf);
+  }
+  b->groups.clear();
+  std::vector<int32_t> order;
+  order.reserve(b->nq);
+  std::vector<int> counts;
+  for (int g = 0; g < 2; ++g) {
+    std::vector<size_t> members;
+    for (size_t i = 0; i < cc.size(); ++i)
+      if (grp[i] == g) members.push_back(i);
+    if (members.empty()) continue;
+    std::stable_sort(members.begin(), members.end(),
+                     [&](size_t a, size_t c) { return cc[a].init_num > cc[c].init_num; });
+    FusedGroup fg;
+    fg.c0 = cc[members[0]];
+    for (size_t i : members) {
+      const SearchCfg& c = cc[i];
+      fg.c0.kb = std::max(fg.c0.kb, c.kb);
+      fg.c0.logcap = std::max(fg.c0.logcap, c.logcap);
+      fg.c0.lsize = std::max(fg.c0.lsize, c.lsize);
+      fg.c0.E = std::max(fg.c0.E, c.E);
+      fg.c0.K = std::max(fg.c0.K, c.K);
+      fg.c0.R = std::max(fg.c0.R, c.R);
+      fg.c0.Rp = std::max(fg.c0.Rp, c.Rp);
+      fg.c0.init_num = std::max(fg.c0.init_num, c.init_num);
+      fg.spill_logcap = std::max(fg.spill_logcap, global_logcap(ix, c));
+    }
+    fg.c0.nleaf = b->lstride;  // the leaf-slot row of every query
+    if (search_lds_bytes(fg.c0, false) > 150 * 1024)
+      return fail(EFG_ERR_ARG, "batch classes too large for one workgroup");
+    fg.off = int(order.size());
+    for (int x = 0; x < 8; ++x)
+      for (size_t r : members)
+        for (size_t j = size_t(x); j < cls[r].q.size(); j += 8) order.push_back(cls[r].q[j]);
+    fg.count = int(order.size()) - fg.off;
+    counts.push_back(fg.count);
+    b->groups.push_back(fg);
+  }
+  const size_t nq = b->nq;
+  std::vector<uint8_t> qc(nq);
+  std::vector<int> ql(nq);
+  for (size_t i = 0; i < cls.size(); ++i)
+    for (int32_t qi : cls[i].q) {
+      qc[size_t(qi)] = uint8_t(i);
+      ql[size_t(qi)] = cc[i].lsize;
+    }
+  if (!w.bcfg.ensure(cc.size()) || !w.bcls.ensure(nq) || !w.blsize.ensure(nq) || !w.border.ensure(nq) ||
+      !w.bcount.ensure(std::max<size_t>(counts.size(), b->counts.size())) ||
+      !w.leaves.ensure(nq * size_t(b->lstride)) || !w.nover_sum.ensure(2))
+    return fail(EFG_ERR_MEMORY, "out of device memory (batch)");
+  // pageable sources: the calls return once the host data is staged
+  if (int rc = check_hip(hipMemcpyAsync(w.bcfg.p, cc.data(), cc.size() * sizeof(SearchCfg), hipMemcpyHostToDevice,
+                                        st), "copy classes"))
+    return rc;
+  if (int rc = check_hip(hipMemcpyAsync(w.bcls.p, qc.data(), nq, hipMemcpyHostToDevice, st), "copy classes"))
+    return rc;
+  if (int rc = check_hip(hipMemcpyAsync(w.blsize.p, ql.data(), nq * 4, hipMemcpyHostToDevice, st), "copy leaves"))
+    return rc;
+  if (int rc = check_hip(hipMemcpyAsync(w.border.p, order.data(), nq * 4, hipMemcpyHostToDevice, st), "copy order"))
+    return rc;
+  if (int rc = check_hip(hipMemcpyAsync(w.bcount.p, counts.data(), counts.size() * 4, hipMemcpyHostToDevice, st),
+                         "copy counts"))
+    return rc;
+  b->fused = true;
+  return EFG_OK;
+}
+
+int batch_streams(efg_index* ix);
+
+// The one-pass launches of a batch: one group on the caller's stream, two
+// forked onto two of the parallel streams (each with a workspace of its own
+// for counters and spill tables) and joined back.
+int fused_launch(efg_index* ix, efg::Workspace& w, std::vector<std::unique_ptr<efg::Workspace>>& cws,
+                 const BatchSetup& b, const float* dq, size_t q_stride, int32_t* d_ids, float* d_dists,
+                 hipStream_t st) {
+  if (b.groups.size() == 1) return fused_launch_group(ix, w, w, b, 0, dq, q_stride, d_ids, d_dists, st);
+  if (int rc = batch_streams(ix)) return rc;
+  while (cws.size() < b.groups.size()) cws.emplace_back(new efg::Workspace());
+  if (int rc = check_hip(hipEventRecord(ix->bfork, st), "batch fork")) return rc;
+  for (size_t g = 0; g < b.groups.size(); ++g) {
+    if (int rc = check_hip(hipStreamWaitEvent(ix->bstreams[g], ix->bfork, 0), "batch fork")) return rc;
+    if (int rc = fused_launch_group(ix, w, *cws[g], b, g, dq, q_stride, d_ids, d_dists, ix->bstreams[g])) return rc;
+    if (int rc = check_hip(hipEventRecord(ix->bjoin[g], ix->bstreams[g]), "batch join")) return rc;
+    if (int rc = check_hip(hipStreamWaitEvent(st, ix->bjoin[g], 0), "batch join")) return rc;
+  }
+  return EFG_OK;
+}
 
 int batch_setup(efg_index* ix, efg::Workspace& w, const float* dq, size_t nq, size_t q_stride, const int* pool,
                 const int* ext, const int* kq, int out_stride, int32_t* d_ids, float* d_dists, bool tune,
@@ -2205,6 +2412,66 @@ int batch_setup(efg_index* ix, efg::Workspace& w, const float* dq, size_t nq, si
                          "copy query counts"))
     return rc;
   int rc = EFG_OK;
+  if (ix->b_mode == 1 && cls.size() <= 255) {
+    // one pass over all classes, in one group or two split by pool size:
+    // the shape (split, threads per query of each group) tuned on this
+    // batch (tune) or kept from the last tuning
+    FusedShape sh{ix->b_split, ix->b_tsmall, ix->b_tlarge};
+    if (tune) {
+      std::vector<FusedShape> shapes;
+      std::vector<int> pools;
+      for (const auto& c : b->cfg) pools.push_back(c.init_num);
+      std::sort(pools.begin(), pools.end());
+      pools.erase(std::unique(pools.begin(), pools.end()), pools.end());
+      for (int tl : {128, 256}) shapes.push_back(FusedShape{0, 128, tl});
+      for (size_t i = 0; i + 1 < pools.size(); ++i)
+        for (int ts : {64, 128})
+          for (int tl : {128, 256}) shapes.push_back(FusedShape{pools[i], ts, tl});
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      hipEventCreate(&e0);
+      hipEventCreate(&e1);
+      float best = 0.f;
+      FusedShape best_sh{};
+      bool found = false;
+      for (const FusedShape& cand : shapes) {
+        if (rc != EFG_OK) break;
+        BatchSetup tb = *b;
+        if (fused_setup(ix, w, cls, cand, &tb, st) != EFG_OK) continue;  // does not fit
+        float sum = 0.f;
+        for (int rep = 0; rep < 3 && rc == EFG_OK; ++rep) {
+          hipEventRecord(e0, st);
+          rc = fused_launch(ix, w, ix->bws, tb, dq, q_stride, d_ids, d_dists, st);
+          hipEventRecord(e1, st);
+          if (rc == EFG_OK) rc = check_hip(hipEventSynchronize(e1), "batch tuning");
+          float ms = 0.f;
+          hipEventElapsedTime(&ms, e0, e1);
+          if (rep > 0) sum += ms;
+        }
+        if (rc == EFG_OK && (!found || sum < best)) {
+          best = sum;
+          best_sh = cand;
+          found = true;
+        }
+      }
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+      if (rc == EFG_OK && found) {
+        sh = best_sh;
+        ix->b_split = sh.split;
+        ix->b_tsmall = sh.t_small;
+        ix->b_tlarge = sh.t_large;
+      }
+    }
+    if (rc == EFG_OK) {
+      rc = fused_setup(ix, w, cls, sh, b, st);
+      if (rc != EFG_OK) rc = fused_setup(ix, w, cls, FusedShape{0, 128, 256}, b, st);
+      if (rc != EFG_OK) {  // too large for one pass: one pass per class
+        b->fused = false;
+        rc = EFG_OK;
+      }
+    }
+    if (b->fused) return rc;
+  }
   if (tune) {  // time 64/128/256 threads per query on each class, keep the fastest
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEventCreate(&e0);
@@ -2271,6 +2538,7 @@ int batch_launch(efg_index* ix, efg::Workspace& w, std::vector<std::unique_ptr<e
   k_fill_results<<<unsigned((m + 255) / 256), 256, 0, st>>>(d_ids, d_dists, m);
   if (!w.nover_sum.ensure(2)) return fail(EFG_ERR_MEMORY, "out of device memory (batch)");
   k_zero_i32<<<1, 64, 0, st>>>(w.nover_sum.p, 2);
+  if (b.fused) return fused_launch(ix, w, cws, b, dq, q_stride, d_ids, d_dists, st);
   const size_t nc = b.cfg.size();
   if (nc == 1) {
     return run_pass_ws(ix, w, b.cfg[0], LearnArgs{}, false, dq, b.nq, q_stride, d_ids, d_dists, nullptr, w.blist.p,

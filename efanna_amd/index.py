@@ -207,6 +207,10 @@ class FIndex:
         check(lib().efg_set_search_precision(self.h, int(precision)))
 
     # --- heterogeneous batch (no counterpart in the reference)
+    def setBatchForm(self, one_pass: bool = True):
+        """Batched search: all classes in one pass, or one pass per class (default)."""
+        check(lib().efg_set_batch_form(self.h, 1 if one_pass else 0))
+
     def knnSearchBatch(self, query, pool, k, extend=None, out_stride=None, tune=False):
         """Per-query pool (init_num), k and extend_to (default pool // 6); returns (ids, dists)
         with out_stride columns (default max k), -1 / FLT_MAX past each query's k."""

@@ -205,11 +205,13 @@ int efg_get_learned_restarts(efg_index* ix, uint64_t* last_restarts);
  * one configuration per knnSearch): query q runs method 0 with init_num
  * pool[q], extend_to extend[q] (nullable: max(1, pool[q] / 6)) and k[q], the
  * epochs and trees of setSearchParams.  Queries are grouped by (pool,
- * extend_to, k) and each group runs with its own threads per query (tuned
- * first when tune != 0).  Row q of ids/dists (out_stride entries, >= every
- * k[q]) holds the k[q] results, then -1 / FLT_MAX.  pool, extend, k are host
- * arrays; the _device variant takes device queries and outputs and is
- * ordered on `stream`. */
+ * extend_to, k) classes.  By default each class runs its own persistent
+ * pass (threads per query tuned per class when tune != 0), the classes on
+ * four parallel streams; efg_set_batch_form selects the one-pass form
+ * instead.  Row q of ids/dists (out_stride entries, >= every k[q]) holds the
+ * k[q] results, then -1 / FLT_MAX.  pool, extend, k are host arrays; the
+ * _device variant takes device queries and outputs and is ordered on
+ * `stream`. */
 int efg_knn_search_batch(efg_index* ix, const float* queries, size_t nq, size_t q_stride, const int* pool,
                          const int* extend, const int* k, int out_stride, int32_t* ids, float* dists, int tune);
 int efg_knn_search_batch_device(efg_index* ix, const float* d_queries, size_t nq, size_t q_stride, const int* pool,
@@ -228,6 +230,14 @@ int efg_batch_plan_create(efg_index* ix, const float* d_queries, size_t nq, size
                           void* stream, efg_batch_plan** plan);
 int efg_batch_plan_run(efg_batch_plan* plan, void* stream);
 int efg_batch_plan_destroy(efg_batch_plan* plan);
+/* Form of the batched search (no counterpart in the reference): 0 = one
+ * pass per class, the classes on four parallel streams, each with its own
+ * threads per query (default); 1 = one pass over all classes: one descent,
+ * then one persistent kernel whose workgroups draw the queries of every
+ * class from one order and take each query's configuration from its class
+ * (with tune != 0 the tuner picks the shape: one group, or two split by pool
+ * size, and the threads per query of each).  Results are the same. */
+int efg_set_batch_form(efg_index* ix, int one_pass);
 
 /* Search without device allocation (no counterpart in the reference).  For
  * callers that cannot allocate device memory: code under HIP graph capture,

@@ -127,7 +127,12 @@ def test_d960_shape_bf16_rows_search(cuda):
     assert rf >= 0.95 and rb >= rf - 0.005, (rf, rb)
 
 
-def test_batch_workload_values_far_queries_match_oracle(cuda):
+@pytest.mark.parametrize("one_pass", [True, False])
+def test_batch_workload_values_far_queries_match_oracle(cuda, one_pass):
+    """The batched workload's values (pools to 1600, k to 100, 30 % uniform
+    far queries), all classes in one pass or one pass per class: far queries
+    outgrow the LDS visited table and continue in a spill table, and every
+    row equals the oracle's search with that query's parameters."""
     base, q, oix = helpers.oracle_index("mixture", n=20_000, nq=200, centres=50, trees=16, ml=6)
     rng = np.random.default_rng(21)
     far = rng.integers(0, 256, (86, base.shape[1])).astype(np.float32)  # 30 % of the queries
@@ -138,10 +143,11 @@ def test_batch_workload_values_far_queries_match_oracle(cuda):
     gix.set_graph(*oix.graph())
     epochs, trees = 4, 16
     gix.setSearchParams(epochs, 400, 66, trees, -1, 0)
+    gix.setBatchForm(one_pass)
     pools = rng.choice([200, 400, 800, 1600], size=nq).astype(np.int32)
     ks = rng.choice([1, 10, 100], size=nq).astype(np.int32)
     ids, d = gix.knnSearchBatch(qq, pools, ks, out_stride=100)
-    assert gix.counters()["overflow"] > 0  # the global-table pass ran inside the batch
+    assert gix.counters()["overflow"] > 0  # queries outgrew the LDS table inside the batch
     assert gix.counters()["overflow2"] == 0
     for (p, k) in set(zip(pools.tolist(), ks.tolist())):
         sel = np.where((pools == p) & (ks == k))[0]

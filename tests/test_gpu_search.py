@@ -237,12 +237,15 @@ def test_autotune_keeps_results(cuda):
     np.testing.assert_array_equal(gids, oids)
 
 
+@pytest.mark.parametrize("one_pass", [True, False])
 @pytest.mark.parametrize("tune", [False, True])
-def test_batch_heterogeneous_matches_oracle(cuda, tune):
-    """Per-query pool, extend_to and k in one call: every row equals the
-    oracle's search with that query's parameters."""
+def test_batch_heterogeneous_matches_oracle(cuda, tune, one_pass):
+    """Per-query pool, extend_to and k in one call, all classes in one pass
+    or one pass per class: every row equals the oracle's search with that
+    query's parameters."""
     base, q, oix = helpers.oracle_index("mixture", n=20_000, nq=200, centres=50)
     gix = _gpu_index(base, oix.trees(), oix.graph())
+    gix.setBatchForm(one_pass)
     epochs, trees = 4, 8
     gix.setSearchParams(epochs, 300, 50, trees, -1, 0)
     rng = np.random.default_rng(11)
@@ -261,13 +264,15 @@ def test_batch_heterogeneous_matches_oracle(cuda, tune):
         assert (ids[sel, k:] == -1).all()
 
 
-def test_batch_plan_graph_replays_match_eager(cuda):
+@pytest.mark.parametrize("one_pass", [True, False])
+def test_batch_plan_graph_replays_match_eager(cuda, one_pass):
     """The batch captured into a HIP graph: every replay, with eager searches
     and synchronisations in between and new queries written into the same
     buffer, equals the eager batch on those queries."""
     import torch
     base, q, oix = helpers.oracle_index("mixture", n=20_000, nq=200, centres=50)
     gix = _gpu_index(base, oix.trees(), oix.graph())
+    gix.setBatchForm(one_pass)
     gix.setSearchParams(4, 300, 50, 8, -1, 0)
     nq = q.shape[0]
     rng = np.random.default_rng(12)
