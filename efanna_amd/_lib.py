@@ -97,6 +97,7 @@ _SIGS = {
     "efg_get_learned_stats": (c_int, [c_vp, c_u64p, c_u64p, c_u64p]),
     "efg_get_learned_restarts": (c_int, [c_vp, c_u64p]),
     "efg_set_batch_form": (c_int, [c_vp, c_int]),
+    "efg_get_batch_shape": (c_int, [c_vp, c_intp, c_intp, c_intp, c_intp]),
 }
 
 _lib = None

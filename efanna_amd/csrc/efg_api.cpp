@@ -630,6 +630,15 @@ int efg_batch_plan_destroy(efg_batch_plan* plan) {
   return EFG_OK;
 }
 
+int efg_get_batch_shape(const efg_index* ix, int* split, int* t_small, int* t_large, int* share_pct) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (split) *split = ix->b_split;
+  if (t_small) *t_small = ix->b_tsmall;
+  if (t_large) *t_large = ix->b_tlarge;
+  if (share_pct) *share_pct = int(ix->b_share * 100.f + 0.5f);
+  return EFG_OK;
+}
+
 int efg_set_batch_form(efg_index* ix, int one_pass) {
   if (!ix) return fail(EFG_ERR_ARG, "null index");
   if (one_pass != 0 && one_pass != 1) return fail(EFG_ERR_ARG, "batch form must be 0 or 1");

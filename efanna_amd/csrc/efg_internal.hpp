@@ -276,6 +276,7 @@ struct efg_index {
   // per query, the rest at b_tlarge (b_split 0: one group)
   int b_mode = 0;
   int b_split = 0, b_tsmall = 128, b_tlarge = 256;
+  float b_share = 1.f;  // two groups: the small group's part of its resident workgroups
 };
 
 namespace efg {

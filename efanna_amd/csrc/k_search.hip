@@ -802,7 +802,18 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
   for (int w = wbase + wfirst; w < wend; w = kDyn ? wnext_s : w + nwg) {
     const int qi = qlist ? qlist[w] : w;
     if constexpr (kBatch) {  // this query's class: pool, extend_to, k, leaves, table size
-      c = la.ccfg[__builtin_amdgcn_readfirstlane(int(la.qcls[__builtin_amdgcn_readfirstlane(qi)]))];
+      // read through the constant address space: the configuration is
+      // uniform and read-only, so it lands in scalar registers as a kernel
+      // argument does
+      using ConstU8 = const __attribute__((address_space(4))) uint8_t;
+      using ConstU32 = const __attribute__((address_space(4))) uint32_t;
+      const int cls = __builtin_amdgcn_readfirstlane(int(((ConstU8*)la.qcls)[__builtin_amdgcn_readfirstlane(qi)]));
+      static_assert(sizeof(SearchCfg) % 4 == 0, "configuration in words");
+      uint32_t wcfg[sizeof(SearchCfg) / 4];
+      ConstU32* src = (ConstU32*)(la.ccfg + cls);
+#pragma unroll
+      for (int i = 0; i < int(sizeof(SearchCfg) / 4); ++i) wcfg[i] = src[i];
+      __builtin_memcpy(&c, wcfg, sizeof(SearchCfg));
       R = c.R;
       Rs = c.Rp;
       logcap = c.logcap;
@@ -2186,6 +2197,7 @@ struct FusedGroup {
   SearchCfg c0{};
   int spill_logcap = 0;
   int off = 0, count = 0;
+  float share = 1.f;  // the part of the resident workgroups its grid takes
 };
 
 // A heterogeneous batch after its classes are formed, their query lists are
@@ -2205,9 +2217,12 @@ struct BatchSetup {
 
 // Shape of the one-pass form: the classes with pools up to `split` form one
 // group at `t_small` threads per query, the others another at `t_large`
-// (split 0: all in one group at t_large).
+// (split 0: all in one group at t_large).  Two groups run at once, the small
+// one's grid taking `share` of its resident workgroups and the large one's
+// the rest of theirs, so that both persistent kernels are resident together.
 struct FusedShape {
   int split = 0, t_small = 128, t_large = 256;
+  float share = 0.5f;
 };
 
 SearchKernel fused_kernel(int tq) {
@@ -2256,7 +2271,7 @@ int fused_launch_group(efg_index* ix, efg::Workspace& w, efg::Workspace& wg, con
       resident > 0)
     per_cu = std::min(per_cu, resident);
   per_cu = std::max(1, std::min(per_cu, 32 / (tq / 64)));
-  size_t grid = size_t(ncu) * per_cu;
+  size_t grid = std::max<size_t>(8, size_t(double(size_t(ncu) * per_cu) * fg.share) & ~size_t(7));
   if (grid > size_t(fg.count)) grid = (size_t(fg.count) + 7) & ~size_t(7);
   if (!wg.spill.ensure(grid << fg.spill_logcap)) return fail(EFG_ERR_MEMORY, "out of device memory (spill tables)");
   LearnArgs la{};
@@ -2264,7 +2279,7 @@ int fused_launch_group(efg_index* ix, efg::Workspace& w, efg::Workspace& wg, con
   la.qcls = w.bcls.p;
   kern<<<unsigned(grid), unsigned(tq), lds, st>>>(c0, la, dq, q_stride, ix->data.p, ix->data_bf16.p, w.leaves.p,
                                                   ix->snode.p, ix->leaf.p, ix->goff.p, ix->gids.p, d_ids, d_dists,
-                                                  nullptr, wg.ndist.p, order, fg.count, nullptr, 1, nullptr,
+                                                  nullptr, wg.ndist.p, order, fg.count, nullptr, 0, nullptr,
                                                   wg.overflow.p, wg.nover.p, ix->work.p, wg.spill.p,
                                                   fg.spill_logcap, wg.nover.p + 4);
   k_add_over<<<1, 64, 0, st>>>(w.nover_sum.p, wg.nover.p);
@@ -2325,6 +2340,10 @@ int fused_setup(efg_index* ix, efg::Workspace& w, const std::vector<BatchClass>&
     fg.count = int(order.size()) - fg.off;
     counts.push_back(fg.count);
     b->groups.push_back(fg);
+  }
+  if (b->groups.size() == 2) {
+    b->groups[0].share = sh.share;
+    b->groups[1].share = 1.f - sh.share;
   }
   const size_t nq = b->nq;
   std::vector<uint8_t> qc(nq);
@@ -2416,17 +2435,17 @@ int batch_setup(efg_index* ix, efg::Workspace& w, const float* dq, size_t nq, si
     // one pass over all classes, in one group or two split by pool size:
     // the shape (split, threads per query of each group) tuned on this
     // batch (tune) or kept from the last tuning
-    FusedShape sh{ix->b_split, ix->b_tsmall, ix->b_tlarge};
+    FusedShape sh{ix->b_split, ix->b_tsmall, ix->b_tlarge, ix->b_share};
     if (tune) {
       std::vector<FusedShape> shapes;
       std::vector<int> pools;
       for (const auto& c : b->cfg) pools.push_back(c.init_num);
       std::sort(pools.begin(), pools.end());
       pools.erase(std::unique(pools.begin(), pools.end()), pools.end());
-      for (int tl : {128, 256}) shapes.push_back(FusedShape{0, 128, tl});
+      for (int tl : {128, 256}) shapes.push_back(FusedShape{0, 128, tl, 1.f});
       for (size_t i = 0; i + 1 < pools.size(); ++i)
         for (int ts : {64, 128})
-          for (int tl : {128, 256}) shapes.push_back(FusedShape{pools[i], ts, tl});
+          for (float sh2 : {0.35f, 0.5f, 0.65f}) shapes.push_back(FusedShape{pools[i], ts, 256, sh2});
       hipEvent_t e0 = nullptr, e1 = nullptr;
       hipEventCreate(&e0);
       hipEventCreate(&e1);
@@ -2460,11 +2479,12 @@ int batch_setup(efg_index* ix, efg::Workspace& w, const float* dq, size_t nq, si
         ix->b_split = sh.split;
         ix->b_tsmall = sh.t_small;
         ix->b_tlarge = sh.t_large;
+        ix->b_share = sh.share;
       }
     }
     if (rc == EFG_OK) {
       rc = fused_setup(ix, w, cls, sh, b, st);
-      if (rc != EFG_OK) rc = fused_setup(ix, w, cls, FusedShape{0, 128, 256}, b, st);
+      if (rc != EFG_OK) rc = fused_setup(ix, w, cls, FusedShape{0, 128, 256, 1.f}, b, st);
       if (rc != EFG_OK) {  // too large for one pass: one pass per class
         b->fused = false;
         rc = EFG_OK;

@@ -211,6 +211,13 @@ class FIndex:
         """Batched search: all classes in one pass, or one pass per class (default)."""
         check(lib().efg_set_batch_form(self.h, 1 if one_pass else 0))
 
+    def batch_shape(self):
+        """The one-pass form's shape: small group's largest pool (0: one group),
+        threads per query of the small and the large group, small group's share."""
+        v = [ctypes.c_int() for _ in range(4)]
+        check(lib().efg_get_batch_shape(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(split=v[0].value, t_small=v[1].value, t_large=v[2].value, share=v[3].value / 100.0)
+
     def knnSearchBatch(self, query, pool, k, extend=None, out_stride=None, tune=False):
         """Per-query pool (init_num), k and extend_to (default pool // 6); returns (ids, dists)
         with out_stride columns (default max k), -1 / FLT_MAX past each query's k."""

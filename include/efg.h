@@ -238,6 +238,11 @@ int efg_batch_plan_destroy(efg_batch_plan* plan);
  * (with tune != 0 the tuner picks the shape: one group, or two split by pool
  * size, and the threads per query of each).  Results are the same. */
 int efg_set_batch_form(efg_index* ix, int one_pass);
+/* The one-pass form's shape in use (no counterpart in the reference): the
+ * largest pool of the small group (0: one group), the threads per query of
+ * the small and the large group, and the small group's share of its
+ * resident workgroups in percent. */
+int efg_get_batch_shape(const efg_index* ix, int* split, int* t_small, int* t_large, int* share_pct);
 
 /* Search without device allocation (no counterpart in the reference).  For
  * callers that cannot allocate device memory: code under HIP graph capture,
