@@ -805,15 +805,21 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
       // read through the constant address space: the configuration is
       // uniform and read-only, so it lands in scalar registers as a kernel
       // argument does
+      // (only the fields that differ between classes: the others stay the
+      // kernel argument's, loop invariant)
       using ConstU8 = const __attribute__((address_space(4))) uint8_t;
-      using ConstU32 = const __attribute__((address_space(4))) uint32_t;
+      using ConstI32 = const __attribute__((address_space(4))) int32_t;
       const int cls = __builtin_amdgcn_readfirstlane(int(((ConstU8*)la.qcls)[__builtin_amdgcn_readfirstlane(qi)]));
-      static_assert(sizeof(SearchCfg) % 4 == 0, "configuration in words");
-      uint32_t wcfg[sizeof(SearchCfg) / 4];
-      ConstU32* src = (ConstU32*)(la.ccfg + cls);
-#pragma unroll
-      for (int i = 0; i < int(sizeof(SearchCfg) / 4); ++i) wcfg[i] = src[i];
-      __builtin_memcpy(&c, wcfg, sizeof(SearchCfg));
+      const SearchCfg* cc = la.ccfg + cls;
+      auto field = [&](const int* f) { return *(ConstI32*)f; };
+      c.lsize = field(&cc->lsize);
+      c.nleaf = field(&cc->nleaf);
+      c.init_num = field(&cc->init_num);
+      c.E = field(&cc->E);
+      c.K = field(&cc->K);
+      c.R = field(&cc->R);
+      c.Rp = field(&cc->Rp);
+      c.logcap = field(&cc->logcap);
       R = c.R;
       Rs = c.Rp;
       logcap = c.logcap;
