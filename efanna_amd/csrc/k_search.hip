@@ -386,7 +386,9 @@ __global__ void k_pack_bf16(const float* __restrict__ X, int D, size_t n, uint16
 }
 
 // Dynamic LDS layout for the search kernel; also used by the host to size it.
-__host__ __device__ inline size_t search_lds_bytes(const SearchCfg& c, bool global_table) {
+// LDS of one query's common state (the layout of Smem), before the arrays
+// of the learned and method 1 searches
+__host__ __device__ inline size_t search_lds_common(const SearchCfg& c, bool global_table) {
   size_t b = 0;
   auto a16 = [](size_t x) { return (x + 15) & ~size_t(15); };
   b += a16(sizeof(float) * c.D);
@@ -394,8 +396,14 @@ __host__ __device__ inline size_t search_lds_bytes(const SearchCfg& c, bool glob
   if (!global_table) b += a16(sizeof(uint32_t) << c.logcap);
   b += a16(sizeof(uint32_t) * ctab_slots(c.threads)) * 2;
   b += a16(sizeof(int32_t) * (c.nleaf + 1)) + a16(sizeof(int32_t) * c.nleaf);
-  b += a16(sizeof(uint32_t) * c.E) + a16(sizeof(int32_t) * (c.E + 1));
+  if (c.method != 1) b += a16(sizeof(uint32_t) * c.E) + a16(sizeof(int32_t) * (c.E + 1));  // method 0 epochs
   b += a16(sizeof(uint32_t) * 256) + a16(sizeof(int) * 4) + a16(sizeof(int) * kMiscInts);
+  return b;
+}
+
+__host__ __device__ inline size_t search_lds_bytes(const SearchCfg& c, bool global_table) {
+  size_t b = search_lds_common(c, global_table);
+  auto a16 = [](size_t x) { return (x + 15) & ~size_t(15); };
   if (c.stop) {
     b += a16(sizeof(int32_t) * c.K);            // top k ids of the last checkpoint
     b += a16(sizeof(float) * (kFeat + 2 * kHid + 8 + kHid + 1 + 2 * kFeat));  // features, hidden units,
@@ -610,8 +618,8 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
     s.ctab_pos = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * kCtab));
     s.lpos = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * (c.nleaf + 1)));
     s.lstart = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * c.nleaf));
-    s.explist = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * c.E));
-    s.expoff = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * (c.E + 1)));
+    s.explist = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * (kMethod == 1 ? 0 : c.E)));
+    s.expoff = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * (kMethod == 1 ? 0 : c.E + 1)));
     s.hist = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * 256));
     s.scan = reinterpret_cast<int*>(take(sizeof(int) * 4));
     s.misc = reinterpret_cast<int*>(take(sizeof(int) * kMiscInts));
@@ -650,9 +658,7 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
   uint64_t* eord = nullptr;
   uint64_t* nwl = nullptr;
   if (kMethod == 1) {
-    SearchCfg c0 = c;
-    c0.method = 0;
-    size_t o = search_lds_bytes(c0, kGlobal);
+    size_t o = search_lds_common(c, kGlobal);
     auto take = [&](size_t bytes) {
       unsigned char* p = smem_raw + o;
       o += align16(bytes);
@@ -1045,21 +1051,63 @@ k_search_nnexp(SearchCfg c, LearnArgs la, const float* __restrict__ Q,
           }
           wave_sync();
           int k = 0;
+          // on a graph whose rows all have one degree (at most 64), the
+          // neighbour ids of the next node to expand (the first flagged entry
+          // after this one as the list stands) are loaded, one per lane,
+          // while this node's distances are measured; pf_node is the node
+          // they belong to, so a list changed by the inserts only costs the
+          // load
+          const bool pf_on = c.deg > 0 && c.deg <= 64;
+          uint32_t pf_node = 0xffffffffu, pf_id = 0xffffffffu;
           while (k < L) {
+            // an entry without its flag only moves k on: go straight to the
+            // first flagged entry at or after k (64 entries per ballot)
+            if (!kf[k]) {
+              int nxt = L;
+              for (int b = k; b < L; b += 64) {
+                const int j = b + lane;
+                const uint64_t mk = __ballot(j < L && kf[j] != 0);
+                if (mk) {
+                  nxt = b + __ffsll((unsigned long long)mk) - 1;
+                  break;
+                }
+              }
+              k = nxt;
+              continue;
+            }
             int nk = L;
             if (kf[k]) {
               wave_sync();
               if (lane == 0) kf[k] = 0;
               const uint32_t nd = ki[k];
-              const uint64_t g0 = goff[nd], deg = goff[nd + 1] - g0;
+              uint64_t g0, deg;
+              if (c.deg > 0) {
+                g0 = uint64_t(nd) * uint64_t(c.deg);
+                deg = uint64_t(c.deg);
+              } else {
+                g0 = goff[nd];
+                deg = goff[nd + 1] - g0;
+              }
               const int maxM = int(deg < uint64_t(c.E) ? deg : uint64_t(c.E));
+              const bool have = pf_on && nd == pf_node;
+              const uint32_t id0 = have ? pf_id : 0xffffffffu;
+              if (pf_on) {
+                int nxt = -1;
+                for (int b = k + 1; b < L && nxt < 0; b += 64) {
+                  const int j = b + lane;
+                  const uint64_t mk = __ballot(j < L && kf[j] != 0);
+                  if (mk) nxt = b + __ffsll((unsigned long long)mk) - 1;
+                }
+                pf_node = nxt >= 0 ? ki[nxt] : 0xffffffffu;
+                pf_id = (nxt >= 0 && lane < maxM) ? gids[uint64_t(pf_node) * uint64_t(c.deg) + lane] : 0xffffffffu;
+              }
               for (int b = 0; b < maxM; b += 64) {
                 if (ins + 64 > limit) {
                   ovf = true;
                   break;
                 }
                 const int m = b + lane;
-                uint32_t id = m < maxM ? gids[g0 + m] : 0xffffffffu;
+                uint32_t id = m < maxM ? (have ? id0 : gids[g0 + m]) : 0xffffffffu;
                 // first occurrence in this slice and not flagged
                 bool dupl = false;
                 for (int o2 = 0; o2 < 64 && b + o2 < maxM; ++o2) {
@@ -1578,7 +1626,8 @@ namespace {
 // new-candidate buffer + merge output).
 void set_threads(SearchCfg& c, int t) {
   c.threads = t;
-  c.kb = std::max(c.init_num, 2 * c.Rp + new_buf(t));
+  // method 1 keeps the seeds and the sorted pool only (no merge buffers)
+  c.kb = c.method == 1 ? std::max(c.init_num, c.Rp) : std::max(c.init_num, 2 * c.Rp + new_buf(t));
 }
 
 // Search configuration from (k, init_num, extend_to, epochs, trees, method)

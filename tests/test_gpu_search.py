@@ -153,6 +153,9 @@ def test_exact_knn_float_data_matches_oracle(cuda):
     (3, 60, 30, 4, 10),     # epochs * extend >= pool: unsorted seeds, pool raised to 90
     (2, 200, 10, 8, 20),    # k > extend_to
     (5, 300, 40, 8, 5),
+    (4, 1200, 200, 8, 10),  # the 1M x 128 search parameters (lists in registers: k + extend_to + 1 <= 256)
+    (2, 600, 300, 8, 10),   # k + extend_to + 1 > 256: the knn list in LDS
+    (2, 400, 100, 8, 80),   # k + 1 > 64: the results list in LDS
 ])
 def test_search_kgraph_matches_oracle(cuda, epochs, pool, extend, trees, k):
     base, q, oix = helpers.oracle_index("uniform")
