@@ -40,6 +40,7 @@ _SIGS = {
     "efg_set_search_precision": (c_int, [c_vp, c_int]),
     "efg_set_build_precision": (c_int, [c_vp, c_int]),
     "efg_set_join_staging_margin": (c_int, [c_vp, ctypes.c_double]),
+    "efg_set_join_routing": (c_int, [c_vp, c_int]),
     "efg_hash_set_codes": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_u64p, c_u64p, c_sz]),
     "efg_hash_load_codes": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, ctypes.c_char_p, ctypes.c_char_p]),
     "efg_knn_search": (c_int, [c_vp, c_int, c_f32p, c_sz, c_sz]),

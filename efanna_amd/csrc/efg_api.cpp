@@ -600,6 +600,13 @@ int efg_set_join_staging_margin(efg_index* ix, double margin) {
   return EFG_OK;
 }
 
+int efg_set_join_routing(efg_index* ix, int direct) {
+  if (!ix) return fail(EFG_ERR_ARG, "null index");
+  if (direct != 0 && direct != 1) return fail(EFG_ERR_ARG, "join routing must be 0 (staged) or 1 (direct)");
+  ix->b_join_direct = direct != 0;
+  return EFG_OK;
+}
+
 int efg_knn_search_batch_device(efg_index* ix, const float* d_queries, size_t nq, size_t q_stride, const int* pool,
                                 const int* extend, const int* k, int out_stride, int32_t* d_ids, float* d_dists,
                                 int tune, void* stream) {

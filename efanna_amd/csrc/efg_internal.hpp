@@ -185,6 +185,7 @@ struct efg_index {
   int s_precision = 0;              // 0 FP32 rows, 1 BF16 rows + FP32 re-rank
   int b_precision = 0;              // build: 0 FP32 tiles, 1 BF16 / 2 FP16 MFMA tiles + FP32 re-rank
   double b_stage_margin = 1.25;     // join staging: margin over the last iteration's kept fraction
+  bool b_join_direct = true;        // join candidates straight into their targets' buckets when they fit
   efg::DevBuf<uint16_t> split_hi, split_lo;  // build precision form: rows as hi + lo 16-bit parts (n x Dk)
   efg::DevBuf<float> split_norm;             // FP32 squared norms of the rows
   int split_Dk = 0, split_kind = 0;

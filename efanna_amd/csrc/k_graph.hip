@@ -28,6 +28,7 @@
 //   above 100 keep the 100 ids with the smallest generator value.
 #include <algorithm>
 #include <chrono>
+#include <type_traits>
 #include <vector>
 
 #include "efg_device.hpp"
@@ -622,17 +623,57 @@ __global__ void k_worst(const uint64_t* __restrict__ pool, const uint32_t* __res
   worst[p] = fill[p] < uint32_t(L) ? kKeyMax : pool[size_t(p) * L + L - 1];
 }
 
-// upper bound of the candidates of one join: two per pair
-__global__ void k_pair_bound(uint32_t n, const uint64_t* __restrict__ new_off, const uint64_t* __restrict__ old_off,
-                             unsigned long long* __restrict__ out) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long v = 0;
-  if (p < n) {
-    unsigned long long na = new_off[p + 1] - new_off[p], nb = old_off[p + 1] - old_off[p];
-    v = na * (na > 0 ? na - 1 : 0) / 2 + na * nb;
+// Bucket room for direct routing: the pairs each row is in, an upper bound
+// of the candidates it can receive (one per pair).  A node's new row pairs
+// with its other m - 1 rows, an old row with its na new ones; nodes without
+// new rows join nothing.  A workgroup takes kRoomNodes consecutive nodes of
+// `order` (tree 0's leaf list, or null: node order), sixteen lanes per node;
+// nodes of one leaf share most of their list rows, so the room is summed per
+// row in an LDS hash first (about a quarter as many global atomics, which
+// are not waited for).  A row that finds no slot within kRoomProbe probes
+// goes to the global sum directly.
+constexpr int kRoomNodes = 256, kRoomSlots = 4096, kRoomProbe = 32;
+constexpr uint32_t kRoomFree = 0xffffffffu;  // an empty LDS slot
+__global__ void __launch_bounds__(kBlock) k_bucket_room(uint32_t n, const uint32_t* __restrict__ order,
+                                                        const uint64_t* __restrict__ new_off,
+                                                        const uint32_t* __restrict__ new_ids,
+                                                        const uint64_t* __restrict__ old_off,
+                                                        const uint32_t* __restrict__ old_ids,
+                                                        uint32_t* __restrict__ room) {
+  __shared__ uint32_t key_s[kRoomSlots], val_s[kRoomSlots];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kRoomSlots; i += kBlock) {
+    key_s[i] = kRoomFree;
+    val_s[i] = 0u;
   }
-  for (int o = 32; o > 0; o >>= 1) v += (unsigned long long)__shfl_down((long long)v, o);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+  __syncthreads();
+  auto add = [&](uint32_t t, uint32_t w) {
+    uint32_t h = (t * 2654435761u) >> 20;  // 12 bits: kRoomSlots
+    for (int k = 0; k < kRoomProbe; ++k, h = (h + 1) & uint32_t(kRoomSlots - 1)) {
+      const uint32_t prev = atomicCAS(&key_s[h], kRoomFree, t);
+      if (prev == kRoomFree || prev == t) {
+        atomicAdd(&val_s[h], w);
+        return;
+      }
+    }
+    atomicAdd(&room[t], w);
+  };
+  const uint32_t i0 = blockIdx.x * uint32_t(kRoomNodes);
+  const int l = tid & 15;
+  for (int j = tid >> 4; j < kRoomNodes; j += kBlock / 16) {
+    const uint32_t i = i0 + uint32_t(j);
+    if (i >= n) break;
+    const uint32_t u = order ? order[i] : i;
+    const uint64_t a0 = new_off[u], b0 = old_off[u];
+    const uint32_t na = uint32_t(new_off[u + 1] - a0), nb = uint32_t(old_off[u + 1] - b0);
+    if (na == 0) continue;
+    if (na + nb > 1)
+      for (uint32_t e = uint32_t(l); e < na; e += 16) add(new_ids[a0 + e], na + nb - 1);
+    for (uint32_t e = uint32_t(l); e < nb; e += 16) add(old_ids[b0 + e], na);
+  }
+  __syncthreads();
+  for (int i = tid; i < kRoomSlots; i += kBlock)
+    if (key_s[i] != kRoomFree) atomicAdd(&room[key_s[i]], val_s[i]);
 }
 
 // A kept candidate staged for its target's bucket.
@@ -660,54 +701,88 @@ __device__ inline uint32_t join_task_count(uint32_t na, uint32_t nb) {
 // (order[i] = node at position i; node i when order is null), so that nodes of one leaf, whose lists
 // share rows, get neighbouring task ranges and run at about the same time on
 // one XCD; nodes with at most node_rows rows get none (the whole-node
-// kernel joins them); *rows accumulates the rows joined, sum of |new| + |old|
-// over the nodes with a non-empty new list (one atomic per wavefront).
-__global__ void k_join_count(uint32_t n, const uint32_t* __restrict__ order, const uint64_t* __restrict__ new_off,
-                             const uint64_t* __restrict__ old_off, uint32_t node_rows, int32_t* __restrict__ cnt,
-                             unsigned long long* __restrict__ rows) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long r = 0;
-  if (i < n) {
+// kernel joins them).  sums[0] accumulates the pairs (na (na - 1) / 2 +
+// na nb per node, two candidates each at most), sums[2] the rows joined
+// (|new| + |old| over the nodes with a non-empty new list): a grid-stride
+// loop, one atomic per workgroup and sum (atomics on one address from every
+// wavefront of a million-thread grid serialise).
+__global__ void __launch_bounds__(kBlock) k_join_count(uint32_t n, const uint32_t* __restrict__ order,
+                                                       const uint64_t* __restrict__ new_off,
+                                                       const uint64_t* __restrict__ old_off, uint32_t node_rows,
+                                                       int32_t* __restrict__ cnt,
+                                                       unsigned long long* __restrict__ sums) {
+  __shared__ unsigned long long red_s[2][kBlock / 64];
+  unsigned long long r = 0, v = 0;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const uint32_t p = order ? order[i] : i;
     const uint32_t na = uint32_t(new_off[p + 1] - new_off[p]), nb = uint32_t(old_off[p + 1] - old_off[p]);
     cnt[i] = na + nb <= node_rows ? 0 : int32_t(join_task_count(na, nb));
-    r = na ? na + nb : 0;
+    r += na ? na + nb : 0;
+    v += (unsigned long long)na * (na > 0 ? na - 1 : 0) / 2 + (unsigned long long)na * nb;
   }
-  for (int o = 32; o > 0; o >>= 1) r += (unsigned long long)__shfl_down((long long)r, o);
-  if ((threadIdx.x & 63) == 0 && r) atomicAdd(rows, r);
+  for (int o = 32; o > 0; o >>= 1) {
+    r += (unsigned long long)__shfl_down((long long)r, o);
+    v += (unsigned long long)__shfl_down((long long)v, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red_s[0][threadIdx.x >> 6] = v;
+    red_s[1][threadIdx.x >> 6] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long sv = 0, sr = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      sv += red_s[0][w];
+      sr += red_s[1][w];
+    }
+    if (sv) atomicAdd(sums, sv);
+    if (sr) atomicAdd(sums + 2, sr);
+  }
 }
 
-// Task records, one wavefront per node (position i of the order): header
-// (node, sizeA | newA << 8 | sizeB << 16 | newB << 24, I == J) and the rows'
-// ids, slots [0, 32) block I, [32, 64) block J (unused when I == J), kNoRow
-// past the blocks' ends.  Nodes of at most node_rows rows have no tasks
-// (k_join_count gave them none).
-__global__ void k_join_tasks(uint32_t n, const uint32_t* __restrict__ order, const uint64_t* __restrict__ new_off,
-                             const uint32_t* __restrict__ new_ids, const uint64_t* __restrict__ old_off,
-                             const uint32_t* __restrict__ old_ids, uint32_t node_rows,
-                             const int32_t* __restrict__ toff, uint4* __restrict__ hdr, uint32_t* __restrict__ tids) {
-  const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+// Task records of the nodes with tasks (position i of the order, tcnt[i] >
+// 0: at least one new row and more than node_rows rows), one wavefront per
+// node: header (node, sizeA | newA << 8 | sizeB << 16 | newB << 24, I == J)
+// and the rows' ids, slots [0, 32) block I, [32, 64) block J (unused when
+// I == J), kNoRow past the blocks' ends.  Each wavefront reads the task
+// counts of 64 positions at a time and takes the nodes that have tasks from
+// a ballot (few nodes have them).
+__global__ void __launch_bounds__(kBlock) k_join_tasks(uint32_t n, const uint32_t* __restrict__ order,
+                                                       const uint64_t* __restrict__ new_off,
+                                                       const uint32_t* __restrict__ new_ids,
+                                                       const uint64_t* __restrict__ old_off,
+                                                       const uint32_t* __restrict__ old_ids,
+                                                       const int32_t* __restrict__ tcnt,
+                                                       const int32_t* __restrict__ toff, uint4* __restrict__ hdr,
+                                                       uint32_t* __restrict__ tids) {
   const int lane = threadIdx.x & 63;
-  if (i >= n) return;
-  const uint32_t p = order ? order[i] : i;
-  const uint64_t a0 = new_off[p], b0 = old_off[p];
-  const int na = int(new_off[p + 1] - a0), m = na + int(old_off[p + 1] - b0);
-  if (na == 0 || uint32_t(m) <= node_rows) return;
-  const int nI = (na + kJBR - 1) / kJBR, nJ = (m + kJBR - 1) / kJBR;
-  size_t t = size_t(toff[i]);
-  for (int I = 0; I < nI; ++I)
-    for (int J = I; J < nJ; ++J, ++t) {
-      const int sizeA = min(kJBR, m - I * kJBR), newA = min(kJBR, na - I * kJBR);
-      const int sizeB = min(kJBR, m - J * kJBR), newB = max(0, min(sizeB, na - J * kJBR));
-      const int side = lane >> 5, r = lane & 31;
-      const int size = side ? sizeB : sizeA, pos = (side ? J : I) * kJBR + r;
-      uint32_t id = kNoRow;
-      if (r < size && !(side && I == J)) id = pos < na ? new_ids[a0 + pos] : old_ids[b0 + pos - na];
-      tids[t * 64 + lane] = id;
-      if (lane == 0)
-        hdr[t] = make_uint4(p, uint32_t(sizeA) | uint32_t(newA) << 8 | uint32_t(sizeB) << 16 | uint32_t(newB) << 24,
-                            I == J ? 1u : 0u, 0u);
+  const uint32_t w0 = (blockIdx.x * kBlock + threadIdx.x) >> 6, nw = gridDim.x * (kBlock / 64);
+  for (uint32_t c0 = w0 * 64; c0 < n; c0 += nw * 64) {
+    uint64_t todo = __ballot(c0 + lane < n && tcnt[c0 + lane] > 0);
+    while (todo) {
+      const uint32_t i = c0 + uint32_t(__ffsll((long long)todo) - 1);
+      todo &= todo - 1;
+      const uint32_t p = order ? order[i] : i;
+      const uint64_t a0 = new_off[p], b0 = old_off[p];
+      const int na = int(new_off[p + 1] - a0), m = na + int(old_off[p + 1] - b0);
+      const int nI = (na + kJBR - 1) / kJBR, nJ = (m + kJBR - 1) / kJBR;
+      size_t t = size_t(toff[i]);
+      for (int I = 0; I < nI; ++I)
+        for (int J = I; J < nJ; ++J, ++t) {
+          const int sizeA = min(kJBR, m - I * kJBR), newA = min(kJBR, na - I * kJBR);
+          const int sizeB = min(kJBR, m - J * kJBR), newB = max(0, min(sizeB, na - J * kJBR));
+          const int side = lane >> 5, r = lane & 31;
+          const int size = side ? sizeB : sizeA, pos = (side ? J : I) * kJBR + r;
+          uint32_t id = kNoRow;
+          if (r < size && !(side && I == J)) id = pos < na ? new_ids[a0 + pos] : old_ids[b0 + pos - na];
+          tids[t * 64 + lane] = id;
+          if (lane == 0)
+            hdr[t] = make_uint4(p,
+                                uint32_t(sizeA) | uint32_t(newA) << 8 | uint32_t(sizeB) << 16 | uint32_t(newB) << 24,
+                                I == J ? 1u : 0u, 0u);
+        }
     }
+  }
 }
 
 // Pair k of a task -> positions (p, q) in blocks I and J.  I != J: all
@@ -830,12 +905,50 @@ constexpr unsigned long long kStageRun = 4096;
 // targets' totals with one atomic per (task, target row) whose result is not
 // waited for, and the candidates to the workgroup's staging run.
 constexpr int kJPairs = kJBR * kJBR / kBlock;  // pairs per thread at most
-template <int kJPF>
+
+// Direct routing (kDirect): instead of staging, a row's kept candidates take
+// consecutive slots of its target's bucket, which starts at boff[target]
+// (room for every pair the target is in); the row's first slot is the
+// bucket's fill before it, an atomic on tcount[target].
+__device__ __forceinline__ void put_direct(uint64_t* __restrict__ bkey, uint32_t* __restrict__ bsrc, uint32_t o,
+                                           uint64_t key, uint32_t src) {
+  bkey[o] = key;
+  bsrc[o] = src;
+}
+// Pairs measured, and with direct routing the candidates kept (the staging
+// counter is free then): one atomic per workgroup each, at its end.
+template <bool kDirect>
+__device__ __forceinline__ void join_totals(unsigned long long pairs, unsigned long long kept,
+                                            unsigned long long* __restrict__ npairs,
+                                            unsigned long long* __restrict__ nkept) {
+  __shared__ unsigned long long red_s[2][kBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    pairs += (unsigned long long)__shfl_down((long long)pairs, o);
+    kept += (unsigned long long)__shfl_down((long long)kept, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red_s[0][threadIdx.x >> 6] = pairs;
+    red_s[1][threadIdx.x >> 6] = kept;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long sp = 0, sk = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      sp += red_s[0][w];
+      sk += red_s[1][w];
+    }
+    if (sp) atomicAdd(npairs, sp);
+    if (kDirect && sk) atomicAdd(nkept, sk);
+  }
+}
+
+template <int kJPF, bool kDirect>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_join_tiles(const float* __restrict__ X, int D, uint32_t ntask, const uint4* __restrict__ hdr,
              const uint32_t* __restrict__ tids, const uint64_t* __restrict__ worst, Stage* __restrict__ stage,
              unsigned long long cap, unsigned long long* __restrict__ nstage, uint32_t* __restrict__ tcount,
-             unsigned long long* __restrict__ npairs) {
+             unsigned long long* __restrict__ npairs, const int32_t* __restrict__ boff,
+             uint64_t* __restrict__ bkey, uint32_t* __restrict__ bsrc) {
   constexpr int kDC = kJPF * 16;  // columns per chunk at most
   extern __shared__ __align__(16) float4 rows_s[];  // [64][(DC + 4) / 4]
   __shared__ uint32_t ids_s[2][64];
@@ -849,7 +962,7 @@ k_join_tiles(const float* __restrict__ X, int D, uint32_t ntask, const uint4* __
   const int nch = (D + DC - 1) / DC;
   const uint32_t xcd = blockIdx.x & 7u, ns = gridDim.x >> 3;
   const uint32_t lo = uint32_t(uint64_t(ntask) * xcd / 8), hi = uint32_t(uint64_t(ntask) * (xcd + 1) / 8);
-  unsigned long long pairs_local = 0;
+  unsigned long long pairs_local = 0, kept_local = 0;
   unsigned long long wbase = 0, wend = 0;  // staging run (wave 0, lane 63)
   uint32_t t = lo + blockIdx.x / 8;
   // prologue: ids of the first task in LDS, of the second in registers
@@ -966,7 +1079,12 @@ k_join_tiles(const float* __restrict__ X, int D, uint32_t ntask, const uint4* __
         }
       }
       __syncthreads();
-      if (tid < 64) {
+      if (tid < 64 && kDirect) {
+        // the row's first slot in its target's bucket
+        const uint32_t cnt = cnt_s[tid];
+        off_s[tid] = cnt ? uint32_t(boff[ids[tid]]) + atomicAdd(&tcount[ids[tid]], cnt) : 0u;
+        kept_local += cnt;
+      } else if (tid < 64) {
         // totals per target (result not waited for), offsets within the task
         const uint32_t cnt = cnt_s[tid];
         if (cnt) atomicAdd(&tcount[ids[tid]], cnt);
@@ -1003,6 +1121,11 @@ k_join_tiles(const float* __restrict__ X, int D, uint32_t ntask, const uint4* __
         uint32_t a, b;
         join_ends(tid + j * kBlock, same, sizeA, newA, sizeB, newB, ids, sa, sb, a, b);
         const uint32_t la = li[j] & 0xffffu, lb = li[j] >> 16;
+        if (kDirect) {
+          if (la != 0xffffu) put_direct(bkey, bsrc, off_s[sa] + la, make_key(dist[j], b), nn | kFirst);
+          if (lb != 0xffffu) put_direct(bkey, bsrc, off_s[sb] + lb, make_key(dist[j], a), nn);
+          continue;
+        }
         if (la != 0xffffu) {
           const unsigned long long i = off_s[sa] + la, at = i < rem ? r0 + i : nr + (i - rem);
           if (at < cap) stage[at] = Stage{a, nn | kFirst, make_key(dist[j], b)};
@@ -1019,13 +1142,12 @@ k_join_tiles(const float* __restrict__ X, int D, uint32_t ntask, const uint4* __
     c = cn;
   }
   // unused rest of the staging run
-  if (tid < 64) {
+  if (tid < 64 && !kDirect) {
     const unsigned long long from = (unsigned long long)__shfl((long long)wbase, 63),
                              to = (unsigned long long)__shfl((long long)wend, 63);
     for (unsigned long long i = from + lane; i < to && i < cap; i += 64) stage[i].target = kNoRow;
   }
-  for (int o = 32; o > 0; o >>= 1) pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
-  if (lane == 0 && pairs_local) atomicAdd(npairs, pairs_local);
+  join_totals<kDirect>(pairs_local, kept_local, npairs, nstage);
 }
 
 // Whole-node join in FP32 for the nodes with at most 32 rows: one wavefront
@@ -1109,13 +1231,15 @@ __device__ __forceinline__ void node_pairs4(const float* __restrict__ X, int D, 
     out[j] = sdist;
   }
 }
+template <bool kDirect>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
 k_join_node_f32(const float* __restrict__ X, int D, uint32_t n, const uint32_t* __restrict__ order,
                 const uint64_t* __restrict__ new_off, const uint32_t* __restrict__ new_ids,
                 const uint64_t* __restrict__ old_off, const uint32_t* __restrict__ old_ids,
                 const uint64_t* __restrict__ worst, Stage* __restrict__ stage, unsigned long long cap,
                 unsigned long long* __restrict__ nstage, uint32_t* __restrict__ tcount,
-                unsigned long long* __restrict__ npairs) {
+                unsigned long long* __restrict__ npairs, const int32_t* __restrict__ boff,
+                uint64_t* __restrict__ bkey, uint32_t* __restrict__ bsrc) {
   constexpr int kW = kBlock / 64;
   __shared__ __align__(16) float rows_s[kW][kNodeRows * kNodeRS];
   __shared__ uint32_t ids_s[kW][64];
@@ -1125,7 +1249,7 @@ k_join_node_f32(const float* __restrict__ X, int D, uint32_t n, const uint32_t* 
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint32_t xcd = blockIdx.x & 7u, nsl = (gridDim.x >> 3) * kW;
   const uint32_t lo = uint32_t(uint64_t(n) * xcd / 8), hi = uint32_t(uint64_t(n) * (xcd + 1) / 8);
-  unsigned long long pairs_local = 0;
+  unsigned long long pairs_local = 0, kept_local = 0;
   unsigned long long wbase = 0, wend = 0;  // staging run
   uint32_t* ids = ids_s[wv];
   float* rows = rows_s[wv];
@@ -1182,29 +1306,34 @@ k_join_node_f32(const float* __restrict__ X, int D, uint32_t n, const uint32_t* 
         }
       }
       wave_sync();
-      // totals per target (result not waited for), offsets within the node
       const uint32_t cnt = cnt_s[wv][lane];
-      if (cnt) atomicAdd(&tcount[id], cnt);
-      uint32_t x = cnt;
-      for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = uint32_t(__shfl_up(int(x), o));
-        if (lane >= o) x += y;
-      }
-      off_s[wv][lane] = x - cnt;
-      const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
-      // this node's slots: the rest of the run, then a new run if needed
-      const unsigned long long rem = wend - wbase;
-      unsigned long long nr = 0;
-      if (tot > rem) {
-        if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
-        nr = (unsigned long long)__shfl((long long)nr, 0);
-      }
-      const unsigned long long r0 = wbase;
-      if (tot > rem) {
-        wbase = nr + (tot - rem);
-        wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
+      unsigned long long rem = 0, nr = 0, r0 = 0;
+      if constexpr (kDirect) {
+        off_s[wv][lane] = cnt ? uint32_t(boff[id]) + atomicAdd(&tcount[id], cnt) : 0u;
+        kept_local += cnt;
       } else {
-        wbase += tot;
+        // totals per target (result not waited for), offsets within the node
+        if (cnt) atomicAdd(&tcount[id], cnt);
+        uint32_t x = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = uint32_t(__shfl_up(int(x), o));
+          if (lane >= o) x += y;
+        }
+        off_s[wv][lane] = x - cnt;
+        const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
+        // this node's slots: the rest of the run, then a new run if needed
+        rem = wend - wbase;
+        if (tot > rem) {
+          if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
+          nr = (unsigned long long)__shfl((long long)nr, 0);
+        }
+        r0 = wbase;
+        if (tot > rem) {
+          wbase = nr + (tot - rem);
+          wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
+        } else {
+          wbase += tot;
+        }
       }
       wave_sync();
 #pragma unroll
@@ -1215,20 +1344,28 @@ k_join_node_f32(const float* __restrict__ X, int D, uint32_t n, const uint32_t* 
         node_ends(pq[j] & 0xff, pq[j] >> 8, na, m, ids, sa, sb, xa, xb);
         const uint32_t la = li[j] & 0xffffu, lb = li[j] >> 16;
         if (la != 0xffffu) {
-          const unsigned long long q = off_s[wv][sa] + la, at = q < rem ? r0 + q : nr + (q - rem);
-          if (at < cap) stage[at] = Stage{xa, nd | kFirst, make_key(dist[j], xb)};
+          if constexpr (kDirect) {
+            put_direct(bkey, bsrc, off_s[wv][sa] + la, make_key(dist[j], xb), nd | kFirst);
+          } else {
+            const unsigned long long q = off_s[wv][sa] + la, at = q < rem ? r0 + q : nr + (q - rem);
+            if (at < cap) stage[at] = Stage{xa, nd | kFirst, make_key(dist[j], xb)};
+          }
         }
         if (lb != 0xffffu) {
-          const unsigned long long q = off_s[wv][sb] + lb, at = q < rem ? r0 + q : nr + (q - rem);
-          if (at < cap) stage[at] = Stage{xb, nd, make_key(dist[j], xa)};
+          if constexpr (kDirect) {
+            put_direct(bkey, bsrc, off_s[wv][sb] + lb, make_key(dist[j], xa), nd);
+          } else {
+            const unsigned long long q = off_s[wv][sb] + lb, at = q < rem ? r0 + q : nr + (q - rem);
+            if (at < cap) stage[at] = Stage{xb, nd, make_key(dist[j], xa)};
+          }
         }
       }
       wave_sync();
     }
   }
-  for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
-  for (int o = 32; o > 0; o >>= 1) pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
-  if (lane == 0 && pairs_local) atomicAdd(npairs, pairs_local);
+  if (!kDirect)
+    for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
+  join_totals<kDirect>(pairs_local, kept_local, npairs, nstage);
 }
 
 // Staged candidates into their targets' buckets.  Consecutive slots with one
@@ -1461,7 +1598,7 @@ __global__ void __launch_bounds__(kBlock) k_leaf_mfma(const uint16_t* __restrict
 // k_join_tasks), the 32 x 32 block of dot products in one accumulator; then
 // the candidates as in k_join_tiles, counted per target row in the
 // wavefront's LDS slice, staged into per-wavefront runs.
-template <bool kF16, bool kLo>
+template <bool kF16, bool kLo, bool kDirect>
 __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict__ xhi, const uint16_t* __restrict__ xlo,
                                                       const float* __restrict__ xn, int Dk, uint32_t ntask,
                                                       const uint4* __restrict__ hdr,
@@ -1469,7 +1606,9 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
                                                       const uint64_t* __restrict__ worst, Stage* __restrict__ stage,
                                                       unsigned long long cap, unsigned long long* __restrict__ nstage,
                                                       uint32_t* __restrict__ tcount,
-                                                      unsigned long long* __restrict__ npairs) {
+                                                      unsigned long long* __restrict__ npairs,
+                                                      const int32_t* __restrict__ boff,
+                                                      uint64_t* __restrict__ bkey, uint32_t* __restrict__ bsrc) {
   constexpr int kW = kBlock / 64;
   __shared__ uint32_t ids_s[kW][64];
   __shared__ uint64_t w_s[kW][64];
@@ -1479,7 +1618,7 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
   const int r32 = lane & 31;
   const uint32_t xcd = blockIdx.x & 7u, nsl = (gridDim.x >> 3) * kW;
   const uint32_t lo = uint32_t(uint64_t(ntask) * xcd / 8), hi = uint32_t(uint64_t(ntask) * (xcd + 1) / 8);
-  unsigned long long pairs_local = 0;
+  unsigned long long pairs_local = 0, kept_local = 0;
   unsigned long long wbase = 0, wend = 0;  // staging run (lane 63)
   uint32_t* ids = ids_s[wv];
   for (uint32_t t = lo + (blockIdx.x >> 3) * kW + wv; t < hi; t += nsl) {
@@ -1517,29 +1656,34 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
       }
     }
     wave_sync();
-    // totals per target (result not waited for), offsets within the task
     const uint32_t cnt = cnt_s[wv][lane];
-    if (cnt) atomicAdd(&tcount[id], cnt);
-    uint32_t x = cnt;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = uint32_t(__shfl_up(int(x), o));
-      if (lane >= o) x += y;
-    }
-    off_s[wv][lane] = x - cnt;
-    const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
-    // this task's slots: the rest of the run, then a new run if needed
-    const unsigned long long rem = wend - wbase;
-    unsigned long long nr = 0;
-    if (tot > rem) {
-      if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
-      nr = (unsigned long long)__shfl((long long)nr, 0);
-    }
-    const unsigned long long r0 = wbase;
-    if (tot > rem) {
-      wbase = nr + (tot - rem);
-      wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
+    unsigned long long rem = 0, nr = 0, r0 = 0;
+    if constexpr (kDirect) {
+      off_s[wv][lane] = cnt ? uint32_t(boff[id]) + atomicAdd(&tcount[id], cnt) : 0u;
+      kept_local += cnt;
     } else {
-      wbase += tot;
+      // totals per target (result not waited for), offsets within the task
+      if (cnt) atomicAdd(&tcount[id], cnt);
+      uint32_t x = cnt;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = uint32_t(__shfl_up(int(x), o));
+        if (lane >= o) x += y;
+      }
+      off_s[wv][lane] = x - cnt;
+      const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
+      // this task's slots: the rest of the run, then a new run if needed
+      rem = wend - wbase;
+      if (tot > rem) {
+        if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
+        nr = (unsigned long long)__shfl((long long)nr, 0);
+      }
+      r0 = wbase;
+      if (tot > rem) {
+        wbase = nr + (tot - rem);
+        wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
+      } else {
+        wbase += tot;
+      }
     }
     wave_sync();
     const uint32_t nn = h.x;
@@ -1552,19 +1696,27 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
       block_ends(p, q, same, sizeA, newA, sizeB, newB, ids, sa, sb, a, b);
       const uint32_t la = li[i] & 0xffffu, lb = li[i] >> 16;
       if (la != 0xffffu) {
-        const unsigned long long j = off_s[wv][sa] + la, at = j < rem ? r0 + j : nr + (j - rem);
-        if (at < cap) stage[at] = Stage{a, nn | kFirst, make_key(dist[i], b)};
+        if constexpr (kDirect) {
+          put_direct(bkey, bsrc, off_s[wv][sa] + la, make_key(dist[i], b), nn | kFirst);
+        } else {
+          const unsigned long long j = off_s[wv][sa] + la, at = j < rem ? r0 + j : nr + (j - rem);
+          if (at < cap) stage[at] = Stage{a, nn | kFirst, make_key(dist[i], b)};
+        }
       }
       if (lb != 0xffffu) {
-        const unsigned long long j = off_s[wv][sb] + lb, at = j < rem ? r0 + j : nr + (j - rem);
-        if (at < cap) stage[at] = Stage{b, nn, make_key(dist[i], a)};
+        if constexpr (kDirect) {
+          put_direct(bkey, bsrc, off_s[wv][sb] + lb, make_key(dist[i], a), nn);
+        } else {
+          const unsigned long long j = off_s[wv][sb] + lb, at = j < rem ? r0 + j : nr + (j - rem);
+          if (at < cap) stage[at] = Stage{b, nn, make_key(dist[i], a)};
+        }
       }
     }
     wave_sync();
   }
-  for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
-  for (int o = 32; o > 0; o >>= 1) pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
-  if (lane == 0 && pairs_local) atomicAdd(npairs, pairs_local);
+  if (!kDirect)
+    for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
+  join_totals<kDirect>(pairs_local, kept_local, npairs, nstage);
 }
 
 // Three 32 x 32 blocks of split dot products of one node's rows, loaded once:
@@ -1641,7 +1793,7 @@ __device__ __forceinline__ void mfma_split3(const uint16_t* __restrict__ hi, con
 // taken in the order of tree 0's leaf list (order, null: node order), one
 // contiguous range per XCD.  Candidates are counted, offset and staged as in
 // k_join_mfma.
-template <bool kF16, bool kLo, int kNT>
+template <bool kF16, bool kLo, int kNT, bool kDirect>
 __global__ void __launch_bounds__(kBlock, kNT == 1 ? 4 : 3) k_join_node_mfma(const uint16_t* __restrict__ xhi,
                                                            const uint16_t* __restrict__ xlo,
                                                            const float* __restrict__ xn, int Dk, uint32_t n,
@@ -1654,7 +1806,10 @@ __global__ void __launch_bounds__(kBlock, kNT == 1 ? 4 : 3) k_join_node_mfma(con
                                                            unsigned long long cap,
                                                            unsigned long long* __restrict__ nstage,
                                                            uint32_t* __restrict__ tcount,
-                                                           unsigned long long* __restrict__ npairs) {
+                                                           unsigned long long* __restrict__ npairs,
+                                                           const int32_t* __restrict__ boff,
+                                                           uint64_t* __restrict__ bkey,
+                                                           uint32_t* __restrict__ bsrc) {
   constexpr int kW = kBlock / 64;
   __shared__ uint32_t ids_s[kW][64];
   __shared__ uint64_t w_s[kW][64];
@@ -1664,7 +1819,7 @@ __global__ void __launch_bounds__(kBlock, kNT == 1 ? 4 : 3) k_join_node_mfma(con
   const int r32 = lane & 31;
   const uint32_t xcd = blockIdx.x & 7u, nsl = (gridDim.x >> 3) * kW;
   const uint32_t lo = uint32_t(uint64_t(n) * xcd / 8), hi = uint32_t(uint64_t(n) * (xcd + 1) / 8);
-  unsigned long long pairs_local = 0;
+  unsigned long long pairs_local = 0, kept_local = 0;
   unsigned long long wbase = 0, wend = 0;  // staging run
   uint32_t* ids = ids_s[wv];
   // the wavefront's nodes i0, i0 + nsl, ... are read 64 at a time, one per
@@ -1729,29 +1884,34 @@ __global__ void __launch_bounds__(kBlock, kNT == 1 ? 4 : 3) k_join_node_mfma(con
         }
       }
       wave_sync();
-      // totals per target (result not waited for), offsets within the node
       const uint32_t cnt = cnt_s[wv][lane];
-      if (cnt) atomicAdd(&tcount[id], cnt);
-      uint32_t x = cnt;
-      for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = uint32_t(__shfl_up(int(x), o));
-        if (lane >= o) x += y;
-      }
-      off_s[wv][lane] = x - cnt;
-      const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
-      // this node's slots: the rest of the run, then a new run if needed
-      const unsigned long long rem = wend - wbase;
-      unsigned long long nr = 0;
-      if (tot > rem) {
-        if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
-        nr = (unsigned long long)__shfl((long long)nr, 0);
-      }
-      const unsigned long long r0 = wbase;
-      if (tot > rem) {
-        wbase = nr + (tot - rem);
-        wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
+      unsigned long long rem = 0, nr = 0, r0 = 0;
+      if constexpr (kDirect) {
+        off_s[wv][lane] = cnt ? uint32_t(boff[id]) + atomicAdd(&tcount[id], cnt) : 0u;
+        kept_local += cnt;
       } else {
-        wbase += tot;
+        // totals per target (result not waited for), offsets within the node
+        if (cnt) atomicAdd(&tcount[id], cnt);
+        uint32_t x = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = uint32_t(__shfl_up(int(x), o));
+          if (lane >= o) x += y;
+        }
+        off_s[wv][lane] = x - cnt;
+        const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
+        // this node's slots: the rest of the run, then a new run if needed
+        rem = wend - wbase;
+        if (tot > rem) {
+          if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
+          nr = (unsigned long long)__shfl((long long)nr, 0);
+        }
+        r0 = wbase;
+        if (tot > rem) {
+          wbase = nr + (tot - rem);
+          wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
+        } else {
+          wbase += tot;
+        }
       }
       wave_sync();
 #pragma unroll
@@ -1771,21 +1931,29 @@ __global__ void __launch_bounds__(kBlock, kNT == 1 ? 4 : 3) k_join_node_mfma(con
           node_ends(pb + p, qb + r32, na, m, ids, sa, sb, xa, xb);
           const uint32_t la = v & 0xffu, lb = v >> 8;
           if (la != 0xffu) {
-            const unsigned long long j = off_s[wv][sa] + la, at = j < rem ? r0 + j : nr + (j - rem);
-            if (at < cap) stage[at] = Stage{xa, nd | kFirst, make_key(dist, xb)};
+            if constexpr (kDirect) {
+              put_direct(bkey, bsrc, off_s[wv][sa] + la, make_key(dist, xb), nd | kFirst);
+            } else {
+              const unsigned long long j = off_s[wv][sa] + la, at = j < rem ? r0 + j : nr + (j - rem);
+              if (at < cap) stage[at] = Stage{xa, nd | kFirst, make_key(dist, xb)};
+            }
           }
           if (lb != 0xffu) {
-            const unsigned long long j = off_s[wv][sb] + lb, at = j < rem ? r0 + j : nr + (j - rem);
-            if (at < cap) stage[at] = Stage{xb, nd, make_key(dist, xa)};
+            if constexpr (kDirect) {
+              put_direct(bkey, bsrc, off_s[wv][sb] + lb, make_key(dist, xa), nd);
+            } else {
+              const unsigned long long j = off_s[wv][sb] + lb, at = j < rem ? r0 + j : nr + (j - rem);
+              if (at < cap) stage[at] = Stage{xb, nd, make_key(dist, xa)};
+            }
           }
         }
       }
       wave_sync();
     }
   }
-  for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
-  for (int o = 32; o > 0; o >>= 1) pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
-  if (lane == 0 && pairs_local) atomicAdd(npairs, pairs_local);
+  if (!kDirect)
+    for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
+  join_totals<kDirect>(pairs_local, kept_local, npairs, nstage);
 }
 
 // FP32 re-rank of the final pools (the first min(fill, 2K) entries) and of
@@ -1860,6 +2028,9 @@ __device__ inline uint32_t cand_tag(uint32_t s) {
   return 0x80000000u | ((s & kFirst) ? 0u : 0x40000000u) | (s & 0x3fffffffu);
 }
 
+// Target x's candidates are bucket slots [toff[x], toff[x + 1]) after the
+// scatter, or [toff[x], toff[x] + tcnt[x]) with direct routing (tcnt given).
+//
 // Per target: pool' = best L of (pool + candidates) by (key, tag), one
 // wavefront per target, in rounds: the running list (at most L entries,
 // sorted, unique) plus the next candidates at or below its worst key (every
@@ -1878,6 +2049,7 @@ __device__ inline uint32_t cand_tag(uint32_t s) {
 // tag) order.
 template <int kCap, bool kById>
 __global__ void __launch_bounds__(kBlock) k_merge_wave(uint32_t n, int L, int CK, const int32_t* __restrict__ toff,
+                                                       const uint32_t* __restrict__ tcnt,
                                                        const uint64_t* __restrict__ bkey,
                                                        const uint32_t* __restrict__ bsrc,
                                                        uint64_t* __restrict__ pool, uint8_t* __restrict__ flag,
@@ -1892,7 +2064,7 @@ __global__ void __launch_bounds__(kBlock) k_merge_wave(uint32_t n, int L, int CK
   uint32_t* tg = ts[w];
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (uint32_t x = blockIdx.x * kW + w; x < n; x += gridDim.x * kW) {
-    const int c0 = toff[x], c1 = toff[x + 1];
+    const int c0 = toff[x], c1 = tcnt ? c0 + int(tcnt[x]) : toff[x + 1];
     if (c0 == c1) continue;
     int cur = int(fill[x]);
     for (int i = lane; i < cur; i += 64) {
@@ -2038,6 +2210,7 @@ __device__ __forceinline__ bool kt_less(uint64_t ka, uint32_t ta, uint64_t kb, u
 // ids (as in k_merge_wave).
 template <int kLMax, bool kById>
 __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK, const int32_t* __restrict__ toff,
+                                                      const uint32_t* __restrict__ tcnt,
                                                       const uint64_t* __restrict__ bkey,
                                                       const uint32_t* __restrict__ bsrc, uint64_t* __restrict__ pool,
                                                       uint8_t* __restrict__ flag, uint32_t* __restrict__ fill,
@@ -2061,7 +2234,7 @@ __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK,
   uint32_t* CT = cs_t[w];
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (uint32_t x = blockIdx.x * kW + w; x < n; x += gridDim.x * kW) {
-    const int c0 = toff[x], c1 = toff[x + 1];
+    const int c0 = toff[x], c1 = tcnt ? c0 + int(tcnt[x]) : toff[x + 1];
     if (c0 == c1) continue;
     int cur = int(fill[x]);
     for (int i = lane; i < cur; i += 64) {
@@ -2662,9 +2835,11 @@ int gpu_refine_iter(efg_index* ix) {
   if (!worst.ensure(n) || !tcount.ensure(n) || !cursor.ensure(n) || !toff.ensure(n + 1) || !scal.ensure(4) ||
       !ctr.ensure(3))
     return fail(EFG_ERR_MEMORY, "out of device memory (join)");
+  int dev = 0, ncu = 256;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   k_worst<<<grid_for(n), 256, 0, st>>>(bs.pool.p, bs.fill.p, n, L, worst.p);
   hipMemsetAsync(ctr.p, 0, 24, st);
-  k_pair_bound<<<grid_for(n), 256, 0, st>>>(n, bs.new_off.p, bs.old_off.p, ctr.p);
   DevBuf<int32_t>&tcnt = sc.tcnt, &tskoff = sc.tskoff;
   if (!tcnt.ensure(n) || !tskoff.ensure(n + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (join tasks)");
   // tree 0's leaf list (a permutation of the nodes) orders the tasks
@@ -2672,7 +2847,8 @@ int gpu_refine_iter(efg_index* ix) {
   // nodes of at most 64 (precision form) or kNodeRows (FP32) rows go to the
   // whole-node kernels, the others to the block tasks
   const uint32_t node_rows = ix->b_precision > 0 ? 64u : uint32_t(kNodeRows);
-  k_join_count<<<grid_for(n), 256, 0, st>>>(n, order, bs.new_off.p, bs.old_off.p, node_rows, tcnt.p, ctr.p + 2);
+  k_join_count<<<unsigned(std::min<size_t>(grid_for(n), size_t(ncu) * 4)), kBlock, 0, st>>>(
+      n, order, bs.new_off.p, bs.old_off.p, node_rows, tcnt.p, ctr.p);
   if (int rc = device_exclusive_scan(tcnt.p, int(n), tskoff.p, scal.p + 2, sums, st)) return rc;
   unsigned long long bound = 0, rows = 0;
   int ntask = 0;
@@ -2681,121 +2857,156 @@ int gpu_refine_iter(efg_index* ix) {
   hipMemcpyAsync(&ntask, scal.p + 2, 4, hipMemcpyDeviceToHost, st);
   if (int rc = check_hip(hipStreamSynchronize(st), "join tasks")) return rc;
   ix->counters[6] += rows;
-  int dev = 0, ncu = 256;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   // eight task ranges, one per XCD: a multiple of eight workgroups
   const size_t join_grid = size_t((ncu * 4 + 7) / 8) * 8;
   const size_t node_grid = size_t(ncu) * 8;  // whole-node FP32 kernel: workgroups of four wavefronts
-  // staging: the kept candidates (two per pair at most) plus one partly used
-  // run per workgroup (per wavefront on the MFMA path, ncu * 8 workgroups of
-  // four, three kernels).  It is sized from the last iteration's kept fraction
-  // with a margin; when the candidates do not fit, the join kernels (which do
-  // not change the pools) run again with room for every reservation counted.
-  const size_t slack = ix->b_precision > 0 ? 3 * size_t(ncu) * 8 * (kBlock / 64) * kStageRun + 1024
-                                           : (join_grid + node_grid * (kBlock / 64)) * kStageRun + 1024;
-  const size_t full = size_t(2 * bound) + slack;
-  size_t cap = std::min(full, size_t(double(2 * bound) * std::min(1.0, ix->b_stage_margin * bs.keep_frac)) + slack);
-  if (!bs.jhdr.ensure(size_t(ntask) + 1) || !bs.jtids.ensure((size_t(ntask) + 1) * 64) ||
-      !bs.jstage.ensure(cap * sizeof(Stage)))
-    return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
-  k_join_tasks<<<unsigned((size_t(n) * 64 + 255) / 256), 256, 0, st>>>(n, order, bs.new_off.p, bs.new_ids.p,
-                                                                      bs.old_off.p, bs.old_ids.p, node_rows,
-                                                                      tskoff.p, bs.jhdr.p, bs.jtids.p);
+  if (!bs.jhdr.ensure(size_t(ntask) + 1) || !bs.jtids.ensure((size_t(ntask) + 1) * 64))
+    return fail(EFG_ERR_MEMORY, "out of device memory (join tasks)");
+  if (ntask)
+    k_join_tasks<<<unsigned(std::min<size_t>((size_t(n) + 255) / 256, size_t(ncu) * 4)), kBlock, 0, st>>>(
+        n, order, bs.new_off.p, bs.new_ids.p, bs.old_off.p, bs.old_ids.p, tcnt.p, tskoff.p, bs.jhdr.p, bs.jtids.p);
   // one chunk of at most 64 columns of both blocks' rows in LDS
   const int DC = std::min(D, 64);
   const size_t lds_join = size_t(64) * (DC + 4) * 4;
-  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_join_tiles<4>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                      int(lds_join));
-  unsigned long long h_ctr[2] = {0, 0};
-  for (int attempt = 0;; ++attempt) {
-    Stage* stage = reinterpret_cast<Stage*>(bs.jstage.p);
-    hipMemsetAsync(ctr.p, 0, 16, st);
-    hipMemsetAsync(tcount.p, 0, size_t(n) * 4, st);
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_join_tiles<4, false>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_join));
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_join_tiles<4, true>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_join));
+  if (ix->b_precision > 0)
+    if (int rc = ensure_split(ix, st)) return rc;
+  // The join kernels: whole nodes, then block tasks.  kD: direct routing
+  // (candidates straight into buckets at toff), else staged.
+  auto run_join = [&](auto direct_c, Stage* stage, unsigned long long cap) {
+    constexpr bool kD = decltype(direct_c)::value;
+    const int32_t* boff = kD ? toff.p : nullptr;
+    uint64_t* bk = kD ? bs.jbkey.p : nullptr;
+    uint32_t* bsr = kD ? bs.jbsrc.p : nullptr;
     if (ix->b_precision > 0) {
       // precision form on MFMA: one wavefront per node of at most 64 rows,
       // then one per block task of the larger nodes
-      if (int rc = ensure_split(ix, st)) return rc;
       const unsigned g16 = unsigned(size_t(ncu) * 8);
       const bool f16 = ix->b_precision == 2;
       const bool lo = ix->split_lo_used;
-      (f16 ? (lo ? k_join_node_mfma<true, true, 1> : k_join_node_mfma<true, false, 1>)
-           : (lo ? k_join_node_mfma<false, true, 1> : k_join_node_mfma<false, false, 1>))<<<g16, kBlock, 0, st>>>(
+      (f16 ? (lo ? k_join_node_mfma<true, true, 1, kD> : k_join_node_mfma<true, false, 1, kD>)
+           : (lo ? k_join_node_mfma<false, true, 1, kD> : k_join_node_mfma<false, false, 1, kD>))<<<g16, kBlock, 0,
+                                                                                                   st>>>(
           ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, n, order, bs.new_off.p, bs.new_ids.p,
-          bs.old_off.p, bs.old_ids.p, worst.p, stage, (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
-      (f16 ? (lo ? k_join_node_mfma<true, true, 3> : k_join_node_mfma<true, false, 3>)
-           : (lo ? k_join_node_mfma<false, true, 3> : k_join_node_mfma<false, false, 3>))<<<g16, kBlock, 0, st>>>(
+          bs.old_off.p, bs.old_ids.p, worst.p, stage, cap, ctr.p, tcount.p, ctr.p + 1, boff, bk, bsr);
+      (f16 ? (lo ? k_join_node_mfma<true, true, 3, kD> : k_join_node_mfma<true, false, 3, kD>)
+           : (lo ? k_join_node_mfma<false, true, 3, kD> : k_join_node_mfma<false, false, 3, kD>))<<<g16, kBlock, 0,
+                                                                                                   st>>>(
           ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, n, order, bs.new_off.p, bs.new_ids.p,
-          bs.old_off.p, bs.old_ids.p, worst.p, stage, (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
+          bs.old_off.p, bs.old_ids.p, worst.p, stage, cap, ctr.p, tcount.p, ctr.p + 1, boff, bk, bsr);
       if (ntask)
-        (f16 ? (lo ? k_join_mfma<true, true> : k_join_mfma<true, false>)
-             : (lo ? k_join_mfma<false, true> : k_join_mfma<false, false>))<<<g16, kBlock, 0, st>>>(
+        (f16 ? (lo ? k_join_mfma<true, true, kD> : k_join_mfma<true, false, kD>)
+             : (lo ? k_join_mfma<false, true, kD> : k_join_mfma<false, false, kD>))<<<g16, kBlock, 0, st>>>(
             ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, uint32_t(ntask), bs.jhdr.p, bs.jtids.p,
-            worst.p, stage, (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
+            worst.p, stage, cap, ctr.p, tcount.p, ctr.p + 1, boff, bk, bsr);
     } else {
-      k_join_node_f32<<<unsigned(node_grid), kBlock, 0, st>>>(ix->data.p, D, n, order, bs.new_off.p, bs.new_ids.p,
-                                                              bs.old_off.p, bs.old_ids.p, worst.p, stage,
-                                                              (unsigned long long)cap, ctr.p, tcount.p, ctr.p + 1);
+      k_join_node_f32<kD><<<unsigned(node_grid), kBlock, 0, st>>>(ix->data.p, D, n, order, bs.new_off.p,
+                                                                  bs.new_ids.p, bs.old_off.p, bs.old_ids.p, worst.p,
+                                                                  stage, cap, ctr.p, tcount.p, ctr.p + 1, boff, bk,
+                                                                  bsr);
       if (ntask)
-        k_join_tiles<4><<<unsigned(join_grid), kBlock, lds_join, st>>>(ix->data.p, D, uint32_t(ntask), bs.jhdr.p,
-                                                                       bs.jtids.p, worst.p, stage,
-                                                                       (unsigned long long)cap, ctr.p, tcount.p,
-                                                                       ctr.p + 1);
+        k_join_tiles<4, kD><<<unsigned(join_grid), kBlock, lds_join, st>>>(
+            ix->data.p, D, uint32_t(ntask), bs.jhdr.p, bs.jtids.p, worst.p, stage, cap, ctr.p, tcount.p, ctr.p + 1,
+            boff, bk, bsr);
     }
-    hipMemcpyAsync(h_ctr, ctr.p, 16, hipMemcpyDeviceToHost, st);
-    if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
-    // every candidate reserved a slot, written or not: a second run fits in
-    // the reservations counted plus one partly used run per workgroup
-    if (h_ctr[0] <= cap) break;
-    if (attempt > 0 || cap >= full) return fail(EFG_ERR_STATE, "join staging overflow");
-    cap = std::min(full, size_t(h_ctr[0]) + slack);
-    ix->counters[7] += 1;
+  };
+  // Direct routing when every target's room (the pairs it is in, 2 x bound
+  // slots in all) fits 31-bit offsets and memory; else staging.
+  const bool direct = ix->b_join_direct && 2 * bound + 1 < (1ull << 31) &&
+                      bs.jbkey.ensure(size_t(2 * bound) + 1) && bs.jbsrc.ensure(size_t(2 * bound) + 1);
+  unsigned long long h_ctr[2] = {0, 0};
+  if (direct) {
+    hipMemsetAsync(cursor.p, 0, size_t(n) * 4, st);
+    k_bucket_room<<<unsigned((size_t(n) + kRoomNodes - 1) / kRoomNodes), kBlock, 0, st>>>(
+        n, order, bs.new_off.p, bs.new_ids.p, bs.old_off.p, bs.old_ids.p, cursor.p);
+    if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(cursor.p), int(n), toff.p, scal.p + 3, sums, st))
+      return rc;
+    hipMemsetAsync(ctr.p, 0, 16, st);
+    hipMemsetAsync(tcount.p, 0, size_t(n) * 4, st);
+    run_join(std::true_type{}, nullptr, 0ull);
+  } else {
+    // staging: the kept candidates (two per pair at most) plus one partly
+    // used run per workgroup (per wavefront on the MFMA path, ncu * 8
+    // workgroups of four, three kernels).  It is sized from the last
+    // iteration's kept fraction with a margin; when the candidates do not
+    // fit, the join kernels (which do not change the pools) run again with
+    // room for every reservation counted.
+    const size_t slack = ix->b_precision > 0 ? 3 * size_t(ncu) * 8 * (kBlock / 64) * kStageRun + 1024
+                                             : (join_grid + node_grid * (kBlock / 64)) * kStageRun + 1024;
+    const size_t full = size_t(2 * bound) + slack;
+    size_t cap =
+        std::min(full, size_t(double(2 * bound) * std::min(1.0, ix->b_stage_margin * bs.keep_frac)) + slack);
     if (!bs.jstage.ensure(cap * sizeof(Stage))) return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
+    for (int attempt = 0;; ++attempt) {
+      hipMemsetAsync(ctr.p, 0, 16, st);
+      hipMemsetAsync(tcount.p, 0, size_t(n) * 4, st);
+      run_join(std::false_type{}, reinterpret_cast<Stage*>(bs.jstage.p), (unsigned long long)cap);
+      hipMemcpyAsync(h_ctr, ctr.p, 16, hipMemcpyDeviceToHost, st);
+      if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
+      // every candidate reserved a slot, written or not: a second run fits in
+      // the reservations counted plus one partly used run per workgroup
+      if (h_ctr[0] <= cap) break;
+      if (attempt > 0 || cap >= full) return fail(EFG_ERR_STATE, "join staging overflow");
+      cap = std::min(full, size_t(h_ctr[0]) + slack);
+      ix->counters[7] += 1;
+      if (!bs.jstage.ensure(cap * sizeof(Stage)))
+        return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
+    }
+    const unsigned long long nst = h_ctr[0];
+    Stage* stage = reinterpret_cast<Stage*>(bs.jstage.p);
+    ix->join_pairs.push_back(h_ctr[1]);
+    ix->counters[3] += h_ctr[1];
+    if (nst >= (1ull << 31)) return fail(EFG_ERR_STATE, "too many join candidates for one iteration");
+    if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(tcount.p), int(n), toff.p, scal.p, sums, st))
+      return rc;
+    int nkept = 0;
+    hipMemcpyAsync(&nkept, scal.p, 4, hipMemcpyDeviceToHost, st);
+    if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
+    bs.keep_frac = bound ? double(nkept) / double(2 * bound) : 1.0;
+    if (!bs.jbkey.ensure(size_t(nkept) + 1) || !bs.jbsrc.ensure(size_t(nkept) + 1))
+      return fail(EFG_ERR_MEMORY, "out of device memory (buckets)");
+    hipMemsetAsync(cursor.p, 0, size_t(n) * 4, st);
+    if (nst)
+      k_join_scatter<<<unsigned((nst + 255) / 256), 256, 0, st>>>(stage, nst, toff.p, cursor.p, bs.jbkey.p,
+                                                                  bs.jbsrc.p);
   }
-  const unsigned long long nst = h_ctr[0];
-  Stage* stage = reinterpret_cast<Stage*>(bs.jstage.p);
-  ix->join_pairs.push_back(h_ctr[1]);
-  ix->counters[3] += h_ctr[1];
-  if (nst >= (1ull << 31)) return fail(EFG_ERR_STATE, "too many join candidates for one iteration");
-  if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(tcount.p), int(n), toff.p, scal.p, sums, st)) return rc;
-  int nkept = 0;
-  hipMemcpyAsync(&nkept, scal.p, 4, hipMemcpyDeviceToHost, st);
-  if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
-  bs.keep_frac = bound ? double(nkept) / double(2 * bound) : 1.0;
-  if (!bs.jbkey.ensure(size_t(nkept) + 1) || !bs.jbsrc.ensure(size_t(nkept) + 1))
-    return fail(EFG_ERR_MEMORY, "out of device memory (buckets)");
-  uint64_t* bkey = bs.jbkey.p;
-  uint32_t* bsrc = bs.jbsrc.p;
-  hipMemsetAsync(cursor.p, 0, size_t(n) * 4, st);
-  if (nst) k_join_scatter<<<unsigned((nst + 255) / 256), 256, 0, st>>>(stage, nst, toff.p, cursor.p, bkey, bsrc);
   hipMemsetAsync(bs.found.p, 0, n, st);
   {
     // one wavefront per target; the LDS window holds at least 2L entries
     const bool byid = ix->b_precision > 0 && !ix->split_exact;  // approximate keys
     const unsigned grid = unsigned(std::min<size_t>((n + 3) / 4, size_t(ncu) * 8));
+    const uint32_t* tc = direct ? tcount.p : nullptr;
+    uint64_t* bkey = bs.jbkey.p;
+    uint32_t* bsrc = bs.jbsrc.p;
     if (L <= 512)
       (L <= 64 ? (byid ? k_merge_reg<64, true> : k_merge_reg<64, false>)
                : L <= 256 ? (byid ? k_merge_reg<256, true> : k_merge_reg<256, false>)
                           : (byid ? k_merge_reg<512, true> : k_merge_reg<512, false>))<<<grid, kBlock, 0, st>>>(
-          n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p, bs.fill.p, bs.found.p);
+          n, L, p.check, toff.p, tc, bkey, bsrc, bs.pool.p, bs.flag.p, bs.fill.p, bs.found.p);
     else if (2 * L <= 256)
       (byid ? k_merge_wave<256, true> : k_merge_wave<256, false>)<<<grid, kBlock, 0, st>>>(
-          n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
-                                                 bs.fill.p, bs.found.p);
+          n, L, p.check, toff.p, tc, bkey, bsrc, bs.pool.p, bs.flag.p, bs.fill.p, bs.found.p);
     else if (2 * L <= 512)
       (byid ? k_merge_wave<512, true> : k_merge_wave<512, false>)<<<grid, kBlock, 0, st>>>(
-          n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
-                                                 bs.fill.p, bs.found.p);
+          n, L, p.check, toff.p, tc, bkey, bsrc, bs.pool.p, bs.flag.p, bs.fill.p, bs.found.p);
     else if (2 * L <= 1024)
       (byid ? k_merge_wave<1024, true> : k_merge_wave<1024, false>)<<<grid * 2, 128, 0, st>>>(
-          n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
-                                                   bs.fill.p, bs.found.p);
+          n, L, p.check, toff.p, tc, bkey, bsrc, bs.pool.p, bs.flag.p, bs.fill.p, bs.found.p);
     else if (byid)
       return fail(EFG_ERR_ARG, "the build's precision form supports pools of up to 512 entries");
     else
-      k_merge_wave<2048, false><<<grid * 2, 128, 0, st>>>(n, L, p.check, toff.p, bkey, bsrc, bs.pool.p, bs.flag.p,
-                                                          bs.fill.p, bs.found.p);
+      k_merge_wave<2048, false><<<grid * 2, 128, 0, st>>>(n, L, p.check, toff.p, tc, bkey, bsrc, bs.pool.p,
+                                                          bs.flag.p, bs.fill.p, bs.found.p);
+    if (direct) hipMemcpyAsync(h_ctr, ctr.p, 16, hipMemcpyDeviceToHost, st);
     if (int rc = check_hip(hipStreamSynchronize(st), "join merge")) return rc;
+    if (direct) {
+      ix->join_pairs.push_back(h_ctr[1]);
+      ix->counters[3] += h_ctr[1];
+      bs.keep_frac = bound ? double(h_ctr[0]) / double(2 * bound) : 1.0;
+    }
   }
   if (int rc = check_hip(hipStreamSynchronize(st), "join merge")) return rc;
   double t1 = now_ms();

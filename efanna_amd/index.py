@@ -202,6 +202,10 @@ class FIndex:
         """Join staging room: margin x the last iteration's kept candidate fraction (default 1.25)."""
         check(lib().efg_set_join_staging_margin(self.h, float(margin)))
 
+    def setJoinRouting(self, direct: bool = True):
+        """Join candidates straight into their targets' buckets (default), or staged first."""
+        check(lib().efg_set_join_routing(self.h, 1 if direct else 0))
+
     def setSearchPrecision(self, precision: int):
         """0 = FP32 rows; 1 = BF16 rows with FP32 re-rank of the best 2k."""
         check(lib().efg_set_search_precision(self.h, int(precision)))

@@ -152,6 +152,16 @@ int efg_set_build_precision(efg_index* ix, int precision);
  * default 1.25.  Results do not depend on it. */
 int efg_set_join_staging_margin(efg_index* ix, double margin);
 
+/* Join candidate routing of the build (no counterpart in the reference; data
+ * movement only).  1 (default): each kept candidate goes straight from the
+ * join kernels into its target row's bucket, which has room for every pair
+ * the row is in (2 x the iteration's pairs slots in all, 12 bytes each); an
+ * iteration whose pairs need 2^31 slots or more, or whose buckets cannot be
+ * allocated, is staged instead.  0: always staged (candidates written in
+ * per-wavefront runs, then scattered into buckets sized by the kept counts;
+ * see efg_set_join_staging_margin).  Results do not depend on it. */
+int efg_set_join_routing(efg_index* ix, int direct);
+
 /* Learned stopping (no counterpart in the reference; a form beside the plain
  * search).  A small network (16 pool-state features -> 64 -> 64 -> 1 logit,
  * ReLU, 5,313 parameters laid out W1[16][64] | b1[64] | W2[64][64] | b2[64] |

@@ -66,9 +66,13 @@ def test_trees_match_oracle_mixture_dfs_phase(cuda):
     _cmp_trees(g, o)
 
 
-def test_build_stages_match_oracle(cuda):
+@pytest.mark.parametrize("direct", [True, False])
+def test_build_stages_match_oracle(cuda, direct):
+    """Every stage equals the oracle, with the join's candidates routed
+    straight into their targets' buckets (default) or staged and scattered."""
     base, _ = helpers.uniform()
     g, o = _pair(base)
+    g.setJoinRouting(direct)
     g.build_stage(0)
     o.build_trees()
     _cmp_trees(g, o)
@@ -115,13 +119,14 @@ def test_build_join_staging_reruns_keep_results(cuda, precision):
     larger iterations join again with room for every candidate (the staging
     always has one 4096-slot run per wavefront besides, so this takes a few
     hundred thousand rows).  Graph and join pairs equal the default build's
-    (the FP32 build equals the oracle in the tests above; the precision form
-    equals it on integer rows)."""
+    (direct routing; the FP32 build equals the oracle in the tests above, the
+    precision form equals it on integer rows)."""
     base, _ = helpers.mixture(n=300_000, nq=1)
     g, _o = _pair(base, trees=8, ml=8, epochs=6)
     g.buildIndex()
     h, _o2 = _pair(base, trees=8, ml=8, epochs=6)
     h.setBuildPrecision(precision)
+    h.setJoinRouting(False)
     h.setJoinStagingMargin(0.01)
     h.buildIndex()
     assert g.counters()["join_reruns"] == 0
@@ -130,13 +135,15 @@ def test_build_join_staging_reruns_keep_results(cuda, precision):
     assert h.join_pairs() == g.join_pairs()
 
 
-@pytest.mark.parametrize("d", [100, 200])
-def test_build_float_data_long_lists_matches_oracle(cuda, d):
+@pytest.mark.parametrize("d,direct", [(100, True), (200, True), (200, False)])
+def test_build_float_data_long_lists_matches_oracle(cuda, d, direct):
     # non-integer rows (every FMA rounds), d % 32 != 0 (tail chunk first),
     # more than 64 columns (several column chunks per join task) and lists
-    # longer than one 32-row block (tasks with I != J)
+    # longer than one 32-row block (tasks with I != J); block tasks with the
+    # candidates routed directly or staged
     base, _ = helpers.mixture(n=6000, nq=1, d=d, centres=30, sigma=0.05, hi=1.0, rounded=False)
     g, o = _pair(base, trees=4, ml=6, epochs=5, check=40, L=60, K=20, S=12)
+    g.setJoinRouting(direct)
     g.build_stage(0)
     o.build_trees()
     g.build_stage(1)
@@ -272,7 +279,8 @@ def test_build_precision_form_pools_unique(cuda, precision):
 def test_build_precision_form_long_lists_equals_fp32_build(cuda, precision):
     """Lists of more than 64 rows: the MFMA block tasks (beside the whole-node
     MFMA kernel) give the FP32 build's pools and graph bit for bit on integer
-    rows."""
+    rows (the FP16 form with its candidates staged, the others routed
+    directly)."""
     base, _ = helpers.mixture(n=20_000, nq=1)
     kw = dict(trees=4, ml=6, epochs=4, check=40, L=80, K=40, S=20)
     g, _o = _pair(base, **kw)
@@ -280,6 +288,7 @@ def test_build_precision_form_long_lists_equals_fp32_build(cuda, precision):
     g.build_stage(1)
     h, _o2 = _pair(base, **kw)
     h.setBuildPrecision(precision)
+    h.setJoinRouting(precision == 1)
     h.build_stage(0)
     h.build_stage(1)
     longest = 0
