@@ -1598,7 +1598,7 @@ __global__ void __launch_bounds__(kBlock) k_leaf_mfma(const uint16_t* __restrict
 // k_join_tasks), the 32 x 32 block of dot products in one accumulator; then
 // the candidates as in k_join_tiles, counted per target row in the
 // wavefront's LDS slice, staged into per-wavefront runs.
-template <bool kF16, bool kLo, bool kDirect>
+template <bool kF16, bool kLo>
 __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict__ xhi, const uint16_t* __restrict__ xlo,
                                                       const float* __restrict__ xn, int Dk, uint32_t ntask,
                                                       const uint4* __restrict__ hdr,
@@ -1606,9 +1606,7 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
                                                       const uint64_t* __restrict__ worst, Stage* __restrict__ stage,
                                                       unsigned long long cap, unsigned long long* __restrict__ nstage,
                                                       uint32_t* __restrict__ tcount,
-                                                      unsigned long long* __restrict__ npairs,
-                                                      const int32_t* __restrict__ boff,
-                                                      uint64_t* __restrict__ bkey, uint32_t* __restrict__ bsrc) {
+                                                      unsigned long long* __restrict__ npairs) {
   constexpr int kW = kBlock / 64;
   __shared__ uint32_t ids_s[kW][64];
   __shared__ uint64_t w_s[kW][64];
@@ -1618,7 +1616,7 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
   const int r32 = lane & 31;
   const uint32_t xcd = blockIdx.x & 7u, nsl = (gridDim.x >> 3) * kW;
   const uint32_t lo = uint32_t(uint64_t(ntask) * xcd / 8), hi = uint32_t(uint64_t(ntask) * (xcd + 1) / 8);
-  unsigned long long pairs_local = 0, kept_local = 0;
+  unsigned long long pairs_local = 0;
   unsigned long long wbase = 0, wend = 0;  // staging run (lane 63)
   uint32_t* ids = ids_s[wv];
   for (uint32_t t = lo + (blockIdx.x >> 3) * kW + wv; t < hi; t += nsl) {
@@ -1656,34 +1654,29 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
       }
     }
     wave_sync();
+    // totals per target (result not waited for), offsets within the task
     const uint32_t cnt = cnt_s[wv][lane];
-    unsigned long long rem = 0, nr = 0, r0 = 0;
-    if constexpr (kDirect) {
-      off_s[wv][lane] = cnt ? uint32_t(boff[id]) + atomicAdd(&tcount[id], cnt) : 0u;
-      kept_local += cnt;
+    if (cnt) atomicAdd(&tcount[id], cnt);
+    uint32_t x = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = uint32_t(__shfl_up(int(x), o));
+      if (lane >= o) x += y;
+    }
+    off_s[wv][lane] = x - cnt;
+    const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
+    // this task's slots: the rest of the run, then a new run if needed
+    const unsigned long long rem = wend - wbase;
+    unsigned long long nr = 0;
+    if (tot > rem) {
+      if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
+      nr = (unsigned long long)__shfl((long long)nr, 0);
+    }
+    const unsigned long long r0 = wbase;
+    if (tot > rem) {
+      wbase = nr + (tot - rem);
+      wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
     } else {
-      // totals per target (result not waited for), offsets within the task
-      if (cnt) atomicAdd(&tcount[id], cnt);
-      uint32_t x = cnt;
-      for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = uint32_t(__shfl_up(int(x), o));
-        if (lane >= o) x += y;
-      }
-      off_s[wv][lane] = x - cnt;
-      const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
-      // this task's slots: the rest of the run, then a new run if needed
-      rem = wend - wbase;
-      if (tot > rem) {
-        if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
-        nr = (unsigned long long)__shfl((long long)nr, 0);
-      }
-      r0 = wbase;
-      if (tot > rem) {
-        wbase = nr + (tot - rem);
-        wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
-      } else {
-        wbase += tot;
-      }
+      wbase += tot;
     }
     wave_sync();
     const uint32_t nn = h.x;
@@ -1696,27 +1689,19 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
       block_ends(p, q, same, sizeA, newA, sizeB, newB, ids, sa, sb, a, b);
       const uint32_t la = li[i] & 0xffffu, lb = li[i] >> 16;
       if (la != 0xffffu) {
-        if constexpr (kDirect) {
-          put_direct(bkey, bsrc, off_s[wv][sa] + la, make_key(dist[i], b), nn | kFirst);
-        } else {
-          const unsigned long long j = off_s[wv][sa] + la, at = j < rem ? r0 + j : nr + (j - rem);
-          if (at < cap) stage[at] = Stage{a, nn | kFirst, make_key(dist[i], b)};
-        }
+        const unsigned long long j = off_s[wv][sa] + la, at = j < rem ? r0 + j : nr + (j - rem);
+        if (at < cap) stage[at] = Stage{a, nn | kFirst, make_key(dist[i], b)};
       }
       if (lb != 0xffffu) {
-        if constexpr (kDirect) {
-          put_direct(bkey, bsrc, off_s[wv][sb] + lb, make_key(dist[i], a), nn);
-        } else {
-          const unsigned long long j = off_s[wv][sb] + lb, at = j < rem ? r0 + j : nr + (j - rem);
-          if (at < cap) stage[at] = Stage{b, nn, make_key(dist[i], a)};
-        }
+        const unsigned long long j = off_s[wv][sb] + lb, at = j < rem ? r0 + j : nr + (j - rem);
+        if (at < cap) stage[at] = Stage{b, nn, make_key(dist[i], a)};
       }
     }
     wave_sync();
   }
-  if (!kDirect)
-    for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
-  join_totals<kDirect>(pairs_local, kept_local, npairs, nstage);
+  for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
+  for (int o = 32; o > 0; o >>= 1) pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
+  if (lane == 0 && pairs_local) atomicAdd(npairs, pairs_local);
 }
 
 // Three 32 x 32 blocks of split dot products of one node's rows, loaded once:
@@ -2833,7 +2818,7 @@ int gpu_refine_iter(efg_index* ix) {
   DevBuf<int32_t>&toff = sc.toff, &sums = sc.sums, &scal = sc.scal;
   DevBuf<unsigned long long>& ctr = sc.ctr;
   if (!worst.ensure(n) || !tcount.ensure(n) || !cursor.ensure(n) || !toff.ensure(n + 1) || !scal.ensure(4) ||
-      !ctr.ensure(3))
+      !ctr.ensure(6))
     return fail(EFG_ERR_MEMORY, "out of device memory (join)");
   int dev = 0, ncu = 256;
   hipGetDevice(&dev);
@@ -2875,15 +2860,18 @@ int gpu_refine_iter(efg_index* ix) {
   if (ix->b_precision > 0)
     if (int rc = ensure_split(ix, st)) return rc;
   // The join kernels: whole nodes, then block tasks.  kD: direct routing
-  // (candidates straight into buckets at toff), else staged.
-  auto run_join = [&](auto direct_c, Stage* stage, unsigned long long cap) {
+  // (candidates straight into buckets at toff), else staged.  Counters:
+  // nst staged slots (direct: candidates kept), cnt per-target counts
+  // (direct: bucket fill), np pairs.
+  auto run_nodes = [&](auto direct_c, Stage* stage, unsigned long long cap, unsigned long long* nst, uint32_t* cnt,
+                       unsigned long long* np) {
     constexpr bool kD = decltype(direct_c)::value;
     const int32_t* boff = kD ? toff.p : nullptr;
     uint64_t* bk = kD ? bs.jbkey.p : nullptr;
     uint32_t* bsr = kD ? bs.jbsrc.p : nullptr;
     if (ix->b_precision > 0) {
-      // precision form on MFMA: one wavefront per node of at most 64 rows,
-      // then one per block task of the larger nodes
+      // precision form on MFMA: one wavefront per node of at most 32, then
+      // of 33 to 64 rows
       const unsigned g16 = unsigned(size_t(ncu) * 8);
       const bool f16 = ix->b_precision == 2;
       const bool lo = ix->split_lo_used;
@@ -2891,33 +2879,81 @@ int gpu_refine_iter(efg_index* ix) {
            : (lo ? k_join_node_mfma<false, true, 1, kD> : k_join_node_mfma<false, false, 1, kD>))<<<g16, kBlock, 0,
                                                                                                    st>>>(
           ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, n, order, bs.new_off.p, bs.new_ids.p,
-          bs.old_off.p, bs.old_ids.p, worst.p, stage, cap, ctr.p, tcount.p, ctr.p + 1, boff, bk, bsr);
+          bs.old_off.p, bs.old_ids.p, worst.p, stage, cap, nst, cnt, np, boff, bk, bsr);
       (f16 ? (lo ? k_join_node_mfma<true, true, 3, kD> : k_join_node_mfma<true, false, 3, kD>)
            : (lo ? k_join_node_mfma<false, true, 3, kD> : k_join_node_mfma<false, false, 3, kD>))<<<g16, kBlock, 0,
                                                                                                    st>>>(
           ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, n, order, bs.new_off.p, bs.new_ids.p,
-          bs.old_off.p, bs.old_ids.p, worst.p, stage, cap, ctr.p, tcount.p, ctr.p + 1, boff, bk, bsr);
-      if (ntask)
-        (f16 ? (lo ? k_join_mfma<true, true, kD> : k_join_mfma<true, false, kD>)
-             : (lo ? k_join_mfma<false, true, kD> : k_join_mfma<false, false, kD>))<<<g16, kBlock, 0, st>>>(
-            ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, uint32_t(ntask), bs.jhdr.p, bs.jtids.p,
-            worst.p, stage, cap, ctr.p, tcount.p, ctr.p + 1, boff, bk, bsr);
+          bs.old_off.p, bs.old_ids.p, worst.p, stage, cap, nst, cnt, np, boff, bk, bsr);
     } else {
       k_join_node_f32<kD><<<unsigned(node_grid), kBlock, 0, st>>>(ix->data.p, D, n, order, bs.new_off.p,
                                                                   bs.new_ids.p, bs.old_off.p, bs.old_ids.p, worst.p,
-                                                                  stage, cap, ctr.p, tcount.p, ctr.p + 1, boff, bk,
-                                                                  bsr);
-      if (ntask)
-        k_join_tiles<4, kD><<<unsigned(join_grid), kBlock, lds_join, st>>>(
-            ix->data.p, D, uint32_t(ntask), bs.jhdr.p, bs.jtids.p, worst.p, stage, cap, ctr.p, tcount.p, ctr.p + 1,
-            boff, bk, bsr);
+                                                                  stage, cap, nst, cnt, np, boff, bk, bsr);
     }
   };
+  // block tasks of the larger nodes; the MFMA form's are always staged
+  auto run_tasks = [&](auto direct_c, Stage* stage, unsigned long long cap, unsigned long long* nst, uint32_t* cnt,
+                       unsigned long long* np) {
+    constexpr bool kD = decltype(direct_c)::value;
+    if (!ntask) return;
+    if (ix->b_precision > 0) {
+      const unsigned g16 = unsigned(size_t(ncu) * 8);
+      const bool f16 = ix->b_precision == 2;
+      const bool lo = ix->split_lo_used;
+      (f16 ? (lo ? k_join_mfma<true, true> : k_join_mfma<true, false>)
+           : (lo ? k_join_mfma<false, true> : k_join_mfma<false, false>))<<<g16, kBlock, 0, st>>>(
+          ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, uint32_t(ntask), bs.jhdr.p, bs.jtids.p,
+          worst.p, stage, cap, nst, cnt, np);
+    } else {
+      k_join_tiles<4, kD><<<unsigned(join_grid), kBlock, lds_join, st>>>(
+          ix->data.p, D, uint32_t(ntask), bs.jhdr.p, bs.jtids.p, worst.p, stage, cap, nst, cnt, np,
+          kD ? toff.p : nullptr, kD ? bs.jbkey.p : nullptr, kD ? bs.jbsrc.p : nullptr);
+    }
+  };
+  auto run_tasks_staged = [&](Stage* stage, unsigned long long c, unsigned long long* nst, uint32_t* cnt,
+                              unsigned long long* np) { run_tasks(std::false_type{}, stage, c, nst, cnt, np); };
   // Direct routing when every target's room (the pairs it is in, 2 x bound
-  // slots in all) fits 31-bit offsets and memory; else staging.
+  // slots in all) fits 31-bit offsets and memory; else staging.  The MFMA
+  // block tasks stay staged (their candidates then appended to the same
+  // buckets by the scatter): routed directly, their scattered writes made
+  // that short kernel up to a third slower (1M x 960), more than the staging
+  // round trip costs.
   const bool direct = ix->b_join_direct && 2 * bound + 1 < (1ull << 31) &&
                       bs.jbkey.ensure(size_t(2 * bound) + 1) && bs.jbsrc.ensure(size_t(2 * bound) + 1);
-  unsigned long long h_ctr[2] = {0, 0};
+  const bool tasks_direct = ix->b_precision == 0;
+  // staging: the kept candidates (two per pair at most) plus one partly used
+  // run per workgroup (per wavefront on the MFMA path, ncu * 8 workgroups of
+  // four: three kernels staged, or the block tasks' alone with direct
+  // routing).  It is sized from the last iteration's kept fraction with a
+  // margin; when the candidates do not fit, the staged kernels (which do not
+  // change the pools) run again with room for every reservation counted.
+  const size_t slack = ix->b_precision > 0
+                           ? size_t(direct ? 1 : 3) * size_t(ncu) * 8 * (kBlock / 64) * kStageRun + 1024
+                           : (join_grid + node_grid * (kBlock / 64)) * kStageRun + 1024;
+  const size_t full = size_t(2 * bound) + slack;
+  size_t cap = std::min(full, size_t(double(2 * bound) * std::min(1.0, ix->b_stage_margin * bs.keep_frac)) + slack);
+  // Stage the candidates of `run` (the join kernels on the staging counters
+  // sc[0], sc[1] and per-target counts cnt), joining again with room for
+  // every reservation when they do not fit; h gets the slots used and pairs.
+  auto staged = [&](auto run, unsigned long long* sc, uint32_t* cnt, unsigned long long* h) -> int {
+    if (!bs.jstage.ensure(cap * sizeof(Stage))) return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
+    for (int attempt = 0;; ++attempt) {
+      hipMemsetAsync(sc, 0, 16, st);
+      hipMemsetAsync(cnt, 0, size_t(n) * 4, st);
+      run(reinterpret_cast<Stage*>(bs.jstage.p), (unsigned long long)cap, sc, cnt, sc + 1);
+      hipMemcpyAsync(h, sc, 16, hipMemcpyDeviceToHost, st);
+      if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
+      // every candidate reserved a slot, written or not: a second run fits in
+      // the reservations counted plus one partly used run per workgroup
+      if (h[0] <= cap) return EFG_OK;
+      if (attempt > 0 || cap >= full) return fail(EFG_ERR_STATE, "join staging overflow");
+      cap = std::min(full, size_t(h[0]) + slack);
+      ix->counters[7] += 1;
+      if (!bs.jstage.ensure(cap * sizeof(Stage)))
+        return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
+    }
+  };
+  unsigned long long h_ctr[2] = {0, 0}, h_task[2] = {0, 0};
   if (direct) {
     hipMemsetAsync(cursor.p, 0, size_t(n) * 4, st);
     k_bucket_room<<<unsigned((size_t(n) + kRoomNodes - 1) / kRoomNodes), kBlock, 0, st>>>(
@@ -2926,35 +2962,23 @@ int gpu_refine_iter(efg_index* ix) {
       return rc;
     hipMemsetAsync(ctr.p, 0, 16, st);
     hipMemsetAsync(tcount.p, 0, size_t(n) * 4, st);
-    run_join(std::true_type{}, nullptr, 0ull);
-  } else {
-    // staging: the kept candidates (two per pair at most) plus one partly
-    // used run per workgroup (per wavefront on the MFMA path, ncu * 8
-    // workgroups of four, three kernels).  It is sized from the last
-    // iteration's kept fraction with a margin; when the candidates do not
-    // fit, the join kernels (which do not change the pools) run again with
-    // room for every reservation counted.
-    const size_t slack = ix->b_precision > 0 ? 3 * size_t(ncu) * 8 * (kBlock / 64) * kStageRun + 1024
-                                             : (join_grid + node_grid * (kBlock / 64)) * kStageRun + 1024;
-    const size_t full = size_t(2 * bound) + slack;
-    size_t cap =
-        std::min(full, size_t(double(2 * bound) * std::min(1.0, ix->b_stage_margin * bs.keep_frac)) + slack);
-    if (!bs.jstage.ensure(cap * sizeof(Stage))) return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
-    for (int attempt = 0;; ++attempt) {
-      hipMemsetAsync(ctr.p, 0, 16, st);
-      hipMemsetAsync(tcount.p, 0, size_t(n) * 4, st);
-      run_join(std::false_type{}, reinterpret_cast<Stage*>(bs.jstage.p), (unsigned long long)cap);
-      hipMemcpyAsync(h_ctr, ctr.p, 16, hipMemcpyDeviceToHost, st);
-      if (int rc = check_hip(hipStreamSynchronize(st), "join")) return rc;
-      // every candidate reserved a slot, written or not: a second run fits in
-      // the reservations counted plus one partly used run per workgroup
-      if (h_ctr[0] <= cap) break;
-      if (attempt > 0 || cap >= full) return fail(EFG_ERR_STATE, "join staging overflow");
-      cap = std::min(full, size_t(h_ctr[0]) + slack);
-      ix->counters[7] += 1;
-      if (!bs.jstage.ensure(cap * sizeof(Stage)))
-        return fail(EFG_ERR_MEMORY, "out of device memory (join staging)");
+    run_nodes(std::true_type{}, nullptr, 0ull, ctr.p, tcount.p, ctr.p + 1);
+    if (tasks_direct) {
+      run_tasks(std::true_type{}, nullptr, 0ull, ctr.p, tcount.p, ctr.p + 1);
+    } else if (ntask) {
+      // the staged tasks' own per-target counts go to `cursor` (unused here)
+      if (int rc = staged(run_tasks_staged, ctr.p + 3, cursor.p, h_task)) return rc;
+      if (h_task[0])
+        k_join_scatter<<<unsigned((h_task[0] + 255) / 256), 256, 0, st>>>(
+            reinterpret_cast<Stage*>(bs.jstage.p), h_task[0], toff.p, tcount.p, bs.jbkey.p, bs.jbsrc.p);
     }
+  } else {
+    auto run_all = [&](Stage* stage, unsigned long long c, unsigned long long* nst, uint32_t* cnt,
+                       unsigned long long* np) {
+      run_nodes(std::false_type{}, stage, c, nst, cnt, np);
+      run_tasks(std::false_type{}, stage, c, nst, cnt, np);
+    };
+    if (int rc = staged(run_all, ctr.p, tcount.p, h_ctr)) return rc;
     const unsigned long long nst = h_ctr[0];
     Stage* stage = reinterpret_cast<Stage*>(bs.jstage.p);
     ix->join_pairs.push_back(h_ctr[1]);
@@ -3003,9 +3027,10 @@ int gpu_refine_iter(efg_index* ix) {
     if (direct) hipMemcpyAsync(h_ctr, ctr.p, 16, hipMemcpyDeviceToHost, st);
     if (int rc = check_hip(hipStreamSynchronize(st), "join merge")) return rc;
     if (direct) {
-      ix->join_pairs.push_back(h_ctr[1]);
-      ix->counters[3] += h_ctr[1];
-      bs.keep_frac = bound ? double(h_ctr[0]) / double(2 * bound) : 1.0;
+      // kept fraction of the directly routed candidates (exact counts)
+      ix->join_pairs.push_back(h_ctr[1] + h_task[1]);
+      ix->counters[3] += h_ctr[1] + h_task[1];
+      bs.keep_frac = h_ctr[1] ? double(h_ctr[0]) / double(2 * h_ctr[1]) : 1.0;
     }
   }
   if (int rc = check_hip(hipStreamSynchronize(st), "join merge")) return rc;

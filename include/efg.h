@@ -155,9 +155,11 @@ int efg_set_join_staging_margin(efg_index* ix, double margin);
 /* Join candidate routing of the build (no counterpart in the reference; data
  * movement only).  1 (default): each kept candidate goes straight from the
  * join kernels into its target row's bucket, which has room for every pair
- * the row is in (2 x the iteration's pairs slots in all, 12 bytes each); an
+ * the row is in (2 x the iteration's pairs slots in all, 12 bytes each),
+ * except the precision form's block tasks (nodes of more than 64 rows),
+ * whose candidates are staged and scattered into the same buckets; an
  * iteration whose pairs need 2^31 slots or more, or whose buckets cannot be
- * allocated, is staged instead.  0: always staged (candidates written in
+ * allocated, is staged entirely.  0: always staged (candidates written in
  * per-wavefront runs, then scattered into buckets sized by the kept counts;
  * see efg_set_join_staging_margin).  Results do not depend on it. */
 int efg_set_join_routing(efg_index* ix, int direct);

@@ -118,8 +118,10 @@ def test_build_join_staging_reruns_keep_results(cuda, precision):
     """A staging margin far below the kept candidate fraction makes the
     larger iterations join again with room for every candidate (the staging
     always has one 4096-slot run per wavefront besides, so this takes a few
-    hundred thousand rows).  Graph and join pairs equal the default build's
-    (direct routing; the FP32 build equals the oracle in the tests above, the
+    hundred thousand rows), here with every join kernel staged; the
+    precision form's block tasks, staged beside direct routing, rerun through
+    the same code.  Graph and join pairs equal the default build's (direct
+    routing; the FP32 build equals the oracle in the tests above, the
     precision form equals it on integer rows)."""
     base, _ = helpers.mixture(n=300_000, nq=1)
     g, _o = _pair(base, trees=8, ml=8, epochs=6)
