@@ -55,6 +55,17 @@ __host__ __device__ inline uint64_t feistel_perm(uint64_t seed, uint64_t stream,
   return x;
 }
 
+// One element of a row as FP32: kRow 0 reads FP32 rows, 1 the high BF16
+// parts and 2 the high FP16 parts of the build's split rows.  The 16-bit
+// forms are used only when every row is exact in them, so every kernel
+// reading them sees the same values as from the FP32 rows.
+template <int kRow>
+__device__ __forceinline__ float row_elem(const void* __restrict__ X, size_t off) {
+  if constexpr (kRow == 0) return static_cast<const float*>(X)[off];
+  else if constexpr (kRow == 1) return __uint_as_float(uint32_t(static_cast<const uint16_t*>(X)[off]) << 16);
+  else return float(static_cast<const _Float16*>(X)[off]);
+}
+
 // (distance, id) packed so that unsigned order = (distance, id) order for
 // non-negative distances.
 __host__ __device__ inline uint64_t make_key(float d, uint32_t id) {

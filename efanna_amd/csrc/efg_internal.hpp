@@ -130,11 +130,13 @@ struct BuildState {
     DevBuf<uint32_t> tcount, cursor;
     DevBuf<int32_t> toff, sums, scal, tcnt, tskoff;
     DevBuf<unsigned long long> ctr;
-    DevBuf<uint32_t> fnew, fold, rcn, rco, rcurn, rcuro, rszn, rszo, ridn, rido;
+    DevBuf<uint32_t> fnew, fold, rcn, rco, rszn, rszo, ridn, rido;
+    DevBuf<uint32_t> rslot;  // n x L: slot of each pool entry in its reverse list (see k_lists)
     DevBuf<int32_t> roffn, roffo, snew, sold, onew, oold, bign, bigo;
     DevBuf<int> nbig;
     void release() {
-      for (DevBuf<uint32_t>* b : {&tcount, &cursor, &fnew, &fold, &rcn, &rco, &rcurn, &rcuro, &rszn, &rszo, &ridn, &rido})
+      for (DevBuf<uint32_t>* b :
+           {&tcount, &cursor, &fnew, &fold, &rcn, &rco, &rszn, &rszo, &ridn, &rido, &rslot})
         b->reset();
       for (DevBuf<int32_t>* b :
            {&toff, &sums, &scal, &tcnt, &tskoff, &roffn, &roffo, &snew, &sold, &onew, &oold, &bign, &bigo})
@@ -344,6 +346,9 @@ int gpu_build_trees(efg_index* ix);
 int device_exclusive_scan(const int32_t* in, int m, int32_t* out, int32_t* total_dev, DevBuf<int32_t>& sums,
                           hipStream_t st);
 // k_graph.hip
+// The split rows of the build's precision form (hi + lo 16-bit parts, n x
+// split_Dk), made once per build; no-op for the FP32 form.
+int ensure_split(efg_index* ix, hipStream_t st);
 int gpu_merge_init(efg_index* ix);
 int gpu_refine_iter(efg_index* ix);
 int gpu_finalize_graph(efg_index* ix);

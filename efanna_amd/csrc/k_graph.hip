@@ -126,12 +126,24 @@ __device__ inline int32_t descend(const int4* __restrict__ node4, int32_t x, con
   }
 }
 
+// descend over rows read through row_elem (FP32 or exact 16-bit rows)
+template <int kRow>
+__device__ inline int32_t descend_rows(const int4* __restrict__ node4, int32_t x, const void* __restrict__ X,
+                                       size_t base) {
+  for (;;) {
+    int4 nd = node4[x];
+    if (nd.z < 0) return x;
+    x = (row_elem<kRow>(X, base + nd.x) < __int_as_float(nd.y)) ? nd.z : nd.w;
+  }
+}
+
 // For each position: walk the ancestors at depth >= ml; where the position is
 // in the smaller child, probe the larger one.  Pass 0 counts records per leaf
 // reached and keeps the leaf of every (position, depth(X) - ml) slot in lam
 // (-1: no probe there); pass 1 reads it back instead of descending again and
 // writes (63 - depth(X)) << 32 | pos into the leaf's bucket.
-__global__ void k_probe(const float* __restrict__ X, int D, const int4* __restrict__ node4,
+template <int kRow>
+__global__ void k_probe(const void* __restrict__ X, int D, const int4* __restrict__ node4,
                         const int32_t* __restrict__ parent, const uint2* __restrict__ range,
                         const int8_t* __restrict__ depth, const uint32_t* __restrict__ leaf_t,
                         const int32_t* __restrict__ leaf_at_t, uint32_t n, int ml, int pass,
@@ -155,7 +167,7 @@ __global__ void k_probe(const float* __restrict__ X, int D, const int4* __restri
     return;
   }
   const uint32_t p = leaf_t[pos];
-  const float* xp = X + size_t(p) * D;
+  const size_t xb = size_t(p) * D;
   while (x >= 0 && depth[x] >= ml) {
     const int4 nd = node4[x];
     const uint2 rl = range[nd.z], rr = range[nd.w];
@@ -164,7 +176,7 @@ __global__ void k_probe(const float* __restrict__ X, int D, const int4* __restri
     int32_t lam = -1;
     if (child == small) {
       const int32_t big = (small == nd.z) ? nd.w : nd.z;
-      lam = descend(node4, big, xp);
+      lam = descend_rows<kRow>(node4, big, X, xb);
       atomicAdd(&cnt[lam], 1u);
     }
     lam_s[size_t(pos) * nlev + (depth[x] - ml)] = lam;
@@ -2408,16 +2420,21 @@ __global__ void k_range(const uint64_t* __restrict__ pool, const uint8_t* __rest
   if (l0 == 0) radiusM[p] = key_dist(pool[size_t(p) * L + R - 1]);
 }
 
-// pass 0: forward counts and reverse counts; pass 1: reverse fill
-// (base_index.hpp:396-420)
+// pass 0: forward counts and reverse counts, each reverse entry's slot in
+// its list (the count before it) kept in rslot (kNoSlot: no reverse entry);
+// pass 1: reverse fill from the kept slots (base_index.hpp:396-420).  The
+// order within a reverse list is the order of pass 0's atomics: lists above
+// the cap are sampled by generator value, and the join's candidates are
+// merged by (key, tag), neither depending on it.
+constexpr uint32_t kNoSlot = 0xffffffffu, kOldSlot = 0x80000000u;
 template <int G>
 __global__ void k_lists(const uint64_t* __restrict__ pool, const uint8_t* __restrict__ flag,
                         const uint32_t* __restrict__ rng, const float* __restrict__ radiusM, uint32_t n, int L, int pass,
                         uint32_t* __restrict__ fcnt_new, uint32_t* __restrict__ fcnt_old,
                         uint32_t* __restrict__ rcnt_new, uint32_t* __restrict__ rcnt_old,
                         const int32_t* __restrict__ roff_new, const int32_t* __restrict__ roff_old,
-                        uint32_t* __restrict__ rcur_new, uint32_t* __restrict__ rcur_old,
-                        uint32_t* __restrict__ rids_new, uint32_t* __restrict__ rids_old) {
+                        uint32_t* __restrict__ rslot, uint32_t* __restrict__ rids_new,
+                        uint32_t* __restrict__ rids_old) {
   const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint32_t p = uint32_t(t / G);
   const int l0 = int(t % G);
@@ -2428,6 +2445,17 @@ __global__ void k_lists(const uint64_t* __restrict__ pool, const uint8_t* __rest
   for (int lb = 0; lb < int(R); lb += G) {
     const int l = lb + l0;
     const bool act = uint32_t(l) < R;
+    if (pass == 1) {
+      if (act) {
+        const uint32_t sl = rslot[size_t(p) * L + l];
+        if (sl != kNoSlot) {
+          const uint32_t o = key_id(pool[size_t(p) * L + l]);
+          if (sl & kOldSlot) rids_old[size_t(roff_old[o]) + (sl & ~kOldSlot)] = p;
+          else rids_new[size_t(roff_new[o]) + sl] = p;
+        }
+      }
+      continue;
+    }
     bool isnew = false, rev = false;
     uint32_t o = 0;
     if (act) {
@@ -2437,13 +2465,10 @@ __global__ void k_lists(const uint64_t* __restrict__ pool, const uint8_t* __rest
       rev = key_dist(e) > radiusM[o];
     }
     fn += uint32_t(__popcll(__ballot(act && isnew) & seg));
-    if (!rev) continue;
-    if (pass == 0) {
-      atomicAdd(isnew ? &rcnt_new[o] : &rcnt_old[o], 1u);
-    } else if (isnew) {
-      rids_new[size_t(roff_new[o]) + atomicAdd(&rcur_new[o], 1u)] = p;
-    } else {
-      rids_old[size_t(roff_old[o]) + atomicAdd(&rcur_old[o], 1u)] = p;
+    if (act) {
+      uint32_t sl = kNoSlot;
+      if (rev) sl = isnew ? atomicAdd(&rcnt_new[o], 1u) : (atomicAdd(&rcnt_old[o], 1u) | kOldSlot);
+      rslot[size_t(p) * L + l] = sl;
     }
   }
   if (pass == 0 && l0 == 0) {
@@ -2604,6 +2629,8 @@ double now_ms() {
 
 int grid_for(size_t n) { return int((n + 255) / 256); }
 
+}  // namespace
+
 // The split rows of the build's precision form (made once per build).
 int ensure_split(efg_index* ix, hipStream_t st) {
   if (ix->b_precision == 0) return EFG_OK;
@@ -2641,6 +2668,9 @@ int ensure_split(efg_index* ix, hipStream_t st) {
   ix->split_kind = ix->b_precision;
   return check_hip(hipGetLastError(), "split rows");
 }
+
+namespace {
+
 void release_split(efg_index* ix) {
   ix->split_hi.reset();
   ix->split_lo.reset();
@@ -2719,11 +2749,17 @@ int gpu_merge_init(efg_index* ix) {
     hipGetDevice(&dv);
     hipDeviceGetAttribute(&ncu_m, hipDeviceAttributeMultiprocessorCount, dv);
   }
+  // probes descend over the rows the trees were built from: the exact 16-bit
+  // rows of the precision form, else FP32
+  const bool rows16 = ix->b_precision > 0 && !ix->split_lo_used;
+  const void* PX = rows16 ? static_cast<const void*>(ix->split_hi.p) : static_cast<const void*>(ix->data.p);
+  const int PD = rows16 ? ix->split_Dk : D;
+  const auto probe = !rows16 ? &k_probe<0> : (ix->b_precision == 2 ? &k_probe<2> : &k_probe<1>);
   for (int t = 0; t < TB; ++t) {
     const uint32_t* lt = ix->leaf.p + size_t(t) * n;
     const int32_t* la = leaf_at.p + size_t(t) * n;
     hipMemsetAsync(cnt.p, 0, size_t(M) * 4, st);
-    k_probe<<<grid_for(n), 256, 0, st>>>(ix->data.p, D, ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la, n,
+    probe<<<grid_for(n), 256, 0, st>>>(PX, PD, ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la, n,
                                          p.merge_level, 0, cnt.p, nullptr, nullptr, nullptr, lam_s.p, nlev);
     if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(cnt.p), M, boff.p, scal.p + 1, sums, st)) return rc;
     int total = 0;
@@ -2731,7 +2767,7 @@ int gpu_merge_init(efg_index* ix) {
     if (int rc = check_hip(hipStreamSynchronize(st), "merge")) return rc;
     if (!rec.ensure(size_t(total) + 1)) return fail(EFG_ERR_MEMORY, "out of device memory (merge records)");
     hipMemsetAsync(cursor.p, 0, size_t(M) * 4, st);
-    k_probe<<<grid_for(n), 256, 0, st>>>(ix->data.p, D, ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la, n,
+    probe<<<grid_for(n), 256, 0, st>>>(PX, PD, ix->node4.p, ix->parent.p, ix->range.p, depth.p, lt, la, n,
                                          p.merge_level, 1, cnt.p, boff.p, cursor.p, rec.p, lam_s.p, nlev);
     hipMemsetAsync(nbig.p, 0, sizeof(int), st);
     k_sort_buckets<<<unsigned(std::min<size_t>((M + 3) / 4, 65536)), kBlock, 0, st>>>(boff.p, M, rec.p, bigl.p,
@@ -3037,12 +3073,12 @@ int gpu_refine_iter(efg_index* ix) {
   double t1 = now_ms();
   // ---- update(100)
   bs.iter += 1;
-  DevBuf<uint32_t>&fnew = sc.fnew, &fold = sc.fold, &rcn = sc.rcn, &rco = sc.rco, &rcurn = sc.rcurn,
-                   &rcuro = sc.rcuro, &rszn = sc.rszn, &rszo = sc.rszo, &ridn = sc.ridn, &rido = sc.rido;
+  DevBuf<uint32_t>&fnew = sc.fnew, &fold = sc.fold, &rcn = sc.rcn, &rco = sc.rco, &rszn = sc.rszn, &rszo = sc.rszo,
+                   &ridn = sc.ridn, &rido = sc.rido;
   DevBuf<int32_t>&roffn = sc.roffn, &roffo = sc.roffo, &snew = sc.snew, &sold = sc.sold, &onew = sc.onew,
                   &oold = sc.oold;
-  if (!fnew.ensure(n) || !fold.ensure(n) || !rcn.ensure(n) || !rco.ensure(n) || !rcurn.ensure(n) ||
-      !rcuro.ensure(n) || !rszn.ensure(n) || !rszo.ensure(n) || !roffn.ensure(n + 1) || !roffo.ensure(n + 1) ||
+  if (!fnew.ensure(n) || !fold.ensure(n) || !rcn.ensure(n) || !rco.ensure(n) || !rszn.ensure(n) || !rszo.ensure(n) || !roffn.ensure(n + 1) || !roffo.ensure(n + 1) ||
+      !sc.rslot.ensure(size_t(n) * L) ||
       !snew.ensure(n + 1) || !sold.ensure(n + 1) || !onew.ensure(n + 1) || !oold.ensure(n + 1))
     return fail(EFG_ERR_MEMORY, "out of device memory (update)");
   // a group of G lanes per node: the pool length rounded up to a power of
@@ -3055,7 +3091,7 @@ int gpu_refine_iter(efg_index* ix) {
   hipMemsetAsync(rcn.p, 0, size_t(n) * 4, st);
   hipMemsetAsync(rco.p, 0, size_t(n) * 4, st);
   EFG_G(k_lists)<<<ggrid, 256, 0, st>>>(bs.pool.p, bs.flag.p, bs.range.p, bs.radiusM.p, n, L, 0, fnew.p, fold.p, rcn.p,
-                                       rco.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+                                       rco.p, nullptr, nullptr, sc.rslot.p, nullptr, nullptr);
   if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(rcn.p), int(n), roffn.p, scal.p, sums, st)) return rc;
   if (int rc = device_exclusive_scan(reinterpret_cast<int32_t*>(rco.p), int(n), roffo.p, scal.p + 1, sums, st)) return rc;
   int rtot[2] = {0, 0};
@@ -3063,10 +3099,8 @@ int gpu_refine_iter(efg_index* ix) {
   if (int rc = check_hip(hipStreamSynchronize(st), "update")) return rc;
   if (!ridn.ensure(size_t(rtot[0]) + 1) || !rido.ensure(size_t(rtot[1]) + 1))
     return fail(EFG_ERR_MEMORY, "out of device memory (reverse lists)");
-  hipMemsetAsync(rcurn.p, 0, size_t(n) * 4, st);
-  hipMemsetAsync(rcuro.p, 0, size_t(n) * 4, st);
   EFG_G(k_lists)<<<ggrid, 256, 0, st>>>(bs.pool.p, bs.flag.p, bs.range.p, bs.radiusM.p, n, L, 1, nullptr, nullptr,
-                                       nullptr, nullptr, roffn.p, roffo.p, rcurn.p, rcuro.p, ridn.p, rido.p);
+                                       nullptr, nullptr, roffn.p, roffo.p, sc.rslot.p, ridn.p, rido.p);
   const int rcap = 100;
   const int schunk = 4096;
   const size_t lds_s = size_t(schunk) * 12;

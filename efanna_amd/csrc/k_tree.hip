@@ -88,7 +88,7 @@ __device__ inline void make_children(int ni, int a, int32_t base, uint2 r, int4 
 // kChunk at a time into registers (the row index is clamped, values past the
 // sample are never used) and summed in row order as the reference does; the
 // variance pass loads them again (from cache).  Row offsets are 32-bit when
-// the rows span fewer than 2^30 floats (fewer registers per load in flight;
+// the rows span fewer than 2^30 elements (fewer registers per load in flight;
 // the kernel is bound by the latency of its short per-node steps, so it
 // needs the occupancy).  The first wavefront of the group picks the top-5 variance
 // dimensions by five arg-max rounds over (variance descending, dimension
@@ -107,8 +107,8 @@ __device__ __forceinline__ auto row_off(uint32_t id, int D) {
   if constexpr (kOff32) return id * uint32_t(D);
   else return size_t(id) * size_t(D);
 }
-template <bool kOff32>
-__global__ void __launch_bounds__(kBlock, 5) k_split_stats(const float* __restrict__ X, int d, int D,
+template <bool kOff32, int kRow>
+__global__ void __launch_bounds__(kBlock, 5) k_split_stats(const void* __restrict__ X, int d, int D,
                                                         const int32_t* __restrict__ active, int na,
                                                         uint2* __restrict__ range, int32_t* __restrict__ tree_of,
                                                         uint32_t* __restrict__ leaf, uint32_t n, uint64_t seed,
@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(kBlock, 5) k_split_stats(const float* __restri
 #pragma unroll
           for (int j = 0; j < kChunk; ++j) {
             const int jj = j0 + j < cnt ? j0 + j : cnt - 1;
-            xv[j] = X[row_off<kOff32>(ids_s[g][jj], D) + k];
+            xv[j] = row_elem<kRow>(X, row_off<kOff32>(ids_s[g][jj], D) + k);
           }
 #pragma unroll
           for (int j = 0; j < kChunk; ++j)
@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(kBlock, 5) k_split_stats(const float* __restri
 #pragma unroll
           for (int j = 0; j < kChunk; ++j) {
             const int jj = j0 + j < cnt ? j0 + j : cnt - 1;
-            xv[j] = X[row_off<kOff32>(ids_s[g][jj], D) + k];
+            xv[j] = row_elem<kRow>(X, row_off<kOff32>(ids_s[g][jj], D) + k);
           }
 #pragma unroll
           for (int j = 0; j < kChunk; ++j)
@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(kBlock, 5) k_split_stats(const float* __restri
     if (part) {
       const int cd = cd_s[g];
       const float cut = mean[cd];
-      if (unsigned(lt) < count) c = classify(X[size_t(my_id) * D + cd], cut);
+      if (unsigned(lt) < count) c = classify(row_elem<kRow>(X, size_t(my_id) * D + cd), cut);
       m0 = __ballot(c == 0);
       m1 = __ballot(c == 1);
       m2 = __ballot(c == 2);
@@ -272,7 +272,8 @@ __device__ inline int find_node(const int32_t* __restrict__ bstart, int na, int 
   return lo;
 }
 
-__global__ void __launch_bounds__(kBlock) k_part_count(const float* __restrict__ X, int D,
+template <int kRow>
+__global__ void __launch_bounds__(kBlock) k_part_count(const void* __restrict__ X, int D,
                                                        const int32_t* __restrict__ active, int na,
                                                        const int32_t* __restrict__ bstart, const int* __restrict__ nblk,
                                                        const uint2* __restrict__ range,
@@ -298,7 +299,7 @@ __global__ void __launch_bounds__(kBlock) k_part_count(const float* __restrict__
       uint32_t pos = s0 + threadIdx.x * 4 + e;
       if (pos < r.y) {
         uint32_t id = leaf[size_t(t) * n + pos];
-        int c = classify(X[size_t(id) * D + nd.x], cut);
+        int c = classify(row_elem<kRow>(X, size_t(id) * D + nd.x), cut);
         cls[size_t(t) * n + pos] = uint8_t(c);
         if (c < 2) packed += 1u << (16 * c);
       }
@@ -543,9 +544,31 @@ int gpu_build_trees(efg_index* ix) {
   const int small = tpn;  // nodes of at most this many ids are partitioned with their statistics
   const size_t lds_stats = size_t(kBlock / tpn) * 2 * ix->d * sizeof(float);
   if (lds_stats > 150 * 1024) return fail(EFG_ERR_ARG, "dimension too large for the split statistics");
-  const bool off32 = size_t(n) * ix->D < (size_t(1) << 30);
-  hipFuncSetAttribute(reinterpret_cast<const void*>(off32 ? &k_split_stats<true> : &k_split_stats<false>),
-                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_stats));
+  // rows read by the statistics and partitions: FP32, or in the build's
+  // precision form the 16-bit high parts when every row is exact in them
+  // (the same values in the same order of operations: the same trees, from
+  // half the bytes)
+  const void* X = ix->data.p;
+  size_t xs = ix->D;
+  int xk = 0;
+  if (ix->b_precision > 0) {
+    if (int rc = ensure_split(ix, st)) return rc;
+    if (!ix->split_lo_used) {
+      X = ix->split_hi.p;
+      xs = size_t(ix->split_Dk);
+      xk = ix->b_precision;
+    }
+  }
+  const bool off32 = size_t(n) * xs < (size_t(1) << 30);
+  using StatsFn = decltype(&k_split_stats<true, 0>);
+  using CountFn = decltype(&k_part_count<0>);
+  const StatsFn stats_fns[2][3] = {{&k_split_stats<false, 0>, &k_split_stats<false, 1>, &k_split_stats<false, 2>},
+                                   {&k_split_stats<true, 0>, &k_split_stats<true, 1>, &k_split_stats<true, 2>}};
+  const CountFn count_fns[3] = {&k_part_count<0>, &k_part_count<1>, &k_part_count<2>};
+  const StatsFn stats_fn = stats_fns[off32 ? 1 : 0][xk];
+  const CountFn count_fn = count_fns[xk];
+  hipFuncSetAttribute(reinterpret_cast<const void*>(stats_fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                      int(lds_stats));
   int32_t h_total = 0;
   while (na > 0) {
     if (bfs && !(na < 1100)) {  // switch to DFS: keep nodes larger than TNS
@@ -567,8 +590,8 @@ int gpu_build_trees(efg_index* ix) {
     {
       const int per = kBlock / tpn;
       unsigned g = unsigned(std::min<int>((na + per - 1) / per, 65535 * 4));
-      (off32 ? k_split_stats<true> : k_split_stats<false>)<<<g, kBlock, lds_stats, st>>>(
-          ix->data.p, int(ix->d), int(ix->D), act.p, na, ix->range.p, tree_of.p, ix->leaf.p, n, p.seed, ix->node4.p,
+      stats_fn<<<g, kBlock, lds_stats, st>>>(
+          X, int(ix->d), int(xs), act.p, na, ix->range.p, tree_of.p, ix->leaf.p, n, p.seed, ix->node4.p,
           ix->parent.p, tpn, small, base);
     }
     // blocks of the nodes larger than `small`
@@ -576,7 +599,7 @@ int gpu_build_trees(efg_index* ix) {
     if (int rc = device_exclusive_scan(nb.p, na, bstart.p, scal.p + 1, sums, st)) return rc;
     {
       unsigned g = unsigned(size_t(ncu) * 16);
-      k_part_count<<<g, kBlock, 0, st>>>(ix->data.p, int(ix->D), act.p, na, bstart.p, scal.p + 1, ix->range.p,
+      count_fn<<<g, kBlock, 0, st>>>(X, int(xs), act.p, na, bstart.p, scal.p + 1, ix->range.p,
                                          tree_of.p, ix->node4.p, ix->leaf.p, n, cls.p, bcnt.p);
       k_part_node<<<unsigned((size_t(na) * 64 + 255) / 256), 256, 0, st>>>(
           act.p, na, bstart.p, bcnt.p, ix->node4.p, ix->parent.p, ix->range.p, tree_of.p, base, small);
