@@ -2234,6 +2234,11 @@ __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK,
     const int c0 = toff[x], c1 = tcnt ? c0 + int(tcnt[x]) : toff[x + 1];
     if (c0 == c1) continue;
     int cur = int(fill[x]);
+    // the first window's candidates are loaded with the pool (one latency
+    // for both; most targets have at most one window)
+    const int i0 = c0 + lane;
+    const uint64_t v0 = i0 < c1 ? bkey[i0] : kKeyMax;
+    const uint32_t s0 = i0 < c1 ? bsrc[i0] : 0u;
     for (int i = lane; i < cur; i += 64) {
       A[i] = pool[size_t(x) * L + i];
       AT[i] = flag[size_t(x) * L + i];
@@ -2243,14 +2248,15 @@ __global__ void __launch_bounds__(kBlock) k_merge_reg(uint32_t n, int L, int CK,
       const uint64_t wk = cur >= L ? A[L - 1] : kKeyMax;
       int m = 0;
       while (base < c1 && m < 64) {
+        const bool pre = base == c0;  // the first window (base only grows)
         const int i = base + lane;
-        const uint64_t v = i < c1 ? bkey[i] : kKeyMax;
+        const uint64_t v = pre ? v0 : (i < c1 ? bkey[i] : kKeyMax);
         const bool keep = i < c1 && v <= wk;
         const uint64_t km = __ballot(keep);
         const int room = 64 - m, rank = __popcll(km & lt), nk = __popcll(km);
         if (keep && rank < room) {
           C[m + rank] = v;
-          CT[m + rank] = cand_tag(bsrc[i]);
+          CT[m + rank] = cand_tag(pre ? s0 : bsrc[i]);
         }
         if (nk > room) {  // the kept candidate of rank `room` opens the next round
           base += __ffsll((long long)__ballot(keep && rank == room)) - 1;
