@@ -1610,7 +1610,7 @@ __global__ void __launch_bounds__(kBlock) k_leaf_mfma(const uint16_t* __restrict
 // k_join_tasks), the 32 x 32 block of dot products in one accumulator; then
 // the candidates as in k_join_tiles, counted per target row in the
 // wavefront's LDS slice, staged into per-wavefront runs.
-template <bool kF16, bool kLo>
+template <bool kF16, bool kLo, bool kDirect>
 __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict__ xhi, const uint16_t* __restrict__ xlo,
                                                       const float* __restrict__ xn, int Dk, uint32_t ntask,
                                                       const uint4* __restrict__ hdr,
@@ -1618,7 +1618,9 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
                                                       const uint64_t* __restrict__ worst, Stage* __restrict__ stage,
                                                       unsigned long long cap, unsigned long long* __restrict__ nstage,
                                                       uint32_t* __restrict__ tcount,
-                                                      unsigned long long* __restrict__ npairs) {
+                                                      unsigned long long* __restrict__ npairs,
+                                                      const int32_t* __restrict__ boff, uint64_t* __restrict__ bkey,
+                                                      uint32_t* __restrict__ bsrc) {
   constexpr int kW = kBlock / 64;
   __shared__ uint32_t ids_s[kW][64];
   __shared__ uint64_t w_s[kW][64];
@@ -1628,7 +1630,7 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
   const int r32 = lane & 31;
   const uint32_t xcd = blockIdx.x & 7u, nsl = (gridDim.x >> 3) * kW;
   const uint32_t lo = uint32_t(uint64_t(ntask) * xcd / 8), hi = uint32_t(uint64_t(ntask) * (xcd + 1) / 8);
-  unsigned long long pairs_local = 0;
+  unsigned long long pairs_local = 0, kept_local = 0;
   unsigned long long wbase = 0, wend = 0;  // staging run (lane 63)
   uint32_t* ids = ids_s[wv];
   for (uint32_t t = lo + (blockIdx.x >> 3) * kW + wv; t < hi; t += nsl) {
@@ -1666,29 +1668,35 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
       }
     }
     wave_sync();
-    // totals per target (result not waited for), offsets within the task
     const uint32_t cnt = cnt_s[wv][lane];
-    if (cnt) atomicAdd(&tcount[id], cnt);
-    uint32_t x = cnt;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = uint32_t(__shfl_up(int(x), o));
-      if (lane >= o) x += y;
-    }
-    off_s[wv][lane] = x - cnt;
-    const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
-    // this task's slots: the rest of the run, then a new run if needed
-    const unsigned long long rem = wend - wbase;
-    unsigned long long nr = 0;
-    if (tot > rem) {
-      if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
-      nr = (unsigned long long)__shfl((long long)nr, 0);
-    }
-    const unsigned long long r0 = wbase;
-    if (tot > rem) {
-      wbase = nr + (tot - rem);
-      wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
+    unsigned long long rem = 0, nr = 0, r0 = 0;
+    if constexpr (kDirect) {
+      // the row's first slot in its target's bucket
+      off_s[wv][lane] = cnt ? uint32_t(boff[id]) + atomicAdd(&tcount[id], cnt) : 0u;
+      kept_local += cnt;
     } else {
-      wbase += tot;
+      // totals per target (result not waited for), offsets within the task
+      if (cnt) atomicAdd(&tcount[id], cnt);
+      uint32_t x = cnt;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = uint32_t(__shfl_up(int(x), o));
+        if (lane >= o) x += y;
+      }
+      off_s[wv][lane] = x - cnt;
+      const unsigned long long tot = (unsigned long long)(uint32_t(__shfl(int(x), 63)));
+      // this task's slots: the rest of the run, then a new run if needed
+      rem = wend - wbase;
+      if (tot > rem) {
+        if (lane == 0) nr = atomicAdd(nstage, tot - rem > kStageRun ? tot - rem : kStageRun);
+        nr = (unsigned long long)__shfl((long long)nr, 0);
+      }
+      r0 = wbase;
+      if (tot > rem) {
+        wbase = nr + (tot - rem);
+        wend = nr + (tot - rem > kStageRun ? tot - rem : kStageRun);
+      } else {
+        wbase += tot;
+      }
     }
     wave_sync();
     const uint32_t nn = h.x;
@@ -1700,6 +1708,11 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
       uint32_t a, b;
       block_ends(p, q, same, sizeA, newA, sizeB, newB, ids, sa, sb, a, b);
       const uint32_t la = li[i] & 0xffffu, lb = li[i] >> 16;
+      if constexpr (kDirect) {
+        if (la != 0xffffu) put_direct(bkey, bsrc, off_s[wv][sa] + la, make_key(dist[i], b), nn | kFirst);
+        if (lb != 0xffffu) put_direct(bkey, bsrc, off_s[wv][sb] + lb, make_key(dist[i], a), nn);
+        continue;
+      }
       if (la != 0xffffu) {
         const unsigned long long j = off_s[wv][sa] + la, at = j < rem ? r0 + j : nr + (j - rem);
         if (at < cap) stage[at] = Stage{a, nn | kFirst, make_key(dist[i], b)};
@@ -1711,9 +1724,14 @@ __global__ void __launch_bounds__(kBlock) k_join_mfma(const uint16_t* __restrict
     }
     wave_sync();
   }
-  for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
-  for (int o = 32; o > 0; o >>= 1) pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
+  if (!kDirect)
+    for (unsigned long long i = wbase + lane; i < wend && i < cap; i += 64) stage[i].target = kNoRow;
+  for (int o = 32; o > 0; o >>= 1) {
+    pairs_local += (unsigned long long)__shfl_down((long long)pairs_local, o);
+    kept_local += (unsigned long long)__shfl_down((long long)kept_local, o);
+  }
   if (lane == 0 && pairs_local) atomicAdd(npairs, pairs_local);
+  if (kDirect && lane == 0 && kept_local) atomicAdd(nstage, kept_local);
 }
 
 // Three 32 x 32 blocks of split dot products of one node's rows, loaded once:
@@ -2933,7 +2951,7 @@ int gpu_refine_iter(efg_index* ix) {
                                                                   stage, cap, nst, cnt, np, boff, bk, bsr);
     }
   };
-  // block tasks of the larger nodes; the MFMA form's are always staged
+  // block tasks of the larger nodes
   auto run_tasks = [&](auto direct_c, Stage* stage, unsigned long long cap, unsigned long long* nst, uint32_t* cnt,
                        unsigned long long* np) {
     constexpr bool kD = decltype(direct_c)::value;
@@ -2942,10 +2960,11 @@ int gpu_refine_iter(efg_index* ix) {
       const unsigned g16 = unsigned(size_t(ncu) * 8);
       const bool f16 = ix->b_precision == 2;
       const bool lo = ix->split_lo_used;
-      (f16 ? (lo ? k_join_mfma<true, true> : k_join_mfma<true, false>)
-           : (lo ? k_join_mfma<false, true> : k_join_mfma<false, false>))<<<g16, kBlock, 0, st>>>(
+      (f16 ? (lo ? k_join_mfma<true, true, kD> : k_join_mfma<true, false, kD>)
+           : (lo ? k_join_mfma<false, true, kD> : k_join_mfma<false, false, kD>))<<<g16, kBlock, 0, st>>>(
           ix->split_hi.p, ix->split_lo.p, ix->split_norm.p, ix->split_Dk, uint32_t(ntask), bs.jhdr.p, bs.jtids.p,
-          worst.p, stage, cap, nst, cnt, np);
+          worst.p, stage, cap, nst, cnt, np, kD ? toff.p : nullptr, kD ? bs.jbkey.p : nullptr,
+          kD ? bs.jbsrc.p : nullptr);
     } else {
       k_join_tiles<4, kD><<<unsigned(join_grid), kBlock, lds_join, st>>>(
           ix->data.p, D, uint32_t(ntask), bs.jhdr.p, bs.jtids.p, worst.p, stage, cap, nst, cnt, np,
@@ -2956,13 +2975,13 @@ int gpu_refine_iter(efg_index* ix) {
                               unsigned long long* np) { run_tasks(std::false_type{}, stage, c, nst, cnt, np); };
   // Direct routing when every target's room (the pairs it is in, 2 x bound
   // slots in all) fits 31-bit offsets and memory; else staging.  The MFMA
-  // block tasks stay staged (their candidates then appended to the same
-  // buckets by the scatter): routed directly, their scattered writes made
-  // that short kernel up to a third slower (1M x 960), more than the staging
-  // round trip costs.
+  // block tasks of rows above 128 columns stay staged (their candidates then
+  // appended to the same buckets by the scatter): routed directly, their
+  // scattered writes made that short kernel up to a third slower (1M x 960),
+  // more than the staging round trip costs.
   const bool direct = ix->b_join_direct && 2 * bound + 1 < (1ull << 31) &&
                       bs.jbkey.ensure(size_t(2 * bound) + 1) && bs.jbsrc.ensure(size_t(2 * bound) + 1);
-  const bool tasks_direct = ix->b_precision == 0;
+  const bool tasks_direct = ix->b_precision == 0 || ix->split_Dk <= 128;
   // staging: the kept candidates (two per pair at most) plus one partly used
   // run per workgroup (per wavefront on the MFMA path, ncu * 8 workgroups of
   // four: three kernels staged, or the block tasks' alone with direct
