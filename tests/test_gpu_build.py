@@ -66,6 +66,24 @@ def test_trees_match_oracle_mixture_dfs_phase(cuda):
     _cmp_trees(g, o)
 
 
+@pytest.mark.parametrize("precision", [1, 2])
+@pytest.mark.parametrize("rows", ["integer", "float"])
+def test_precision_form_trees_match_oracle(cuda, precision, rows):
+    """The precision form builds its trees from the 16-bit high parts of the
+    split rows when every row is exact in them (integer rows 0..255, in BF16
+    and in FP16), else from the FP32 rows (float rows); either way the trees
+    are the oracle's, through both the BFS and the DFS phase."""
+    if rows == "integer":
+        base, _ = helpers.mixture(n=40_000, nq=1)
+    else:
+        base, _ = helpers.mixture(n=40_000, nq=1, d=96, centres=40, sigma=0.05, hi=1.0, rounded=False)
+    g, o = _pair(base, trees=8, S=6, tns=12)
+    g.setBuildPrecision(precision)
+    g.buildTrees()
+    o.build_trees()
+    _cmp_trees(g, o)
+
+
 @pytest.mark.parametrize("direct", [True, False])
 def test_build_stages_match_oracle(cuda, direct):
     """Every stage equals the oracle, with the join's candidates routed
